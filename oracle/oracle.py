@@ -1,0 +1,142 @@
+"""THE ORACLE's Python driver -- test infrastructure only (tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg). It runs tests/scenarios.py-style worlds on the scalar C restatement of the reference's flight-modes world
+(oracle_modes.c: src/exact_ballpark/modes.py's ModeBallpark, one world at a time) and returns every ball's state.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
+ND, NQ = 27, 12
+D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
+           "ovz", "mass", "radius", "max_vel", "agility", "sf", "follow_range", "time_factor", "last_collision", "pad")
+Q_NAMES = ("id", "mode", "effect_stamp", "follow_id", "owner_id", "follow_ptr", "massive", "sensor", "interactive",
+           "free", "moribund", "n_followers")
+_LIB = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, d, i64, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int
+        pd = ctypes.POINTER(ctypes.c_double)
+        pq = ctypes.POINTER(ctypes.c_int64)
+        L.or_park_new.restype = vp
+        L.or_park_new.argtypes = [d, i, i, i64]
+        L.or_park_free.argtypes = [vp]
+        L.or_add_ball.argtypes = [vp, i64, d, d, d, pd, pd, d, d, i, i, i, i64, i64, i, i64]
+        for name in ("or_goto_point", "or_goto_direction"):
+            getattr(L, name).argtypes = [vp, i64, d, d, d]
+        L.or_stop.argtypes = [vp, i64]
+        L.or_follow.argtypes = [vp, i64, i64, d]
+        L.or_orbit.argtypes = [vp, i64, i64, d]
+        L.or_warp_to.argtypes = [vp, i64, d, d, d, d, i64]
+        L.or_entity_warp_in.argtypes = [vp, i64, d, d, d, i64]
+        L.or_evolve.argtypes = [vp]
+        L.or_count.argtypes = [vp]
+        L.or_get_ball.argtypes = [vp, i, pd, pq]
+        L.or_get_followers.argtypes = [vp, i, pq]
+        L.or_maths.argtypes = [i, i64, pd, pd]
+        _LIB = L
+    return _LIB
+
+
+class World:
+    """One world of the oracle: add balls, command them, tick, read them back."""
+
+    def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int):
+        self.L = lib()
+        self.p = self.L.or_park_new(float(friction), int(tick_ms), int(bool(new_orbit)), int(start))
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.L.or_park_free(self.p)
+            self.p = None
+
+    def add(self, b: dict) -> None:
+        pos = (ctypes.c_double * 3)(*[float(c) for c in b["pos"]])
+        vel = (ctypes.c_double * 3)(*[float(c) for c in b["vel"]])
+        rc = self.L.or_add_ball(self.p, int(b["id"]), float(b["mass"]), float(b["radius"]), float(b["max_vel"]),
+                                pos, vel, float(b["agility"]), float(b["sf"]), int(bool(b["free"])),
+                                int(bool(b["massive"])), int(bool(b["interactive"])), int(b["cloaked"]),
+                                int(b["bubble"]), int(bool(b["moribund"])), int(b["mode"]))
+        if rc:
+            raise ValueError(f"ball {b['id']} refused")
+
+    def command(self, c) -> None:
+        _, op, a, b, x, y, z, rr, w = c
+        L, p = self.L, self.p
+        if op == GOTO_OP:
+            L.or_goto_point(p, a, x, y, z)
+        elif op == DIR_OP:
+            L.or_goto_direction(p, a, x, y, z)
+        elif op == STOP_OP:
+            L.or_stop(p, a)
+        elif op == FOLLOW_OP:
+            L.or_follow(p, a, b, rr)
+        elif op == ORBIT_OP:
+            L.or_orbit(p, a, b, rr)
+        elif op == WARP_OP:
+            L.or_warp_to(p, a, x, y, z, rr, w)
+        elif op == WARPIN_OP:
+            L.or_entity_warp_in(p, a, x, y, z, w)
+        else:
+            raise ValueError(op)
+
+    def evolve(self) -> None:
+        self.L.or_evolve(self.p)
+
+    def state(self) -> tuple:
+        """(doubles [n, 27] as float64, ints [n, 12], followers: list of sorted id lists), balls by ascending id."""
+        n = self.L.or_count(self.p)
+        d = np.zeros((n, ND), dtype=np.float64)
+        q = np.zeros((n, NQ), dtype=np.int64)
+        fol = []
+        buf = (ctypes.c_int64 * 64)()
+        for k in range(n):
+            self.L.or_get_ball(self.p, k, d[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               q[k].ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+            m = self.L.or_get_followers(self.p, k, buf)
+            fol.append(list(buf[:m]))
+        return d, q, fol
+
+
+def run_world(w: dict, record_every_step: bool = True):
+    """Yields (step, doubles, ints, followers) after each tick of world w (every step, or the last only)."""
+    o = World(w["friction"], w["tick"], w["new_orbit"], w["start"])
+    for b in w["balls"]:
+        o.add(b)
+    for step in range(1, w["steps"] + 1):
+        for c in w["cmds"]:
+            if c[0] == step:
+                o.command(c)
+        o.evolve()
+        if record_every_step or step == w["steps"]:
+            yield (step, *o.state())
+
+
+def maths(func: str, x: np.ndarray) -> np.ndarray:
+    """The oracle's glibc function (sin, cos, log, exp, sincos_sin, sincos_cos) elementwise."""
+    codes = {"sin": 0, "cos": 1, "log": 2, "exp": 3, "sincos_sin": 4, "sincos_cos": 5}
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    rc = lib().or_maths(codes[func], x.size, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc:
+        raise ValueError(func)
+    return out

@@ -1,0 +1,685 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * THE ORACLE -- test infrastructure only. A scalar CPU restatement of the reference's flight-modes world
+ * (src/exact_ballpark/ref.py and src/exact_ballpark/modes.py: `Ballpark`, `ModeBallpark`), one world at a time, in
+ * the reference's own object structure: balls kept by id, explicit follower sets, each tick's phases over the balls
+ * in ascending id (modes.py:702-745). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as
+ * the checker or the CPU baseline; the product never links or calls it.
+ *
+ * What pins it: tests/test_oracle_golden.py holds every state this file produces to golden vectors written by the
+ * reference's own ModeBallpark (tests/golden/make_golden_modes.py), bit for bit; the maths it calls (glibc's sincos,
+ * log, exp) is pinned by tests/test_maths_cpu.py. Old position and velocity (mOldPos, mOldVel), which ModeBallpark
+ * does not keep, follow FullBallpark's commit (src/exact_ballpark/full.py:2531-2557); those two fields are pinned by
+ * the closed forms in tests/test_oracle_invariants.py only.
+ *
+ * Compiled with -O2 -ffp-contract=off -fno-fast-math: every + - * / sqrt is one correctly rounded IEEE operation in
+ * the reference's order (Python evaluates left to right), and only the maths header's explicit fused steps fuse.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../exact-ballpark_amd/csrc/xb_maths_double.h"
+
+/* DSTBALLMODE (modes.py:65-67, ref.py:34) */
+enum { GOTO = 0, FOLLOW = 1, STOP = 2, WARP = 3, ORBIT = 4, MISSILE = 5, MUSHROOM = 6, BOID = 7, TROLL = 8,
+       MINIBALL = 9, FIELD = 10, RIGID = 11, FORMATION = 12 };
+
+#define AU 0.1495978707e12                     /* modes.py:72 */
+#define ORBITAL_PRECESSION 0.001               /* modes.py:73 */
+#define WARP_AU_PER_SECOND 0.001               /* modes.py:74 */
+#define WARP_ACCELERATION (1.0 / 1000)         /* modes.py:75 */
+#define WARP_DECELERATION (1.0 / 3000)         /* modes.py:76 */
+#define MAX_ALIGN_TICKS 180                    /* modes.py:77 */
+#define MIN_WARP_DISTANCE_SQ 10000000000.0     /* modes.py:78 */
+#define STOP_DAMP 0.9345794392523364485981308411215   /* ref.py:37 */
+#define MAX_FOLLOWERS 64
+#define MAX_BALLS 256
+
+typedef struct {
+    int64_t id;
+    double mass, radius, max_vel, agility, sf, follow_range;   /* radius .. follow_range are floats held in double */
+    double pos[3], vel[3], gp[3], last_g[3], last_c[3], old_pos[3], old_vel[3];
+    double tf, last_collision;
+    int64_t mode, follow_id, owner_id, effect_stamp;
+    int64_t follow_ptr;                        /* the target's id; -1 for none (modes.py: None) */
+    int is_free, is_massive, is_interactive, is_moribund, sensor;
+    int64_t cloaked, bubble;
+    int nf;
+    int64_t followers[MAX_FOLLOWERS];
+} OBall;
+
+typedef struct {
+    double friction, dt;
+    int64_t current_time;
+    int use_new_orbit;
+    int n;
+    OBall balls[MAX_BALLS];                    /* kept sorted by id */
+} OPark;
+
+/* ------------------------------------------------------------------------------------------ vector helpers */
+
+static double lensq(const double v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+static double length(const double v[3]) { return sqrt(lensq(v)); }
+static void normalized(const double v[3], double out[3]) {     /* ref.py:56-62 */
+    double norm = length(v);
+    if (norm == 0.0) {
+        out[0] = v[0], out[1] = v[1], out[2] = v[2];
+        return;
+    }
+    double inv = 1.0 / norm;
+    out[0] = v[0] * inv, out[1] = v[1] * inv, out[2] = v[2] * inv;
+}
+static void cross(const double u[3], const double v[3], double o[3]) {   /* modes.py:89-91 */
+    double r0 = u[1] * v[2] - u[2] * v[1], r1 = u[2] * v[0] - u[0] * v[2], r2 = u[0] * v[1] - u[1] * v[0];
+    o[0] = r0, o[1] = r1, o[2] = r2;
+}
+static int finite3(const double v[3]) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
+static double c_float(double x) { return (double)(float)x; }   /* ref.py:48-53: a C++ float parameter */
+static double trunc7(double x) {                               /* modes.py:94-96 */
+    double y = x * 10000000;
+    int64_t i = (y != y) ? 0 : (y >= 9223372036854775808.0) ? INT64_MAX : (y < -9223372036854775808.0) ? INT64_MIN
+                                                                                                            : (int64_t)y;
+    return (double)i / 10000000;
+}
+static int64_t bits_of(double x) { int64_t i; memcpy(&i, &x, 8); return i; }
+static double double_of(int64_t i) { double x; memcpy(&x, &i, 8); return x; }
+
+/* ---------------------------------------------------------------------------------------------- the park */
+
+static OBall* find(OPark* p, int64_t id) {
+    int lo = 0, hi = p->n - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) / 2;
+        if (p->balls[mid].id == id) return &p->balls[mid];
+        if (p->balls[mid].id < id) lo = mid + 1; else hi = mid - 1;
+    }
+    return NULL;
+}
+
+static void fol_add(OBall* b, int64_t id) {
+    for (int i = 0; i < b->nf; ++i) if (b->followers[i] == id) return;
+    if (b->nf < MAX_FOLLOWERS) b->followers[b->nf++] = id;
+}
+static void fol_discard(OBall* b, int64_t id) {
+    for (int i = 0; i < b->nf; ++i)
+        if (b->followers[i] == id) {
+            b->followers[i] = b->followers[--b->nf];
+            return;
+        }
+}
+static int cmp_i64(const void* a, const void* b) {
+    int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+OPark* or_park_new(double friction, int tick_interval_ms, int use_new_orbit, int64_t current_time) {
+    OPark* p = (OPark*)calloc(1, sizeof(OPark));
+    p->friction = friction;
+    p->dt = tick_interval_ms * 0.001;
+    p->use_new_orbit = use_new_orbit;
+    p->current_time = current_time;
+    return p;
+}
+void or_park_free(OPark* p) { free(p); }
+int or_park_size(void) { return (int)sizeof(OPark); }
+int or_ball_size(void) { return (int)sizeof(OBall); }
+
+static void refresh_time_factor(OPark* p, OBall* b) {     /* ref.py:82-87 */
+    double m = b->mass * b->agility;
+    b->tf = (m <= 0.0) ? 0.0 : xb_exp(-p->friction * p->dt / m);
+}
+
+/* AddBall (ref.py:89-102; modes.py:180-192), with FullBallpark's old state of a new ball (full.py:1247-1248). */
+int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel, const double* pos, const double* vel,
+                double agility, double sf, int is_free, int is_massive, int is_interactive, int64_t cloaked,
+                int64_t bubble, int is_moribund, int64_t mode) {
+    if (p->n >= MAX_BALLS || find(p, id)) return -1;
+    int k = p->n;
+    while (k > 0 && p->balls[k - 1].id > id) {
+        p->balls[k] = p->balls[k - 1];
+        --k;
+    }
+    OBall* b = &p->balls[k];
+    p->n++;
+    memset(b, 0, sizeof(*b));
+    b->id = id;
+    radius = c_float(radius), max_vel = c_float(max_vel), agility = c_float(agility), sf = c_float(sf);
+    b->mass = mass < 0.0 ? 0.0 : mass;
+    b->radius = radius < 0.0 ? 0.0 : radius;
+    b->max_vel = max_vel < 0.0 ? 0.0 : max_vel;
+    b->agility = agility <= 0.0 ? 1.0 : agility;
+    b->sf = sf < 0.0 ? 0.0 : sf;
+    for (int i = 0; i < 3; ++i) {
+        b->pos[i] = pos[i];
+        b->vel[i] = vel[i];
+        b->old_pos[i] = pos[i] - vel[i] * p->dt;
+        b->old_vel[i] = vel[i];
+    }
+    b->mode = mode;
+    b->follow_range = 10.0;
+    b->last_collision = -1.0;
+    b->follow_ptr = -1;
+    b->is_free = is_free, b->is_massive = is_massive, b->is_interactive = is_interactive;
+    b->cloaked = cloaked, b->bubble = bubble, b->is_moribund = is_moribund;
+    refresh_time_factor(p, b);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------- commands */
+
+static int in_follow_modes(int64_t m) { return m == FOLLOW || m == ORBIT || m == MISSILE || m == FORMATION; }
+
+static void stop_ball(OPark* p, OBall* b) {                       /* Stop(Ball*), modes.py:206-220 (master) */
+    if (in_follow_modes(b->mode)) {
+        OBall* t = find(p, b->follow_ptr);
+        if (t) fol_discard(t, b->id);
+        b->effect_stamp = 0;
+        b->follow_ptr = -1;
+        b->follow_id = 0;
+        b->owner_id = 0;
+        b->follow_range = 0.0;
+    }
+    b->mode = STOP;
+}
+
+void or_stop(OPark* p, int64_t id) {                               /* modes.py:224-229 */
+    OBall* b = find(p, id);
+    if (!b || b->mode == STOP) return;
+    stop_ball(p, b);
+}
+
+static void goto_point_ball(OPark* p, OBall* b, const double pt[3]) {   /* modes.py:238-246 */
+    if (!finite3(pt)) return;
+    stop_ball(p, b);
+    b->gp[0] = pt[0], b->gp[1] = pt[1], b->gp[2] = pt[2];
+    if (b->sf == 0.0) b->sf = 1.0;
+    b->mode = GOTO;
+}
+
+void or_goto_point(OPark* p, int64_t id, double x, double y, double z) {
+    OBall* b = find(p, id);
+    double pt[3] = {x, y, z};
+    if (b) goto_point_ball(p, b, pt);
+}
+
+static void goto_direction_ball(OPark* p, OBall* b, const double d0[3]) {   /* modes.py:252-257 */
+    if (!b || !finite3(d0)) return;
+    double d[3], pt[3];
+    normalized(d0, d);
+    for (int i = 0; i < 3; ++i) pt[i] = b->pos[i] + d[i] * 1.0e17;
+    goto_point_ball(p, b, pt);
+}
+
+void or_goto_direction(OPark* p, int64_t id, double x, double y, double z) {
+    double d[3] = {x, y, z};
+    goto_direction_ball(p, find(p, id), d);
+}
+
+static void follow_cmd(OPark* p, int64_t src, int64_t dst, double range, int64_t mode) {   /* modes.py:267-282 */
+    double rng = c_float(range);
+    if (!isfinite(rng)) return;
+    OBall* ball = find(p, src);
+    if (!ball || src == dst) return;
+    OBall* dest = find(p, dst);
+    if (!dest || dest->is_moribund || ball->bubble != dest->bubble || dest->cloaked) return;
+    stop_ball(p, ball);
+    ball->follow_id = dst;
+    ball->follow_ptr = dst;
+    ball->follow_range = rng;
+    ball->mode = mode;
+    fol_add(dest, ball->id);
+}
+
+void or_follow(OPark* p, int64_t src, int64_t dst, double range) { follow_cmd(p, src, dst, range, FOLLOW); }
+void or_orbit(OPark* p, int64_t src, int64_t dst, double range) { follow_cmd(p, src, dst, range, ORBIT); }
+
+void or_warp_to(OPark* p, int64_t id, double x, double y, double z, double min_range, int64_t warp_factor) {
+    /* WarpTo, modes.py:284-312 */
+    if (!(isfinite(x) && isfinite(y) && isfinite(z))) return;
+    if (min_range < 0.0) min_range = 0.0;
+    if (warp_factor <= 0) warp_factor = 1;
+    OBall* b = find(p, id);
+    if (!b) return;
+    double dst[3] = {x, y, z};
+    double d[3] = {b->pos[0] - x, b->pos[1] - y, b->pos[2] - z};
+    if (lensq(d) < MIN_WARP_DISTANCE_SQ) {
+        goto_point_ball(p, b, dst);
+        return;
+    }
+    if (b->mode == WARP && dst[0] == b->gp[0] && dst[1] == b->gp[1] && dst[2] == b->gp[2]) return;
+    stop_ball(p, b);
+    b->gp[0] = x, b->gp[1] = y, b->gp[2] = z;
+    b->effect_stamp = -1;
+    b->follow_range = 0.0;
+    b->follow_id = bits_of(min_range);
+    b->owner_id = warp_factor;
+    b->mode = WARP;
+}
+
+static void stop_all_followers(OPark* p, OBall* b) {               /* modes.py:410-426, ascending id */
+    if (!b->nf) return;
+    int64_t order[MAX_FOLLOWERS];
+    int nf = b->nf;
+    memcpy(order, b->followers, sizeof(int64_t) * nf);
+    qsort(order, nf, sizeof(int64_t), cmp_i64);
+    for (int i = 0; i < nf; ++i) {
+        OBall* f = find(p, order[i]);
+        if (!f || f->follow_id != b->id || f->follow_ptr != b->id) {
+            fol_discard(b, order[i]);
+            continue;
+        }
+        if (f->mode == MISSILE || (f->is_interactive && f->mode == ORBIT)) {
+            if (f->mode == MISSILE) f->is_massive = 0;
+            double v[3] = {f->vel[0], f->vel[1], f->vel[2]};
+            goto_direction_ball(p, f, v);
+        } else {
+            stop_ball(p, f);
+        }
+    }
+}
+
+static void real_warp(OPark* p, OBall* b) {                        /* modes.py:391-403 */
+    stop_all_followers(p, b);
+    double dst[3] = {b->gp[0], b->gp[1], b->gp[2]};
+    double delta0[3] = {b->pos[0] - dst[0], b->pos[1] - dst[1], b->pos[2] - dst[2]}, delta[3];
+    normalized(delta0, delta);
+    double mr = double_of(b->follow_id);
+    for (int i = 0; i < 3; ++i) dst[i] = dst[i] + delta[i] * mr;
+    for (int i = 0; i < 3; ++i) delta[i] = dst[i] - b->pos[i];
+    b->gp[0] = dst[0], b->gp[1] = dst[1], b->gp[2] = dst[2];
+    b->effect_stamp = p->current_time;
+    b->last_collision = length(delta);
+    b->is_massive = 0;
+    b->sensor = 0;
+}
+
+void or_entity_warp_in(OPark* p, int64_t id, double x, double y, double z, int64_t warp_factor) {
+    /* EntityWarpIn, modes.py:314-330 */
+    or_warp_to(p, id, x, y, z, 0.0, warp_factor);
+    OBall* b = find(p, id);
+    if (!b || b->mode == GOTO) return;
+    real_warp(p, b);
+    double d0[3] = {x, y, z}, d[3];
+    normalized(d0, d);
+    for (int i = 0; i < 3; ++i) b->vel[i] = d[i] * AU;
+    b->effect_stamp = p->current_time - 5 > 0 ? p->current_time - 5 : 0;
+    double back[3] = {b->pos[0] - b->gp[0], b->pos[1] - b->gp[1], b->pos[2] - b->gp[2]};
+    b->last_collision = length(back);
+}
+
+/* --------------------------------------------------------------------------------------------- behaviours */
+
+static void goto_thrust(const OPark* p, const OBall* b, double out[3]) {   /* ref.py:151-160 */
+    double a[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]};
+    double length2 = lensq(a);
+    double dist = b->sf * b->max_vel * p->dt;
+    double max_thrust = p->friction * b->sf * b->max_vel / (b->mass * b->agility);
+    double unit[3];
+    normalized(a, unit);
+    double scale = max_thrust;
+    if (length2 < dist * dist) {
+        double coff = length2 / (dist * dist);
+        scale = max_thrust * coff * coff;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = unit[i] * scale;
+}
+
+static void evolve_follow(OPark* p, OBall* b, double out[3]) {            /* modes.py:428-442 */
+    OBall* other = find(p, b->follow_ptr);
+    double op[3] = {other->pos[0], other->pos[1], other->pos[2]};
+    double delta[3] = {b->pos[0] - op[0], b->pos[1] - op[1], b->pos[2] - op[2]};
+    double dist = length(delta);
+    double r = b->follow_range + b->radius + other->radius;
+    if (dist == 0.0) {
+        b->gp[0] = op[0] + 1.0 * r, b->gp[1] = op[1] + 0.0 * r, b->gp[2] = op[2] + 0.0 * r;
+    } else {
+        double inv = 1.0 / dist;
+        for (int i = 0; i < 3; ++i) b->gp[i] = op[i] + delta[i] * r * inv;
+    }
+    goto_thrust(p, b, out);
+}
+
+static void precessing_w(const OPark* p, const OBall* b, double w[3]) {   /* modes.py:454-458, 480-486 */
+    double phi1 = p->current_time * ORBITAL_PRECESSION;
+    double phi2 = (double)(b->id & 0x000000000000FFFFLL) + p->current_time * ORBITAL_PRECESSION;
+    double s1, c1, s2, c2;
+    xb_sincos(phi1, &s1, &c1);
+    xb_sincos(phi2, &s2, &c2);
+    w[0] = trunc7(c1 * c2), w[1] = trunc7(s2), w[2] = trunc7(s1 * c2);
+}
+
+static void evolve_old_orbit(OPark* p, OBall* b, double out[3]) {         /* modes.py:444-478 */
+    double cruise = b->sf * b->max_vel;
+    double max_thrust = p->friction * cruise / (b->mass * b->agility);
+    OBall* other = find(p, b->follow_ptr);
+    double r = b->follow_range + b->radius + other->radius;
+    double to0[3] = {other->pos[0] - b->pos[0], other->pos[1] - b->pos[1], other->pos[2] - b->pos[2]}, to[3];
+    double dist = length(to0);
+    normalized(to0, to);
+    double wt[3], wc[3], w[3];
+    precessing_w(p, b, wt);
+    cross(wt, to, wc);
+    normalized(wc, w);
+    double to_comp = dist * dist - r * r;
+    if (to_comp >= 0.0) {
+        double rad_comp = r * sqrt(to_comp) / dist;
+        double s = to_comp / dist;
+        double t2[3];
+        for (int i = 0; i < 3; ++i) t2[i] = to[i] * s + w[i] * rad_comp;
+        normalized(t2, to);
+    }
+    double rf = trunc7(xb_exp(-(r - dist) * (r - dist) / 40000.0));
+    double phi1 = (-to[0]) * w[0] + (-to[1]) * w[1] + (-to[2]) * w[2];
+    double tf = 1.0 + rf * rf * (phi1 * phi1 - 1.0);
+    if (tf > 0.0) tf = rf * phi1 + sqrt(tf);
+    else tf = rf * phi1;
+    tf *= (dist - r >= 0.0) ? 1.0 : -1.0;
+    for (int i = 0; i < 3; ++i) out[i] = (w[i] * rf + to[i] * tf) * max_thrust;
+    for (int i = 0; i < 3; ++i) b->gp[i] = b->pos[i] + out[i] * (10.0 * AU);
+}
+
+static void orbit_thrust(const OPark* p, const OBall* b, const double target[3], double out[3]) {   /* 586-601 */
+    double fom = p->friction / (b->mass * b->agility);
+    double max_accel = fom * b->sf * b->max_vel;
+    out[0] = out[1] = out[2] = 0.0;
+    if (max_accel <= 0) return;
+    double a[3];
+    for (int i = 0; i < 3; ++i) a[i] = (target[i] - b->pos[i]) * fom - b->vel[i];
+    double tacc = length(a) / max_accel;
+    if (tacc > 1) {
+        double inv = 1.0 / tacc;
+        for (int i = 0; i < 3; ++i) a[i] = a[i] * inv;
+    }
+    if (tacc < 0.1) return;
+    out[0] = a[0], out[1] = a[1], out[2] = a[2];
+}
+
+static void goto_thrust_follow(const OPark* p, const OBall* b, const double target[3], const double tv[3],
+                               double weight, double out[3]) {             /* modes.py:557-584 */
+    double speed = length(tv);
+    if (speed < 5e-2 * b->max_vel * b->sf) {
+        if (p->use_new_orbit && b->mode == ORBIT) {
+            orbit_thrust(p, b, target, out);
+            return;
+        }
+        goto_thrust(p, b, out);
+        return;
+    }
+    double fom = p->friction / (b->mass * b->agility);
+    double max_accel = fom * b->sf * b->max_vel;
+    out[0] = out[1] = out[2] = 0.0;
+    if (max_accel <= 0) return;
+    double one_minus = 1 - b->tf;
+    double inv = 1.0 / (p->dt * fom - one_minus);
+    double a[3], avel[3];
+    for (int i = 0; i < 3; ++i) a[i] = ((target[i] - b->pos[i]) * fom - b->vel[i] * one_minus) * fom * inv;
+    inv = 1.0 / (1.0 - b->tf);
+    for (int i = 0; i < 3; ++i) avel[i] = (tv[i] - b->vel[i] * b->tf) * fom * inv;
+    inv = 1.0 / (1.0 + weight);
+    for (int i = 0; i < 3; ++i) a[i] = (a[i] + avel[i] * weight) * inv;
+    double accel = lensq(a);
+    if (accel > max_accel * max_accel) {
+        double s = max_accel / sqrt(accel);
+        for (int i = 0; i < 3; ++i) a[i] = a[i] * s;
+    }
+    out[0] = a[0], out[1] = a[1], out[2] = a[2];
+}
+
+static void evolve_new_orbit(OPark* p, OBall* b, double out[3]) {          /* modes.py:488-555 */
+    double cruise = b->sf * b->max_vel;
+    double tau = (b->mass * b->agility) / p->friction;
+    double max_thrust = cruise / tau;
+    OBall* other = find(p, b->follow_ptr);
+    double r = b->follow_range + b->radius + other->radius;
+    /* GetOrbitalNormal (modes.py:488-507) */
+    double to[3];
+    for (int i = 0; i < 3; ++i) to[i] = other->pos[i] - b->pos[i];
+    for (int i = 0; i < 3; ++i) to[i] = to[i] + other->vel[i] * p->dt;
+    double dist = length(to);
+    if (dist > 0) {
+        double inv = 1.0 / dist;
+        for (int i = 0; i < 3; ++i) to[i] = to[i] * inv;
+    } else {
+        to[0] = 0.0, to[1] = 0.0, to[2] = 1.0;
+    }
+    double o[3], normal[3];
+    cross(to, b->vel, o);
+    double length2 = lensq(o);
+    if (length2 > 1e-12) {
+        double inv = 1.0 / sqrt(length2);
+        for (int i = 0; i < 3; ++i) normal[i] = o[i] * inv;
+    } else {
+        precessing_w(p, b, normal);
+    }
+    double rc[3], radial[3];
+    cross(normal, to, rc);
+    normalized(rc, radial);
+    double speed = length(b->vel);
+    double max_speed = (speed > cruise) ? speed : cruise;              /* max(cruise, |v|) */
+    double angle = 0.1 * (r - dist) / max_speed;
+    if (angle < -1) angle = -1.0;
+    if (angle > 1) angle = 1.0;
+    if (dist < 0.1) dist = 0.1;
+    double sq = sqrt(dist * max_thrust);
+    double est_v = (sq < max_speed) ? sq : max_speed;                   /* min(maxSpeed, sqrt(...)) */
+    double est_d = est_v * p->dt;
+    double gp[3], gv[3];
+    if (fabs(angle) < 1e-10) {
+        double t = est_d / dist, st, ct;
+        xb_sincos(t, &st, &ct);
+        for (int i = 0; i < 3; ++i) gp[i] = to[i] * (dist * (1.0 - ct)) + radial[i] * (dist * st);
+        for (int i = 0; i < 3; ++i) gv[i] = radial[i] * (ct * est_v) + to[i] * (st * est_v);
+    } else {
+        double factor = est_d * angle / dist / sqrt(angle * angle + 1);
+        double t;
+        if (factor < -0.999) t = xb_log(0.001) / angle;
+        else if (fabs(factor) > 1e-10) t = xb_log(1 + factor) / angle;
+        else t = factor / angle;
+        double dexp = dist * xb_exp(angle * t), st, ct;
+        xb_sincos(t, &st, &ct);
+        for (int i = 0; i < 3; ++i) gp[i] = to[i] * (dist - dexp * ct) + radial[i] * (dexp * st);
+        double g0[3];
+        for (int i = 0; i < 3; ++i) g0[i] = to[i] * (-angle * ct + st) + radial[i] * (angle * st + ct);
+        normalized(g0, gv);
+        for (int i = 0; i < 3; ++i) gv[i] = gv[i] * est_v;
+    }
+    for (int i = 0; i < 3; ++i) b->gp[i] = b->pos[i] + gp[i];
+    for (int i = 0; i < 3; ++i) gv[i] = gv[i] + other->vel[i];
+    double target[3] = {b->gp[0], b->gp[1], b->gp[2]};
+    goto_thrust_follow(p, b, target, gv, 1.0, out);
+}
+
+static int is_aligned_for_warp(const OBall* b) {                         /* modes.py:342-355 */
+    if (!(b->mode == WARP && b->effect_stamp < 0)) return 0;
+    double g[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]}, d[3], u[3];
+    normalized(g, d);
+    normalized(b->vel, u);
+    if (fabs(1.0 - (u[0] * d[0] + u[1] * d[1] + u[2] * d[2])) < 0.01 &&
+        lensq(b->vel) > 0.5625 * b->max_vel * b->max_vel)
+        return 1;
+    int64_t es = b->effect_stamp < 0 ? -b->effect_stamp : b->effect_stamp;
+    return es > MAX_ALIGN_TICKS;
+}
+
+static void behaviour(OPark* p, OBall* b) {                              /* modes.py:359-389 */
+    double a[3] = {0.0, 0.0, 0.0};
+    b->last_c[0] = b->last_c[1] = b->last_c[2] = 0.0;
+    int64_t m = b->mode;
+    if (m == WARP) {
+        if (b->effect_stamp >= 0) {
+        } else if (is_aligned_for_warp(b)) {
+            real_warp(p, b);
+        } else {
+            b->effect_stamp -= 1;
+            goto_thrust(p, b, a);
+        }
+    } else if (m == GOTO) {
+        goto_thrust(p, b, a);
+    } else if (m == FOLLOW) {
+        evolve_follow(p, b, a);
+    } else if (m == ORBIT) {
+        if (p->use_new_orbit) evolve_new_orbit(p, b, a);
+        else evolve_old_orbit(p, b, a);
+    } else if (m == STOP) {
+        b->vel[1] = (b->vel[1] - 0.07 * b->vel[1]) * STOP_DAMP;
+    }
+    b->last_g[0] = a[0], b->last_g[1] = a[1], b->last_g[2] = a[2];
+}
+
+/* --------------------------------------------------------------------------------------- phase 3 and 4 */
+
+static void integrate(const OPark* p, const double pp[3], const double vv[3], const double a[3], double m,
+                      double tf, double pn[3], double vn[3]) {       /* ref.py:162-176 with t = dt */
+    double k = p->friction, t = p->dt;
+    double po[3] = {pp[0], pp[1], pp[2]}, vo[3] = {vv[0], vv[1], vv[2]};
+    if (t == 0.0) {
+        for (int i = 0; i < 3; ++i) pn[i] = po[i], vn[i] = vo[i];
+        return;
+    }
+    double k2 = k * k, ook2 = 1.0 / k2, ook = 1.0 / k;
+    if (k < 1e-10 * m * t) {
+        double inv_m = 1.0 / m;
+        for (int i = 0; i < 3; ++i)
+            pn[i] = (m * (a[i] * (-k * t) + k * (a[i] * t + vo[i] * k * inv_m * t)) + po[i] * k2) * ook2;
+    } else {
+        for (int i = 0; i < 3; ++i)
+            pn[i] = (m * (m * a[i] * (tf - 1.0) + k * (a[i] * t + vo[i] - tf * vo[i])) + po[i] * k2) * ook2;
+    }
+    for (int i = 0; i < 3; ++i) vn[i] = (m * a[i] - (m * a[i] - vo[i] * k) * tf) * ook;
+}
+
+static void warp_distance(OPark* p, OBall* b, double pp[3], double vv[3], double t) {   /* modes.py:605-666 */
+    double wf = (double)b->owner_id, wd = b->last_collision;
+    double s = wf * WARP_AU_PER_SECOND * AU;
+    double acc = wf * WARP_ACCELERATION;
+    double d0 = wf * WARP_DECELERATION;
+    double dec = (2.0 < d0) ? 2.0 : d0;
+    double cap = (wd + 1) * acc * dec / (acc + dec);
+    s = (cap < s) ? cap : s;
+    double t_a = xb_log(s / acc) / acc;
+    double t_c = (wd / s) - (1.0 / acc) - (1.0 / dec) + (1.0 / s);
+    double t_d = xb_log(s / dec) / dec;
+    double d_a = s / acc, d_c = s * t_c, d_d = s / dec - 1;
+    double g[3] = {b->gp[0] - pp[0], b->gp[1] - pp[1], b->gp[2] - pp[2]}, d[3];
+    normalized(g, d);
+    double rest;
+    int drop = 0;
+    if (t < t_a) {
+        double speed = acc * xb_exp(acc * t);
+        double cur = length(vv);
+        if (cur > speed) speed = cur;
+        for (int i = 0; i < 3; ++i) vv[i] = d[i] * speed;
+        double x = xb_exp(acc * t);
+        rest = d_a + d_c + d_d - x;
+        for (int i = 0; i < 3; ++i) pp[i] = b->gp[i] - d[i] * rest;
+    } else if ((t - t_a) < t_c) {
+        for (int i = 0; i < 3; ++i) vv[i] = d[i] * s;
+        double x = s * (t - t_a) + d_a;
+        rest = d_a + d_c + d_d - x;
+        for (int i = 0; i < 3; ++i) pp[i] = b->gp[i] - d[i] * rest;
+    } else if ((t - t_c - t_a) < (t_d + 1)) {
+        double speed = s * xb_exp(-dec * (t - t_c - t_a));
+        for (int i = 0; i < 3; ++i) vv[i] = d[i] * speed;
+        double x = s / dec - (s / dec) * xb_exp(-dec * (t - t_c - t_a)) + d_a + d_c;
+        rest = d_a + d_c + d_d - x;
+        for (int i = 0; i < 3; ++i) pp[i] = b->gp[i] - d[i] * rest;
+        double half = b->max_vel / 2.0;
+        if (speed < ((100.0 < half) ? 100.0 : half)) drop = 1;
+    } else {
+        for (int i = 0; i < 3; ++i) pp[i] = b->gp[i], vv[i] = 0.0;
+        drop = 1;
+    }
+    if (drop) {
+        b->is_massive = 1;
+        b->sensor = 1;
+        stop_ball(p, b);
+    }
+}
+
+void or_evolve(OPark* p) {                                              /* modes.py:702-745 */
+    double np[MAX_BALLS][3], nv[MAX_BALLS][3];
+    int live[MAX_BALLS];
+    for (int i = 0; i < p->n; ++i) {                                    /* (1) behaviours, ascending id */
+        OBall* b = &p->balls[i];
+        live[i] = b->is_free && !b->is_moribund;
+        if (live[i]) behaviour(p, b);
+    }
+    for (int i = 0; i < p->n; ++i) {                                    /* (3) end-of-tick states */
+        OBall* b = &p->balls[i];
+        if (!live[i]) continue;
+        if (b->is_massive) b->last_collision = -1.0;
+        double pp[3] = {b->pos[0], b->pos[1], b->pos[2]}, vv[3] = {b->vel[0], b->vel[1], b->vel[2]};
+        double a[3] = {b->last_g[0] + b->last_c[0], b->last_g[1] + b->last_c[1], b->last_g[2] + b->last_c[2]};
+        if (b->mode == WARP && b->effect_stamp >= 0) {
+            warp_distance(p, b, pp, vv, (double)(p->current_time - b->effect_stamp) * p->dt);
+            if (b->mode == STOP) {                                      /* TRV-714: from the drop-out point */
+                for (int k = 0; k < 3; ++k) b->pos[k] = pp[k], b->vel[k] = vv[k];
+                integrate(p, pp, vv, a, b->mass * b->agility, b->tf, np[i], nv[i]);
+            } else {
+                for (int k = 0; k < 3; ++k) np[i][k] = pp[k], nv[i][k] = vv[k];
+            }
+        } else {
+            integrate(p, pp, vv, a, b->mass * b->agility, b->tf, np[i], nv[i]);
+        }
+    }
+    for (int i = 0; i < p->n; ++i) {                                    /* (4) commit */
+        OBall* b = &p->balls[i];
+        if (!live[i]) continue;
+        for (int k = 0; k < 3; ++k) {
+            b->old_pos[k] = b->pos[k], b->old_vel[k] = b->vel[k];
+            b->pos[k] = np[i][k], b->vel[k] = nv[i][k];
+        }
+    }
+    p->current_time += 1;                                               /* (7) */
+}
+
+/* ------------------------------------------------------------------------------------------ state access */
+
+int or_count(const OPark* p) { return p->n; }
+int64_t or_time(const OPark* p) { return p->current_time; }
+
+/* Ball k (ascending id) as 27 doubles and 12 int64: pos, vel, goto, last_g, old_pos, old_vel (18), mass, radius,
+   max_vel, agility, sf, follow_range, tf, last_collision (26), pad; id, mode, effect_stamp, follow_id, owner_id,
+   follow_ptr (an id or -1), massive, sensor, interactive, free, moribund, number of followers. */
+int or_get_ball(const OPark* p, int k, double* d, int64_t* q) {
+    if (k < 0 || k >= p->n) return -1;
+    const OBall* b = &p->balls[k];
+    for (int i = 0; i < 3; ++i) {
+        d[i] = b->pos[i], d[3 + i] = b->vel[i], d[6 + i] = b->gp[i], d[9 + i] = b->last_g[i];
+        d[12 + i] = b->old_pos[i], d[15 + i] = b->old_vel[i];
+    }
+    d[18] = b->mass, d[19] = b->radius, d[20] = b->max_vel, d[21] = b->agility, d[22] = b->sf;
+    d[23] = b->follow_range, d[24] = b->tf, d[25] = b->last_collision, d[26] = 0.0;
+    q[0] = b->id, q[1] = b->mode, q[2] = b->effect_stamp, q[3] = b->follow_id, q[4] = b->owner_id;
+    q[5] = b->follow_ptr, q[6] = b->is_massive, q[7] = b->sensor, q[8] = b->is_interactive, q[9] = b->is_free;
+    q[10] = b->is_moribund, q[11] = b->nf;
+    return 0;
+}
+
+/* The follower set of ball k, ascending id; returns its size. */
+int or_get_followers(const OPark* p, int k, int64_t* out) {
+    if (k < 0 || k >= p->n) return -1;
+    const OBall* b = &p->balls[k];
+    memcpy(out, b->followers, sizeof(int64_t) * b->nf);
+    qsort(out, b->nf, sizeof(int64_t), cmp_i64);
+    return b->nf;
+}
+
+/* The maths the oracle calls, for the tests that pin it (0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 its cosine) */
+int or_maths(int func, int64_t n, const double* x, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        double s, c;
+        switch (func) {
+            case 0: out[i] = xb_sin(x[i]); break;
+            case 1: out[i] = xb_cos(x[i]); break;
+            case 2: out[i] = xb_log(x[i]); break;
+            case 3: out[i] = xb_exp(x[i]); break;
+            case 4: xb_sincos(x[i], &s, &c); out[i] = s; break;
+            case 5: xb_sincos(x[i], &s, &c); out[i] = c; break;
+            default: return -1;
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,244 @@
+"""Seeded flight-mode worlds for the parity tests: every world is a few balls and commands applied before given ticks.
+
+A world is a dict: friction, tick (ms), new_orbit (the orbit law), start (the world clock), steps (ticks to run),
+balls (AddBall's arguments and flags) and cmds, each (step, op, a, b, x, y, z, r, w): before tick `step` (1-based),
+command `op` on ball `a` (b: the target ball's id; x, y, z: a point or direction; r: a range; w: a warp factor).
+The kinds follow what the reference's own scenario generator exercises (tests/mode_cases.py:98-452 in the reference):
+FOLLOW, both ORBIT laws with near, at, far and on-centre starts, the new orbit's steep spiral, WARP (short, medium,
+long, aligned and unaligned starts, followers released before and after their own turn), EntityWarpIn, the guards,
+and low frictions (the integrator's Taylor branch).
+
+`flight_park(seed, worlds, ships)` is the flight-modes park workload: ships per world uniform in a +/-50 km cube, at
+rest, mass U(1e6, 1e8) kg, agility U(0.1, 5), max speed U(100, 3000) m/s, radius U(20, 500) m, each given one mode
+uniform over {stop, goto, follow, orbit, warp} at tick 1: goto a random point within 1e10 m, follow or orbit another
+ship of its world at U(500, 20000) m, warp to a random point (minimum range 20 km, warp factor 20).
+"""
+
+from __future__ import annotations
+
+import math
+import random
+import struct
+
+GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
+STOP, BOID, RIGID = 2, 7, 11
+AU = 0.1495978707e12
+
+
+def _f32(x: float) -> float:
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+def _unit(r: random.Random) -> list:
+    while True:
+        u = [r.gauss(0.0, 1.0) for _ in range(3)]
+        n = math.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+        if n > 1e-3:
+            return [c / n for c in u]
+
+
+def _ball(r: random.Random, bid: int, **kw) -> dict:
+    b = {"id": bid, "mass": 10 ** r.uniform(5.5, 9.5),
+         "radius": r.choice([0.0, r.uniform(1.0, 50.0), r.uniform(50.0, 500.0)]),
+         "max_vel": r.choice([r.uniform(50.0, 1500.0), r.uniform(50.0, 1500.0), r.uniform(1.0, 199.0)]),
+         "agility": r.uniform(0.05, 2.0), "sf": r.choice([1.0, r.uniform(0.1, 1.0)]),
+         "pos": [r.uniform(-1e5, 1e5) for _ in range(3)],
+         "vel": r.choice([[0.0, 0.0, 0.0], [r.uniform(-300.0, 300.0) for _ in range(3)]]),
+         "free": True, "massive": r.random() < 0.5, "interactive": r.random() < 0.5, "cloaked": 0, "bubble": 7,
+         "moribund": False, "mode": STOP}
+    b.update(kw)
+    return b
+
+
+def _cmd(step, op, a, b=0, x=0.0, y=0.0, z=0.0, rr=0.0, w=0) -> tuple:
+    return (step, op, a, b, float(x), float(y), float(z), float(rr), int(w))
+
+
+def _range(r: random.Random) -> float:
+    return r.choice([0.0, 1.0, r.uniform(0.0, 500.0), r.uniform(0.0, 5000.0), r.uniform(500.0, 30000.0),
+                     -r.uniform(0.0, 300.0)])
+
+
+def _r_of(rng: float, b1: dict, b2: dict) -> float:
+    return _f32(rng) + _f32(b1["radius"]) + _f32(b2["radius"])
+
+
+def _orbit_range(r: random.Random, b1: dict, b2: dict) -> float:
+    """Never r exactly 0: an old-style orbiter on its target's centre with r = 0 divides 0 by 0 and converts the NaN
+    to an integer, which C leaves undefined."""
+    rng = _range(r)
+    return 1.0 if _r_of(rng, b1, b2) == 0.0 else rng
+
+
+def _near(r: random.Random, centre: list, rr: float) -> list:
+    kind = r.choice(["inside", "at", "outside", "far", "on", "axis"])
+    if kind == "on":
+        return list(centre)
+    if kind == "axis":
+        return [centre[0] + abs(rr), centre[1], centre[2]]
+    d = {"inside": r.uniform(0.0, 1.0) * abs(rr), "at": abs(rr) * r.uniform(0.98, 1.02),
+         "outside": abs(rr) * r.uniform(1.0, 3.0) + r.uniform(0.0, 2000.0), "far": r.uniform(1e4, 2e5)}[kind]
+    u = _unit(r)
+    return [centre[i] + d * u[i] for i in range(3)]
+
+
+def _ids(r: random.Random, n: int, big: bool) -> list:
+    ids: set = set()
+    while len(ids) < n:
+        ids.add(r.randrange(1, 2 ** 40) if big else r.randrange(1, 1000))
+    out = sorted(ids)
+    if r.random() < 0.5:
+        r.shuffle(out)
+    return out
+
+
+def cases(seed: int, scale: int = 1) -> list:
+    """The mixed scenarios (about 200 x scale worlds) for seed."""
+    r = random.Random(seed)
+    out: list = []
+
+    def world(kind, balls, cmds, steps, *, tick=1000, new_orbit=False, friction=1.0e6, start=None):
+        out.append({"kind": kind, "friction": friction, "tick": tick, "steps": steps, "new_orbit": new_orbit,
+                    "start": r.randrange(0, 2_000_000) if start is None else start, "balls": balls, "cmds": cmds})
+
+    def friction_choice():
+        return r.choice([1.0e6] * 6 + [10 ** r.uniform(-2.0, 0.0), 10 ** r.uniform(3.0, 7.0)])
+
+    def tick_choice(long_ok=False):
+        ticks = [250, 500, 1000, 1000, 1000, 1000, 2000]
+        return r.choice(ticks + ([5000, 10000, 20000] if long_ok and r.random() < 0.3 else []))
+
+    for _ in range(30 * scale):                    # FOLLOW
+        ids = _ids(r, r.randint(2, 4), big=False)
+        target = _ball(r, ids[0], free=r.random() < 0.85)
+        balls, cmds = [target], []
+        if target["free"] and r.random() < 0.5:
+            cmds.append(_cmd(1, GOTO_OP, target["id"], x=r.uniform(-2e5, 2e5), y=r.uniform(-2e5, 2e5),
+                             z=r.uniform(-2e5, 2e5)))
+        for bid in ids[1:]:
+            rng = _range(r)
+            f = _ball(r, bid)
+            f["pos"] = _near(r, target["pos"], _r_of(rng, f, target))
+            balls.append(f)
+            cmds.append(_cmd(r.choice([1, 1, 1, 2, 5]), FOLLOW_OP, bid, target["id"], rr=rng))
+        if len(ids) > 2 and r.random() < 0.3:
+            cmds.append(_cmd(1, FOLLOW_OP, ids[1], ids[-1], rr=_range(r)))
+        world("follow", balls, cmds, r.randint(1, 30), tick=tick_choice(), friction=friction_choice())
+
+    for i in range(60 * scale):                    # ORBIT, both laws
+        new = i % 2 == 1
+        ids = _ids(r, r.randint(2, 4), big=True)
+        target = _ball(r, ids[0], free=r.random() < 0.85,
+                       vel=r.choice([[0.0, 0.0, 0.0], [r.uniform(-50, 50) for _ in range(3)]]))
+        balls, cmds = [target], []
+        if target["free"] and r.random() < 0.4:
+            cmds.append(_cmd(1, DIR_OP, target["id"], *_unit(r)))
+        for bid in ids[1:]:
+            o = _ball(r, bid)
+            rng = _orbit_range(r, o, target)
+            if r.random() < 0.15:
+                o["sf"] = 0.0
+            if r.random() < 0.3:
+                o["mass"], o["agility"] = 10 ** r.uniform(8.5, 9.5), r.uniform(1.0, 2.0)
+            o["pos"] = _near(r, target["pos"], _r_of(rng, o, target))
+            balls.append(o)
+            cmds.append(_cmd(r.choice([1, 1, 1, 3]), ORBIT_OP, bid, target["id"], rr=rng))
+        world("orbit_new" if new else "orbit_old", balls, cmds, r.randint(1, 40), tick=tick_choice(long_ok=new),
+              new_orbit=new, friction=friction_choice())
+
+    for _ in range(10 * scale):                    # the new orbit's steepest spiral
+        ids = _ids(r, 2, big=True)
+        target = _ball(r, ids[0], vel=[0.0, 0.0, 0.0])
+        o = _ball(r, ids[1], mass=10 ** r.uniform(5.5, 7.5), sf=1.0, max_vel=r.uniform(100.0, 1500.0))
+        long_tick = r.random() < 0.4
+        rng = r.uniform(0.0, 200.0) if long_tick else -r.uniform(1000.0, 20000.0)
+        u = _unit(r)
+        d = r.uniform(2000.0, 20000.0) if long_tick else r.uniform(1.0, 200.0)
+        o["pos"] = [target["pos"][i] + d * u[i] for i in range(3)]
+        world("orbit_new", [target, o], [_cmd(1, ORBIT_OP, ids[1], ids[0], rr=rng)], r.randint(1, 20),
+              tick=r.choice([10000, 20000]) if long_tick else 1000, new_orbit=True)
+
+    for _ in range(50 * scale):                    # WARP, alone or with followers
+        ids = _ids(r, r.randint(1, 4), big=False)
+        ship = _ball(r, ids[0], interactive=r.random() < 0.5)
+        kind = r.choice(["long", "long", "medium", "medium", "short", "still"])
+        dist = {"long": 10 ** r.uniform(9.0, 12.0), "medium": 10 ** r.uniform(5.0, 9.0),
+                "short": r.uniform(0.0, 99999.0), "still": 10 ** r.uniform(6.0, 10.0)}[kind]
+        u = _unit(r)
+        dest = [ship["pos"][i] + dist * u[i] for i in range(3)]
+        w = r.choice([r.randint(1500, 6000), r.randint(1500, 20000), 3000, 10000, 20])
+        min_range = r.choice([0.0, 20000.0, r.uniform(0.0, 1e6), -r.uniform(1.0, 100.0), dist * r.uniform(1.0, 1.5)])
+        if kind != "still" and r.random() < 0.7:   # aligned at speed: activates at once
+            sp = ship["max_vel"] * r.uniform(0.8, 1.0)
+            ship["vel"] = [u[i] * sp for i in range(3)]
+        balls = [ship]
+        cmds = [_cmd(1, WARP_OP, ship["id"], x=dest[0], y=dest[1], z=dest[2], rr=min_range, w=w)]
+        for bid in ids[1:]:                        # followers and orbiters of the warper, some interactive
+            f = _ball(r, bid, interactive=r.random() < 0.6)
+            f["pos"] = _near(r, ship["pos"], r.uniform(100.0, 5000.0))
+            balls.append(f)
+            cmds.append(_cmd(1, r.choice([FOLLOW_OP, ORBIT_OP]), bid, ship["id"], rr=r.uniform(0.0, 3000.0)))
+        if r.random() < 0.2:
+            cmds.append(_cmd(r.randint(2, 6), r.choice([STOP_OP, WARP_OP]), ship["id"], x=dest[0], y=dest[1],
+                             z=dest[2], rr=min_range, w=w))
+        world("warp", balls, cmds, r.randint(5, 40), tick=tick_choice(long_ok=True), new_orbit=r.random() < 0.5,
+              friction=r.choice([1.0e6, 1.0e6, 10 ** r.uniform(3.0, 7.0)]))
+
+    for _ in range(15 * scale):                    # EntityWarpIn, with a follower
+        ids = _ids(r, 2, big=False)
+        ship = _ball(r, ids[0])
+        f = _ball(r, ids[1], interactive=True)
+        u = _unit(r)
+        pt = [ship["pos"][i] + u[i] * 10 ** r.uniform(6.0, 11.0) for i in range(3)]
+        cmds = [_cmd(1, ORBIT_OP, ids[1], ids[0], rr=r.uniform(0.0, 3000.0)),
+                _cmd(r.randint(1, 3), WARPIN_OP, ids[0], x=pt[0], y=pt[1], z=pt[2], w=r.choice([3, 20, 3000]))]
+        world("warpin", [ship, f], cmds, r.randint(2, 25), tick=tick_choice(), start=r.choice([0, 3, 100, 5000]))
+
+    for _ in range(20 * scale):                    # GOTO, direction, STOP, guards, inert and fixed balls
+        ids = _ids(r, r.randint(2, 4), big=r.random() < 0.5)
+        balls = [_ball(r, bid, mode=r.choice([STOP, STOP, STOP, BOID, RIGID]), free=r.random() < 0.9,
+                       moribund=r.random() < 0.1, cloaked=r.choice([0, 0, 0, 1]), bubble=r.choice([7, 7, 7, 8]))
+                 for bid in ids]
+        cmds = []
+        for _ in range(r.randint(1, 6)):
+            a, b = r.choice(ids), r.choice(ids + [999999])
+            op = r.choice([GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP])
+            p = [r.uniform(-1e6, 1e6) for _ in range(3)]
+            rr = r.choice([r.uniform(0.0, 2000.0), float("nan"), float("inf"), 1e39])
+            cmds.append(_cmd(r.randint(1, 4), op, a, b, x=p[0], y=p[1], z=p[2], rr=rr))
+        cmds.sort(key=lambda c: c[0])
+        world("guards", balls, cmds, r.randint(1, 15), tick=tick_choice(), friction=friction_choice())
+    return out
+
+
+def flight_park(seed: int, worlds: int, ships: int = 4, start: int = 0) -> list:
+    """The flight-modes park: `worlds` worlds of `ships` ships, modes uniform over stop, goto, follow, orbit and warp
+    given at tick 1 (each world's own clock starts at `start`)."""
+    r = random.Random(seed)
+    out = []
+    for _ in range(worlds):
+        ids = list(range(1, ships + 1))
+        balls = []
+        for bid in ids:
+            balls.append({"id": bid, "mass": r.uniform(1e6, 1e8), "radius": r.uniform(20.0, 500.0),
+                          "max_vel": r.uniform(100.0, 3000.0), "agility": r.uniform(0.1, 5.0), "sf": 1.0,
+                          "pos": [r.uniform(-5e4, 5e4) for _ in range(3)], "vel": [0.0, 0.0, 0.0], "free": True,
+                          "massive": True, "interactive": r.random() < 0.5, "cloaked": 0, "bubble": 0,
+                          "moribund": False, "mode": STOP})
+        cmds = []
+        for bid in ids:
+            mode = r.randrange(5)
+            others = [o for o in ids if o != bid]
+            if mode == 1:
+                u, d = _unit(r), r.uniform(0.0, 1e10)
+                cmds.append(_cmd(1, GOTO_OP, bid, x=u[0] * d, y=u[1] * d, z=u[2] * d))
+            elif mode == 2 and others:
+                cmds.append(_cmd(1, FOLLOW_OP, bid, r.choice(others), rr=r.uniform(500.0, 20000.0)))
+            elif mode == 3 and others:
+                cmds.append(_cmd(1, ORBIT_OP, bid, r.choice(others), rr=r.uniform(500.0, 20000.0)))
+            elif mode == 4:
+                u, d = _unit(r), r.uniform(0.0, 1e10)
+                cmds.append(_cmd(1, WARP_OP, bid, x=u[0] * d, y=u[1] * d, z=u[2] * d, rr=20000.0, w=20))
+        out.append({"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": 0, "new_orbit": False,
+                    "start": start, "balls": balls, "cmds": cmds})
+    return out
