@@ -1,0 +1,111 @@
+// SPDX-License-Identifier: MIT
+//
+// The CPU batched path: the same tick source as the gfx950 kernels (xb_modes_tick.h) built for the host, the worlds'
+// groups run in parallel with OpenMP. It serves as the host cross-check of the kernels and as the CPU baseline the
+// benchmark times beside the GPU. Built with g++ -ffp-contract=off, no fast-math.
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#define XB_HD
+#include "xb_modes_tick.h"
+#include "../../include/xb.h"
+
+namespace {
+
+XbPark host_park(const xb_modes_state& s) {
+    XbPark P;
+    P.f = s.f64;
+    P.q = s.i64;
+    P.b = s.u8;
+    P.stride = s.stride;
+    P.friction = s.friction;
+    P.dt = s.dt;
+    P.k2 = s.friction * s.friction;
+    P.ook2 = 1.0 / P.k2;
+    P.ook = 1.0 / s.friction;
+    P.new_orbit = s.new_orbit;
+    return P;
+}
+
+struct Scratch {
+    std::vector<double> d;
+    std::vector<int64_t> id;
+    std::vector<uint8_t> act;
+    XbRecords R;
+    void size(int m) {
+        if ((int)act.size() >= m) return;
+        d.assign((size_t)8 * m, 0.0);
+        id.assign(m, 0);
+        act.assign(m, 0);
+        R.p0x = d.data();
+        R.p0y = R.p0x + m;
+        R.p0z = R.p0y + m;
+        R.v0x = R.p0z + m;
+        R.v0y = R.v0x + m;
+        R.v0z = R.v0y + m;
+        R.vy_after = R.v0z + m;
+        R.radius = R.vy_after + m;
+        R.id = id.data();
+        R.act = act.data();
+    }
+};
+
+void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, int64_t lo, int64_t cnt) {
+    S.size((int)cnt);
+    auto row = [&](int64_t k) -> int64_t { return s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k; };
+    for (int64_t k = 0; k < cnt; ++k) xb_record_a(P, S.R, row(k), (int)k);
+    for (int64_t k = 0; k < cnt; ++k) xb_record_b(P, S.R, row(k), (int)k, 0, (int)cnt);
+    for (int64_t k = 0; k < cnt; ++k) xb_tick_row(P, S.R, row(k), (int)k, 0, (int)cnt);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xb_cpu_abi(void) { return XB_ABI_VERSION; }
+
+// One tick of the park in host memory; tiles as xb_modes_tick's (NULL: the whole park is one group). threads <= 0:
+// OpenMP's default.
+int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
+    if (!state || state->n < 0 || state->stride < state->n || state->dynamical) return -1;
+    if (state->n == 0) return 0;
+    const XbPark P = host_park(*state);
+    if (!state->tiles) {
+        Scratch S;
+        run_group(P, *state, S, 0, state->n);
+        return 0;
+    }
+    const int nt = state->n_tiles;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+        Scratch S;
+#pragma omp for schedule(static)
+        for (int g = 0; g < nt; ++g)
+            run_group(P, *state, S, state->tiles[g], (int64_t)state->tiles[g + 1] - state->tiles[g]);
+    }
+    return 0;
+}
+
+int xb_cpu_maths_eval(int func, int64_t n, const double* x, double* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        double sn, cs;
+        switch (func) {
+            case 0: out[i] = xb_sin(x[i]); break;
+            case 1: out[i] = xb_cos(x[i]); break;
+            case 2: out[i] = xb_log(x[i]); break;
+            case 3: out[i] = xb_exp(x[i]); break;
+            case 4: xb_sincos(x[i], &sn, &cs); out[i] = sn; break;
+            case 5: xb_sincos(x[i], &sn, &cs); out[i] = cs; break;
+            case 6: out[i] = __builtin_sqrt(x[i]); break;
+            case 7: out[i] = x[i] / x[n + i]; break;
+            default: return -1;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// SPDX-License-Identifier: MIT
+//
+// The flight-modes tick on gfx950: one lane per ball row, one workgroup per tile of whole worlds.
+//
+// Tiled launch (the park's worlds laid out contiguously): a workgroup reads its rows' tick-start records into LDS
+// (step A), derives each row's vertical velocity after its own turn (step B), then runs every row's tick (step C)
+// with targets read from LDS. One pass over HBM: each state field is read once and written once, structure of
+// arrays, so a wavefront's loads and stores of one field are whole cache lines. The general launch (any row order)
+// runs the same three steps as three launches through caller-owned scratch.
+//
+// Exactness: compiled with -ffp-contract=off, no fast-math, denormals kept; the only fused operations are the maths
+// header's explicit ones. The tick logic is xb_modes_tick.h, the same source the host build (xb_cpu.cpp) compiles.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#define XB_HD __device__
+#define XB_TABLE __constant__
+#include "xb_modes_tick.h"
+#include "../../include/xb.h"
+
+namespace {
+
+__device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
+    XbPark P;
+    P.f = s.f64;
+    P.q = s.i64;
+    P.b = s.u8;
+    P.stride = s.stride;
+    P.friction = s.friction;
+    P.dt = s.dt;
+    P.k2 = s.friction * s.friction;
+    P.ook2 = 1.0 / P.k2;
+    P.ook = 1.0 / s.friction;
+    P.new_orbit = s.new_orbit;
+    return P;
+}
+
+constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 1;
+
+__global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
+    extern __shared__ double xb_lds[];
+    const int nb = blockDim.x;
+    const int g = blockIdx.x;
+    const int lo = s.tiles[g];
+    const int cnt = s.tiles[g + 1] - lo;
+    const int j = threadIdx.x;
+    XbRecords R;
+    R.p0x = xb_lds;
+    R.p0y = R.p0x + nb;
+    R.p0z = R.p0y + nb;
+    R.v0x = R.p0z + nb;
+    R.v0y = R.v0x + nb;
+    R.v0z = R.v0y + nb;
+    R.vy_after = R.v0z + nb;
+    R.radius = R.vy_after + nb;
+    R.id = reinterpret_cast<int64_t*>(R.radius + nb);
+    R.act = reinterpret_cast<uint8_t*>(R.id + nb);
+    const XbPark P = xb_park(s);
+    const bool on = j < cnt;
+    const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    if (on) xb_record_a(P, R, r, j);
+    __syncthreads();
+    if (on) xb_record_b(P, R, r, j, 0, cnt);
+    __syncthreads();
+    if (on) xb_tick_row(P, R, r, j, 0, cnt);
+}
+
+__device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
+    XbRecords R;
+    R.p0x = s.scratch_f64 + XB_S_POS0 * s.stride;
+    R.p0y = R.p0x + s.stride;
+    R.p0z = R.p0y + s.stride;
+    R.v0x = s.scratch_f64 + XB_S_VEL0 * s.stride;
+    R.v0y = R.v0x + s.stride;
+    R.v0z = R.v0y + s.stride;
+    R.vy_after = s.scratch_f64 + XB_S_VY_AFTER * s.stride;
+    R.radius = s.f64 + XB_F_RADIUS * s.stride;
+    R.id = s.i64 + XB_I_IDS * s.stride;
+    R.act = s.scratch_u8;
+    return R;
+}
+
+__global__ void __launch_bounds__(256) xb_modes_step_a(xb_modes_state s) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.n) return;
+    XbRecords R = xb_global_records(s);
+    const XbPark P = xb_park(s);
+    // the record's id and radius rows are the park's own (read only during the tick)
+    XbV3 pos = xb_ld3(P, XB_F_POS, r), vel = xb_ld3(P, XB_F_VEL, r);
+    int64_t es = XB_Q(P, XB_I_EFFECT_STAMP, r);
+    bool act = xb_live(P, r) && XB_Q(P, XB_I_MODE, r) == XB_MODE_WARP && es < 0 &&
+               xb_aligned(xb_ld3(P, XB_F_GOTO, r), pos, vel, XB_F(P, XB_F_MAX_VEL, r), es);
+    R.act[r] = act ? 1 : 0;
+    R.p0x[r] = pos.x, R.p0y[r] = pos.y, R.p0z[r] = pos.z;
+    R.v0x[r] = vel.x, R.v0y[r] = vel.y, R.v0z[r] = vel.z;
+}
+
+__global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.n) return;
+    xb_record_b(xb_park(s), xb_global_records(s), r, (int)r, 0, (int)s.n);
+}
+
+__global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.n) return;
+    xb_tick_row(xb_park(s), xb_global_records(s), r, (int)r, 0, (int)s.n);
+}
+
+__global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ x, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sn, cs;
+    switch (func) {
+        case 0: out[i] = xb_sin(x[i]); break;
+        case 1: out[i] = xb_cos(x[i]); break;
+        case 2: out[i] = xb_log(x[i]); break;
+        case 3: out[i] = xb_exp(x[i]); break;
+        case 4: xb_sincos(x[i], &sn, &cs); out[i] = sn; break;
+        case 5: xb_sincos(x[i], &sn, &cs); out[i] = cs; break;
+        case 6: out[i] = __builtin_sqrt(x[i]); break;
+        case 7: out[i] = x[i] / x[n + i]; break;
+        default: out[i] = xb_nan(); break;
+    }
+}
+
+int xb_launch_rc() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+bool xb_block_ok(int block) { return block > 0 && block <= 256 && block % 64 == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int xb_abi(void) { return XB_ABI_VERSION; }
+
+const char* xb_error_string(int rc) {
+    if (rc == 0) return "success";
+    if (rc < 0) return "refused before launch (block size, sizes or arguments)";
+    return hipGetErrorString((hipError_t)rc);
+}
+
+int xb_layout(int64_t* out) {
+    out[0] = XB_NF;
+    out[1] = XB_NI;
+    out[2] = XB_NB;
+    out[3] = XB_NS;
+    out[4] = (int64_t)sizeof(xb_modes_state);
+    return 0;
+}
+
+int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
+    if (!state || state->n < 0 || state->stride < state->n || !xb_block_ok(block)) return -1;
+    if (state->n == 0) return 0;
+    if (state->dynamical) return -1;                           // dynamical orientation: not in this build
+    if (!state->f64 || !state->i64 || !state->u8) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    if (state->tiles) {
+        if (state->n_tiles <= 0) return -1;
+        const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
+        xb_modes_tiled_kernel<<<state->n_tiles, block, lds, st>>>(*state);
+        return xb_launch_rc();
+    }
+    if (!state->scratch_f64 || !state->scratch_u8) return -1;
+    const unsigned grid = (unsigned)((state->n + block - 1) / block);
+    xb_modes_step_a<<<grid, block, 0, st>>>(*state);
+    int rc = xb_launch_rc();
+    if (rc) return rc;
+    xb_modes_step_b<<<grid, block, 0, st>>>(*state);
+    rc = xb_launch_rc();
+    if (rc) return rc;
+    xb_modes_step_c<<<grid, block, 0, st>>>(*state);
+    return xb_launch_rc();
+}
+
+int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block) {
+    if (n < 0 || func < 0 || func > 7 || !xb_block_ok(block)) return -1;
+    if (n == 0) return 0;
+    xb_maths_kernel<<<(unsigned)((n + block - 1) / block), block, 0, (hipStream_t)stream>>>(func, n, x, out);
+    return xb_launch_rc();
+}
+
+int xb_modes_info(int which, int64_t* out) {
+    hipFuncAttributes fa;
+    hipError_t e;
+    if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel));
+    else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c));
+    else if (which == 2) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_a));
+    else return -1;
+    if (e != hipSuccess) return (int)e;
+    out[0] = fa.numRegs;
+    out[1] = (int64_t)fa.localSizeBytes;
+    out[2] = (int64_t)fa.sharedSizeBytes;
+    out[3] = fa.maxThreadsPerBlock;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,582 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * One flight-modes tick of one ball row: destiny's Ballpark::Evolve (GOTO, STOP, FOLLOW, ORBIT under both laws, WARP
+ * with its alignment, activation, warp path and drop-out, the release of a warper's followers, the closed-form
+ * integrator), as the reference's batched park runs it per row (src/exact_ballpark/vec_modes.py:740-939 on tensors,
+ * src/exact_ballpark/vec_modes_cuda.cu:540-717 on CUDA) and its scalar world defines it (src/exact_ballpark/modes.py:
+ * 359-745, ref.py:151-211).
+ *
+ * A tick of a group of whole worlds runs in three steps, each over every row of the group:
+ *   A. `xb_record_a`: what the row shows the others in phase 1 -- its tick-start position and velocity, its id and
+ *      radius, whether it starts warp proper at its own turn (`act`);
+ *   B. `xb_record_b`: its vertical velocity after its own turn (STOP's damping, counting a release before its turn);
+ *   C. `xb_tick_row`: the row's whole tick, reading other rows only through those records.
+ * A row's follow target is a row of its own world, so a group of whole worlds needs nothing outside it: the GPU keeps
+ * the records of a workgroup's worlds in LDS, the host in arrays.
+ *
+ * Exactness: every operation is one correctly rounded IEEE operation in the reference's order; files including this
+ * are compiled with -ffp-contract=off and without fast-math, so nothing fuses except the maths header's explicit
+ * xb_fma steps (glibc's own). Float fields are held in doubles and only read, compared and stored here.
+ */
+#ifndef XB_MODES_TICK_H
+#define XB_MODES_TICK_H
+
+#include <stdint.h>
+
+#include "xb_layout.h"
+#include "xb_maths_double.h"
+
+#define XB_AU 0.1495978707e12                   /* Ballpark.h (modes.py:72) */
+#define XB_ORBITAL_PRECESSION 0.001             /* modes.py:73 */
+#define XB_WARP_AU_PER_SECOND 0.001             /* modes.py:74 */
+#define XB_WARP_ACCELERATION (1.0 / 1000)       /* modes.py:75 */
+#define XB_WARP_DECELERATION (1.0 / 3000)       /* modes.py:76 */
+#define XB_MAX_ALIGN_TICKS 180                  /* modes.py:77 */
+#define XB_STOP_DAMP 0.9345794392523364485981308411215   /* ref.py:37 */
+#define XB_STOP_DECAY 0.07                      /* EvolveStop, ref.py:203 */
+#define XB_TRUNC7 10000000.0                    /* modes.py:94-96 */
+#define XB_RF_SCALE 40000.0                     /* modes.py:467 */
+#define XB_GOTO_FAR (10.0 * XB_AU)              /* modes.py:477 */
+#define XB_NORMAL_MIN 1e-12                     /* modes.py:503 */
+#define XB_ANGLE_SCALE 0.1                      /* modes.py:519, 526 */
+#define XB_ANGLE_ZERO 1e-10                     /* modes.py:531, 542 */
+#define XB_FACTOR_FLOOR (-0.999)                /* modes.py:539 */
+#define XB_LOG_FLOOR 0.001                      /* modes.py:541 */
+#define XB_SLOW_TARGET 5e-2                     /* modes.py:560 */
+#define XB_THRUST_OFF 0.1                       /* modes.py:598 */
+#define XB_ALIGN_COS 0.01                       /* modes.py:348 */
+#define XB_ALIGN_SPEED2 0.5625                  /* modes.py:349 */
+#define XB_DROP_SPEED 100.0                     /* modes.py:650 */
+#define XB_DIR_DIST 1.0e17                      /* modes.py:257 */
+#define XB_TAYLOR_LIMIT 1e-10                   /* ref.py:168 */
+
+/* ----------------------------------------------------------------------------------- double 3-vectors */
+
+struct XbV3 {
+    double x, y, z;
+};
+
+static inline XB_HD XbV3 v3(double x, double y, double z) { XbV3 v = {x, y, z}; return v; }
+static inline XB_HD XbV3 v3_add(XbV3 a, XbV3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline XB_HD XbV3 v3_sub(XbV3 a, XbV3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline XB_HD XbV3 v3_scale(XbV3 a, double c) { return v3(a.x * c, a.y * c, a.z * c); }
+static inline XB_HD double v3_lensq(XbV3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static inline XB_HD double v3_len(XbV3 a) { return __builtin_sqrt(v3_lensq(a)); }
+static inline XB_HD XbV3 v3_unit(XbV3 a) {       /* Vector3d::Normalize: times 1/|a|; zero stays zero */
+    double norm = v3_len(a);
+    if (norm == 0.0) return a;
+    return v3_scale(a, 1.0 / norm);
+}
+static inline XB_HD XbV3 v3_cross(XbV3 u, XbV3 v) {
+    return v3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+static inline XB_HD bool xb_finite(double x) {
+    return (xb_bits(x) & 0x7ff0000000000000ULL) != 0x7ff0000000000000ULL;
+}
+static inline XB_HD bool v3_finite(XbV3 a) { return xb_finite(a.x) && xb_finite(a.y) && xb_finite(a.z); }
+
+/* (int64_t)x as aarch64 converts it: toward zero, saturating, NaN to 0 */
+static inline XB_HD int64_t xb_to_int64(double x) {
+    if (x != x) return 0;
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+
+static inline XB_HD double xb_trunc7(double x) { return (double)xb_to_int64(x * XB_TRUNC7) / XB_TRUNC7; }
+
+/* ----------------------------------------------------------------------------------------- the park */
+
+struct XbPark {
+    double* f;
+    int64_t* q;
+    uint8_t* b;
+    int64_t stride;
+    double friction, dt;
+    double k2, ook2, ook;          /* friction^2, 1/friction^2, 1/friction: the integrator's per-park constants */
+    int new_orbit;
+};
+
+/* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
+struct XbRecords {
+    uint8_t* act;
+    double *p0x, *p0y, *p0z, *v0x, *v0y, *v0z, *vy_after;
+    int64_t* id;
+    double* radius;
+};
+
+static inline XB_HD XbV3 xb_ld3(const XbPark& P, int f, int64_t r) {
+    return v3(P.f[f * P.stride + r], P.f[(f + 1) * P.stride + r], P.f[(f + 2) * P.stride + r]);
+}
+static inline XB_HD void xb_st3(const XbPark& P, int f, int64_t r, XbV3 v) {
+    P.f[f * P.stride + r] = v.x;
+    P.f[(f + 1) * P.stride + r] = v.y;
+    P.f[(f + 2) * P.stride + r] = v.z;
+}
+#define XB_F(P, fld, r) ((P).f[(int64_t)(fld) * (P).stride + (r)])
+#define XB_Q(P, fld, r) ((P).q[(int64_t)(fld) * (P).stride + (r)])
+#define XB_B(P, fld, r) ((P).b[(int64_t)(fld) * (P).stride + (r)])
+
+/* One ball as its thread holds it during the tick. */
+struct XbShip {
+    XbV3 pos, vel, gp, last_g;
+    double mass, tf, last_collision, radius, max_vel, agility, sf, follow_range;
+    int64_t id, mode, effect_stamp, follow_id, owner_id, follow_ptr, clock;
+    bool massive, interactive, sensor;
+};
+
+/* ------------------------------------------------------------------------------------------- modes */
+
+/* GotoThrust (ref.py:151-160): toward the target at maxThrust, tapered by (L^2 / D^2)^2 within one tick's distance */
+static inline XB_HD XbV3 xb_goto_thrust(const XbShip& s, XbV3 target, const XbPark& P) {
+    XbV3 a = v3_sub(target, s.pos);
+    double length2 = v3_lensq(a);
+    double dist = s.sf * s.max_vel * P.dt;
+    double max_thrust = P.friction * s.sf * s.max_vel / (s.mass * s.agility);
+    XbV3 unit = v3_unit(a);
+    double dd = dist * dist;
+    if (length2 < dd) {
+        double coff = length2 / dd;
+        return v3_scale(unit, max_thrust * coff * coff);
+    }
+    return v3_scale(unit, max_thrust);
+}
+
+/* EvolveFollow (modes.py:428-442): the point at followRange + both radii from the target's tick-start position */
+static inline XB_HD XbV3 xb_follow(XbShip& s, XbV3 op, double other_radius, const XbPark& P) {
+    XbV3 delta = v3_sub(s.pos, op);
+    double dist = v3_len(delta);
+    double r = s.follow_range + s.radius + other_radius;
+    if (dist == 0.0) {
+        s.gp = v3(op.x + 1.0 * r, op.y + 0.0 * r, op.z + 0.0 * r);
+    } else {
+        double inv = 1.0 / dist;
+        s.gp = v3(op.x + delta.x * r * inv, op.y + delta.y * r * inv, op.z + delta.z * r * inv);
+    }
+    return xb_goto_thrust(s, s.gp, P);
+}
+
+/* DefaultOrbitalNormal / the old orbit's w (modes.py:454-458, 480-486): truncated to 7 decimals, not normalised */
+static inline XB_HD XbV3 xb_precessing(const XbShip& s) {
+    double t = (double)s.clock;
+    double phi1 = t * XB_ORBITAL_PRECESSION;
+    double phi2 = (double)(s.id & 0xFFFFLL) + t * XB_ORBITAL_PRECESSION;
+    double s1, c1, s2, c2;
+    xb_sincos(phi1, &s1, &c1);
+    xb_sincos(phi2, &s2, &c2);
+    return v3(xb_trunc7(c1 * c2), xb_trunc7(s2), xb_trunc7(s1 * c2));
+}
+
+/* EvolveOldStyleOrbit (modes.py:444-478) */
+static inline XB_HD XbV3 xb_old_orbit(XbShip& s, XbV3 op, double other_radius, const XbPark& P) {
+    double cruise = s.sf * s.max_vel;
+    double max_thrust = P.friction * cruise / (s.mass * s.agility);
+    double r = s.follow_range + s.radius + other_radius;
+    XbV3 to = v3_sub(op, s.pos);
+    double dist = v3_len(to);
+    to = v3_unit(to);
+    XbV3 w = v3_unit(v3_cross(xb_precessing(s), to));
+    double to_comp = dist * dist - r * r;
+    if (to_comp >= 0.0) {
+        double rad_comp = r * __builtin_sqrt(to_comp) / dist;
+        double sc = to_comp / dist;
+        to = v3_unit(v3(to.x * sc + w.x * rad_comp, to.y * sc + w.y * rad_comp, to.z * sc + w.z * rad_comp));
+    }
+    double dr = r - dist;
+    double rf = xb_trunc7(xb_exp(-dr * dr / XB_RF_SCALE));
+    double phi = (-to.x) * w.x + (-to.y) * w.y + (-to.z) * w.z;
+    double tf = 1.0 + rf * rf * (phi * phi - 1.0);
+    if (tf > 0.0) tf = rf * phi + __builtin_sqrt(tf);
+    else tf = rf * phi;
+    tf = tf * ((dist - r >= 0.0) ? 1.0 : -1.0);
+    XbV3 a = v3((w.x * rf + to.x * tf) * max_thrust, (w.y * rf + to.y * tf) * max_thrust,
+                (w.z * rf + to.z * tf) * max_thrust);
+    s.gp = v3_add(s.pos, v3_scale(a, XB_GOTO_FAR));
+    return a;
+}
+
+/* OrbitThrust (modes.py:586-601) */
+static inline XB_HD XbV3 xb_orbit_thrust(const XbShip& s, XbV3 target, const XbPark& P) {
+    double fom = P.friction / (s.mass * s.agility);
+    double max_accel = fom * s.sf * s.max_vel;
+    if (max_accel <= 0.0) return v3(0.0, 0.0, 0.0);
+    XbV3 a = v3((target.x - s.pos.x) * fom - s.vel.x, (target.y - s.pos.y) * fom - s.vel.y,
+                (target.z - s.pos.z) * fom - s.vel.z);
+    double tacc = v3_len(a) / max_accel;
+    if (tacc > 1.0) a = v3_scale(a, 1.0 / tacc);
+    if (tacc < XB_THRUST_OFF) return v3(0.0, 0.0, 0.0);
+    return a;
+}
+
+/* GotoThrustFollow with weight 1 (modes.py:557-584); the new orbit is its only caller */
+static inline XB_HD XbV3 xb_goto_thrust_follow(const XbShip& s, XbV3 target, XbV3 tv, const XbPark& P) {
+    double speed = v3_len(tv);
+    if (speed < XB_SLOW_TARGET * s.max_vel * s.sf) {
+        if (P.new_orbit && s.mode == XB_MODE_ORBIT) return xb_orbit_thrust(s, target, P);
+        return xb_goto_thrust(s, s.gp, P);
+    }
+    double fom = P.friction / (s.mass * s.agility);
+    double max_accel = fom * s.sf * s.max_vel;
+    if (max_accel <= 0.0) return v3(0.0, 0.0, 0.0);
+    double one_minus = 1.0 - s.tf;
+    double inv = 1.0 / (P.dt * fom - one_minus);
+    XbV3 a = v3(((target.x - s.pos.x) * fom - s.vel.x * one_minus) * fom * inv,
+                ((target.y - s.pos.y) * fom - s.vel.y * one_minus) * fom * inv,
+                ((target.z - s.pos.z) * fom - s.vel.z * one_minus) * fom * inv);
+    inv = 1.0 / (1.0 - s.tf);
+    XbV3 av = v3((tv.x - s.vel.x * s.tf) * fom * inv, (tv.y - s.vel.y * s.tf) * fom * inv,
+                 (tv.z - s.vel.z * s.tf) * fom * inv);
+    const double weight = 1.0;
+    inv = 1.0 / (1.0 + weight);
+    a = v3((a.x + av.x * weight) * inv, (a.y + av.y * weight) * inv, (a.z + av.z * weight) * inv);
+    double accel = v3_lensq(a);
+    if (accel > max_accel * max_accel) a = v3_scale(a, max_accel / __builtin_sqrt(accel));
+    return a;
+}
+
+/* EvolveNewStyleOrbit with GetOrbitalNormal, dynamical orientation off (modes.py:488-555); ov: the target's velocity
+   as this ball sees it at its own turn */
+static inline XB_HD XbV3 xb_new_orbit(XbShip& s, XbV3 op, XbV3 ov, double other_radius, const XbPark& P) {
+    double cruise = s.sf * s.max_vel;
+    double tau = (s.mass * s.agility) / P.friction;
+    double max_thrust = cruise / tau;
+    double r = s.follow_range + s.radius + other_radius;
+    XbV3 to = v3_sub(op, s.pos);
+    to = v3(to.x + ov.x * P.dt, to.y + ov.y * P.dt, to.z + ov.z * P.dt);
+    double dist = v3_len(to);
+    if (dist > 0.0) to = v3_scale(to, 1.0 / dist);
+    else to = v3(0.0, 0.0, 1.0);
+    XbV3 o = v3_cross(to, s.vel);
+    double length2 = v3_lensq(o);
+    XbV3 normal = (length2 > XB_NORMAL_MIN) ? v3_scale(o, 1.0 / __builtin_sqrt(length2)) : xb_precessing(s);
+    XbV3 radial = v3_unit(v3_cross(normal, to));
+    double speed = v3_len(s.vel);
+    double max_speed = (cruise < speed) ? speed : cruise;
+    double angle = XB_ANGLE_SCALE * (r - dist) / max_speed;
+    if (angle < -1.0) angle = -1.0;
+    if (angle > 1.0) angle = 1.0;
+    if (dist < XB_ANGLE_SCALE) dist = XB_ANGLE_SCALE;
+    double sq = __builtin_sqrt(dist * max_thrust);
+    double est_v = (sq < max_speed) ? sq : max_speed;
+    double est_d = est_v * P.dt;
+    XbV3 gp, gv;
+    double st, ct;
+    if (xb_fabs(angle) < XB_ANGLE_ZERO) {
+        double t = est_d / dist;
+        xb_sincos(t, &st, &ct);
+        double cp = dist * (1.0 - ct), sp = dist * st, cv = ct * est_v, sv = st * est_v;
+        gp = v3(to.x * cp + radial.x * sp, to.y * cp + radial.y * sp, to.z * cp + radial.z * sp);
+        gv = v3(radial.x * cv + to.x * sv, radial.y * cv + to.y * sv, radial.z * cv + to.z * sv);
+    } else {
+        double factor = est_d * angle / dist / __builtin_sqrt(angle * angle + 1.0);
+        double t;
+        if (factor < XB_FACTOR_FLOOR) t = xb_log(XB_LOG_FLOOR) / angle;
+        else if (xb_fabs(factor) > XB_ANGLE_ZERO) t = xb_log(1.0 + factor) / angle;
+        else t = factor / angle;
+        double dexp = dist * xb_exp(angle * t);
+        xb_sincos(t, &st, &ct);
+        double cp = dist - dexp * ct, sp = dexp * st;
+        gp = v3(to.x * cp + radial.x * sp, to.y * cp + radial.y * sp, to.z * cp + radial.z * sp);
+        double ca = -angle * ct + st, sa = angle * st + ct;
+        gv = v3_scale(v3_unit(v3(to.x * ca + radial.x * sa, to.y * ca + radial.y * sa, to.z * ca + radial.z * sa)),
+                      est_v);
+    }
+    s.gp = v3_add(s.pos, gp);
+    return xb_goto_thrust_follow(s, s.gp, v3_add(gv, ov), P);
+}
+
+/* ------------------------------------------------------------------------------------------------ warp */
+
+/* IsAlignedForWarp of an aligning ball (modes.py:342-355) */
+static inline XB_HD bool xb_aligned(XbV3 gp, XbV3 pos, XbV3 vel, double max_vel, int64_t effect_stamp) {
+    XbV3 d = v3_unit(v3_sub(gp, pos));
+    XbV3 u = v3_unit(vel);
+    if (xb_fabs(1.0 - (u.x * d.x + u.y * d.y + u.z * d.z)) < XB_ALIGN_COS &&
+        v3_lensq(vel) > XB_ALIGN_SPEED2 * max_vel * max_vel)
+        return true;
+    return (effect_stamp < 0 ? -effect_stamp : effect_stamp) > XB_MAX_ALIGN_TICKS;
+}
+
+/* RealWarp, the warper's own part (modes.py:391-403): the exit point min range short of the goto point */
+static inline XB_HD void xb_real_warp(XbShip& s) {
+    XbV3 dst = s.gp;
+    XbV3 delta = v3_unit(v3_sub(s.pos, dst));
+    double mr = xb_dbl((uint64_t)s.follow_id);
+    dst = v3_add(dst, v3_scale(delta, mr));
+    delta = v3_sub(dst, s.pos);
+    s.gp = dst;
+    s.effect_stamp = s.clock;
+    s.last_collision = v3_len(delta);
+    s.massive = false;
+    s.sensor = false;
+}
+
+/* SetupWarpConstants and WarpDistance (modes.py:605-666): the new position and velocity on the warp line toward the
+   exit point; drops the ball out of warp (massive and sensing again, STOP) when slow enough or past deceleration */
+static inline XB_HD void xb_warp_path(XbShip& s, XbV3* pp, XbV3* vp, double t) {
+    double wf = (double)s.owner_id;
+    double wd = s.last_collision;
+    double sp = wf * XB_WARP_AU_PER_SECOND * XB_AU;
+    double acc = wf * XB_WARP_ACCELERATION;
+    double d0 = wf * XB_WARP_DECELERATION;
+    double dec = (2.0 < d0) ? 2.0 : d0;
+    double cap = (wd + 1.0) * acc * dec / (acc + dec);
+    sp = (cap < sp) ? cap : sp;
+    double t_a = xb_log(sp / acc) / acc;
+    double t_c = (wd / sp) - (1.0 / acc) - (1.0 / dec) + (1.0 / sp);
+    double t_d = xb_log(sp / dec) / dec;
+    double d_a = sp / acc, d_c = sp * t_c, d_d = sp / dec - 1.0;
+    XbV3 d = v3_unit(v3_sub(s.gp, *pp));
+    double total = d_a + d_c + d_d;
+    bool drop = false;
+    if (t < t_a) {
+        double e = xb_exp(acc * t);
+        double speed = acc * e;
+        double cur = v3_len(*vp);
+        if (cur > speed) speed = cur;
+        *vp = v3_scale(d, speed);
+        *pp = v3_sub(s.gp, v3_scale(d, total - e));
+    } else if ((t - t_a) < t_c) {
+        *vp = v3_scale(d, sp);
+        *pp = v3_sub(s.gp, v3_scale(d, total - (sp * (t - t_a) + d_a)));
+    } else if ((t - t_c - t_a) < (t_d + 1.0)) {
+        double e = xb_exp(-dec * (t - t_c - t_a));
+        double speed = sp * e;
+        *vp = v3_scale(d, speed);
+        double sd = sp / dec;
+        *pp = v3_sub(s.gp, v3_scale(d, total - (sd - sd * e + d_a + d_c)));
+        double half = s.max_vel / 2.0;
+        drop = speed < ((XB_DROP_SPEED < half) ? XB_DROP_SPEED : half);
+    } else {
+        *pp = s.gp;
+        *vp = v3(0.0, 0.0, 0.0);
+        drop = true;
+    }
+    if (drop) {
+        s.massive = true;
+        s.sensor = true;
+        s.mode = XB_MODE_STOP;
+    }
+}
+
+/* ------------------------------------------------------------------------- StopAllFollowers, Stop(Ball*) */
+
+static inline XB_HD void xb_clear_follow(XbShip& s) {
+    s.effect_stamp = 0;
+    s.follow_ptr = -1;
+    s.follow_id = 0;
+    s.owner_id = 0;
+    s.follow_range = 0.0;
+}
+
+/* The release of one follower of a ball that starts warp proper (modes.py:410-426): an interactive orbiter flies on
+   by GotoDirection along its velocity (refused, so left as it is, when the velocity or the point is not finite), any
+   other member stops (Stop(Ball*) clears the follow fields of a FOLLOW or ORBIT ball). Returns the mode it leaves. */
+static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
+    if (s.interactive && s.mode == XB_MODE_ORBIT) {
+        if (!v3_finite(s.vel)) return s.mode;
+        XbV3 p = v3_add(s.pos, v3_scale(v3_unit(s.vel), XB_DIR_DIST));
+        if (!v3_finite(p)) return s.mode;
+        if (apply) {
+            xb_clear_follow(s);
+            s.gp = p;
+            if (s.sf == 0.0) s.sf = 1.0;
+            s.mode = XB_MODE_GOTO;
+        }
+        return XB_MODE_GOTO;
+    }
+    if (apply) {
+        if (s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT) xb_clear_follow(s);
+        s.mode = XB_MODE_STOP;
+    }
+    return XB_MODE_STOP;
+}
+
+/* ---------------------------------------------------------------------------------------- integrator */
+
+/* Ballpark::Integrate over one tick (ref.py:162-176 with t = dt, the cached decay factor) */
+static inline XB_HD void xb_integrate(XbV3* pp, XbV3* vp, XbV3 a, double m, double tf, const XbPark& P) {
+    const double k = P.friction, t = P.dt;
+    if (t == 0.0) return;
+    XbV3 p = *pp, v = *vp;
+    XbV3 pn;
+    if (k < XB_TAYLOR_LIMIT * m * t) {
+        double inv_m = 1.0 / m;
+        double kt = -k * t;
+        pn = v3((m * (a.x * kt + k * (a.x * t + v.x * k * inv_m * t)) + p.x * P.k2) * P.ook2,
+                (m * (a.y * kt + k * (a.y * t + v.y * k * inv_m * t)) + p.y * P.k2) * P.ook2,
+                (m * (a.z * kt + k * (a.z * t + v.z * k * inv_m * t)) + p.z * P.k2) * P.ook2);
+    } else {
+        double tf1 = tf - 1.0;
+        pn = v3((m * (m * a.x * tf1 + k * (a.x * t + v.x - tf * v.x)) + p.x * P.k2) * P.ook2,
+                (m * (m * a.y * tf1 + k * (a.y * t + v.y - tf * v.y)) + p.y * P.k2) * P.ook2,
+                (m * (m * a.z * tf1 + k * (a.z * t + v.z - tf * v.z)) + p.z * P.k2) * P.ook2);
+    }
+    *vp = v3((m * a.x - (m * a.x - v.x * k) * tf) * P.ook, (m * a.y - (m * a.y - v.y * k) * tf) * P.ook,
+             (m * a.z - (m * a.z - v.z * k) * tf) * P.ook);
+    *pp = pn;
+}
+
+/* ------------------------------------------------------------------------------------ the three steps */
+
+static inline XB_HD bool xb_live(const XbPark& P, int64_t r) {
+    return XB_B(P, XB_B_FREE, r) && !XB_B(P, XB_B_MORIBUND, r);
+}
+
+/* Step A for row r at place j of its group */
+static inline XB_HD void xb_record_a(const XbPark& P, const XbRecords& R, int64_t r, int j) {
+    XbV3 pos = xb_ld3(P, XB_F_POS, r), vel = xb_ld3(P, XB_F_VEL, r);
+    int64_t es = XB_Q(P, XB_I_EFFECT_STAMP, r);
+    bool act = xb_live(P, r) && XB_Q(P, XB_I_MODE, r) == XB_MODE_WARP && es < 0 &&
+               xb_aligned(xb_ld3(P, XB_F_GOTO, r), pos, vel, XB_F(P, XB_F_MAX_VEL, r), es);
+    R.act[j] = act ? 1 : 0;
+    R.p0x[j] = pos.x, R.p0y[j] = pos.y, R.p0z[j] = pos.z;
+    R.v0x[j] = vel.x, R.v0y[j] = vel.y, R.v0z[j] = vel.z;
+    R.id[j] = XB_Q(P, XB_I_IDS, r);
+    R.radius[j] = XB_F(P, XB_F_RADIUS, r);
+}
+
+/* The place of row r's follow target in its group (rows of a world keep their order and spacing in the group), or -1
+   when it has none or the target is not in the follower set as StopAllFollowers tests it (its id is the follow id) */
+static inline XB_HD int xb_member_place(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
+    int64_t t = XB_Q(P, XB_I_FOLLOW_PTR, r);
+    if (t < 0) return -1;
+    int64_t tj = (int64_t)j + (t - r);
+    if (tj < lo || tj >= hi) return -1;
+    return R.id[tj] == XB_Q(P, XB_I_FOLLOW_ID, r) ? (int)tj : -1;
+}
+
+/* Step B: the vertical velocity row r shows after its own turn (no dynamical orientation) */
+static inline XB_HD void xb_record_b(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
+    double vy = R.v0y[j];
+    if (xb_live(P, r)) {
+        int64_t m = XB_Q(P, XB_I_MODE, r);
+        int tj = xb_member_place(P, R, r, j, lo, hi);
+        if (tj >= 0 && R.id[tj] < R.id[j] && R.act[tj]) {
+            XbShip f;
+            f.mode = m;
+            f.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
+            f.pos = v3(R.p0x[j], R.p0y[j], R.p0z[j]);
+            f.vel = v3(R.v0x[j], R.v0y[j], R.v0z[j]);
+            m = xb_release(f, false);
+        }
+        if (m == XB_MODE_STOP) vy = (vy - XB_STOP_DECAY * vy) * XB_STOP_DAMP;
+    }
+    R.vy_after[j] = vy;
+}
+
+/* Step C: row r's whole tick in place (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi) */
+static inline XB_HD void xb_tick_row(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
+    XbShip s;
+    s.pos = v3(R.p0x[j], R.p0y[j], R.p0z[j]);
+    s.vel = v3(R.v0x[j], R.v0y[j], R.v0z[j]);
+    s.gp = xb_ld3(P, XB_F_GOTO, r);
+    s.last_g = xb_ld3(P, XB_F_LAST_G, r);
+    s.mass = XB_F(P, XB_F_MASS, r);
+    s.tf = XB_F(P, XB_F_TIME_FACTOR, r);
+    s.last_collision = XB_F(P, XB_F_LAST_COLLISION, r);
+    s.radius = R.radius[j];
+    s.max_vel = XB_F(P, XB_F_MAX_VEL, r);
+    s.agility = XB_F(P, XB_F_AGILITY, r);
+    s.sf = XB_F(P, XB_F_SPEED_FRACTION, r);
+    s.follow_range = XB_F(P, XB_F_FOLLOW_RANGE, r);
+    s.id = R.id[j];
+    s.mode = XB_Q(P, XB_I_MODE, r);
+    s.effect_stamp = XB_Q(P, XB_I_EFFECT_STAMP, r);
+    s.follow_id = XB_Q(P, XB_I_FOLLOW_ID, r);
+    s.owner_id = XB_Q(P, XB_I_OWNER_ID, r);
+    s.follow_ptr = XB_Q(P, XB_I_FOLLOW_PTR, r);
+    s.clock = XB_Q(P, XB_I_CLOCK, r);
+    s.massive = XB_B(P, XB_B_MASSIVE, r) != 0;
+    s.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
+    s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
+    const bool live = xb_live(P, r);
+    const bool act_self = R.act[j] != 0;
+    /* phase 1: a release when the target starts warp proper, before or after this ball's own turn by their ids */
+    bool rel_before = false, rel_after = false;
+    int tj = xb_member_place(P, R, r, j, lo, hi);
+    if (tj >= 0 && R.act[tj]) {
+        if (R.id[tj] < s.id) rel_before = true;
+        else rel_after = true;
+    }
+    if (rel_before) xb_release(s, true);
+    if (live) {
+        XbV3 a = v3(0.0, 0.0, 0.0);
+        int64_t t = s.follow_ptr;
+        int64_t tp = (t >= 0) ? (int64_t)j + (t - r) : -1;
+        bool target_ok = t >= 0 && tp >= lo && tp < hi;
+        if (s.mode == XB_MODE_WARP) {
+            if (s.effect_stamp >= 0) {
+            } else if (act_self) {
+                xb_real_warp(s);
+            } else {
+                s.effect_stamp -= 1;
+                a = xb_goto_thrust(s, s.gp, P);
+            }
+        } else if (s.mode == XB_MODE_GOTO) {
+            a = xb_goto_thrust(s, s.gp, P);
+        } else if ((s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT) && !target_ok) {
+        } else if (s.mode == XB_MODE_FOLLOW) {
+            a = xb_follow(s, v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]), R.radius[tp], P);
+        } else if (s.mode == XB_MODE_ORBIT) {
+            XbV3 op = v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]);
+            if (P.new_orbit) {
+                XbV3 ov = v3(R.v0x[tp], (R.id[tp] < s.id) ? R.vy_after[tp] : R.v0y[tp], R.v0z[tp]);
+                a = xb_new_orbit(s, op, ov, R.radius[tp], P);
+            } else {
+                a = xb_old_orbit(s, op, R.radius[tp], P);
+            }
+        } else if (s.mode == XB_MODE_STOP) {
+            s.vel.y = (s.vel.y - XB_STOP_DECAY * s.vel.y) * XB_STOP_DAMP;
+        }
+        s.last_g = a;
+    }
+    if (rel_after) xb_release(s, true);
+    /* phases 3 and 4 */
+    if (live) {
+        if (s.massive) s.last_collision = -1.0;
+        XbV3 p = s.pos, v = s.vel;
+        XbV3 a = v3(s.last_g.x + 0.0, s.last_g.y + 0.0, s.last_g.z + 0.0);     /* mLastG + mLastC (zero) */
+        double m = s.mass * s.agility;
+        if (s.mode == XB_MODE_WARP && s.effect_stamp >= 0) {
+            xb_warp_path(s, &p, &v, (double)(s.clock - s.effect_stamp) * P.dt);
+            if (s.mode == XB_MODE_STOP) {                     /* TRV-714: one step more from the drop-out point */
+                s.pos = p;
+                s.vel = v;
+                xb_integrate(&p, &v, a, m, s.tf, P);
+            }
+        } else {
+            xb_integrate(&p, &v, a, m, s.tf, P);
+        }
+        xb_st3(P, XB_F_OLD_POS, r, s.pos);
+        xb_st3(P, XB_F_OLD_VEL, r, s.vel);
+        s.pos = p;
+        s.vel = v;
+    }
+    if (live && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
+        /* PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937): fixed, at rest, RIGID */
+        s.vel = v3(0.0, 0.0, 0.0);
+        xb_st3(P, XB_F_OLD_VEL, r, s.vel);
+        s.last_g = s.vel;
+        XB_B(P, XB_B_FREE, r) = 0;
+        XB_B(P, XB_B_INTERACTIVE, r) = 0;
+        s.mode = XB_MODE_RIGID;
+    }
+    xb_st3(P, XB_F_POS, r, s.pos);
+    xb_st3(P, XB_F_VEL, r, s.vel);
+    xb_st3(P, XB_F_GOTO, r, s.gp);
+    xb_st3(P, XB_F_LAST_G, r, s.last_g);
+    XB_F(P, XB_F_LAST_COLLISION, r) = s.last_collision;
+    XB_F(P, XB_F_SPEED_FRACTION, r) = s.sf;
+    XB_F(P, XB_F_FOLLOW_RANGE, r) = s.follow_range;
+    XB_Q(P, XB_I_MODE, r) = s.mode;
+    XB_Q(P, XB_I_EFFECT_STAMP, r) = s.effect_stamp;
+    XB_Q(P, XB_I_FOLLOW_ID, r) = s.follow_id;
+    XB_Q(P, XB_I_OWNER_ID, r) = s.owner_id;
+    XB_Q(P, XB_I_FOLLOW_PTR, r) = s.follow_ptr;
+    XB_Q(P, XB_I_CLOCK, r) = s.clock + 1;
+    XB_B(P, XB_B_MASSIVE, r) = s.massive ? 1 : 0;
+    XB_B(P, XB_B_SENSOR, r) = s.sensor ? 1 : 0;
+}
+
+#endif

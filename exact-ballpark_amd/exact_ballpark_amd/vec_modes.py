@@ -1,0 +1,566 @@
+"""The drop-in flight-modes park: `VecModesPark`, the reference's batched park API (src/exact_ballpark/vec_modes.py:193-
+960 and the setters and OnTick of src/exact_ballpark/vec_orient.py:293-879), on state laid out for the gfx950 kernel.
+
+The park's per-ball state lives in three device blocks, one per element type, one block row per field (structure of
+arrays; exact-ballpark_amd/csrc/xb_layout.h). Every attribute the reference's park exposes (`pos` [n, 3], `mass` [n],
+`mode` [n], `free` [n] bool, ...) is a view of its block rows with the reference's shape and dtype; assigning one copies
+into the block, so the blocks never move and a captured HIP graph keeps its buffers without a re-sync. `evolve()` is
+one launch of the HIP kernel (libxb_hip.so, xb_modes_tick) on torch's current stream, no host synchronisation; on a
+CPU park it is the host build of the same kernel source (libxb_cpu.so). The commands and setters are the reference's:
+elementwise selections between ticks, each a correctly rounded operation in the reference's order.
+
+Worlds: `world` groups the rows; a FOLLOW or ORBIT ball's `follow_ptr` is its target's row, in its world. When every
+world's rows are contiguous and no world has more rows than a workgroup (`block`), the park runs the tiled launch
+(whole worlds per workgroup, their tick-start records in LDS); otherwise the general launch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+F64, I64, U8 = torch.float64, torch.int64, torch.uint8
+GOTO, FOLLOW, STOP, WARP, ORBIT, MISSILE = 0, 1, 2, 3, 4, 5
+BOID, TROLL, MINIBALL, FIELD, RIGID, FORMATION = 7, 8, 9, 10, 11, 12
+INERT_MODES = (BOID, MINIBALL, FIELD, RIGID)
+LOADABLE = (STOP,) + INERT_MODES
+AU = 0.1495978707e12
+MIN_WARP_DISTANCE_SQ = 10000000000.0        # WarpTo's short warp (modes.py:78)
+MAX_CATCH_UP_TICKS = 5                      # OnTick's resynchronisation (full.py:92)
+TROLL_MIN_DELAY = 1                         # SetBallTroll (vec_modes.py:90)
+DIR_DIST = 1.0e17                           # GotoDirection's point
+
+# (name, block, first block row, columns; 0 for a [n] field)
+FIELDS = (
+    ("pos", "f", 0, 3), ("vel", "f", 3, 3), ("old_pos", "f", 6, 3), ("old_vel", "f", 9, 3), ("goto", "f", 12, 3),
+    ("last_g", "f", 15, 3), ("mass", "f", 18, 0), ("time_factor", "f", 19, 0), ("last_collision", "f", 20, 0),
+    ("radius", "f", 21, 0), ("max_vel", "f", 22, 0), ("agility", "f", 23, 0), ("speed_fraction", "f", 24, 0),
+    ("follow_range", "f", 25, 0), ("max_ang_vel", "f", 26, 0), ("rot_agility", "f", 27, 0), ("rot_roll", "f", 28, 0),
+    ("rot", "f", 29, 4), ("old_rot", "f", 33, 4), ("ang_vel", "f", 37, 3), ("old_ang_vel", "f", 40, 3),
+    ("torque", "f", 43, 3),
+    ("ids", "q", 0, 0), ("mode", "q", 1, 0), ("effect_stamp", "q", 2, 0), ("follow_id", "q", 3, 0),
+    ("owner_id", "q", 4, 0), ("follow_ptr", "q", 5, 0), ("clock", "q", 6, 0), ("world", "q", 7, 0),
+    ("cloaked", "q", 8, 0), ("bubble", "q", 9, 0),
+    ("free", "b", 0, 0), ("moribund", "b", 1, 0), ("massive", "b", 2, 0), ("interactive", "b", 3, 0),
+    ("sensor", "b", 4, 0),
+)
+FIELD_NAMES = tuple(f[0] for f in FIELDS)
+
+
+def f32(x: torch.Tensor) -> torch.Tensor:
+    """A double rounded to the nearest float and back (a C++ float member; out of range an infinity)."""
+    return x.to(torch.float32).to(F64)
+
+
+def _dot(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _sqrt(x: torch.Tensor) -> torch.Tensor:
+    """IEEE square root, correctly rounded, by the library's own kernel (or its host build): torch's CPU float64 sqrt
+    is not correctly rounded in every case."""
+    return N.maths("sqrt", x)
+
+
+def _unit(d: torch.Tensor) -> torch.Tensor:
+    """Vector3d::Normalize on rows: times the reciprocal of the length; a zero row stays as it is."""
+    norm = _sqrt(_dot(d, d))
+    inv = torch.ones_like(norm) / torch.where(norm == 0.0, 1.0, norm)
+    return torch.where((norm == 0.0)[:, None], d, d * inv[:, None])
+
+
+def _field_property(name: str, blk: str, row: int, cols: int):
+    def get(self):
+        if blk == "f":
+            base = self._f
+        elif blk == "q":
+            base = self._q
+        else:
+            base = self._b.view(torch.bool)
+        if cols:
+            return base[row:row + cols, :self.n].t()
+        return base[row, :self.n]
+
+    def set_(self, value):
+        dst = get(self)
+        if not torch.is_tensor(value):
+            value = torch.as_tensor(value)
+        dst.copy_(value.to(device=dst.device).expand_as(dst) if value.dim() < dst.dim() else value)
+
+    return property(get, set_, doc=f"[n{', ' + str(cols) if cols else ''}] view of the state block ({name})")
+
+
+class _Fields(type):
+    def __new__(mcs, name, bases, ns):
+        for fname, blk, row, cols in FIELDS:
+            ns.setdefault(fname, _field_property(fname, blk, row, cols))
+        return super().__new__(mcs, name, bases, ns)
+
+
+class VecModesPark(metaclass=_Fields):
+    """n balls, one per row, in independent worlds, with destiny's flight modes; `evolve()` is one kernel launch a
+    tick. The reference's constructor (src/exact_ballpark/vec_modes.py:199-217)."""
+
+    def __init__(self, n: int, friction: float = 1.0e6, tick_interval_ms: int = 1000, device="cuda",
+                 dynamical_orientation: bool = False, use_new_orbit: bool = False, block: int = 256,
+                 ypr: str = "host"):
+        if dynamical_orientation:
+            raise NotImplementedError("dynamical orientation is not in this build of the flight-modes kernel")
+        if ypr != "host":
+            raise ValueError('ypr must be "host" (the yaw, pitch and roll are display values, not kept here)')
+        self.n = int(n)
+        self.friction = float(friction)
+        self.tick_interval = int(tick_interval_ms)
+        self.dt = tick_interval_ms * 0.001
+        self.fdt = float(torch.tensor(self.dt, dtype=F64).to(torch.float32))
+        self.ffriction = float(torch.tensor(self.friction, dtype=F64).to(torch.float32))
+        self.dynamical_orientation = False
+        self.use_new_orbit = bool(use_new_orbit)
+        self.block = int(block)
+        self.dev = torch.device(device)
+        if self.dev.type == "cuda" and self.dev.index is None:
+            self.dev = torch.device("cuda", torch.cuda.current_device())
+        if self.dev.type == "cuda":
+            N.hip()                                  # loud failure now if the kernels are not built
+        else:
+            N.cpu()
+        self.stride = max(64, (self.n + 63) // 64 * 64)
+        self._f = torch.zeros(N.NF, self.stride, dtype=F64, device=self.dev)
+        self._q = torch.zeros(N.NI, self.stride, dtype=I64, device=self.dev)
+        self._b = torch.zeros(8, self.stride, dtype=U8, device=self.dev)
+        self._sf = torch.zeros(N.NS, self.stride, dtype=F64, device=self.dev)      # the general launch's scratch
+        self._sb = torch.zeros(self.stride, dtype=U8, device=self.dev)
+        self.current_time = 0
+        self.refiled = torch.zeros(self.n, dtype=I64, device=self.dev)
+        self.last_collision = -1.0
+        self.follow_range = 10.0
+        self.ids = torch.arange(1, self.n + 1, dtype=I64, device=self.dev)
+        self.follow_ptr = -1
+        self.bubble = -1
+        self.max_ang_vel = 1.0
+        self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
+        self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
+        self.old_rot = self.rot.clone()
+        self.tiles = None
+        self._state = None
+        self._set_tiles()
+
+    # ---- the state as the kernel takes it ----
+
+    def _set_tiles(self) -> None:
+        """Tiles of whole worlds for the tiled launch when every world's rows are contiguous and fit a workgroup;
+        None (the general launch) otherwise."""
+        self.tiles = None
+        self._state = None
+        n = self.n
+        if n == 0:
+            return
+        w = self.world.cpu().numpy()
+        starts = np.concatenate(([0], np.flatnonzero(w[1:] != w[:-1]) + 1))
+        sizes = np.diff(np.concatenate((starts, [n])))
+        if len(np.unique(w[starts])) != len(starts) or int(sizes.max()) > self.block:
+            return
+        tiles = [0]
+        for s0, size in zip(starts.tolist(), sizes.tolist()):
+            if s0 + size - tiles[-1] > self.block:
+                tiles.append(s0)
+        tiles.append(n)
+        self.tiles = torch.tensor(tiles, dtype=torch.int32, device=self.dev)
+
+    def state(self) -> N.ModesState:
+        """The C ABI's xb_modes_state for this park (its blocks never move, so it is made once)."""
+        if self._state is None:
+            s = N.ModesState()
+            s.f64, s.i64, s.u8 = self._f.data_ptr(), self._q.data_ptr(), self._b.data_ptr()
+            s.scratch_f64, s.scratch_u8 = self._sf.data_ptr(), self._sb.data_ptr()
+            s.tiles = self.tiles.data_ptr() if self.tiles is not None else None
+            s.rowmap = None
+            s.n, s.stride = self.n, self.stride
+            s.n_tiles = (len(self.tiles) - 1) if self.tiles is not None else 0
+            s.new_orbit = int(self.use_new_orbit)
+            s.friction, s.dt, s.ffriction, s.fdt = self.friction, self.dt, self.ffriction, self.fdt
+            s.dynamical, s.reserved = 0, 0
+            self._state = s
+        self._state.new_orbit = int(self.use_new_orbit)
+        return self._state
+
+    def row_tensors(self) -> list:
+        """The per-row state (the reference's `row_tensors`, vec_orient.py:814-818): every field of the blocks."""
+        return list(FIELD_NAMES)
+
+    # ---- loading: the Python AddBall of new balls (vec_modes.py:221-282) ----
+
+    def _decay(self, m: torch.Tensor) -> torch.Tensor:
+        """SetBallTimeFactor: 0 if mass x agility <= 0, else glibc's exp(-k dt / m) (NaN for a NaN m)."""
+        x = torch.full_like(m, -self.friction * self.dt) / m
+        return torch.where(m <= 0.0, 0.0, N.maths("exp", x))
+
+    def load(self, rows: list) -> None:
+        """Every ball as the Python AddBall adds a new one (vec_modes.py:221-282): keys id, world, time (the world's
+        clock), mass, radius, max_vel, pos, vel, agility, speed_fraction, free, massive, interactive, cloaked, bubble,
+        moribund, mode."""
+        n = len(rows)
+        if n != self.n:
+            raise ValueError(f"{n} rows for a park of {self.n}")
+        t = lambda xs, dt=F64: torch.tensor(xs, dtype=dt)
+        g = lambda key, d: [r.get(key, d) for r in rows]
+        self.load_arrays(
+            ids=t([int(r["id"]) for r in rows], I64), world=t([int(x) for x in g("world", 0)], I64),
+            time=t([int(x) for x in g("time", 0)], I64), mass=t([float(r["mass"]) for r in rows]),
+            radius=t([float(r["radius"]) for r in rows]), max_vel=t([float(r["max_vel"]) for r in rows]),
+            pos=t([[float(c) for c in r["pos"]] for r in rows]).reshape(n, 3),
+            vel=t([[float(c) for c in r["vel"]] for r in rows]).reshape(n, 3),
+            agility=t([float(r["agility"]) for r in rows]),
+            speed_fraction=t([float(r["speed_fraction"]) for r in rows]),
+            free=t([bool(x) for x in g("free", True)], torch.bool),
+            massive=t([bool(x) for x in g("massive", False)], torch.bool),
+            interactive=t([bool(x) for x in g("interactive", False)], torch.bool),
+            cloaked=t([int(x) for x in g("cloaked", 0)], I64), bubble=t([int(x) for x in g("bubble", -1)], I64),
+            moribund=t([bool(x) for x in g("moribund", False)], torch.bool),
+            mode=t([int(x) for x in g("mode", STOP)], I64))
+
+    def load_arrays(self, *, ids, world, time, mass, radius, max_vel, pos, vel, agility, speed_fraction, free,
+                    massive, interactive, cloaked, bubble, moribund, mode) -> None:
+        """`load` from [n] and [n, 3] tensors: AddBall's clamps (the floats rounded to float, then `x < 0 ? 0 : x`,
+        agility `<= 0 ? 1`, NaN and -0 kept), the decay factor, Ball::Ball's defaults (follow range 10, warp length
+        -1, no follow fields, the sensor off, the turning values 1 rad/s and 0.9)."""
+        dev = self.dev
+        c = lambda x, dt=F64: torch.as_tensor(x).to(device=dev, dtype=dt)
+        if int(((c(mode, I64) != STOP) & ~torch.isin(c(mode, I64), torch.tensor(INERT_MODES, device=dev))).sum()):
+            raise ValueError("a ball starts in STOP or an inert mode")
+        m = c(mass)
+        self.mass = torch.where(m < 0.0, 0.0, m)
+        x = f32(c(radius))
+        self.radius = torch.where(x < 0.0, 0.0, x)
+        x = f32(c(max_vel))
+        self.max_vel = torch.where(x < 0.0, 0.0, x)
+        x = f32(c(agility))
+        self.agility = torch.where(x <= 0.0, 1.0, x)
+        x = f32(c(speed_fraction))
+        self.speed_fraction = torch.where(x < 0.0, 0.0, x)
+        p, v = c(pos), c(vel)
+        self.pos, self.vel = p, v
+        self.old_pos = p - v * self.dt
+        self.old_vel = v
+        self.goto, self.last_g = 0.0, 0.0
+        self.time_factor = self._decay(self.mass * self.agility)
+        self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=dev).expand(self.n, 4)
+        self.old_rot = self.rot.clone()
+        self.ang_vel, self.old_ang_vel, self.torque, self.rot_roll = 0.0, 0.0, 0.0, 0.0
+        self.max_ang_vel = 1.0
+        self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
+        self.ids, self.world, self.clock, self.mode = c(ids, I64), c(world, I64), c(time, I64), c(mode, I64)
+        self.free, self.massive = c(free, torch.bool), c(massive, torch.bool)
+        self.interactive, self.moribund = c(interactive, torch.bool), c(moribund, torch.bool)
+        self.cloaked, self.bubble = c(cloaked, I64), c(bubble, I64)
+        self.sensor = False
+        self.effect_stamp, self.follow_id, self.owner_id = 0, 0, 0
+        self.follow_ptr = -1
+        self.follow_range = 10.0
+        self.last_collision = -1.0
+        self._set_tiles()
+
+    # ---- the follower bookkeeping: Stop(Ball*) (Ballpark.cpp:4578; vec_modes.py:286-311) ----
+
+    def _following(self) -> torch.Tensor:
+        return (self.mode == FOLLOW) | (self.mode == ORBIT)
+
+    def _stop_rows(self, mask: torch.Tensor) -> None:
+        follower = mask & self._following()
+        self.effect_stamp = torch.where(follower, 0, self.effect_stamp)
+        self.follow_ptr = torch.where(follower, -1, self.follow_ptr)
+        self.follow_id = torch.where(follower, 0, self.follow_id)
+        self.owner_id = torch.where(follower, 0, self.owner_id)
+        self.follow_range = torch.where(follower, 0.0, self.follow_range)
+        self.mode = torch.where(mask, STOP, self.mode)
+
+    def _members(self, rows: torch.Tensor) -> torch.Tensor:
+        """The balls in the follower set of a row in `rows` as StopAllFollowers acts on it: follow_ptr names the row
+        and the follow id is its id."""
+        fp = self.follow_ptr
+        safe = torch.where(fp >= 0, fp, 0)
+        return (fp >= 0) & (self.follow_id == self.ids[safe]) & rows[safe]
+
+    def followers(self, row: int) -> list:
+        sel = (self.follow_ptr == row) & (self.follow_id == self.ids[row])
+        return sorted(self.ids[sel].tolist())
+
+    # ---- commands (vec_modes.py:315-417) ----
+
+    def stop(self, mask: torch.Tensor) -> None:
+        self._stop_rows(mask & (self.mode != STOP))
+
+    def goto_point(self, mask: torch.Tensor, point: torch.Tensor) -> None:
+        point = point.to(device=self.dev, dtype=F64)
+        ok = mask & torch.isfinite(point).all(dim=1)
+        self._stop_rows(ok)
+        self.goto = torch.where(ok[:, None], point, self.goto)
+        self.speed_fraction = torch.where(ok & (self.speed_fraction == 0.0), 1.0, self.speed_fraction)
+        self.mode = torch.where(ok, GOTO, self.mode)
+
+    def goto_direction(self, mask: torch.Tensor, direction: torch.Tensor) -> None:
+        d = direction.to(device=self.dev, dtype=F64)
+        ok = mask & torch.isfinite(d).all(dim=1)
+        self.goto_point(ok, self.pos + _unit(d) * DIR_DIST)
+
+    def _follow(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor, mode: int) -> None:
+        rng = f32(range_.to(device=self.dev, dtype=F64))
+        target = target.to(device=self.dev, dtype=I64)
+        inside = (target >= 0) & (target < self.n)
+        safe = torch.where(inside, target, 0)
+        rows = torch.arange(self.n, device=self.dev)
+        ok = (mask & torch.isfinite(rng) & inside & (safe != rows) & (self.world[safe] == self.world)
+              & ~self.moribund[safe] & (self.bubble[safe] == self.bubble) & (self.cloaked[safe] == 0))
+        self._stop_rows(ok)
+        self.follow_id = torch.where(ok, self.ids[safe], self.follow_id)
+        self.follow_ptr = torch.where(ok, safe, self.follow_ptr)
+        self.follow_range = torch.where(ok, rng, self.follow_range)
+        self.mode = torch.where(ok, mode, self.mode)
+
+    def follow(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor) -> None:
+        self._follow(mask, target, range_, FOLLOW)
+
+    def orbit(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor) -> None:
+        self._follow(mask, target, range_, ORBIT)
+
+    def warp_to(self, mask: torch.Tensor, point: torch.Tensor, min_range: torch.Tensor,
+                warp_factor: torch.Tensor) -> None:
+        point = point.to(device=self.dev, dtype=F64)
+        mr = min_range.to(device=self.dev, dtype=F64)
+        wf = warp_factor.to(device=self.dev, dtype=I64)
+        ok = mask & torch.isfinite(point).all(dim=1)
+        mr = torch.where(mr < 0.0, 0.0, mr)
+        wf = torch.where(wf <= 0, 1, wf)
+        d = self.pos - point
+        short = ok & (_dot(d, d) < MIN_WARP_DISTANCE_SQ)
+        self.goto_point(short, point)
+        ok = ok & ~short & ~((self.mode == WARP) & (self.goto == point).all(dim=1))
+        self._stop_rows(ok)
+        self.goto = torch.where(ok[:, None], point, self.goto)
+        self.effect_stamp = torch.where(ok, -1, self.effect_stamp)
+        self.follow_range = torch.where(ok, 0.0, self.follow_range)
+        self.follow_id = torch.where(ok, mr.contiguous().view(I64), self.follow_id)
+        self.owner_id = torch.where(ok, wf, self.owner_id)
+        self.mode = torch.where(ok, WARP, self.mode)
+
+    def _release_followers(self, warping: torch.Tensor) -> None:
+        f = self._members(warping)
+        orbiter = f & self.interactive & (self.mode == ORBIT)
+        self._stop_rows(f & ~orbiter)
+        self.goto_direction(orbiter, self.vel.clone())
+
+    def entity_warp_in(self, mask: torch.Tensor, point: torch.Tensor, warp_factor: torch.Tensor) -> None:
+        point = point.to(device=self.dev, dtype=F64)
+        self.warp_to(mask, point, torch.zeros_like(point[:, 0]), warp_factor)
+        go = mask & (self.mode != GOTO)
+        self._release_followers(go)
+        delta = _unit(self.pos - self.goto)
+        mr = self.follow_id.contiguous().view(F64)
+        dst = self.goto + delta * mr[:, None]
+        self.goto = torch.where(go[:, None], dst, self.goto)
+        self.massive = torch.where(go, False, self.massive)
+        self.sensor = torch.where(go, False, self.sensor)
+        self.vel = torch.where(go[:, None], _unit(point) * AU, self.vel)
+        self.effect_stamp = torch.where(go, torch.clamp(self.clock - 5, min=0), self.effect_stamp)
+        back = self.pos - self.goto
+        self.last_collision = torch.where(go, _sqrt(_dot(back, back)), self.last_collision)
+
+    # ---- setters (vec_orient.py:403-466; SetBallVelocity without dynamical orientation, vec_modes.py:421-428) ----
+
+    def set_mass(self, mask: torch.Tensor, mass: torch.Tensor) -> None:
+        m = mass.to(device=self.dev, dtype=F64)
+        ok = mask & ~(m <= 0.0)
+        self.mass = torch.where(ok, m, self.mass)
+        self.time_factor = torch.where(ok, self._decay(self.mass * self.agility), self.time_factor)
+
+    def set_agility(self, mask: torch.Tensor, agility: torch.Tensor) -> None:
+        a = f32(agility.to(device=self.dev, dtype=F64))
+        ok = mask & ~(a <= 0.0)
+        self.agility = torch.where(ok, a, self.agility)
+        self.time_factor = torch.where(ok, self._decay(self.mass * self.agility), self.time_factor)
+
+    def set_max_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
+        v = f32(v.to(device=self.dev, dtype=F64))
+        ok = mask & ~(v < 0.0)
+        self.max_vel = torch.where(ok, v, self.max_vel)
+        self.refiled += ok.to(I64)
+
+    def set_speed_fraction(self, mask: torch.Tensor, s: torch.Tensor) -> None:
+        s = f32(s.to(device=self.dev, dtype=F64))
+        ok = mask & torch.isfinite(s)
+        s = torch.where(s < 0.0, 0.0, torch.where(s > 1.0, 1.0, s))
+        self.speed_fraction = torch.where(ok, s, self.speed_fraction)
+
+    def set_radius(self, mask: torch.Tensor, r: torch.Tensor) -> None:
+        r = f32(r.to(device=self.dev, dtype=F64))
+        ok = mask & ~(r < 0.0)
+        self.radius = torch.where(ok, r, self.radius)
+        self.refiled += ok.to(I64)
+
+    def set_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
+        v = v.to(device=self.dev, dtype=F64)
+        self.vel = torch.where(mask[:, None], v, self.vel)
+        self.old_vel = torch.where(mask[:, None], v, self.old_vel)
+
+    def set_free(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
+        """SetBallFree (vec_modes.py:438-451)."""
+        flag = flag.to(self.dev) != 0
+        change = mask & (flag != self.free)
+        fix = change & ~flag
+        self.free = torch.where(change, flag, self.free)
+        self._stop_rows(fix)
+        self.set_velocity(fix, torch.zeros_like(self.vel))
+        self.last_g = torch.where(fix[:, None], 0.0, self.last_g)
+
+    def set_ball_rigid(self, mask: torch.Tensor) -> None:
+        self._stop_rows(mask)
+        self.mode = torch.where(mask, RIGID, self.mode)
+
+    def set_boid(self, mask: torch.Tensor, on: torch.Tensor) -> None:
+        on = on.to(self.dev) != 0
+        self.mode = torch.where(mask & on, BOID, self.mode)
+        self._stop_rows(mask & ~on)
+
+    def set_interactive(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
+        self.interactive = torch.where(mask, flag.to(self.dev) != 0, self.interactive)
+
+    def set_ball_troll(self, mask: torch.Tensor, delay: torch.Tensor) -> None:
+        d = delay.to(device=self.dev, dtype=I64)
+        d = torch.where(d < TROLL_MIN_DELAY, TROLL_MIN_DELAY, d)
+        self._stop_rows(mask)
+        self.set_free(mask, torch.ones_like(mask))
+        self.set_interactive(mask, torch.ones_like(mask))
+        self.effect_stamp = torch.where(mask, self.clock + d, self.effect_stamp)
+        self.mode = torch.where(mask, TROLL, self.mode)
+
+    # ---- one tick (Ballpark::Evolve; vec_modes.py:943-960) ----
+
+    def _tick(self, block: int | None = None) -> None:
+        s = self.state()
+        if self.n == 0:
+            return
+        b = self.block if block is None else int(block)
+        if self.dev.type == "cuda":
+            with torch.cuda.device(self.dev):
+                rc = N.hip().xb_modes_tick(ctypes.byref(s), torch.cuda.current_stream(self.dev).cuda_stream, b)
+            N.check(rc, "xb_modes_tick")
+        else:
+            rc = N.cpu().xb_cpu_modes_tick(ctypes.byref(s), 0)
+            if rc:
+                raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
+
+    def _save(self) -> tuple:
+        return self._f.clone(), self._q.clone(), self._b.clone()
+
+    def _put_back(self, saved: tuple, active: torch.Tensor) -> None:
+        act = active.to(device=self.dev, dtype=torch.bool).reshape(self.n)
+        sel = torch.zeros(self.stride, dtype=torch.bool, device=self.dev)
+        sel[:self.n] = act
+        for blk, old in zip((self._f, self._q, self._b), saved):
+            blk.copy_(torch.where(sel[None, :], blk, old))
+
+    def evolve(self, lib=None, active: torch.Tensor | None = None) -> None:
+        """One tick of every world, in place, no host synchronisation. `active` [n]: OnTick's per-world mask (a world's
+        rows together): only those rows tick; every other row keeps every field bit for bit (the reference's
+        masked_tick, vec_orient.py:217-254: the tick reads only rows of each row's own world)."""
+        if lib is not None:
+            raise ValueError("lib: this park binds its own library")
+        if active is not None:
+            saved = self._save()
+            self._tick()
+            self._put_back(saved, active)
+        else:
+            self._tick()
+        self.current_time += 1
+
+    def evolve_torch(self, active: torch.Tensor | None = None) -> None:
+        """The reference's CPU tick name: here the same kernel source built for the host (a CPU park's evolve)."""
+        if self.dev.type == "cuda":
+            raise ValueError("evolve_torch is the CPU park's tick; a GPU park ticks with evolve()")
+        self.evolve(active=active)
+
+    # ---- OnTick (vec_orient.py:834-879) ----
+
+    def _world_split(self, x: torch.Tensor) -> torch.Tensor:
+        order = torch.argsort(self.world, stable=True)
+        w, v = self.world[order], x[order]
+        return ((w[1:] == w[:-1]) & (v[1:] != v[:-1])).any()
+
+    def on_tick(self, sim_time, tick=None, after=None) -> list:
+        """Ballpark::OnTick for every world at its own simulation time (100 ns units, a world's rows sharing it): less
+        than one whole interval since the world's `time`: nothing; a world's first call evolves once; every other
+        world runs its whole intervals under OnTick's mask, the remainder carried. Returns the masks of the ticks run."""
+        n, d = self.n, self.dev
+        if getattr(self, "time", None) is None:
+            self.time = torch.zeros(n, dtype=I64, device=d)
+            self.first_time = torch.ones(n, dtype=torch.bool, device=d)
+        sim = torch.as_tensor(sim_time, dtype=I64).to(d).reshape(n)
+        interval = int(round(self.dt * 1000.0)) * 10000
+        last = sim - self.time
+        go = ~(last < interval)
+        k = torch.where(go, torch.div(last, interval, rounding_mode="floor"), 0)
+        first = go & self.first_time
+        rest = go & ~first
+        top = torch.maximum(torch.where(rest, k, 0).max(), first.any().to(I64)) if n else torch.zeros((), dtype=I64)
+        top, split = torch.stack((top.to(I64), self._world_split(sim).to(I64))).tolist() if n else (0, 0)
+        if split:
+            raise ValueError("on_tick: a world's rows hold different simulation times")
+        tick = tick or (lambda rows: self.evolve(active=rows))
+        masks = []
+        for j in range(top):
+            m = (rest & (k > j)) | (first & (j == 0))
+            tick(None if bool(m.all()) else m)
+            if after is not None:
+                after(m)
+            masks.append(m)
+        self.first_time = self.first_time & ~first
+        self.time = torch.where(go, sim - torch.where(rest, last - k * interval, 0), self.time)
+        return masks
+
+
+class ModesGraph:
+    """A VecModesPark's tick captured once as a HIP graph on the park's own state blocks, so each replay is one tick
+    (the reference's ModesGraph, src/exact_ballpark/vec_modes_cuda.py:371-432). Commands and setters write into the
+    blocks, so nothing needs re-binding between replays. With `masked=True` the capture holds OnTick's per-world mask
+    as a device buffer: the blocks saved, the tick, the rows outside the mask put back; `tick(active)` writes the mask
+    and replays."""
+
+    def __init__(self, park: VecModesPark, block: int | None = None, lib=None, masked: bool = False):
+        if park.dev.type != "cuda":
+            raise ValueError("a HIP graph needs a park on a GPU")
+        if lib is not None:
+            raise ValueError("lib: the park binds its own library")
+        self.park, self.block = park, park.block if block is None else int(block)
+        self.masked = masked
+        self.mask = torch.ones(park.stride, dtype=torch.bool, device=park.dev)
+        self.saved = [torch.empty_like(b) for b in (park._f, park._q, park._b)] if masked else None
+        before = park._save()                            # warm-up: the module loaded before the capture,
+        park._tick(self.block)                           # the park's state put back after it
+        for b, old in zip((park._f, park._q, park._b), before):
+            b.copy_(old)
+        torch.cuda.synchronize(park.dev)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            if masked:
+                for s, b in zip(self.saved, (park._f, park._q, park._b)):
+                    s.copy_(b)
+            park._tick(self.block)
+            if masked:
+                for s, b in zip(self.saved, (park._f, park._q, park._b)):
+                    b.copy_(torch.where(self.mask[None, :], b, s))
+
+    def tick(self, active: torch.Tensor | None = None) -> None:
+        if active is not None and not self.masked:
+            raise ValueError("a graph made without OnTick's mask (masked=True) ticks every row")
+        if self.masked:
+            if active is None:
+                self.mask.fill_(True)
+            else:
+                self.mask[:self.park.n].copy_(active.to(device=self.park.dev, dtype=torch.bool).reshape(-1))
+        self.graph.replay()
+        self.park.current_time += 1

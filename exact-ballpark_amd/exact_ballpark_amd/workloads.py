@@ -1,0 +1,123 @@
+"""Seeded synthetic workloads, generated with numpy in one pass so a park of 65,536 worlds is built in about a second.
+
+`flight_park(seed, worlds, ships)`: the flight-modes park. Each world has `ships` ships with ids 1..ships; per ship,
+position uniform in a +/-50 km cube, at rest, mass U(1e6, 1e8) kg, agility U(0.1, 5), maximum speed U(100, 3000) m/s,
+radius U(20, 500) m, speed fraction 1, massive, interactive with probability 1/2. At tick 1 each ship gets a mode
+uniform over {stop, goto, follow, orbit, warp}: GotoPoint to a point uniform in the ball of radius 1e10 m (1e7 km)
+around the origin, FollowBall or Orbit of another ship of its world chosen uniformly at a range U(500, 20000) m, or
+WarpTo a point as GotoPoint's with the thunker's defaults (minimum range 20 km, warp factor 20).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP = 1, 2, 3, 4, 5, 6
+
+
+def _ball_points(rng: np.random.Generator, n: int, radius: float) -> np.ndarray:
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    return u * (radius * rng.uniform(0.0, 1.0, n) ** (1.0 / 3.0))[:, None]
+
+
+def flight_park(seed: int, worlds: int, ships: int = 4) -> dict:
+    """Arrays for worlds x ships rows (world-major) and each row's tick-1 command."""
+    rng = np.random.default_rng(seed)
+    n = worlds * ships
+    world = np.repeat(np.arange(worlds, dtype=np.int64), ships)
+    ids = np.tile(np.arange(1, ships + 1, dtype=np.int64), worlds)
+    rows = {
+        "id": ids, "world": world, "time": np.zeros(n, dtype=np.int64),
+        "mass": rng.uniform(1e6, 1e8, n), "radius": rng.uniform(20.0, 500.0, n),
+        "max_vel": rng.uniform(100.0, 3000.0, n), "agility": rng.uniform(0.1, 5.0, n),
+        "speed_fraction": np.ones(n), "pos": rng.uniform(-5e4, 5e4, (n, 3)), "vel": np.zeros((n, 3)),
+        "free": np.ones(n, dtype=bool), "massive": np.ones(n, dtype=bool),
+        "interactive": rng.uniform(0.0, 1.0, n) < 0.5, "moribund": np.zeros(n, dtype=bool),
+        "cloaked": np.zeros(n, dtype=np.int64), "bubble": np.zeros(n, dtype=np.int64),
+        "mode": np.full(n, 2, dtype=np.int64),
+    }
+    kind = rng.integers(0, 5, n)                       # 0 stop, 1 goto, 2 follow, 3 orbit, 4 warp
+    op = np.select([kind == 1, kind == 2, kind == 3, kind == 4], [GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP], 0)
+    point = _ball_points(rng, n, 1e10)
+    other = rng.integers(0, ships - 1, n) if ships > 1 else np.zeros(n, dtype=np.int64)
+    own = ids - 1
+    other = np.where(other >= own, other + 1, other)   # another ship of the world, uniform
+    target = world * ships + other
+    rng_ = rng.uniform(500.0, 20000.0, n)
+    if ships == 1:
+        op = np.where((op == FOLLOW_OP) | (op == ORBIT_OP), 0, op)
+    cmds = {"op": op.astype(np.int64), "point": point, "target": target.astype(np.int64), "range": rng_,
+            "min_range": np.full(n, 20000.0), "warp_factor": np.full(n, 20, dtype=np.int64)}
+    return {"n": n, "worlds": worlds, "ships": ships, "rows": rows, "cmds": cmds}
+
+
+def load_flight_park(park, wl: dict) -> None:
+    """Loads the rows into a VecModesPark of wl["n"] rows and applies the tick-1 commands through its API."""
+    r = wl["rows"]
+    dev = park.dev
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt).to(dev)
+    park.load_arrays(ids=t(r["id"], torch.int64), world=t(r["world"], torch.int64), time=t(r["time"], torch.int64),
+                     mass=t(r["mass"]), radius=t(r["radius"]), max_vel=t(r["max_vel"]), pos=t(r["pos"]),
+                     vel=t(r["vel"]), agility=t(r["agility"]), speed_fraction=t(r["speed_fraction"]),
+                     free=t(r["free"], torch.bool), massive=t(r["massive"], torch.bool),
+                     interactive=t(r["interactive"], torch.bool), cloaked=t(r["cloaked"], torch.int64),
+                     bubble=t(r["bubble"], torch.int64), moribund=t(r["moribund"], torch.bool),
+                     mode=t(r["mode"], torch.int64))
+    apply_commands(park, wl["cmds"])
+
+
+def apply_commands(park, c: dict, mask=None) -> None:
+    dev = park.dev
+    op = torch.as_tensor(c["op"]).to(dev)
+    if mask is not None:
+        op = torch.where(mask, op, 0)
+    pt = torch.as_tensor(c["point"], dtype=torch.float64).to(dev)
+    tg = torch.as_tensor(c["target"], dtype=torch.int64).to(dev)
+    rr = torch.as_tensor(c["range"], dtype=torch.float64).to(dev)
+    park.goto_point(op == GOTO_OP, pt)
+    park.follow(op == FOLLOW_OP, tg, rr)
+    park.orbit(op == ORBIT_OP, tg, rr)
+    park.warp_to(op == WARP_OP, pt, torch.as_tensor(c["min_range"], dtype=torch.float64).to(dev),
+                 torch.as_tensor(c["warp_factor"], dtype=torch.int64).to(dev))
+
+
+def world_scenario(wl: dict, wi: int, steps: int) -> dict:
+    """World wi of the workload as a scenario dict (balls and tick-1 commands), for the oracle."""
+    s = wl["ships"]
+    r, c = wl["rows"], wl["cmds"]
+    balls, cmds = [], []
+    for k in range(wi * s, wi * s + s):
+        balls.append({"id": int(r["id"][k]), "mass": float(r["mass"][k]), "radius": float(r["radius"][k]),
+                      "max_vel": float(r["max_vel"][k]), "agility": float(r["agility"][k]),
+                      "sf": float(r["speed_fraction"][k]), "pos": [float(v) for v in r["pos"][k]],
+                      "vel": [float(v) for v in r["vel"][k]], "free": bool(r["free"][k]),
+                      "massive": bool(r["massive"][k]), "interactive": bool(r["interactive"][k]),
+                      "cloaked": int(r["cloaked"][k]), "bubble": int(r["bubble"][k]),
+                      "moribund": bool(r["moribund"][k]), "mode": int(r["mode"][k])})
+    for o in (GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP):    # apply_commands' order
+        for k in range(wi * s, wi * s + s):
+            if int(c["op"][k]) != o:
+                continue
+            p = [float(v) for v in c["point"][k]]
+            tgt = int(r["id"][int(c["target"][k])])
+            cmds.append((1, o, int(r["id"][k]), tgt, p[0], p[1], p[2],
+                         float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
+                         int(c["warp_factor"][k])))
+    return {"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": steps, "new_orbit": False, "start": 0,
+            "balls": balls, "cmds": cmds}
+
+
+def select_worlds(wl: dict, order) -> dict:
+    """The workload with its worlds in `order` (a subset or a permutation of the world indices), rows world-major in
+    that order and every follow target's row renumbered; each world keeps its world id."""
+    s = wl["ships"]
+    order = np.asarray(order, dtype=np.int64)
+    rows = (order[:, None] * s + np.arange(s)[None, :]).reshape(-1)
+    new_row = np.full(wl["n"], -1, dtype=np.int64)
+    new_row[rows] = np.arange(rows.size)
+    r = {k: v[rows] for k, v in wl["rows"].items()}
+    c = {k: v[rows] for k, v in wl["cmds"].items()}
+    c["target"] = new_row[c["target"]]
+    return {"n": int(rows.size), "worlds": int(order.size), "ships": s, "rows": r, "cmds": c}

@@ -1,0 +1,83 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * The C ABI of libxb_hip.so (the batched destiny tick on gfx950) and libxb_cpu.so (the same tick source built for the
+ * host). Plain pointers and sizes: the caller owns every buffer, nothing here allocates or frees, no call synchronises
+ * the host, and every launch goes on the caller's HIP stream, so a caller may capture the launches in a HIP graph.
+ *
+ * Return codes: 0 success; -1 refused before any launch (a bad block size, size or argument; see xb_error_string);
+ * a positive value is the hipError_t of the launch.
+ *
+ * Each entry point names the reference interface it replaces (the ctypes-bound functions of the reference's
+ * src/exact_ballpark/*_cuda.cu); the reference-side bindings are in INTEGRATION.md.
+ */
+#ifndef XB_H
+#define XB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XB_ABI_VERSION 1
+
+/* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
+   exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
+   reference's `struct Modes` of 38 tensor pointers (src/exact_ballpark/vec_modes_cuda.cu:83-96).
+
+   Two launch shapes:
+   - tiled (`tiles` not NULL): one launch; workgroup g runs the rows tiles[g] .. tiles[g+1] - 1 (at most `block` rows,
+     no world split across tiles: every follow target of a row lies in its tile), its worlds' tick-start records held
+     in LDS. `rowmap`, when not NULL, maps the tiled index space to physical rows (each world's rows consecutive and in
+     order in both), so worlds can be regrouped without moving their state.
+   - general (`tiles` NULL): any row order; two launches through the caller's scratch (XB_NS float64 rows and one
+     uint8 row of `stride` each). */
+typedef struct xb_modes_state {
+    double* f64;
+    int64_t* i64;
+    uint8_t* u8;
+    double* scratch_f64;
+    uint8_t* scratch_u8;
+    const int32_t* tiles;
+    const int32_t* rowmap;
+    int64_t n;
+    int64_t stride;
+    int32_t n_tiles;
+    int32_t new_orbit;
+    double friction;
+    double dt;
+    float ffriction;
+    float fdt;
+    int32_t dynamical;
+    int32_t reserved;
+} xb_modes_state;
+
+/* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */
+int xb_abi(void);
+
+/* A message for a return code. Replaces xbpo_error_string (src/exact_ballpark/vec_orient_cuda.cu:497-503). */
+const char* xb_error_string(int rc);
+
+/* The state layout as the library was built: out[0..4] = XB_NF, XB_NI, XB_NB, XB_NS, sizeof(xb_modes_state).
+   Plays the part of xbps_layout (src/exact_ballpark/vec_space.cu:1853-1868) for the modes state. */
+int xb_layout(int64_t* out);
+
+/* One tick of every row of the park (Ballpark::Evolve per world, in place), on `stream` (a hipStream_t; NULL is the
+   null stream). block: a multiple of 64, at most 256. Replaces xbpm_tick (src/exact_ballpark/vec_modes_cuda.cu:754). */
+int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
+
+/* The tick's double functions alone, out[i] = f(x[i]) (func 0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 sincos's
+   cosine, 6 IEEE sqrt, 7 IEEE division x[i] / x[n + i] with out of n elements). Replaces xbpm_dlibm
+   (src/exact_ballpark/vec_modes_cuda.cu:779). */
+int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block);
+
+/* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass; out[0..3] =
+   scalar registers, vector registers, private (scratch) bytes per lane, LDS bytes. Replaces xbpm_info
+   (src/exact_ballpark/vec_modes_cuda.cu:766). */
+int xb_modes_info(int which, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -52,6 +52,7 @@ def lib() -> ctypes.CDLL:
         L.or_get_ball.argtypes = [vp, i, pd, pq]
         L.or_get_followers.argtypes = [vp, i, pq]
         L.or_maths.argtypes = [i, i64, pd, pd]
+        L.or_evolve_all.argtypes = [ctypes.POINTER(vp), i, i]
         _LIB = L
     return _LIB
 
@@ -140,3 +141,22 @@ def maths(func: str, x: np.ndarray) -> np.ndarray:
     if rc:
         raise ValueError(func)
     return out
+
+
+def time_worlds(worlds: list, ticks: int) -> tuple:
+    """Builds every world (balls, tick-1 commands applied), then times `ticks` ticks of all of them in one C loop on
+    one core; returns (seconds, the worlds' oracle objects for checking)."""
+    import time
+    objs = []
+    for w in worlds:
+        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"])
+        for b in w["balls"]:
+            o.add(b)
+        for c in w["cmds"]:
+            if c[0] == 1:
+                o.command(c)
+        objs.append(o)
+    arr = (ctypes.c_void_p * len(objs))(*[o.p for o in objs])
+    t0 = time.perf_counter()
+    lib().or_evolve_all(arr, len(objs), int(ticks))
+    return time.perf_counter() - t0, objs

@@ -683,3 +683,9 @@ int or_maths(int func, int64_t n, const double* x, double* out) {
     }
     return 0;
 }
+
+/* Every park in `parks` ticked `ticks` times, each world to its end before the next (the CPU baseline's timed loop). */
+void or_evolve_all(OPark** parks, int np, int ticks) {
+    for (int i = 0; i < np; ++i)
+        for (int t = 0; t < ticks; ++t) or_evolve(parks[i]);
+}

@@ -1,0 +1,179 @@
+"""Runs tests/scenarios.py worlds through the drop-in park API (exact_ballpark_amd.vec_modes.VecModesPark) and through
+the oracle, and compares them field by field, bit for bit (two NaNs equal, as the reference compares,
+docs/exactness.md:33-41 in the reference).
+
+Worlds of one (friction, tick, orbit law) share one park, each world's balls consecutive rows in the order listed (so
+ids are not in row order); before each tick the j-th command of every world is applied together, op by op, as the
+reference's batch adapter does (tests/modes_batch.py:100-140 in the reference).
+"""
+
+from __future__ import annotations
+
+from collections import defaultdict
+
+import numpy as np
+import torch
+
+import oracle as O
+import scenarios as S
+from exact_ballpark_amd.vec_modes import FOLLOW, ORBIT, ModesGraph, VecModesPark
+
+F64, I64 = torch.float64, torch.int64
+# the oracle's state columns the park is compared on
+D_COLS = list(range(26))                  # pos .. last_collision (the pad left out)
+Q_COLS = list(range(11))                  # id .. moribund
+
+
+def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
+    n, dev = park.n, park.dev
+    mask = torch.zeros(n, dtype=torch.bool)
+    target = torch.full((n,), -1, dtype=I64)
+    xyz = torch.zeros(n, 3, dtype=F64)
+    rr = torch.zeros(n, dtype=F64)
+    w = torch.zeros(n, dtype=I64)
+    for wi, (_, _, a, b, x, y, z, r, ww) in items:
+        row = index.get((wi, a))
+        if row is None:
+            continue
+        mask[row] = True
+        target[row] = index.get((wi, b), -1)
+        xyz[row] = torch.tensor((x, y, z), dtype=F64)
+        rr[row], w[row] = r, ww
+    mask, target, xyz, rr, w = (t.to(dev) for t in (mask, target, xyz, rr, w))
+    if op == S.GOTO_OP:
+        park.goto_point(mask, xyz)
+    elif op == S.DIR_OP:
+        park.goto_direction(mask, xyz)
+    elif op == S.STOP_OP:
+        park.stop(mask)
+    elif op == S.FOLLOW_OP:
+        park.follow(mask, target, rr)
+    elif op == S.ORBIT_OP:
+        park.orbit(mask, target, rr)
+    elif op == S.WARP_OP:
+        park.warp_to(mask, xyz, rr, w)
+    elif op == S.WARPIN_OP:
+        park.entity_warp_in(mask, xyz, w)
+    else:
+        raise ValueError(op)
+
+
+def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
+    """{world index: (d, q, followers)} with balls by ascending id, in the oracle's column order."""
+    n = park.n
+    f = park._f[:, :n].cpu().numpy()
+    q = park._q[:, :n].cpu().numpy()
+    b = park._b[:, :n].cpu().numpy()
+    ids = q[0]
+    mode, fptr = q[1], q[5]
+    fol = defaultdict(list)
+    for row in range(n):
+        if mode[row] in (FOLLOW, ORBIT) and fptr[row] >= 0:
+            fol[int(fptr[row])].append(int(ids[row]))
+    out = {}
+    for wi in members:
+        rows = sorted(rows_of[wi], key=lambda r: ids[r])
+        d = np.stack([np.concatenate((f[0:3, r], f[3:6, r], f[12:15, r], f[15:18, r], f[6:9, r], f[9:12, r],
+                                      [f[18, r], f[21, r], f[22, r], f[23, r], f[24, r], f[25, r], f[19, r],
+                                       f[20, r]])) for r in rows])
+        qq = np.stack([[ids[r], mode[r], q[2, r], q[3, r], q[4, r], ids[fptr[r]] if fptr[r] >= 0 else -1, b[2, r],
+                        b[4, r], b[3, r], b[0, r], b[1, r]] for r in rows]).astype(np.int64)
+        out[wi] = (d, qq, [sorted(fol.get(r, [])) for r in rows])
+    return out
+
+
+def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bool = False,
+             general: bool = False, order=None) -> dict:
+    """{(world index, step): (d, q, followers)} after each recorded tick (`record`: a set of steps, None for all).
+    `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending)."""
+    groups = defaultdict(list)
+    for wi in (order if order is not None else range(len(worlds))):
+        w = worlds[wi]
+        groups[(w["friction"], w["tick"], bool(w["new_orbit"]))].append(wi)
+    out = {}
+    for (friction, tick, new_orbit), members in groups.items():
+        balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
+        park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
+                            use_new_orbit=new_orbit, block=block)
+        park.load([{"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
+                    "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
+                    "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
+                    "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
+                    "moribund": b["moribund"], "mode": b["mode"]} for wi, b in balls])
+        if general:
+            park.tiles = None
+            park._state = None
+        index = {(wi, b["id"]): i for i, (wi, b) in enumerate(balls)}
+        rows_of = defaultdict(list)
+        for i, (wi, _) in enumerate(balls):
+            rows_of[wi].append(i)
+        per_step = defaultdict(lambda: defaultdict(list))
+        for wi in members:
+            for c in worlds[wi]["cmds"]:
+                per_step[c[0]][wi].append(c)
+        last = max(worlds[wi]["steps"] for wi in members)
+        g = ModesGraph(park) if graph else None
+        for s in range(1, last + 1):
+            todo = per_step.get(s, {})
+            for j in range(max((len(v) for v in todo.values()), default=0)):
+                by_op = defaultdict(list)
+                for wi, cmds in todo.items():
+                    if j < len(cmds):
+                        by_op[cmds[j][1]].append((wi, cmds[j]))
+                for op, items in by_op.items():
+                    _apply(park, op, items, index)
+            if g is not None:
+                g.tick()
+            else:
+                park.evolve()
+            live = [wi for wi in members if s <= worlds[wi]["steps"] and (record is None or s in record)]
+            if live:
+                for wi, v in _snapshot(park, live, rows_of).items():
+                    out[(wi, s)] = v
+    return out
+
+
+def run_oracle(worlds: list, indices, record=None) -> dict:
+    out = {}
+    for wi in indices:
+        for step, d, q, fol in O.run_world(worlds[wi]):
+            if record is None or step in record:
+                out[(wi, step)] = (d[:, D_COLS], q[:, Q_COLS], fol)
+    return out
+
+
+def _same_bits(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a, b = a.copy(), b.copy()
+    both = np.isnan(a) & np.isnan(b)
+    a[both] = 0.0
+    b[both] = 0.0
+    return a.view(np.uint64) == b.view(np.uint64)
+
+
+def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
+    """The differences (at most `limit` described), as strings; every key of `want` must be in `got`."""
+    bad = []
+    for key in sorted(want):
+        if key not in got:
+            bad.append(f"{key}: missing")
+            continue
+        gd, gq, gf = got[key]
+        wd, wq, wf = want[key]
+        if gd.shape != wd.shape:
+            bad.append(f"{key}: {gd.shape} balls vs {wd.shape}")
+            continue
+        same_d = _same_bits(gd, wd)
+        same_q = gq == wq
+        if same_d.all() and same_q.all() and gf == wf:
+            continue
+        if len(bad) < limit:
+            names_d = [O.D_NAMES[c] for c in D_COLS]
+            names_q = [O.Q_NAMES[c] for c in Q_COLS]
+            diffs = [f"{names_d[c]}[{k}] {gd[k, c]!r} vs {wd[k, c]!r}" for k, c in zip(*np.nonzero(~same_d))][:6]
+            diffs += [f"{names_q[c]}[{k}] {gq[k, c]} vs {wq[k, c]}" for k, c in zip(*np.nonzero(~same_q))][:6]
+            if gf != wf:
+                diffs.append(f"followers {gf} vs {wf}")
+            bad.append(f"world {key[0]} ({worlds[key[0]]['kind']}) step {key[1]}: " + "; ".join(diffs))
+        else:
+            bad.append(f"{key}")
+    return bad

@@ -211,34 +211,9 @@ def cases(seed: int, scale: int = 1) -> list:
     return out
 
 
-def flight_park(seed: int, worlds: int, ships: int = 4, start: int = 0) -> list:
-    """The flight-modes park: `worlds` worlds of `ships` ships, modes uniform over stop, goto, follow, orbit and warp
-    given at tick 1 (each world's own clock starts at `start`)."""
-    r = random.Random(seed)
-    out = []
-    for _ in range(worlds):
-        ids = list(range(1, ships + 1))
-        balls = []
-        for bid in ids:
-            balls.append({"id": bid, "mass": r.uniform(1e6, 1e8), "radius": r.uniform(20.0, 500.0),
-                          "max_vel": r.uniform(100.0, 3000.0), "agility": r.uniform(0.1, 5.0), "sf": 1.0,
-                          "pos": [r.uniform(-5e4, 5e4) for _ in range(3)], "vel": [0.0, 0.0, 0.0], "free": True,
-                          "massive": True, "interactive": r.random() < 0.5, "cloaked": 0, "bubble": 0,
-                          "moribund": False, "mode": STOP})
-        cmds = []
-        for bid in ids:
-            mode = r.randrange(5)
-            others = [o for o in ids if o != bid]
-            if mode == 1:
-                u, d = _unit(r), r.uniform(0.0, 1e10)
-                cmds.append(_cmd(1, GOTO_OP, bid, x=u[0] * d, y=u[1] * d, z=u[2] * d))
-            elif mode == 2 and others:
-                cmds.append(_cmd(1, FOLLOW_OP, bid, r.choice(others), rr=r.uniform(500.0, 20000.0)))
-            elif mode == 3 and others:
-                cmds.append(_cmd(1, ORBIT_OP, bid, r.choice(others), rr=r.uniform(500.0, 20000.0)))
-            elif mode == 4:
-                u, d = _unit(r), r.uniform(0.0, 1e10)
-                cmds.append(_cmd(1, WARP_OP, bid, x=u[0] * d, y=u[1] * d, z=u[2] * d, rr=20000.0, w=20))
-        out.append({"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": 0, "new_orbit": False,
-                    "start": start, "balls": balls, "cmds": cmds})
-    return out
+def flight_park(seed: int, worlds: int, ships: int = 4) -> list:
+    """The flight-modes park (exact_ballpark_amd.workloads.flight_park, the benchmark's generator) as scenario worlds:
+    `worlds` worlds of `ships` ships, modes uniform over stop, goto, follow, orbit and warp given at tick 1."""
+    from exact_ballpark_amd import workloads as WL
+    wl = WL.flight_park(seed, worlds, ships)
+    return [WL.world_scenario(wl, wi, 0) for wi in range(worlds)]

@@ -1,0 +1,211 @@
+"""The gfx950 flight-modes kernel against the oracle, bit for bit (two NaNs equal), through the drop-in park API:
+
+- the golden worlds (every kind of scenario), every ball after every tick, in both launch shapes;
+- the flight-modes park workload (1,024 worlds x 4 ships, 100 ticks, modes uniform over stop, goto, follow, orbit and
+  warp), at recorded ticks, and against the host build of the same source;
+- the tick's maths on the device against the host (sin, cos, log, exp, sincos; sqrt and division IEEE);
+- properties at full size (65,536 worlds x 4 ships): tiled and general launches identical, a run repeated identical, a
+  world alone identical to the same world in the batch and in reversed order, block sizes 64, 128 and 256 identical;
+- the HIP graph replay identical to eager ticks, with eager commands between replays; OnTick's masked tick.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import golden_fixture as G
+import park_driver as D
+import scenarios as S
+from exact_ballpark_amd import _native as N
+from exact_ballpark_amd import workloads as WL
+from exact_ballpark_amd.vec_modes import ModesGraph, VecModesPark
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _bits(a):
+    return np.asarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return G.load()
+
+
+@pytest.fixture(scope="module")
+def golden_oracle(golden):
+    return D.run_oracle(golden["worlds"], golden["kept"])
+
+
+@pytest.mark.parametrize("name", ["sin", "cos", "log", "exp", "sincos_sin", "sincos_cos"])
+def test_device_maths_equals_host(name):
+    import oracle as O
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.uniform(-10, 10, 100000), rng.uniform(-1e4, 1e4, 100000),
+                        np.exp(rng.uniform(-700, 700, 100000)) * rng.choice([-1, 1], 100000),
+                        rng.uniform(-1e8, 1e8, 50000), rng.uniform(1e8, 1e300, 20000),
+                        [0.0, -0.0, math.inf, -math.inf, math.nan, 5e-324, float.fromhex("0x1.ad1fb54442d18p+0")]])
+    if name == "log":
+        x = np.abs(x)
+    got = N.maths(name, torch.from_numpy(x).to(DEV)).cpu().numpy()
+    want = O.maths(name, x)
+    same = (_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), f"{int((~same).sum())} differ, first at x = {x[~same][:3]}"
+
+
+def test_device_sqrt_and_division_are_ieee():
+    rng = np.random.default_rng(22)
+    k = np.arange(1, 200001, dtype=np.float64)
+    x = np.concatenate([np.exp(rng.uniform(-744, 709, 400000)), k * k, k * k + 1, k * k - 1,
+                        rng.uniform(0, 1e-310, 10000), [0.0, -0.0, np.inf, 4.0]])
+    got = N.maths("sqrt", torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert (_bits(got) == _bits(np.sqrt(x))).all()
+    y = np.exp(rng.uniform(-300, 300, x.size)) * rng.choice([-1, 1], x.size)
+    q = N.maths("div", torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV)).cpu().numpy()
+    with np.errstate(all="ignore"):
+        assert (_bits(q) == _bits(x / y)).all()
+
+
+@pytest.mark.parametrize("general", [False, True], ids=["tiled", "general"])
+def test_golden_worlds_match_oracle(golden, golden_oracle, general):
+    got = D.run_park(golden["worlds"], DEV, general=general)
+    bad = D.compare(got, golden_oracle, golden["worlds"])
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+
+
+def test_golden_worlds_block_64(golden, golden_oracle):
+    got = D.run_park(golden["worlds"], DEV, block=64)
+    bad = D.compare(got, golden_oracle, golden["worlds"])
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+
+
+def _flight(worlds: int, steps: int, seed: int = 1) -> list:
+    ws = S.flight_park(seed, worlds)
+    for w in ws:
+        w["steps"] = steps
+    return ws
+
+
+def test_flight_park_matches_oracle():
+    ws = _flight(1024, 100)
+    record = {1, 2, 10, 50, 100}
+    want = D.run_oracle(ws, range(len(ws)), record)
+    got = D.run_park(ws, DEV, record=record)
+    bad = D.compare(got, want, ws)
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+    cpu = D.run_park(ws, "cpu", record={100})
+    assert not D.compare(cpu, {k: v for k, v in got.items() if k[1] == 100}, ws)
+
+
+def _big_park(worlds: int, block: int = 256, seed: int = 3, order=None) -> VecModesPark:
+    wl = WL.flight_park(seed, worlds)
+    if order is not None:
+        wl = WL.select_worlds(wl, order)
+    park = VecModesPark(wl["n"], device=DEV, block=block)
+    WL.load_flight_park(park, wl)
+    return park
+
+
+def _blocks(park: VecModesPark) -> list:
+    return [b[:, :park.n].clone() for b in (park._f, park._q, park._b)]
+
+
+def _same(a: list, b: list) -> bool:
+    return all(torch.equal(x.view(torch.int64) if x.dtype == torch.float64 else x,
+                           y.view(torch.int64) if y.dtype == torch.float64 else y) for x, y in zip(a, b))
+
+
+def test_full_size_tiled_equals_general_and_repeats():
+    W, T = 65536, 20
+    runs = []
+    for general in (False, True, False):
+        park = _big_park(W)
+        assert park.tiles is not None
+        if general:
+            park.tiles, park._state = None, None
+        for _ in range(T):
+            park.evolve()
+        torch.cuda.synchronize()
+        runs.append(_blocks(park))
+    assert _same(runs[0], runs[1]), "tiled and general launches differ"
+    assert _same(runs[0], runs[2]), "a repeated run differs"
+
+
+def test_batch_independence_and_block_sizes():
+    W, T = 4096, 30
+    base = _big_park(W)
+    for _ in range(T):
+        base.evolve()
+    ref = _blocks(base)
+    rev = list(range(W))[::-1]
+    park = _big_park(W, order=rev)
+    for _ in range(T):
+        park.evolve()
+    got = _blocks(park)
+    # row 4k + j of the reversed park holds ship j of world W-1-k: compare through that permutation
+    perm = torch.tensor([4 * wi + j for wi in rev for j in range(4)], device=DEV)
+    keep_q = [0, 1, 2, 3, 4, 6, 7, 8, 9]                  # follow_ptr holds row numbers
+    for a, b in zip(ref, got):
+        x = a[:, perm]
+        if x.dtype == torch.float64:
+            x, b = x.view(torch.int64), b.view(torch.int64)
+        elif x.dtype == torch.int64:
+            x, b = x[keep_q], b[keep_q]
+        assert torch.equal(x, b), "a world's result depends on where it sits in the batch"
+    for block in (64, 128):
+        p = _big_park(W, block=block)
+        for _ in range(T):
+            p.evolve()
+        assert _same(_blocks(p), ref), f"block {block} differs"
+    alone = _big_park(W, order=[5])
+    for _ in range(T):
+        alone.evolve()
+    one = _blocks(alone)
+    x = [b[:, 20:24] for b in ref]
+    x[1] = x[1][keep_q]
+    one[1] = one[1][keep_q]
+    assert _same(x, one), "a world alone differs from the same world in the batch"
+
+
+def test_graph_replay_equals_eager():
+    W, T = 2048, 25
+    eager = _big_park(W)
+    graphed = _big_park(W)
+    g = ModesGraph(graphed)
+    assert _same(_blocks(eager), _blocks(graphed)), "building the graph changed the park"
+    for t in range(T):
+        if t == 10:                                  # eager commands between replays
+            for p in (eager, graphed):
+                m = torch.zeros(p.n, dtype=torch.bool, device=DEV)
+                m[::7] = True
+                p.stop(m)
+                p.set_speed_fraction(m, torch.full((p.n,), 0.5, dtype=torch.float64, device=DEV))
+        eager.evolve()
+        g.tick()
+        torch.cuda.synchronize()
+        assert _same(_blocks(eager), _blocks(graphed)), f"replay {t} differs from the eager tick"
+
+
+def test_masked_tick_and_graph():
+    W = 1024
+    full = _big_park(W)
+    masked = _big_park(W)
+    gm_park = _big_park(W)
+    g = ModesGraph(gm_park, masked=True)
+    active = (masked.world % 3 != 0)
+    before = _blocks(masked)
+    masked.evolve(active=active)
+    g.tick(active)
+    full.evolve()
+    torch.cuda.synchronize()
+    sel = active.cpu()
+    for a, b, f, m in zip(before, _blocks(masked), _blocks(full), _blocks(gm_park)):
+        a, b, f, m = (x.cpu() for x in (a, b, f, m))
+        if a.dtype == torch.float64:
+            a, b, f, m = (x.view(torch.int64) for x in (a, b, f, m))
+        assert torch.equal(b[:, ~sel], a[:, ~sel]), "a row outside the mask changed"
+        assert torch.equal(b[:, sel], f[:, sel]), "a row inside the mask differs from the unmasked tick"
+        assert torch.equal(m, b), "the masked graph differs from the masked eager tick"
