@@ -11,6 +11,7 @@
 
 #define XB_HD
 #include "xb_modes_tick.h"
+#include "xb_maths_float.h"
 #include "../../include/xb.h"
 
 namespace {
@@ -102,6 +103,14 @@ int xb_cpu_maths_eval(int func, int64_t n, const double* x, double* out) {
             case 5: xb_sincos(x[i], &sn, &cs); out[i] = cs; break;
             case 6: out[i] = __builtin_sqrt(x[i]); break;
             case 7: out[i] = x[i] / x[n + i]; break;
+            case 8: out[i] = (double)xb_expf((float)x[i]); break;
+            case 9: out[i] = (double)xb_logf((float)x[i]); break;
+            case 10: out[i] = (double)xb_sinf((float)x[i]); break;
+            case 11: out[i] = (double)xb_cosf((float)x[i]); break;
+            case 12: out[i] = (double)xb_acosf((float)x[i]); break;
+            case 13: out[i] = (double)xb_atanf((float)x[i]); break;
+            case 14: out[i] = (double)xb_atan2f((float)x[i], (float)x[n + i]); break;
+            case 15: out[i] = xb_pow(x[i], x[n + i]); break;
             default: return -1;
         }
     }

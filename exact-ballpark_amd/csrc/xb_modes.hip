@@ -17,6 +17,7 @@
 #define XB_HD __device__
 #define XB_TABLE __constant__
 #include "xb_modes_tick.h"
+#include "xb_maths_float.h"
 #include "../../include/xb.h"
 
 namespace {
@@ -121,6 +122,14 @@ __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ 
         case 5: xb_sincos(x[i], &sn, &cs); out[i] = cs; break;
         case 6: out[i] = __builtin_sqrt(x[i]); break;
         case 7: out[i] = x[i] / x[n + i]; break;
+        case 8: out[i] = (double)xb_expf((float)x[i]); break;
+        case 9: out[i] = (double)xb_logf((float)x[i]); break;
+        case 10: out[i] = (double)xb_sinf((float)x[i]); break;
+        case 11: out[i] = (double)xb_cosf((float)x[i]); break;
+        case 12: out[i] = (double)xb_acosf((float)x[i]); break;
+        case 13: out[i] = (double)xb_atanf((float)x[i]); break;
+        case 14: out[i] = (double)xb_atan2f((float)x[i], (float)x[n + i]); break;
+        case 15: out[i] = xb_pow(x[i], x[n + i]); break;
         default: out[i] = xb_nan(); break;
     }
 }
@@ -178,7 +187,7 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
 }
 
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block) {
-    if (n < 0 || func < 0 || func > 7 || !xb_block_ok(block)) return -1;
+    if (n < 0 || func < 0 || func > 15 || !xb_block_ok(block)) return -1;
     if (n == 0) return 0;
     xb_maths_kernel<<<(unsigned)((n + block - 1) / block), block, 0, (hipStream_t)stream>>>(func, n, x, out);
     return xb_launch_rc();

@@ -89,15 +89,18 @@ def check(rc: int, fn: str) -> None:
         raise RuntimeError(f"{fn} failed ({rc}: {msg})")
 
 
-MATHS = {"sin": 0, "cos": 1, "log": 2, "exp": 3, "sincos_sin": 4, "sincos_cos": 5, "sqrt": 6, "div": 7}
+MATHS = {"sin": 0, "cos": 1, "log": 2, "exp": 3, "sincos_sin": 4, "sincos_cos": 5, "sqrt": 6, "div": 7, "expf": 8,
+         "logf": 9, "sinf": 10, "cosf": 11, "acosf": 12, "atanf": 13, "atan2f": 14, "pow": 15}
+BINARY = ("div", "atan2f", "pow")
 
 
 def maths(name: str, x: torch.Tensor, y: torch.Tensor | None = None, block: int = 256) -> torch.Tensor:
-    """The tick's own double function `name` elementwise (on the device of x: the GPU kernel or the host build);
-    "div" is x / y."""
+    """The tick's own maths function `name` elementwise (on the device of x: the GPU kernel or the host build): the
+    double sin, cos, log, exp, sincos's two parts, IEEE sqrt and division x / y; the float expf, logf, sinf, cosf,
+    acosf, atanf, atan2f(x, y) (x the ordinate) on floats held in doubles; the double pow(x, y)."""
     x = x.to(torch.float64).contiguous()
     n = x.numel()
-    inp = torch.cat((x.reshape(-1), y.to(torch.float64).reshape(-1))) if name == "div" else x.reshape(-1)
+    inp = torch.cat((x.reshape(-1), y.to(torch.float64).reshape(-1))) if name in BINARY else x.reshape(-1)
     out = torch.empty(n, dtype=torch.float64, device=x.device)
     if x.device.type == "cuda":
         with torch.cuda.device(x.device):

@@ -67,7 +67,9 @@ int xb_layout(int64_t* out);
 int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
 
 /* The tick's double functions alone, out[i] = f(x[i]) (func 0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 sincos's
-   cosine, 6 IEEE sqrt, 7 IEEE division x[i] / x[n + i] with out of n elements). Replaces xbpm_dlibm
+   cosine, 6 IEEE sqrt, 7 IEEE division x[i] / x[n + i] with out of n elements; on floats held in doubles 8 expf, 9
+   logf, 10 sinf, 11 cosf, 12 acosf, 13 atanf, 14 atan2f(x[i], x[n + i]); 15 the double pow(x[i], x[n + i])). Replaces
+   xbpm_dlibm and xbpo_libm
    (src/exact_ballpark/vec_modes_cuda.cu:779). */
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block);
 

@@ -116,3 +116,25 @@ def test_host_build_sqrt_and_division_are_ieee():
     y = np.exp(rng.uniform(-300, 300, x.size))
     q = N.maths("div", torch.from_numpy(x), torch.from_numpy(y)).numpy()
     assert (_bits(q) == _bits(x / y)).all()
+
+
+def _sweep(tmp_path, unfused: bool, stride: int) -> dict:
+    import json
+    import subprocess
+    src = Path(__file__).resolve().parent / "maths_float_sweep.cpp"
+    exe = tmp_path / ("sweep_unfused" if unfused else "sweep")
+    cmd = ["g++", "-O2", "-fopenmp", "-ffp-contract=off", "-march=x86-64-v3", str(src), "-o", str(exe), "-lm"]
+    if unfused:
+        cmd.insert(1, "-DXB_FDLIBM_FMAF(a,b,c)=((a)*(b)+(c))")
+    subprocess.run(cmd, check=True)
+    out = subprocess.run([str(exe), str(stride)], check=True, capture_output=True, text=True).stdout
+    return json.loads(out)
+
+
+def test_float_maths_sweep_against_c_library(tmp_path):
+    """Every 16th float bit pattern (exhaustive with stride 1: logf, sinf, cosf 0 differences; expf 2, both ties)."""
+    r = _sweep(tmp_path, unfused=False, stride=16)
+    assert r["expf"] == 0 and r["logf"] == 0 and r["sinf"] == 0 and r["cosf"] == 0 and r["pow"] == 0
+    u = _sweep(tmp_path, unfused=True, stride=16)
+    assert u["acosf"] == 0 and u["atanf"] == 0 and u["atan2f"] == 0
+    assert r["acosf"] > 0 and r["atanf"] > 0      # the fused steps do change bits: the two builds really differ
