@@ -54,12 +54,18 @@ struct Scratch {
     }
 };
 
-void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, int64_t lo, int64_t cnt) {
+void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector<XbRow>& rows, int64_t lo,
+               int64_t cnt) {
     S.size((int)cnt);
-    auto row = [&](int64_t k) -> int64_t { return s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k; };
-    for (int64_t k = 0; k < cnt; ++k) xb_record_a(P, S.R, row(k), (int)k);
-    for (int64_t k = 0; k < cnt; ++k) xb_record_b(P, S.R, row(k), (int)k, 0, (int)cnt);
-    for (int64_t k = 0; k < cnt; ++k) xb_tick_row(P, S.R, row(k), (int)k, 0, (int)cnt);
+    rows.resize((size_t)cnt);
+    for (int64_t k = 0; k < cnt; ++k) xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
+    const int ticks = s.ticks > 0 ? s.ticks : 1;
+    for (int t = 0; t < ticks; ++t) {
+        for (int64_t k = 0; k < cnt; ++k) xb_record_a(rows[k], S.R, (int)k);
+        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt);
+        for (int64_t k = 0; k < cnt; ++k) xb_tick_row(rows[k], S.R, (int)k, 0, (int)cnt, P);
+    }
+    for (int64_t k = 0; k < cnt; ++k) xb_row_store(P, rows[k]);
 }
 
 }  // namespace
@@ -76,7 +82,8 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
     const XbPark P = host_park(*state);
     if (!state->tiles) {
         Scratch S;
-        run_group(P, *state, S, 0, state->n);
+        std::vector<XbRow> rows;
+        run_group(P, *state, S, rows, 0, state->n);
         return 0;
     }
     const int nt = state->n_tiles;
@@ -84,9 +91,10 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
 #pragma omp parallel
     {
         Scratch S;
-#pragma omp for schedule(static)
+        std::vector<XbRow> rows;
+#pragma omp for schedule(dynamic, 16)
         for (int g = 0; g < nt; ++g)
-            run_group(P, *state, S, state->tiles[g], (int64_t)state->tiles[g + 1] - state->tiles[g]);
+            run_group(P, *state, S, rows, state->tiles[g], (int64_t)state->tiles[g + 1] - state->tiles[g]);
     }
     return 0;
 }

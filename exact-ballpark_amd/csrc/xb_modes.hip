@@ -60,11 +60,18 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     const XbPark P = xb_park(s);
     const bool on = j < cnt;
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
-    if (on) xb_record_a(P, R, r, j);
-    __syncthreads();
-    if (on) xb_record_b(P, R, r, j, 0, cnt);
-    __syncthreads();
-    if (on) xb_tick_row(P, R, r, j, 0, cnt);
+    const int ticks = s.ticks > 0 ? s.ticks : 1;
+    XbRow w;
+    if (on) xb_row_load(P, r, w);
+    for (int t = 0; t < ticks; ++t) {
+        if (on) xb_record_a(w, R, j);
+        __syncthreads();
+        if (on) xb_record_b(w, R, j, 0, cnt);
+        __syncthreads();
+        if (on) xb_tick_row(w, R, j, 0, cnt, P);
+        __syncthreads();
+    }
+    if (on) xb_row_store(P, w);
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -87,26 +94,32 @@ __global__ void __launch_bounds__(256) xb_modes_step_a(xb_modes_state s) {
     if (r >= s.n) return;
     XbRecords R = xb_global_records(s);
     const XbPark P = xb_park(s);
-    // the record's id and radius rows are the park's own (read only during the tick)
-    XbV3 pos = xb_ld3(P, XB_F_POS, r), vel = xb_ld3(P, XB_F_VEL, r);
-    int64_t es = XB_Q(P, XB_I_EFFECT_STAMP, r);
-    bool act = xb_live(P, r) && XB_Q(P, XB_I_MODE, r) == XB_MODE_WARP && es < 0 &&
-               xb_aligned(xb_ld3(P, XB_F_GOTO, r), pos, vel, XB_F(P, XB_F_MAX_VEL, r), es);
-    R.act[r] = act ? 1 : 0;
-    R.p0x[r] = pos.x, R.p0y[r] = pos.y, R.p0z[r] = pos.z;
-    R.v0x[r] = vel.x, R.v0y[r] = vel.y, R.v0z[r] = vel.z;
+    XbRow w;
+    xb_row_load(P, r, w);
+    const XbShip& sh = w.s;
+    bool act = xb_row_live(w) && sh.mode == XB_MODE_WARP && sh.effect_stamp < 0 &&
+               xb_aligned(sh.gp, sh.pos, sh.vel, sh.max_vel, sh.effect_stamp);
+    R.act[r] = act ? 1 : 0;           // the id and radius records are the park's own rows, read only
+    R.p0x[r] = sh.pos.x, R.p0y[r] = sh.pos.y, R.p0z[r] = sh.pos.z;
+    R.v0x[r] = sh.vel.x, R.v0y[r] = sh.vel.y, R.v0z[r] = sh.vel.z;
 }
 
 __global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= s.n) return;
-    xb_record_b(xb_park(s), xb_global_records(s), r, (int)r, 0, (int)s.n);
+    XbRow w;
+    xb_row_load(xb_park(s), r, w);
+    xb_record_b(w, xb_global_records(s), (int)r, 0, (int)s.n);
 }
 
 __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= s.n) return;
-    xb_tick_row(xb_park(s), xb_global_records(s), r, (int)r, 0, (int)s.n);
+    const XbPark P = xb_park(s);
+    XbRow w;
+    xb_row_load(P, r, w);
+    xb_tick_row(w, xb_global_records(s), (int)r, 0, (int)s.n, P);
+    xb_row_store(P, w);
 }
 
 __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ x, double* __restrict__ out) {
@@ -174,7 +187,7 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
         xb_modes_tiled_kernel<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
-    if (!state->scratch_f64 || !state->scratch_u8) return -1;
+    if (!state->scratch_f64 || !state->scratch_u8 || state->ticks > 1) return -1;
     const unsigned grid = (unsigned)((state->n + block - 1) / block);
     xb_modes_step_a<<<grid, block, 0, st>>>(*state);
     int rc = xb_launch_rc();

@@ -6,7 +6,8 @@
  * src/exact_ballpark/vec_modes_cuda.cu:540-717 on CUDA) and its scalar world defines it (src/exact_ballpark/modes.py:
  * 359-745, ref.py:151-211).
  *
- * A tick of a group of whole worlds runs in three steps, each over every row of the group:
+ * A row's state is loaded into registers once (`xb_row_load`), ticked, and stored once (`xb_row_store`); a tick of a
+ * group of whole worlds runs in three steps, each over every row of the group:
  *   A. `xb_record_a`: what the row shows the others in phase 1 -- its tick-start position and velocity, its id and
  *      radius, whether it starts warp proper at its own turn (`act`);
  *   B. `xb_record_b`: its vertical velocity after its own turn (STOP's damping, counting a release before its turn);
@@ -419,45 +420,108 @@ static inline XB_HD void xb_integrate(XbV3* pp, XbV3* vp, XbV3 a, double m, doub
 
 /* ------------------------------------------------------------------------------------ the three steps */
 
-static inline XB_HD bool xb_live(const XbPark& P, int64_t r) {
-    return XB_B(P, XB_B_FREE, r) && !XB_B(P, XB_B_MORIBUND, r);
+
+/* A row's whole state as its lane holds it across the ticks of one launch: loaded once, ticked in registers, stored
+   once. The flags are the row's own (free, moribund, interactive, massive, sensor). */
+struct XbRow {
+    XbShip s;
+    XbV3 old_pos, old_vel;
+    bool free_, moribund;
+    int64_t row;                   /* its physical row */
+};
+
+static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
+    XbShip& s = w.s;
+    s.pos = xb_ld3(P, XB_F_POS, r);
+    s.vel = xb_ld3(P, XB_F_VEL, r);
+    s.gp = xb_ld3(P, XB_F_GOTO, r);
+    s.last_g = xb_ld3(P, XB_F_LAST_G, r);
+    w.old_pos = xb_ld3(P, XB_F_OLD_POS, r);
+    w.old_vel = xb_ld3(P, XB_F_OLD_VEL, r);
+    s.mass = XB_F(P, XB_F_MASS, r);
+    s.tf = XB_F(P, XB_F_TIME_FACTOR, r);
+    s.last_collision = XB_F(P, XB_F_LAST_COLLISION, r);
+    s.radius = XB_F(P, XB_F_RADIUS, r);
+    s.max_vel = XB_F(P, XB_F_MAX_VEL, r);
+    s.agility = XB_F(P, XB_F_AGILITY, r);
+    s.sf = XB_F(P, XB_F_SPEED_FRACTION, r);
+    s.follow_range = XB_F(P, XB_F_FOLLOW_RANGE, r);
+    s.id = XB_Q(P, XB_I_IDS, r);
+    s.mode = XB_Q(P, XB_I_MODE, r);
+    s.effect_stamp = XB_Q(P, XB_I_EFFECT_STAMP, r);
+    s.follow_id = XB_Q(P, XB_I_FOLLOW_ID, r);
+    s.owner_id = XB_Q(P, XB_I_OWNER_ID, r);
+    s.follow_ptr = XB_Q(P, XB_I_FOLLOW_PTR, r);
+    s.clock = XB_Q(P, XB_I_CLOCK, r);
+    w.free_ = XB_B(P, XB_B_FREE, r) != 0;
+    w.moribund = XB_B(P, XB_B_MORIBUND, r) != 0;
+    s.massive = XB_B(P, XB_B_MASSIVE, r) != 0;
+    s.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
+    s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
+    w.row = r;
 }
 
-/* Step A for row r at place j of its group */
-static inline XB_HD void xb_record_a(const XbPark& P, const XbRecords& R, int64_t r, int j) {
-    XbV3 pos = xb_ld3(P, XB_F_POS, r), vel = xb_ld3(P, XB_F_VEL, r);
-    int64_t es = XB_Q(P, XB_I_EFFECT_STAMP, r);
-    bool act = xb_live(P, r) && XB_Q(P, XB_I_MODE, r) == XB_MODE_WARP && es < 0 &&
-               xb_aligned(xb_ld3(P, XB_F_GOTO, r), pos, vel, XB_F(P, XB_F_MAX_VEL, r), es);
+/* Every field the tick may change (radius, mass, agility, maximum speed, decay, ids, flags moribund: never) */
+static inline XB_HD void xb_row_store(const XbPark& P, const XbRow& w) {
+    const int64_t r = w.row;
+    const XbShip& s = w.s;
+    xb_st3(P, XB_F_POS, r, s.pos);
+    xb_st3(P, XB_F_VEL, r, s.vel);
+    xb_st3(P, XB_F_GOTO, r, s.gp);
+    xb_st3(P, XB_F_LAST_G, r, s.last_g);
+    xb_st3(P, XB_F_OLD_POS, r, w.old_pos);
+    xb_st3(P, XB_F_OLD_VEL, r, w.old_vel);
+    XB_F(P, XB_F_LAST_COLLISION, r) = s.last_collision;
+    XB_F(P, XB_F_SPEED_FRACTION, r) = s.sf;
+    XB_F(P, XB_F_FOLLOW_RANGE, r) = s.follow_range;
+    XB_Q(P, XB_I_MODE, r) = s.mode;
+    XB_Q(P, XB_I_EFFECT_STAMP, r) = s.effect_stamp;
+    XB_Q(P, XB_I_FOLLOW_ID, r) = s.follow_id;
+    XB_Q(P, XB_I_OWNER_ID, r) = s.owner_id;
+    XB_Q(P, XB_I_FOLLOW_PTR, r) = s.follow_ptr;
+    XB_Q(P, XB_I_CLOCK, r) = s.clock;
+    XB_B(P, XB_B_FREE, r) = w.free_ ? 1 : 0;
+    XB_B(P, XB_B_MASSIVE, r) = s.massive ? 1 : 0;
+    XB_B(P, XB_B_INTERACTIVE, r) = s.interactive ? 1 : 0;
+    XB_B(P, XB_B_SENSOR, r) = s.sensor ? 1 : 0;
+}
+
+static inline XB_HD bool xb_row_live(const XbRow& w) { return w.free_ && !w.moribund; }
+
+/* Step A for the row at place j of its group */
+static inline XB_HD void xb_record_a(const XbRow& w, const XbRecords& R, int j) {
+    const XbShip& s = w.s;
+    bool act = xb_row_live(w) && s.mode == XB_MODE_WARP && s.effect_stamp < 0 &&
+               xb_aligned(s.gp, s.pos, s.vel, s.max_vel, s.effect_stamp);
     R.act[j] = act ? 1 : 0;
-    R.p0x[j] = pos.x, R.p0y[j] = pos.y, R.p0z[j] = pos.z;
-    R.v0x[j] = vel.x, R.v0y[j] = vel.y, R.v0z[j] = vel.z;
-    R.id[j] = XB_Q(P, XB_I_IDS, r);
-    R.radius[j] = XB_F(P, XB_F_RADIUS, r);
+    R.p0x[j] = s.pos.x, R.p0y[j] = s.pos.y, R.p0z[j] = s.pos.z;
+    R.v0x[j] = s.vel.x, R.v0y[j] = s.vel.y, R.v0z[j] = s.vel.z;
+    R.id[j] = s.id;
+    R.radius[j] = s.radius;
 }
 
-/* The place of row r's follow target in its group (rows of a world keep their order and spacing in the group), or -1
+/* The place of the row's follow target in its group (a world's rows keep their order and spacing in the group), or -1
    when it has none or the target is not in the follower set as StopAllFollowers tests it (its id is the follow id) */
-static inline XB_HD int xb_member_place(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
-    int64_t t = XB_Q(P, XB_I_FOLLOW_PTR, r);
+static inline XB_HD int xb_member_place(const XbRow& w, const XbRecords& R, int j, int lo, int hi) {
+    int64_t t = w.s.follow_ptr;
     if (t < 0) return -1;
-    int64_t tj = (int64_t)j + (t - r);
+    int64_t tj = (int64_t)j + (t - w.row);
     if (tj < lo || tj >= hi) return -1;
-    return R.id[tj] == XB_Q(P, XB_I_FOLLOW_ID, r) ? (int)tj : -1;
+    return R.id[tj] == w.s.follow_id ? (int)tj : -1;
 }
 
-/* Step B: the vertical velocity row r shows after its own turn (no dynamical orientation) */
-static inline XB_HD void xb_record_b(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
-    double vy = R.v0y[j];
-    if (xb_live(P, r)) {
-        int64_t m = XB_Q(P, XB_I_MODE, r);
-        int tj = xb_member_place(P, R, r, j, lo, hi);
-        if (tj >= 0 && R.id[tj] < R.id[j] && R.act[tj]) {
+/* Step B: the vertical velocity the row shows after its own turn (no dynamical orientation) */
+static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, int lo, int hi) {
+    double vy = w.s.vel.y;
+    if (xb_row_live(w)) {
+        int64_t m = w.s.mode;
+        int tj = xb_member_place(w, R, j, lo, hi);
+        if (tj >= 0 && R.id[tj] < w.s.id && R.act[tj]) {
             XbShip f;
             f.mode = m;
-            f.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
-            f.pos = v3(R.p0x[j], R.p0y[j], R.p0z[j]);
-            f.vel = v3(R.v0x[j], R.v0y[j], R.v0z[j]);
+            f.interactive = w.s.interactive;
+            f.pos = w.s.pos;
+            f.vel = w.s.vel;
             m = xb_release(f, false);
         }
         if (m == XB_MODE_STOP) vy = (vy - XB_STOP_DECAY * vy) * XB_STOP_DAMP;
@@ -465,36 +529,15 @@ static inline XB_HD void xb_record_b(const XbPark& P, const XbRecords& R, int64_
     R.vy_after[j] = vy;
 }
 
-/* Step C: row r's whole tick in place (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi) */
-static inline XB_HD void xb_tick_row(const XbPark& P, const XbRecords& R, int64_t r, int j, int lo, int hi) {
-    XbShip s;
-    s.pos = v3(R.p0x[j], R.p0y[j], R.p0z[j]);
-    s.vel = v3(R.v0x[j], R.v0y[j], R.v0z[j]);
-    s.gp = xb_ld3(P, XB_F_GOTO, r);
-    s.last_g = xb_ld3(P, XB_F_LAST_G, r);
-    s.mass = XB_F(P, XB_F_MASS, r);
-    s.tf = XB_F(P, XB_F_TIME_FACTOR, r);
-    s.last_collision = XB_F(P, XB_F_LAST_COLLISION, r);
-    s.radius = R.radius[j];
-    s.max_vel = XB_F(P, XB_F_MAX_VEL, r);
-    s.agility = XB_F(P, XB_F_AGILITY, r);
-    s.sf = XB_F(P, XB_F_SPEED_FRACTION, r);
-    s.follow_range = XB_F(P, XB_F_FOLLOW_RANGE, r);
-    s.id = R.id[j];
-    s.mode = XB_Q(P, XB_I_MODE, r);
-    s.effect_stamp = XB_Q(P, XB_I_EFFECT_STAMP, r);
-    s.follow_id = XB_Q(P, XB_I_FOLLOW_ID, r);
-    s.owner_id = XB_Q(P, XB_I_OWNER_ID, r);
-    s.follow_ptr = XB_Q(P, XB_I_FOLLOW_PTR, r);
-    s.clock = XB_Q(P, XB_I_CLOCK, r);
-    s.massive = XB_B(P, XB_B_MASSIVE, r) != 0;
-    s.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
-    s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
-    const bool live = xb_live(P, r);
+/* Step C: the row's whole tick (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi). Other rows are
+   read only through the records. */
+static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
+    XbShip& s = w.s;
+    const bool live = xb_row_live(w);
     const bool act_self = R.act[j] != 0;
     /* phase 1: a release when the target starts warp proper, before or after this ball's own turn by their ids */
     bool rel_before = false, rel_after = false;
-    int tj = xb_member_place(P, R, r, j, lo, hi);
+    int tj = xb_member_place(w, R, j, lo, hi);
     if (tj >= 0 && R.act[tj]) {
         if (R.id[tj] < s.id) rel_before = true;
         else rel_after = true;
@@ -503,7 +546,7 @@ static inline XB_HD void xb_tick_row(const XbPark& P, const XbRecords& R, int64_
     if (live) {
         XbV3 a = v3(0.0, 0.0, 0.0);
         int64_t t = s.follow_ptr;
-        int64_t tp = (t >= 0) ? (int64_t)j + (t - r) : -1;
+        int64_t tp = (t >= 0) ? (int64_t)j + (t - w.row) : -1;
         bool target_ok = t >= 0 && tp >= lo && tp < hi;
         if (s.mode == XB_MODE_WARP) {
             if (s.effect_stamp >= 0) {
@@ -548,35 +591,21 @@ static inline XB_HD void xb_tick_row(const XbPark& P, const XbRecords& R, int64_
         } else {
             xb_integrate(&p, &v, a, m, s.tf, P);
         }
-        xb_st3(P, XB_F_OLD_POS, r, s.pos);
-        xb_st3(P, XB_F_OLD_VEL, r, s.vel);
+        w.old_pos = s.pos;
+        w.old_vel = s.vel;
         s.pos = p;
         s.vel = v;
     }
     if (live && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
         /* PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937): fixed, at rest, RIGID */
         s.vel = v3(0.0, 0.0, 0.0);
-        xb_st3(P, XB_F_OLD_VEL, r, s.vel);
+        w.old_vel = s.vel;
         s.last_g = s.vel;
-        XB_B(P, XB_B_FREE, r) = 0;
-        XB_B(P, XB_B_INTERACTIVE, r) = 0;
+        w.free_ = false;
+        s.interactive = false;
         s.mode = XB_MODE_RIGID;
     }
-    xb_st3(P, XB_F_POS, r, s.pos);
-    xb_st3(P, XB_F_VEL, r, s.vel);
-    xb_st3(P, XB_F_GOTO, r, s.gp);
-    xb_st3(P, XB_F_LAST_G, r, s.last_g);
-    XB_F(P, XB_F_LAST_COLLISION, r) = s.last_collision;
-    XB_F(P, XB_F_SPEED_FRACTION, r) = s.sf;
-    XB_F(P, XB_F_FOLLOW_RANGE, r) = s.follow_range;
-    XB_Q(P, XB_I_MODE, r) = s.mode;
-    XB_Q(P, XB_I_EFFECT_STAMP, r) = s.effect_stamp;
-    XB_Q(P, XB_I_FOLLOW_ID, r) = s.follow_id;
-    XB_Q(P, XB_I_OWNER_ID, r) = s.owner_id;
-    XB_Q(P, XB_I_FOLLOW_PTR, r) = s.follow_ptr;
-    XB_Q(P, XB_I_CLOCK, r) = s.clock + 1;
-    XB_B(P, XB_B_MASSIVE, r) = s.massive ? 1 : 0;
-    XB_B(P, XB_B_SENSOR, r) = s.sensor ? 1 : 0;
+    s.clock = s.clock + 1;
 }
 
 #endif

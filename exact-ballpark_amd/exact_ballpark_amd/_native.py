@@ -27,7 +27,7 @@ class ModesState(ctypes.Structure):
                 ("rowmap", ctypes.c_void_p), ("n", ctypes.c_int64), ("stride", ctypes.c_int64),
                 ("n_tiles", ctypes.c_int32), ("new_orbit", ctypes.c_int32), ("friction", ctypes.c_double),
                 ("dt", ctypes.c_double), ("ffriction", ctypes.c_float), ("fdt", ctypes.c_float),
-                ("dynamical", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32)]
 
 
 _LIBS: dict = {}

@@ -184,7 +184,7 @@ class VecModesPark(metaclass=_Fields):
             s.n_tiles = (len(self.tiles) - 1) if self.tiles is not None else 0
             s.new_orbit = int(self.use_new_orbit)
             s.friction, s.dt, s.ffriction, s.fdt = self.friction, self.dt, self.ffriction, self.fdt
-            s.dynamical, s.reserved = 0, 0
+            s.dynamical, s.ticks = 0, 1
             self._state = s
         self._state.new_orbit = int(self.use_new_orbit)
         return self._state

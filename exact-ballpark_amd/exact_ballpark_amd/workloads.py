@@ -121,3 +121,89 @@ def select_worlds(wl: dict, order) -> dict:
     c = {k: v[rows] for k, v in wl["cmds"].items()}
     c["target"] = new_row[c["target"]]
     return {"n": int(rows.size), "worlds": int(order.size), "ships": s, "rows": r, "cmds": c}
+
+
+def truncated_geometric_p(mean: float, top: int) -> float:
+    """p of the geometric law on {1..top}, P(k) proportional to (1-p)^(k-1), whose mean is `mean` (bisection)."""
+    k = np.arange(1, top + 1, dtype=np.float64)
+    lo, hi = 1e-9, 1.0 - 1e-12
+    for _ in range(200):
+        p = 0.5 * (lo + hi)
+        w = (1.0 - p) ** (k - 1)
+        m = float((k * w).sum() / w.sum())
+        lo, hi = (p, hi) if m > mean else (lo, p)
+    return 0.5 * (lo + hi)
+
+
+def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32) -> dict:
+    """The ragged flight park: ships per world drawn from the geometric law on {1..max_ships} with mean `mean_ships`,
+    each ship as flight_park's (modes uniform over stop, goto, follow, orbit, warp at tick 1). Rows world-major."""
+    rng = np.random.default_rng(seed)
+    p = truncated_geometric_p(mean_ships, max_ships)
+    k = np.arange(1, max_ships + 1)
+    w = (1.0 - p) ** (k - 1)
+    counts = rng.choice(k, size=worlds, p=w / w.sum()).astype(np.int64)
+    n = int(counts.sum())
+    world = np.repeat(np.arange(worlds, dtype=np.int64), counts)
+    start = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    own = np.arange(n, dtype=np.int64) - np.repeat(start, counts)
+    ids = own + 1
+    rows = {
+        "id": ids, "world": world, "time": np.zeros(n, dtype=np.int64),
+        "mass": rng.uniform(1e6, 1e8, n), "radius": rng.uniform(20.0, 500.0, n),
+        "max_vel": rng.uniform(100.0, 3000.0, n), "agility": rng.uniform(0.1, 5.0, n),
+        "speed_fraction": np.ones(n), "pos": rng.uniform(-5e4, 5e4, (n, 3)), "vel": np.zeros((n, 3)),
+        "free": np.ones(n, dtype=bool), "massive": np.ones(n, dtype=bool),
+        "interactive": rng.uniform(0.0, 1.0, n) < 0.5, "moribund": np.zeros(n, dtype=bool),
+        "cloaked": np.zeros(n, dtype=np.int64), "bubble": np.zeros(n, dtype=np.int64),
+        "mode": np.full(n, 2, dtype=np.int64),
+    }
+    size = np.repeat(counts, counts)
+    kind = rng.integers(0, 5, n)
+    kind = np.where((size == 1) & ((kind == 2) | (kind == 3)), 0, kind)   # a lone ship has nothing to follow
+    op = np.select([kind == 1, kind == 2, kind == 3, kind == 4], [GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP], 0)
+    point = _ball_points(rng, n, 1e10)
+    other = np.floor(rng.uniform(0.0, 1.0, n) * np.maximum(size - 1, 1)).astype(np.int64)
+    other = np.where(other >= own, other + 1, other)
+    target = np.repeat(start, counts) + np.minimum(other, size - 1)
+    cmds = {"op": op.astype(np.int64), "point": point, "target": target.astype(np.int64),
+            "range": rng.uniform(500.0, 20000.0, n), "min_range": np.full(n, 20000.0),
+            "warp_factor": np.full(n, 20, dtype=np.int64)}
+    return {"n": n, "worlds": worlds, "ships": None, "counts": counts, "start": start, "rows": rows, "cmds": cmds}
+
+
+def padded(wl: dict, width: int) -> dict:
+    """The ragged workload with every world padded to `width` rows: the padding rows are fixed balls (not free, so no
+    tick touches them) with ids above the world's own; real rows keep their order and their results."""
+    counts, start = wl["counts"], wl["start"]
+    if int(counts.max()) > width:
+        raise ValueError("a world is wider than the padding")
+    W = wl["worlds"]
+    dest = (np.repeat(np.arange(W) * width, counts) + (np.arange(wl["n"]) - np.repeat(start, counts)))
+    n = W * width
+    rows = {}
+    for key, v in wl["rows"].items():
+        out = np.zeros((n,) + v.shape[1:], dtype=v.dtype)
+        out[dest] = v
+        rows[key] = out
+    slot = np.tile(np.arange(width), W)
+    real = np.zeros(n, dtype=bool)
+    real[dest] = True
+    rows["id"] = np.where(real, rows["id"], slot + 1 + width)
+    rows["world"] = np.repeat(np.arange(W, dtype=np.int64), width)
+    rows["free"] = np.where(real, rows["free"], False)
+    rows["mass"] = np.where(real, rows["mass"], 1.0)
+    rows["max_vel"] = np.where(real, rows["max_vel"], 1.0)
+    rows["agility"] = np.where(real, rows["agility"], 1.0)
+    rows["speed_fraction"] = np.where(real, rows["speed_fraction"], 1.0)
+    rows["mode"] = np.where(real, rows["mode"], 2)
+    cmds = {}
+    for key, v in wl["cmds"].items():
+        out = np.zeros((n,) + v.shape[1:], dtype=v.dtype)
+        out[dest] = v
+        cmds[key] = out
+    new_row = np.full(wl["n"], -1, dtype=np.int64)
+    new_row[np.arange(wl["n"])] = dest
+    cmds["target"] = np.where(real, new_row[np.clip(cmds["target"], 0, wl["n"] - 1)], 0)
+    return {"n": n, "worlds": W, "ships": width, "counts": counts, "start": np.arange(W) * width, "rows": rows,
+            "cmds": cmds, "real": real, "dest": dest}

@@ -30,8 +30,10 @@ extern "C" {
      no world split across tiles: every follow target of a row lies in its tile), its worlds' tick-start records held
      in LDS. `rowmap`, when not NULL, maps the tiled index space to physical rows (each world's rows consecutive and in
      order in both), so worlds can be regrouped without moving their state.
-   - general (`tiles` NULL): any row order; two launches through the caller's scratch (XB_NS float64 rows and one
-     uint8 row of `stride` each). */
+   - general (`tiles` NULL): any row order; three launches through the caller's scratch (XB_NS float64 rows and one
+     uint8 row of `stride` each), one tick.
+   `ticks` > 1 (tiled only) runs that many ticks in the one launch: each lane keeps its row's state in registers from
+   the first tick to the last, so every field is read from and written to HBM once per launch, not once per tick. */
 typedef struct xb_modes_state {
     double* f64;
     int64_t* i64;
@@ -49,7 +51,7 @@ typedef struct xb_modes_state {
     float ffriction;
     float fdt;
     int32_t dynamical;
-    int32_t reserved;
+    int32_t ticks;                 /* ticks per launch (tiled shape only; 0 or 1: one) */
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */
