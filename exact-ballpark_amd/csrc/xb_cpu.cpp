@@ -65,7 +65,10 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
         for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt);
         for (int64_t k = 0; k < cnt; ++k) xb_tick_row(rows[k], S.R, (int)k, 0, (int)cnt, P);
     }
-    for (int64_t k = 0; k < cnt; ++k) xb_row_store(P, rows[k]);
+    for (int64_t k = 0; k < cnt; ++k) {
+        xb_row_store(P, rows[k]);
+        if (s.path) s.path[rows[k].row] = rows[k].path;
+    }
 }
 
 }  // namespace

@@ -37,7 +37,51 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     return P;
 }
 
-constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 1;
+constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 4 + 1;     // the records, the lane schedule (int32), act
+constexpr int XB_MAX_WAVES = 4;                          // 256 lanes
+constexpr int XB_KEY_BUCKETS = 16;
+
+// The place (row of the tile) lane j runs: j itself in row order; otherwise the lanes sorted by their rows' keys, a
+// stable counting sort by ballot (a lane's rank among equal keys: its wave's lower lanes, then the lower waves), so
+// the assignment is deterministic. Places past the tile's row count take the last bucket.
+__device__ __forceinline__ int xb_schedule_place(const xb_modes_state& s, const XbPark& P, int j, int cnt, int lo,
+                                                 int* perm) {
+    __shared__ int cnt_w[XB_MAX_WAVES][XB_KEY_BUCKETS];
+    __shared__ int off_w[XB_MAX_WAVES][XB_KEY_BUCKETS];
+    int key = XB_KEY_BUCKETS - 1;
+    if (j < cnt) {
+        const int64_t r = s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j);
+        if (s.schedule == 2) {
+            key = s.keys[r];
+        } else {
+            const bool live = XB_B(P, XB_B_FREE, r) && !XB_B(P, XB_B_MORIBUND, r);
+            key = xb_branch_key(XB_Q(P, XB_I_MODE, r), XB_Q(P, XB_I_EFFECT_STAMP, r), live, P.new_orbit);
+        }
+        key = key < XB_KEY_BUCKETS - 1 ? key : XB_KEY_BUCKETS - 2;
+    }
+    const int wave = j >> 6, lane = j & 63;
+    const unsigned long long below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    int rank = 0;
+    for (int b = 0; b < XB_KEY_BUCKETS; ++b) {
+        const unsigned long long m = __ballot(key == b);
+        if (lane == 0) cnt_w[wave][b] = __popcll(m);
+        if (key == b) rank = __popcll(m & below);
+    }
+    __syncthreads();
+    if (j == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        int total = 0;
+        for (int b = 0; b < XB_KEY_BUCKETS; ++b)
+            for (int w = 0; w < nw; ++w) {
+                off_w[w][b] = total;
+                total += cnt_w[w][b];
+            }
+    }
+    __syncthreads();
+    perm[off_w[wave][key] + rank] = j;
+    __syncthreads();
+    return perm[j];
+}
 
 __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     extern __shared__ double xb_lds[];
@@ -45,7 +89,6 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     const int g = blockIdx.x;
     const int lo = s.tiles[g];
     const int cnt = s.tiles[g + 1] - lo;
-    const int j = threadIdx.x;
     XbRecords R;
     R.p0x = xb_lds;
     R.p0y = R.p0x + nb;
@@ -56,8 +99,10 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     R.vy_after = R.v0z + nb;
     R.radius = R.vy_after + nb;
     R.id = reinterpret_cast<int64_t*>(R.radius + nb);
-    R.act = reinterpret_cast<uint8_t*>(R.id + nb);
+    int* perm = reinterpret_cast<int*>(R.id + nb);
+    R.act = reinterpret_cast<uint8_t*>(perm + nb);
     const XbPark P = xb_park(s);
+    const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
     const bool on = j < cnt;
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
     const int ticks = s.ticks > 0 ? s.ticks : 1;
@@ -71,7 +116,10 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
         if (on) xb_tick_row(w, R, j, 0, cnt, P);
         __syncthreads();
     }
-    if (on) xb_row_store(P, w);
+    if (on) {
+        xb_row_store(P, w);
+        if (s.path) s.path[r] = w.path;
+    }
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -120,6 +168,7 @@ __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     xb_row_load(P, r, w);
     xb_tick_row(w, xb_global_records(s), (int)r, 0, (int)s.n, P);
     xb_row_store(P, w);
+    if (s.path) s.path[r] = w.path;
 }
 
 __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ x, double* __restrict__ out) {
@@ -181,13 +230,14 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     if (state->dynamical) return -1;                           // dynamical orientation: not in this build
     if (!state->f64 || !state->i64 || !state->u8) return -1;
     hipStream_t st = (hipStream_t)stream;
+    if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
     if (state->tiles) {
         if (state->n_tiles <= 0) return -1;
         const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
         xb_modes_tiled_kernel<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
-    if (!state->scratch_f64 || !state->scratch_u8 || state->ticks > 1) return -1;
+    if (!state->scratch_f64 || !state->scratch_u8 || state->ticks > 1 || state->schedule) return -1;
     const unsigned grid = (unsigned)((state->n + block - 1) / block);
     xb_modes_step_a<<<grid, block, 0, st>>>(*state);
     int rc = xb_launch_rc();

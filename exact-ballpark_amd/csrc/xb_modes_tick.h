@@ -427,8 +427,31 @@ struct XbRow {
     XbShip s;
     XbV3 old_pos, old_vel;
     bool free_, moribund;
+    uint8_t path;                  /* the path class its last tick executed (XB_PATH_*) */
     int64_t row;                   /* its physical row */
 };
+
+/* Path classes: what a row's tick executes beyond the integrator, from cheapest to dearest in the double functions it
+   calls. The built-in scheduling key predicts the class from the tick-start state; the tick records the one it ran. */
+#define XB_PATH_IDLE 0             /* not free, or moribund: nothing */
+#define XB_PATH_COAST 1            /* STOP, inert modes, a troll: the integrator alone */
+#define XB_PATH_GOTO 2             /* GOTO, WARP aligning: GotoThrust */
+#define XB_PATH_FOLLOW 3           /* FOLLOW */
+#define XB_PATH_ORBIT_OLD 4        /* the old orbit: two sincos, exp */
+#define XB_PATH_ORBIT_NEW 5        /* the new orbit: sincos, exp, log */
+#define XB_PATH_WARP 6             /* warp proper: the warp path, two logs and an exp */
+#define XB_PATH_WARP_EDGE 7        /* activation (RealWarp) or dropping out: the warp path and more */
+
+static inline XB_HD int xb_branch_key(int64_t mode, int64_t effect_stamp, bool live, int new_orbit) {
+    if (!live) return XB_PATH_IDLE;
+    switch (mode) {
+        case XB_MODE_GOTO: return XB_PATH_GOTO;
+        case XB_MODE_FOLLOW: return XB_PATH_FOLLOW;
+        case XB_MODE_ORBIT: return new_orbit ? XB_PATH_ORBIT_NEW : XB_PATH_ORBIT_OLD;
+        case XB_MODE_WARP: return effect_stamp < 0 ? XB_PATH_GOTO : XB_PATH_WARP;
+        default: return XB_PATH_COAST;
+    }
+}
 
 static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     XbShip& s = w.s;
@@ -458,6 +481,7 @@ static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     s.massive = XB_B(P, XB_B_MASSIVE, r) != 0;
     s.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
     s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
+    w.path = XB_PATH_IDLE;
     w.row = r;
 }
 
@@ -543,6 +567,7 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         else rel_after = true;
     }
     if (rel_before) xb_release(s, true);
+    w.path = live ? XB_PATH_COAST : XB_PATH_IDLE;
     if (live) {
         XbV3 a = v3(0.0, 0.0, 0.0);
         int64_t t = s.follow_ptr;
@@ -550,18 +575,24 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         bool target_ok = t >= 0 && tp >= lo && tp < hi;
         if (s.mode == XB_MODE_WARP) {
             if (s.effect_stamp >= 0) {
+                w.path = XB_PATH_WARP;
             } else if (act_self) {
                 xb_real_warp(s);
+                w.path = XB_PATH_WARP_EDGE;
             } else {
                 s.effect_stamp -= 1;
                 a = xb_goto_thrust(s, s.gp, P);
+                w.path = XB_PATH_GOTO;
             }
         } else if (s.mode == XB_MODE_GOTO) {
             a = xb_goto_thrust(s, s.gp, P);
+            w.path = XB_PATH_GOTO;
         } else if ((s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT) && !target_ok) {
         } else if (s.mode == XB_MODE_FOLLOW) {
             a = xb_follow(s, v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]), R.radius[tp], P);
+            w.path = XB_PATH_FOLLOW;
         } else if (s.mode == XB_MODE_ORBIT) {
+            w.path = P.new_orbit ? XB_PATH_ORBIT_NEW : XB_PATH_ORBIT_OLD;
             XbV3 op = v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]);
             if (P.new_orbit) {
                 XbV3 ov = v3(R.v0x[tp], (R.id[tp] < s.id) ? R.vy_after[tp] : R.v0y[tp], R.v0z[tp]);
@@ -587,6 +618,7 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
                 s.pos = p;
                 s.vel = v;
                 xb_integrate(&p, &v, a, m, s.tf, P);
+                w.path = XB_PATH_WARP_EDGE;
             }
         } else {
             xb_integrate(&p, &v, a, m, s.tf, P);

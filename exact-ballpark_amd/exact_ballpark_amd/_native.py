@@ -16,7 +16,7 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 1
+ABI_VERSION = 2
 NF, NI, NB, NS = 46, 10, 5, 7
 
 
@@ -27,7 +27,9 @@ class ModesState(ctypes.Structure):
                 ("rowmap", ctypes.c_void_p), ("n", ctypes.c_int64), ("stride", ctypes.c_int64),
                 ("n_tiles", ctypes.c_int32), ("new_orbit", ctypes.c_int32), ("friction", ctypes.c_double),
                 ("dt", ctypes.c_double), ("ffriction", ctypes.c_float), ("fdt", ctypes.c_float),
-                ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32)]
+                ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32),
+                ("schedule", ctypes.c_int32), ("n_keys", ctypes.c_int32), ("keys", ctypes.c_void_p),
+                ("path", ctypes.c_void_p)]
 
 
 _LIBS: dict = {}

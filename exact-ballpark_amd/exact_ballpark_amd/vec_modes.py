@@ -148,6 +148,9 @@ class VecModesPark(metaclass=_Fields):
         self.old_rot = self.rot.clone()
         self.tiles = None
         self._state = None
+        self.schedule = 0              # lanes of a tile: 0 row order, 1 the built-in branch key, 2 `keys`
+        self.keys = None               # [n] uint8 buckets for schedule 2 (the learned scheduler's)
+        self.path = None               # [n] uint8: each row's executed path class, when recorded
         self._set_tiles()
 
     # ---- the state as the kernel takes it ----
@@ -186,8 +189,19 @@ class VecModesPark(metaclass=_Fields):
             s.friction, s.dt, s.ffriction, s.fdt = self.friction, self.dt, self.ffriction, self.fdt
             s.dynamical, s.ticks = 0, 1
             self._state = s
-        self._state.new_orbit = int(self.use_new_orbit)
-        return self._state
+        s = self._state
+        s.new_orbit = int(self.use_new_orbit)
+        tiled = self.tiles is not None
+        s.schedule = int(self.schedule) if tiled else 0
+        s.n_keys = 16
+        s.keys = self.keys.data_ptr() if (self.keys is not None and s.schedule == 2) else None
+        s.path = self.path.data_ptr() if self.path is not None else None
+        return s
+
+    def record_paths(self, on: bool = True) -> None:
+        """Record each row's executed path class (XB_PATH_*: 0 idle .. 7 warp edge) in `path` at every tick."""
+        self.path = torch.zeros(self.n, dtype=U8, device=self.dev) if on else None
+        self._state = None
 
     def row_tensors(self) -> list:
         """The per-row state (the reference's `row_tensors`, vec_orient.py:814-818): every field of the blocks."""
@@ -440,11 +454,16 @@ class VecModesPark(metaclass=_Fields):
 
     # ---- one tick (Ballpark::Evolve; vec_modes.py:943-960) ----
 
-    def _tick(self, block: int | None = None) -> None:
+    def _tick(self, block: int | None = None, ticks: int = 1) -> None:
         s = self.state()
         if self.n == 0:
             return
         b = self.block if block is None else int(block)
+        if ticks > 1 and self.tiles is None:
+            for _ in range(ticks):
+                self._tick(block)
+            return
+        s.ticks = int(ticks)
         if self.dev.type == "cuda":
             with torch.cuda.device(self.dev):
                 rc = N.hip().xb_modes_tick(ctypes.byref(s), torch.cuda.current_stream(self.dev).cuda_stream, b)
@@ -477,6 +496,12 @@ class VecModesPark(metaclass=_Fields):
         else:
             self._tick()
         self.current_time += 1
+
+    def evolve_ticks(self, ticks: int) -> None:
+        """`ticks` ticks of every world with no command between them: one launch when the park is tiled (each lane
+        keeps its row in registers across the ticks), bit for bit the same as `ticks` calls of evolve()."""
+        self._tick(ticks=int(ticks))
+        self.current_time += int(ticks)
 
     def evolve_torch(self, active: torch.Tensor | None = None) -> None:
         """The reference's CPU tick name: here the same kernel source built for the host (a CPU park's evolve)."""
@@ -530,17 +555,21 @@ class ModesGraph:
     as a device buffer: the blocks saved, the tick, the rows outside the mask put back; `tick(active)` writes the mask
     and replays."""
 
-    def __init__(self, park: VecModesPark, block: int | None = None, lib=None, masked: bool = False):
+    def __init__(self, park: VecModesPark, block: int | None = None, lib=None, masked: bool = False,
+                 ticks: int = 1):
         if park.dev.type != "cuda":
             raise ValueError("a HIP graph needs a park on a GPU")
         if lib is not None:
             raise ValueError("lib: the park binds its own library")
         self.park, self.block = park, park.block if block is None else int(block)
         self.masked = masked
+        self.ticks = int(ticks)
+        if masked and self.ticks != 1:
+            raise ValueError("OnTick's mask applies to one tick at a time")
         self.mask = torch.ones(park.stride, dtype=torch.bool, device=park.dev)
         self.saved = [torch.empty_like(b) for b in (park._f, park._q, park._b)] if masked else None
         before = park._save()                            # warm-up: the module loaded before the capture,
-        park._tick(self.block)                           # the park's state put back after it
+        park._tick(self.block, self.ticks)               # the park's state put back after it
         for b, old in zip((park._f, park._q, park._b), before):
             b.copy_(old)
         torch.cuda.synchronize(park.dev)
@@ -549,7 +578,7 @@ class ModesGraph:
             if masked:
                 for s, b in zip(self.saved, (park._f, park._q, park._b)):
                     s.copy_(b)
-            park._tick(self.block)
+            park._tick(self.block, self.ticks)
             if masked:
                 for s, b in zip(self.saved, (park._f, park._q, park._b)):
                     b.copy_(torch.where(self.mask[None, :], b, s))
@@ -563,4 +592,4 @@ class ModesGraph:
             else:
                 self.mask[:self.park.n].copy_(active.to(device=self.park.dev, dtype=torch.bool).reshape(-1))
         self.graph.replay()
-        self.park.current_time += 1
+        self.park.current_time += self.ticks

@@ -83,12 +83,20 @@ def apply_commands(park, c: dict, mask=None) -> None:
                  torch.as_tensor(c["warp_factor"], dtype=torch.int64).to(dev))
 
 
+def world_rows(wl: dict, wi: int) -> range:
+    """The rows of world wi (world-major layouts: fixed ships per world, or the ragged park's start and counts)."""
+    if wl.get("start") is not None:
+        return range(int(wl["start"][wi]), int(wl["start"][wi] + wl["counts"][wi]))
+    s = wl["ships"]
+    return range(wi * s, wi * s + s)
+
+
 def world_scenario(wl: dict, wi: int, steps: int) -> dict:
     """World wi of the workload as a scenario dict (balls and tick-1 commands), for the oracle."""
-    s = wl["ships"]
     r, c = wl["rows"], wl["cmds"]
+    rows = world_rows(wl, wi)
     balls, cmds = [], []
-    for k in range(wi * s, wi * s + s):
+    for k in rows:
         balls.append({"id": int(r["id"][k]), "mass": float(r["mass"][k]), "radius": float(r["radius"][k]),
                       "max_vel": float(r["max_vel"][k]), "agility": float(r["agility"][k]),
                       "sf": float(r["speed_fraction"][k]), "pos": [float(v) for v in r["pos"][k]],
@@ -97,7 +105,7 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
                       "cloaked": int(r["cloaked"][k]), "bubble": int(r["bubble"][k]),
                       "moribund": bool(r["moribund"][k]), "mode": int(r["mode"][k])})
     for o in (GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP):    # apply_commands' order
-        for k in range(wi * s, wi * s + s):
+        for k in rows:
             if int(c["op"][k]) != o:
                 continue
             p = [float(v) for v in c["point"][k]]

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 1
+#define XB_ABI_VERSION 2
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -33,7 +33,9 @@ extern "C" {
    - general (`tiles` NULL): any row order; three launches through the caller's scratch (XB_NS float64 rows and one
      uint8 row of `stride` each), one tick.
    `ticks` > 1 (tiled only) runs that many ticks in the one launch: each lane keeps its row's state in registers from
-   the first tick to the last, so every field is read from and written to HBM once per launch, not once per tick. */
+   the first tick to the last, so every field is read from and written to HBM once per launch, not once per tick.
+   `schedule` assigns a tile's rows to its lanes so that each wavefront runs rows of one branch class (a row's result
+   never depends on its lane: it reads other rows only through the tile's records). */
 typedef struct xb_modes_state {
     double* f64;
     int64_t* i64;
@@ -52,6 +54,11 @@ typedef struct xb_modes_state {
     float fdt;
     int32_t dynamical;
     int32_t ticks;                 /* ticks per launch (tiled shape only; 0 or 1: one) */
+    int32_t schedule;              /* lanes of a tile: 0 in row order; 1 sorted by the built-in branch key;
+                                      2 sorted by keys[row] (tiled shape only) */
+    int32_t n_keys;                /* the keys' bucket count, at most 16 */
+    const uint8_t* keys;           /* schedule 2: each row's bucket, below n_keys */
+    uint8_t* path;                 /* if not NULL: each row's executed path class in its last tick (0..7) */
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */

@@ -23,3 +23,21 @@ def test_cpu_park_matches_oracle(golden, oracle_states, general):
     got = D.run_park(worlds, "cpu", general=general)
     bad = D.compare(got, oracle_states, worlds)
     assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+
+
+def test_cpu_resident_ticks_equal_single_ticks():
+    import torch
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    wl = WL.flight_park(2, 512)
+    a = VecModesPark(wl["n"], device="cpu")
+    b = VecModesPark(wl["n"], device="cpu")
+    WL.load_flight_park(a, wl)
+    WL.load_flight_park(b, wl)
+    for _ in range(9):
+        a.evolve()
+    b.evolve_ticks(4)
+    b.evolve_ticks(5)
+    for x, y in zip((a._f, a._q, a._b), (b._f, b._q, b._b)):
+        assert torch.equal(x.view(torch.int64) if x.dtype == torch.float64 else x,
+                           y.view(torch.int64) if y.dtype == torch.float64 else y)

@@ -209,3 +209,57 @@ def test_masked_tick_and_graph():
         assert torch.equal(b[:, ~sel], a[:, ~sel]), "a row outside the mask changed"
         assert torch.equal(b[:, sel], f[:, sel]), "a row inside the mask differs from the unmasked tick"
         assert torch.equal(m, b), "the masked graph differs from the masked eager tick"
+
+
+@pytest.mark.parametrize("schedule,ticks", [(1, 1), (2, 1), (0, 5), (1, 7)])
+def test_schedules_and_resident_ticks_equal_plain(schedule, ticks):
+    """Sorting a tile's lanes by branch key (built in, or random keys) and running several ticks per launch (rows held
+    in registers) give the plain launch's bits, at full size."""
+    W, T = 65536, 14
+    plain = _big_park(W)
+    for _ in range(T):
+        plain.evolve()
+    park = _big_park(W)
+    park.schedule = schedule
+    if schedule == 2:
+        g = torch.Generator().manual_seed(5)
+        park.keys = torch.randint(0, 16, (park.n,), generator=g, dtype=torch.uint8).to(DEV)
+    done = 0
+    while done < T:
+        k = min(ticks, T - done)
+        park.evolve_ticks(k)
+        done += k
+    torch.cuda.synchronize()
+    assert _same(_blocks(plain), _blocks(park))
+    assert park.current_time == plain.current_time == T
+
+
+def test_ragged_park_tiled_padded_and_alone_agree():
+    """The ragged park (ships per world geometric on 1..32, mean 6): packed tiles, the padded-to-32 layout, the lane
+    schedule and a world taken alone give the same bits for every real row."""
+    wl = WL.ragged_park(4, 8192)
+    T = 12
+    packed = VecModesPark(wl["n"], device=DEV, block=256)
+    WL.load_flight_park(packed, wl)
+    pad = WL.padded(wl, 32)
+    padded = VecModesPark(pad["n"], device=DEV, block=256)
+    WL.load_flight_park(padded, pad)
+    sched = VecModesPark(wl["n"], device=DEV, block=128)
+    WL.load_flight_park(sched, wl)
+    sched.schedule = 1
+    for p in (packed, padded, sched):
+        assert p.tiles is not None
+        for _ in range(T):
+            p.evolve()
+    torch.cuda.synchronize()
+    a, b, c = _blocks(packed), _blocks(padded), _blocks(sched)
+    dest = torch.as_tensor(pad["dest"], device=DEV)
+    keep_q = [0, 1, 2, 3, 4, 6, 7, 8, 9]
+    for x, y, z in zip(a, b, c):
+        y = y[:, dest]
+        if x.dtype == torch.float64:
+            x, y, z = (t.view(torch.int64) for t in (x, y, z))
+        elif x.dtype == torch.int64:
+            x, y, z = x[keep_q], y[keep_q], z[keep_q]
+        assert torch.equal(x, y), "the padded layout differs from the packed one"
+        assert torch.equal(x, z), "the scheduled lanes differ from the plain ones"
