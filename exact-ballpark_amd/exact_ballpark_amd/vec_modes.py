@@ -147,6 +147,7 @@ class VecModesPark(metaclass=_Fields):
         self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
         self.old_rot = self.rot.clone()
         self.tiles = None
+        self.rowmap = None             # [n] int32: the tiled launch's rows in a regrouped world order (None: row order)
         self._state = None
         self.schedule = 0              # lanes of a tile: 0 row order, 1 the built-in branch key, 2 `keys`
         self.keys = None               # [n] uint8 buckets for schedule 2 (the learned scheduler's)
@@ -157,8 +158,9 @@ class VecModesPark(metaclass=_Fields):
 
     def _set_tiles(self) -> None:
         """Tiles of whole worlds for the tiled launch when every world's rows are contiguous and fit a workgroup;
-        None (the general launch) otherwise."""
+        None (the general launch) otherwise. Restores the row order (rowmap None)."""
         self.tiles = None
+        self.rowmap = None
         self._state = None
         n = self.n
         if n == 0:
@@ -181,10 +183,7 @@ class VecModesPark(metaclass=_Fields):
             s = N.ModesState()
             s.f64, s.i64, s.u8 = self._f.data_ptr(), self._q.data_ptr(), self._b.data_ptr()
             s.scratch_f64, s.scratch_u8 = self._sf.data_ptr(), self._sb.data_ptr()
-            s.tiles = self.tiles.data_ptr() if self.tiles is not None else None
-            s.rowmap = None
             s.n, s.stride = self.n, self.stride
-            s.n_tiles = (len(self.tiles) - 1) if self.tiles is not None else 0
             s.new_orbit = int(self.use_new_orbit)
             s.friction, s.dt, s.ffriction, s.fdt = self.friction, self.dt, self.ffriction, self.fdt
             s.dynamical, s.ticks = 0, 1
@@ -192,6 +191,9 @@ class VecModesPark(metaclass=_Fields):
         s = self._state
         s.new_orbit = int(self.use_new_orbit)
         tiled = self.tiles is not None
+        s.tiles = self.tiles.data_ptr() if tiled else None
+        s.n_tiles = (len(self.tiles) - 1) if tiled else 0
+        s.rowmap = self.rowmap.data_ptr() if (tiled and self.rowmap is not None) else None
         s.schedule = int(self.schedule) if tiled else 0
         s.n_keys = 16
         s.keys = self.keys.data_ptr() if (self.keys is not None and s.schedule == 2) else None

@@ -143,9 +143,20 @@ def truncated_geometric_p(mean: float, top: int) -> float:
     return 0.5 * (lo + hi)
 
 
-def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32) -> dict:
+# World activity profiles of mix="profiles": (weight, probabilities of stop, goto, follow, orbit, warp at tick 1)
+PROFILES = {
+    "quiet": (0.40, (0.90, 0.10, 0.00, 0.00, 0.00)),       # parked or drifting
+    "travel": (0.25, (0.10, 0.50, 0.00, 0.00, 0.40)),      # going places, at sub-light and in warp
+    "combat": (0.20, (0.05, 0.05, 0.45, 0.45, 0.00)),      # following and orbiting each other
+    "mixed": (0.15, (0.20, 0.20, 0.20, 0.20, 0.20)),
+}
+
+
+def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32, mix: str = "uniform") -> dict:
     """The ragged flight park: ships per world drawn from the geometric law on {1..max_ships} with mean `mean_ships`,
-    each ship as flight_park's (modes uniform over stop, goto, follow, orbit, warp at tick 1). Rows world-major."""
+    each ship as flight_park's. mix="uniform": every ship's tick-1 mode uniform over stop, goto, follow, orbit, warp;
+    mix="profiles": each world first draws an activity profile (PROFILES), its ships' modes then that profile's law, so
+    worlds differ in cost from one another as a server's do. Rows world-major."""
     rng = np.random.default_rng(seed)
     p = truncated_geometric_p(mean_ships, max_ships)
     k = np.arange(1, max_ships + 1)
@@ -167,7 +178,15 @@ def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int 
         "mode": np.full(n, 2, dtype=np.int64),
     }
     size = np.repeat(counts, counts)
-    kind = rng.integers(0, 5, n)
+    if mix == "uniform":
+        kind = rng.integers(0, 5, n)
+    elif mix == "profiles":
+        names = list(PROFILES)
+        prof = rng.choice(len(names), size=worlds, p=[PROFILES[k][0] for k in names])
+        cum = np.cumsum(np.array([PROFILES[k][1] for k in names]), axis=1)[np.repeat(prof, counts)]
+        kind = np.minimum((rng.uniform(0.0, 1.0, n)[:, None] >= cum).sum(axis=1), 4)
+    else:
+        raise ValueError(f"mix: 'uniform' or 'profiles', not {mix!r}")
     kind = np.where((size == 1) & ((kind == 2) | (kind == 3)), 0, kind)   # a lone ship has nothing to follow
     op = np.select([kind == 1, kind == 2, kind == 3, kind == 4], [GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP], 0)
     point = _ball_points(rng, n, 1e10)
@@ -177,7 +196,8 @@ def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int 
     cmds = {"op": op.astype(np.int64), "point": point, "target": target.astype(np.int64),
             "range": rng.uniform(500.0, 20000.0, n), "min_range": np.full(n, 20000.0),
             "warp_factor": np.full(n, 20, dtype=np.int64)}
-    return {"n": n, "worlds": worlds, "ships": None, "counts": counts, "start": start, "rows": rows, "cmds": cmds}
+    return {"n": n, "worlds": worlds, "ships": None, "counts": counts, "start": start, "rows": rows, "cmds": cmds,
+            "mix": mix}
 
 
 def padded(wl: dict, width: int) -> dict:

@@ -87,6 +87,39 @@ int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* strea
    (src/exact_ballpark/vec_modes_cuda.cu:766). */
 int xb_modes_info(int which, int64_t* out);
 
+/* ---- the learned scheduler (no counterpart in the reference: it orders work, never changes a result) ---- */
+
+#define XB_MLP_IN 12
+#define XB_MLP_HIDDEN 64
+#define XB_MLP_OUT 8
+#define XB_MLP_PARAMS (XB_MLP_IN * XB_MLP_HIDDEN + XB_MLP_HIDDEN + XB_MLP_HIDDEN * XB_MLP_HIDDEN + XB_MLP_HIDDEN + \
+                       XB_MLP_HIDDEN * XB_MLP_OUT + XB_MLP_OUT)
+#define XB_MLP_CHUNKS 64 /* the backward pass sums each gradient over this many runs of samples, then the runs in order */
+
+/* Each world's 12 features (float32 [n_worlds][12]): its mode histogram (stop and inert, goto, follow, orbit, warp; as
+   fractions of its ships), ships / 32, aligning warpers, idle rows, log10(1 + the nearest ship's distance to the
+   centroid) / 10, log10(1 + the mean distance) / 10, the divergence cost of its last tick (the summed weights of the
+   distinct path classes its rows executed) / 38 and its rows' summed path weights per ship / 9.
+   World w is the rows world_start[w] .. world_start[w] + world_count[w] - 1; `path` may be NULL. */
+int xb_world_features(const xb_modes_state* state, const int32_t* world_start, const int32_t* world_count,
+                      int32_t n_worlds, const uint8_t* path, float* features, void* stream);
+
+/* The network 12 -> 64 -> 64 -> 8 (ReLU, ReLU, logits), float32. params: W1[12][64], b1[64], W2[64][64], b2[64],
+   W3[64][8], b3[8] (XB_MLP_PARAMS). h1, h2 ([n][64]) keep the hidden activations for the backward pass (NULL:
+   not kept). */
+int xb_mlp_forward(const float* params, const float* x, int32_t n, float* h1, float* h2, float* logits, void* stream);
+
+/* The mean softmax cross-entropy over n samples against labels (0..7): per-sample loss[n] (each over n), the
+   per-sample deltas dl[n][8], d2[n][64], d1[n][64] (scratch), and grads[XB_MLP_PARAMS]. Each gradient is summed over
+   XB_MLP_CHUNKS consecutive runs of samples in sample order into work[XB_MLP_CHUNKS][XB_MLP_PARAMS] (scratch), then
+   over the runs in order: the result depends on n alone, never on timing. */
+int xb_mlp_backward(const float* params, const float* x, const float* h1, const float* h2, const float* logits,
+                    const int32_t* labels, int32_t n, float* dl, float* d2, float* d1, float* loss, float* work,
+                    float* grads, void* stream);
+
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) of learning rate lr at step >= 1, moments m and v in place. */
+int xb_mlp_adam(float* params, const float* grads, float* m, float* v, int32_t step, float lr, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
