@@ -3,11 +3,15 @@
 
 Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
 - ragged (the primary line): the ragged flight-modes park, 65,536 worlds with ships per world geometric on 1..32
-  (mean 6), modes uniform over stop, goto, follow, orbit and warp given at tick 1. Measured three ways: the plain
-  padded-to-32 launch (every world padded to 32 rows, lanes in row order, one tick a launch), and the autotuned packed
-  launch (whole worlds packed into tiles, workgroup size, lane schedule and ticks per launch chosen by the autotuner
-  on the park's own state). `value` is the autotuned form.
-- flight: 65,536 worlds x 4 ships of the same law.
+  (mean 6), modes uniform over stop, goto, follow, orbit and warp given at tick 1.
+- profiles: the same ragged park with world activity profiles (quiet, travelling, in combat, mixed), the worlds
+  differing in cost from one another as a server's do: the learned scheduler's workload.
+- flight: 65,536 worlds x 4 ships, modes uniform.
+Each ragged workload is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
+row order, one tick a launch) and as the autotuned packed launch: whole worlds packed into tiles, with workgroup size,
+lane schedule, ticks per launch and world order (the row order, or the learned scheduler's regrouping, trained on the
+GPU on the park's own recorded ticks) chosen by the autotuner on the park's own state. `value` is the autotuned form;
+`learned_scheduler` gives the best shape in each world order, each measured as the primary line is.
 A step is one tick of every world; the timed ticks run as HIP graph replays with the state resident in HBM. Each
 kernel's HBM roofline fraction uses algorithmic bytes: 361 B a ship-tick (each persistent per-ship field the tick reads
 or writes counted once at destiny's sizes); `achieved` is those bytes over the measured time per tick. With several
@@ -35,6 +39,14 @@ METRIC = "bit-exact world-ticks/s on 1 MI355X (ragged composed park); p99 tick l
 BYTES_PER_SHIP_TICK = 361          # 164 read + 197 written, flight-modes park without dynamical orientation
 HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
 TRAFFIC_FILE = ROOT / "profiles" / "modes_tick_traffic.json"
+WORKLOADS = {
+    "ragged": "ragged flight-modes park: {worlds} worlds, ships per world geometric on 1..32 (mean 6), modes uniform over "
+              "stop/goto/follow/orbit/warp; no dynamical orientation, missiles or collisions",
+    "profiles": "ragged flight-modes park with world activity profiles: {worlds} worlds, ships per world geometric on "
+                "1..32 (mean 6), each world quiet, travelling, in combat or mixed (its ships' modes drawn from that "
+                "profile's law); no dynamical orientation, missiles or collisions",
+    "flight": "flight-modes park: {worlds} worlds x 4 ships, modes uniform over stop/goto/follow/orbit/warp",
+}
 
 
 def _measure(park, ticks_per_launch: int, steps: int, warmup: int) -> dict:
@@ -90,8 +102,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--workload", choices=("ragged", "flight"), default="ragged",
-                    help="the primary line's workload (the other is measured beside it unless --only)")
+    ap.add_argument("--workload", choices=("ragged", "profiles", "flight"), default="ragged",
+                    help="the primary line's workload (the others are measured beside it unless --only)")
     ap.add_argument("--only", action="store_true", help="measure the primary workload only")
     ap.add_argument("--block", type=int, default=0, help="fix the workgroup size (0: autotune)")
     ap.add_argument("--schedule", type=int, default=-1, help="fix the lane schedule (-1: autotune)")
@@ -99,6 +111,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="leave the learned scheduler's order out of the tuning")
     ap.add_argument("--no-single", action="store_true", help="skip the single-tick measurement (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -106,43 +119,64 @@ def main() -> int:
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world_size != 1:
         print(f"bench.py runs on one GPU only (asked for {max(a.gpus, world_size)}): the worlds are independent and "
-              "this build measures one GPU's share; DESIGN.md says how the park would shard", file=sys.stderr)
+              "this benchmark measures one GPU's share (worlds shard by index across GPUs with no communication)",
+              file=sys.stderr)
         return 2
 
     import torch
 
     from exact_ballpark_amd import _native as N
     from exact_ballpark_amd import autotune as AT
+    from exact_ballpark_amd import scheduler as S
     from exact_ballpark_amd import workloads as WL
     from exact_ballpark_amd.vec_modes import VecModesPark
 
     torch.cuda.set_device(0)
 
     def build(kind: str):
-        wl = WL.ragged_park(1, a.worlds) if kind == "ragged" else WL.flight_park(1, a.worlds, 4)
+        if kind == "flight":
+            wl = WL.flight_park(1, a.worlds, 4)
+        else:
+            wl = WL.ragged_park(1, a.worlds, mix="profiles" if kind == "profiles" else "uniform")
         park = VecModesPark(wl["n"], device="cuda", block=256)
         WL.load_flight_park(park, wl)
         return wl, park
 
     def tuned(kind: str) -> dict:
         wl, park = build(kind)
+        park.record_paths(True)                        # the counters the learned scheduler trains and predicts on
         for _ in range(4):
             park.evolve()
+        done = 4
+        orders, training = {}, None
+        if not a.no_learned and kind != "flight":
+            t0 = time.perf_counter()
+            L = S.LearnedScheduler(park, seed=0)
+            training = L.train(ticks=8, epochs=20)
+            torch.cuda.synchronize()
+            training["seconds"] = time.perf_counter() - t0
+            done += 8
+            orders["learned"] = L.regroup
         fixed = a.block and a.schedule >= 0 and a.ticks_per_launch
         if fixed:
             park.block, park.schedule = a.block, a.schedule
             park._set_tiles()
-            shape = {"block": a.block, "schedule": a.schedule, "ticks": a.ticks_per_launch, "autotuned": False}
+            shape = {"block": a.block, "schedule": a.schedule, "ticks": a.ticks_per_launch, "order": "rows",
+                     "autotuned": False}
         else:
             r = AT.tune(park, blocks=(a.block,) if a.block else (64, 128, 256),
                         schedules=(a.schedule,) if a.schedule >= 0 else (0, 1),
-                        ticks=(a.ticks_per_launch,) if a.ticks_per_launch else (1, 4, 8, 16))
-            shape = {"block": r["block"], "schedule": r["schedule"], "ticks": r["ticks"], "autotuned": True,
-                     "trials_ms_per_tick": r["trials"]}
+                        ticks=(a.ticks_per_launch,) if a.ticks_per_launch else (1, 4, 8, 16), orders=orders)
+            shape = {"block": r["block"], "schedule": r["schedule"], "ticks": r["ticks"], "order": r["order"],
+                     "autotuned": True, "trials_ms_per_tick": r["trials"]}
         print(f"# {kind}: {wl['n']} ships in {a.worlds} worlds, shape {shape['block']}/{shape['schedule']}/"
-              f"{shape['ticks']}", file=sys.stderr, flush=True)
+              f"{shape['ticks']}/{shape['order']}", file=sys.stderr, flush=True)
         m = _measure(park, shape["ticks"], a.steps, a.warmup)
-        one = _measure(park, 1, max(32, a.steps // 4), 4) if (shape["ticks"] > 1 and not a.no_single) else m
+        done += m["ticks_run"]
+        one = m
+        if shape["ticks"] > 1 and not a.no_single:
+            one = _measure(park, 1, max(32, a.steps // 4), 4)
+            done += one["ticks_run"]
         n = wl["n"]
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
@@ -152,7 +186,37 @@ def main() -> int:
                "roofline": _roofline(n, m["ms_per_tick"], _traffic(f"{kind}_t{shape['ticks']}")),
                "single_tick": {"ms_per_tick": one["ms_per_tick"],
                                "roofline": _roofline(n, one["ms_per_tick"], _traffic(f"{kind}_t1"))}}
-        res["_ticks_done"] = 4 + m["ticks_run"] + (one["ticks_run"] if one is not m else 0)
+        if orders and not fixed:
+            # the best shape of each world order, each measured as the primary was
+            best = {}
+            for key, ms in r["trials"].items():
+                b, sc, t, o = key.split("_", 3)
+                if o not in best or ms < best[o][0]:
+                    best[o] = (ms, int(b[1:]), int(sc[1:]), int(t[1:]))
+            cmp = {}
+            for o, (_, b, sc, t) in best.items():
+                park.block, park.schedule = b, sc
+                park._set_tiles()
+                if orders.get(o) is not None:
+                    orders[o]()
+                mo = _measure(park, t, a.steps, a.warmup)
+                done += mo["ticks_run"]
+                cmp[o] = {"shape": f"{b}/{sc}/{t}", "ms_per_tick": mo["ms_per_tick"],
+                          "world_ticks_per_s": a.worlds / (mo["ms_per_tick"] / 1000.0)}
+            res["learned_scheduler"] = {"training": training, "orders": cmp,
+                                        "gain_over_rows": cmp["rows"]["ms_per_tick"] / cmp["learned"]["ms_per_tick"]}
+        if not a.no_padded and kind != "flight":
+            pad = WL.padded(wl, 32)
+            pp = VecModesPark(pad["n"], device="cuda", block=256)
+            WL.load_flight_park(pp, pad)
+            mp = _measure(pp, 1, max(32, a.steps // 4), 4)
+            res["padded_baseline"] = {"layout": "every world padded to 32 rows, lanes in row order, worlds in row "
+                                                "order, one tick a launch",
+                                      "world_ticks_per_s": a.worlds / (mp["ms_per_tick"] / 1000.0),
+                                      "ms_per_tick": mp["ms_per_tick"],
+                                      "speedup": mp["ms_per_tick"] / m["ms_per_tick"]}
+            del pp
+        res["_ticks_done"] = done
         res["_park"], res["_wl"] = park, wl
         return res
 
@@ -165,26 +229,18 @@ def main() -> int:
         print(f"# copy probe: {a.copy_probe} MiB x 8", file=sys.stderr, flush=True)
     primary = tuned(a.workload)
     out_extra = {}
-    if not a.no_padded and a.workload == "ragged":
-        wl = primary["_wl"]
-        pad = WL.padded(wl, 32)
-        pp = VecModesPark(pad["n"], device="cuda", block=256)
-        WL.load_flight_park(pp, pad)
-        m = _measure(pp, 1, max(32, a.steps // 4), 4)
-        out_extra["padded_baseline"] = {"layout": "every world padded to 32 rows, lanes in row order, one tick a launch",
-                                        "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
-                                        "ms_per_tick": m["ms_per_tick"]}
     if not a.only:
-        other = tuned("flight" if a.workload == "ragged" else "ragged")
-        out_extra[other["workload"] + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}
+        for kind in ("ragged", "profiles", "flight"):
+            if kind != a.workload:
+                other = tuned(kind)
+                out_extra[kind + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}
+                del other
     p = primary
     out = {
         "metric": METRIC, "value": p["world_ticks_per_s"], "unit": "world-ticks/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": p["ms_per_tick"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": ("ragged flight-modes park: 65,536 worlds, ships per world geometric on 1..32 (mean 6)"
-                                if p["workload"] == "ragged" else "flight-modes park: 65,536 worlds x 4 ships")
-                   + ", modes uniform over stop/goto/follow/orbit/warp, no dynamical orientation, no collisions",
+        "config": {"workload": WORKLOADS[p["workload"]].format(worlds=a.worlds),
                    "worlds": a.worlds, "ships": p["ships"], "seed": 1, "shape": p["shape"]},
         "ship_ticks_per_s": p["ship_ticks_per_s"],
         "latency_ms": {"tick_p50": p["tick_latency_ms"]["p50"], "tick_p99": p["tick_latency_ms"]["p99"],
@@ -192,6 +248,9 @@ def main() -> int:
         "roofline": p["roofline"], "single_tick": p["single_tick"],
         "kernel": N.kernel_info(0),
     }
+    for key in ("learned_scheduler", "padded_baseline"):
+        if key in p:
+            out[key] = p[key]
     out.update(out_extra)
     if not a.no_cpu:
         out["cpu_baseline"], out["exact"] = cpu_baseline(p["_park"], p["_wl"], p["_ticks_done"], a.cpu_seconds)
