@@ -112,6 +112,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="leave the learned scheduler's order out of the tuning")
+    ap.add_argument("--episodes", type=int, default=200000, help="the serving line's episodes (0: no serving line)")
+    ap.add_argument("--arrival-rate", type=float, default=5000.0, help="episodes arriving per serving step (1 s)")
+    ap.add_argument("--serve-check", type=int, default=48, help="finished episodes checked against the oracle")
     ap.add_argument("--no-single", action="store_true", help="skip the single-tick measurement (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -171,12 +174,20 @@ def main() -> int:
                      "autotuned": True, "trials_ms_per_tick": r["trials"]}
         print(f"# {kind}: {wl['n']} ships in {a.worlds} worlds, shape {shape['block']}/{shape['schedule']}/"
               f"{shape['ticks']}/{shape['order']}", file=sys.stderr, flush=True)
-        m = _measure(park, shape["ticks"], a.steps, a.warmup)
-        done += m["ticks_run"]
+        snap, at_snap = park._save(), done          # every measurement below starts from this state
+        last = [0]
+
+        def measure(ticks_per_launch, steps, warmup):
+            for blk, old in zip((park._f, park._q, park._b), snap):
+                blk.copy_(old)
+            r = _measure(park, ticks_per_launch, steps, warmup)
+            last[0] = r["ticks_run"]
+            return r
+
+        m = measure(shape["ticks"], a.steps, a.warmup)
         one = m
         if shape["ticks"] > 1 and not a.no_single:
-            one = _measure(park, 1, max(32, a.steps // 4), 4)
-            done += one["ticks_run"]
+            one = measure(1, max(32, a.steps // 4), 4)
         n = wl["n"]
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
@@ -195,12 +206,13 @@ def main() -> int:
                     best[o] = (ms, int(b[1:]), int(sc[1:]), int(t[1:]))
             cmp = {}
             for o, (_, b, sc, t) in best.items():
+                for blk, old in zip((park._f, park._q, park._b), snap):
+                    blk.copy_(old)                  # the order predicted from the state the timing starts at
                 park.block, park.schedule = b, sc
                 park._set_tiles()
                 if orders.get(o) is not None:
                     orders[o]()
-                mo = _measure(park, t, a.steps, a.warmup)
-                done += mo["ticks_run"]
+                mo = measure(t, a.steps, a.warmup)
                 cmp[o] = {"shape": f"{b}/{sc}/{t}", "ms_per_tick": mo["ms_per_tick"],
                           "world_ticks_per_s": a.worlds / (mo["ms_per_tick"] / 1000.0)}
             res["learned_scheduler"] = {"training": training, "orders": cmp,
@@ -216,7 +228,7 @@ def main() -> int:
                                       "ms_per_tick": mp["ms_per_tick"],
                                       "speedup": mp["ms_per_tick"] / m["ms_per_tick"]}
             del pp
-        res["_ticks_done"] = done
+        res["_ticks_done"] = at_snap + last[0]
         res["_park"], res["_wl"] = park, wl
         return res
 
@@ -252,10 +264,80 @@ def main() -> int:
         if key in p:
             out[key] = p[key]
     out.update(out_extra)
+    srv = None
+    if a.episodes:
+        out["serving"], srv = serving(a)
     if not a.no_cpu:
         out["cpu_baseline"], out["exact"] = cpu_baseline(p["_park"], p["_wl"], p["_ticks_done"], a.cpu_seconds)
+        if srv is not None:
+            out["serving"]["cpu_baseline"], out["serving"]["exact"] = cpu_baseline_serving(srv)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serving(a) -> tuple:
+    """The rollout-serving line: a Poisson stream of episodes (2..64 ships, 50..2,000 ticks, a seeded policy's command
+    per ship every 1..10 ticks) admitted under the HBM budget into one resident park, one serving step a simulated
+    second (1 s ticks), continuous batching to the last episode's end."""
+    import numpy as np
+    import torch
+
+    from exact_ballpark_amd import serve as SV
+
+    E = a.episodes
+    seeds = np.arange(1, E + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(12345)
+    arrivals = SV.poisson_arrivals(5, E, a.arrival_rate)
+    ships, length = SV.episode_shape(seeds)
+    runs = np.array([SV.RowAllocator.run_size(x) for x in range(65)])[ships]
+    keep = seeds[np.linspace(0, E - 1, min(a.serve_check, E)).astype(int)]
+    warm = SV.RolloutServer(1024, device="cuda")                    # modules loaded before the timed stream
+    SV.serve(warm, seeds[:4], np.zeros(4, dtype=np.int64), max_steps=3)
+    del warm
+    srv = SV.RolloutServer(int(runs.sum()), budget_bytes=200e9, device="cuda", keep=[int(x) for x in keep])
+    peak = {"live": 0, "rows": 0}
+
+    def on_step(info):
+        peak["live"] = max(peak["live"], info["live"])
+        peak["rows"] = max(peak["rows"], srv.rows.allocated)
+        if srv.step_no % 250 == 0:
+            print(f"# serving step {srv.step_no}: {info}", file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    st = SV.serve(srv, seeds, arrivals, on_step=on_step)
+    res = np.array([srv.results[int(x)] for x in seeds], dtype=object)
+    wait = np.array([r[2] - r[1] for r in res], dtype=np.int64)
+    ship_ticks = int((ships * length).sum())
+    out = {"workload": f"{E} episodes, Poisson arrivals at {a.arrival_rate:g} a step (1 s simulated), 2..64 ships, "
+                       "50..2,000 ticks, a seeded policy's command per ship every 1..10 ticks; HBM budget 200 GB",
+           "episodes_done": st["episodes_done"], "steps": st["steps"], "wall_s": st["wall_s"],
+           "world_ticks_per_s": st["world_ticks_per_s"], "ship_ticks_per_s": ship_ticks / st["wall_s"],
+           "step_latency_ms": st["step_ms"], "peak_live_episodes": peak["live"],
+           "peak_rows_allocated": peak["rows"], "peak_hbm_bytes": peak["rows"] * srv.bytes_per_row,
+           "queue_wait_steps": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99)),
+                                "max": int(wait.max())}}
+    print(f"# serving: {st}", file=sys.stderr, flush=True)
+    return out, srv
+
+
+def cpu_baseline_serving(srv) -> tuple:
+    """The oracle on the kept episodes (their policy replayed on the host), timed on one core; each final state
+    compared with the server's bit for bit."""
+    import serve_check as C
+
+    import numpy as np
+
+    from exact_ballpark_amd import serve as SV
+
+    bad, timing = 0, []
+    for seed, kept in srv.kept.items():
+        bad += not C.same(C.kept_as_oracle(kept), C.oracle_final(seed, timing))
+    dt = sum(timing)
+    ks = np.array(list(srv.kept), dtype=np.uint64)
+    wt = int(SV.episode_shape(ks)[1].sum())
+    cpu = {"value": wt / dt, "unit": "world-ticks/s", "cores": 1, "kind": "port",
+           "sample": f"{len(ks)} finished episodes (evenly spaced in the stream), each replayed alone in the oracle "
+                     "with its policy's commands, 1 thread"}
+    return cpu, {"episodes_checked": len(ks), "episodes_differing": bad, "bit_exact": bad == 0}
 
 
 def cpu_baseline(park, wl: dict, ticks: int, seconds: float) -> tuple:

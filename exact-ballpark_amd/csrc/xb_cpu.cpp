@@ -12,6 +12,7 @@
 #define XB_HD
 #include "xb_modes_tick.h"
 #include "xb_maths_float.h"
+#include "xb_commands.h"
 #include "../../include/xb.h"
 
 namespace {
@@ -99,6 +100,40 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
         for (int g = 0; g < nt; ++g)
             run_group(P, *state, S, rows, state->tiles[g], (int64_t)state->tiles[g + 1] - state->tiles[g]);
     }
+    return 0;
+}
+
+int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
+                          const double* point, const int64_t* target, const double* range, const double* min_range,
+                          const int64_t* warp_factor) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || state->n < 0 || state->stride < state->n || n_rows < 0 ||
+        !op || !point || !target || !range || !min_range || !warp_factor || (!rows && n_rows > state->n))
+        return -1;
+    const XbPark P = host_park(*state);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r >= 0 && r < state->n) xb_cmd_apply(P, state->n, r, op, point, target, range, min_range, warp_factor);
+    }
+    return 0;
+}
+
+int xb_cpu_pack_tiles(const int32_t* sizes, int64_t n, int32_t block, int32_t* tiles, int64_t* n_tiles) {
+    if (!sizes || !tiles || !n_tiles || n < 0 || block <= 0) return -1;
+    int64_t k = 0, acc = 0, row = 0;
+    tiles[0] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t sz = sizes[i];
+        if (sz < 0 || sz > block) return -1;
+        if (acc + sz > block) {
+            tiles[++k] = (int32_t)row;
+            acc = 0;
+        }
+        acc += sz;
+        row += sz;
+    }
+    if (row > INT32_MAX) return -1;
+    if (n > 0) tiles[++k] = (int32_t)row;
+    *n_tiles = k;
     return 0;
 }
 

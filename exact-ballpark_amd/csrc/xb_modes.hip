@@ -18,6 +18,7 @@
 #define XB_TABLE __constant__
 #include "xb_modes_tick.h"
 #include "xb_maths_float.h"
+#include "xb_commands.h"
 #include "../../include/xb.h"
 
 namespace {
@@ -171,6 +172,18 @@ __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     if (s.path) s.path[r] = w.path;
 }
 
+// One lane per listed row (rows NULL: rows 0 .. n_rows - 1): the row's command of the batch, if any.
+__global__ void xb_commands_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows,
+                                   const uint8_t* __restrict__ op, const double* __restrict__ point,
+                                   const int64_t* __restrict__ target, const double* __restrict__ range,
+                                   const double* __restrict__ min_range, const int64_t* __restrict__ warp_factor) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    xb_cmd_apply(xb_park(s), s.n, r, op, point, target, range, min_range, warp_factor);
+}
+
 __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ x, double* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -246,6 +259,18 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     rc = xb_launch_rc();
     if (rc) return rc;
     xb_modes_step_c<<<grid, block, 0, st>>>(*state);
+    return xb_launch_rc();
+}
+
+int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
+                      const double* point, const int64_t* target, const double* range, const double* min_range,
+                      const int64_t* warp_factor, void* stream) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || state->n < 0 || state->stride < state->n || n_rows < 0 ||
+        !op || !point || !target || !range || !min_range || !warp_factor || (!rows && n_rows > state->n))
+        return -1;
+    if (n_rows == 0) return 0;
+    xb_commands_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        *state, rows, n_rows, op, point, target, range, min_range, warp_factor);
     return xb_launch_rc();
 }
 

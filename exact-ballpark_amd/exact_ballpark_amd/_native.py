@@ -20,6 +20,12 @@ ABI_VERSION = 2
 NF, NI, NB, NS = 46, 10, 5, 7
 
 
+class EpisodeRows(ctypes.Structure):
+    """include/xb.h's xb_episode_rows: the serving loop's per-row episode arrays."""
+    _fields_ = [("seed", ctypes.c_void_p), ("ship", ctypes.c_void_p), ("size", ctypes.c_void_p),
+                ("base", ctypes.c_void_p), ("admit", ctypes.c_void_p), ("next_cmd", ctypes.c_void_p)]
+
+
 class ModesState(ctypes.Structure):
     """include/xb.h's xb_modes_state."""
     _fields_ = [("f64", ctypes.c_void_p), ("i64", ctypes.c_void_p), ("u8", ctypes.c_void_p),
@@ -58,6 +64,8 @@ def hip() -> ctypes.CDLL:
         L.xb_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_int]
         L.xb_modes_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.xb_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
+            [ctypes.c_void_p] * 7
         _check_layout(L, "libxb_hip.so")
         _LIBS["hip"] = L
     return _LIBS["hip"]
@@ -71,6 +79,10 @@ def cpu() -> ctypes.CDLL:
             raise NativeError("libxb_cpu.so: ABI version mismatch")
         L.xb_cpu_modes_tick.argtypes = [ctypes.POINTER(ModesState), ctypes.c_int]
         L.xb_cpu_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_cpu_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
+            [ctypes.c_void_p] * 6
+        L.xb_cpu_pack_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_int64)]
         _LIBS["cpu"] = L
     return _LIBS["cpu"]
 

@@ -364,6 +364,33 @@ class VecModesPark(metaclass=_Fields):
         self.owner_id = torch.where(ok, wf, self.owner_id)
         self.mode = torch.where(ok, WARP, self.mode)
 
+    def commands(self, op: torch.Tensor, point: torch.Tensor, target: torch.Tensor, range_: torch.Tensor,
+                 min_range: torch.Tensor, warp_factor: torch.Tensor, rows: torch.Tensor | None = None) -> None:
+        """At most one command per row in one launch (xb_modes_commands): op [n] uint8 (0 none, 3 stop, 1 goto_point,
+        4 follow, 5 orbit, 6 warp_to) with point [n, 3], target [n] (a row), range_ [n], min_range [n] and
+        warp_factor [n], each row's own arguments. `rows` (int32): only those rows (others' ops are not read). The
+        result is that of the masked commands above applied op by op (stop, goto_point, follow, orbit, warp_to): a
+        command writes only its own row and reads of its target only what no command writes."""
+        n, d = self.n, self.dev
+        args = [op.to(device=d, dtype=U8).contiguous(), point.to(device=d, dtype=F64).contiguous(),
+                target.to(device=d, dtype=I64).contiguous(), range_.to(device=d, dtype=F64).contiguous(),
+                min_range.to(device=d, dtype=F64).contiguous(), warp_factor.to(device=d, dtype=I64).contiguous()]
+        if any(a.shape[0] != n for a in args) or args[1].shape != (n, 3):
+            raise ValueError("commands: every argument has one entry (point: one [3] row) per park row")
+        if rows is not None:
+            rows = rows.to(device=d, dtype=torch.int32).contiguous()
+        nr = n if rows is None else rows.numel()
+        ptrs = [a.data_ptr() for a in args]
+        s = self.state()
+        if d.type == "cuda":
+            with torch.cuda.device(d):
+                rc = N.hip().xb_modes_commands(ctypes.byref(s), None if rows is None else rows.data_ptr(), nr, *ptrs,
+                                               torch.cuda.current_stream(d).cuda_stream)
+            N.check(rc, "xb_modes_commands")
+        else:
+            rc = N.cpu().xb_cpu_modes_commands(ctypes.byref(s), None if rows is None else rows.data_ptr(), nr, *ptrs)
+            N.check(rc, "xb_cpu_modes_commands")
+
     def _release_followers(self, warping: torch.Tensor) -> None:
         f = self._members(warping)
         orbiter = f & self.interactive & (self.mode == ORBIT)

@@ -82,6 +82,16 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
    (src/exact_ballpark/vec_modes_cuda.cu:779). */
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block);
 
+/* One flight-mode command per listed row, applied in place between ticks: for each row r of rows[0 .. n_rows - 1]
+   (rows NULL: rows 0 .. n_rows - 1), op[r] 0 none, 3 Stop, 1 GotoPoint(point[r][3]), 4 Follow and 5 Orbit (target row
+   target[r], range[r]), 6 WarpTo(point[r], min_range[r], warp_factor[r]); a row listed twice is a race. The result is
+   the park's op-by-op commands' (a command writes only its own row). Replaces the flight-mode commands of the
+   server's pre-tick kernel (cmd_stop, cmd_goto_point, cmd_follow, cmd_warp_to;
+   src/exact_ballpark/vec_server_cuda.cu:117-186, applied by `apply` at :317). */
+int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
+                      const double* point, const int64_t* target, const double* range, const double* min_range,
+                      const int64_t* warp_factor, void* stream);
+
 /* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass; out[0..3] =
    scalar registers, vector registers, private (scratch) bytes per lane, LDS bytes. Replaces xbpm_info
    (src/exact_ballpark/vec_modes_cuda.cu:766). */
@@ -119,6 +129,50 @@ int xb_mlp_backward(const float* params, const float* x, const float* h1, const 
 
 /* One Adam step (beta 0.9, 0.999, eps 1e-8) of learning rate lr at step >= 1, moments m and v in place. */
 int xb_mlp_adam(float* params, const float* grads, float* m, float* v, int32_t step, float lr, void* stream);
+
+/* ---- the rollout-serving loop (no counterpart in the reference: episodes admitted into a resident park) ---- */
+
+/* Per row of the resident park (capacity rows each): its episode's seed, the row's ship index in the episode, the
+   episode's ship count and first row, the serving step the episode was admitted at, and the episode tick of the row's
+   next command (updated by xb_serve_policy). */
+typedef struct xb_episode_rows {
+    const uint64_t* seed;
+    const int32_t* ship;
+    const int32_t* size;
+    const int64_t* base;
+    const int64_t* admit;
+    int64_t* next_cmd;
+} xb_episode_rows;
+
+/* The seeded random policy for serving step `step`, for the rows live[0 .. n_live - 1]: a row whose episode tick
+   (step - admit + 1) is its next command tick gets op[row] = the command (3 stop, 1 goto, 4 follow, 5 orbit, 6 warp)
+   with point[row][3], target[row] (a row of another ship of its episode) and range[row], and its next command tick
+   moves on by 1 + (a draw mod 10); every other listed row gets op[row] = 0. Each draw is the counter-based hash of
+   exact-ballpark_amd/csrc/xb_hash.h on (episode seed, field, ship, episode tick). */
+int xb_serve_policy(const xb_episode_rows* rows, const int32_t* live, int64_t n_live, int64_t step, uint8_t* op,
+                    double* point, int64_t* target, double* range, void* stream);
+
+/* ---- libxb_cpu.so: the host build of the tick and the host runtime's helpers ---- */
+
+/* The ABI version of the host library (XB_ABI_VERSION). */
+int xb_cpu_abi(void);
+
+/* xb_modes_tick on a park in host memory (OpenMP over the tiles; threads <= 0: OpenMP's default). The CPU batched
+   path, in place of the reference's torch tick on the CPU (src/exact_ballpark/vec_modes.py:943-960). */
+int xb_cpu_modes_tick(const xb_modes_state* state, int threads);
+
+/* xb_modes_commands on a park in host memory (no stream). */
+int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
+                          const double* point, const int64_t* target, const double* range, const double* min_range,
+                          const int64_t* warp_factor);
+
+/* xb_maths_eval's functions on the host (same codes, no stream). */
+int xb_cpu_maths_eval(int func, int64_t n, const double* x, double* out);
+
+/* Greedy packing of worlds in the given order into tiles of at most `block` rows (a world never split):
+   tiles[0] = 0, tiles[k + 1] = the first row after tile k; *n_tiles = the tile count. tiles must hold n + 1 entries.
+   -1 if a world is wider than the block. */
+int xb_cpu_pack_tiles(const int32_t* sizes, int64_t n, int32_t block, int32_t* tiles, int64_t* n_tiles);
 
 #ifdef __cplusplus
 }

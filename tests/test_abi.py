@@ -34,8 +34,21 @@ def test_hip_library_exports_every_declared_symbol():
 
 def test_cpu_library_exports_its_symbols():
     lib = N.cpu()
-    for name in ("xb_cpu_abi", "xb_cpu_modes_tick", "xb_cpu_maths_eval"):
-        assert hasattr(lib, name)
+    names = [n for n in declared() if n.startswith("xb_cpu_")]
+    assert {"xb_cpu_abi", "xb_cpu_modes_tick", "xb_cpu_maths_eval", "xb_cpu_pack_tiles"} <= set(names)
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_pack_tiles():
+    import numpy as np
+    sizes = np.array([3, 5, 2, 6, 1, 4], dtype=np.int32)
+    tiles = np.zeros(len(sizes) + 1, dtype=np.int32)
+    nt = ctypes.c_int64()
+    rc = N.cpu().xb_cpu_pack_tiles(sizes.ctypes.data, len(sizes), 8, tiles.ctypes.data, ctypes.byref(nt))
+    assert rc == 0 and nt.value == 3
+    assert tiles[:4].tolist() == [0, 8, 16, 21]              # [3 5] [2 6] [1 4]
+    assert N.cpu().xb_cpu_pack_tiles(sizes.ctypes.data, len(sizes), 4, tiles.ctypes.data, ctypes.byref(nt)) == -1
 
 
 def test_layout_matches_loader():
