@@ -50,15 +50,18 @@ WORKLOADS = {
 
 
 def _measure(park, ticks_per_launch: int, steps: int, warmup: int) -> dict:
-    """Times `steps` ticks (graph replays of `ticks_per_launch` ticks each) and per-launch latencies."""
+    """Times `steps` ticks as graph replays of several launches of `ticks_per_launch` ticks each (about 64 ticks a
+    replay, so the launches run back to back and the events time the kernel), then the latency of single launches."""
     import numpy as np
     import torch
     from exact_ballpark_amd.vec_modes import ModesGraph
-    g = ModesGraph(park, ticks=ticks_per_launch)
-    for _ in range(max(1, warmup // ticks_per_launch)):
+    launches = max(1, 64 // ticks_per_launch)
+    g = ModesGraph(park, ticks=ticks_per_launch, launches=launches)
+    per_replay = ticks_per_launch * launches
+    for _ in range(max(1, warmup // per_replay)):
         g.tick()
     torch.cuda.synchronize()
-    reps = max(1, steps // ticks_per_launch)
+    reps = max(1, steps // per_replay)
     stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -69,25 +72,32 @@ def _measure(park, ticks_per_launch: int, steps: int, warmup: int) -> dict:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end)
-    lat_n = max(20, min(reps, 100))
+    one = ModesGraph(park, ticks=ticks_per_launch)
+    lat_n = 100
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(lat_n)]
     for e0, e1 in evs:
         e0.record(stream)
-        g.tick()
+        one.tick()
         e1.record(stream)
     torch.cuda.synchronize()
     lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
-    ticks = reps * ticks_per_launch
-    return {"ticks": ticks, "ms": ms, "ms_per_tick": ms / ticks, "wall_s": wall,
+    ticks = reps * per_replay
+    return {"ticks": ticks, "ms": ms, "ms_per_tick": ms / ticks, "ms_per_launch": ms / (reps * launches),
+            "launches_timed": reps * launches, "wall_s": wall,
             "launch_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(lat.max()), "launches": lat_n},
-            "ticks_run": max(1, warmup // ticks_per_launch) * ticks_per_launch + ticks + lat_n * ticks_per_launch}
+            "ticks_run": (max(1, warmup // per_replay) + reps) * per_replay + lat_n * ticks_per_launch}
 
 
-def _roofline(n_ships: int, ms_per_tick: float, traffic) -> dict:
-    achieved = BYTES_PER_SHIP_TICK * n_ships / (ms_per_tick / 1000.0) / 1e9
+def _roofline(n_ships: int, m: dict, ticks_per_launch: int, traffic) -> dict:
+    """The tick kernel against the HBM roofline: algorithmic bytes of a launch (361 B a ship-tick x ships x ticks) over
+    the launch's average duration (HIP events over back-to-back launches); `traffic`: the measured HBM bytes a launch
+    (rocprofv3 counters, calibrated), when profiled for this workload and shape."""
+    per_launch = BYTES_PER_SHIP_TICK * n_ships * ticks_per_launch
+    achieved = per_launch / (m["ms_per_launch"] / 1000.0) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic, "bytes_per_tick_algorithmic": BYTES_PER_SHIP_TICK * n_ships}
+            "traffic": traffic, "bytes_per_launch_algorithmic": per_launch, "ticks_per_launch": ticks_per_launch,
+            "launch_us": m["ms_per_launch"] * 1000.0, "launches_timed": m["launches_timed"]}
 
 
 def _traffic(key: str):
@@ -108,6 +118,8 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0, help="fix the workgroup size (0: autotune)")
     ap.add_argument("--schedule", type=int, default=-1, help="fix the lane schedule (-1: autotune)")
     ap.add_argument("--ticks-per-launch", type=int, default=0, help="fix the ticks per launch (0: autotune)")
+    ap.add_argument("--order", choices=("rows", "learned"), default="rows",
+                    help="the world order when the shape is fixed (--block, --schedule, --ticks-per-launch)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
@@ -164,7 +176,11 @@ def main() -> int:
         if fixed:
             park.block, park.schedule = a.block, a.schedule
             park._set_tiles()
-            shape = {"block": a.block, "schedule": a.schedule, "ticks": a.ticks_per_launch, "order": "rows",
+            if a.order == "learned" and orders.get("learned") is None:
+                raise SystemExit("--order learned needs the learned scheduler (not --no-learned, not the flight park)")
+            if a.order == "learned":
+                orders["learned"]()
+            shape = {"block": a.block, "schedule": a.schedule, "ticks": a.ticks_per_launch, "order": a.order,
                      "autotuned": False}
         else:
             r = AT.tune(park, blocks=(a.block,) if a.block else (64, 128, 256),
@@ -175,11 +191,17 @@ def main() -> int:
         print(f"# {kind}: {wl['n']} ships in {a.worlds} worlds, shape {shape['block']}/{shape['schedule']}/"
               f"{shape['ticks']}/{shape['order']}", file=sys.stderr, flush=True)
         snap, at_snap = park._save(), done          # every measurement below starts from this state
+        path0 = park.path.clone() if park.path is not None else None
         last = [0]
 
-        def measure(ticks_per_launch, steps, warmup):
+        def restore():
             for blk, old in zip((park._f, park._q, park._b), snap):
                 blk.copy_(old)
+            if path0 is not None:
+                park.path.copy_(path0)              # the recorded paths too: the learned order is predicted from them
+
+        def measure(ticks_per_launch, steps, warmup):
+            restore()
             r = _measure(park, ticks_per_launch, steps, warmup)
             last[0] = r["ticks_run"]
             return r
@@ -194,9 +216,9 @@ def main() -> int:
                "ship_ticks_per_s": n / (m["ms_per_tick"] / 1000.0), "ms_per_tick": m["ms_per_tick"],
                "launch_latency_ms": m["launch_ms"],
                "tick_latency_ms": one["launch_ms"],
-               "roofline": _roofline(n, m["ms_per_tick"], _traffic(f"{kind}_t{shape['ticks']}")),
+               "roofline": _roofline(n, m, shape["ticks"], _traffic(f"{kind}_{shape['order']}_t{shape['ticks']}")),
                "single_tick": {"ms_per_tick": one["ms_per_tick"],
-                               "roofline": _roofline(n, one["ms_per_tick"], _traffic(f"{kind}_t1"))}}
+                               "roofline": _roofline(n, one, 1, _traffic(f"{kind}_{shape['order']}_t1"))}}
         if orders and not fixed:
             # the best shape of each world order, each measured as the primary was
             best = {}
@@ -206,8 +228,7 @@ def main() -> int:
                     best[o] = (ms, int(b[1:]), int(sc[1:]), int(t[1:]))
             cmp = {}
             for o, (_, b, sc, t) in best.items():
-                for blk, old in zip((park._f, park._q, park._b), snap):
-                    blk.copy_(old)                  # the order predicted from the state the timing starts at
+                restore()                           # the order predicted from the state the timing starts at
                 park.block, park.schedule = b, sc
                 park._set_tiles()
                 if orders.get(o) is not None:

@@ -28,6 +28,7 @@ def time_shape(park: VecModesPark, block: int, schedule: int, ticks: int, reps: 
     enough launches for 32 ticks."""
     reps = reps or max(3, 32 // ticks)
     saved = park._save()
+    path0 = park.path.clone() if park.path is not None else None
     t0, keys0 = park.current_time, park.keys
     old = (park.block, park.schedule)
     park.block = block
@@ -51,6 +52,8 @@ def time_shape(park: VecModesPark, block: int, schedule: int, ticks: int, reps: 
         ms = s.elapsed_time(e) / (reps * ticks)
     finally:
         _restore(park, saved)
+        if path0 is not None:
+            park.path.copy_(path0)
         park.current_time, park.keys = t0, keys0
         park.block, park.schedule = old
         park._set_tiles()

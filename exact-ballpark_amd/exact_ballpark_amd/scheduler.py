@@ -226,17 +226,15 @@ def install_order(park, start: torch.Tensor, count: torch.Tensor, order: torch.T
     rowmap = torch.repeat_interleave(st - (ends - cnt), cnt) + torch.arange(park.n, device=park.dev)
     if not torch.equal(torch.sort(rowmap).values, torch.arange(park.n, device=park.dev)):
         raise ValueError("the order must hold every world once")
-    tiles, acc = [0], 0
-    for sz in cnt.cpu().tolist():
-        if acc + sz > park.block:
-            tiles.append(tiles[-1] + acc)
-            acc = 0
-        acc += sz
-    tiles.append(park.n)
-    park.tiles = torch.tensor(tiles, dtype=torch.int32, device=park.dev)
+    sizes = cnt.to(torch.int32).cpu().numpy()
+    tiles = np.zeros(sizes.size + 1, dtype=np.int32)
+    nt = ctypes.c_int64()
+    N.check(N.cpu().xb_cpu_pack_tiles(sizes.ctypes.data, sizes.size, park.block, tiles.ctypes.data, ctypes.byref(nt)),
+            "xb_cpu_pack_tiles")
+    park.tiles = torch.from_numpy(tiles[:nt.value + 1].copy()).to(park.dev)
     park.rowmap = rowmap.to(torch.int32)
     park._state = None
-    return {"tiles": len(tiles) - 1}
+    return {"tiles": int(nt.value)}
 
 
 def modeled_cost(park, block: int = 64) -> int:

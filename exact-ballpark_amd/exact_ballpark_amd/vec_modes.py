@@ -579,22 +579,25 @@ class VecModesPark(metaclass=_Fields):
 
 class ModesGraph:
     """A VecModesPark's tick captured once as a HIP graph on the park's own state blocks, so each replay is one tick
-    (the reference's ModesGraph, src/exact_ballpark/vec_modes_cuda.py:371-432). Commands and setters write into the
+    (the reference's ModesGraph, src/exact_ballpark/vec_modes_cuda.py:371-432); with `ticks` ticks a launch and
+    `launches` launches a replay, one replay runs ticks x launches ticks back to back. Commands and setters write into the
     blocks, so nothing needs re-binding between replays. With `masked=True` the capture holds OnTick's per-world mask
     as a device buffer: the blocks saved, the tick, the rows outside the mask put back; `tick(active)` writes the mask
     and replays."""
 
     def __init__(self, park: VecModesPark, block: int | None = None, lib=None, masked: bool = False,
-                 ticks: int = 1):
+                 ticks: int = 1, launches: int = 1):
         if park.dev.type != "cuda":
             raise ValueError("a HIP graph needs a park on a GPU")
         if lib is not None:
             raise ValueError("lib: the park binds its own library")
         self.park, self.block = park, park.block if block is None else int(block)
         self.masked = masked
-        self.ticks = int(ticks)
-        if masked and self.ticks != 1:
+        self.ticks, self.launches = int(ticks), int(launches)
+        if masked and (self.ticks != 1 or self.launches != 1):
             raise ValueError("OnTick's mask applies to one tick at a time")
+        if self.launches < 1 or self.ticks < 1:
+            raise ValueError("ticks and launches are at least 1")
         self.mask = torch.ones(park.stride, dtype=torch.bool, device=park.dev)
         self.saved = [torch.empty_like(b) for b in (park._f, park._q, park._b)] if masked else None
         before = park._save()                            # warm-up: the module loaded before the capture,
@@ -607,7 +610,8 @@ class ModesGraph:
             if masked:
                 for s, b in zip(self.saved, (park._f, park._q, park._b)):
                     s.copy_(b)
-            park._tick(self.block, self.ticks)
+            for _ in range(self.launches):
+                park._tick(self.block, self.ticks)
             if masked:
                 for s, b in zip(self.saved, (park._f, park._q, park._b)):
                     b.copy_(torch.where(self.mask[None, :], b, s))
@@ -621,4 +625,4 @@ class ModesGraph:
             else:
                 self.mask[:self.park.n].copy_(active.to(device=self.park.dev, dtype=torch.bool).reshape(-1))
         self.graph.replay()
-        self.park.current_time += self.ticks
+        self.park.current_time += self.ticks * self.launches
