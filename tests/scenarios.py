@@ -217,3 +217,147 @@ def flight_park(seed: int, worlds: int, ships: int = 4) -> list:
     from exact_ballpark_amd import workloads as WL
     wl = WL.flight_park(seed, worlds, ships)
     return [WL.world_scenario(wl, wi, 0) for wi in range(worlds)]
+
+
+# ---- dynamical orientation (the turning worlds) ----
+
+VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP = 8, 9, 10, 11, 12     # SetBallVelocity, SetBallRotation, ...
+
+
+def _do_ball(r: random.Random, bid: int, **kw) -> dict:
+    """A ball for the turning worlds: as _ball, not massive (no collision gradient), no bubble or cloak, STOP, with
+    the turning values (maximum angular speed, rotational agility)."""
+    b = _ball(r, bid, massive=False, bubble=-1, cloaked=0, moribund=False, mode=STOP)
+    b["max_ang_vel"] = r.choice([1.0, 1.0, r.uniform(0.05, 0.5), r.uniform(0.5, 3.0), 0.0])
+    b["rot_agility"] = r.choice([0.9, 0.9, r.uniform(0.05, 2.0), r.uniform(2.0, 10.0)])
+    b.update(kw)
+    return b
+
+
+def _quat(r: random.Random) -> list:
+    q = [r.gauss(0.0, 1.0) for _ in range(4)]
+    n = math.sqrt(sum(c * c for c in q))
+    kind = r.choice(["unit", "unit", "scaled", "tiny"])
+    s = {"unit": 1.0 / n, "scaled": r.uniform(0.2, 5.0) / n, "tiny": 1e-25 / n}[kind]
+    return [c * s for c in q]
+
+
+def do_cases(seed: int, scale: int = 1) -> list:
+    """The turning worlds (about 140 x scale): GOTO ahead, beside and behind the heading, FOLLOW and both ORBIT laws
+    (the new orbit's normal from the heading when the orbiter is at rest), WARP (aligning, then turning toward the
+    exit point in warp proper), EntityWarpIn (the rotation snapped), and the orientation setters (SetBallVelocity's
+    snap, SetBallRotation, SetBallAngularVelocity, SetMaxAngularSpeed, SetBallAngularAgility) with their refusals."""
+    r = random.Random(seed)
+    out: list = []
+
+    def world(kind, balls, cmds, steps, *, tick=1000, new_orbit=False, friction=1.0e6):
+        cmds = sorted(cmds, key=lambda c: c[0])
+        out.append({"kind": kind, "friction": friction, "tick": tick, "steps": steps, "new_orbit": new_orbit,
+                    "start": 0, "dynamical": True, "balls": balls, "cmds": cmds})
+
+    def friction_choice():
+        return r.choice([1.0e6] * 5 + [10 ** r.uniform(4.0, 7.0)])
+
+    def tick_choice():
+        return r.choice([250, 1000, 1000, 1000, 2000])
+
+    def turning(bid):
+        cmds = []
+        if r.random() < 0.4:
+            q = _quat(r)
+            cmds.append(_cmd(1, ROT_OP, bid, 0, q[0], q[1], q[2], rr=q[3]))
+        if r.random() < 0.4:
+            w = [r.uniform(-0.5, 0.5) for _ in range(3)]
+            cmds.append(_cmd(1, ANGVEL_OP, bid, 0, *w))
+        return cmds
+
+    for _ in range(40 * scale):                    # GOTO in every direction from the heading
+        ids = _ids(r, r.randint(1, 3), big=False)
+        balls, cmds = [], []
+        for bid in ids:
+            b = _do_ball(r, bid, free=r.random() < 0.9)
+            balls.append(b)
+            cmds += turning(bid)
+            u = _unit(r)
+            d = r.choice([r.uniform(1e3, 1e5), r.uniform(1e5, 1e8)])
+            cmds.append(_cmd(1, r.choice([GOTO_OP, GOTO_OP, DIR_OP]), bid, 0,
+                             *[b["pos"][i] + d * u[i] for i in range(3)]))
+            if r.random() < 0.4:
+                u = _unit(r)
+                cmds.append(_cmd(r.randint(2, 12), r.choice([GOTO_OP, DIR_OP, STOP_OP]), bid, 0,
+                                 *[b["pos"][i] + 1e6 * u[i] for i in range(3)]))
+        world("do_goto", balls, cmds, r.randint(3, 40), tick=tick_choice(), friction=friction_choice())
+
+    for i in range(40 * scale):                    # FOLLOW and both ORBIT laws
+        new = i % 2 == 1
+        ids = _ids(r, r.randint(2, 4), big=r.random() < 0.5)
+        target = _do_ball(r, ids[0], vel=r.choice([[0.0, 0.0, 0.0], [r.uniform(-50, 50) for _ in range(3)]]))
+        balls, cmds = [target], turning(ids[0])
+        if r.random() < 0.4:
+            cmds.append(_cmd(1, DIR_OP, target["id"], 0, *_unit(r)))
+        for bid in ids[1:]:
+            o = _do_ball(r, bid, vel=r.choice([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0],
+                                               [r.uniform(-100, 100) for _ in range(3)]]))
+            rng = _orbit_range(r, o, target)
+            o["pos"] = _near(r, target["pos"], _r_of(rng, o, target))
+            balls.append(o)
+            cmds += turning(bid)
+            cmds.append(_cmd(r.choice([1, 1, 2]), r.choice([ORBIT_OP, ORBIT_OP, FOLLOW_OP]), bid, target["id"],
+                             rr=rng))
+        world("do_orbit_new" if new else "do_orbit_old", balls, cmds, r.randint(2, 35), tick=tick_choice(),
+              new_orbit=new, friction=friction_choice())
+
+    for _ in range(25 * scale):                    # WARP: align, then turn toward the exit point in warp proper
+        ids = _ids(r, r.randint(1, 3), big=False)
+        ship = _do_ball(r, ids[0], max_vel=r.uniform(100.0, 1500.0))
+        u = _unit(r)
+        dist = 10 ** r.uniform(6.0, 11.0)
+        dest = [ship["pos"][i] + dist * u[i] for i in range(3)]
+        if r.random() < 0.5:
+            sp = ship["max_vel"] * r.uniform(0.8, 1.0)
+            ship["vel"] = [u[i] * sp for i in range(3)]
+        balls, cmds = [ship], turning(ids[0])
+        cmds.append(_cmd(1, WARP_OP, ship["id"], 0, *dest, rr=r.choice([20000.0, 0.0, 1e6]),
+                         w=r.choice([20, 3000, 10000])))
+        for bid in ids[1:]:
+            f = _do_ball(r, bid, interactive=r.random() < 0.6)
+            f["pos"] = _near(r, ship["pos"], r.uniform(100.0, 5000.0))
+            balls.append(f)
+            cmds.append(_cmd(1, r.choice([FOLLOW_OP, ORBIT_OP]), bid, ship["id"], rr=r.uniform(0.0, 3000.0)))
+        world("do_warp", balls, cmds, r.randint(5, 45), tick=tick_choice(), new_orbit=r.random() < 0.5)
+
+    for _ in range(10 * scale):                    # EntityWarpIn: the rotation snapped to the new velocity
+        ids = _ids(r, 2, big=False)
+        ship = _do_ball(r, ids[0])
+        f = _do_ball(r, ids[1], interactive=True)
+        u = _unit(r)
+        pt = [ship["pos"][i] + u[i] * 10 ** r.uniform(6.0, 11.0) for i in range(3)]
+        cmds = [_cmd(1, ORBIT_OP, ids[1], ids[0], rr=r.uniform(0.0, 3000.0)),
+                _cmd(r.randint(1, 3), WARPIN_OP, ids[0], 0, *pt, w=r.choice([3, 20, 3000]))]
+        world("do_warpin", [ship, f], cmds, r.randint(2, 25), tick=tick_choice())
+
+    for _ in range(25 * scale):                    # the setters, and their refusals
+        ids = _ids(r, r.randint(1, 3), big=False)
+        balls = [_do_ball(r, bid, free=r.random() < 0.9) for bid in ids]
+        cmds = []
+        for _ in range(r.randint(2, 8)):
+            bid = r.choice(ids)
+            op = r.choice([VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, GOTO_OP, STOP_OP])
+            step = r.randint(1, 6)
+            if op == VEL_OP:
+                v = r.choice([[0.0, 0.0, 0.0], [r.uniform(-300, 300) for _ in range(3)],
+                              [1e-7, 0.0, 0.0], [0.0, 0.0, r.uniform(-50, 50)]])
+                cmds.append(_cmd(step, op, bid, 0, *v))
+            elif op == ROT_OP:
+                q = _quat(r)
+                cmds.append(_cmd(step, op, bid, 0, q[0], q[1], q[2], rr=q[3]))
+            elif op == ANGVEL_OP:
+                cmds.append(_cmd(step, op, bid, 0, *[r.uniform(-1.0, 1.0) for _ in range(3)]))
+            elif op in (MAXANG_OP, ROTAG_OP):
+                cmds.append(_cmd(step, op, bid, 0, rr=r.choice([r.uniform(0.01, 3.0), 0.0, -r.uniform(0.1, 1.0)])))
+            elif op == GOTO_OP:
+                cmds.append(_cmd(step, op, bid, 0, *[r.uniform(-1e6, 1e6) for _ in range(3)]))
+            else:
+                cmds.append(_cmd(step, op, bid))
+        world("do_setters", balls, cmds, r.randint(3, 20), tick=tick_choice(), friction=friction_choice())
+    return out
