@@ -15,6 +15,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
 
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
+VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP = 8, 9, 10, 11, 12
+NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
            "ovz", "mass", "radius", "max_vel", "agility", "sf", "follow_range", "time_factor", "last_collision", "pad")
@@ -53,6 +55,14 @@ def lib() -> ctypes.CDLL:
         L.or_get_followers.argtypes = [vp, i, pq]
         L.or_maths.argtypes = [i, i64, pd, pd]
         L.or_evolve_all.argtypes = [ctypes.POINTER(vp), i, i]
+        L.or_park_set_dynamical.argtypes = [vp, i]
+        L.or_add_turning.argtypes = [vp, i64, d, d]
+        L.or_set_velocity.argtypes = [vp, i64, d, d, d]
+        L.or_set_angular_velocity.argtypes = [vp, i64, d, d, d]
+        L.or_set_rotation.argtypes = [vp, i64, d, d, d, d]
+        L.or_set_max_angular_speed.argtypes = [vp, i64, d]
+        L.or_set_rot_agility.argtypes = [vp, i64, d]
+        L.or_get_turning.argtypes = [vp, i, pd]
         _LIB = L
     return _LIB
 
@@ -60,9 +70,11 @@ def lib() -> ctypes.CDLL:
 class World:
     """One world of the oracle: add balls, command them, tick, read them back."""
 
-    def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int):
+    def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int, dynamical: bool = False):
         self.L = lib()
         self.p = self.L.or_park_new(float(friction), int(tick_ms), int(bool(new_orbit)), int(start))
+        self.dynamical = bool(dynamical)
+        self.L.or_park_set_dynamical(self.p, int(self.dynamical))
 
     def __del__(self):
         if getattr(self, "p", None):
@@ -78,6 +90,9 @@ class World:
                                 int(b["bubble"]), int(bool(b["moribund"])), int(b["mode"]))
         if rc:
             raise ValueError(f"ball {b['id']} refused")
+        if self.dynamical:                             # AddDynamicallyOrientedBall and the AddBall's snap
+            self.L.or_add_turning(self.p, int(b["id"]), float(b.get("max_ang_vel", 1.0)),
+                                  float(b.get("rot_agility", 0.9)))
 
     def command(self, c) -> None:
         _, op, a, b, x, y, z, rr, w = c
@@ -96,11 +111,30 @@ class World:
             L.or_warp_to(p, a, x, y, z, rr, w)
         elif op == WARPIN_OP:
             L.or_entity_warp_in(p, a, x, y, z, w)
+        elif op == VEL_OP:
+            L.or_set_velocity(p, a, x, y, z)
+        elif op == ROT_OP:
+            L.or_set_rotation(p, a, x, y, z, rr)
+        elif op == ANGVEL_OP:
+            L.or_set_angular_velocity(p, a, x, y, z)
+        elif op == MAXANG_OP:
+            L.or_set_max_angular_speed(p, a, rr)
+        elif op == ROTAG_OP:
+            L.or_set_rot_agility(p, a, rr)
         else:
             raise ValueError(op)
 
     def evolve(self) -> None:
         self.L.or_evolve(self.p)
+
+    def turning(self) -> np.ndarray:
+        """[n, 20] float64, balls by ascending id: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel,
+        rot_agility."""
+        n = self.L.or_count(self.p)
+        t = np.zeros((n, NT), dtype=np.float64)
+        for k in range(n):
+            self.L.or_get_turning(self.p, k, t[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return t
 
     def state(self) -> tuple:
         """(doubles [n, 27] as float64, ints [n, 12], followers: list of sorted id lists), balls by ascending id."""
@@ -119,7 +153,7 @@ class World:
 
 def run_world(w: dict, record_every_step: bool = True):
     """Yields (step, doubles, ints, followers) after each tick of world w (every step, or the last only)."""
-    o = World(w["friction"], w["tick"], w["new_orbit"], w["start"])
+    o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False))
     for b in w["balls"]:
         o.add(b)
     for step in range(1, w["steps"] + 1):

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "../exact-ballpark_amd/csrc/xb_maths_double.h"
+#include "../exact-ballpark_amd/csrc/xb_maths_float.h"
 
 /* DSTBALLMODE (modes.py:65-67, ref.py:34) */
 enum { GOTO = 0, FOLLOW = 1, STOP = 2, WARP = 3, ORBIT = 4, MISSILE = 5, MUSHROOM = 6, BOID = 7, TROLL = 8,
@@ -48,12 +49,15 @@ typedef struct {
     int64_t cloaked, bubble;
     int nf;
     int64_t followers[MAX_FOLLOWERS];
+    /* dynamical orientation (full.py): floats held in double */
+    double rot[4], old_rot[4], ang_vel[3], old_ang_vel[3], torque[3], rot_roll, max_ang_vel, rot_agility;
 } OBall;
 
 typedef struct {
     double friction, dt;
     int64_t current_time;
     int use_new_orbit;
+    int dynamical;                             /* FullBallpark's dynamical_orientation */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
 } OPark;
@@ -163,7 +167,485 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->follow_ptr = -1;
     b->is_free = is_free, b->is_massive = is_massive, b->is_interactive = is_interactive;
     b->cloaked = cloaked, b->bubble = bubble, b->is_moribund = is_moribund;
+    b->rot[3] = b->old_rot[3] = 1.0;                                 /* Ball::Ball: identity, at rest, 1 rad/s, 0.9 */
+    b->max_ang_vel = 1.0;
+    b->rot_agility = (double)0.9f;
     refresh_time_factor(p, b);
+    return 0;
+}
+
+/* --------------------------------------------------------------- dynamical orientation (full.py, MOV-728) */
+/* C++ float arithmetic as full.py computes it: each operation in double, rounded to binary32 (F, full.py:111-121;
+   for + - * / and sqrt of floats that is the float operation). Vectors, quaternions and matrices are carbonengine's
+   (full.py:179-325). The float transcendentals are glibc's (the product's maths headers, pinned on their own). */
+
+static double F(double x) { return (double)(float)x; }
+static double fdiv_(double a, double b) { return F(a / b); }
+static double fmax_(double a, double b) { return a < b ? b : a; }                 /* std::max */
+static double fmin_(double a, double b) { return b < a ? b : a; }                 /* std::min */
+static int to_int32_(double x) {                                                 /* full.py:155-164 */
+    if (x != x) return 0;
+    if (x >= 2147483648.0) return 2147483647;
+    if (x < -2147483648.0) return -2147483647 - 1;
+    return (int)x;
+}
+static double sinf_(double x) { return (double)xb_sinf((float)x); }
+static double cosf_(double x) { return (double)xb_cosf((float)x); }
+static double expf_(double x) { return (double)xb_expf((float)x); }
+static double logf_(double x) { return (double)xb_logf((float)x); }
+static double acosf_(double x) { return (double)xb_acosf((float)x); }
+static double atan2f_(double y, double x) { return (double)xb_atan2f((float)y, (float)x); }
+
+static void v3f(double x, double y, double z, double o[3]) { o[0] = F(x), o[1] = F(y), o[2] = F(z); }
+static void v3_addf(const double a[3], const double b[3], double o[3]) { v3f(a[0] + b[0], a[1] + b[1], a[2] + b[2], o); }
+static void v3_subf(const double a[3], const double b[3], double o[3]) { v3f(a[0] - b[0], a[1] - b[1], a[2] - b[2], o); }
+static void v3_mulf(const double a[3], double f, double o[3]) { v3f(a[0] * f, a[1] * f, a[2] * f, o); }
+static void v3_divf(const double a[3], double f, double o[3]) { v3_mulf(a, fdiv_(1.0, f), o); }
+static double v3_dotf(const double a[3], const double b[3]) { return F(F(F(a[0] * b[0]) + F(a[1] * b[1])) + F(a[2] * b[2])); }
+static double v3_lenf(const double a[3]) { return F(sqrt(v3_dotf(a, a))); }
+static void v3_crossf(const double a[3], const double b[3], double o[3]) {
+    double x = F(F(a[1] * b[2]) - F(a[2] * b[1])), y = F(F(a[2] * b[0]) - F(a[0] * b[2]));
+    double z = F(F(a[0] * b[1]) - F(a[1] * b[0]));
+    o[0] = x, o[1] = y, o[2] = z;
+}
+static void v3_normalizef(const double v[3], double o[3]) {                      /* full.py:215-222 */
+    double mx = fmax_(fmax_(fabs(v[0]), fabs(v[1])), fabs(v[2])), mini[3];
+    v3_divf(v, mx != 0.0 ? mx : 1.0, mini);
+    double length = v3_lenf(mini);
+    if (length != 0.0) length = fdiv_(1.0, length);
+    v3_mulf(mini, length, o);
+}
+static void q_mulf(const double q[4], const double o[4], double r[4]) {          /* full.py:233-240 */
+    double x = q[0], y = q[1], z = q[2], w = q[3], ox = o[0], oy = o[1], oz = o[2], ow = o[3];
+    double rx = F(F(F(F(ow * x) + F(ox * w)) + F(oy * z)) - F(oz * y));
+    double ry = F(F(F(F(ow * y) - F(ox * z)) + F(oy * w)) + F(oz * x));
+    double rz = F(F(F(F(ow * z) + F(ox * y)) - F(oy * x)) + F(oz * w));
+    double rw = F(F(F(F(ow * w) - F(ox * x)) - F(oy * y)) - F(oz * z));
+    r[0] = rx, r[1] = ry, r[2] = rz, r[3] = rw;
+}
+static double q_lensqf(const double q[4]) { return F(F(F(F(q[0] * q[0]) + F(q[1] * q[1])) + F(q[2] * q[2])) + F(q[3] * q[3])); }
+static double q_lenf(const double q[4]) { return F(sqrt(q_lensqf(q))); }
+static void q_divf(const double q[4], double f, double r[4]) {
+    double d = fdiv_(1.0, f);
+    r[0] = F(q[0] * d), r[1] = F(q[1] * d), r[2] = F(q[2] * d), r[3] = F(q[3] * d);
+}
+static void q_conjf(const double q[4], double r[4]) { r[0] = -q[0], r[1] = -q[1], r[2] = -q[2], r[3] = q[3]; }
+static void q_rotationf(const double axis[3], double angle, double r[4]) {       /* full.py:260-265 */
+    double t[3];
+    v3_normalizef(axis, t);
+    double h = fdiv_(angle, 2.0), s = sinf_(h);
+    r[0] = F(s * t[0]), r[1] = F(s * t[1]), r[2] = F(s * t[2]), r[3] = cosf_(h);
+}
+static void m_rotation_qf(const double q[4], double m[4][4]) {                  /* full.py:283-292 */
+    double x = q[0], y = q[1], z = q[2], w = q[3];
+    m[0][0] = F(1.0 - F(2.0 * F(F(y * y) + F(z * z)))), m[0][1] = F(2.0 * F(F(x * y) + F(z * w)));
+    m[0][2] = F(2.0 * F(F(x * z) - F(y * w))), m[0][3] = 0.0;
+    m[1][0] = F(2.0 * F(F(x * y) - F(z * w))), m[1][1] = F(1.0 - F(2.0 * F(F(x * x) + F(z * z))));
+    m[1][2] = F(2.0 * F(F(y * z) + F(x * w))), m[1][3] = 0.0;
+    m[2][0] = F(2.0 * F(F(x * z) + F(y * w))), m[2][1] = F(2.0 * F(F(y * z) - F(x * w)));
+    m[2][2] = F(1.0 - F(2.0 * F(F(x * x) + F(y * y)))), m[2][3] = 0.0;
+    m[3][0] = m[3][1] = m[3][2] = 0.0, m[3][3] = 1.0;
+}
+static void transform_coordf(const double v[3], double m[4][4], double o[3]) {  /* full.py:314-324 */
+    double x = v[0], y = v[1], z = v[2];
+    double norm = F(F(F(F(m[0][3] * x) + F(m[1][3] * y)) + F(m[2][3] * z)) + m[3][3]);
+    if (norm != 0.0) {
+        double r[3];
+        for (int j = 0; j < 3; ++j) r[j] = fdiv_(F(F(F(F(x * m[0][j]) + F(y * m[1][j])) + F(z * m[2][j])) + m[3][j]), norm);
+        o[0] = r[0], o[1] = r[1], o[2] = r[2];
+    } else {
+        o[0] = o[1] = o[2] = 0.0;
+    }
+}
+static void heading_of(const double rot[4], double h[3]) {
+    double m[4][4], z[3] = {0.0, 0.0, 1.0};
+    m_rotation_qf(rot, m);
+    transform_coordf(z, m, h);
+}
+
+static double tau_of(const OPark* p, const OBall* b) {                          /* full.py:2121-2123 */
+    return fdiv_(F(F(F(b->mass) * b->agility) * b->rot_agility), F(p->friction));
+}
+static double t_scale_of(const OPark* p, const OBall* b) {                      /* full.py:2125-2127 */
+    return fdiv_(F(p->friction), F(F(F(b->mass) * b->agility) * b->rot_agility));
+}
+
+/* SnapOrientation (full.py:2091-2110): turned to the velocity and rolled upright, the rotation stopped. */
+static void snap_orientation(OBall* b) {
+    if (!(lensq(b->vel) > 1e-12)) return;
+    double dir[3], z[3] = {0.0, 0.0, 1.0}, up0[3] = {0.0, 1.0, 0.0};
+    v3f(b->vel[0], b->vel[1], b->vel[2], dir);
+    double len_dir = v3_lenf(dir);
+    if (len_dir < 1e-12) {
+        b->rot[0] = b->rot[1] = b->rot[2] = 0.0, b->rot[3] = 1.0;
+    } else {
+        double vec[3], rot[4], cj[4], m[4][4], up[3], heading[3], roll_q[4], out[4];
+        v3_crossf(z, dir, vec);
+        double angle = acosf_(fmax_(-1.0, fmin_(1.0, fdiv_(dir[2], len_dir))));
+        q_rotationf(vec, angle, rot);
+        q_conjf(rot, cj);
+        m_rotation_qf(cj, m);
+        transform_coordf(up0, m, up);
+        double up_roll = up[2] < F(0.99) ? atan2f_(-up[0], up[1]) : 0.0;
+        heading_of(rot, heading);
+        q_rotationf(heading, up_roll, roll_q);
+        q_mulf(rot, roll_q, out);
+        memcpy(b->rot, out, sizeof(out));
+    }
+    memcpy(b->old_rot, b->rot, sizeof(b->rot));
+    for (int i = 0; i < 3; ++i) b->ang_vel[i] = b->old_ang_vel[i] = 0.0;
+}
+
+/* CalculateN2Alpha2 (full.py:2133-2153): the braking ticks n2, kappa2 = tf^n2, the braking torque */
+static void n2_alpha2_v(const double omega0[3], double len_omega0, const OBall* b, double tf, double dt,
+                        double t_scale, int* n2, double* kappa2, double alpha2[3]) {
+    if (len_omega0 > 1e-10) {
+        double factor = fdiv_(b->max_ang_vel, len_omega0);
+        double c = fdiv_(logf_(fdiv_(factor, F(1.0 + factor))), F(-dt * t_scale));
+        *n2 = to_int32_(isfinite(c) ? ceil(c) : c);
+        *kappa2 = F(xb_pow(tf, (double)*n2));
+        double neg[3] = {-omega0[0], -omega0[1], -omega0[2]};
+        v3_mulf(neg, fdiv_(F(t_scale * *kappa2), F(1 - *kappa2)), alpha2);
+        return;
+    }
+    *n2 = 0, *kappa2 = 1.0, alpha2[0] = alpha2[1] = alpha2[2] = 0.0;
+}
+static void n2_alpha2_f(double omega0, const OBall* b, double tf, double dt, double t_scale, int* n2,
+                        double* kappa2, double* alpha2) {
+    if (fabs(omega0) > 1e-10) {
+        double factor = fdiv_(b->max_ang_vel, fabs(omega0));
+        double c = fdiv_(logf_(fdiv_(factor, F(1.0 + factor))), F(-dt * t_scale));
+        *n2 = to_int32_(isfinite(c) ? ceil(c) : c);
+        *kappa2 = F(xb_pow(tf, (double)*n2));
+        *alpha2 = F(-omega0 * fdiv_(F(t_scale * *kappa2), F(1 - *kappa2)));
+        return;
+    }
+    *n2 = 0, *kappa2 = 1.0, *alpha2 = 0.0;
+}
+
+/* GotoTorqueBreaks (full.py:2155-2222) */
+static void goto_torque_breaks(const OPark* p, const OBall* b, const double direction[3], double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    double t_scale = t_scale_of(p, b);
+    double max_alpha = F(b->max_ang_vel * t_scale);
+    if (max_alpha <= 0.0) return;
+    double len_dir = v3_lenf(direction);
+    if (len_dir == 0.0) return;
+    double h0[3], heading[3], rc[3], rot[3];
+    heading_of(b->rot, h0);
+    v3_normalizef(h0, heading);
+    v3_crossf(heading, direction, rc);
+    v3_divf(rc, len_dir, rot);
+    double rot_len = v3_lenf(rot);
+    double rot_angle = acosf_(fmax_(-1.0, fmin_(1.0, fdiv_(v3_dotf(direction, heading), len_dir))));
+    if (rot_len > 1e-12) {
+        v3_mulf(rot, fdiv_(rot_angle, rot_len), rot);
+    } else if (rot_angle > 1) {
+        rot[0] = -rot_angle, rot[1] = 0.0, rot[2] = 0.0;
+    }
+    double hw[3], omega0[3];
+    v3_mulf(heading, v3_dotf(heading, b->ang_vel), hw);
+    v3_subf(b->ang_vel, hw, omega0);
+    double len_omega0 = v3_lenf(omega0);
+    double dt = p->dt;
+    if (len_omega0 * dt < 1e-4 && rot_angle < 1e-4) return;
+    double fdt = F(dt);
+    double tf = expf_(F(-fdt * t_scale));
+    double tau = fdiv_(1.0, t_scale);
+    int n1 = 0, n2;
+    double alpha1[3] = {0.0, 0.0, 0.0}, kappa1 = 1.0, kappa2, alpha2[3];
+    n2_alpha2_v(omega0, len_omega0, b, tf, fdt, t_scale, &n2, &kappa2, alpha2);
+    for (int round = 0; round < 5; ++round) {
+        if (n2 > 0) {
+            double a1[3], a2[3], w1[3], w2[3], theta2[3], diff[3];
+            v3_mulf(alpha2, tau, a1);
+            v3_mulf(a1, F(F(n2 * dt) - F(tau * F(1.0 - kappa2))), a2);
+            v3_mulf(omega0, tau, w1);
+            v3_mulf(w1, F(1 - kappa2), w2);
+            v3_addf(a2, w2, theta2);
+            v3_subf(rot, theta2, diff);
+            if (v3_lenf(diff) < 1e-3) {
+                n1 = 0;
+                break;
+            }
+        }
+        kappa1 = 1.0;
+        n1 = 1;
+        while (n1 < 100) {
+            kappa1 = F(kappa1 * tf);
+            double wm[3], num3[3];
+            if (n2 == 0) {
+                v3_mulf(omega0, F(tau * F(1.0 - kappa1)), wm);
+                v3_subf(rot, wm, num3);
+                v3_divf(num3, F(tau * F(F(n1 * dt) - F(tau * F(1.0 - kappa1)))), alpha1);
+            } else {
+                double num = F(tau - fdiv_(F(F(kappa1 * kappa2) * F(n2 * dt)), F(1 - kappa2)));
+                double den = F(tau * F(F(n1 * dt) - fdiv_(F(F(F(n2 * dt) * kappa2) * F(1 - kappa1)), F(1 - kappa2))));
+                v3_mulf(omega0, num, wm);
+                v3_subf(rot, wm, num3);
+                v3_divf(num3, den, alpha1);
+            }
+            if (v3_lenf(alpha1) < max_alpha) break;
+            n1 += 1;
+        }
+        int n2_old = n2;
+        double a1[3], w1[3], omega1[3];
+        v3_mulf(alpha1, F(tau * F(1 - kappa1)), a1);
+        v3_mulf(omega0, kappa1, w1);
+        v3_addf(a1, w1, omega1);
+        n2_alpha2_v(omega1, v3_lenf(omega1), b, tf, fdt, t_scale, &n2, &kappa2, alpha2);
+        if (n2_old == n2) break;
+    }
+    if (n1 > 0) {
+        if (n1 > 1) {
+            double u[3];
+            v3_normalizef(alpha1, u);
+            v3_mulf(u, max_alpha, alpha1);
+        }
+        memcpy(out, alpha1, sizeof(alpha1));
+        return;
+    }
+    if (n2 > 0) memcpy(out, alpha2, sizeof(alpha2));
+}
+
+/* GotoRollBreaks (full.py:2224-2290) */
+static double goto_roll_breaks(const OPark* p, const OBall* b, const double heading_torque[3]) {
+    double t_scale = t_scale_of(p, b);
+    double max_alpha = F(t_scale * b->max_ang_vel);
+    if (max_alpha <= 0) return 0.0;
+    double cj[4], mat[4][4], ship_omega[3];
+    q_conjf(b->rot, cj);
+    m_rotation_qf(cj, mat);
+    transform_coordf(b->ang_vel, mat, ship_omega);
+    double omega_roll = ship_omega[2];
+    double len_torque = v3_lenf(heading_torque);
+    double turn_roll = 0.0, weight = 0.0;
+    if (len_torque > 0.05 * max_alpha) {
+        double ship_torque[3];
+        transform_coordf(heading_torque, mat, ship_torque);
+        double flat[3] = {ship_omega[0], ship_omega[1], 0.0};
+        if (v3_dotf(flat, ship_torque) >= 0 || v3_lenf(flat) < F(F(0.01) * b->max_ang_vel)) {
+            turn_roll = atan2f_(-ship_torque[1], -ship_torque[0]);
+            weight = fdiv_(len_torque, t_scale);
+            if (weight > 1.0) weight = 1.0;
+        }
+    }
+    double up0[3] = {0.0, 1.0, 0.0}, up[3];
+    transform_coordf(up0, mat, up);
+    double up_roll = up[2] < F(0.95) ? atan2f_(-up[0], up[1]) : 0.0;
+    double desired = weight < F(0.05) ? up_roll : F(weight * turn_roll);
+    if (fabs(omega_roll) < 1e-4 && fabs(desired) < 1e-4) return 0.0;
+    double dt = p->dt, fdt = F(dt);
+    double tf = expf_(F(-fdt * t_scale));
+    double tau = fdiv_(1.0, t_scale);
+    int n1 = 0, n2;
+    double alpha1 = 0.0, kappa1 = 1.0, kappa2, alpha2;
+    n2_alpha2_f(omega_roll, b, tf, fdt, t_scale, &n2, &kappa2, &alpha2);
+    for (int round = 0; round < 5; ++round) {
+        if (n2 > 0) {
+            double theta2 = F(F(F(tau * alpha2) * F(F(n2 * dt) - F(tau * F(1.0 - kappa2)))) +
+                              F(F(tau * omega_roll) * F(1 - kappa2)));
+            if (fabs(F(desired - theta2)) < 1e-3) {
+                n1 = 0;
+                break;
+            }
+        }
+        kappa1 = 1.0;
+        n1 = 1;
+        while (n1 < 100) {
+            kappa1 = F(kappa1 * tf);
+            if (n2 == 0) {
+                alpha1 = fdiv_(F(desired - F(omega_roll * F(tau * F(1.0 - kappa1)))),
+                               F(tau * F(F(n1 * dt) - F(tau * F(1.0 - kappa1)))));
+            } else {
+                double num = F(tau - fdiv_(F(F(kappa1 * kappa2) * F(n2 * dt)), F(1 - kappa2)));
+                double den = F(tau * F(F(n1 * dt) - fdiv_(F(F(F(n2 * dt) * kappa2) * F(1 - kappa1)), F(1 - kappa2))));
+                alpha1 = fdiv_(F(desired - F(omega_roll * num)), den);
+            }
+            if (fabs(alpha1) < max_alpha) break;
+            n1 += 1;
+        }
+        int n2_old = n2;
+        double omega1 = F(F(F(alpha1 * tau) * F(1 - kappa1)) + F(omega_roll * kappa1));
+        n2_alpha2_f(omega1, b, tf, fdt, t_scale, &n2, &kappa2, &alpha2);
+        if (n2_old == n2) break;
+    }
+    if (n1 > 0) {
+        if (n1 > 1) alpha1 = F((alpha1 >= 0.0 ? 1.0 : -1.0) * max_alpha);
+        return alpha1;
+    }
+    if (n2 > 0) return alpha2;
+    return 0.0;
+}
+
+/* ApplyTorque (full.py:2292-2318) over one tick (fraction 1): (rotation, angular velocity) */
+static void apply_torque(const OPark* p, const double rotation0[4], const double angvel[3], const double torque[3],
+                         double roll, double tau, double rot_out[4], double w_out[3]) {
+    double fdt = F(p->dt), fraction = 1.0;
+    double tf = expf_(fdiv_(F(-fdt * fraction), tau));
+    double heading[3];
+    heading_of(rotation0, heading);
+    double omega_roll = v3_dotf(angvel, heading);
+    double hr[3], omega_heading[3];
+    v3_mulf(heading, omega_roll, hr);
+    v3_subf(angvel, hr, omega_heading);
+    double span = F(F(fraction * fdt) - F(tau * F(1.0 - tf)));
+    double rot_roll = F(tau * F(F(roll * span) + F(omega_roll * F(1.0 - tf))));
+    double t1[3], t2[3], t3[3], rot_heading[3];
+    v3_mulf(torque, span, t1);
+    v3_mulf(omega_heading, F(1.0 - tf), t2);
+    v3_addf(t1, t2, t3);
+    v3_mulf(t3, tau, rot_heading);
+    double angle_heading = v3_lenf(rot_heading);
+    int apply_heading = angle_heading > 1e-10;
+    double quat_heading[4] = {0.0, 0.0, 0.0, 1.0}, heading_rotated[3];
+    memcpy(heading_rotated, heading, sizeof(heading));
+    if (apply_heading) {
+        double m[4][4];
+        q_rotationf(rot_heading, v3_lenf(rot_heading), quat_heading);
+        m_rotation_qf(quat_heading, m);
+        transform_coordf(heading, m, heading_rotated);
+    }
+    double w1[3], w2[3], w[3], w3[3];
+    v3_mulf(torque, F(tau * F(1.0 - tf)), w1);
+    v3_mulf(omega_heading, tf, w2);
+    v3_addf(w1, w2, w);
+    v3_mulf(heading_rotated, F(F(tau * roll) - F(F(F(tau * roll) - omega_roll) * tf)), w3);
+    v3_addf(w, w3, w);
+    double rotation[4], tmp[4], rq[4];
+    memcpy(rotation, rotation0, sizeof(rotation));
+    if (apply_heading) {
+        q_mulf(rotation, quat_heading, tmp);
+        memcpy(rotation, tmp, sizeof(tmp));
+    }
+    q_rotationf(heading_rotated, rot_roll, rq);
+    q_mulf(rotation, rq, tmp);
+    memcpy(rotation, tmp, sizeof(tmp));
+    double norm = q_lenf(rotation);
+    if (norm > 0) q_divf(rotation, norm, rotation);
+    memcpy(rot_out, rotation, sizeof(rotation));
+    memcpy(w_out, w, sizeof(w));
+}
+
+/* EvolveBehaviorForBall's dynamical-orientation branch (full.py:2459-2485); a: the mode's thrust, replaced */
+static void orient(OPark* p, OBall* b, double a[3]) {
+    double accel = F(length(a));
+    double max_accel = fdiv_(F(F(F(p->friction) * b->sf) * b->max_vel), F(F(b->mass) * b->agility));
+    double av[3];
+    v3f(a[0], a[1], a[2], av);
+    if (accel > 1e-2 * max_accel) {
+        goto_torque_breaks(p, b, av, b->torque);
+        b->rot_roll = goto_roll_breaks(p, b, b->torque);
+        double rotation[4], w[3], heading[3];
+        apply_torque(p, b->rot, b->ang_vel, b->torque, b->rot_roll, tau_of(p, b), rotation, w);
+        heading_of(rotation, heading);
+        double cosangle = fmin_(1.0, fdiv_(v3_dotf(heading, av), accel));
+        if (cosangle <= 0.0) {
+            a[0] = a[1] = a[2] = 0.0;
+            return;
+        }
+        double h[3];
+        v3_mulf(heading, F(F(cosangle * cosangle) * accel), h);
+        a[0] = h[0], a[1] = h[1], a[2] = h[2];
+        return;
+    }
+    if (b->mode == WARP && !(b->effect_stamp < 0)) {
+        double g[3];
+        v3f(b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2], g);
+        goto_torque_breaks(p, b, g, b->torque);
+    } else {
+        b->torque[0] = b->torque[1] = b->torque[2] = 0.0;
+    }
+    b->rot_roll = goto_roll_breaks(p, b, b->torque);
+}
+
+/* The commit of the rotation a tick on (full.py:2523-2531, 2558-2563): new values where finite */
+static void commit_rotation(OPark* p, OBall* b) {
+    double rot[4], w[3];
+    apply_torque(p, b->rot, b->ang_vel, b->torque, b->rot_roll, tau_of(p, b), rot, w);
+    if (isfinite(q_lensqf(rot))) {
+        memcpy(b->old_rot, b->rot, sizeof(rot));
+        memcpy(b->rot, rot, sizeof(rot));
+    } else {
+        memcpy(b->old_rot, rot, sizeof(rot));
+    }
+    if (isfinite(v3_dotf(w, w))) {
+        memcpy(b->old_ang_vel, b->ang_vel, sizeof(w));
+        memcpy(b->ang_vel, w, sizeof(w));
+    } else {
+        memcpy(b->old_ang_vel, w, sizeof(w));
+    }
+}
+
+void or_park_set_dynamical(OPark* p, int on) { p->dynamical = on; }
+
+/* AddDynamicallyOrientedBall's turning part and the Python AddBall's snap (full.py:1290-1330): identity rotation, at
+   rest, the old rotation rot x RotationQuaternion(angVel, -dt |angVel|), the turning values clamped */
+int or_add_turning(OPark* p, int64_t id, double max_ang_vel, double rot_agility) {
+    OBall* b = find(p, id);
+    if (!b) return -1;
+    max_ang_vel = F(max_ang_vel), rot_agility = F(rot_agility);
+    double rot[4] = {0.0, 0.0, 0.0, 1.0}, w[3] = {0.0, 0.0, 0.0}, back[4];
+    memcpy(b->rot, rot, sizeof(rot));
+    memcpy(b->ang_vel, w, sizeof(w));
+    q_rotationf(w, F(-F(p->dt) * v3_lenf(w)), back);
+    q_mulf(rot, back, b->old_rot);
+    memcpy(b->old_ang_vel, w, sizeof(w));
+    b->max_ang_vel = max_ang_vel < 0.0 ? 0.0 : max_ang_vel;
+    b->rot_agility = rot_agility <= 0.0 ? 1.0 : rot_agility;
+    snap_orientation(b);
+    return 0;
+}
+
+/* The setters (full.py:1583-1631) */
+void or_set_velocity(OPark* p, int64_t id, double x, double y, double z) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    b->vel[0] = b->old_vel[0] = x, b->vel[1] = b->old_vel[1] = y, b->vel[2] = b->old_vel[2] = z;
+    if (p->dynamical) snap_orientation(b);
+}
+void or_set_max_angular_speed(OPark* p, int64_t id, double w) {
+    w = F(w);
+    OBall* b = find(p, id);
+    if (w < 0.0 || !b) return;
+    b->max_ang_vel = w;
+}
+void or_set_angular_velocity(OPark* p, int64_t id, double wx, double wy, double wz) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    v3f(wx, wy, wz, b->ang_vel);
+    memcpy(b->old_ang_vel, b->ang_vel, sizeof(b->ang_vel));
+}
+void or_set_rotation(OPark* p, int64_t id, double rx, double ry, double rz, double rw) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    double rot[4] = {F(rx), F(ry), F(rz), F(rw)};
+    double len = q_lenf(rot);
+    if (len > 1e-20) {
+        q_divf(rot, len, rot);
+        memcpy(b->rot, rot, sizeof(rot));
+        memcpy(b->old_rot, rot, sizeof(rot));
+        for (int i = 0; i < 3; ++i) b->ang_vel[i] = b->old_ang_vel[i] = 0.0;
+    }
+}
+void or_set_rot_agility(OPark* p, int64_t id, double a) {
+    a = F(a);
+    OBall* b = find(p, id);
+    if (!b || a <= 0.0) return;
+    b->rot_agility = a;
+}
+
+/* Ball k's turning state: rot (4), old_rot (4), ang_vel (3), old_ang_vel (3), torque (3), rot_roll, max_ang_vel,
+   rot_agility (20 doubles) */
+int or_get_turning(const OPark* p, int k, double* d) {
+    if (k < 0 || k >= p->n) return -1;
+    const OBall* b = &p->balls[k];
+    memcpy(d, b->rot, 4 * 8);
+    memcpy(d + 4, b->old_rot, 4 * 8);
+    memcpy(d + 8, b->ang_vel, 3 * 8);
+    memcpy(d + 11, b->old_ang_vel, 3 * 8);
+    memcpy(d + 14, b->torque, 3 * 8);
+    d[17] = b->rot_roll, d[18] = b->max_ang_vel, d[19] = b->rot_agility;
     return 0;
 }
 
@@ -307,6 +789,7 @@ void or_entity_warp_in(OPark* p, int64_t id, double x, double y, double z, int64
     b->effect_stamp = p->current_time - 5 > 0 ? p->current_time - 5 : 0;
     double back[3] = {b->pos[0] - b->gp[0], b->pos[1] - b->gp[1], b->pos[2] - b->gp[2]};
     b->last_collision = length(back);
+    if (p->dynamical) snap_orientation(b);                          /* full.py:1574-1581 */
 }
 
 /* --------------------------------------------------------------------------------------------- behaviours */
@@ -450,8 +933,19 @@ static void evolve_new_orbit(OPark* p, OBall* b, double out[3]) {          /* mo
     if (length2 > 1e-12) {
         double inv = 1.0 / sqrt(length2);
         for (int i = 0; i < 3; ++i) normal[i] = o[i] * inv;
-    } else {
+    } else if (!p->dynamical) {
         precessing_w(p, b, normal);
+    } else {                                                          /* the heading (full.py:2404-2414) */
+        double h[3];
+        heading_of(b->rot, h);
+        cross(to, h, o);
+        length2 = lensq(o);
+        if (length2 <= 1e-12) {
+            precessing_w(p, b, normal);
+        } else {
+            double inv = 1.0 / sqrt(length2);
+            for (int i = 0; i < 3; ++i) normal[i] = o[i] * inv;
+        }
     }
     double rc[3], radial[3];
     cross(normal, to, rc);
@@ -523,8 +1017,9 @@ static void behaviour(OPark* p, OBall* b) {                              /* mode
         if (p->use_new_orbit) evolve_new_orbit(p, b, a);
         else evolve_old_orbit(p, b, a);
     } else if (m == STOP) {
-        b->vel[1] = (b->vel[1] - 0.07 * b->vel[1]) * STOP_DAMP;
+        if (!p->dynamical) b->vel[1] = (b->vel[1] - 0.07 * b->vel[1]) * STOP_DAMP;
     }
+    if (p->dynamical) orient(p, b, a);                              /* full.py:2441-2446 */
     b->last_g[0] = a[0], b->last_g[1] = a[1], b->last_g[2] = a[2];
 }
 
@@ -631,6 +1126,7 @@ void or_evolve(OPark* p) {                                              /* modes
             b->old_pos[k] = b->pos[k], b->old_vel[k] = b->vel[k];
             b->pos[k] = np[i][k], b->vel[k] = nv[i][k];
         }
+        if (p->dynamical) commit_rotation(p, b);
     }
     p->current_time += 1;                                               /* (7) */
 }
