@@ -29,6 +29,7 @@ XbPark host_park(const xb_modes_state& s) {
     P.ook2 = 1.0 / P.k2;
     P.ook = 1.0 / s.friction;
     P.new_orbit = s.new_orbit;
+    P.dynamical = s.dynamical;
     return P;
 }
 
@@ -59,15 +60,23 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
                int64_t cnt) {
     S.size((int)cnt);
     rows.resize((size_t)cnt);
-    for (int64_t k = 0; k < cnt; ++k) xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
+    const bool dyn = s.dynamical != 0;
+    for (int64_t k = 0; k < cnt; ++k) {
+        xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
+        if (dyn) xb_turn_load(P, rows[k].row, rows[k].t);
+    }
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
         for (int64_t k = 0; k < cnt; ++k) xb_record_a(rows[k], S.R, (int)k);
-        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt);
-        for (int64_t k = 0; k < cnt; ++k) xb_tick_row(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        for (int64_t k = 0; k < cnt; ++k) {
+            if (dyn) xb_tick_row<true>(rows[k], S.R, (int)k, 0, (int)cnt, P);
+            else xb_tick_row<false>(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        }
     }
     for (int64_t k = 0; k < cnt; ++k) {
         xb_row_store(P, rows[k]);
+        if (dyn && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
         if (s.path) s.path[rows[k].row] = rows[k].path;
     }
 }
@@ -81,7 +90,7 @@ int xb_cpu_abi(void) { return XB_ABI_VERSION; }
 // One tick of the park in host memory; tiles as xb_modes_tick's (NULL: the whole park is one group). threads <= 0:
 // OpenMP's default.
 int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
-    if (!state || state->n < 0 || state->stride < state->n || state->dynamical) return -1;
+    if (!state || state->n < 0 || state->stride < state->n) return -1;
     if (state->n == 0) return 0;
     const XbPark P = host_park(*state);
     if (!state->tiles) {
@@ -113,6 +122,30 @@ int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int6
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r >= 0 && r < state->n) xb_cmd_apply(P, state->n, r, op, point, target, range, min_range, warp_factor);
+    }
+    return 0;
+}
+
+int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what) {
+    if (!state || !state->f64 || state->n < 0 || state->stride < state->n || n_rows < 0 || what < 0 || what > 1 ||
+        (!rows && n_rows > state->n))
+        return -1;
+    const XbPark P = host_park(*state);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r < 0 || r >= state->n) continue;
+        XbTurn t;
+        xb_turn_load(P, r, t);
+        t.old_rot = xo_q((float)XB_F(P, XB_F_OLD_ROT, r), (float)XB_F(P, XB_F_OLD_ROT + 1, r),
+                         (float)XB_F(P, XB_F_OLD_ROT + 2, r), (float)XB_F(P, XB_F_OLD_ROT + 3, r));
+        t.old_w = xo_v((float)XB_F(P, XB_F_OLD_ANG_VEL, r), (float)XB_F(P, XB_F_OLD_ANG_VEL + 1, r),
+                       (float)XB_F(P, XB_F_OLD_ANG_VEL + 2, r));
+        t.torque = xo_v((float)XB_F(P, XB_F_TORQUE, r), (float)XB_F(P, XB_F_TORQUE + 1, r),
+                        (float)XB_F(P, XB_F_TORQUE + 2, r));
+        t.roll = (float)XB_F(P, XB_F_ROT_ROLL, r);
+        if (what == 1) t.old_rot = xo_new_ball_old_rot(t.rot, t.w, state->fdt);
+        xo_snap(xb_ld3(P, XB_F_VEL, r), t);
+        xb_turn_store(P, r, t);
     }
     return 0;
 }

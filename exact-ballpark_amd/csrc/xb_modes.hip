@@ -35,6 +35,7 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.ook2 = 1.0 / P.k2;
     P.ook = 1.0 / s.friction;
     P.new_orbit = s.new_orbit;
+    P.dynamical = s.dynamical;
     return P;
 }
 
@@ -84,6 +85,7 @@ __device__ __forceinline__ int xb_schedule_place(const xb_modes_state& s, const 
     return perm[j];
 }
 
+template <bool DO>
 __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
@@ -108,17 +110,21 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     XbRow w;
-    if (on) xb_row_load(P, r, w);
+    if (on) {
+        xb_row_load(P, r, w);
+        if (DO) xb_turn_load(P, r, w.t);
+    }
     for (int t = 0; t < ticks; ++t) {
         if (on) xb_record_a(w, R, j);
         __syncthreads();
-        if (on) xb_record_b(w, R, j, 0, cnt);
+        if (on) xb_record_b(w, R, j, 0, cnt, P);
         __syncthreads();
-        if (on) xb_tick_row(w, R, j, 0, cnt, P);
+        if (on) xb_tick_row<DO>(w, R, j, 0, cnt, P);
         __syncthreads();
     }
     if (on) {
         xb_row_store(P, w);
+        if (DO && w.turned) xb_turn_store(P, r, w.t);
         if (s.path) s.path[r] = w.path;
     }
 }
@@ -158,18 +164,43 @@ __global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
     if (r >= s.n) return;
     XbRow w;
     xb_row_load(xb_park(s), r, w);
-    xb_record_b(w, xb_global_records(s), (int)r, 0, (int)s.n);
+    xb_record_b(w, xb_global_records(s), (int)r, 0, (int)s.n, xb_park(s));
 }
 
+template <bool DO>
 __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= s.n) return;
     const XbPark P = xb_park(s);
     XbRow w;
     xb_row_load(P, r, w);
-    xb_tick_row(w, xb_global_records(s), (int)r, 0, (int)s.n, P);
+    if (DO) xb_turn_load(P, r, w.t);
+    xb_tick_row<DO>(w, xb_global_records(s), (int)r, 0, (int)s.n, P);
     xb_row_store(P, w);
+    if (DO && w.turned) xb_turn_store(P, r, w.t);
     if (s.path) s.path[r] = w.path;
+}
+
+// Dynamical orientation's set-up of the listed rows (rows NULL: rows 0 .. n_rows - 1): what 0 SnapOrientation to the
+// row's velocity; 1 a new ball's old rotation (rot x RotationQuaternion(angVel, -dt |angVel|)), then the snap.
+__global__ void xb_orient_setup_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows, int what) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    const XbPark P = xb_park(s);
+    XbTurn t;
+    xb_turn_load(P, r, t);
+    t.old_rot = xo_q((float)XB_F(P, XB_F_OLD_ROT, r), (float)XB_F(P, XB_F_OLD_ROT + 1, r),
+                     (float)XB_F(P, XB_F_OLD_ROT + 2, r), (float)XB_F(P, XB_F_OLD_ROT + 3, r));
+    t.old_w = xo_v((float)XB_F(P, XB_F_OLD_ANG_VEL, r), (float)XB_F(P, XB_F_OLD_ANG_VEL + 1, r),
+                   (float)XB_F(P, XB_F_OLD_ANG_VEL + 2, r));
+    t.torque = xo_v((float)XB_F(P, XB_F_TORQUE, r), (float)XB_F(P, XB_F_TORQUE + 1, r),
+                    (float)XB_F(P, XB_F_TORQUE + 2, r));
+    t.roll = (float)XB_F(P, XB_F_ROT_ROLL, r);
+    if (what == 1) t.old_rot = xo_new_ball_old_rot(t.rot, t.w, s.fdt);
+    xo_snap(xb_ld3(P, XB_F_VEL, r), t);
+    xb_turn_store(P, r, t);
 }
 
 // One lane per listed row (rows NULL: rows 0 .. n_rows - 1): the row's command of the batch, if any.
@@ -240,14 +271,15 @@ int xb_layout(int64_t* out) {
 int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     if (!state || state->n < 0 || state->stride < state->n || !xb_block_ok(block)) return -1;
     if (state->n == 0) return 0;
-    if (state->dynamical) return -1;                           // dynamical orientation: not in this build
     if (!state->f64 || !state->i64 || !state->u8) return -1;
+    const bool dyn = state->dynamical != 0;
     hipStream_t st = (hipStream_t)stream;
     if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
     if (state->tiles) {
         if (state->n_tiles <= 0) return -1;
         const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
-        xb_modes_tiled_kernel<<<state->n_tiles, block, lds, st>>>(*state);
+        if (dyn) xb_modes_tiled_kernel<true><<<state->n_tiles, block, lds, st>>>(*state);
+        else xb_modes_tiled_kernel<false><<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
     if (!state->scratch_f64 || !state->scratch_u8 || state->ticks > 1 || state->schedule) return -1;
@@ -258,7 +290,18 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     xb_modes_step_b<<<grid, block, 0, st>>>(*state);
     rc = xb_launch_rc();
     if (rc) return rc;
-    xb_modes_step_c<<<grid, block, 0, st>>>(*state);
+    if (dyn) xb_modes_step_c<true><<<grid, block, 0, st>>>(*state);
+    else xb_modes_step_c<false><<<grid, block, 0, st>>>(*state);
+    return xb_launch_rc();
+}
+
+int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what, void* stream) {
+    if (!state || !state->f64 || state->n < 0 || state->stride < state->n || n_rows < 0 || what < 0 || what > 1 ||
+        (!rows && n_rows > state->n))
+        return -1;
+    if (n_rows == 0) return 0;
+    xb_orient_setup_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows,
+                                                                                               what);
     return xb_launch_rc();
 }
 
@@ -284,8 +327,9 @@ int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* strea
 int xb_modes_info(int which, int64_t* out) {
     hipFuncAttributes fa;
     hipError_t e;
-    if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel));
-    else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c));
+    if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel<false>));
+    else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c<false>));
+    else if (which == 3) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel<true>));
     else if (which == 2) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_a));
     else return -1;
     if (e != hipSuccess) return (int)e;

@@ -96,6 +96,7 @@ struct XbPark {
     double friction, dt;
     double k2, ook2, ook;          /* friction^2, 1/friction^2, 1/friction: the integrator's per-park constants */
     int new_orbit;
+    int dynamical;                 /* dynamical orientation (MOV-728) */
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -125,6 +126,8 @@ struct XbShip {
     int64_t id, mode, effect_stamp, follow_id, owner_id, follow_ptr, clock;
     bool massive, interactive, sensor;
 };
+
+#include "xb_orient.h"
 
 /* ------------------------------------------------------------------------------------------- modes */
 
@@ -235,9 +238,11 @@ static inline XB_HD XbV3 xb_goto_thrust_follow(const XbShip& s, XbV3 target, XbV
     return a;
 }
 
-/* EvolveNewStyleOrbit with GetOrbitalNormal, dynamical orientation off (modes.py:488-555); ov: the target's velocity
-   as this ball sees it at its own turn */
-static inline XB_HD XbV3 xb_new_orbit(XbShip& s, XbV3 op, XbV3 ov, double other_radius, const XbPark& P) {
+/* EvolveNewStyleOrbit with GetOrbitalNormal (modes.py:488-555); ov: the target's velocity as this ball sees it at its
+   own turn; rot: with dynamical orientation the ball's tick-start rotation, whose heading gives the normal when the
+   velocity's does not (full.py:2385-2414), else NULL */
+static inline XB_HD XbV3 xb_new_orbit(XbShip& s, XbV3 op, XbV3 ov, double other_radius, const XbPark& P,
+                                      const XoQ* rot) {
     double cruise = s.sf * s.max_vel;
     double tau = (s.mass * s.agility) / P.friction;
     double max_thrust = cruise / tau;
@@ -249,7 +254,17 @@ static inline XB_HD XbV3 xb_new_orbit(XbShip& s, XbV3 op, XbV3 ov, double other_
     else to = v3(0.0, 0.0, 1.0);
     XbV3 o = v3_cross(to, s.vel);
     double length2 = v3_lensq(o);
-    XbV3 normal = (length2 > XB_NORMAL_MIN) ? v3_scale(o, 1.0 / __builtin_sqrt(length2)) : xb_precessing(s);
+    XbV3 normal;
+    if (length2 > XB_NORMAL_MIN) {
+        normal = v3_scale(o, 1.0 / __builtin_sqrt(length2));
+    } else if (!rot) {
+        normal = xb_precessing(s);
+    } else {
+        const XoV3 h = xo_heading(*rot);
+        o = v3_cross(to, v3((double)h.x, (double)h.y, (double)h.z));
+        length2 = v3_lensq(o);
+        normal = (length2 <= XB_NORMAL_MIN) ? xb_precessing(s) : v3_scale(o, 1.0 / __builtin_sqrt(length2));
+    }
     XbV3 radial = v3_unit(v3_cross(normal, to));
     double speed = v3_len(s.vel);
     double max_speed = (cruise < speed) ? speed : cruise;
@@ -425,6 +440,8 @@ static inline XB_HD void xb_integrate(XbV3* pp, XbV3* vp, XbV3 a, double m, doub
    once. The flags are the row's own (free, moribund, interactive, massive, sensor). */
 struct XbRow {
     XbShip s;
+    XbTurn t;                      /* with dynamical orientation: the turning state */
+    bool turned;                   /* the turning state was committed (stored back then) */
     XbV3 old_pos, old_vel;
     bool free_, moribund;
     uint8_t path;                  /* the path class its last tick executed (XB_PATH_*) */
@@ -483,6 +500,7 @@ static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
     w.path = XB_PATH_IDLE;
     w.row = r;
+    w.turned = false;
 }
 
 /* Every field the tick may change (radius, mass, agility, maximum speed, decay, ids, flags moribund: never) */
@@ -510,6 +528,28 @@ static inline XB_HD void xb_row_store(const XbPark& P, const XbRow& w) {
     XB_B(P, XB_B_SENSOR, r) = s.sensor ? 1 : 0;
 }
 
+/* The turning state (dynamical orientation): floats held in the float64 block rows */
+static inline XB_HD void xb_turn_load(const XbPark& P, int64_t r, XbTurn& t) {
+    t.rot = xo_q((float)XB_F(P, XB_F_ROT, r), (float)XB_F(P, XB_F_ROT + 1, r), (float)XB_F(P, XB_F_ROT + 2, r),
+                 (float)XB_F(P, XB_F_ROT + 3, r));
+    t.w = xo_v((float)XB_F(P, XB_F_ANG_VEL, r), (float)XB_F(P, XB_F_ANG_VEL + 1, r),
+               (float)XB_F(P, XB_F_ANG_VEL + 2, r));
+    t.max_ang_vel = (float)XB_F(P, XB_F_MAX_ANG_VEL, r);
+    t.rot_agility = (float)XB_F(P, XB_F_ROT_AGILITY, r);
+}
+static inline XB_HD void xb_turn_store(const XbPark& P, int64_t r, const XbTurn& t) {
+    XB_F(P, XB_F_ROT, r) = t.rot.x, XB_F(P, XB_F_ROT + 1, r) = t.rot.y;
+    XB_F(P, XB_F_ROT + 2, r) = t.rot.z, XB_F(P, XB_F_ROT + 3, r) = t.rot.w;
+    XB_F(P, XB_F_OLD_ROT, r) = t.old_rot.x, XB_F(P, XB_F_OLD_ROT + 1, r) = t.old_rot.y;
+    XB_F(P, XB_F_OLD_ROT + 2, r) = t.old_rot.z, XB_F(P, XB_F_OLD_ROT + 3, r) = t.old_rot.w;
+    XB_F(P, XB_F_ANG_VEL, r) = t.w.x, XB_F(P, XB_F_ANG_VEL + 1, r) = t.w.y, XB_F(P, XB_F_ANG_VEL + 2, r) = t.w.z;
+    XB_F(P, XB_F_OLD_ANG_VEL, r) = t.old_w.x, XB_F(P, XB_F_OLD_ANG_VEL + 1, r) = t.old_w.y;
+    XB_F(P, XB_F_OLD_ANG_VEL + 2, r) = t.old_w.z;
+    XB_F(P, XB_F_TORQUE, r) = t.torque.x, XB_F(P, XB_F_TORQUE + 1, r) = t.torque.y;
+    XB_F(P, XB_F_TORQUE + 2, r) = t.torque.z;
+    XB_F(P, XB_F_ROT_ROLL, r) = t.roll;
+}
+
 static inline XB_HD bool xb_row_live(const XbRow& w) { return w.free_ && !w.moribund; }
 
 /* Step A for the row at place j of its group */
@@ -534,8 +574,8 @@ static inline XB_HD int xb_member_place(const XbRow& w, const XbRecords& R, int 
     return R.id[tj] == w.s.follow_id ? (int)tj : -1;
 }
 
-/* Step B: the vertical velocity the row shows after its own turn (no dynamical orientation) */
-static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, int lo, int hi) {
+/* Step B: the vertical velocity the row shows after its own turn (STOP's damping, without dynamical orientation) */
+static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
     double vy = w.s.vel.y;
     if (xb_row_live(w)) {
         int64_t m = w.s.mode;
@@ -548,15 +588,18 @@ static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, 
             f.vel = w.s.vel;
             m = xb_release(f, false);
         }
-        if (m == XB_MODE_STOP) vy = (vy - XB_STOP_DECAY * vy) * XB_STOP_DAMP;
+        if (m == XB_MODE_STOP && !P.dynamical) vy = (vy - XB_STOP_DECAY * vy) * XB_STOP_DAMP;
     }
     R.vy_after[j] = vy;
 }
 
 /* Step C: the row's whole tick (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi). Other rows are
    read only through the records. */
+template <bool DO>
 static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
     XbShip& s = w.s;
+    XoQ rot1 = xo_q(0.0f, 0.0f, 0.0f, 1.0f);     /* with dynamical orientation: the rotation and angular velocity */
+    XoV3 w1 = xo_v(0.0f, 0.0f, 0.0f);           /* a tick on */
     const bool live = xb_row_live(w);
     const bool act_self = R.act[j] != 0;
     /* phase 1: a release when the target starts warp proper, before or after this ball's own turn by their ids */
@@ -596,13 +639,14 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
             XbV3 op = v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]);
             if (P.new_orbit) {
                 XbV3 ov = v3(R.v0x[tp], (R.id[tp] < s.id) ? R.vy_after[tp] : R.v0y[tp], R.v0z[tp]);
-                a = xb_new_orbit(s, op, ov, R.radius[tp], P);
+                a = xb_new_orbit(s, op, ov, R.radius[tp], P, DO ? &w.t.rot : (const XoQ*)0);
             } else {
                 a = xb_old_orbit(s, op, R.radius[tp], P);
             }
         } else if (s.mode == XB_MODE_STOP) {
-            s.vel.y = (s.vel.y - XB_STOP_DECAY * s.vel.y) * XB_STOP_DAMP;
+            if (!DO) s.vel.y = (s.vel.y - XB_STOP_DECAY * s.vel.y) * XB_STOP_DAMP;
         }
+        if (DO) a = xo_orient(s, w.t, a, P.friction, P.dt, &rot1, &w1);
         s.last_g = a;
     }
     if (rel_after) xb_release(s, true);
@@ -627,6 +671,10 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         w.old_vel = s.vel;
         s.pos = p;
         s.vel = v;
+        if (DO) {
+            xo_commit(w.t, rot1, w1);
+            w.turned = true;
+        }
     }
     if (live && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
         /* PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937): fixed, at rest, RIGID */

@@ -109,8 +109,6 @@ class VecModesPark(metaclass=_Fields):
     def __init__(self, n: int, friction: float = 1.0e6, tick_interval_ms: int = 1000, device="cuda",
                  dynamical_orientation: bool = False, use_new_orbit: bool = False, block: int = 256,
                  ypr: str = "host"):
-        if dynamical_orientation:
-            raise NotImplementedError("dynamical orientation is not in this build of the flight-modes kernel")
         if ypr != "host":
             raise ValueError('ypr must be "host" (the yaw, pitch and roll are display values, not kept here)')
         self.n = int(n)
@@ -119,7 +117,7 @@ class VecModesPark(metaclass=_Fields):
         self.dt = tick_interval_ms * 0.001
         self.fdt = float(torch.tensor(self.dt, dtype=F64).to(torch.float32))
         self.ffriction = float(torch.tensor(self.friction, dtype=F64).to(torch.float32))
-        self.dynamical_orientation = False
+        self.dynamical_orientation = bool(dynamical_orientation)
         self.use_new_orbit = bool(use_new_orbit)
         self.block = int(block)
         self.dev = torch.device(device)
@@ -186,10 +184,11 @@ class VecModesPark(metaclass=_Fields):
             s.n, s.stride = self.n, self.stride
             s.new_orbit = int(self.use_new_orbit)
             s.friction, s.dt, s.ffriction, s.fdt = self.friction, self.dt, self.ffriction, self.fdt
-            s.dynamical, s.ticks = 0, 1
+            s.ticks = 1
             self._state = s
         s = self._state
         s.new_orbit = int(self.use_new_orbit)
+        s.dynamical = int(self.dynamical_orientation)
         tiled = self.tiles is not None
         s.tiles = self.tiles.data_ptr() if tiled else None
         s.n_tiles = (len(self.tiles) - 1) if tiled else 0
@@ -219,7 +218,7 @@ class VecModesPark(metaclass=_Fields):
     def load(self, rows: list) -> None:
         """Every ball as the Python AddBall adds a new one (vec_modes.py:221-282): keys id, world, time (the world's
         clock), mass, radius, max_vel, pos, vel, agility, speed_fraction, free, massive, interactive, cloaked, bubble,
-        moribund, mode."""
+        moribund, mode; with dynamical orientation also max_ang_vel (default 1) and rot_agility (default 0.9)."""
         n = len(rows)
         if n != self.n:
             raise ValueError(f"{n} rows for a park of {self.n}")
@@ -238,13 +237,18 @@ class VecModesPark(metaclass=_Fields):
             interactive=t([bool(x) for x in g("interactive", False)], torch.bool),
             cloaked=t([int(x) for x in g("cloaked", 0)], I64), bubble=t([int(x) for x in g("bubble", -1)], I64),
             moribund=t([bool(x) for x in g("moribund", False)], torch.bool),
-            mode=t([int(x) for x in g("mode", STOP)], I64))
+            mode=t([int(x) for x in g("mode", STOP)], I64),
+            max_ang_vel=t([float(x) for x in g("max_ang_vel", 1.0)]),
+            rot_agility=t([float(x) for x in g("rot_agility", 0.9)]))
 
     def load_arrays(self, *, ids, world, time, mass, radius, max_vel, pos, vel, agility, speed_fraction, free,
-                    massive, interactive, cloaked, bubble, moribund, mode) -> None:
+                    massive, interactive, cloaked, bubble, moribund, mode, max_ang_vel=None, rot_agility=None) -> None:
         """`load` from [n] and [n, 3] tensors: AddBall's clamps (the floats rounded to float, then `x < 0 ? 0 : x`,
         agility `<= 0 ? 1`, NaN and -0 kept), the decay factor, Ball::Ball's defaults (follow range 10, warp length
-        -1, no follow fields, the sensor off, the turning values 1 rad/s and 0.9)."""
+        -1, no follow fields, the sensor off, the turning values 1 rad/s and 0.9). With dynamical orientation
+        AddDynamicallyOrientedBall's turning values (max_ang_vel `< 0 ? 0`, rot_agility `<= 0 ? 1`, as floats; default
+        1 and 0.9), the identity rotation at rest, its old rotation, and the rotation snapped to the velocity
+        (src/exact_ballpark/vec_modes.py:252-264)."""
         dev = self.dev
         c = lambda x, dt=F64: torch.as_tensor(x).to(device=dev, dtype=dt)
         if int(((c(mode, I64) != STOP) & ~torch.isin(c(mode, I64), torch.tensor(INERT_MODES, device=dev))).sum()):
@@ -268,8 +272,14 @@ class VecModesPark(metaclass=_Fields):
         self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=dev).expand(self.n, 4)
         self.old_rot = self.rot.clone()
         self.ang_vel, self.old_ang_vel, self.torque, self.rot_roll = 0.0, 0.0, 0.0, 0.0
-        self.max_ang_vel = 1.0
-        self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
+        if self.dynamical_orientation:
+            x = f32(c(1.0 if max_ang_vel is None else max_ang_vel).expand(self.n))
+            self.max_ang_vel = torch.where(x < 0.0, 0.0, x)
+            x = f32(c(0.9 if rot_agility is None else rot_agility).expand(self.n))
+            self.rot_agility = torch.where(x <= 0.0, 1.0, x)
+        else:
+            self.max_ang_vel = 1.0
+            self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
         self.ids, self.world, self.clock, self.mode = c(ids, I64), c(world, I64), c(time, I64), c(mode, I64)
         self.free, self.massive = c(free, torch.bool), c(massive, torch.bool)
         self.interactive, self.moribund = c(interactive, torch.bool), c(moribund, torch.bool)
@@ -280,6 +290,8 @@ class VecModesPark(metaclass=_Fields):
         self.follow_range = 10.0
         self.last_collision = -1.0
         self._set_tiles()
+        if self.dynamical_orientation:
+            self._orient_setup(None, 1)
 
     # ---- the follower bookkeeping: Stop(Ball*) (Ballpark.cpp:4578; vec_modes.py:286-311) ----
 
@@ -409,6 +421,8 @@ class VecModesPark(metaclass=_Fields):
         self.massive = torch.where(go, False, self.massive)
         self.sensor = torch.where(go, False, self.sensor)
         self.vel = torch.where(go[:, None], _unit(point) * AU, self.vel)
+        if self.dynamical_orientation:                  # SnapOrientation to the new velocity (full.py:1574-1581)
+            self._orient_setup(go, 0)
         self.effect_stamp = torch.where(go, torch.clamp(self.clock - 5, min=0), self.effect_stamp)
         back = self.pos - self.goto
         self.last_collision = torch.where(go, _sqrt(_dot(back, back)), self.last_collision)
@@ -446,9 +460,66 @@ class VecModesPark(metaclass=_Fields):
         self.refiled += ok.to(I64)
 
     def set_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
+        """SetBallVelocity: the new and old velocity; with dynamical orientation the rotation snapped to it."""
         v = v.to(device=self.dev, dtype=F64)
         self.vel = torch.where(mask[:, None], v, self.vel)
         self.old_vel = torch.where(mask[:, None], v, self.old_vel)
+        if self.dynamical_orientation:
+            self._orient_setup(mask, 0)
+
+    # ---- dynamical orientation's setters (vec_orient.py:438-464) ----
+
+    def _orient_setup(self, mask: torch.Tensor | None, what: int) -> None:
+        """xb_modes_orient_setup on the rows in mask (None: every row): 0 SnapOrientation, 1 a new ball's set-up."""
+        if self.n == 0:
+            return
+        rows = None
+        if mask is not None:
+            rows = torch.nonzero(mask.to(self.dev).reshape(self.n)).reshape(-1).to(torch.int32)
+            if rows.numel() == 0:
+                return
+        nr = self.n if rows is None else rows.numel()
+        s = self.state()
+        ptr = None if rows is None else rows.data_ptr()
+        if self.dev.type == "cuda":
+            with torch.cuda.device(self.dev):
+                rc = N.hip().xb_modes_orient_setup(ctypes.byref(s), ptr, nr, int(what),
+                                                   torch.cuda.current_stream(self.dev).cuda_stream)
+            N.check(rc, "xb_modes_orient_setup")
+        else:
+            N.check(N.cpu().xb_cpu_modes_orient_setup(ctypes.byref(s), ptr, nr, int(what)),
+                    "xb_cpu_modes_orient_setup")
+
+    def set_max_angular_speed(self, mask: torch.Tensor, w: torch.Tensor) -> None:
+        """SetMaxAngularSpeed: the float ignored if negative."""
+        w = f32(w.to(device=self.dev, dtype=F64))
+        self.max_ang_vel = torch.where(mask & ~(w < 0.0), w, self.max_ang_vel)
+
+    def set_rot_agility(self, mask: torch.Tensor, a: torch.Tensor) -> None:
+        """SetBallAngularAgility: the float ignored unless positive."""
+        a = f32(a.to(device=self.dev, dtype=F64))
+        self.rot_agility = torch.where(mask & ~(a <= 0.0), a, self.rot_agility)
+
+    def set_angular_velocity(self, mask: torch.Tensor, w: torch.Tensor) -> None:
+        """SetBallAngularVelocity: the new and old angular velocity, as floats."""
+        w = f32(w.to(device=self.dev, dtype=F64))
+        self.ang_vel = torch.where(mask[:, None], w, self.ang_vel)
+        self.old_ang_vel = torch.where(mask[:, None], w, self.old_ang_vel)
+
+    def set_rotation(self, mask: torch.Tensor, q: torch.Tensor) -> None:
+        """SetBallRotation: the floats normalised when longer than 1e-20 (otherwise ignored), the rotation stopped.
+        Each float operation as a double operation rounded to float (the same for + - * / sqrt of floats)."""
+        r = f32(q.to(device=self.dev, dtype=F64))
+        x, y, z, w = r.unbind(1)
+        sq = f32(f32(f32(f32(x * x) + f32(y * y)) + f32(z * z)) + f32(w * w))
+        length = f32(_sqrt(sq))
+        ok = mask & (length > 1e-20)
+        d = f32(torch.ones_like(length) / torch.where(ok, length, 1.0))
+        r = f32(r * d[:, None])
+        self.rot = torch.where(ok[:, None], r, self.rot)
+        self.old_rot = torch.where(ok[:, None], r, self.old_rot)
+        self.ang_vel = torch.where(ok[:, None], 0.0, self.ang_vel)
+        self.old_ang_vel = torch.where(ok[:, None], 0.0, self.old_ang_vel)
 
     def set_free(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
         """SetBallFree (vec_modes.py:438-451)."""

@@ -52,7 +52,9 @@ typedef struct xb_modes_state {
     double dt;
     float ffriction;
     float fdt;
-    int32_t dynamical;
+    int32_t dynamical;             /* dynamical orientation (MOV-728): the turning rows of f64 (rot, ang_vel, max_ang_vel,
+                                      rot_agility in; old_rot, old_ang_vel, torque, rot_roll out), floats held in
+                                      doubles; STOP without its damping; the new orbit's normal from the heading */
     int32_t ticks;                 /* ticks per launch (tiled shape only; 0 or 1: one) */
     int32_t schedule;              /* lanes of a tile: 0 in row order; 1 sorted by the built-in branch key;
                                       2 sorted by keys[row] (tiled shape only) */
@@ -82,6 +84,13 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
    (src/exact_ballpark/vec_modes_cuda.cu:779). */
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block);
 
+/* Dynamical orientation's set-up of the listed rows (rows NULL: rows 0 .. n_rows - 1), in place: what 0
+   SnapOrientation (the rotation turned to the row's velocity and rolled upright, the rotation stopped; nothing below
+   1e-6 m/s), as SetBallVelocity and EntityWarpIn do it; what 1 a new ball's old rotation, rot x RotationQuaternion(
+   angVel, -dt |angVel|), then the snap, as the Python AddBall does it. Replaces the batched _snap_orientation and the
+   load's turning set-up (src/exact_ballpark/vec_orient.py:556-575, src/exact_ballpark/vec_modes.py:256-264). */
+int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what, void* stream);
+
 /* One flight-mode command per listed row, applied in place between ticks: for each row r of rows[0 .. n_rows - 1]
    (rows NULL: rows 0 .. n_rows - 1), op[r] 0 none, 3 Stop, 1 GotoPoint(point[r][3]), 4 Follow and 5 Orbit (target row
    target[r], range[r]), 6 WarpTo(point[r], min_range[r], warp_factor[r]); a row listed twice is a race. The result is
@@ -92,7 +101,8 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
                       const double* point, const int64_t* target, const double* range, const double* min_range,
                       const int64_t* warp_factor, void* stream);
 
-/* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass; out[0..3] =
+/* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass, 3 tiled with
+   dynamical orientation; out[0..3] =
    scalar registers, vector registers, private (scratch) bytes per lane, LDS bytes. Replaces xbpm_info
    (src/exact_ballpark/vec_modes_cuda.cu:766). */
 int xb_modes_info(int which, int64_t* out);
@@ -160,6 +170,9 @@ int xb_cpu_abi(void);
 /* xb_modes_tick on a park in host memory (OpenMP over the tiles; threads <= 0: OpenMP's default). The CPU batched
    path, in place of the reference's torch tick on the CPU (src/exact_ballpark/vec_modes.py:943-960). */
 int xb_cpu_modes_tick(const xb_modes_state* state, int threads);
+
+/* xb_modes_orient_setup on a park in host memory (no stream). */
+int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what);
 
 /* xb_modes_commands on a park in host memory (no stream). */
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,

@@ -21,6 +21,8 @@ from exact_ballpark_amd.vec_modes import FOLLOW, ORBIT, ModesGraph, VecModesPark
 F64, I64 = torch.float64, torch.int64
 # the oracle's state columns the park is compared on
 D_COLS = list(range(26))                  # pos .. last_collision (the pad left out)
+TURN_NAMES = ("rx", "ry", "rz", "rw", "orx", "ory", "orz", "orw", "wx", "wy", "wz", "owx", "owy", "owz", "tx", "ty",
+              "tz", "rot_roll", "max_ang_vel", "rot_agility")
 Q_COLS = list(range(11))                  # id .. moribund
 
 
@@ -54,6 +56,16 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.warp_to(mask, xyz, rr, w)
     elif op == S.WARPIN_OP:
         park.entity_warp_in(mask, xyz, w)
+    elif op == S.VEL_OP:
+        park.set_velocity(mask, xyz)
+    elif op == S.ROT_OP:
+        park.set_rotation(mask, torch.cat((xyz, rr[:, None]), dim=1))
+    elif op == S.ANGVEL_OP:
+        park.set_angular_velocity(mask, xyz)
+    elif op == S.MAXANG_OP:
+        park.set_max_angular_speed(mask, rr)
+    elif op == S.ROTAG_OP:
+        park.set_rot_agility(mask, rr)
     else:
         raise ValueError(op)
 
@@ -78,6 +90,10 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
                                        f[20, r]])) for r in rows])
         qq = np.stack([[ids[r], mode[r], q[2, r], q[3, r], q[4, r], ids[fptr[r]] if fptr[r] >= 0 else -1, b[2, r],
                         b[4, r], b[3, r], b[0, r], b[1, r]] for r in rows]).astype(np.int64)
+        if park.dynamical_orientation:
+            t = np.stack([np.concatenate((f[29:33, r], f[33:37, r], f[37:40, r], f[40:43, r], f[43:46, r],
+                                          [f[28, r], f[26, r], f[27, r]])) for r in rows])
+            d = np.concatenate((d, t), axis=1)
         out[wi] = (d, qq, [sorted(fol.get(r, [])) for r in rows])
     return out
 
@@ -89,17 +105,18 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
     groups = defaultdict(list)
     for wi in (order if order is not None else range(len(worlds))):
         w = worlds[wi]
-        groups[(w["friction"], w["tick"], bool(w["new_orbit"]))].append(wi)
+        groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)))].append(wi)
     out = {}
-    for (friction, tick, new_orbit), members in groups.items():
+    for (friction, tick, new_orbit, dynamical), members in groups.items():
         balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
-                            use_new_orbit=new_orbit, block=block)
+                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical)
         park.load([{"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                     "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
                     "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
-                    "moribund": b["moribund"], "mode": b["mode"]} for wi, b in balls])
+                    "moribund": b["moribund"], "mode": b["mode"], "max_ang_vel": b.get("max_ang_vel", 1.0),
+                    "rot_agility": b.get("rot_agility", 0.9)} for wi, b in balls])
         if general:
             park.tiles = None
             park._state = None
@@ -133,12 +150,30 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
     return out
 
 
+def _run_turning(w: dict):
+    """O.run_world for a world with dynamical orientation: the doubles with the turning state appended."""
+    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], True)
+    for b in w["balls"]:
+        o.add(b)
+    by_step = defaultdict(list)
+    for c in w["cmds"]:
+        by_step[c[0]].append(c)
+    for step in range(1, w["steps"] + 1):
+        for c in by_step[step]:
+            o.command(c)
+        o.evolve()
+        d, q, fol = o.state()
+        yield step, np.concatenate((d[:, D_COLS], o.turning()), axis=1), q, fol      # 46 columns
+
+
 def run_oracle(worlds: list, indices, record=None) -> dict:
     out = {}
     for wi in indices:
-        for step, d, q, fol in O.run_world(worlds[wi]):
+        w = worlds[wi]
+        runs = _run_turning(w) if w.get("dynamical") else O.run_world(w)
+        for step, d, q, fol in runs:
             if record is None or step in record:
-                out[(wi, step)] = (d[:, D_COLS], q[:, Q_COLS], fol)
+                out[(wi, step)] = (d if w.get("dynamical") else d[:, D_COLS], q[:, Q_COLS], fol)
     return out
 
 
@@ -167,7 +202,7 @@ def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
         if same_d.all() and same_q.all() and gf == wf:
             continue
         if len(bad) < limit:
-            names_d = [O.D_NAMES[c] for c in D_COLS]
+            names_d = [O.D_NAMES[c] for c in D_COLS] + list(TURN_NAMES)
             names_q = [O.Q_NAMES[c] for c in Q_COLS]
             diffs = [f"{names_d[c]}[{k}] {gd[k, c]!r} vs {wd[k, c]!r}" for k, c in zip(*np.nonzero(~same_d))][:6]
             diffs += [f"{names_q[c]}[{k}] {gq[k, c]} vs {wq[k, c]}" for k, c in zip(*np.nonzero(~same_q))][:6]

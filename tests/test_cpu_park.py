@@ -41,3 +41,20 @@ def test_cpu_resident_ticks_equal_single_ticks():
     for x, y in zip((a._f, a._q, a._b), (b._f, b._q, b._b)):
         assert torch.equal(x.view(torch.int64) if x.dtype == torch.float64 else x,
                            y.view(torch.int64) if y.dtype == torch.float64 else y)
+
+
+@pytest.fixture(scope="module")
+def turning():
+    g = G.load("orient_golden.npz")
+    return g, D.run_oracle(g["worlds"], g["kept"])
+
+
+@pytest.mark.parametrize("general", [False, True], ids=["tiled", "general"])
+def test_cpu_park_dynamical_orientation_matches_oracle(turning, general):
+    """The turning worlds (dynamical orientation, its setters and snaps) through the park on the host build, every
+    field including the rotation, angular velocity, torque and roll, against the oracle (itself held to the reference's
+    FullBallpark vectors)."""
+    g, want = turning
+    got = D.run_park(g["worlds"], "cpu", general=general)
+    bad = D.compare(got, want, g["worlds"])
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
