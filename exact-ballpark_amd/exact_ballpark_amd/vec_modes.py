@@ -74,6 +74,23 @@ def _unit(d: torch.Tensor) -> torch.Tensor:
     return torch.where((norm == 0.0)[:, None], d, d * inv[:, None])
 
 
+def rotation_ypr(yaw: torch.Tensor, pitch: torch.Tensor, roll: torch.Tensor) -> torch.Tensor:
+    """RotationQuaternion(yaw, pitch, roll) of the floats, [n, 4] (x, y, z, w) floats held in doubles: DirectXMath's
+    XMQuaternionRotationRollPitchYaw, its scalar path (the reference's q_rotation_ypr, src/exact_ballpark/vec_modes.py:
+    184-190, full.py:268-280): each half angle, glibc's cosf and sinf of it, the four sums of float products left to
+    right. Each float operation as a double operation rounded to float (the same for * + - of floats)."""
+    yaw, pitch, roll = (f32(x.to(F64)) for x in (yaw, pitch, roll))
+    hp, hy, hr = f32(pitch * 0.5), f32(yaw * 0.5), f32(roll * 0.5)
+    cp, sp = N.maths("cosf", hp), N.maths("sinf", hp)
+    cy, sy = N.maths("cosf", hy), N.maths("sinf", hy)
+    cr, sr = N.maths("cosf", hr), N.maths("sinf", hr)
+    x = f32(f32(f32(cr * sp) * cy) + f32(f32(sr * cp) * sy))
+    y = f32(f32(f32(cr * cp) * sy) - f32(f32(sr * sp) * cy))
+    z = f32(f32(f32(sr * cp) * cy) - f32(f32(cr * sp) * sy))
+    w = f32(f32(f32(cr * cp) * cy) + f32(f32(sr * sp) * sy))
+    return torch.stack((x, y, z, w), dim=1)
+
+
 def _field_property(name: str, blk: str, row: int, cols: int):
     def get(self):
         if blk == "f":
@@ -505,6 +522,11 @@ class VecModesPark(metaclass=_Fields):
         w = f32(w.to(device=self.dev, dtype=F64))
         self.ang_vel = torch.where(mask[:, None], w, self.ang_vel)
         self.old_ang_vel = torch.where(mask[:, None], w, self.old_ang_vel)
+
+    def set_rotation_ypr(self, mask: torch.Tensor, ypr: torch.Tensor) -> None:
+        """SetBallRotation(RotationQuaternion(yaw, pitch, roll)) with ypr [n, 3] (yaw, pitch, roll; each as a float)."""
+        ypr = ypr.to(device=self.dev, dtype=F64)
+        self.set_rotation(mask, rotation_ypr(ypr[:, 0], ypr[:, 1], ypr[:, 2]))
 
     def set_rotation(self, mask: torch.Tensor, q: torch.Tensor) -> None:
         """SetBallRotation: the floats normalised when longer than 1e-20 (otherwise ignored), the rotation stopped.

@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
 
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
-VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP = 8, 9, 10, 11, 12
+VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, YPR_OP = 8, 9, 10, 11, 12, 13
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -62,6 +62,7 @@ def lib() -> ctypes.CDLL:
         L.or_set_rotation.argtypes = [vp, i64, d, d, d, d]
         L.or_set_max_angular_speed.argtypes = [vp, i64, d]
         L.or_set_rot_agility.argtypes = [vp, i64, d]
+        L.or_set_rotation_ypr.argtypes = [vp, i64, d, d, d]
         L.or_get_turning.argtypes = [vp, i, pd]
         _LIB = L
     return _LIB
@@ -121,6 +122,8 @@ class World:
             L.or_set_max_angular_speed(p, a, rr)
         elif op == ROTAG_OP:
             L.or_set_rot_agility(p, a, rr)
+        elif op == YPR_OP:
+            L.or_set_rotation_ypr(p, a, x, y, z)
         else:
             raise ValueError(op)
 

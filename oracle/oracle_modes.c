@@ -628,6 +628,18 @@ void or_set_rotation(OPark* p, int64_t id, double rx, double ry, double rz, doub
         for (int i = 0; i < 3; ++i) b->ang_vel[i] = b->old_ang_vel[i] = 0.0;
     }
 }
+/* RotationQuaternion(yaw, pitch, roll) (full.py:268-280: DirectXMath's XMQuaternionRotationRollPitchYaw, scalar
+   path) of the floats, into SetBallRotation */
+void or_set_rotation_ypr(OPark* p, int64_t id, double yaw, double pitch, double roll) {
+    yaw = F(yaw), pitch = F(pitch), roll = F(roll);
+    double halfpitch = F(pitch * 0.5), cp = cosf_(halfpitch), sp = sinf_(halfpitch);
+    double halfyaw = F(yaw * 0.5), cy = cosf_(halfyaw), sy = sinf_(halfyaw);
+    double halfroll = F(roll * 0.5), cr = cosf_(halfroll), sr = sinf_(halfroll);
+    double x = F(F(F(cr * sp) * cy) + F(F(sr * cp) * sy)), y = F(F(F(cr * cp) * sy) - F(F(sr * sp) * cy));
+    double z = F(F(F(sr * cp) * cy) - F(F(cr * sp) * sy)), w = F(F(F(cr * cp) * cy) + F(F(sr * sp) * sy));
+    or_set_rotation(p, id, x, y, z, w);
+}
+
 void or_set_rot_agility(OPark* p, int64_t id, double a) {
     a = F(a);
     OBall* b = find(p, id);

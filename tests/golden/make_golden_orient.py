@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Writes tests/golden/orient_golden.npz: the reference's own scalar ballpark with dynamical orientation
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=True)`, a client's view) run on the seeded turning
-worlds of tests/scenarios.py (`do_cases`), every ball's state after every tick, as bits.
+worlds of tests/scenarios.py (`do_cases`), every ball's state after every tick, as bits. Rotations given as yaw, pitch and roll go through the
+reference's RotationQuaternion(yaw, pitch, roll) (full.py:268-280, DirectXMath's scalar path) into SetBallRotation.
 
 Run only where the reference is mounted (/root/reference); the fixture it writes is committed and nothing else reads
 the reference. The reference takes the C library's float functions through ctypes (full.py:121-133) and its double
@@ -119,6 +120,8 @@ def _apply(park, c) -> None:
         park.set_max_angular_speed(a, rr)
     elif op == S.ROTAG_OP:
         park.set_ball_angular_agility(a, rr)
+    elif op == S.YPR_OP:                              # RotationQuaternion(yaw, pitch, roll) of the floats
+        park.set_ball_rotation(a, *full.q_rotation_ypr(full._f(x), full._f(y), full._f(z)))
     else:
         raise ValueError(op)
 

@@ -66,6 +66,8 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.set_max_angular_speed(mask, rr)
     elif op == S.ROTAG_OP:
         park.set_rot_agility(mask, rr)
+    elif op == S.YPR_OP:
+        park.set_rotation_ypr(mask, xyz)
     else:
         raise ValueError(op)
 

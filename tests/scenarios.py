@@ -222,6 +222,7 @@ def flight_park(seed: int, worlds: int, ships: int = 4) -> list:
 # ---- dynamical orientation (the turning worlds) ----
 
 VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP = 8, 9, 10, 11, 12     # SetBallVelocity, SetBallRotation, ...
+YPR_OP = 13                                                           # SetBallRotation(RotationQuaternion(y, p, r))
 
 
 def _do_ball(r: random.Random, bid: int, **kw) -> dict:
@@ -263,9 +264,11 @@ def do_cases(seed: int, scale: int = 1) -> list:
 
     def turning(bid):
         cmds = []
-        if r.random() < 0.4:
+        if r.random() < 0.3:
             q = _quat(r)
             cmds.append(_cmd(1, ROT_OP, bid, 0, q[0], q[1], q[2], rr=q[3]))
+        elif r.random() < 0.4:
+            cmds.append(_cmd(1, YPR_OP, bid, 0, *[r.uniform(-math.pi, math.pi) for _ in range(3)]))
         if r.random() < 0.4:
             w = [r.uniform(-0.5, 0.5) for _ in range(3)]
             cmds.append(_cmd(1, ANGVEL_OP, bid, 0, *w))
@@ -342,7 +345,7 @@ def do_cases(seed: int, scale: int = 1) -> list:
         cmds = []
         for _ in range(r.randint(2, 8)):
             bid = r.choice(ids)
-            op = r.choice([VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, GOTO_OP, STOP_OP])
+            op = r.choice([VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, GOTO_OP, STOP_OP, YPR_OP])
             step = r.randint(1, 6)
             if op == VEL_OP:
                 v = r.choice([[0.0, 0.0, 0.0], [r.uniform(-300, 300) for _ in range(3)],
@@ -353,6 +356,10 @@ def do_cases(seed: int, scale: int = 1) -> list:
                 cmds.append(_cmd(step, op, bid, 0, q[0], q[1], q[2], rr=q[3]))
             elif op == ANGVEL_OP:
                 cmds.append(_cmd(step, op, bid, 0, *[r.uniform(-1.0, 1.0) for _ in range(3)]))
+            elif op == YPR_OP:
+                ypr = r.choice([[r.uniform(-math.pi, math.pi) for _ in range(3)], [0.0, 0.0, 0.0],
+                                [r.uniform(-10.0, 10.0), 0.0, r.uniform(-1e4, 1e4)]])
+                cmds.append(_cmd(step, op, bid, 0, *ypr))
             elif op in (MAXANG_OP, ROTAG_OP):
                 cmds.append(_cmd(step, op, bid, 0, rr=r.choice([r.uniform(0.01, 3.0), 0.0, -r.uniform(0.1, 1.0)])))
             elif op == GOTO_OP:
