@@ -37,6 +37,8 @@ for _p in (ROOT / "exact-ballpark_amd", ROOT / "oracle", ROOT / "tests"):
 
 METRIC = "bit-exact world-ticks/s on 1 MI355X (ragged composed park); p99 tick latency"
 BYTES_PER_SHIP_TICK = 361          # 164 read + 197 written, flight-modes park without dynamical orientation
+BYTES_PER_SHIP_TICK_DO = 469       # with it: + 36 read (rot, ang_vel, max_ang_vel, rot_agility as floats)
+                                   # + 72 written (rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll)
 HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
 TRAFFIC_FILE = ROOT / "profiles" / "modes_tick_traffic.json"
 WORKLOADS = {
@@ -46,6 +48,9 @@ WORKLOADS = {
                 "1..32 (mean 6), each world quiet, travelling, in combat or mixed (its ships' modes drawn from that "
                 "profile's law); no dynamical orientation, missiles or collisions",
     "flight": "flight-modes park: {worlds} worlds x 4 ships, modes uniform over stop/goto/follow/orbit/warp",
+    "turning": "turning park: {worlds} worlds x 4 ships with dynamical orientation, each world a 2 km cluster, "
+               "rotations from yaw, pitch and roll uniform in +/-pi, every ship going to a point of its cluster "
+               "(70%), orbiting (15%) or following (15%); no collisions",
 }
 
 
@@ -89,11 +94,12 @@ def _measure(park, ticks_per_launch: int, steps: int, warmup: int) -> dict:
             "ticks_run": (max(1, warmup // per_replay) + reps) * per_replay + lat_n * ticks_per_launch}
 
 
-def _roofline(n_ships: int, m: dict, ticks_per_launch: int, traffic) -> dict:
-    """The tick kernel against the HBM roofline: algorithmic bytes of a launch (361 B a ship-tick x ships x ticks) over
-    the launch's average duration (HIP events over back-to-back launches); `traffic`: the measured HBM bytes a launch
-    (rocprofv3 counters, calibrated), when profiled for this workload and shape."""
-    per_launch = BYTES_PER_SHIP_TICK * n_ships * ticks_per_launch
+def _roofline(n_ships: int, m: dict, ticks_per_launch: int, traffic, per_ship: int = BYTES_PER_SHIP_TICK) -> dict:
+    """The tick kernel against the HBM roofline: algorithmic bytes of a launch (361 B a ship-tick, 469 with dynamical
+    orientation, x ships x ticks) over the launch's average duration (HIP events over back-to-back launches);
+    `traffic`: the measured HBM bytes a launch (rocprofv3 counters, calibrated), when profiled for this workload and
+    shape."""
+    per_launch = per_ship * n_ships * ticks_per_launch
     achieved = per_launch / (m["ms_per_launch"] / 1000.0) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic, "bytes_per_launch_algorithmic": per_launch, "ticks_per_launch": ticks_per_launch,
@@ -112,7 +118,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--workload", choices=("ragged", "profiles", "flight"), default="ragged",
+    ap.add_argument("--workload", choices=("ragged", "profiles", "flight", "turning"), default="ragged",
                     help="the primary line's workload (the others are measured beside it unless --only)")
     ap.add_argument("--only", action="store_true", help="measure the primary workload only")
     ap.add_argument("--block", type=int, default=0, help="fix the workgroup size (0: autotune)")
@@ -151,9 +157,11 @@ def main() -> int:
     def build(kind: str):
         if kind == "flight":
             wl = WL.flight_park(1, a.worlds, 4)
+        elif kind == "turning":
+            wl = WL.turning_park(1, a.worlds, 4)
         else:
             wl = WL.ragged_park(1, a.worlds, mix="profiles" if kind == "profiles" else "uniform")
-        park = VecModesPark(wl["n"], device="cuda", block=256)
+        park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=kind == "turning")
         WL.load_flight_park(park, wl)
         return wl, park
 
@@ -164,7 +172,7 @@ def main() -> int:
             park.evolve()
         done = 4
         orders, training = {}, None
-        if not a.no_learned and kind != "flight":
+        if not a.no_learned and kind in ("ragged", "profiles"):
             t0 = time.perf_counter()
             L = S.LearnedScheduler(park, seed=0)
             training = L.train(ticks=8, epochs=20)
@@ -211,14 +219,16 @@ def main() -> int:
         if shape["ticks"] > 1 and not a.no_single:
             one = measure(1, max(32, a.steps // 4), 4)
         n = wl["n"]
+        per_ship = BYTES_PER_SHIP_TICK_DO if kind == "turning" else BYTES_PER_SHIP_TICK
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
                "ship_ticks_per_s": n / (m["ms_per_tick"] / 1000.0), "ms_per_tick": m["ms_per_tick"],
                "launch_latency_ms": m["launch_ms"],
                "tick_latency_ms": one["launch_ms"],
-               "roofline": _roofline(n, m, shape["ticks"], _traffic(f"{kind}_{shape['order']}_t{shape['ticks']}")),
+               "roofline": _roofline(n, m, shape["ticks"], _traffic(f"{kind}_{shape['order']}_t{shape['ticks']}"),
+                                     per_ship),
                "single_tick": {"ms_per_tick": one["ms_per_tick"],
-                               "roofline": _roofline(n, one, 1, _traffic(f"{kind}_{shape['order']}_t1"))}}
+                               "roofline": _roofline(n, one, 1, _traffic(f"{kind}_{shape['order']}_t1"), per_ship)}}
         if orders and not fixed:
             # the best shape of each world order, each measured as the primary was
             best = {}
@@ -238,7 +248,7 @@ def main() -> int:
                           "world_ticks_per_s": a.worlds / (mo["ms_per_tick"] / 1000.0)}
             res["learned_scheduler"] = {"training": training, "orders": cmp,
                                         "gain_over_rows": cmp["rows"]["ms_per_tick"] / cmp["learned"]["ms_per_tick"]}
-        if not a.no_padded and kind != "flight":
+        if not a.no_padded and kind in ("ragged", "profiles"):
             pad = WL.padded(wl, 32)
             pp = VecModesPark(pad["n"], device="cuda", block=256)
             WL.load_flight_park(pp, pad)
@@ -263,10 +273,13 @@ def main() -> int:
     primary = tuned(a.workload)
     out_extra = {}
     if not a.only:
-        for kind in ("ragged", "profiles", "flight"):
+        for kind in ("ragged", "profiles", "flight", "turning"):
             if kind != a.workload:
                 other = tuned(kind)
                 out_extra[kind + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}
+                if kind == "turning" and not a.no_cpu:
+                    out_extra[kind + "_park"]["cpu_baseline"], out_extra[kind + "_park"]["exact"] = cpu_baseline(
+                        other["_park"], other["_wl"], other["_ticks_done"], a.cpu_seconds / 2, max_worlds=1024)
                 del other
     p = primary
     out = {
@@ -361,7 +374,7 @@ def cpu_baseline_serving(srv) -> tuple:
     return cpu, {"episodes_checked": len(ks), "episodes_differing": bad, "bit_exact": bad == 0}
 
 
-def cpu_baseline(park, wl: dict, ticks: int, seconds: float) -> tuple:
+def cpu_baseline(park, wl: dict, ticks: int, seconds: float, max_worlds: int = 16384) -> tuple:
     """The oracle on one core over a sample of the benchmark's own worlds for the same number of ticks; its final
     states are compared with the GPU park's bit for bit."""
     import numpy as np
@@ -374,7 +387,7 @@ def cpu_baseline(park, wl: dict, ticks: int, seconds: float) -> tuple:
     probe = [WL.world_scenario(wl, wi, ticks) for wi in range(min(64, W))]
     dt, _ = O.time_worlds(probe, ticks)
     per_world = dt / len(probe)
-    m = int(max(64, min(W, 16384, seconds / max(per_world, 1e-9))))
+    m = int(max(64, min(W, max_worlds, seconds / max(per_world, 1e-9))))
     idx = np.linspace(0, W - 1, m).astype(int).tolist()
     sample = [WL.world_scenario(wl, wi, ticks) for wi in idx]
     print(f"# cpu baseline: oracle on {m} worlds x {ticks} ticks", file=sys.stderr, flush=True)
@@ -398,6 +411,9 @@ def cpu_baseline(park, wl: dict, ticks: int, seconds: float) -> tuple:
     od = np.concatenate([st[0][:, D.D_COLS] for st in states])
     oq = np.concatenate([st[1][:, D.Q_COLS] for st in states])
     same = D._same_bits(gd, od).all(axis=1) & (gq == oq).all(axis=1)
+    if park.dynamical_orientation:                 # the turning state too
+        tcols = [29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 28, 26, 27]
+        same &= D._same_bits(f[tcols][:, rows].T, np.concatenate([o.turning() for o in objs])).all(axis=1)
     world_of = np.repeat(np.arange(m), counts[idx])
     bad = int(m - np.bincount(world_of[same], minlength=m).__eq__(counts[idx]).sum())
     cpu = {"value": m * ticks / dt, "unit": "world-ticks/s", "cores": 1, "kind": "port",

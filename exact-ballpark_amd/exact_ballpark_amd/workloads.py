@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP = 1, 2, 3, 4, 5, 6
+YPR_OP = 13                                        # SetBallRotation(RotationQuaternion(yaw, pitch, roll))
 
 
 def _ball_points(rng: np.random.Generator, n: int, radius: float) -> np.ndarray:
@@ -53,8 +54,44 @@ def flight_park(seed: int, worlds: int, ships: int = 4) -> dict:
     return {"n": n, "worlds": worlds, "ships": ships, "rows": rows, "cmds": cmds}
 
 
+def turning_park(seed: int, worlds: int, ships: int = 4) -> dict:
+    """The turning park (dynamical orientation): each world's ships in a 2 km cluster around a centre uniform in a
+    +/-1e6 m cube, at rest, mass U(1e6, 1e8) kg, agility U(0.1, 5), maximum speed U(100, 3000) m/s, radius U(20, 500)
+    m, maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), not massive (no collisions here), each
+    rotated by RotationQuaternion(yaw, pitch, roll) with the three angles U(-pi, pi). At tick 1 each ship goes to a
+    point of its world's cluster (GotoPoint; 70%), orbits (15%) or follows (15%) another ship of its world at
+    U(200, 2000) m: every ship turns toward its thrust, all the time."""
+    rng = np.random.default_rng(seed)
+    n = worlds * ships
+    world = np.repeat(np.arange(worlds, dtype=np.int64), ships)
+    ids = np.tile(np.arange(1, ships + 1, dtype=np.int64), worlds)
+    centre = np.repeat(rng.uniform(-1e6, 1e6, (worlds, 3)), ships, axis=0)
+    rows = {
+        "id": ids, "world": world, "time": np.zeros(n, dtype=np.int64),
+        "mass": rng.uniform(1e6, 1e8, n), "radius": rng.uniform(20.0, 500.0, n),
+        "max_vel": rng.uniform(100.0, 3000.0, n), "agility": rng.uniform(0.1, 5.0, n),
+        "speed_fraction": np.ones(n), "pos": centre + rng.uniform(-1000.0, 1000.0, (n, 3)), "vel": np.zeros((n, 3)),
+        "free": np.ones(n, dtype=bool), "massive": np.zeros(n, dtype=bool),
+        "interactive": rng.uniform(0.0, 1.0, n) < 0.5, "moribund": np.zeros(n, dtype=bool),
+        "cloaked": np.zeros(n, dtype=np.int64), "bubble": np.zeros(n, dtype=np.int64),
+        "mode": np.full(n, 2, dtype=np.int64), "max_ang_vel": rng.uniform(0.2, 2.0, n),
+        "rot_agility": rng.uniform(0.3, 3.0, n),
+    }
+    ypr = rng.uniform(-np.pi, np.pi, (n, 3))
+    u = rng.uniform(0.0, 1.0, n)
+    op = np.where(u < 0.7, GOTO_OP, np.where(u < 0.85, ORBIT_OP, FOLLOW_OP))
+    own = ids - 1
+    other = rng.integers(0, ships - 1, n)
+    other = np.where(other >= own, other + 1, other)
+    cmds = {"op": op.astype(np.int64), "point": centre + rng.uniform(-1000.0, 1000.0, (n, 3)),
+            "target": (world * ships + other).astype(np.int64), "range": rng.uniform(200.0, 2000.0, n),
+            "min_range": np.full(n, 20000.0), "warp_factor": np.full(n, 20, dtype=np.int64)}
+    return {"n": n, "worlds": worlds, "ships": ships, "rows": rows, "cmds": cmds, "ypr": ypr, "dynamical": True}
+
+
 def load_flight_park(park, wl: dict) -> None:
-    """Loads the rows into a VecModesPark of wl["n"] rows and applies the tick-1 commands through its API."""
+    """Loads the rows into a VecModesPark of wl["n"] rows and applies the tick-1 commands through its API (a turning
+    park's rotations first)."""
     r = wl["rows"]
     dev = park.dev
     t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt).to(dev)
@@ -64,7 +101,11 @@ def load_flight_park(park, wl: dict) -> None:
                      free=t(r["free"], torch.bool), massive=t(r["massive"], torch.bool),
                      interactive=t(r["interactive"], torch.bool), cloaked=t(r["cloaked"], torch.int64),
                      bubble=t(r["bubble"], torch.int64), moribund=t(r["moribund"], torch.bool),
-                     mode=t(r["mode"], torch.int64))
+                     mode=t(r["mode"], torch.int64),
+                     max_ang_vel=t(r["max_ang_vel"]) if "max_ang_vel" in r else None,
+                     rot_agility=t(r["rot_agility"]) if "rot_agility" in r else None)
+    if wl.get("ypr") is not None:
+        park.set_rotation_ypr(torch.ones(wl["n"], dtype=torch.bool, device=dev), t(wl["ypr"]))
     apply_commands(park, wl["cmds"])
 
 
@@ -104,6 +145,13 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
                       "massive": bool(r["massive"][k]), "interactive": bool(r["interactive"][k]),
                       "cloaked": int(r["cloaked"][k]), "bubble": int(r["bubble"][k]),
                       "moribund": bool(r["moribund"][k]), "mode": int(r["mode"][k])})
+        if "max_ang_vel" in r:
+            balls[-1]["max_ang_vel"] = float(r["max_ang_vel"][k])
+            balls[-1]["rot_agility"] = float(r["rot_agility"][k])
+    if wl.get("ypr") is not None:
+        for k in rows:
+            y = wl["ypr"][k]
+            cmds.append((1, YPR_OP, int(r["id"][k]), 0, float(y[0]), float(y[1]), float(y[2]), 0.0, 0))
     for o in (GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP):    # apply_commands' order
         for k in rows:
             if int(c["op"][k]) != o:
@@ -114,7 +162,7 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
                          float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
                          int(c["warp_factor"][k])))
     return {"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": steps, "new_orbit": False, "start": 0,
-            "balls": balls, "cmds": cmds}
+            "balls": balls, "cmds": cmds, "dynamical": bool(wl.get("dynamical", False))}
 
 
 def select_worlds(wl: dict, order) -> dict:

@@ -186,7 +186,7 @@ def time_worlds(worlds: list, ticks: int) -> tuple:
     import time
     objs = []
     for w in worlds:
-        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"])
+        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False))
         for b in w["balls"]:
             o.add(b)
         for c in w["cmds"]:

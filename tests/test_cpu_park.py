@@ -58,3 +58,24 @@ def test_cpu_park_dynamical_orientation_matches_oracle(turning, general):
     got = D.run_park(g["worlds"], "cpu", general=general)
     bad = D.compare(got, want, g["worlds"])
     assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+
+
+def test_cpu_turning_park_matches_oracle():
+    """The turning park workload (dynamical orientation, rotations from yaw, pitch and roll, ships turning toward
+    points of their cluster, orbiting and following) on the host build against the oracle."""
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    wl = WL.turning_park(5, 96)
+    park = VecModesPark(wl["n"], device="cpu", dynamical_orientation=True)
+    WL.load_flight_park(park, wl)
+    T = 30
+    for _ in range(T):
+        park.evolve()
+    worlds = [WL.world_scenario(wl, wi, T) for wi in range(wl["worlds"])]
+    want = D.run_oracle(worlds, range(len(worlds)), {T})
+    rows_of = {wi: list(range(wi * 4, wi * 4 + 4)) for wi in range(wl["worlds"])}
+    got = {(wi, T): v for wi, v in D._snapshot(park, list(range(wl["worlds"])), rows_of).items()}
+    bad = D.compare(got, want, worlds)
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+    rw = park.rot[:, 3]
+    assert (rw != 1.0).all() and (park.torque != 0.0).any()

@@ -263,3 +263,52 @@ def test_ragged_park_tiled_padded_and_alone_agree():
             x, y, z = x[keep_q], y[keep_q], z[keep_q]
         assert torch.equal(x, y), "the padded layout differs from the packed one"
         assert torch.equal(x, z), "the scheduled lanes differ from the plain ones"
+
+
+# ---- dynamical orientation ----
+
+@pytest.fixture(scope="module")
+def turning():
+    g = G.load("orient_golden.npz")
+    return g, D.run_oracle(g["worlds"], g["kept"])
+
+
+@pytest.mark.parametrize("general", [False, True], ids=["tiled", "general"])
+def test_turning_worlds_match_oracle(turning, general):
+    """The turning worlds (dynamical orientation: the torque and roll searches, ApplyTorque, the heading thrust, the
+    snaps and the setters) on the device, every field against the oracle (held to the reference's FullBallpark)."""
+    g, want = turning
+    got = D.run_park(g["worlds"], DEV, general=general)
+    bad = D.compare(got, want, g["worlds"])
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
+
+
+def test_turning_park_full_size():
+    """The turning park at 65,536 worlds x 4: the tiled and general launches and 8 ticks a launch identical; 256
+    worlds evenly spaced against the oracle after 40 ticks."""
+    W, T = 65536, 40
+    wl = WL.turning_park(3, W)
+    runs = []
+    for shape in ("tiled", "general", "resident"):
+        park = VecModesPark(wl["n"], device=DEV, dynamical_orientation=True)
+        WL.load_flight_park(park, wl)
+        if shape == "general":
+            park.tiles, park._state = None, None
+        if shape == "resident":
+            for _ in range(T // 8):
+                park.evolve_ticks(8)
+        else:
+            for _ in range(T):
+                park.evolve()
+        torch.cuda.synchronize()
+        runs.append((park, _blocks(park)))
+    assert _same(runs[0][1], runs[1][1]) and _same(runs[0][1], runs[2][1])
+    park = runs[0][0]
+    sample = np.linspace(0, W - 1, 256).astype(int).tolist()
+    worlds = [WL.world_scenario(wl, wi, T) for wi in sample]
+    want = D.run_oracle(worlds, range(len(worlds)), {T})
+    rows_of = {k: list(range(wi * 4, wi * 4 + 4)) for k, wi in enumerate(sample)}
+    snap = D._snapshot(park, list(range(len(sample))), rows_of)
+    got = {(k, T): v for k, v in snap.items()}
+    bad = D.compare(got, want, worlds)
+    assert not bad, f"{len(bad)} differences: " + " | ".join(bad[:5])
