@@ -13,6 +13,7 @@
 #include "xb_modes_tick.h"
 #include "xb_maths_float.h"
 #include "xb_commands.h"
+#include "xb_rand.h"
 #include "../../include/xb.h"
 
 namespace {
@@ -147,6 +148,19 @@ int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, 
         xo_snap(xb_ld3(P, XB_F_VEL, r), t);
         xb_turn_store(P, r, t);
     }
+    return 0;
+}
+
+int xb_cpu_rand_seed(const uint32_t* seeds, int64_t n_worlds, int32_t* states) {
+    if (n_worlds < 0 || (n_worlds > 0 && (!seeds || !states))) return -1;
+    for (int64_t w = 0; w < n_worlds; ++w) xb_srand(seeds[w], states + w * XB_RAND_WORDS);
+    return 0;
+}
+
+int xb_cpu_rand_draw(int32_t* states, int64_t n_worlds, int32_t draws, int32_t* out) {
+    if (n_worlds < 0 || draws < 0 || (n_worlds > 0 && (!states || (draws > 0 && !out)))) return -1;
+    for (int64_t w = 0; w < n_worlds; ++w)
+        for (int32_t k = 0; k < draws; ++k) out[w * (int64_t)draws + k] = xb_rand(states + w * XB_RAND_WORDS);
     return 0;
 }
 

@@ -64,6 +64,8 @@ def hip() -> ctypes.CDLL:
         L.xb_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_int]
         L.xb_modes_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.xb_rand_seed.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_rand_draw.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         L.xb_modes_orient_setup.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                             ctypes.c_void_p]
         L.xb_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
@@ -81,6 +83,8 @@ def cpu() -> ctypes.CDLL:
             raise NativeError("libxb_cpu.so: ABI version mismatch")
         L.xb_cpu_modes_tick.argtypes = [ctypes.POINTER(ModesState), ctypes.c_int]
         L.xb_cpu_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_cpu_rand_seed.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.xb_cpu_rand_draw.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
         L.xb_cpu_modes_orient_setup.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
                                                 ctypes.c_int]
         L.xb_cpu_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \

@@ -162,6 +162,16 @@ typedef struct xb_episode_rows {
 int xb_serve_policy(const xb_episode_rows* rows, const int32_t* live, int64_t n_live, int64_t step, uint8_t* op,
                     double* point, int64_t* target, double* range, void* stream);
 
+/* ---- glibc's rand() per world (destiny's floatrand for shuffled sensors) ---- */
+
+/* srand(seeds[w]) for each world: states[w][33] int32 = glibc's TYPE_3 table (31 words), then its front and rear
+   indices. Replaces the reference's seeding from its own generator (src/exact_ballpark/vec_space.py:148-158). */
+int xb_rand_seed(const uint32_t* seeds, int64_t n_worlds, int32_t* states, void* stream);
+
+/* `draws` rand() values from each world's generator, out[w][draws], the states advanced in place. Replaces the space
+   kernel's glibc_rand (src/exact_ballpark/glibc_rand.cuh:9-26). */
+int xb_rand_draw(int32_t* states, int64_t n_worlds, int32_t draws, int32_t* out, void* stream);
+
 /* ---- libxb_cpu.so: the host build of the tick and the host runtime's helpers ---- */
 
 /* The ABI version of the host library (XB_ABI_VERSION). */
@@ -178,6 +188,10 @@ int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, 
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
                           const double* point, const int64_t* target, const double* range, const double* min_range,
                           const int64_t* warp_factor);
+
+/* xb_rand_seed and xb_rand_draw on the host (no stream). */
+int xb_cpu_rand_seed(const uint32_t* seeds, int64_t n_worlds, int32_t* states);
+int xb_cpu_rand_draw(int32_t* states, int64_t n_worlds, int32_t draws, int32_t* out);
 
 /* xb_maths_eval's functions on the host (same codes, no stream). */
 int xb_cpu_maths_eval(int func, int64_t n, const double* x, double* out);
