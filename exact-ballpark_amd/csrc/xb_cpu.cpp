@@ -204,6 +204,9 @@ int xb_cpu_maths_eval(int func, int64_t n, const double* x, double* out) {
             case 13: out[i] = (double)xb_atanf((float)x[i]); break;
             case 14: out[i] = (double)xb_atan2f((float)x[i], (float)x[n + i]); break;
             case 15: out[i] = xb_pow(x[i], x[n + i]); break;
+            case 16: out[i] = xb_asin(x[i]); break;
+            case 17: out[i] = xb_atan2(x[i], x[n + i]); break;
+            case 18: out[i] = (double)xb_powf((float)x[i], (float)x[n + i]); break;
             default: return -1;
         }
     }

@@ -1,7 +1,8 @@
 /* SPDX-License-Identifier: LGPL-2.1-or-later
  *
  * The GNU C Library 2.34's double sin, cos, sincos (IBM Accurate Mathematical Library, s_sin.c, s_sincos.c,
- * branred.c), log and exp (Arm optimized-routines, e_log.c, e_exp.c) as glibc's aarch64 build computes them: every
+ * branred.c), asin and atan2 (the same library's e_asin.c, e_atan2.c), log and exp (Arm optimized-routines, e_log.c,
+ * e_exp.c) as glibc's aarch64 build computes them: every
  * step a correctly rounded IEEE operation, with a fused multiply-add exactly at the steps GCC fused there and nowhere
  * else. The algorithms and constants are glibc's; the reference restates them at src/exact_ballpark/dlibm.py:429-691
  * (sin, cos, sincos, log, branred) and src/exact_ballpark/vec_exp.py:137-183 (exp).
@@ -391,6 +392,225 @@ static inline XB_HD double xb_exp(double x) {
     double y2 = (lo + hi) - 1.0;
     if (y2 == 0.0) y2 = 0.0;
     return y2 * XB_EX_TWO_M1022;
+}
+
+/* ------------------------------------------------------------------------------------------------ asin */
+/* IBM's asin as glibc 2.34's aarch64 build computes it (sysdeps/ieee754/dbl-64/e_asin.c, its tables asncs, inroot
+   and powtwo; no slow paths): the argument below 2^-26; a Taylor polynomial below 0.125; six ranges of the asncs table
+   (a point and a polynomial in the distance to it) below 0.96875; below 1 pi/2 - 2 asin(sqrt(z)), z = (1 - |x|) / 2,
+   the square root by a Newton step from inroot and powtwo; pi/2 at 1; NaN beyond and for NaN. The fused steps are
+   glibc's. The reference restates it at src/exact_ballpark/dlibm_inv.py:1626-1705. */
+
+static inline XB_HD double xb_asin_table(double xa, int n, int top) {
+    const double* a = XB_IV_ASIN_ASNCS;
+    const double xx = xa - a[n];
+    double poly = xb_fma(a[n + top], xx, a[n + top - 1]);
+    for (int j = top - 2; j > 1; --j) poly = xb_fma(poly, xx, a[n + j]);
+    const double p = xb_fma(xx * xx, poly, a[n + top + 1]);
+    const double t = xb_fma(a[n + 1], xx, p);
+    return t + a[n + top + 2];
+}
+
+static inline XB_HD double xb_asin(double x) {
+    const uint64_t u = xb_bits(x);
+    const int32_t m = (int32_t)(uint32_t)(u >> 32);
+    const uint32_t k = (uint32_t)(u >> 32) & 0x7fffffffu;
+    if (k < XB_IV_ASIN_K_TINY) return x;
+    if (k < XB_IV_ASIN_K_TAYLOR) {
+        const double x2 = x * x;
+        double p = xb_fma(x2, XB_IV_ASIN_F6, XB_IV_ASIN_F5);
+        const double t = x * x2;
+        p = xb_fma(p, x2, XB_IV_ASIN_F4);
+        p = xb_fma(p, x2, XB_IV_ASIN_F3);
+        p = xb_fma(p, x2, XB_IV_ASIN_F2);
+        p = xb_fma(p, x2, XB_IV_ASIN_F1);
+        return xb_fma(p, t, x);
+    }
+    const double xa = m > 0 ? x : -x;
+    double res;
+    if (k < XB_IV_ASIN_K_HALF) {
+        int n;
+        if (k < XB_IV_ASIN_K_QUARTER) n = XB_IV_ASIN_N_Q1_MUL * (int)((k >> 15) & 0x1f);
+        else n = XB_IV_ASIN_N_Q1_MUL * (int)((k >> 14) & 0x3f) + XB_IV_ASIN_N_Q2_OFF;
+        res = xb_asin_table(xa, n, 6);
+    } else if (k < XB_IV_ASIN_K_075) {
+        res = xb_asin_table(xa, XB_IV_ASIN_N_H_OFF + 3 * (int)((k >> 11) & 0x1fc), 7);
+    } else if (k < XB_IV_ASIN_K_0921875) {
+        res = xb_asin_table(xa, XB_IV_ASIN_N_A_OFF + XB_IV_ASIN_N_A_MUL * (int)((k >> 13) & 0x7f), 8);
+    } else if (k < XB_IV_ASIN_K_0953125) {
+        res = xb_asin_table(xa, XB_IV_ASIN_N_B_OFF + 14 * (int)((k >> 13) & 0x7f), 9);
+    } else if (k < XB_IV_ASIN_K_096875) {
+        res = xb_asin_table(xa, XB_IV_ASIN_N_C_OFF + 15 * (int)((k >> 13) & 0x7f), 10);
+    } else if (k < XB_IV_ASIN_K_NEAR_ONE) {
+        const double z = (m > 0 ? XB_IV_ASIN_ONE - x : x + XB_IV_ASIN_ONE) * XB_IV_ASIN_HALF;
+        const uint64_t zb = xb_bits(z);
+        double t = XB_IV_ASIN_INROOT[(zb >> 46) & 0x7f] * XB_IV_ASIN_POWTWO[XB_IV_ASIN_POWTWO_TOP - (int)(zb >> 53)];
+        double p = xb_fma(z, XB_IV_ASIN_F6, XB_IV_ASIN_F5);
+        p = xb_fma(p, z, XB_IV_ASIN_F4);
+        const double r = xb_fmsub(t * t, z, XB_IV_ASIN_ONE);
+        p = xb_fma(p, z, XB_IV_ASIN_F3);
+        double q = xb_fma(r, XB_IV_ASIN_RT3, XB_IV_ASIN_RT2);
+        p = xb_fma(p, z, XB_IV_ASIN_F2);
+        q = xb_fma(q, r, XB_IV_ASIN_RT1);
+        p = xb_fma(p, z, XB_IV_ASIN_F1);
+        q = xb_fma(q, r, XB_IV_ASIN_RT0);
+        p = p * z;
+        t = q * t;
+        const double c = z * t;
+        const double w = xb_fmsub(t * XB_IV_ASIN_HALF, c, XB_IV_ASIN_THREE_HALVES);
+        const double y = (c + XB_IV_ASIN_T24) - XB_IV_ASIN_T24;
+        const double ty = xb_fma(w, c, y);
+        const double e = xb_fmsub(y, y, z);
+        const double res1 = xb_fmsub(y, XB_IV_ASIN_TWO, XB_IV_ASIN_HP0);
+        const double cc = e / ty;
+        double s = y + cc;
+        double cor = xb_fmsub(cc, XB_IV_ASIN_TWO, XB_IV_ASIN_HP1);
+        s = s + s;
+        cor = xb_fmsub(s, p, cor);
+        res = cor + res1;
+    } else if (k == XB_IV_ASIN_K_ONE && (uint32_t)u == 0u) {
+        return m > 0 ? XB_IV_ASIN_HP0 : XB_IV_ASIN_NEG_HP0;
+    } else if (k > XB_IV_ASIN_K_INF || (k == XB_IV_ASIN_K_INF && (uint32_t)u != 0u)) {
+        return x + x;
+    } else {
+        return xb_nan();
+    }
+    return m > 0 ? res : -res;
+}
+
+/* ----------------------------------------------------------------------------------------------- atan2 */
+/* IBM's atan2 as glibc 2.34's aarch64 build computes it (sysdeps/ieee754/dbl-64/e_atan2.c, its table cij; no slow
+   paths): C99 Annex F's special values; by the exponents' distance pi/2, a quotient, or u = the smaller of |x| and |y|
+   over the larger with its error du (one fused step), atan(u) from an odd polynomial below 1/16 or the cij table,
+   placed in its quadrant with pi/2 or pi in two parts. The reference restates it at src/exact_ballpark/dlibm_inv.py:
+   1708-1843. */
+
+static inline XB_HD double xb_atan2_signed(double z, double y) {     /* z's magnitude with y's sign */
+    return xb_dbl((xb_bits(y) & 0x8000000000000000ULL) | (xb_bits(z) & 0x7fffffffffffffffULL));
+}
+
+static inline XB_HD double xb_atan2_poly(double v) {
+    double p = xb_fma(v, XB_IV_ATAN2_D13, XB_IV_ATAN2_D11);
+    p = xb_fma(p, v, XB_IV_ATAN2_D9);
+    p = xb_fma(p, v, XB_IV_ATAN2_D7);
+    p = xb_fma(p, v, XB_IV_ATAN2_D5);
+    return xb_fma(p, v, XB_IV_ATAN2_D3);
+}
+
+static inline XB_HD const double* xb_atan2_row(double u) {         /* cij's row nearest u (7 doubles a row) */
+    const int i = (int)(xb_fma(u, XB_IV_ATAN2_TWO8, XB_IV_ATAN2_TWO52) - XB_IV_ATAN2_TWO52) - XB_IV_ATAN2_I_OFF;
+    return XB_IV_ATAN2_CIJ + 7 * i;
+}
+
+static inline XB_HD double xb_atan2(double y, double x) {
+    const uint64_t bx = xb_bits(x), by = xb_bits(y);
+    const int32_t ux = (int32_t)(uint32_t)(bx >> 32), uy = (int32_t)(uint32_t)(by >> 32);
+    const uint32_t dx = (uint32_t)bx, dy = (uint32_t)by;
+    if ((ux & 0x7ff00000) == 0x7ff00000 && ((ux & 0xfffff) | dx) != 0) return x + y;
+    const int y_inf = (uy & 0x7ff00000) == 0x7ff00000;
+    if (y_inf && ((uy & 0xfffff) | dy) != 0) return y + y;
+    if (!y_inf && uy == 0) {
+        if (dy == 0) return ux < 0 ? XB_IV_ATAN2_OPI : 0.0;
+        if (x == 0.0) return XB_IV_ATAN2_HPI;
+    } else {
+        if (dy == 0 && uy == (int32_t)0x80000000) return ux < 0 ? XB_IV_ATAN2_MOPI : -0.0;
+        if (x == 0.0) return uy < 0 ? XB_IV_ATAN2_MHPI : XB_IV_ATAN2_HPI;
+    }
+    if (ux == 0x7ff00000 && dx == 0) {
+        if (uy == 0x7ff00000 && dy == 0) return XB_IV_ATAN2_QPI;
+        if (uy == (int32_t)0xfff00000 && dy == 0) return XB_IV_ATAN2_MQPI;
+        return uy < 0 ? -0.0 : 0.0;
+    }
+    if (ux == (int32_t)0xfff00000 && dx == 0) {
+        if (uy == 0x7ff00000 && dy == 0) return XB_IV_ATAN2_TQPI;
+        if (uy == (int32_t)0xfff00000 && dy == 0) return XB_IV_ATAN2_MTQPI;
+        return uy < 0 ? XB_IV_ATAN2_MOPI : XB_IV_ATAN2_OPI;
+    }
+    if (uy == 0x7ff00000) return XB_IV_ATAN2_HPI;
+    if (uy == (int32_t)0xfff00000 && dy == 0) return XB_IV_ATAN2_MHPI;
+    double ax = x < 0.0 ? -x : x, ay = y < 0.0 ? -y : y;
+    const int32_t de = (int32_t)((uint32_t)(uy & 0x7ff00000) - (uint32_t)(ux & 0x7ff00000));
+    if (de >= XB_IV_ATAN2_EP) return y > 0.0 ? XB_IV_ATAN2_HPI : XB_IV_ATAN2_MHPI;
+    if (de <= XB_IV_ATAN2_EM) {
+        if (x > 0.0) return xb_atan2_signed(ay / ax, y);
+        return y > 0.0 ? XB_IV_ATAN2_OPI : XB_IV_ATAN2_MOPI;
+    }
+    if (ax < XB_IV_ATAN2_TWOM500 || ay < XB_IV_ATAN2_TWOM500) ax = ax * XB_IV_ATAN2_TWO500, ay = ay * XB_IV_ATAN2_TWO500;
+    if (ax > XB_IV_ATAN2_TWO500 || ay > XB_IV_ATAN2_TWO500) ax = ax * XB_IV_ATAN2_TWOM500, ay = ay * XB_IV_ATAN2_TWOM500;
+    double u, du;
+    if (ay < ax) {
+        u = ay / ax;
+        const double v = ax * u, vv = xb_fnmsub(ax, u, v);
+        du = ((ay - v) - vv) / ax;
+    } else {
+        u = ax / ay;
+        const double v = ay * u, vv = xb_fnmsub(ay, u, v);
+        du = ((ax - v) - vv) / ay;
+    }
+    if (x > 0.0) {
+        if (ay < ax) {                                   /* atan(ay / ax) */
+            if (u < XB_IV_ATAN2_INV16) {
+                const double v = u * u;
+                const double zz = xb_fma(u * v, xb_atan2_poly(v), du);
+                return xb_atan2_signed(u + zz, y);
+            }
+            const double* row = xb_atan2_row(u);
+            const double t3 = u - row[0];
+            const double v = du + t3;
+            const double dv = xb_fabs(t3) > xb_fabs(du) ? (t3 - v) + du : (du - v) + t3;
+            double p = xb_fma(row[6], v, row[5]);
+            p = xb_fma(p, v, row[4]);
+            p = xb_fma(p, v, row[3]);
+            double zz = (v * v) * p;
+            zz = xb_fma(dv, row[2], zz);
+            zz = xb_fma(v, row[2], zz);
+            return xb_atan2_signed(zz + row[1], y);
+        }
+        if (u < XB_IV_ATAN2_INV16) {                     /* pi/2 - atan(ax / ay) */
+            const double v = u * u;
+            const double zz = (u * v) * xb_atan2_poly(v);
+            const double t2 = XB_IV_ATAN2_HPI - u;
+            const double cor = xb_fabs(u) < XB_IV_ATAN2_HPI ? (XB_IV_ATAN2_HPI - t2) - u : XB_IV_ATAN2_HPI - (u + t2);
+            return xb_atan2_signed((((cor + XB_IV_ATAN2_HPI1) - du) - zz) + t2, y);
+        }
+        const double* row = xb_atan2_row(u);
+        const double v = (u - row[0]) + du;
+        double p = xb_fma(row[6], v, row[5]);
+        p = xb_fma(p, v, row[4]);
+        p = xb_fma(p, v, row[3]);
+        p = xb_fma(p, v, row[2]);
+        return xb_atan2_signed((XB_IV_ATAN2_HPI - row[1]) + xb_fmsub(p, v, XB_IV_ATAN2_HPI1), y);
+    }
+    if (ax < ay) {                                       /* pi/2 + atan(ax / ay) */
+        if (u < XB_IV_ATAN2_INV16) {
+            const double v = u * u;
+            const double zz = (v * u) * xb_atan2_poly(v);
+            const double t2 = u + XB_IV_ATAN2_HPI;
+            const double cor = xb_fabs(u) < XB_IV_ATAN2_HPI ? (XB_IV_ATAN2_HPI - t2) + u : (u - t2) + XB_IV_ATAN2_HPI;
+            return xb_atan2_signed((((cor + XB_IV_ATAN2_HPI1) + du) + zz) + t2, y);
+        }
+        const double* row = xb_atan2_row(u);
+        const double v = (u - row[0]) + du;
+        double p = xb_fma(row[6], v, row[5]);
+        p = xb_fma(p, v, row[4]);
+        p = xb_fma(p, v, row[3]);
+        p = xb_fma(p, v, row[2]);
+        return xb_atan2_signed((row[1] + XB_IV_ATAN2_HPI) + xb_fma(p, v, XB_IV_ATAN2_HPI1), y);
+    }
+    if (u < XB_IV_ATAN2_INV16) {                         /* pi - atan(ay / ax) */
+        const double v = u * u;
+        const double zz = (v * u) * xb_atan2_poly(v);
+        const double t2 = XB_IV_ATAN2_OPI - u;
+        const double cor = xb_fabs(u) < XB_IV_ATAN2_OPI ? (XB_IV_ATAN2_OPI - t2) - u : XB_IV_ATAN2_OPI - (t2 + u);
+        return xb_atan2_signed((((cor + XB_IV_ATAN2_OPI1) - du) - zz) + t2, y);
+    }
+    const double* row = xb_atan2_row(u);
+    const double v = (u - row[0]) + du;
+    double p = xb_fma(row[6], v, row[5]);
+    p = xb_fma(p, v, row[4]);
+    p = xb_fma(p, v, row[3]);
+    p = xb_fma(p, v, row[2]);
+    return xb_atan2_signed((XB_IV_ATAN2_OPI - row[1]) + xb_fmsub(p, v, XB_IV_ATAN2_OPI1), y);
 }
 
 #endif

@@ -415,4 +415,112 @@ static inline XB_HD double xb_pow(double x, double y) {
     return xb_pow_exp(ehi, elo, negative);
 }
 
+/* ------------------------------------------------------------------------------------------------ powf */
+/* powf from ARM's optimized-routines as glibc 2.34's aarch64 build computes it (sysdeps/ieee754/flt-32/e_powf.c, its
+   tables __powf_log2_data and __exp2f_data): log2 x scaled by 32 from one of 16 points and a quintic in double, times
+   y, then 2^(y log2 x) from a 32-entry table and a cubic, rounded to float once. The scaled exponent is rounded to the
+   nearest integer with ties away from zero (the TOINT_INTRINSICS build: frinta and fcvtas), not by the 1.5 2^52 shift
+   of other builds, which rounds ties to even. Nine multiply-adds are fused, as that build fuses them. The special
+   cases, the parity of y for a negative x, the scaling of a subnormal x and the overflow and underflow thresholds are
+   that routine's; the flags and errno are not modelled, and the rounding mode is to nearest, so the directed-rounding
+   check before the overflow threshold is never taken. The reference restates it at src/exact_ballpark/dlibm_powf.py:
+   193-295. */
+
+static inline XB_HD int xb_powf_signaling(uint32_t u) {
+    return ((u ^ (uint32_t)XB_PF_POWF_QNAN_BIT) & 0x7fffffffu) > (uint32_t)XB_PF_POWF_QNAN;
+}
+
+static inline XB_HD float xb_powf_xflow(uint32_t sign, uint32_t tbits) {     /* (sign ? -t : t) * t in float */
+    const float t = xb_flt(tbits);
+    return (sign ? -t : t) * t;
+}
+
+static inline XB_HD double xb_powf_log2(uint32_t ix) {        /* 32 log2 x for the bits of a positive normal x */
+    const uint32_t tmp = ix + (uint32_t)XB_PF_POWF_NEG_OFF;
+    const int i = (int)((tmp >> XB_PF_POWF_I_SHIFT) & ((1u << XB_PF_POWF_I_BITS) - 1u));
+    const uint32_t top = tmp & (uint32_t)XB_PF_POWF_TOP_MASK;
+    const uint32_t iz = ix - top;
+    const int32_t k = (int32_t)top >> XB_PF_POWF_K_SHIFT;
+    const double invc = XB_PF_POWF_LOG2_INVC[i], logc = XB_PF_POWF_LOG2_LOGC[i];
+    const double z = (double)xb_flt(iz);
+    const double r = xb_fma(invc, z, -1.0);
+    const double y0 = (double)k + logc;
+    const double p = xb_fma(r, XB_PF_POWF_A[2], XB_PF_POWF_A[3]);
+    double q = xb_fma(r, XB_PF_POWF_A[4], y0);
+    const double r2 = r * r;
+    const double y = xb_fma(r, XB_PF_POWF_A[0], XB_PF_POWF_A[1]);
+    const double r4 = r2 * r2;
+    q = xb_fma(r2, p, q);
+    return xb_fma(y, r4, q);
+}
+
+static inline XB_HD float xb_powf_exp2(double xd, uint32_t sign_bias) {   /* 2^(xd / 32) in float, its sign */
+    const double kd = xb_round_away(xd);
+    const int64_t ki = (int64_t)kd;
+    const double r = xd - kd;
+    double y = xb_fma(r, XB_PF_POWF_EXP2_C[2], 1.0);
+    const double z = xb_fma(r, XB_PF_POWF_EXP2_C[0], XB_PF_POWF_EXP2_C[1]);
+    const double r2 = r * r;
+    uint64_t t = XB_PF_POWF_EXP2_TAB[ki & 31];
+    t += ((uint64_t)ki + (uint64_t)sign_bias) << XB_PF_POWF_EXP2_SHIFT;
+    y = xb_fma(z, r2, y);
+    return (float)(y * xb_dbl(t));
+}
+
+static inline XB_HD float xb_powf_y_special(float x, float y, uint32_t ix, uint32_t iy) {  /* y zero, inf or NaN */
+    const uint32_t two_iy = iy << 1;
+    if (two_iy == 0u) return xb_powf_signaling(ix) ? x + y : 1.0f;
+    if (ix == (uint32_t)XB_PF_POWF_ONE_BITS) return xb_powf_signaling(iy) ? x + y : 1.0f;
+    const uint32_t two_ix = ix << 1;
+    if (two_iy != (uint32_t)XB_PF_POWF_INF2 || two_ix > (uint32_t)XB_PF_POWF_INF2) return x + y;
+    if (two_ix == (uint32_t)XB_PF_POWF_ONE2) return 1.0f;
+    if ((two_ix < (uint32_t)XB_PF_POWF_ONE2) == !(iy >> 31)) return 0.0f;
+    return y * y;
+}
+
+static inline XB_HD float xb_powf(float x, float y) {
+    uint32_t ix = xb_fbits(x);
+    const uint32_t iy = xb_fbits(y);
+    const int zinf_y = (iy << 1) - 1u >= (uint32_t)XB_PF_POWF_ZINF;
+    uint32_t sign_bias = 0;
+    if (ix - (uint32_t)XB_PF_POWF_X_MIN > (uint32_t)XB_PF_POWF_X_SPAN) {   /* below 2^-126, negative, inf or NaN */
+        if (zinf_y) return xb_powf_y_special(x, y, ix, iy);
+        const uint32_t two_ix = ix << 1;
+        if (two_ix - 1u >= (uint32_t)XB_PF_POWF_ZINF) {                  /* x zero, infinite or NaN */
+            float x2 = x * x;
+            uint32_t sb = 0;
+            if (ix >> 31) {                                                /* y an odd integer: the sign stays */
+                const uint32_t e = (iy >> 23) & 0xffu;
+                if (e - (uint32_t)XB_PF_POWF_E_ONE <= (uint32_t)XB_PF_POWF_E_SPAN) {
+                    const uint32_t bit = 1u << (XB_PF_POWF_INT_E - e);
+                    if (!(iy & (bit - 1u)) && (iy & bit)) x2 = -x2, sb = 1;
+                }
+            }
+            if (two_ix == 0u && (iy >> 31)) return (sb ? -1.0f : 1.0f) / 0.0f;
+            return (iy >> 31) ? 1.0f / x2 : x2;
+        }
+        if (ix >> 31) {                                                    /* a finite x < 0: y must be an integer */
+            const uint32_t e = (iy >> 23) & 0xffu;
+            if (e <= (uint32_t)XB_PF_POWF_E_HALF) return (x - x) / (x - x);
+            if (e <= (uint32_t)XB_PF_POWF_INT_E) {
+                const uint32_t bit = 1u << (XB_PF_POWF_INT_E - e);
+                if (iy & (bit - 1u)) return (x - x) / (x - x);
+                if (iy & bit) sign_bias = XB_PF_POWF_SIGN_BIAS;
+            }
+            ix &= 0x7fffffffu;
+        }
+        if (ix <= (uint32_t)XB_PF_POWF_SUBNORMAL_MAX)                     /* subnormal: scaled by 2^23, then back */
+            ix = (xb_fbits(x * xb_flt((uint32_t)XB_PF_POWF_TWO23_F)) & 0x7fffffffu) - (uint32_t)XB_PF_POWF_NORM_SUB;
+    } else if (zinf_y) {
+        return xb_powf_y_special(x, y, ix, iy);
+    }
+    const double ylogx = (double)y * xb_powf_log2(ix);
+    if (((xb_bits(ylogx) >> 47) & 0xffffu) > (uint64_t)XB_PF_POWF_BIG) {  /* |y log2 x| >= 126 */
+        if (ylogx > XB_PF_POWF_OFLOW) return xb_powf_xflow(sign_bias, (uint32_t)XB_PF_POWF_OFLOWF_X);
+        if (ylogx <= xb_dbl(XB_PF_POWF_UFLOW)) return xb_powf_xflow(sign_bias, (uint32_t)XB_PF_POWF_UFLOWF_X);
+        if (ylogx < xb_dbl(XB_PF_POWF_MAY_UFLOW)) return xb_powf_xflow(sign_bias, (uint32_t)XB_PF_POWF_MAY_UFLOWF_X);
+    }
+    return xb_powf_exp2(ylogx, sign_bias);
+}
+
 #endif

@@ -236,6 +236,9 @@ __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ 
         case 13: out[i] = (double)xb_atanf((float)x[i]); break;
         case 14: out[i] = (double)xb_atan2f((float)x[i], (float)x[n + i]); break;
         case 15: out[i] = xb_pow(x[i], x[n + i]); break;
+        case 16: out[i] = xb_asin(x[i]); break;
+        case 17: out[i] = xb_atan2(x[i], x[n + i]); break;
+        case 18: out[i] = (double)xb_powf((float)x[i], (float)x[n + i]); break;
         default: out[i] = xb_nan(); break;
     }
 }
@@ -318,7 +321,7 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
 }
 
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block) {
-    if (n < 0 || func < 0 || func > 15 || !xb_block_ok(block)) return -1;
+    if (n < 0 || func < 0 || func > 18 || !xb_block_ok(block)) return -1;
     if (n == 0) return 0;
     xb_maths_kernel<<<(unsigned)((n + block - 1) / block), block, 0, (hipStream_t)stream>>>(func, n, x, out);
     return xb_launch_rc();

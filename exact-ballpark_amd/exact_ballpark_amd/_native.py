@@ -112,14 +112,16 @@ def check(rc: int, fn: str) -> None:
 
 
 MATHS = {"sin": 0, "cos": 1, "log": 2, "exp": 3, "sincos_sin": 4, "sincos_cos": 5, "sqrt": 6, "div": 7, "expf": 8,
-         "logf": 9, "sinf": 10, "cosf": 11, "acosf": 12, "atanf": 13, "atan2f": 14, "pow": 15}
-BINARY = ("div", "atan2f", "pow")
+         "logf": 9, "sinf": 10, "cosf": 11, "acosf": 12, "atanf": 13, "atan2f": 14, "pow": 15, "asin": 16,
+         "atan2": 17, "powf": 18}
+BINARY = ("div", "atan2f", "pow", "atan2", "powf")
 
 
 def maths(name: str, x: torch.Tensor, y: torch.Tensor | None = None, block: int = 256) -> torch.Tensor:
     """The tick's own maths function `name` elementwise (on the device of x: the GPU kernel or the host build): the
     double sin, cos, log, exp, sincos's two parts, IEEE sqrt and division x / y; the float expf, logf, sinf, cosf,
-    acosf, atanf, atan2f(x, y) (x the ordinate) on floats held in doubles; the double pow(x, y)."""
+    acosf, atanf, atan2f(x, y) (x the ordinate) and powf(x, y) on floats held in doubles; the double pow(x, y), asin
+    and atan2(x, y) (x the ordinate)."""
     x = x.to(torch.float64).contiguous()
     n = x.numel()
     inp = torch.cat((x.reshape(-1), y.to(torch.float64).reshape(-1))) if name in BINARY else x.reshape(-1)

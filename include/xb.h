@@ -79,9 +79,10 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
 
 /* The tick's double functions alone, out[i] = f(x[i]) (func 0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 sincos's
    cosine, 6 IEEE sqrt, 7 IEEE division x[i] / x[n + i] with out of n elements; on floats held in doubles 8 expf, 9
-   logf, 10 sinf, 11 cosf, 12 acosf, 13 atanf, 14 atan2f(x[i], x[n + i]); 15 the double pow(x[i], x[n + i])). Replaces
-   xbpm_dlibm and xbpo_libm
-   (src/exact_ballpark/vec_modes_cuda.cu:779). */
+   logf, 10 sinf, 11 cosf, 12 acosf, 13 atanf, 14 atan2f(x[i], x[n + i]); 15 the double pow(x[i], x[n + i]); 16 the
+   double asin, 17 the double atan2(x[i], x[n + i]); 18 powf(x[i], x[n + i]) on floats held in doubles). Replaces
+   xbpm_dlibm (src/exact_ballpark/vec_modes_cuda.cu:779), xbpo_libm (vec_orient_cuda.cu:520), xbpy_dlibm
+   (vec_ypr_cuda.cu:102) and xbpc_powf (vec_combat_cuda.py:321). */
 int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* stream, int block);
 
 /* Dynamical orientation's set-up of the listed rows (rows NULL: rows 0 .. n_rows - 1), in place: what 0

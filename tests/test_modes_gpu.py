@@ -3,7 +3,8 @@
 - the golden worlds (every kind of scenario), every ball after every tick, in both launch shapes;
 - the flight-modes park workload (1,024 worlds x 4 ships, 100 ticks, modes uniform over stop, goto, follow, orbit and
   warp), at recorded ticks, and against the host build of the same source;
-- the tick's maths on the device against the host (sin, cos, log, exp, sincos; sqrt and division IEEE);
+- the tick's maths on the device against the host (sin, cos, log, exp, sincos; sqrt and division IEEE), and the
+  float functions, pow, powf, asin and atan2 of the device against the host build of the same source;
 - properties at full size (65,536 worlds x 4 ships): tiled and general launches identical, a run repeated identical, a
   world alone identical to the same world in the batch and in reversed order, block sizes 64, 128 and 256 identical;
 - the HIP graph replay identical to eager ticks, with eager commands between replays; OnTick's masked tick.
@@ -67,6 +68,38 @@ def test_device_sqrt_and_division_are_ieee():
     q = N.maths("div", torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV)).cpu().numpy()
     with np.errstate(all="ignore"):
         assert (_bits(q) == _bits(x / y)).all()
+
+
+UNARY_F = ("expf", "logf", "sinf", "cosf", "acosf", "atanf", "asin")
+BINARY_F = ("atan2f", "pow", "atan2", "powf")
+
+
+@pytest.mark.parametrize("name", UNARY_F + BINARY_F)
+def test_device_float_and_inverse_maths_equal_host(name):
+    """Seeded arguments over the functions' domains and their special values: random bit patterns (as floats for the
+    float functions), the unit interval, wide magnitudes; the device result equal to libxb_cpu.so's bit for bit."""
+    rng = np.random.default_rng(23)
+    m = 200000
+    raw = rng.integers(0, 2 ** 64, m, dtype=np.uint64).view(np.float64)
+    if name in ("expf", "logf", "sinf", "cosf", "acosf", "atanf", "atan2f", "powf"):
+        with np.errstate(invalid="ignore"):                 # signalling NaNs among the patterns
+            raw = rng.integers(0, 2 ** 32, m, dtype=np.uint64).astype(np.uint32).view(np.float32).astype(np.float64)
+    x = np.concatenate([raw, rng.uniform(-1, 1, m), rng.uniform(-100, 100, m),
+                        np.exp(rng.uniform(-300, 300, m)) * rng.choice([-1, 1], m),
+                        [0.0, -0.0, 1.0, -1.0, math.inf, -math.inf, math.nan, 5e-324, 0.25, 0.5]])
+    y = None
+    if name in BINARY_F:
+        y = np.roll(x, 12345).copy()
+        if name == "pow":
+            x = np.abs(x)
+            y = np.round(y) % 200 - 100
+        if name == "powf":
+            y[m:2 * m] = 0.25
+    args = [torch.from_numpy(x)] + ([torch.from_numpy(y)] if y is not None else [])
+    want = N.maths(name, *args).numpy()
+    got = N.maths(name, *[a.to(DEV) for a in args]).cpu().numpy()
+    same = (_bits(got) == _bits(want)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), f"{int((~same).sum())} differ, first at x = {x[~same][:3]}"
 
 
 @pytest.mark.parametrize("general", [False, True], ids=["tiled", "general"])
