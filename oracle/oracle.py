@@ -64,6 +64,10 @@ def lib() -> ctypes.CDLL:
         L.or_set_rot_agility.argtypes = [vp, i64, d]
         L.or_set_rotation_ypr.argtypes = [vp, i64, d, d, d]
         L.or_get_turning.argtypes = [vp, i, pd]
+        L.or_park_set_collisions.argtypes = [vp, i]
+        L.or_overflow.argtypes = [vp]
+        L.or_set_junk.argtypes = [vp, i64, i]
+        L.or_get_collide.argtypes = [vp, i, pd, pq, pq]
         _LIB = L
     return _LIB
 
@@ -71,11 +75,14 @@ def lib() -> ctypes.CDLL:
 class World:
     """One world of the oracle: add balls, command them, tick, read them back."""
 
-    def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int, dynamical: bool = False):
+    def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int, dynamical: bool = False,
+                 collisions: bool = False):
         self.L = lib()
         self.p = self.L.or_park_new(float(friction), int(tick_ms), int(bool(new_orbit)), int(start))
         self.dynamical = bool(dynamical)
+        self.collisions = bool(collisions)
         self.L.or_park_set_dynamical(self.p, int(self.dynamical))
+        self.L.or_park_set_collisions(self.p, int(self.collisions))
 
     def __del__(self):
         if getattr(self, "p", None):
@@ -91,6 +98,8 @@ class World:
                                 int(b["bubble"]), int(bool(b["moribund"])), int(b["mode"]))
         if rc:
             raise ValueError(f"ball {b['id']} refused")
+        if b.get("junk"):
+            self.L.or_set_junk(self.p, int(b["id"]), 1)
         if self.dynamical:                             # AddDynamicallyOrientedBall and the AddBall's snap
             self.L.or_add_turning(self.p, int(b["id"]), float(b.get("max_ang_vel", 1.0)),
                                   float(b.get("rot_agility", 0.9)))
@@ -139,6 +148,22 @@ class World:
             self.L.or_get_turning(self.p, k, t[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return t
 
+    def collide(self) -> tuple:
+        """(last_c [n, 3] float64, filing [n, 7] int64: level, key, side, hits: list of the collision lists), balls by
+        ascending id. Raises if a collision list overflowed the oracle's bound."""
+        if self.L.or_overflow(self.p):
+            raise OverflowError("a collision list past the oracle's bound, or a ball wider than a top box")
+        n = self.L.or_count(self.p)
+        c = np.zeros((n, 3), dtype=np.float64)
+        f = np.zeros((n, 8), dtype=np.int64)
+        hits = []
+        buf = (ctypes.c_int64 * 64)()
+        for k in range(n):
+            m = self.L.or_get_collide(self.p, k, c[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      f[k].ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), buf)
+            hits.append(list(buf[:m]))
+        return c, f[:, :7], hits
+
     def state(self) -> tuple:
         """(doubles [n, 27] as float64, ints [n, 12], followers: list of sorted id lists), balls by ascending id."""
         n = self.L.or_count(self.p)
@@ -156,7 +181,8 @@ class World:
 
 def run_world(w: dict, record_every_step: bool = True):
     """Yields (step, doubles, ints, followers) after each tick of world w (every step, or the last only)."""
-    o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False))
+    o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False),
+              w.get("collisions", False))
     for b in w["balls"]:
         o.add(b)
     for step in range(1, w["steps"] + 1):
@@ -186,7 +212,8 @@ def time_worlds(worlds: list, ticks: int) -> tuple:
     import time
     objs = []
     for w in worlds:
-        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False))
+        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], w.get("dynamical", False),
+                  w.get("collisions", False))
         for b in w["balls"]:
             o.add(b)
         for c in w["cmds"]:

@@ -37,6 +37,11 @@ enum { GOTO = 0, FOLLOW = 1, STOP = 2, WARP = 3, ORBIT = 4, MISSILE = 5, MUSHROO
 #define STOP_DAMP 0.9345794392523364485981308411215   /* ref.py:37 */
 #define MAX_FOLLOWERS 64
 #define MAX_BALLS 256
+#define MAX_HITS 64                            /* a tick's collision list; more sets the park's overflow flag */
+#define NUM_LEVELS 8                           /* space.py:69-77 */
+#define BIGBOX (65536.0 * 480.0 * 0.25)
+#define LIGHT_BALL_RATIO 25.0
+#define RESPONSE_SCALE 0.85
 
 typedef struct {
     int64_t id;
@@ -51,6 +56,12 @@ typedef struct {
     int64_t followers[MAX_FOLLOWERS];
     /* dynamical orientation (full.py): floats held in double */
     double rot[4], old_rot[4], ang_vel[3], old_ang_vel[3], torque[3], rot_roll, max_ang_vel, rot_agility;
+    /* the partition and the gradient (space.py): the filing (the level and key of the ball's box, the side on each
+       axis its inflated sphere reaches), space junk, and the tick's collision list */
+    int is_junk, level, mod[3];
+    int64_t key[3];
+    int nh;
+    int64_t hits[MAX_HITS];
 } OBall;
 
 typedef struct {
@@ -58,6 +69,8 @@ typedef struct {
     int64_t current_time;
     int use_new_orbit;
     int dynamical;                             /* FullBallpark's dynamical_orientation */
+    int collisions;                            /* SpaceBallpark: the partition and the gradient method */
+    int overflow;                              /* a collision list past MAX_HITS */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
 } OPark;
@@ -130,6 +143,8 @@ void or_park_free(OPark* p) { free(p); }
 int or_park_size(void) { return (int)sizeof(OPark); }
 int or_ball_size(void) { return (int)sizeof(OBall); }
 
+static void file_ball(OPark* p, OBall* b);
+
 static void refresh_time_factor(OPark* p, OBall* b) {     /* ref.py:82-87 */
     double m = b->mass * b->agility;
     b->tf = (m <= 0.0) ? 0.0 : xb_exp(-p->friction * p->dt / m);
@@ -171,6 +186,7 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->max_ang_vel = 1.0;
     b->rot_agility = (double)0.9f;
     refresh_time_factor(p, b);
+    if (p->collisions) file_ball(p, b);                              /* space.py:1150 */
     return 0;
 }
 
@@ -1105,18 +1121,378 @@ static void warp_distance(OPark* p, OBall* b, double pp[3], double vv[3], double
     }
 }
 
+/* ------------------------------------------------------------------ the partition (space.py, client's view) */
+/* Destiny's Partition as far as collisions read it. A ball's filing is what Ball::InsertInBoxes leaves
+   (space.py:729-817): the level of its inflated radius, the key of the box holding its centre, and on each axis the
+   side (-1, 0, +1) where the inflated sphere reaches the next box; its boxes are the product of {key} or {key, key + side}
+   per axis, each clamped into the grid. A box exists while a ball is filed in it or below it (GetBox creates the
+   parents, ReleaseBox drops a box left with neither balls nor children, space.py:596-681), so here the boxes are
+   derived from the filings when GetNearbyBalls walks them: the same boxes, children and balls the reference's tree holds
+   on a client's view. */
+
+static double level_width(int i) { return BIGBOX / (double)(1LL << (2 * i)); }
+static int64_t level_grid(int i) { return 1LL << (2 * i - 2 * NUM_LEVELS + 39); }
+
+static int64_t to_int64_(double x) {                                    /* space.py:89-98 (aarch64's fcvtzs) */
+    if (x != x) return 0;
+    if (x >= 9223372036854775808.0) return INT64_MAX;
+    if (x < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)x;
+}
+static int64_t clamp_key(int64_t k, int64_t n) { return k < 0 ? 0 : k >= n ? n - 1 : k; }
+
+static double ieee_div(double a, double b) { return a / b; }           /* modes.py:99-105: C's division */
+
+static double inflated_radius(const OPark* p, const OBall* b) {        /* space.py:710-719 */
+    if (!b->is_free) return b->radius;
+    double m = b->mass * b->agility;
+    double a = ieee_div(p->friction * b->sf * b->max_vel, m);
+    double v = sqrt(lensq(b->vel));
+    double k = p->friction;
+    double mps = ieee_div(m * a - (m * a - v * k) * b->tf, k);
+    return b->radius + p->dt * mps;
+}
+
+static int level_for(double radius) {                                   /* space.py:623-631 */
+    int level = 0;
+    for (int i = 0; i < NUM_LEVELS; ++i) {
+        if (3.0 * radius > level_width(i)) break;
+        level = i;
+    }
+    return level;
+}
+
+/* InsertInBoxes's outcome for the ball (space.py:729-817): the level, the clamped key, the sides. A ball wider than a
+   top box (filed in all 27 top boxes) is outside this oracle: the park's overflow flag is set. */
+static void file_ball(OPark* p, OBall* b) {
+    double infl = inflated_radius(p, b);
+    int level = level_for(infl);
+    double w = level_width(level);
+    int64_t n = level_grid(level);
+    double fac = 1.0 / w, half = 0.5 * w * (double)n, center = w * (double)n * 0.5;
+    for (int q = 0; q < 3; ++q) {
+        int64_t k = clamp_key(to_int64_((b->pos[q] + half) * fac), n);
+        double lo = (double)k * w - center, hi = (double)(k + 1) * w - center;
+        b->key[q] = k;
+        b->mod[q] = (lo + infl - b->pos[q] > 0.0) ? -1 : (hi - infl - b->pos[q] < 0.0) ? 1 : 0;
+    }
+    b->level = level;
+    if (level == 0 && b->radius * 2 > level_width(0)) p->overflow = 1;
+}
+
+typedef struct { int level; int64_t k[3]; } OBox;
+
+static int box_cmp(const OBox* a, const OBox* b) {                      /* the key tuple's order */
+    for (int q = 0; q < 3; ++q)
+        if (a->k[q] != b->k[q]) return a->k[q] < b->k[q] ? -1 : 1;
+    return 0;
+}
+static int box_eq(const OBox* a, const OBox* b) { return a->level == b->level && box_cmp(a, b) == 0; }
+static int box_qsort(const void* a, const void* b) { return box_cmp((const OBox*)a, (const OBox*)b); }
+
+static int ball_boxes(const OBall* b, OBox out[8]) {                   /* its boxes, sorted by key, distinct */
+    int64_t n = level_grid(b->level);
+    int cnt = 0;
+    for (int i = 0; i < 8; ++i) {
+        OBox x = {b->level, {0, 0, 0}};
+        int skip = 0;
+        for (int q = 0; q < 3; ++q) {
+            int use = (i >> q) & 1;
+            if (use && b->mod[q] == 0) skip = 1;
+            x.k[q] = clamp_key(b->key[q] + (use ? b->mod[q] : 0), n);
+        }
+        if (skip) continue;
+        int dup = 0;
+        for (int j = 0; j < cnt; ++j) dup |= box_eq(&out[j], &x);
+        if (!dup) out[cnt++] = x;
+    }
+    qsort(out, cnt, sizeof(OBox), box_qsort);
+    return cnt;
+}
+
+static OBox parent_of(OBox x) {
+    OBox y = {x.level - 1, {x.k[0] >> 2, x.k[1] >> 2, x.k[2] >> 2}};
+    return y;
+}
+
+typedef struct {                       /* the boxes that exist, and each one's search flag (Box::mHandled) */
+    int n;
+    OBox box[MAX_BALLS * 8 * NUM_LEVELS];
+    int handled[MAX_BALLS * 8 * NUM_LEVELS];
+} OBoxes;
+
+static int box_index(const OBoxes* U, OBox x) {
+    for (int i = 0; i < U->n; ++i)
+        if (box_eq(&U->box[i], &x)) return i;
+    return -1;
+}
+
+static void boxes_build(const OPark* p, OBoxes* U) {
+    U->n = 0;
+    for (int i = 0; i < p->n; ++i) {
+        OBox bx[8];
+        int c = ball_boxes(&p->balls[i], bx);
+        for (int j = 0; j < c; ++j) {
+            OBox x = bx[j];
+            for (;;) {
+                if (box_index(U, x) < 0) {
+                    U->box[U->n] = x;
+                    U->handled[U->n++] = 0;
+                }
+                if (x.level == 0) break;
+                x = parent_of(x);
+            }
+        }
+    }
+}
+
+static int ball_in_box(const OBall* c, OBox x) {
+    if (c->level != x.level) return 0;
+    OBox bx[8];
+    int cnt = ball_boxes(c, bx);
+    for (int j = 0; j < cnt; ++j)
+        if (box_eq(&bx[j], &x)) return 1;
+    return 0;
+}
+
+static int box_ball_count(const OPark* p, OBox x) {
+    int cnt = 0;
+    for (int i = 0; i < p->n; ++i) cnt += ball_in_box(&p->balls[i], x);
+    return cnt;
+}
+
+static void explode_children(const OPark* p, const OBoxes* U, OBox x, int* out, int* n_out) {   /* space.py:887-893 */
+    OBox kids[64];
+    int nk = 0;
+    for (int i = 0; i < U->n; ++i) {
+        const OBox* c = &U->box[i];
+        if (c->level == x.level + 1 && (c->k[0] >> 2) == x.k[0] && (c->k[1] >> 2) == x.k[1] && (c->k[2] >> 2) == x.k[2])
+            kids[nk++] = *c;
+    }
+    qsort(kids, nk, sizeof(OBox), box_qsort);
+    for (int i = 0; i < nk; ++i) {
+        explode_children(p, U, kids[i], out, n_out);
+        if (box_ball_count(p, kids[i]) > 0) out[(*n_out)++] = box_index(U, kids[i]);
+    }
+}
+
+/* GetNearbyBalls (space.py:819-885) for the gradient's search (collisionCase1: exclude_fixed, exclude_inert and
+   exclude_harmonics off, exclude_missiles and exclude_cloaked on), on a client's view: the ball's boxes by key; for
+   each its non-empty ancestors bottom-up until one already searched, its descendants with balls in post-order, then
+   the box itself if another ball is in it; then the balls of those boxes by id, each once. No ball here is a force
+   field (the harmonic filter passes every ball). Returns the candidates' indices in order. */
+static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out) {
+    const OBall* b = &p->balls[bi];
+    boxes_build(p, U);
+    OBox bx[8];
+    int nb = ball_boxes(b, bx);
+    int sbox[MAX_BALLS * 8 * NUM_LEVELS];
+    int ns = 0;
+    for (int j = 0; j < nb; ++j) {
+        int bj = box_index(U, bx[j]);
+        if (U->handled[bj]) continue;
+        OBox x = bx[j];
+        while (x.level > 0) {
+            x = parent_of(x);
+            int xi = box_index(U, x);
+            if (U->handled[xi]) break;
+            if (box_ball_count(p, x) > 0) {
+                sbox[ns++] = xi;
+                U->handled[xi] = 1;
+            }
+        }
+        explode_children(p, U, bx[j], sbox, &ns);
+        if (box_ball_count(p, bx[j]) > 1) {
+            sbox[ns++] = bj;
+            U->handled[bj] = 1;
+        }
+    }
+    int handled[MAX_BALLS] = {0};
+    int cnt = 0;
+    for (int s = 0; s < ns; ++s) {
+        OBox x = U->box[sbox[s]];
+        U->handled[sbox[s]] = 0;
+        for (int i = 0; i < p->n; ++i) {                                /* balls kept by id: the box's order */
+            const OBall* o = &p->balls[i];
+            if (!ball_in_box(o, x) || handled[i] || i == bi) continue;
+            if (o->is_moribund) continue;
+            if (o->cloaked || !o->is_massive) continue;
+            if (o->mode == MISSILE) continue;
+            handled[i] = 1;
+            out[cnt++] = i;
+        }
+    }
+    return cnt;
+}
+
+/* --------------------------------------------------------------------------- the gradient (space.py:1612-1743) */
+
+static void integrate_t(const OPark* p, const double pp[3], const double vv[3], const double a[3], double m,
+                        double tf, double t, double pn[3], double vn[3]) {    /* space.py:1612-1628 */
+    double k = p->friction;
+    if (t == 0.0) {
+        for (int i = 0; i < 3; ++i) pn[i] = pp[i], vn[i] = vv[i];
+        return;
+    }
+    if (t != p->dt) tf = xb_exp(ieee_div(-k, m) * t);
+    double k2 = k * k, ook2 = 1.0 / k2, ook = 1.0 / k;
+    double po[3] = {pp[0], pp[1], pp[2]}, vo[3] = {vv[0], vv[1], vv[2]};
+    if (k < 1e-10 * m * t) {
+        double inv_m = ieee_div(1.0, m);
+        for (int i = 0; i < 3; ++i)
+            pn[i] = (m * (a[i] * (-k * t) + k * (a[i] * t + vo[i] * k * inv_m * t)) + po[i] * k2) * ook2;
+    } else {
+        for (int i = 0; i < 3; ++i)
+            pn[i] = (m * (m * a[i] * (tf - 1.0) + k * (a[i] * t + vo[i] - tf * vo[i])) + po[i] * k2) * ook2;
+    }
+    for (int i = 0; i < 3; ++i) vn[i] = (m * a[i] - (m * a[i] - vo[i] * k) * tf) * ook;
+}
+
+static double dot3(const double u[3], const double v[3]) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+
+static double collide_two_spheres(const double p0[3], const double p1[3], const double q0[3], const double q1[3],
+                                  double cr) {                          /* space.py:1630-1648 */
+    double p0q0[3] = {p0[0] - q0[0], p0[1] - q0[1], p0[2] - q0[2]};
+    double p0q0_2 = dot3(p0q0, p0q0);
+    if (p0q0_2 <= cr * cr) return 0.0;
+    double dpq[3];
+    for (int i = 0; i < 3; ++i) dpq[i] = (p1[i] - p0[i]) - (q1[i] - q0[i]);
+    double a = dot3(dpq, dpq), b = 2.0 * dot3(p0q0, dpq), c = p0q0_2 - cr * cr;
+    if (a == 0.0 && b == 0.0) return -1.0;
+    double det = b * b - 4.0 * a * c;
+    if (det < 0.0) return -1.0;
+    det = sqrt(det);
+    double s2 = ieee_div((-b - det) * 0.5, a);
+    if (s2 >= 0.0 && s2 <= 1.0) return s2;
+    return -1.0;
+}
+
+static void add_hit(OPark* p, OBall* b, int64_t id) {
+    if (b->nh < MAX_HITS) b->hits[b->nh++] = id;
+    else p->overflow = 1;
+}
+
+static void potential(OPark* p, OBall* me, OBall* other, int depth) {   /* space.py:1663-1743 */
+    double cr = me->radius + other->radius;
+    double m1 = me->is_free ? me->mass * me->agility : 1.0e34;
+    double m2 = other->is_free ? other->mass * other->agility : 1.0e34;
+    double k = p->friction, dt = p->dt;
+    double p0[3], q0[3], p1[3], q1[3], vp1[3], vq1[3], tmp3[3], a1[3];
+    for (int i = 0; i < 3; ++i) p0[i] = me->pos[i], q0[i] = other->pos[i];
+    integrate_t(p, me->pos, me->vel, me->last_g, m1, me->tf, dt, p1, tmp3);
+    integrate_t(p, other->pos, other->vel, other->last_g, m2, other->tf, dt, q1, tmp3);
+    double s = collide_two_spheres(p0, p1, q0, q1, cr);
+    if (s == -1.0) return;
+    for (int i = 0; i < 3; ++i) p1[i] = me->pos[i], vp1[i] = me->vel[i], q1[i] = other->pos[i], vq1[i] = other->vel[i];
+    double tf = me->tf;
+    if (s > 0.0) {
+        double pa[3], va[3], qa[3], wa[3], normal[3], d[3];
+        integrate_t(p, p1, vp1, me->last_g, m1, tf, s * dt, pa, va);
+        integrate_t(p, q1, vq1, other->last_g, m2, other->tf, s * dt, qa, wa);
+        for (int i = 0; i < 3; ++i) d[i] = qa[i] - pa[i];
+        normalized(d, normal);
+        double v1 = dot3(va, normal), v2 = dot3(wa, normal);
+        if (!other->is_free) {                                          /* COL-707: reflected off a fixed ball */
+            double c = 2.0 * v1;
+            for (int i = 0; i < 3; ++i) va[i] = va[i] - normal[i] * c;
+        } else {                                                        /* COL-708: exchanged with a free one */
+            double mm1 = me->mass, mm2 = other->mass;
+            double v1p = ieee_div(mm1 * v1 - mm2 * v1 + 2.0 * mm2 * v2, mm1 + mm2);
+            for (int i = 0; i < 3; ++i) va[i] = va[i] + normal[i] * (v1p - v1);
+        }
+        double pe[3], ve[3];
+        integrate_t(p, pa, va, me->last_g, m1, tf, (1.0 - s) * dt, pe, ve);
+        const double* g = me->last_g;
+        const double* v0 = me->vel;
+        double inv_m = ieee_div(1.0, m1), inv_t = ieee_div(1.0, tf - 1.0);
+        for (int i = 0; i < 3; ++i)
+            a1[i] = -((((g[i] * -m1) + g[i] * (tf * m1)) - (v0[i] * tf) * k) + ve[i] * k) * inv_m * inv_t;
+    } else {                                                            /* COL-709: already overlapping */
+        double p0q0[3] = {q0[0] - p0[0], q0[1] - p0[1], q0[2] - p0[2]}, normal[3];
+        double p0q0_2 = dot3(p0q0, p0q0);
+        if (p0q0_2 == 0.0) {
+            normal[0] = me->id > other->id ? 1.0 : -1.0;
+            normal[1] = normal[2] = 0.0;
+        } else {
+            double d[3] = {q1[0] - p1[0], q1[1] - p1[1], q1[2] - p1[2]};
+            normalized(d, normal);
+        }
+        double dist = cr - sqrt(p0q0_2) + 1;
+        double d1 = ieee_div(m2, m1 + m2) * dist;
+        double d2 = ieee_div(m1, m1 + m2) * dist;
+        if (me->is_free && other->is_free && depth < 2) {              /* COL-710: pushed apart, then again */
+            double tm[3], to[3];
+            for (int i = 0; i < 3; ++i) {
+                tm[i] = me->pos[i], to[i] = other->pos[i];
+                me->pos[i] = tm[i] - normal[i] * d1;
+            }
+            for (int i = 0; i < 3; ++i) other->pos[i] = to[i] + normal[i] * d2;
+            potential(p, me, other, depth + 1);
+            for (int i = 0; i < 3; ++i) me->pos[i] = tm[i], other->pos[i] = to[i];
+            if (ieee_div(m2, m1) > LIGHT_BALL_RATIO) {
+                double u[3];
+                normalized(me->last_c, u);
+                for (int i = 0; i < 3; ++i) me->last_c[i] = u[i] * dist;
+            }
+            double dv[3] = {me->vel[0] - other->vel[0], me->vel[1] - other->vel[1], me->vel[2] - other->vel[2]};
+            if (lensq(dv) > 0.0001) return;
+        }
+        double normal_comp = dot3(me->last_g, normal);
+        double tmp = 1.0 / (m1 + dt * k);
+        double c = ieee_div(-d1, dt * tmp * m1);
+        double inv_dt = 1.0 / dt;
+        for (int i = 0; i < 3; ++i) a1[i] = (normal[i] * c - me->vel[i]) * inv_dt - normal[i] * normal_comp;
+    }
+    if ((me->mode != MISSILE || other->id != me->follow_id) && s >= me->last_collision) {   /* COL-711 */
+        double lc[3] = {a1[0] * RESPONSE_SCALE, a1[1] * RESPONSE_SCALE, a1[2] * RESPONSE_SCALE};
+        if (s == me->last_collision) {
+            if (lensq(lc) > lensq(me->last_c)) me->last_c[0] = lc[0], me->last_c[1] = lc[1], me->last_c[2] = lc[2];
+        } else {
+            me->last_c[0] = lc[0], me->last_c[1] = lc[1], me->last_c[2] = lc[2];
+            me->last_collision = s;
+        }
+    }
+    add_hit(p, me, other->id);
+}
+
+static OBoxes g_boxes;                                                  /* the walk's work space (one thread) */
+
+static void gradient(OPark* p, int bi) {                                /* space.py:1650-1661 */
+    OBall* b = &p->balls[bi];
+    int cand[MAX_BALLS];
+    int nc = nearby_balls(p, &g_boxes, bi, cand);
+    for (int c = 0; c < nc; ++c) {
+        OBall* n = &p->balls[cand[c]];
+        if (((n->mode == MISSILE || n->mode == MUSHROOM) && b->id == n->owner_id) ||
+            (n->mode == MISSILE && b->id == n->follow_id) || (n->is_junk && !b->is_junk))
+            continue;
+        potential(p, b, n, 0);
+    }
+}
+
+/* SpaceBallpark's refiling (InsertBallInBoxes) where the reference calls it; a no-op without collisions. */
+static void refile(OPark* p, OBall* b) {
+    if (p->collisions) file_ball(p, b);
+}
+
 void or_evolve(OPark* p) {                                              /* modes.py:702-745 */
     double np[MAX_BALLS][3], nv[MAX_BALLS][3];
     int live[MAX_BALLS];
     for (int i = 0; i < p->n; ++i) {                                    /* (1) behaviours, ascending id */
         OBall* b = &p->balls[i];
         live[i] = b->is_free && !b->is_moribund;
-        if (live[i]) behaviour(p, b);
+        if (live[i]) {
+            b->nh = 0;                                                  /* space.py:1994-1997 */
+            behaviour(p, b);
+        }
     }
     for (int i = 0; i < p->n; ++i) {                                    /* (3) end-of-tick states */
         OBall* b = &p->balls[i];
         if (!live[i]) continue;
-        if (b->is_massive) b->last_collision = -1.0;
+        if (b->is_massive) {
+            b->last_collision = -1.0;
+            if (p->collisions) gradient(p, i);                          /* space.py:1650, after the lower ids */
+        }
         double pp[3] = {b->pos[0], b->pos[1], b->pos[2]}, vv[3] = {b->vel[0], b->vel[1], b->vel[2]};
         double a[3] = {b->last_g[0] + b->last_c[0], b->last_g[1] + b->last_c[1], b->last_g[2] + b->last_c[2]};
         if (b->mode == WARP && b->effect_stamp >= 0) {
@@ -1124,6 +1500,7 @@ void or_evolve(OPark* p) {                                              /* modes
             if (b->mode == STOP) {                                      /* TRV-714: from the drop-out point */
                 for (int k = 0; k < 3; ++k) b->pos[k] = pp[k], b->vel[k] = vv[k];
                 integrate(p, pp, vv, a, b->mass * b->agility, b->tf, np[i], nv[i]);
+                if (lensq(b->vel) < 0.1) refile(p, b);                  /* modes.py:730-732 */
             } else {
                 for (int k = 0; k < 3; ++k) np[i][k] = pp[k], nv[i][k] = vv[k];
             }
@@ -1134,11 +1511,14 @@ void or_evolve(OPark* p) {                                              /* modes
     for (int i = 0; i < p->n; ++i) {                                    /* (4) commit */
         OBall* b = &p->balls[i];
         if (!live[i]) continue;
+        double d[3];
         for (int k = 0; k < 3; ++k) {
             b->old_pos[k] = b->pos[k], b->old_vel[k] = b->vel[k];
             b->pos[k] = np[i][k], b->vel[k] = nv[i][k];
+            d[k] = b->pos[k] - b->old_pos[k];
         }
         if (p->dynamical) commit_rotation(p, b);
+        if (!(lensq(d) < 0.1)) refile(p, b);                            /* modes.py:737-743 */
     }
     p->current_time += 1;                                               /* (7) */
 }
@@ -1173,6 +1553,25 @@ int or_get_followers(const OPark* p, int k, int64_t* out) {
     memcpy(out, b->followers, sizeof(int64_t) * b->nf);
     qsort(out, b->nf, sizeof(int64_t), cmp_i64);
     return b->nf;
+}
+
+/* The collision park's switch (before any ball is added), space junk, the overflow flag. */
+void or_park_set_collisions(OPark* p, int on) { p->collisions = on; }
+int or_overflow(const OPark* p) { return p->overflow; }
+void or_set_junk(OPark* p, int64_t id, int junk) {
+    OBall* b = find(p, id);
+    if (b) b->is_junk = junk;
+}
+
+/* Ball k's collision state: d = mLastC (3); q = level, key (3), side (3), the collision list's length; hits = the
+   list. Returns the list's length. */
+int or_get_collide(const OPark* p, int k, double* d, int64_t* q, int64_t* hits) {
+    if (k < 0 || k >= p->n) return -1;
+    const OBall* b = &p->balls[k];
+    for (int i = 0; i < 3; ++i) d[i] = b->last_c[i], q[1 + i] = b->key[i], q[4 + i] = b->mod[i];
+    q[0] = b->level, q[7] = b->nh;
+    memcpy(hits, b->hits, sizeof(int64_t) * b->nh);
+    return b->nh;
 }
 
 /* The maths the oracle calls, for the tests that pin it (0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 its cosine) */

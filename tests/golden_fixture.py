@@ -1,5 +1,6 @@
-"""Loads tests/golden/*.npz: the reference's own scalar flight-modes world, every ball after every tick (written by
-tests/golden/make_golden_modes.py), in the oracle's column order."""
+"""Loads tests/golden/*.npz: the reference's own scalar worlds, every ball after every tick (written by the scripts
+in tests/golden/), in the oracle's column order; each record's list is the ball's followers, or its collision list
+in the collision fixture."""
 
 from __future__ import annotations
 
@@ -21,10 +22,11 @@ def load(name: str = "modes_golden.npz") -> dict:
         w["cmds"] = [tuple(c) for c in w["cmds"]]
     rec = {}
     d = z["d"].view(np.float64)
-    fo, fi = z["fol_off"], z["fol_ids"]
-    for r in range(len(z["world"])):
-        key = (int(z["world"][r]), int(z["step"][r]))
-        rec.setdefault(key, []).append((int(z["ball"][r]), d[r], z["q"][r], [int(x) for x in fi[fo[r]:fo[r + 1]]]))
+    fo, fi = (z["hit_off"], z["hit_ids"]) if "hit_off" in z else (z["fol_off"], z["fol_ids"])   # the per-ball lists
+    world, step, ball, q = z["world"], z["step"], z["ball"], z["q"]    # each read of z[...] decompresses it again
+    for r in range(len(world)):
+        key = (int(world[r]), int(step[r]))
+        rec.setdefault(key, []).append((int(ball[r]), d[r], q[r], [int(x) for x in fi[fo[r]:fo[r + 1]]]))
     out = {}
     for key, balls in rec.items():
         balls.sort(key=lambda t: t[0])

@@ -368,3 +368,180 @@ def do_cases(seed: int, scale: int = 1) -> list:
                 cmds.append(_cmd(step, op, bid))
         world("do_setters", balls, cmds, r.randint(3, 20), tick=tick_choice(), friction=friction_choice())
     return out
+
+
+# ----------------------------------------------------------------------------------------------- collisions
+VEL_OP = 8                                                            # SetBallVelocity
+
+
+def _coll_ball(r: random.Random, bid: int, centre: list, spread: float, **kw) -> dict:
+    """A ball for the collision worlds: massive and free unless told otherwise, near `centre`."""
+    b = _ball(r, bid, massive=True, interactive=r.random() < 0.5)
+    b["radius"] = r.choice([r.uniform(20.0, 500.0), r.uniform(20.0, 500.0), r.uniform(1.0, 20.0)])
+    b["mass"] = r.choice([10 ** r.uniform(6.0, 8.0), 10 ** r.uniform(6.0, 8.0), 10 ** r.uniform(4.5, 9.5)])
+    b["pos"] = [centre[i] + r.uniform(-spread, spread) for i in range(3)]
+    b["vel"] = r.choice([[0.0, 0.0, 0.0], [r.uniform(-100.0, 100.0) for _ in range(3)]])
+    b["junk"] = False
+    b.update(kw)
+    return b
+
+
+def _sphere(r: random.Random, bid: int, centre: list, spread: float, **kw) -> dict:
+    """A fixed, massive sphere (a static obstacle in the gradient: mass 1e34 in the response)."""
+    return _coll_ball(r, bid, centre, spread, free=False, vel=[0.0, 0.0, 0.0],
+                      radius=r.choice([r.uniform(100.0, 5000.0), r.uniform(10.0, 300.0)]), **kw)
+
+
+def collide_cases(seed: int, scale: int = 1) -> list:
+    """The collision worlds (about 150 x scale) for seed: clusters of massive balls, free and fixed, meeting head on,
+    starting overlapped (coincident centres among them, so Potential recurses), against fixed spheres, across
+    partition levels (radii from 1 m to 50 km, so the contact order walks ancestors and descendants), in symmetric
+    approaches (equal impact times), with the other modes (follow and orbit at touching ranges, a warp that drops out
+    into a cluster) and with the candidate filters (non-massive, cloaked, space junk)."""
+    r = random.Random(seed)
+    out: list = []
+
+    def world(kind, balls, cmds, steps, *, tick=1000, friction=1.0e6, new_orbit=False):
+        out.append({"kind": kind, "friction": friction, "tick": tick, "steps": steps, "new_orbit": new_orbit,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": cmds, "collisions": True})
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    def filters(balls):
+        for b in balls:
+            x = r.random()
+            if x < 0.08:
+                b["massive"] = False
+            elif x < 0.14:
+                b["cloaked"] = 1
+            elif x < 0.2:
+                b["junk"] = True
+
+    for _ in range(40 * scale):                    # head on: goto through the cluster
+        c = centre()
+        ids = _ids(r, r.randint(2, 7), big=r.random() < 0.3)
+        balls, cmds = [], []
+        for k, bid in enumerate(ids):
+            if k >= 2 and r.random() < 0.3:
+                balls.append(_sphere(r, bid, c, 1500.0))
+                continue
+            b = _coll_ball(r, bid, c, 2000.0)
+            balls.append(b)
+            tgt = [2 * c[i] - b["pos"][i] + r.uniform(-300.0, 300.0) for i in range(3)]
+            cmds.append(_cmd(r.choice([1, 1, 2, 4]), GOTO_OP, bid, x=tgt[0], y=tgt[1], z=tgt[2]))
+        filters(balls)
+        if r.random() < 0.3:
+            cmds.append(_cmd(r.randint(3, 10), STOP_OP, r.choice(ids)))
+        world("head_on", balls, cmds, r.randint(10, 40), tick=r.choice([1000, 1000, 500, 2000]))
+
+    for _ in range(30 * scale):                    # overlapped starts, coincident centres
+        c = centre()
+        ids = _ids(r, r.randint(2, 5), big=False)
+        balls = []
+        for k, bid in enumerate(ids):
+            b = _coll_ball(r, bid, c, 150.0, free=r.random() < 0.85)
+            if not b["free"]:
+                b["vel"] = [0.0, 0.0, 0.0]
+            if k > 0 and r.random() < 0.3:
+                b["pos"] = list(balls[r.randrange(k)]["pos"])
+            if k > 0 and r.random() < 0.3 and b["free"] and balls[0]["free"]:
+                b["vel"] = list(balls[0]["vel"])
+            balls.append(b)
+        filters(balls)
+        cmds = []
+        for bid in ids:
+            if r.random() < 0.4:
+                p = [c[i] + r.uniform(-3000.0, 3000.0) for i in range(3)]
+                cmds.append(_cmd(r.choice([1, 2, 5]), GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        world("overlap", balls, cmds, r.randint(5, 25), tick=r.choice([1000, 1000, 250, 2000]),
+              friction=r.choice([1.0e6, 1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]))
+
+    for _ in range(20 * scale):                    # a ship through a field of fixed spheres
+        c = centre()
+        ids = _ids(r, r.randint(3, 9), big=False)
+        ship = _coll_ball(r, ids[0], c, 10.0, vel=[0.0, 0.0, 0.0])
+        u = _unit(r)
+        balls = [ship]
+        for bid in ids[1:]:
+            d = r.uniform(200.0, 6000.0)
+            off = [r.uniform(-400.0, 400.0) for _ in range(3)]
+            s = _sphere(r, bid, c, 0.0)
+            s["pos"] = [c[i] + u[i] * d + off[i] for i in range(3)]
+            balls.append(s)
+        filters(balls[1:])
+        tgt = [c[i] + u[i] * 9000.0 for i in range(3)]
+        cmds = [_cmd(1, GOTO_OP, ids[0], x=tgt[0], y=tgt[1], z=tgt[2])]
+        if r.random() < 0.5:
+            v = [u[i] * ship["max_vel"] for i in range(3)]
+            cmds.append(_cmd(1, VEL_OP, ids[0], x=v[0], y=v[1], z=v[2]))
+        world("spheres", balls, cmds, r.randint(10, 40))
+
+    for _ in range(20 * scale):                    # partition levels: radii 1 m to 50 km near box boundaries
+        lvl_w = [7864320.0 / 4 ** i for i in range(8)]
+        base = lvl_w[r.randint(3, 7)]
+        c = [round(r.uniform(-1e6, 1e6) / base) * base for _ in range(3)]
+        ids = _ids(r, r.randint(3, 8), big=r.random() < 0.3)
+        balls, cmds = [], []
+        for bid in ids:
+            rad = 10 ** r.uniform(0.0, 4.7)
+            b = _coll_ball(r, bid, c, 0.0, radius=rad, free=r.random() < 0.7)
+            b["pos"] = [c[i] + r.choice([-1, 1]) * r.uniform(0.0, 1.5 * rad + 600.0) for i in range(3)]
+            if not b["free"]:
+                b["vel"] = [0.0, 0.0, 0.0]
+            balls.append(b)
+            if b["free"] and r.random() < 0.7:
+                p = [c[i] + r.uniform(-5000.0, 5000.0) for i in range(3)]
+                cmds.append(_cmd(r.choice([1, 2, 3]), GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        filters(balls)
+        world("levels", balls, cmds, r.randint(8, 30))
+
+    for _ in range(10 * scale):                    # symmetric approaches: equal impact times
+        c = centre()
+        ids = _ids(r, 3, big=False)
+        rad, sp, d = r.uniform(50.0, 300.0), r.uniform(20.0, 200.0), r.uniform(800.0, 3000.0)
+        mass, ag, mv = 10 ** r.uniform(6.0, 8.0), r.uniform(0.5, 2.0), r.uniform(100.0, 500.0)
+        mid = _coll_ball(r, ids[0], c, 0.0, radius=rad, mass=mass, agility=ag, max_vel=mv, sf=1.0,
+                         vel=[0.0, 0.0, 0.0], free=r.random() < 0.7)
+        left = _coll_ball(r, ids[1], c, 0.0, radius=rad, mass=mass, agility=ag, max_vel=mv, sf=1.0,
+                          pos=[c[0] - d, c[1], c[2]], vel=[sp, 0.0, 0.0])
+        right = _coll_ball(r, ids[2], c, 0.0, radius=rad, mass=mass, agility=ag, max_vel=mv, sf=1.0,
+                           pos=[c[0] + d, c[1], c[2]], vel=[-sp, 0.0, 0.0])
+        world("symmetric", [mid, left, right], [], r.randint(10, 30))
+
+    for _ in range(20 * scale):                    # the other modes: follow and orbit touching, a warp into a cluster
+        c = centre()
+        ids = _ids(r, r.randint(3, 6), big=False)
+        balls = [_coll_ball(r, bid, c, 1500.0) for bid in ids]
+        filters(balls[1:])
+        cmds = []
+        kind = r.choice(["follow", "orbit", "warp", "warp"])
+        if kind == "warp":
+            u = _unit(r)
+            w = balls[0]
+            w["pos"] = [c[i] + u[i] * r.uniform(1.2e5, 4e5) for i in range(3)]
+            w["sf"], w["massive"] = 1.0, True
+            w["vel"] = [-u[i] * w["max_vel"] for i in range(3)]
+            cmds.append(_cmd(1, WARP_OP, w["id"], x=c[0], y=c[1], z=c[2], rr=r.uniform(0.0, 500.0),
+                             w=r.choice([3000, 3000, 6000, 1500])))
+        for b in balls[1:]:
+            if r.random() < 0.6:
+                op = FOLLOW_OP if kind != "orbit" else ORBIT_OP
+                t = r.choice([x for x in balls if x is not b])
+                cmds.append(_cmd(r.choice([1, 2]), op, b["id"], t["id"], rr=r.uniform(0.0, 100.0)))
+        world(kind, balls, cmds, r.randint(20, 45) if kind == "warp" else r.randint(10, 40), new_orbit=r.random() < 0.5)
+
+    for _ in range(15 * scale):                    # a warp dropping out into a cluster, ids either side of the warper
+        c = centre()
+        ids = _ids(r, r.randint(4, 7), big=False)
+        k = r.randrange(len(ids))
+        balls = [_coll_ball(r, bid, c, 600.0, radius=r.uniform(100.0, 600.0)) for bid in ids]
+        filters(balls)
+        w = balls[k]
+        u = _unit(r)
+        w.update(pos=[c[i] + u[i] * r.uniform(1.2e5, 3e5) for i in range(3)], sf=1.0, massive=True, cloaked=0,
+                 junk=False, max_vel=r.uniform(200.0, 1500.0))
+        w["vel"] = [-u[i] * w["max_vel"] for i in range(3)]
+        cmds = [_cmd(1, WARP_OP, w["id"], x=c[0], y=c[1], z=c[2], rr=r.uniform(0.0, 300.0), w=r.choice([3000, 6000]))]
+        world("dropout", balls, cmds, r.randint(25, 40))
+    return out
