@@ -31,6 +31,10 @@ XbPark host_park(const xb_modes_state& s) {
     P.ook = 1.0 / s.friction;
     P.new_orbit = s.new_orbit;
     P.dynamical = s.dynamical;
+    P.cf = s.coll_f64;
+    P.cq = s.coll_i64;
+    P.hits = s.hits;
+    P.hits_cap = s.hits_cap;
     return P;
 }
 
@@ -55,7 +59,87 @@ struct Scratch {
         R.id = id.data();
         R.act = act.data();
     }
+    /* the collision records (XbCollRecs) of a group */
+    std::vector<double> cd;
+    std::vector<int64_t> cq;
+    std::vector<int8_t> cb;
+    std::vector<uint8_t> cflags;
+    XbCollRecs C;
+    void size_coll(int m) {
+        if ((int)cflags.size() >= m) return;
+        cd.assign((size_t)15 * m, 0.0);
+        cq.assign((size_t)4 * m, 0);
+        cb.assign((size_t)4 * m, 0);
+        cflags.assign(m, 0);
+        double* d0 = cd.data();
+        C.px = d0, C.py = d0 + m, C.pz = d0 + 2 * m, C.vx = d0 + 3 * m, C.vy = d0 + 4 * m, C.vz = d0 + 5 * m;
+        C.gx = d0 + 6 * m, C.gy = d0 + 7 * m, C.gz = d0 + 8 * m, C.ex = d0 + 9 * m, C.ey = d0 + 10 * m;
+        C.ez = d0 + 11 * m, C.mass = d0 + 12 * m, C.m = d0 + 13 * m, C.tf = d0 + 14 * m;
+        C.kx = cq.data(), C.ky = C.kx + m, C.kz = C.ky + m, C.owner = C.kz + m;
+        C.level = cb.data(), C.mx = C.level + m, C.my = C.mx + m, C.mz = C.my + m;
+        C.flags = cflags.data();
+    }
 };
+
+/* A group's ticks with the gradient collision method: xb_collide_tiled_kernel's phases, each over the whole group
+   before the next (each world's rows consecutive, at most 64). */
+template <bool DO>
+void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector<XbRow>& rows, int64_t lo,
+                    int64_t cnt) {
+    S.size((int)cnt);
+    S.size_coll((int)cnt);
+    rows.resize((size_t)cnt);
+    std::vector<XbCollRow> cr((size_t)cnt);
+    std::vector<int> wlo((size_t)cnt), whi((size_t)cnt);
+    std::vector<XoQ> rot1((size_t)cnt);
+    std::vector<XoV3> w1((size_t)cnt);
+    std::vector<XbV3> pe((size_t)cnt), ve((size_t)cnt);
+    std::vector<char> warped((size_t)cnt);
+    for (int64_t k = 0; k < cnt; ++k) {
+        xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
+        if (DO) xb_turn_load(P, rows[k].row, rows[k].t);
+        xb_coll_load(P, rows[k].row, cr[k]);
+    }
+    for (int64_t k = 0; k < cnt; ++k) {
+        const int64_t wk = XB_Q(P, XB_I_WORLD, rows[k].row);
+        int a = (int)k, b = (int)k + 1;
+        while (a > 0 && XB_Q(P, XB_I_WORLD, rows[a - 1].row) == wk) --a;
+        while (b < cnt && XB_Q(P, XB_I_WORLD, rows[b].row) == wk) ++b;
+        wlo[k] = a, whi[k] = b;
+    }
+    const int ticks = s.ticks > 0 ? s.ticks : 1;
+    for (int t = 0; t < ticks; ++t) {
+        for (int64_t k = 0; k < cnt; ++k) xb_record_a(rows[k], S.R, (int)k);
+        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        for (int64_t k = 0; k < cnt; ++k) {
+            rot1[k] = xo_q(0.0f, 0.0f, 0.0f, 1.0f);
+            w1[k] = xo_v(0.0f, 0.0f, 0.0f);
+            xb_phase1<DO>(rows[k], S.R, (int)k, 0, (int)cnt, P, rot1[k], w1[k]);
+            if (xb_row_live(rows[k])) {
+                cr[k].last_c = v3(0.0, 0.0, 0.0);
+                cr[k].nh = 0;
+            }
+            xb_coll_record(rows[k], cr[k], S.C, (int)k, P);
+        }
+        for (int64_t k = 0; k < cnt; ++k) {
+            const XbShip& sh = rows[k].s;
+            warped[k] = xb_row_live(rows[k]) && sh.mode == XB_MODE_WARP && sh.effect_stamp >= 0;
+            pe[k] = v3(0.0, 0.0, 0.0), ve[k] = pe[k];
+            if (warped[k]) xb_coll_phase3_warp(rows[k], cr[k], S.C, (int)k, P, &pe[k], &ve[k]);
+        }
+        for (int64_t k = 0; k < cnt; ++k) {
+            xb_coll_phase34<DO>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k], whi[k], P,
+                                rot1[k], w1[k]);
+            xb_tick_tail(rows[k]);
+        }
+    }
+    for (int64_t k = 0; k < cnt; ++k) {
+        xb_row_store(P, rows[k]);
+        if (DO && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
+        xb_coll_store(P, rows[k].row, cr[k]);
+        if (s.path) s.path[rows[k].row] = rows[k].path;
+    }
+}
 
 void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector<XbRow>& rows, int64_t lo,
                int64_t cnt) {
@@ -94,6 +178,26 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
     if (!state || state->n < 0 || state->stride < state->n) return -1;
     if (state->n == 0) return 0;
     const XbPark P = host_park(*state);
+    if (state->collisions) {
+        if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 || state->hits_cap < 0 ||
+            (state->hits_cap > 0 && !state->hits))
+            return -1;
+        const int nt = state->n_tiles;
+        const bool dyn = state->dynamical != 0;
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+        {
+            Scratch S;
+            std::vector<XbRow> rows;
+#pragma omp for schedule(dynamic, 16)
+            for (int g = 0; g < nt; ++g) {
+                const int64_t lo = state->tiles[g], cnt = (int64_t)state->tiles[g + 1] - state->tiles[g];
+                if (dyn) run_group_coll<true>(P, *state, S, rows, lo, cnt);
+                else run_group_coll<false>(P, *state, S, rows, lo, cnt);
+            }
+        }
+        return 0;
+    }
     if (!state->tiles) {
         Scratch S;
         std::vector<XbRow> rows;
@@ -109,6 +213,24 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
 #pragma omp for schedule(dynamic, 16)
         for (int g = 0; g < nt; ++g)
             run_group(P, *state, S, rows, state->tiles[g], (int64_t)state->tiles[g + 1] - state->tiles[g]);
+    }
+    return 0;
+}
+
+int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || !state->coll_f64 || !state->coll_i64 || state->n < 0 ||
+        state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
+        return -1;
+    const XbPark P = host_park(*state);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r < 0 || r >= state->n) continue;
+        XbRow w;
+        xb_row_load(P, r, w);
+        XbCollRow c;
+        xb_coll_load(P, r, c);
+        xb_file(w.s, w.free_, P, c.fil);
+        xb_coll_store(P, r, c);
     }
     return 0;
 }

@@ -36,6 +36,10 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.ook = 1.0 / s.friction;
     P.new_orbit = s.new_orbit;
     P.dynamical = s.dynamical;
+    P.cf = s.coll_f64;
+    P.cq = s.coll_i64;
+    P.hits = s.hits;
+    P.hits_cap = s.hits_cap;
     return P;
 }
 
@@ -127,6 +131,116 @@ __global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
         if (DO && w.turned) xb_turn_store(P, r, w.t);
         if (s.path) s.path[r] = w.path;
     }
+}
+
+// The collision records of a tile in LDS, after the tick's own records (XB_LDS_BYTES_PER_ROW a lane).
+constexpr int XB_COLL_LDS_BYTES_PER_ROW = 15 * 8 + 4 * 8 + 4 + 1;
+
+__device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
+    XbCollRecs C;
+    double* d = base;
+    C.px = d, C.py = d + nb, C.pz = d + 2 * nb, C.vx = d + 3 * nb, C.vy = d + 4 * nb, C.vz = d + 5 * nb;
+    C.gx = d + 6 * nb, C.gy = d + 7 * nb, C.gz = d + 8 * nb, C.ex = d + 9 * nb, C.ey = d + 10 * nb;
+    C.ez = d + 11 * nb, C.mass = d + 12 * nb, C.m = d + 13 * nb, C.tf = d + 14 * nb;
+    int64_t* q = reinterpret_cast<int64_t*>(d + 15 * nb);
+    C.kx = q, C.ky = q + nb, C.kz = q + 2 * nb, C.owner = q + 3 * nb;
+    int8_t* b = reinterpret_cast<int8_t*>(q + 4 * nb);
+    C.level = b, C.mx = b + nb, C.my = b + 2 * nb, C.mz = b + 3 * nb;
+    C.flags = reinterpret_cast<uint8_t*>(b + 4 * nb);
+    return C;
+}
+
+// The tiled tick with the gradient collision method: the phases of Ballpark::Evolve with a barrier between them, as
+// the reference's order needs (modes.py:702-745): (A, B) the tick-start records; (1) the behaviours, then each row's
+// view for the gradients; (3a) the rows in warp proper, whose drop-out makes them massive for the balls after them;
+// (3b) each massive row's gradient over its world's rows, its end state; (4) the commit and the refiling.
+template <bool DO>
+__global__ void __launch_bounds__(256) xb_collide_tiled_kernel(xb_modes_state s) {
+    extern __shared__ double xb_lds[];
+    const int nb = blockDim.x;
+    const int g = blockIdx.x;
+    const int lo = s.tiles[g];
+    const int cnt = s.tiles[g + 1] - lo;
+    XbRecords R;
+    R.p0x = xb_lds;
+    R.p0y = R.p0x + nb;
+    R.p0z = R.p0y + nb;
+    R.v0x = R.p0z + nb;
+    R.v0y = R.v0x + nb;
+    R.v0z = R.v0y + nb;
+    R.vy_after = R.v0z + nb;
+    R.radius = R.vy_after + nb;
+    R.id = reinterpret_cast<int64_t*>(R.radius + nb);
+    int* perm = reinterpret_cast<int*>(R.id + nb);
+    R.act = reinterpret_cast<uint8_t*>(perm + nb);
+    const XbCollRecs C = xb_coll_lds(xb_lds + (XB_LDS_BYTES_PER_ROW * nb) / 8, nb);
+    const XbPark P = xb_park(s);
+    const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
+    const bool on = j < cnt;
+    const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    const int ticks = s.ticks > 0 ? s.ticks : 1;
+    XbRow w;
+    XbCollRow c;
+    if (on) {
+        xb_row_load(P, r, w);
+        if (DO) xb_turn_load(P, r, w.t);
+        xb_coll_load(P, r, c);
+    }
+    C.kx[j] = on ? XB_Q(P, XB_I_WORLD, r) : -1;     // the world's places: a run of equal world ids
+    __syncthreads();
+    int wlo = j, whi = j + 1;
+    if (on) {
+        while (wlo > 0 && C.kx[wlo - 1] == C.kx[j]) --wlo;
+        while (whi < cnt && C.kx[whi] == C.kx[j]) ++whi;
+    }
+    __syncthreads();
+    for (int t = 0; t < ticks; ++t) {
+        if (on) xb_record_a(w, R, j);
+        __syncthreads();
+        if (on) xb_record_b(w, R, j, 0, cnt, P);
+        __syncthreads();
+        XoQ rot1 = xo_q(0.0f, 0.0f, 0.0f, 1.0f);
+        XoV3 w1 = xo_v(0.0f, 0.0f, 0.0f);
+        if (on) {
+            xb_phase1<DO>(w, R, j, 0, cnt, P, rot1, w1);
+            if (xb_row_live(w)) {
+                c.last_c = v3(0.0, 0.0, 0.0);
+                c.nh = 0;
+            }
+            xb_coll_record(w, c, C, j, P);
+        }
+        __syncthreads();
+        const bool warped = on && xb_row_live(w) && w.s.mode == XB_MODE_WARP && w.s.effect_stamp >= 0;
+        XbV3 p = v3(0.0, 0.0, 0.0), v = p;
+        if (warped) xb_coll_phase3_warp(w, c, C, j, P, &p, &v);
+        __syncthreads();
+        if (on) {
+            xb_coll_phase34<DO>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1);
+            xb_tick_tail(w);
+        }
+        __syncthreads();
+    }
+    if (on) {
+        xb_row_store(P, w);
+        if (DO && w.turned) xb_turn_store(P, r, w.t);
+        xb_coll_store(P, r, c);
+        if (s.path) s.path[r] = w.path;
+    }
+}
+
+// One lane per listed row: Ball::InsertInBoxes's outcome into the filing rows.
+__global__ void xb_collide_file_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    const XbPark P = xb_park(s);
+    XbRow w;
+    xb_row_load(P, r, w);
+    XbCollRow c;
+    xb_coll_load(P, r, c);
+    xb_file(w.s, w.free_, P, c.fil);
+    xb_coll_store(P, r, c);
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -278,6 +392,15 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     const bool dyn = state->dynamical != 0;
     hipStream_t st = (hipStream_t)stream;
     if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
+    if (state->collisions) {
+        if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 ||
+            (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
+            return -1;
+        const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
+        if (dyn) xb_collide_tiled_kernel<true><<<state->n_tiles, block, lds, st>>>(*state);
+        else xb_collide_tiled_kernel<false><<<state->n_tiles, block, lds, st>>>(*state);
+        return xb_launch_rc();
+    }
     if (state->tiles) {
         if (state->n_tiles <= 0) return -1;
         const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
@@ -305,6 +428,15 @@ int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int6
     if (n_rows == 0) return 0;
     xb_orient_setup_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows,
                                                                                                what);
+    return xb_launch_rc();
+}
+
+int xb_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, void* stream) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || !state->coll_f64 || !state->coll_i64 || state->n < 0 ||
+        state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
+        return -1;
+    if (n_rows == 0) return 0;
+    xb_collide_file_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows);
     return xb_launch_rc();
 }
 

@@ -97,6 +97,10 @@ struct XbPark {
     double k2, ook2, ook;          /* friction^2, 1/friction^2, 1/friction: the integrator's per-park constants */
     int new_orbit;
     int dynamical;                 /* dynamical orientation (MOV-728) */
+    double* cf;                    /* the collision blocks (with the gradient method): float64 mLastC */
+    int64_t* cq;                   /* int64 filing and the collision list's length */
+    int64_t* hits;                 /* the collision lists, hits_cap rows */
+    int hits_cap;
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -595,14 +599,15 @@ static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, 
 
 /* Step C: the row's whole tick (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi). Other rows are
    read only through the records. */
+/* Phase 1 of the row at place j (Ballpark::Evolve's behaviours, modes.py:707-716): a release when the target starts
+   warp proper, before or after this ball's own turn by their ids; the behaviour (mLastG; with dynamical orientation
+   the rotation a tick on, into rot1 and w1); mLastC cleared. */
 template <bool DO>
-static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
+static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P, XoQ& rot1,
+                                   XoV3& w1) {
     XbShip& s = w.s;
-    XoQ rot1 = xo_q(0.0f, 0.0f, 0.0f, 1.0f);     /* with dynamical orientation: the rotation and angular velocity */
-    XoV3 w1 = xo_v(0.0f, 0.0f, 0.0f);           /* a tick on */
     const bool live = xb_row_live(w);
     const bool act_self = R.act[j] != 0;
-    /* phase 1: a release when the target starts warp proper, before or after this ball's own turn by their ids */
     bool rel_before = false, rel_after = false;
     int tj = xb_member_place(w, R, j, lo, hi);
     if (tj >= 0 && R.act[tj]) {
@@ -650,34 +655,43 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         s.last_g = a;
     }
     if (rel_after) xb_release(s, true);
-    /* phases 3 and 4 */
-    if (live) {
-        if (s.massive) s.last_collision = -1.0;
-        XbV3 p = s.pos, v = s.vel;
-        XbV3 a = v3(s.last_g.x + 0.0, s.last_g.y + 0.0, s.last_g.z + 0.0);     /* mLastG + mLastC (zero) */
-        double m = s.mass * s.agility;
-        if (s.mode == XB_MODE_WARP && s.effect_stamp >= 0) {
-            xb_warp_path(s, &p, &v, (double)(s.clock - s.effect_stamp) * P.dt);
-            if (s.mode == XB_MODE_STOP) {                     /* TRV-714: one step more from the drop-out point */
-                s.pos = p;
-                s.vel = v;
-                xb_integrate(&p, &v, a, m, s.tf, P);
-                w.path = XB_PATH_WARP_EDGE;
-            }
-        } else {
-            xb_integrate(&p, &v, a, m, s.tf, P);
-        }
-        w.old_pos = s.pos;
-        w.old_vel = s.vel;
+}
+
+/* Phase 3 of a row in warp proper (modes.py:724-732): its warp path; dropping out (TRV-714), the drop-out point
+   becomes its state and it takes one more step from there with a (mLastG + mLastC). */
+static inline XB_HD void xb_phase3_warp(XbRow& w, XbV3 a, const XbPark& P, XbV3* pp, XbV3* vp) {
+    XbShip& s = w.s;
+    XbV3 p = s.pos, v = s.vel;
+    xb_warp_path(s, &p, &v, (double)(s.clock - s.effect_stamp) * P.dt);
+    if (s.mode == XB_MODE_STOP) {
         s.pos = p;
         s.vel = v;
-        if (DO) {
-            xo_commit(w.t, rot1, w1);
-            w.turned = true;
-        }
+        xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
+        w.path = XB_PATH_WARP_EDGE;
     }
-    if (live && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
-        /* PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937): fixed, at rest, RIGID */
+    *pp = p;
+    *vp = v;
+}
+
+/* Phase 4 (modes.py:735-743): the start-of-tick state becomes the old state, the end-of-tick state the state. */
+template <bool DO>
+static inline XB_HD void xb_phase4(XbRow& w, XbV3 p, XbV3 v, XoQ rot1, XoV3 w1) {
+    XbShip& s = w.s;
+    w.old_pos = s.pos;
+    w.old_vel = s.vel;
+    s.pos = p;
+    s.vel = v;
+    if (DO) {
+        xo_commit(w.t, rot1, w1);
+        w.turned = true;
+    }
+}
+
+/* The end of a row's tick: PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937: fixed, at rest,
+   RIGID), and the world's clock. */
+static inline XB_HD void xb_tick_tail(XbRow& w) {
+    XbShip& s = w.s;
+    if (xb_row_live(w) && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
         s.vel = v3(0.0, 0.0, 0.0);
         w.old_vel = s.vel;
         s.last_g = s.vel;
@@ -686,6 +700,137 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         s.mode = XB_MODE_RIGID;
     }
     s.clock = s.clock + 1;
+}
+
+/* One tick of the row at place j of its group [lo, hi), without collisions (mLastC stays zero). */
+template <bool DO>
+static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
+    XbShip& s = w.s;
+    XoQ rot1 = xo_q(0.0f, 0.0f, 0.0f, 1.0f);     /* with dynamical orientation: the rotation and angular velocity */
+    XoV3 w1 = xo_v(0.0f, 0.0f, 0.0f);           /* a tick on */
+    xb_phase1<DO>(w, R, j, lo, hi, P, rot1, w1);
+    if (xb_row_live(w)) {
+        if (s.massive) s.last_collision = -1.0;
+        XbV3 p = s.pos, v = s.vel;
+        XbV3 a = v3(s.last_g.x + 0.0, s.last_g.y + 0.0, s.last_g.z + 0.0);     /* mLastG + mLastC (zero) */
+        if (s.mode == XB_MODE_WARP && s.effect_stamp >= 0) xb_phase3_warp(w, a, P, &p, &v);
+        else xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
+        xb_phase4<DO>(w, p, v, rot1, w1);
+    }
+    xb_tick_tail(w);
+}
+
+#include "xb_collide.h"
+
+/* ------------------------------------------------------------------------------ the tick with collisions */
+
+/* A row's collision state as its lane holds it: mLastC, the filing, the collision list's length, space junk. */
+struct XbCollRow {
+    XbV3 last_c;
+    XbFiling fil;
+    int64_t nh, cloaked;
+    bool junk;
+};
+
+static inline XB_HD void xb_coll_load(const XbPark& P, int64_t r, XbCollRow& c) {
+    const int64_t st = P.stride;
+    c.last_c = v3(P.cf[(XB_C_LAST_C + 0) * st + r], P.cf[(XB_C_LAST_C + 1) * st + r], P.cf[(XB_C_LAST_C + 2) * st + r]);
+    c.fil.level = (int)P.cq[XB_C_LEVEL * st + r];
+    for (int q = 0; q < 3; ++q) {
+        c.fil.key[q] = P.cq[(XB_C_KEY + q) * st + r];
+        c.fil.mod[q] = (int)P.cq[(XB_C_SIDE + q) * st + r];
+    }
+    c.nh = P.cq[XB_C_HITS_N * st + r];
+    c.cloaked = XB_Q(P, XB_I_CLOAKED, r);
+    c.junk = XB_B(P, XB_B_JUNK, r) != 0;
+}
+
+static inline XB_HD void xb_coll_store(const XbPark& P, int64_t r, const XbCollRow& c) {
+    const int64_t st = P.stride;
+    P.cf[(XB_C_LAST_C + 0) * st + r] = c.last_c.x;
+    P.cf[(XB_C_LAST_C + 1) * st + r] = c.last_c.y;
+    P.cf[(XB_C_LAST_C + 2) * st + r] = c.last_c.z;
+    P.cq[XB_C_LEVEL * st + r] = c.fil.level;
+    for (int q = 0; q < 3; ++q) {
+        P.cq[(XB_C_KEY + q) * st + r] = c.fil.key[q];
+        P.cq[(XB_C_SIDE + q) * st + r] = c.fil.mod[q];
+    }
+    P.cq[XB_C_HITS_N * st + r] = c.nh;
+}
+
+/* The row's tick-start view for the others' gradients (after the behaviours): position, velocity, mLastG, the tick's
+   straight prediction, masses, decay, filing, flags. */
+static inline XB_HD void xb_coll_record(const XbRow& w, const XbCollRow& c, const XbCollRecs& C, int j,
+                                        const XbPark& P) {
+    const XbShip& s = w.s;
+    const double m = w.free_ ? s.mass * s.agility : XB_FIXED_MASS;
+    XbV3 e = s.pos, ev = s.vel;
+    xb_integrate(&e, &ev, s.last_g, m, s.tf, P);
+    C.px[j] = s.pos.x, C.py[j] = s.pos.y, C.pz[j] = s.pos.z;
+    C.vx[j] = s.vel.x, C.vy[j] = s.vel.y, C.vz[j] = s.vel.z;
+    C.gx[j] = s.last_g.x, C.gy[j] = s.last_g.y, C.gz[j] = s.last_g.z;
+    C.ex[j] = e.x, C.ey[j] = e.y, C.ez[j] = e.z;
+    C.mass[j] = s.mass, C.m[j] = m, C.tf[j] = s.tf;
+    C.owner[j] = s.owner_id;
+    xb_rec_put_filing(C, j, c.fil);
+    uint8_t f = 0;
+    if (s.massive) f |= XB_CF_MASSIVE;
+    if (w.free_) f |= XB_CF_FREE;
+    if (w.moribund || c.cloaked != 0 || s.mode == XB_MODE_MISSILE) f |= XB_CF_SKIP;
+    if (c.junk) f |= XB_CF_JUNK;
+    if (s.mode == XB_MODE_MUSHROOM) f |= XB_CF_MUSHROOM;
+    C.flags[j] = f;
+}
+
+/* Phase 3 of a row in warp proper, with collisions: as xb_phase3_warp; a row that drops out is massive from then on,
+   for the balls after it in id order (its record updated: the drop-out point, its prediction, its filing when it
+   came to rest and was filed again, space.py's InsertBallInBoxes). */
+static inline XB_HD void xb_coll_phase3_warp(XbRow& w, XbCollRow& c, const XbCollRecs& C, int j, const XbPark& P,
+                                             XbV3* pp, XbV3* vp) {
+    XbShip& s = w.s;
+    const XbV3 a = v3(s.last_g.x + c.last_c.x, s.last_g.y + c.last_c.y, s.last_g.z + c.last_c.z);
+    xb_phase3_warp(w, a, P, pp, vp);
+    if (s.mode != XB_MODE_STOP) return;
+    if (v3_lensq(s.vel) < 0.1) xb_file(s, w.free_, P, c.fil);
+    const double m = s.mass * s.agility;
+    XbV3 e = s.pos, ev = s.vel;
+    xb_integrate(&e, &ev, s.last_g, m, s.tf, P);
+    C.px[j] = s.pos.x, C.py[j] = s.pos.y, C.pz[j] = s.pos.z;
+    C.vx[j] = s.vel.x, C.vy[j] = s.vel.y, C.vz[j] = s.vel.z;
+    C.ex[j] = e.x, C.ey[j] = e.y, C.ez[j] = e.z;
+    xb_rec_put_filing(C, j, c.fil);
+    C.flags[j] = (uint8_t)((C.flags[j] & ~XB_CF_MASSIVE) | XB_CF_DROPPED);
+}
+
+/* Phase 3's gradient and end state for a row not in warp proper, then phase 4 and the refiling of a row that moved
+   at least sqrt(0.1) m (modes.py:717-743). The row's world is at places [wlo, whi) of the group. */
+template <bool DO>
+static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, XbV3 p, XbV3 v, const XbCollRecs& C,
+                                         const XbRecords& R, int j, int wlo, int whi, const XbPark& P, XoQ rot1,
+                                         XoV3 w1) {
+    XbShip& s = w.s;
+    if (!xb_row_live(w)) return;
+    if (!warped) {
+        if (s.massive) {
+            s.last_collision = -1.0;
+            XbCollMe me;
+            me.pos = s.pos, me.vel = s.vel, me.g = s.last_g;
+            me.m = s.mass * s.agility, me.mass = s.mass, me.tf = s.tf, me.radius = s.radius;
+            me.id = s.id, me.mode = s.mode, me.follow_id = s.follow_id;
+            me.last_c = c.last_c, me.last_collision = s.last_collision;
+            me.nh = c.nh, me.hits = P.hits + w.row, me.stride = P.stride, me.cap = P.hits_cap;
+            xb_gradient(me, c.fil, c.junk, C, R.id, R.radius, j, wlo, whi, P);
+            c.last_c = me.last_c;
+            s.last_collision = me.last_collision;
+            c.nh = me.nh;
+        }
+        p = s.pos, v = s.vel;
+        const XbV3 a = v3(s.last_g.x + c.last_c.x, s.last_g.y + c.last_c.y, s.last_g.z + c.last_c.z);
+        xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
+    }
+    xb_phase4<DO>(w, p, v, rot1, w1);
+    const XbV3 d = v3_sub(s.pos, w.old_pos);
+    if (!(v3_lensq(d) < 0.1)) xb_file(s, w.free_, P, c.fil);
 }
 
 #endif

@@ -16,8 +16,9 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 2
-NF, NI, NB, NS = 46, 10, 5, 7
+ABI_VERSION = 3
+NF, NI, NB, NS = 46, 10, 6, 7
+NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 
 
 class EpisodeRows(ctypes.Structure):
@@ -35,7 +36,8 @@ class ModesState(ctypes.Structure):
                 ("dt", ctypes.c_double), ("ffriction", ctypes.c_float), ("fdt", ctypes.c_float),
                 ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32),
                 ("schedule", ctypes.c_int32), ("n_keys", ctypes.c_int32), ("keys", ctypes.c_void_p),
-                ("path", ctypes.c_void_p)]
+                ("path", ctypes.c_void_p), ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
+                ("hits", ctypes.c_void_p), ("hits_cap", ctypes.c_int32), ("collisions", ctypes.c_int32)]
 
 
 _LIBS: dict = {}
@@ -66,6 +68,7 @@ def hip() -> ctypes.CDLL:
         L.xb_modes_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
         L.xb_rand_seed.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.xb_rand_draw.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_collide_file.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.xb_modes_orient_setup.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                             ctypes.c_void_p]
         L.xb_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
@@ -85,6 +88,7 @@ def cpu() -> ctypes.CDLL:
         L.xb_cpu_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.xb_cpu_rand_seed.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.xb_cpu_rand_draw.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
+        L.xb_cpu_collide_file.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64]
         L.xb_cpu_modes_orient_setup.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
                                                 ctypes.c_int]
         L.xb_cpu_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \

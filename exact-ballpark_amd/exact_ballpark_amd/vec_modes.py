@@ -47,9 +47,14 @@ FIELDS = (
     ("owner_id", "q", 4, 0), ("follow_ptr", "q", 5, 0), ("clock", "q", 6, 0), ("world", "q", 7, 0),
     ("cloaked", "q", 8, 0), ("bubble", "q", 9, 0),
     ("free", "b", 0, 0), ("moribund", "b", 1, 0), ("massive", "b", 2, 0), ("interactive", "b", 3, 0),
-    ("sensor", "b", 4, 0),
+    ("sensor", "b", 4, 0), ("junk", "b", 5, 0),
 )
 FIELD_NAMES = tuple(f[0] for f in FIELDS)
+# with the gradient collision method: mLastC, the filing (the level and key of the ball's box, its side on each axis),
+# the tick's collision list's length
+COLL_FIELDS = (("last_c", "cf", 0, 3), ("level", "cq", 0, 0), ("key", "cq", 1, 3), ("side", "cq", 4, 3),
+               ("hits_n", "cq", 7, 0))
+MAX_COLLIDE_WORLD = 64                      # rows a world with collisions (one wavefront's contact mask)
 
 
 def f32(x: torch.Tensor) -> torch.Tensor:
@@ -97,6 +102,10 @@ def _field_property(name: str, blk: str, row: int, cols: int):
             base = self._f
         elif blk == "q":
             base = self._q
+        elif blk == "cf" or blk == "cq":
+            base = self._cf if blk == "cf" else self._cq
+            if base is None:
+                raise AttributeError(f"{name}: the park was made without collisions")
         else:
             base = self._b.view(torch.bool)
         if cols:
@@ -114,7 +123,7 @@ def _field_property(name: str, blk: str, row: int, cols: int):
 
 class _Fields(type):
     def __new__(mcs, name, bases, ns):
-        for fname, blk, row, cols in FIELDS:
+        for fname, blk, row, cols in FIELDS + COLL_FIELDS:
             ns.setdefault(fname, _field_property(fname, blk, row, cols))
         return super().__new__(mcs, name, bases, ns)
 
@@ -125,7 +134,7 @@ class VecModesPark(metaclass=_Fields):
 
     def __init__(self, n: int, friction: float = 1.0e6, tick_interval_ms: int = 1000, device="cuda",
                  dynamical_orientation: bool = False, use_new_orbit: bool = False, block: int = 256,
-                 ypr: str = "host"):
+                 ypr: str = "host", collisions: bool = False):
         if ypr != "host":
             raise ValueError('ypr must be "host" (the yaw, pitch and roll are display values, not kept here)')
         self.n = int(n)
@@ -150,6 +159,12 @@ class VecModesPark(metaclass=_Fields):
         self._b = torch.zeros(8, self.stride, dtype=U8, device=self.dev)
         self._sf = torch.zeros(N.NS, self.stride, dtype=F64, device=self.dev)      # the general launch's scratch
         self._sb = torch.zeros(self.stride, dtype=U8, device=self.dev)
+        self.collisions = bool(collisions)      # the gradient method (SpaceBallpark's partition and Gradient)
+        self._cf = self._cq = None
+        self._hits = torch.zeros(1, self.stride, dtype=I64, device=self.dev)
+        if self.collisions:
+            self._cf = torch.zeros(N.NCF, self.stride, dtype=F64, device=self.dev)
+            self._cq = torch.zeros(N.NCI, self.stride, dtype=I64, device=self.dev)
         self.current_time = 0
         self.refiled = torch.zeros(self.n, dtype=I64, device=self.dev)
         self.last_collision = -1.0
@@ -214,6 +229,10 @@ class VecModesPark(metaclass=_Fields):
         s.n_keys = 16
         s.keys = self.keys.data_ptr() if (self.keys is not None and s.schedule == 2) else None
         s.path = self.path.data_ptr() if self.path is not None else None
+        s.collisions = int(self.collisions)
+        if self.collisions:
+            s.coll_f64, s.coll_i64 = self._cf.data_ptr(), self._cq.data_ptr()
+            s.hits, s.hits_cap = self._hits.data_ptr(), self._hits.shape[0]
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -256,10 +275,12 @@ class VecModesPark(metaclass=_Fields):
             moribund=t([bool(x) for x in g("moribund", False)], torch.bool),
             mode=t([int(x) for x in g("mode", STOP)], I64),
             max_ang_vel=t([float(x) for x in g("max_ang_vel", 1.0)]),
-            rot_agility=t([float(x) for x in g("rot_agility", 0.9)]))
+            rot_agility=t([float(x) for x in g("rot_agility", 0.9)]),
+            junk=t([bool(x) for x in g("junk", False)], torch.bool))
 
     def load_arrays(self, *, ids, world, time, mass, radius, max_vel, pos, vel, agility, speed_fraction, free,
-                    massive, interactive, cloaked, bubble, moribund, mode, max_ang_vel=None, rot_agility=None) -> None:
+                    massive, interactive, cloaked, bubble, moribund, mode, max_ang_vel=None, rot_agility=None,
+                    junk=None) -> None:
         """`load` from [n] and [n, 3] tensors: AddBall's clamps (the floats rounded to float, then `x < 0 ? 0 : x`,
         agility `<= 0 ? 1`, NaN and -0 kept), the decay factor, Ball::Ball's defaults (follow range 10, warp length
         -1, no follow fields, the sensor off, the turning values 1 rad/s and 0.9). With dynamical orientation
@@ -306,9 +327,68 @@ class VecModesPark(metaclass=_Fields):
         self.follow_ptr = -1
         self.follow_range = 10.0
         self.last_collision = -1.0
+        self.junk = False if junk is None else c(junk, torch.bool)
         self._set_tiles()
         if self.dynamical_orientation:
             self._orient_setup(None, 1)
+        if self.collisions:
+            self._collide_setup()
+
+    # ---- the gradient collision method's state (space.py's SpaceBallpark, a client's view) ----
+
+    def _collide_setup(self) -> None:
+        """After a load: the collision lists' room (3 (k - 1) + 1 entries for a largest world of k rows: Potential
+        appends at most three entries a candidate, one per recursion depth), mLastC and the lists cleared, every ball
+        filed (AddBall's InsertBallInBoxes)."""
+        if self.n == 0:
+            return
+        w = self.world.cpu().numpy()
+        starts = np.concatenate(([0], np.flatnonzero(w[1:] != w[:-1]) + 1))
+        biggest = int(np.diff(np.concatenate((starts, [self.n]))).max())
+        if self.tiles is None or biggest > MAX_COLLIDE_WORLD:
+            raise ValueError(f"collisions need each world's rows consecutive and at most {MAX_COLLIDE_WORLD} rows a "
+                             f"world, within a workgroup of {self.block}")
+        cap = 3 * (biggest - 1) + 1
+        self._hits = torch.full((cap, self.stride), -1, dtype=I64, device=self.dev)
+        self._cf.zero_()
+        self._cq.zero_()
+        self._state = None
+        self._refile(None)
+
+    def _refile(self, mask: torch.Tensor | None) -> None:
+        """InsertBallInBoxes for the rows in mask (None: every row): their filing from their current state."""
+        if not self.collisions or self.n == 0:
+            return
+        rows = None
+        if mask is not None:
+            rows = torch.nonzero(mask.to(self.dev).reshape(self.n)).reshape(-1).to(torch.int32)
+            if rows.numel() == 0:
+                return
+        nr = self.n if rows is None else rows.numel()
+        s = self.state()
+        ptr = None if rows is None else rows.data_ptr()
+        if self.dev.type == "cuda":
+            with torch.cuda.device(self.dev):
+                rc = N.hip().xb_collide_file(ctypes.byref(s), ptr, nr, torch.cuda.current_stream(self.dev).cuda_stream)
+            N.check(rc, "xb_collide_file")
+        else:
+            N.check(N.cpu().xb_cpu_collide_file(ctypes.byref(s), ptr, nr), "xb_cpu_collide_file")
+
+    @property
+    def hits(self) -> torch.Tensor:
+        """[n, cap] int64: each row's collision list of its last tick (the other balls' ids in Potential's order), its
+        first hits_n entries; entries past them are left from earlier ticks."""
+        if not self.collisions:
+            raise AttributeError("hits: the park was made without collisions")
+        return self._hits[:, :self.n].t()
+
+    def collision_lists(self) -> list:
+        """Each row's collision list of its last tick, as Python lists (the reference's `collisions`)."""
+        n = self.hits_n.cpu().tolist()
+        h = self.hits.cpu().numpy()
+        if max(n, default=0) > h.shape[1]:
+            raise RuntimeError("a collision list past its room")
+        return [[int(x) for x in h[r, :k]] for r, k in enumerate(n)]
 
     # ---- the follower bookkeeping: Stop(Ball*) (Ballpark.cpp:4578; vec_modes.py:286-311) ----
 
@@ -463,6 +543,7 @@ class VecModesPark(metaclass=_Fields):
         ok = mask & ~(v < 0.0)
         self.max_vel = torch.where(ok, v, self.max_vel)
         self.refiled += ok.to(I64)
+        self._refile(ok)
 
     def set_speed_fraction(self, mask: torch.Tensor, s: torch.Tensor) -> None:
         s = f32(s.to(device=self.dev, dtype=F64))
@@ -475,6 +556,7 @@ class VecModesPark(metaclass=_Fields):
         ok = mask & ~(r < 0.0)
         self.radius = torch.where(ok, r, self.radius)
         self.refiled += ok.to(I64)
+        self._refile(ok)
 
     def set_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
         """SetBallVelocity: the new and old velocity; with dynamical orientation the rotation snapped to it."""
@@ -551,6 +633,7 @@ class VecModesPark(metaclass=_Fields):
         self.free = torch.where(change, flag, self.free)
         self._stop_rows(fix)
         self.set_velocity(fix, torch.zeros_like(self.vel))
+        self._refile(fix)
         self.last_g = torch.where(fix[:, None], 0.0, self.last_g)
 
     def set_ball_rigid(self, mask: torch.Tensor) -> None:
@@ -595,14 +678,17 @@ class VecModesPark(metaclass=_Fields):
             if rc:
                 raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
 
+    def _blocks(self) -> tuple:
+        return (self._f, self._q, self._b) + ((self._cf, self._cq, self._hits) if self.collisions else ())
+
     def _save(self) -> tuple:
-        return self._f.clone(), self._q.clone(), self._b.clone()
+        return tuple(b.clone() for b in self._blocks())
 
     def _put_back(self, saved: tuple, active: torch.Tensor) -> None:
         act = active.to(device=self.dev, dtype=torch.bool).reshape(self.n)
         sel = torch.zeros(self.stride, dtype=torch.bool, device=self.dev)
         sel[:self.n] = act
-        for blk, old in zip((self._f, self._q, self._b), saved):
+        for blk, old in zip(self._blocks(), saved):
             blk.copy_(torch.where(sel[None, :], blk, old))
 
     def evolve(self, lib=None, active: torch.Tensor | None = None) -> None:
@@ -692,21 +778,21 @@ class ModesGraph:
         if self.launches < 1 or self.ticks < 1:
             raise ValueError("ticks and launches are at least 1")
         self.mask = torch.ones(park.stride, dtype=torch.bool, device=park.dev)
-        self.saved = [torch.empty_like(b) for b in (park._f, park._q, park._b)] if masked else None
+        self.saved = [torch.empty_like(b) for b in park._blocks()] if masked else None
         before = park._save()                            # warm-up: the module loaded before the capture,
         park._tick(self.block, self.ticks)               # the park's state put back after it
-        for b, old in zip((park._f, park._q, park._b), before):
+        for b, old in zip(park._blocks(), before):
             b.copy_(old)
         torch.cuda.synchronize(park.dev)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             if masked:
-                for s, b in zip(self.saved, (park._f, park._q, park._b)):
+                for s, b in zip(self.saved, park._blocks()):
                     s.copy_(b)
             for _ in range(self.launches):
                 park._tick(self.block, self.ticks)
             if masked:
-                for s, b in zip(self.saved, (park._f, park._q, park._b)):
+                for s, b in zip(self.saved, park._blocks()):
                     b.copy_(torch.where(self.mask[None, :], b, s))
 
     def tick(self, active: torch.Tensor | None = None) -> None:

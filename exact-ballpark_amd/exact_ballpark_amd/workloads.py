@@ -89,6 +89,46 @@ def turning_park(seed: int, worlds: int, ships: int = 4) -> dict:
     return {"n": n, "worlds": worlds, "ships": ships, "rows": rows, "cmds": cmds, "ypr": ypr, "dynamical": True}
 
 
+def collide_park(seed: int, worlds: int, ships: int = 4, spheres: int = 4, dynamical: bool = False) -> dict:
+    """The turning-and-collisions park's bodies (the gradient method): each world's ships in a 2 km cluster around a
+    centre uniform in a +/-1e6 m cube, at rest, mass U(1e6, 1e8) kg, agility U(0.1, 5), maximum speed U(100, 3000)
+    m/s, radius U(20, 500) m, massive; and `spheres` fixed, massive spheres of radius U(100, 5000) m in a 4 km cluster
+    around the same centre. At tick 1 each ship goes to a point of its world's cluster (GotoPoint), so ships cross,
+    meet each other and hit or start inside the spheres. With `dynamical`, the turning values and rotations of
+    turning_park (maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), RotationQuaternion(yaw, pitch,
+    roll) of angles U(-pi, pi))."""
+    rng = np.random.default_rng(seed)
+    per = ships + spheres
+    n = worlds * per
+    world = np.repeat(np.arange(worlds, dtype=np.int64), per)
+    ids = np.tile(np.arange(1, per + 1, dtype=np.int64), worlds)
+    ship = np.tile(np.arange(per) < ships, worlds)
+    centre = np.repeat(rng.uniform(-1e6, 1e6, (worlds, 3)), per, axis=0)
+    rows = {
+        "id": ids, "world": world, "time": np.zeros(n, dtype=np.int64),
+        "mass": rng.uniform(1e6, 1e8, n), "radius": np.where(ship, rng.uniform(20.0, 500.0, n),
+                                                             rng.uniform(100.0, 5000.0, n)),
+        "max_vel": rng.uniform(100.0, 3000.0, n), "agility": rng.uniform(0.1, 5.0, n),
+        "speed_fraction": np.ones(n),
+        "pos": centre + np.where(ship[:, None], rng.uniform(-1000.0, 1000.0, (n, 3)),
+                                 rng.uniform(-2000.0, 2000.0, (n, 3))),
+        "vel": np.zeros((n, 3)), "free": ship.copy(), "massive": np.ones(n, dtype=bool),
+        "interactive": rng.uniform(0.0, 1.0, n) < 0.5, "moribund": np.zeros(n, dtype=bool),
+        "cloaked": np.zeros(n, dtype=np.int64), "bubble": np.zeros(n, dtype=np.int64),
+        "mode": np.full(n, 2, dtype=np.int64), "junk": np.zeros(n, dtype=bool),
+    }
+    ypr = None
+    if dynamical:
+        rows["max_ang_vel"] = rng.uniform(0.2, 2.0, n)
+        rows["rot_agility"] = rng.uniform(0.3, 3.0, n)
+        ypr = rng.uniform(-np.pi, np.pi, (n, 3))
+    cmds = {"op": np.where(ship, GOTO_OP, 0).astype(np.int64), "point": centre + rng.uniform(-1000.0, 1000.0, (n, 3)),
+            "target": np.arange(n, dtype=np.int64), "range": np.zeros(n), "min_range": np.full(n, 20000.0),
+            "warp_factor": np.full(n, 20, dtype=np.int64)}
+    return {"n": n, "worlds": worlds, "ships": per, "rows": rows, "cmds": cmds, "ypr": ypr, "dynamical": dynamical,
+            "collisions": True}
+
+
 def load_flight_park(park, wl: dict) -> None:
     """Loads the rows into a VecModesPark of wl["n"] rows and applies the tick-1 commands through its API (a turning
     park's rotations first)."""
@@ -103,7 +143,8 @@ def load_flight_park(park, wl: dict) -> None:
                      bubble=t(r["bubble"], torch.int64), moribund=t(r["moribund"], torch.bool),
                      mode=t(r["mode"], torch.int64),
                      max_ang_vel=t(r["max_ang_vel"]) if "max_ang_vel" in r else None,
-                     rot_agility=t(r["rot_agility"]) if "rot_agility" in r else None)
+                     rot_agility=t(r["rot_agility"]) if "rot_agility" in r else None,
+                     junk=t(r["junk"], torch.bool) if "junk" in r else None)
     if wl.get("ypr") is not None:
         park.set_rotation_ypr(torch.ones(wl["n"], dtype=torch.bool, device=dev), t(wl["ypr"]))
     apply_commands(park, wl["cmds"])
@@ -144,7 +185,8 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
                       "vel": [float(v) for v in r["vel"][k]], "free": bool(r["free"][k]),
                       "massive": bool(r["massive"][k]), "interactive": bool(r["interactive"][k]),
                       "cloaked": int(r["cloaked"][k]), "bubble": int(r["bubble"][k]),
-                      "moribund": bool(r["moribund"][k]), "mode": int(r["mode"][k])})
+                      "moribund": bool(r["moribund"][k]), "mode": int(r["mode"][k]),
+                      "junk": bool(r["junk"][k]) if "junk" in r else False})
         if "max_ang_vel" in r:
             balls[-1]["max_ang_vel"] = float(r["max_ang_vel"][k])
             balls[-1]["rot_agility"] = float(r["rot_agility"][k])
@@ -162,7 +204,8 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
                          float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
                          int(c["warp_factor"][k])))
     return {"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": steps, "new_orbit": False, "start": 0,
-            "balls": balls, "cmds": cmds, "dynamical": bool(wl.get("dynamical", False))}
+            "balls": balls, "cmds": cmds, "dynamical": bool(wl.get("dynamical", False)),
+            "collisions": bool(wl.get("collisions", False))}
 
 
 def select_worlds(wl: dict, order) -> dict:

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 2
+#define XB_ABI_VERSION 3
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -61,6 +61,17 @@ typedef struct xb_modes_state {
     int32_t n_keys;                /* the keys' bucket count, at most 16 */
     const uint8_t* keys;           /* schedule 2: each row's bucket, below n_keys */
     uint8_t* path;                 /* if not NULL: each row's executed path class in its last tick (0..7) */
+    /* The gradient collision method (SpaceBallpark's partition and Gradient, a client's view), tiled shape only, on
+       when `collisions` is set: each world at most 64 rows, consecutive in the tile. coll_f64 holds XB_NCF rows
+       (mLastC), coll_i64 XB_NCI rows (the filing: level, key x3, side x3; the tick's collision list's length), hits
+       hits_cap rows (the tick's collision list in order: entry i of row r at hits[i * stride + r]; a list past
+       hits_cap keeps counting, its entries past it dropped). Space junk is u8 row XB_B_JUNK, cloaking i64 row
+       XB_I_CLOAKED. The filing must be current before the first tick (xb_collide_file). */
+    double* coll_f64;
+    int64_t* coll_i64;
+    int64_t* hits;
+    int32_t hits_cap;
+    int32_t collisions;
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */
@@ -91,6 +102,12 @@ int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* strea
    angVel, -dt |angVel|), then the snap, as the Python AddBall does it. Replaces the batched _snap_orientation and the
    load's turning set-up (src/exact_ballpark/vec_orient.py:556-575, src/exact_ballpark/vec_modes.py:256-264). */
 int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what, void* stream);
+
+/* The partition filing of the listed rows (rows NULL: rows 0 .. n_rows - 1), in place: Ball::InsertInBoxes's outcome
+   (the level of the inflated radius, the key of the box holding the centre, the side on each axis where the inflated
+   sphere reaches the next box) into coll_i64, as AddBall, SetBallRadius, SetMaxSpeed, SetBallPosition and fixing a
+   ball file it. Replaces vec_collide_cuda.filing (src/exact_ballpark/vec_collide_cuda.cu:1214-1243, `filing_of`). */
+int xb_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, void* stream);
 
 /* One flight-mode command per listed row, applied in place between ticks: for each row r of rows[0 .. n_rows - 1]
    (rows NULL: rows 0 .. n_rows - 1), op[r] 0 none, 3 Stop, 1 GotoPoint(point[r][3]), 4 Follow and 5 Orbit (target row
@@ -184,6 +201,9 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads);
 
 /* xb_modes_orient_setup on a park in host memory (no stream). */
 int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what);
+
+/* xb_collide_file on a park in host memory (no stream). */
+int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows);
 
 /* xb_modes_commands on a park in host memory (no stream). */
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,

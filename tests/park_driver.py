@@ -96,7 +96,15 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
             t = np.stack([np.concatenate((f[29:33, r], f[33:37, r], f[37:40, r], f[40:43, r], f[43:46, r],
                                           [f[28, r], f[26, r], f[27, r]])) for r in rows])
             d = np.concatenate((d, t), axis=1)
-        out[wi] = (d, qq, [sorted(fol.get(r, [])) for r in rows])
+        lists = [sorted(fol.get(r, [])) for r in rows]
+        if park.collisions:                 # mLastC; the filing; the collision lists beside the followers
+            cf = park._cf[:, :n].cpu().numpy()
+            cq = park._cq[:, :n].cpu().numpy()
+            hits = park._hits[:, :n].cpu().numpy()
+            d = np.concatenate((d, np.stack([cf[0:3, r] for r in rows])), axis=1)
+            qq = np.concatenate((qq, np.stack([cq[0:7, r] for r in rows])), axis=1)
+            lists = [(lists[k], [int(x) for x in hits[:cq[7, r], r]]) for k, r in enumerate(rows)]
+        out[wi] = (d, qq, lists)
     return out
 
 
@@ -107,18 +115,19 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
     groups = defaultdict(list)
     for wi in (order if order is not None else range(len(worlds))):
         w = worlds[wi]
-        groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)))].append(wi)
+        groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)),
+                bool(w.get("collisions", False)))].append(wi)
     out = {}
-    for (friction, tick, new_orbit, dynamical), members in groups.items():
+    for (friction, tick, new_orbit, dynamical, coll), members in groups.items():
         balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
-                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical)
+                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll)
         park.load([{"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                     "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
                     "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
                     "moribund": b["moribund"], "mode": b["mode"], "max_ang_vel": b.get("max_ang_vel", 1.0),
-                    "rot_agility": b.get("rot_agility", 0.9)} for wi, b in balls])
+                    "rot_agility": b.get("rot_agility", 0.9), "junk": b.get("junk", False)} for wi, b in balls])
         if general:
             park.tiles = None
             park._state = None
@@ -168,10 +177,34 @@ def _run_turning(w: dict):
         yield step, np.concatenate((d[:, D_COLS], o.turning()), axis=1), q, fol      # 46 columns
 
 
+def _run_collide(w: dict):
+    """O.run_world for a world with collisions: mLastC after the doubles, the filing after the ints, each ball's
+    (followers, collision list)."""
+    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], bool(w.get("dynamical")), True)
+    for b in w["balls"]:
+        o.add(b)
+    by_step = defaultdict(list)
+    for c in w["cmds"]:
+        by_step[c[0]].append(c)
+    for step in range(1, w["steps"] + 1):
+        for c in by_step[step]:
+            o.command(c)
+        o.evolve()
+        d, q, fol = o.state()
+        lc, filing, hits = o.collide()
+        yield (step, np.concatenate((d[:, D_COLS], lc), axis=1), np.concatenate((q[:, Q_COLS], filing), axis=1),
+               list(zip(fol, hits)))
+
+
 def run_oracle(worlds: list, indices, record=None) -> dict:
     out = {}
     for wi in indices:
         w = worlds[wi]
+        if w.get("collisions"):
+            for step, d, q, lists in _run_collide(w):
+                if record is None or step in record:
+                    out[(wi, step)] = (d, q, [(list(f), list(h)) for f, h in lists])
+            continue
         runs = _run_turning(w) if w.get("dynamical") else O.run_world(w)
         for step, d, q, fol in runs:
             if record is None or step in record:
@@ -204,12 +237,13 @@ def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
         if same_d.all() and same_q.all() and gf == wf:
             continue
         if len(bad) < limit:
-            names_d = [O.D_NAMES[c] for c in D_COLS] + list(TURN_NAMES)
-            names_q = [O.Q_NAMES[c] for c in Q_COLS]
+            names_d = [O.D_NAMES[c] for c in D_COLS]
+            names_d += ["cx", "cy", "cz"] if gd.shape[1] == len(D_COLS) + 3 else list(TURN_NAMES)
+            names_q = [O.Q_NAMES[c] for c in Q_COLS] + ["level", "kx", "ky", "kz", "sx", "sy", "sz"]
             diffs = [f"{names_d[c]}[{k}] {gd[k, c]!r} vs {wd[k, c]!r}" for k, c in zip(*np.nonzero(~same_d))][:6]
             diffs += [f"{names_q[c]}[{k}] {gq[k, c]} vs {wq[k, c]}" for k, c in zip(*np.nonzero(~same_q))][:6]
             if gf != wf:
-                diffs.append(f"followers {gf} vs {wf}")
+                diffs.append(f"lists {gf} vs {wf}")
             bad.append(f"world {key[0]} ({worlds[key[0]]['kind']}) step {key[1]}: " + "; ".join(diffs))
         else:
             bad.append(f"{key}")
