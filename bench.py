@@ -39,6 +39,8 @@ METRIC = "bit-exact world-ticks/s on 1 MI355X (ragged composed park); p99 tick l
 BYTES_PER_SHIP_TICK = 361          # 164 read + 197 written, flight-modes park without dynamical orientation
 BYTES_PER_SHIP_TICK_DO = 469       # with it: + 36 read (rot, ang_vel, max_ang_vel, rot_agility as floats)
                                    # + 72 written (rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll)
+BYTES_PER_SHIP_TICK_COLL = 449     # with gradient collisions: + 28 read (the filing: key 24, level and sides 4)
+                                   # + 60 written (mLastC 24, the collision list's length 8, the filing 28)
 HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
 TRAFFIC_FILE = ROOT / "profiles" / "modes_tick_traffic.json"
 WORKLOADS = {
@@ -51,6 +53,9 @@ WORKLOADS = {
     "turning": "turning park: {worlds} worlds x 4 ships with dynamical orientation, each world a 2 km cluster, "
                "rotations from yaw, pitch and roll uniform in +/-pi, every ship going to a point of its cluster "
                "(70%), orbiting (15%) or following (15%); no collisions",
+    "collide": "collision park: {worlds} worlds x (4 free massive ships in a 2 km cluster + 4 fixed massive spheres "
+               "of radius 100-5000 m), every ship going to a point of its cluster; the gradient collision method "
+               "(partition filing, walk-order contacts, Potential); no dynamical orientation or static collidables",
 }
 
 
@@ -118,7 +123,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--workload", choices=("ragged", "profiles", "flight", "turning"), default="ragged",
+    ap.add_argument("--workload", choices=("ragged", "profiles", "flight", "turning", "collide"), default="ragged",
                     help="the primary line's workload (the others are measured beside it unless --only)")
     ap.add_argument("--only", action="store_true", help="measure the primary workload only")
     ap.add_argument("--block", type=int, default=0, help="fix the workgroup size (0: autotune)")
@@ -159,9 +164,12 @@ def main() -> int:
             wl = WL.flight_park(1, a.worlds, 4)
         elif kind == "turning":
             wl = WL.turning_park(1, a.worlds, 4)
+        elif kind == "collide":
+            wl = WL.collide_park(1, a.worlds, 4, 4)
         else:
             wl = WL.ragged_park(1, a.worlds, mix="profiles" if kind == "profiles" else "uniform")
-        park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=kind == "turning")
+        park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=kind == "turning",
+                            collisions=kind == "collide")
         WL.load_flight_park(park, wl)
         return wl, park
 
@@ -203,7 +211,7 @@ def main() -> int:
         last = [0]
 
         def restore():
-            for blk, old in zip((park._f, park._q, park._b), snap):
+            for blk, old in zip(park._blocks(), snap):
                 blk.copy_(old)
             if path0 is not None:
                 park.path.copy_(path0)              # the recorded paths too: the learned order is predicted from them
@@ -219,7 +227,7 @@ def main() -> int:
         if shape["ticks"] > 1 and not a.no_single:
             one = measure(1, max(32, a.steps // 4), 4)
         n = wl["n"]
-        per_ship = BYTES_PER_SHIP_TICK_DO if kind == "turning" else BYTES_PER_SHIP_TICK
+        per_ship = {"turning": BYTES_PER_SHIP_TICK_DO, "collide": BYTES_PER_SHIP_TICK_COLL}.get(kind, BYTES_PER_SHIP_TICK)
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
                "ship_ticks_per_s": n / (m["ms_per_tick"] / 1000.0), "ms_per_tick": m["ms_per_tick"],
@@ -273,11 +281,11 @@ def main() -> int:
     primary = tuned(a.workload)
     out_extra = {}
     if not a.only:
-        for kind in ("ragged", "profiles", "flight", "turning"):
+        for kind in ("ragged", "profiles", "flight", "turning", "collide"):
             if kind != a.workload:
                 other = tuned(kind)
                 out_extra[kind + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}
-                if kind == "turning" and not a.no_cpu:
+                if kind in ("turning", "collide") and not a.no_cpu:
                     out_extra[kind + "_park"]["cpu_baseline"], out_extra[kind + "_park"]["exact"] = cpu_baseline(
                         other["_park"], other["_wl"], other["_ticks_done"], a.cpu_seconds / 2, max_worlds=1024)
                 del other
@@ -414,6 +422,15 @@ def cpu_baseline(park, wl: dict, ticks: int, seconds: float, max_worlds: int = 1
     if park.dynamical_orientation:                 # the turning state too
         tcols = [29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 28, 26, 27]
         same &= D._same_bits(f[tcols][:, rows].T, np.concatenate([o.turning() for o in objs])).all(axis=1)
+    if park.collisions:                            # mLastC, the filing and the collision lists too
+        cf = park._cf[:, :park.n].cpu().numpy()
+        cq = park._cq[:, :park.n].cpu().numpy()
+        hits = park._hits[:, :park.n].cpu().numpy()
+        coll = [o.collide() for o in objs]
+        same &= D._same_bits(cf[0:3][:, rows].T, np.concatenate([c[0] for c in coll])).all(axis=1)
+        same &= (cq[0:7][:, rows].T == np.concatenate([c[1] for c in coll])).all(axis=1)
+        lists = [h for c in coll for h in c[2]]
+        same &= np.array([list(hits[:cq[7, r], r]) == lists[k] for k, r in enumerate(rows)])
     world_of = np.repeat(np.arange(m), counts[idx])
     bad = int(m - np.bincount(world_of[same], minlength=m).__eq__(counts[idx]).sum())
     cpu = {"value": m * ticks / dt, "unit": "world-ticks/s", "cores": 1, "kind": "port",

@@ -17,7 +17,7 @@ from .vec_modes import ModesGraph, VecModesPark
 
 
 def _restore(park: VecModesPark, saved: tuple) -> None:
-    for b, old in zip((park._f, park._q, park._b), saved):
+    for b, old in zip(park._blocks(), saved):
         b.copy_(old)
 
 

@@ -92,7 +92,8 @@ def test_device_float_and_inverse_maths_equal_host(name):
         y = np.roll(x, 12345).copy()
         if name == "pow":
             x = np.abs(x)
-            y = np.round(y) % 200 - 100
+            with np.errstate(invalid="ignore"):             # the infinities and NaNs stay NaN
+                y = np.round(y) % 200 - 100
         if name == "powf":
             y[m:2 * m] = 0.25
     args = [torch.from_numpy(x)] + ([torch.from_numpy(y)] if y is not None else [])
