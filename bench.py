@@ -2,24 +2,29 @@
 """Benchmark: bit-exact world-ticks/s of the batched destiny tick on one MI355X.
 
 Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
-- ragged (the primary line): the ragged flight-modes park, 65,536 worlds with ships per world geometric on 1..32
-  (mean 6), modes uniform over stop, goto, follow, orbit and warp given at tick 1.
+- composed (the primary line): the ragged composed park without missiles or static collidables, 65,536 worlds with
+  massive ships per world geometric on 1..32 (mean 6), modes uniform over stop, goto, follow, orbit and warp given at
+  tick 1, dynamical orientation (rotations from yaw, pitch and roll) and the gradient collision method.
+- ragged: the same worlds as a flight-modes park (no orientation, no collisions).
 - profiles: the same ragged park with world activity profiles (quiet, travelling, in combat, mixed), the worlds
   differing in cost from one another as a server's do: the learned scheduler's workload.
 - flight: 65,536 worlds x 4 ships, modes uniform.
-Each ragged workload is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
+- turning: 65,536 worlds x 4 ships in 2 km clusters with dynamical orientation.
+- collide: 65,536 worlds x (4 ships + 4 fixed spheres) in 2 km clusters, the gradient collision method.
+Each ragged workload (composed, ragged, profiles) is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
 row order, one tick a launch) and as the autotuned packed launch: whole worlds packed into tiles, with workgroup size,
 lane schedule, ticks per launch and world order (the row order, or the learned scheduler's regrouping, trained on the
 GPU on the park's own recorded ticks) chosen by the autotuner on the park's own state. `value` is the autotuned form;
 `learned_scheduler` gives the best shape in each world order, each measured as the primary line is.
 A step is one tick of every world; the timed ticks run as HIP graph replays with the state resident in HBM. Each
-kernel's HBM roofline fraction uses algorithmic bytes: 361 B a ship-tick (each persistent per-ship field the tick reads
-or writes counted once at destiny's sizes); `achieved` is those bytes over the measured time per tick. With several
+kernel's HBM roofline fraction uses algorithmic bytes: 361 B a ship-tick for the flight modes (each persistent
+per-ship field the tick reads or writes counted once at destiny's sizes), 469 with dynamical orientation, 449 with
+collisions, 557 with both; `achieved` is those bytes over the measured time per tick. With several
 ticks per launch the rows stay in registers between ticks, so the HBM traffic per tick (`traffic`, measured with
 rocprofv3) is a fraction of the algorithmic bytes; the single-tick launch's figures are given beside it.
 
-The CPU baseline is the oracle (a scalar C port of the reference's flight world, one core) on a bounded sample of the
-same worlds; its final states check the GPU's bit for bit.
+The CPU baseline is the oracle (a scalar C port of the reference's world, one core) on a bounded sample of the same
+worlds; its final states check the GPU's bit for bit.
 """
 
 from __future__ import annotations
@@ -53,6 +58,9 @@ WORKLOADS = {
     "turning": "turning park: {worlds} worlds x 4 ships with dynamical orientation, each world a 2 km cluster, "
                "rotations from yaw, pitch and roll uniform in +/-pi, every ship going to a point of its cluster "
                "(70%), orbiting (15%) or following (15%); no collisions",
+    "composed": "ragged composed park without missiles or static collidables: {worlds} worlds, ships per world "
+                "geometric on 1..32 (mean 6), massive, modes uniform over stop/goto/follow/orbit/warp, dynamical "
+                "orientation (rotations from yaw, pitch and roll uniform in +/-pi), gradient collisions",
     "collide": "collision park: {worlds} worlds x (4 free massive ships in a 2 km cluster + 4 fixed massive spheres "
                "of radius 100-5000 m), every ship going to a point of its cluster; the gradient collision method "
                "(partition filing, walk-order contacts, Potential); no dynamical orientation or static collidables",
@@ -123,7 +131,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--workload", choices=("ragged", "profiles", "flight", "turning", "collide"), default="ragged",
+    ap.add_argument("--workload", choices=("composed", "ragged", "profiles", "flight", "turning", "collide"),
+                    default="composed",
                     help="the primary line's workload (the others are measured beside it unless --only)")
     ap.add_argument("--only", action="store_true", help="measure the primary workload only")
     ap.add_argument("--block", type=int, default=0, help="fix the workgroup size (0: autotune)")
@@ -166,10 +175,12 @@ def main() -> int:
             wl = WL.turning_park(1, a.worlds, 4)
         elif kind == "collide":
             wl = WL.collide_park(1, a.worlds, 4, 4)
+        elif kind == "composed":
+            wl = WL.composed_park(1, a.worlds)
         else:
             wl = WL.ragged_park(1, a.worlds, mix="profiles" if kind == "profiles" else "uniform")
-        park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=kind == "turning",
-                            collisions=kind == "collide")
+        park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=bool(wl.get("dynamical")),
+                            collisions=bool(wl.get("collisions")))
         WL.load_flight_park(park, wl)
         return wl, park
 
@@ -180,7 +191,7 @@ def main() -> int:
             park.evolve()
         done = 4
         orders, training = {}, None
-        if not a.no_learned and kind in ("ragged", "profiles"):
+        if not a.no_learned and kind in ("ragged", "profiles", "composed"):
             t0 = time.perf_counter()
             L = S.LearnedScheduler(park, seed=0)
             training = L.train(ticks=8, epochs=20)
@@ -227,7 +238,9 @@ def main() -> int:
         if shape["ticks"] > 1 and not a.no_single:
             one = measure(1, max(32, a.steps // 4), 4)
         n = wl["n"]
-        per_ship = {"turning": BYTES_PER_SHIP_TICK_DO, "collide": BYTES_PER_SHIP_TICK_COLL}.get(kind, BYTES_PER_SHIP_TICK)
+        per_ship = {"turning": BYTES_PER_SHIP_TICK_DO, "collide": BYTES_PER_SHIP_TICK_COLL,
+                    "composed": BYTES_PER_SHIP_TICK_DO + BYTES_PER_SHIP_TICK_COLL - BYTES_PER_SHIP_TICK}.get(
+                        kind, BYTES_PER_SHIP_TICK)
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
                "ship_ticks_per_s": n / (m["ms_per_tick"] / 1000.0), "ms_per_tick": m["ms_per_tick"],
@@ -256,9 +269,10 @@ def main() -> int:
                           "world_ticks_per_s": a.worlds / (mo["ms_per_tick"] / 1000.0)}
             res["learned_scheduler"] = {"training": training, "orders": cmp,
                                         "gain_over_rows": cmp["rows"]["ms_per_tick"] / cmp["learned"]["ms_per_tick"]}
-        if not a.no_padded and kind in ("ragged", "profiles"):
+        if not a.no_padded and kind in ("ragged", "profiles", "composed"):
             pad = WL.padded(wl, 32)
-            pp = VecModesPark(pad["n"], device="cuda", block=256)
+            pp = VecModesPark(pad["n"], device="cuda", block=256, dynamical_orientation=bool(pad.get("dynamical")),
+                              collisions=bool(pad.get("collisions")))
             WL.load_flight_park(pp, pad)
             mp = _measure(pp, 1, max(32, a.steps // 4), 4)
             res["padded_baseline"] = {"layout": "every world padded to 32 rows, lanes in row order, worlds in row "
@@ -281,7 +295,7 @@ def main() -> int:
     primary = tuned(a.workload)
     out_extra = {}
     if not a.only:
-        for kind in ("ragged", "profiles", "flight", "turning", "collide"):
+        for kind in ("composed", "ragged", "profiles", "flight", "turning", "collide"):
             if kind != a.workload:
                 other = tuned(kind)
                 out_extra[kind + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}

@@ -291,6 +291,20 @@ def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int 
             "mix": mix}
 
 
+def composed_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32, mix: str = "uniform") -> dict:
+    """The ragged composed park without missiles or static collidables: ragged_park's worlds (massive ships), with
+    dynamical orientation (maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), each ship rotated by
+    RotationQuaternion(yaw, pitch, roll) of angles U(-pi, pi)) and the gradient collision method."""
+    wl = ragged_park(seed, worlds, mean_ships, max_ships, mix)
+    rng = np.random.default_rng(seed + 0x5EED)
+    n = wl["n"]
+    wl["rows"]["max_ang_vel"] = rng.uniform(0.2, 2.0, n)
+    wl["rows"]["rot_agility"] = rng.uniform(0.3, 3.0, n)
+    wl["rows"]["junk"] = np.zeros(n, dtype=bool)
+    wl.update(ypr=rng.uniform(-np.pi, np.pi, (n, 3)), dynamical=True, collisions=True)
+    return wl
+
+
 def padded(wl: dict, width: int) -> dict:
     """The ragged workload with every world padded to `width` rows: the padding rows are fixed balls (not free, so no
     tick touches them) with ids above the world's own; real rows keep their order and their results."""
@@ -324,5 +338,10 @@ def padded(wl: dict, width: int) -> dict:
     new_row = np.full(wl["n"], -1, dtype=np.int64)
     new_row[np.arange(wl["n"])] = dest
     cmds["target"] = np.where(real, new_row[np.clip(cmds["target"], 0, wl["n"] - 1)], 0)
+    ypr = None
+    if wl.get("ypr") is not None:
+        ypr = np.zeros((n, 3))
+        ypr[dest] = wl["ypr"]
     return {"n": n, "worlds": W, "ships": width, "counts": counts, "start": np.arange(W) * width, "rows": rows,
-            "cmds": cmds, "real": real, "dest": dest}
+            "cmds": cmds, "real": real, "dest": dest, "ypr": ypr, "dynamical": wl.get("dynamical", False),
+            "collisions": wl.get("collisions", False)}

@@ -3,12 +3,15 @@
 gradient collision method (src/exact_ballpark/space.py, `SpaceBallpark`, a client's view) run on the seeded collision
 worlds of tests/scenarios.py (`collide_cases`), every ball's state after every tick, as bits: the flight fields, the
 collision acceleration (mLastC) and time (mLastCollision), the collision list in order, and the filing (the level and
-key of the ball's box and its side on each axis).
+key of the ball's box and its side on each axis). With `--do` it writes collide_do_golden.npz: the same worlds with
+dynamical orientation (`do_collide_cases`) on the reference's FullBallpark(dynamical_orientation=True), the turning
+state recorded too; rotations given as yaw, pitch and roll go through its RotationQuaternion into SetBallRotation.
 
 Run only where the reference is mounted (/root/reference); the fixture it writes is committed and nothing else reads
 the reference. The reference's double exp, log, sincos and pow go to its own transcriptions of glibc 2.34 aarch64
 (vec_exp.exp, dlibm.log, dlibm.sincos, vec_libm.pow), the functions its kernels are held to, so the vectors do not
-depend on this machine's C library. Python 3.10 has no math.fma, which the transcriptions take at import: an exact
+depend on this machine's C library; so do FullBallpark's float functions (vec_libm's expf, logf, acosf, sinf, cosf,
+atan2f). Python 3.10 has no math.fma, which the transcriptions take at import: an exact
 rational a b + c rounded once stands in for it.
 """
 
@@ -42,6 +45,7 @@ def _exact_fma(a: float, b: float, c: float) -> float:
 math.fma = _exact_fma                                       # before the transcriptions are imported
 import torch  # noqa: E402
 import exact_ballpark.dlibm as dlibm  # noqa: E402
+import exact_ballpark.full as full  # noqa: E402
 import exact_ballpark.modes as modes  # noqa: E402
 import exact_ballpark.ref as ref  # noqa: E402
 import exact_ballpark.space as space  # noqa: E402
@@ -72,9 +76,21 @@ _maths.exp = _glibc_exp
 _maths.log = dlibm.log
 _maths.pow = _glibc_pow
 _maths.sin = _maths.asin = _maths.atan2 = _refused
-for mod in (ref, modes, space):
+for mod in (ref, modes, space, full):
     mod.math = _maths
 modes._sincos = dlibm.sincos
+
+
+def _f1(fn):
+    return lambda x: float(fn(vec_libm.f32(*_t(x))).item())
+
+
+full._expf, full._logf, full._acosf = _f1(vec_libm.expf), _f1(vec_libm.logf), _f1(vec_libm.acosf)
+full._sinf, full._cosf = _f1(vec_libm.sinf), _f1(vec_libm.cosf)
+full._atan2f = lambda y, x: float(vec_libm.atan2f(vec_libm.f32(*_t(y)), vec_libm.f32(*_t(x))).item())
+full._asinf = full._powf = _refused
+TURNING = ("rx", "ry", "rz", "rw", "orx", "ory", "orz", "orw", "wx", "wy", "wz", "owx", "owy", "owz", "tx", "ty", "tz",
+           "rot_roll", "max_ang_vel", "rot_agility")
 
 DOUBLES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
            "ovz", "mass", "radius", "max_vel", "agility", "sf", "follow_range", "time_factor", "last_collision",
@@ -101,13 +117,17 @@ def _apply(park, c) -> None:
         park.entity_warp_in(a, x, y, z, w)
     elif op == S.VEL_OP:
         park.set_velocity(a, x, y, z)
+    elif op == S.YPR_OP:                              # RotationQuaternion(yaw, pitch, roll) of the floats
+        park.set_ball_rotation(a, *full.q_rotation_ypr(full._f(x), full._f(y), full._f(z)))
     else:
         raise ValueError(op)
 
 
-def _state(b) -> tuple:
+def _state(b, turning: bool = False) -> tuple:
     d = [*b.pos, *b.vel, *b.goto, *b.last_g, *b.old_pos, *b.old_vel, b.mass, b.radius, b.max_vel, b.agility,
          b.speed_fraction, b.follow_range, b.time_factor, b.last_collision, *b.last_c]
+    if turning:
+        d += [*b.rot, *b.old_rot, *b.ang_vel, *b.old_ang_vel, *b.torque, b.rot_roll, b.max_ang_vel, b.rot_agility]
     box = b.my_box
     q = [b.id, b.mode, b.effect_stamp, b.follow_id, b.owner_id, -1 if b.follow_ptr is None else b.follow_ptr,
          int(b.is_massive), int(b.sensor_active), box.level, *box.key, *b.mod]
@@ -116,19 +136,25 @@ def _state(b) -> tuple:
 
 def run_world(w: dict) -> list:
     """[(step, ball index, doubles, ints, collision list)] at step 0 (the balls as added) and after every tick."""
-    park = space.SpaceBallpark(friction=w["friction"], tick_interval_ms=w["tick"], use_new_orbit=w["new_orbit"])
+    do = bool(w.get("dynamical"))
+    if do:
+        park = full.FullBallpark(friction=w["friction"], tick_interval_ms=w["tick"], use_new_orbit=w["new_orbit"],
+                                 dynamical_orientation=True)
+    else:
+        park = space.SpaceBallpark(friction=w["friction"], tick_interval_ms=w["tick"], use_new_orbit=w["new_orbit"])
     park.current_time = w["start"]
     for b in w["balls"]:
+        kw = {"max_ang_vel": b["max_ang_vel"], "rot_agility": b["rot_agility"]} if do else {}
         park.add_ball(b["id"], b["mass"], b["radius"], b["max_vel"], is_free=b["free"], is_massive=b["massive"],
                       is_interactive=b["interactive"], is_space_junk=b["junk"], x=b["pos"][0], y=b["pos"][1],
                       z=b["pos"][2], vx=b["vel"][0], vy=b["vel"][1], vz=b["vel"][2], agility=b["agility"],
-                      speed_fraction=b["sf"])
+                      speed_fraction=b["sf"], **kw)
         if b["cloaked"]:
             park.set_cloaked(b["id"], b["cloaked"])
     order = sorted(b["id"] for b in w["balls"])
     out = []
     for k, bid in enumerate(order):
-        d, q = _state(park.balls[bid])
+        d, q = _state(park.balls[bid], do)
         out.append((0, k, d, q, list(park.balls[bid].collisions)))
     for step in range(1, w["steps"] + 1):
         for c in w["cmds"]:
@@ -136,13 +162,13 @@ def run_world(w: dict) -> list:
                 _apply(park, c)
         park.evolve()
         for k, bid in enumerate(order):
-            d, q = _state(park.balls[bid])
+            d, q = _state(park.balls[bid], do)
             out.append((step, k, d, q, list(park.balls[bid].collisions)))
     return out
 
 
-def main(seed: int = 1, scale: int = 1, name: str = "collide_golden.npz") -> None:
-    worlds = S.collide_cases(seed, scale)
+def main(seed: int = 1, scale: int = 1, name: str = "collide_golden.npz", do: bool = False) -> None:
+    worlds = S.do_collide_cases(seed, scale) if do else S.collide_cases(seed, scale)
     kept, skipped = [], []
     rec_world, rec_step, rec_ball, rec_d, rec_q, hit_off, hit_ids = [], [], [], [], [], [0], []
     for wi, w in enumerate(worlds):
@@ -166,12 +192,14 @@ def main(seed: int = 1, scale: int = 1, name: str = "collide_golden.npz") -> Non
                         world=np.array(rec_world, dtype=np.int32), step=np.array(rec_step, dtype=np.int32),
                         ball=np.array(rec_ball, dtype=np.int32), d=d, q=np.array(rec_q, dtype=np.int64),
                         hit_off=np.array(hit_off, dtype=np.int64), hit_ids=np.array(hit_ids, dtype=np.int64),
-                        doubles=np.array(DOUBLES), ints=np.array(INTS))
+                        doubles=np.array(DOUBLES + (TURNING if do else ())), ints=np.array(INTS))
     n_hits = sum(1 for i in range(1, len(hit_off)) if hit_off[i] > hit_off[i - 1])
     print(f"{name}: {len(kept)} worlds kept, {len(skipped)} skipped {skipped[:5]}, {len(rec_d)} records, "
           f"{n_hits} with collisions", file=sys.stderr)
 
 
 if __name__ == "__main__":
-    main(int(sys.argv[1]) if len(sys.argv) > 1 else 1, int(sys.argv[2]) if len(sys.argv) > 2 else 1,
-         sys.argv[3] if len(sys.argv) > 3 else "collide_golden.npz")
+    args = [x for x in sys.argv[1:] if x != "--do"]
+    do = "--do" in sys.argv[1:]
+    main(int(args[0]) if len(args) > 0 else 1, int(args[1]) if len(args) > 1 else 1,
+         args[2] if len(args) > 2 else ("collide_do_golden.npz" if do else "collide_golden.npz"), do)

@@ -192,7 +192,8 @@ def _run_collide(w: dict):
         o.evolve()
         d, q, fol = o.state()
         lc, filing, hits = o.collide()
-        yield (step, np.concatenate((d[:, D_COLS], lc), axis=1), np.concatenate((q[:, Q_COLS], filing), axis=1),
+        parts = (d[:, D_COLS], o.turning(), lc) if w.get("dynamical") else (d[:, D_COLS], lc)
+        yield (step, np.concatenate(parts, axis=1), np.concatenate((q[:, Q_COLS], filing), axis=1),
                list(zip(fol, hits)))
 
 
@@ -238,7 +239,8 @@ def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
             continue
         if len(bad) < limit:
             names_d = [O.D_NAMES[c] for c in D_COLS]
-            names_d += ["cx", "cy", "cz"] if gd.shape[1] == len(D_COLS) + 3 else list(TURN_NAMES)
+            names_d += list(TURN_NAMES) if gd.shape[1] >= len(D_COLS) + len(TURN_NAMES) else []
+            names_d += ["cx", "cy", "cz"]
             names_q = [O.Q_NAMES[c] for c in Q_COLS] + ["level", "kx", "ky", "kz", "sx", "sy", "sz"]
             diffs = [f"{names_d[c]}[{k}] {gd[k, c]!r} vs {wd[k, c]!r}" for k, c in zip(*np.nonzero(~same_d))][:6]
             diffs += [f"{names_q[c]}[{k}] {gq[k, c]} vs {wq[k, c]}" for k, c in zip(*np.nonzero(~same_q))][:6]

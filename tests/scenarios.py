@@ -545,3 +545,21 @@ def collide_cases(seed: int, scale: int = 1) -> list:
         cmds = [_cmd(1, WARP_OP, w["id"], x=c[0], y=c[1], z=c[2], rr=r.uniform(0.0, 300.0), w=r.choice([3000, 6000]))]
         world("dropout", balls, cmds, r.randint(25, 40))
     return out
+
+
+def do_collide_cases(seed: int, scale: int = 1) -> list:
+    """The collision worlds with dynamical orientation (about 155 x scale): collide_cases' worlds, each ball with a
+    maximum angular speed U(0.2, 2) rad/s and rotational agility U(0.3, 3), about half of them rotated at tick 1 by
+    RotationQuaternion(yaw, pitch, roll) of angles U(-pi, pi)."""
+    r = random.Random(seed ^ 0x5DEECE66D)
+    out = collide_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+            if r.random() < 0.5:
+                w["cmds"].append(_cmd(1, YPR_OP, b["id"], x=r.uniform(-math.pi, math.pi), y=r.uniform(-math.pi, math.pi),
+                                      z=r.uniform(-math.pi, math.pi)))
+        w["cmds"].sort(key=lambda c: c[0])
+    return out

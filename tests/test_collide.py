@@ -31,6 +31,14 @@ def test_collision_worlds_match_oracle(worlds, want, block):
     assert not bad, f"{len(bad)} differ: {bad}"
 
 
+def test_collision_worlds_with_orientation_match_oracle():
+    """Dynamical orientation with collisions (the oracle pinned to the reference's FullBallpark on these worlds)."""
+    w = S.do_collide_cases(1)
+    got = D.run_park(w, "cpu", block=128)
+    bad = D.compare(got, D.run_oracle(w, range(len(w))), w)
+    assert not bad, f"{len(bad)} differ: {bad}"
+
+
 def test_collision_worlds_in_reverse_order(worlds, want):
     got = D.run_park(worlds, "cpu", order=list(reversed(range(len(worlds)))))
     assert not D.compare(got, want, worlds)

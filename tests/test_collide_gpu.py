@@ -103,3 +103,41 @@ def test_world_alone_equals_batch():
         rows = list(WL.world_rows(wl, wi))
         for x, y in zip(_blocks(full), _blocks(p)):
             assert torch.equal(x[..., rows].view(torch.uint8), y[..., :len(rows)].view(torch.uint8))
+
+
+def test_collision_worlds_with_orientation_match_oracle():
+    w = S.do_collide_cases(1)
+    got = D.run_park(w, DEV, block=128)
+    bad = D.compare(got, D.run_oracle(w, range(len(w))), w)
+    assert not bad, f"{len(bad)} differ: {bad}"
+
+
+def test_composed_park_host_oracle_and_padded():
+    """The ragged composed park (ragged worlds of 1..32 massive ships, dynamical orientation, collisions): the device
+    against the host build after 20 ticks, sampled worlds against the oracle, and the padded-to-32 layout's real rows
+    equal to the packed layout's (follow pointers mapped through the padding)."""
+    wl = WL.composed_park(5, 4096)
+    g, h = _load(wl, DEV), _load(wl, "cpu")
+    for _ in range(20):
+        g.evolve()
+        h.evolve()
+    assert _same(_blocks(g), _blocks(h))
+    for wi in range(0, 4096, 211):
+        sc = WL.world_scenario(wl, wi, 20)
+        w = D.run_oracle([sc], [0], record={20})[(0, 20)]
+        got = D._snapshot(g, [wi], {wi: list(WL.world_rows(wl, wi))})[wi]
+        assert D._same_bits(got[0], w[0]).all() and (got[1] == w[1]).all() and got[2] == w[2], wi
+    pad = WL.padded(wl, 32)
+    p = _load(pad, DEV)
+    for _ in range(20):
+        p.evolve()
+    rows = torch.as_tensor(pad["dest"])
+    back = torch.full((pad["n"],), -1, dtype=torch.int64)
+    back[rows] = torch.arange(wl["n"])
+    for k, (x, y) in enumerate(zip(_blocks(g)[:3], _blocks(p)[:3])):
+        x, y = x[..., :wl["n"]], y[..., rows]
+        if k == 1:                                  # follow pointers: rows of each layout
+            fp = y[5]
+            y = y.clone()
+            y[5] = torch.where(fp >= 0, back[fp.clamp(min=0)], fp)
+        assert torch.equal(x.view(torch.uint8), y.view(torch.uint8)), k

@@ -62,3 +62,46 @@ def test_oracle_collisions_match_reference(golden):
                 qcols = sorted(set(np.nonzero(q != gq)[1].tolist()))
                 bad.append((wi, golden["worlds"][wi]["kind"], step, cols[:6], qcols[:6], hits != gh))
     assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"
+
+
+@pytest.fixture(scope="module")
+def golden_do():
+    return G.load("collide_do_golden.npz")
+
+
+def run_do(w):
+    """As run, with dynamical orientation: the turning state after mLastC."""
+    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], True, True)
+    for b in w["balls"]:
+        o.add(b)
+    by_step = defaultdict(list)
+    for c in w["cmds"]:
+        by_step[c[0]].append(c)
+    out = []
+    for step in range(0, w["steps"] + 1):
+        if step:
+            for c in by_step[step]:
+                o.command(c)
+            o.evolve()
+        d, q, _ = o.state()
+        lc, filing, hits = o.collide()
+        out.append((step, np.concatenate([d[:, :26], lc, o.turning()], axis=1),
+                    np.concatenate([q[:, G.Q_FROM_ORACLE], filing], axis=1), hits))
+    return out
+
+
+def test_oracle_collisions_with_orientation_match_reference(golden_do):
+    """The same worlds with dynamical orientation against the reference's FullBallpark(dynamical_orientation=True):
+    the heading thrust's mLastG feeds the gradient, the rotation a tick on commits beside the collision response."""
+    bad = []
+    turned = 0
+    for wi in golden_do["kept"]:
+        for step, d, q, hits in run_do(golden_do["worlds"][wi]):
+            gd, gq, gh = golden_do["records"][(wi, step)]
+            same_d = _same_bits(d, gd)
+            turned += int((gd[:, 29] != 1.0).sum())
+            if not (same_d.all() and (q == gq).all() and hits == gh):
+                cols = sorted(set(np.nonzero(~same_d)[1].tolist()))
+                bad.append((wi, golden_do["worlds"][wi]["kind"], step, cols[:6], hits != gh))
+    assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"
+    assert turned > 1000
