@@ -81,7 +81,7 @@ struct Scratch {
     }
 };
 
-/* A group's ticks with the gradient collision method: xb_collide_tiled_kernel's phases, each over the whole group
+/* A group's ticks with the gradient collision method: xb_collide_tiled_body's phases, each over the whole group
    before the next (each world's rows consecutive, at most 64). */
 template <bool DO>
 void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector<XbRow>& rows, int64_t lo,

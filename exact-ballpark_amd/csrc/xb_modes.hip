@@ -44,6 +44,22 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
 }
 
 constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 4 + 1;     // the records, the lane schedule (int32), act
+
+// Waves a SIMD each tiled kernel is built for. A row's whole state lives in registers across a launch's ticks, and
+// with dynamical orientation or collisions the compiler would take every register a lane has: one wave a SIMD, so a
+// CU's one workgroup idles at each barrier while its slowest wave finishes. Held to more waves, it keeps the hottest
+// state in registers and moves the rest to scratch. Measured on 65,536 worlds (ms a tick, 1 / 2 / 3 / 4 waves): the
+// flight park 0.033 / 0.033 / 0.029 / 0.033, with orientation 0.66 / 0.43 / 0.41 / 0.36, with collisions 0.30 / 0.20 /
+// 0.29 / 0.41, with both 1.21 / 0.76 / 0.88 / 1.02.
+#ifndef XB_WAVES_FLIGHT
+#define XB_WAVES_FLIGHT 3
+#endif
+#ifndef XB_WAVES_TURNING
+#define XB_WAVES_TURNING 4
+#endif
+#ifndef XB_WAVES_COLLIDE
+#define XB_WAVES_COLLIDE 2
+#endif
 constexpr int XB_MAX_WAVES = 4;                          // 256 lanes
 constexpr int XB_KEY_BUCKETS = 16;
 
@@ -90,7 +106,7 @@ __device__ __forceinline__ int xb_schedule_place(const xb_modes_state& s, const 
 }
 
 template <bool DO>
-__global__ void __launch_bounds__(256) xb_modes_tiled_kernel(xb_modes_state s) {
+__device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
     const int g = blockIdx.x;
@@ -155,7 +171,7 @@ __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
 // view for the gradients; (3a) the rows in warp proper, whose drop-out makes them massive for the balls after them;
 // (3b) each massive row's gradient over its world's rows, its end state; (4) the commit and the refiling.
 template <bool DO>
-__global__ void __launch_bounds__(256) xb_collide_tiled_kernel(xb_modes_state s) {
+__device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
     const int g = blockIdx.x;
@@ -241,6 +257,23 @@ __global__ void xb_collide_file_kernel(xb_modes_state s, const int32_t* __restri
     xb_coll_load(P, r, c);
     xb_file(w.s, w.free_, P, c.fil);
     xb_coll_store(P, r, c);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_FLIGHT)))
+xb_modes_tiled_flight(xb_modes_state s) {
+    xb_modes_tiled_body<false>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_TURNING)))
+xb_modes_tiled_turning(xb_modes_state s) {
+    xb_modes_tiled_body<true>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_flight(xb_modes_state s) {
+    xb_collide_tiled_body<false>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_turning(xb_modes_state s) {
+    xb_collide_tiled_body<true>(s);
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -397,15 +430,15 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
             (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
             return -1;
         const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
-        if (dyn) xb_collide_tiled_kernel<true><<<state->n_tiles, block, lds, st>>>(*state);
-        else xb_collide_tiled_kernel<false><<<state->n_tiles, block, lds, st>>>(*state);
+        if (dyn) xb_collide_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
+        else xb_collide_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
     if (state->tiles) {
         if (state->n_tiles <= 0) return -1;
         const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
-        if (dyn) xb_modes_tiled_kernel<true><<<state->n_tiles, block, lds, st>>>(*state);
-        else xb_modes_tiled_kernel<false><<<state->n_tiles, block, lds, st>>>(*state);
+        if (dyn) xb_modes_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
+        else xb_modes_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
     if (!state->scratch_f64 || !state->scratch_u8 || state->ticks > 1 || state->schedule) return -1;
@@ -462,9 +495,11 @@ int xb_maths_eval(int func, int64_t n, const double* x, double* out, void* strea
 int xb_modes_info(int which, int64_t* out) {
     hipFuncAttributes fa;
     hipError_t e;
-    if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel<false>));
+    if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_flight));
     else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c<false>));
-    else if (which == 3) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_kernel<true>));
+    else if (which == 3) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_turning));
+    else if (which == 4) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_flight));
+    else if (which == 5) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_turning));
     else if (which == 2) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_a));
     else return -1;
     if (e != hipSuccess) return (int)e;
