@@ -50,7 +50,8 @@ constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 4 + 1;     // the records, the lane
 // CU's one workgroup idles at each barrier while its slowest wave finishes. Held to more waves, it keeps the hottest
 // state in registers and moves the rest to scratch. Measured on 65,536 worlds (ms a tick, 1 / 2 / 3 / 4 waves): the
 // flight park 0.033 / 0.033 / 0.029 / 0.033, with orientation 0.66 / 0.43 / 0.41 / 0.36, with collisions 0.30 / 0.20 /
-// 0.29 / 0.41, with both 1.21 / 0.76 / 0.88 / 1.02.
+// 0.29 / 0.41, with both 1.21 / 0.76 / 0.88 / 1.02 (8 ticks a launch). The ragged flight park's single tick, lanes by
+// branch key: 72 us at two waves, 67 at three.
 #ifndef XB_WAVES_FLIGHT
 #define XB_WAVES_FLIGHT 3
 #endif

@@ -120,9 +120,9 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
                       const int64_t* warp_factor, void* stream);
 
 /* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass, 3 tiled with
-   dynamical orientation; out[0..3] =
-   scalar registers, vector registers, private (scratch) bytes per lane, LDS bytes. Replaces xbpm_info
-   (src/exact_ballpark/vec_modes_cuda.cu:766). */
+   dynamical orientation, 4 tiled with collisions, 5 with both;
+   out[0..3] = vector registers a lane (hipFuncAttributes numRegs), private (scratch) bytes a lane, static LDS bytes,
+   maximum threads a workgroup. Replaces xbpm_info (src/exact_ballpark/vec_modes_cuda.cu:766). */
 int xb_modes_info(int which, int64_t* out);
 
 /* ---- the learned scheduler (no counterpart in the reference: it orders work, never changes a result) ---- */
