@@ -314,7 +314,8 @@ def main() -> int:
         "latency_ms": {"tick_p50": p["tick_latency_ms"]["p50"], "tick_p99": p["tick_latency_ms"]["p99"],
                        "launch_p50": p["launch_latency_ms"]["p50"], "launch_p99": p["launch_latency_ms"]["p99"]},
         "roofline": p["roofline"], "single_tick": p["single_tick"],
-        "kernel": N.kernel_info(0),
+        "kernel": N.kernel_info({(False, False): 0, (True, False): 3, (False, True): 4, (True, True): 5}[
+            (bool(p["_park"].dynamical_orientation), bool(p["_park"].collisions))]),
     }
     for key in ("learned_scheduler", "padded_baseline"):
         if key in p:
