@@ -128,30 +128,6 @@ static inline XB_HD void xo_n2_alpha2_f(const XoShip& k, float omega0, int* n2, 
     }
 }
 
-/* The searches' screen. GotoTorqueBreaks and GotoRollBreaks look for the first n1 (1..99) whose acceleration
-   alpha1 = (target - omega0 num) / den is below max_alpha, num and den chains of rounded float operations on kappa1 =
-   tf^n1 (by repeated products). A lane runs 40 of those iterations on average and up to a few hundred, each with three
-   correctly rounded divisions and a square root. The screen first estimates num, den and |target - omega0 num|^2 from
-   the same kappa1 with products only (1/(1 - kappa2) taken once), with bounds on how far the exact chain can be from
-   the estimate: each of the exact chain's and the estimate's values is within a few roundings (2^-21 of the
-   magnitudes it is computed from) of the real value, and the bounds allow 2^-16. Only when the estimate clears
-   max_alpha by those bounds and a further 2^-11 is the iteration known to fail without the exact chain; any other
-   outcome (near the threshold, or a non-finite value: every comparison with a NaN is false) runs the exact chain as
-   before, and so does the last iteration, whose alpha1 the search keeps. The exact chain decides every result. */
-#ifndef XO_SCREEN
-#define XO_SCREEN 1                                  /* 0: every iteration runs the exact chain (the checks' reference) */
-#endif
-#define XO_SCREEN_ERR 1.52587890625e-05f            /* 2^-16 */
-#define XO_SCREEN_MARGIN 1.00048828125f              /* 1 + 2^-11 */
-
-/* Whether |n| / |d| >= max_alpha certainly holds for the exact chain, from estimates n_e (its squared length n2e and
-   its error bound en) and d_e (error bound ed). A squared length below 2^-100 (where subnormal squares would lose the
-   relative accuracy the bounds assume) always takes the exact chain. */
-static inline XB_HD bool xo_screen_fails(float n2e, float en, float de, float ed, float max_alpha) {
-    const float r = (en + max_alpha * (xo_abs(de) + ed)) * XO_SCREEN_MARGIN;
-    return n2e > 7.88860905221011805e-31f && n2e > r * r;
-}
-
 /* GotoTorqueBreaks (full.py:2155-2222): the heading torque toward `direction` */
 static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ rot, XoV3 w) {
     const XoV3 zero = xo_v(0.0f, 0.0f, 0.0f);
@@ -185,45 +161,19 @@ static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ r
         }
         kappa1 = 1.0f;
         n1 = 1;
-        const float n2dt_s = xo_n2dt(n2, k.dt);
-        const float r2 = n2 == 0 ? 0.0f : 1.0f / (1.0f - kappa2);
-        const XoV3 at = xo_v(xo_abs(target.x), xo_abs(target.y), xo_abs(target.z));
-        const XoV3 aw = xo_v(xo_abs(omega0.x), xo_abs(omega0.y), xo_abs(omega0.z));
         while (n1 < 100) {
             kappa1 = kappa1 * k.tf;
             const float n1dt = xo_n2dt(n1, k.dt);
-            bool fails = false;
-            if (XO_SCREEN && n1 < 99) {                     /* the screen (the last iteration's alpha1 is kept) */
-                float num_e, den_e, snum, sden;
-                if (n2 == 0) {
-                    num_e = tau * (1.0f - kappa1);
-                    den_e = tau * (n1dt - num_e);
-                    snum = xo_abs(num_e);
-                    sden = xo_abs(tau) * (xo_abs(n1dt) + snum);
-                } else {
-                    const float b = kappa1 * kappa2 * n2dt_s * r2;
-                    const float c = n2dt_s * kappa2 * (1.0f - kappa1) * r2;
-                    num_e = tau - b;
-                    den_e = tau * (n1dt - c);
-                    snum = xo_abs(tau) + xo_abs(b);
-                    sden = xo_abs(tau) * (xo_abs(n1dt) + xo_abs(c));
-                }
-                const XoV3 ne = xo_sub(target, xo_mul(omega0, num_e));
-                const float en = XO_SCREEN_ERR * ((at.x + at.y + at.z) + (aw.x + aw.y + aw.z) * (xo_abs(num_e) + snum));
-                fails = xo_screen_fails(xo_dot(ne, ne), en, den_e, XO_SCREEN_ERR * sden, max_alpha);
+            if (n2 == 0) {
+                alpha1 = xo_div(xo_sub(target, xo_mul(omega0, tau * (1.0f - kappa1))),
+                                tau * (n1dt - tau * (1.0f - kappa1)));
+            } else {
+                const float n2dt = xo_n2dt(n2, k.dt);
+                const float num = tau - ((kappa1 * kappa2) * n2dt) / (1.0f - kappa2);
+                const float den = tau * (n1dt - ((n2dt * kappa2) * (1.0f - kappa1)) / (1.0f - kappa2));
+                alpha1 = xo_div(xo_sub(target, xo_mul(omega0, num)), den);
             }
-            if (!fails) {
-                if (n2 == 0) {
-                    alpha1 = xo_div(xo_sub(target, xo_mul(omega0, tau * (1.0f - kappa1))),
-                                    tau * (n1dt - tau * (1.0f - kappa1)));
-                } else {
-                    const float n2dt = xo_n2dt(n2, k.dt);
-                    const float num = tau - ((kappa1 * kappa2) * n2dt) / (1.0f - kappa2);
-                    const float den = tau * (n1dt - ((n2dt * kappa2) * (1.0f - kappa1)) / (1.0f - kappa2));
-                    alpha1 = xo_div(xo_sub(target, xo_mul(omega0, num)), den);
-                }
-                if (xo_len(alpha1) < max_alpha) break;
-            }
+            if (xo_len(alpha1) < max_alpha) break;
             n1 += 1;
         }
         const int n2_old = n2;
@@ -273,42 +223,18 @@ static inline XB_HD float xo_roll_breaks(const XoShip& k, XoV3 heading_torque, X
         }
         kappa1 = 1.0f;
         n1 = 1;
-        const float n2dt_s = xo_n2dt(n2, k.dt);
-        const float r2 = n2 == 0 ? 0.0f : 1.0f / (1.0f - kappa2);
         while (n1 < 100) {
             kappa1 = kappa1 * k.tf;
             const float n1dt = xo_n2dt(n1, k.dt);
-            bool fails = false;
-            if (XO_SCREEN && n1 < 99) {                     /* the screen, as in xo_torque_breaks */
-                float num_e, den_e, snum, sden;
-                if (n2 == 0) {
-                    num_e = tau * (1.0f - kappa1);
-                    den_e = tau * (n1dt - num_e);
-                    snum = xo_abs(num_e);
-                    sden = xo_abs(tau) * (xo_abs(n1dt) + snum);
-                } else {
-                    const float b = kappa1 * kappa2 * n2dt_s * r2;
-                    const float c = n2dt_s * kappa2 * (1.0f - kappa1) * r2;
-                    num_e = tau - b;
-                    den_e = tau * (n1dt - c);
-                    snum = xo_abs(tau) + xo_abs(b);
-                    sden = xo_abs(tau) * (xo_abs(n1dt) + xo_abs(c));
-                }
-                const float ne = desired - omega_roll * num_e;
-                const float en = XO_SCREEN_ERR * (xo_abs(desired) + xo_abs(omega_roll) * (xo_abs(num_e) + snum));
-                fails = xo_screen_fails(ne * ne, en, den_e, XO_SCREEN_ERR * sden, max_alpha);
+            if (n2 == 0) {
+                alpha1 = (desired - omega_roll * (tau * (1.0f - kappa1))) / (tau * (n1dt - tau * (1.0f - kappa1)));
+            } else {
+                const float n2dt = xo_n2dt(n2, k.dt);
+                const float num = tau - ((kappa1 * kappa2) * n2dt) / (1.0f - kappa2);
+                const float den = tau * (n1dt - ((n2dt * kappa2) * (1.0f - kappa1)) / (1.0f - kappa2));
+                alpha1 = (desired - omega_roll * num) / den;
             }
-            if (!fails) {
-                if (n2 == 0) {
-                    alpha1 = (desired - omega_roll * (tau * (1.0f - kappa1))) / (tau * (n1dt - tau * (1.0f - kappa1)));
-                } else {
-                    const float n2dt = xo_n2dt(n2, k.dt);
-                    const float num = tau - ((kappa1 * kappa2) * n2dt) / (1.0f - kappa2);
-                    const float den = tau * (n1dt - ((n2dt * kappa2) * (1.0f - kappa1)) / (1.0f - kappa2));
-                    alpha1 = (desired - omega_roll * num) / den;
-                }
-                if (xo_abs(alpha1) < max_alpha) break;
-            }
+            if (xo_abs(alpha1) < max_alpha) break;
             n1 += 1;
         }
         const int n2_old = n2;
