@@ -95,10 +95,13 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
     std::vector<XoV3> w1((size_t)cnt);
     std::vector<XbV3> pe((size_t)cnt), ve((size_t)cnt);
     std::vector<char> warped((size_t)cnt);
+    const bool ypr = !DO && s.ypr;
+    std::vector<XbYpr> ys(ypr ? (size_t)cnt : 0);
     for (int64_t k = 0; k < cnt; ++k) {
         xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
         if (DO) xb_turn_load(P, rows[k].row, rows[k].t);
         xb_coll_load(P, rows[k].row, cr[k]);
+        if (ypr) xb_ypr_load(s.ypr, s.stride, rows[k].row, ys[k]);
     }
     for (int64_t k = 0; k < cnt; ++k) {
         const int64_t wk = XB_Q(P, XB_I_WORLD, rows[k].row);
@@ -128,8 +131,12 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
             if (warped[k]) xb_coll_phase3_warp(rows[k], cr[k], S.C, (int)k, P, &pe[k], &ve[k]);
         }
         for (int64_t k = 0; k < cnt; ++k) {
-            xb_coll_phase34<DO>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k], whi[k], P,
-                                rot1[k], w1[k]);
+            if (ypr)
+                xb_coll_phase34<DO, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k],
+                                          whi[k], P, rot1[k], w1[k], &ys[k]);
+            else
+                xb_coll_phase34<DO, false>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k],
+                                           whi[k], P, rot1[k], w1[k]);
             xb_tick_tail(rows[k]);
         }
     }
@@ -137,6 +144,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
         xb_row_store(P, rows[k]);
         if (DO && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
         xb_coll_store(P, rows[k].row, cr[k]);
+        if (ypr) xb_ypr_store(s.ypr, s.stride, rows[k].row, ys[k]);
         if (s.path) s.path[rows[k].row] = rows[k].path;
     }
 }
@@ -146,9 +154,12 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
     S.size((int)cnt);
     rows.resize((size_t)cnt);
     const bool dyn = s.dynamical != 0;
+    const bool ypr = !dyn && s.ypr;
+    std::vector<XbYpr> ys(ypr ? (size_t)cnt : 0);
     for (int64_t k = 0; k < cnt; ++k) {
         xb_row_load(P, s.rowmap ? (int64_t)s.rowmap[lo + k] : lo + k, rows[k]);
         if (dyn) xb_turn_load(P, rows[k].row, rows[k].t);
+        if (ypr) xb_ypr_load(s.ypr, s.stride, rows[k].row, ys[k]);
     }
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
@@ -156,12 +167,14 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
         for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
         for (int64_t k = 0; k < cnt; ++k) {
             if (dyn) xb_tick_row<true>(rows[k], S.R, (int)k, 0, (int)cnt, P);
+            else if (ypr) xb_tick_row<false, true>(rows[k], S.R, (int)k, 0, (int)cnt, P, &ys[k]);
             else xb_tick_row<false>(rows[k], S.R, (int)k, 0, (int)cnt, P);
         }
     }
     for (int64_t k = 0; k < cnt; ++k) {
         xb_row_store(P, rows[k]);
         if (dyn && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
+        if (ypr) xb_ypr_store(s.ypr, s.stride, rows[k].row, ys[k]);
         if (s.path) s.path[rows[k].row] = rows[k].path;
     }
 }
@@ -177,6 +190,7 @@ int xb_cpu_abi(void) { return XB_ABI_VERSION; }
 int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
     if (!state || state->n < 0 || state->stride < state->n) return -1;
     if (state->n == 0) return 0;
+    if (state->dynamical && state->ypr) return -1;
     const XbPark P = host_park(*state);
     if (state->collisions) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 || state->hits_cap < 0 ||
@@ -231,6 +245,24 @@ int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_
         xb_coll_load(P, r, c);
         xb_file(w.s, w.free_, P, c.fil);
         xb_coll_store(P, r, c);
+    }
+    return 0;
+}
+
+int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || !state->ypr || state->n < 0 ||
+        state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
+        return -1;
+    const XbPark P = host_park(*state);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r < 0 || r >= state->n) continue;
+        XbRow w;
+        xb_row_load(P, r, w);
+        XbYpr y;
+        xb_ypr_load(state->ypr, state->stride, r, y);
+        xb_yaw_pitch_roll(y, w.s, false, w.s.clock, true, P);
+        xb_ypr_store(state->ypr, state->stride, r, y);
     }
     return 0;
 }

@@ -106,7 +106,7 @@ __device__ __forceinline__ int xb_schedule_place(const xb_modes_state& s, const 
     return perm[j];
 }
 
-template <bool DO>
+template <bool DO, bool YPR>
 __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
@@ -131,21 +131,24 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     XbRow w;
+    XbYpr y;
     if (on) {
         xb_row_load(P, r, w);
         if (DO) xb_turn_load(P, r, w.t);
+        if (YPR) xb_ypr_load(s.ypr, s.stride, r, y);
     }
     for (int t = 0; t < ticks; ++t) {
         if (on) xb_record_a(w, R, j);
         __syncthreads();
         if (on) xb_record_b(w, R, j, 0, cnt, P);
         __syncthreads();
-        if (on) xb_tick_row<DO>(w, R, j, 0, cnt, P);
+        if (on) xb_tick_row<DO, YPR>(w, R, j, 0, cnt, P, &y);
         __syncthreads();
     }
     if (on) {
         xb_row_store(P, w);
         if (DO && w.turned) xb_turn_store(P, r, w.t);
+        if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
         if (s.path) s.path[r] = w.path;
     }
 }
@@ -171,7 +174,7 @@ __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
 // the reference's order needs (modes.py:702-745): (A, B) the tick-start records; (1) the behaviours, then each row's
 // view for the gradients; (3a) the rows in warp proper, whose drop-out makes them massive for the balls after them;
 // (3b) each massive row's gradient over its world's rows, its end state; (4) the commit and the refiling.
-template <bool DO>
+template <bool DO, bool YPR>
 __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
@@ -198,10 +201,12 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     XbRow w;
     XbCollRow c;
+    XbYpr y;
     if (on) {
         xb_row_load(P, r, w);
         if (DO) xb_turn_load(P, r, w.t);
         xb_coll_load(P, r, c);
+        if (YPR) xb_ypr_load(s.ypr, s.stride, r, y);
     }
     C.kx[j] = on ? XB_Q(P, XB_I_WORLD, r) : -1;     // the world's places: a run of equal world ids
     __syncthreads();
@@ -232,7 +237,7 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
         if (warped) xb_coll_phase3_warp(w, c, C, j, P, &p, &v);
         __syncthreads();
         if (on) {
-            xb_coll_phase34<DO>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1);
+            xb_coll_phase34<DO, YPR>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1, &y);
             xb_tick_tail(w);
         }
         __syncthreads();
@@ -241,6 +246,7 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
         xb_row_store(P, w);
         if (DO && w.turned) xb_turn_store(P, r, w.t);
         xb_coll_store(P, r, c);
+        if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
         if (s.path) s.path[r] = w.path;
     }
 }
@@ -260,21 +266,45 @@ __global__ void xb_collide_file_kernel(xb_modes_state s, const int32_t* __restri
     xb_coll_store(P, r, c);
 }
 
+// One lane per listed row: the display orientation snapped to the row's state (its clock as `now`; no hits).
+__global__ void xb_ypr_snap_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    const XbPark P = xb_park(s);
+    XbRow w;
+    xb_row_load(P, r, w);
+    XbYpr y;
+    xb_ypr_load(s.ypr, s.stride, r, y);
+    xb_yaw_pitch_roll(y, w.s, false, w.s.clock, true, P);
+    xb_ypr_store(s.ypr, s.stride, r, y);
+}
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_FLIGHT)))
 xb_modes_tiled_flight(xb_modes_state s) {
-    xb_modes_tiled_body<false>(s);
+    xb_modes_tiled_body<false, false>(s);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_TURNING)))
 xb_modes_tiled_turning(xb_modes_state s) {
-    xb_modes_tiled_body<true>(s);
+    xb_modes_tiled_body<true, false>(s);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
 xb_collide_tiled_flight(xb_modes_state s) {
-    xb_collide_tiled_body<false>(s);
+    xb_collide_tiled_body<false, false>(s);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
 xb_collide_tiled_turning(xb_modes_state s) {
-    xb_collide_tiled_body<true>(s);
+    xb_collide_tiled_body<true, false>(s);
+}
+// The flight tick keeping the display orientation (without dynamical orientation only).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_FLIGHT)))
+xb_modes_tiled_flight_ypr(xb_modes_state s) {
+    xb_modes_tiled_body<false, true>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_flight_ypr(xb_modes_state s) {
+    xb_collide_tiled_body<false, true>(s);
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -315,7 +345,7 @@ __global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
     xb_record_b(w, xb_global_records(s), (int)r, 0, (int)s.n, xb_park(s));
 }
 
-template <bool DO>
+template <bool DO, bool YPR>
 __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= s.n) return;
@@ -323,9 +353,12 @@ __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     XbRow w;
     xb_row_load(P, r, w);
     if (DO) xb_turn_load(P, r, w.t);
-    xb_tick_row<DO>(w, xb_global_records(s), (int)r, 0, (int)s.n, P);
+    XbYpr y;
+    if (YPR) xb_ypr_load(s.ypr, s.stride, r, y);
+    xb_tick_row<DO, YPR>(w, xb_global_records(s), (int)r, 0, (int)s.n, P, &y);
     xb_row_store(P, w);
     if (DO && w.turned) xb_turn_store(P, r, w.t);
+    if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
     if (s.path) s.path[r] = w.path;
 }
 
@@ -424,14 +457,17 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     if (state->n == 0) return 0;
     if (!state->f64 || !state->i64 || !state->u8) return -1;
     const bool dyn = state->dynamical != 0;
+    const bool ypr = state->ypr != nullptr;
     hipStream_t st = (hipStream_t)stream;
     if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
+    if (dyn && ypr) return -1;      // dynamical orientation keeps the rotation, not the display orientation
     if (state->collisions) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 ||
             (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
             return -1;
         const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
         if (dyn) xb_collide_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
+        else if (ypr) xb_collide_tiled_flight_ypr<<<state->n_tiles, block, lds, st>>>(*state);
         else xb_collide_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
@@ -439,6 +475,7 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
         if (state->n_tiles <= 0) return -1;
         const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
         if (dyn) xb_modes_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
+        else if (ypr) xb_modes_tiled_flight_ypr<<<state->n_tiles, block, lds, st>>>(*state);
         else xb_modes_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
         return xb_launch_rc();
     }
@@ -450,8 +487,9 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     xb_modes_step_b<<<grid, block, 0, st>>>(*state);
     rc = xb_launch_rc();
     if (rc) return rc;
-    if (dyn) xb_modes_step_c<true><<<grid, block, 0, st>>>(*state);
-    else xb_modes_step_c<false><<<grid, block, 0, st>>>(*state);
+    if (dyn) xb_modes_step_c<true, false><<<grid, block, 0, st>>>(*state);
+    else if (ypr) xb_modes_step_c<false, true><<<grid, block, 0, st>>>(*state);
+    else xb_modes_step_c<false, false><<<grid, block, 0, st>>>(*state);
     return xb_launch_rc();
 }
 
@@ -471,6 +509,15 @@ int xb_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_
         return -1;
     if (n_rows == 0) return 0;
     xb_collide_file_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows);
+    return xb_launch_rc();
+}
+
+int xb_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, void* stream) {
+    if (!state || !state->f64 || !state->i64 || !state->u8 || !state->ypr || state->n < 0 ||
+        state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
+        return -1;
+    if (n_rows == 0) return 0;
+    xb_ypr_snap_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows);
     return xb_launch_rc();
 }
 
@@ -497,10 +544,12 @@ int xb_modes_info(int which, int64_t* out) {
     hipFuncAttributes fa;
     hipError_t e;
     if (which == 0) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_flight));
-    else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c<false>));
+    else if (which == 1) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_c<false, false>));
     else if (which == 3) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_turning));
     else if (which == 4) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_flight));
     else if (which == 5) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_turning));
+    else if (which == 6) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_flight_ypr));
+    else if (which == 7) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_flight_ypr));
     else if (which == 2) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_a));
     else return -1;
     if (e != hipSuccess) return (int)e;

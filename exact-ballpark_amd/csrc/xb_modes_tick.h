@@ -599,6 +599,8 @@ static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, 
 
 /* Step C: the row's whole tick (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi). Other rows are
    read only through the records. */
+#include "xb_ypr.h"
+
 /* Phase 1 of the row at place j (Ballpark::Evolve's behaviours, modes.py:707-716): a release when the target starts
    warp proper, before or after this ball's own turn by their ids; the behaviour (mLastG; with dynamical orientation
    the rotation a tick on, into rot1 and w1); mLastC cleared. */
@@ -702,9 +704,11 @@ static inline XB_HD void xb_tick_tail(XbRow& w) {
     s.clock = s.clock + 1;
 }
 
-/* One tick of the row at place j of its group [lo, hi), without collisions (mLastC stays zero). */
-template <bool DO>
-static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P) {
+/* One tick of the row at place j of its group [lo, hi), without collisions (mLastC stays zero). With YPR (and without
+   dynamical orientation) the display orientation `y` evolves at the commit. */
+template <bool DO, bool YPR = false>
+static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo, int hi, const XbPark& P,
+                                     XbYpr* y = nullptr) {
     XbShip& s = w.s;
     XoQ rot1 = xo_q(0.0f, 0.0f, 0.0f, 1.0f);     /* with dynamical orientation: the rotation and angular velocity */
     XoV3 w1 = xo_v(0.0f, 0.0f, 0.0f);           /* a tick on */
@@ -716,6 +720,7 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         if (s.mode == XB_MODE_WARP && s.effect_stamp >= 0) xb_phase3_warp(w, a, P, &p, &v);
         else xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
         xb_phase4<DO>(w, p, v, rot1, w1);
+        if (YPR && !DO) xb_yaw_pitch_roll(*y, s, false, s.clock, false, P);
     }
     xb_tick_tail(w);
 }
@@ -804,10 +809,10 @@ static inline XB_HD void xb_coll_phase3_warp(XbRow& w, XbCollRow& c, const XbCol
 
 /* Phase 3's gradient and end state for a row not in warp proper, then phase 4 and the refiling of a row that moved
    at least sqrt(0.1) m (modes.py:717-743). The row's world is at places [wlo, whi) of the group. */
-template <bool DO>
+template <bool DO, bool YPR = false>
 static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, XbV3 p, XbV3 v, const XbCollRecs& C,
                                          const XbRecords& R, int j, int wlo, int whi, const XbPark& P, XoQ rot1,
-                                         XoV3 w1) {
+                                         XoV3 w1, XbYpr* y = nullptr) {
     XbShip& s = w.s;
     if (!xb_row_live(w)) return;
     if (!warped) {
@@ -829,6 +834,7 @@ static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, Xb
         xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
     }
     xb_phase4<DO>(w, p, v, rot1, w1);
+    if (YPR && !DO) xb_yaw_pitch_roll(*y, s, c.nh > 0, s.clock, false, P);
     const XbV3 d = v3_sub(s.pos, w.old_pos);
     if (!(v3_lensq(d) < 0.1)) xb_file(s, w.free_, P, c.fil);
 }

@@ -55,6 +55,11 @@ FIELD_NAMES = tuple(f[0] for f in FIELDS)
 COLL_FIELDS = (("last_c", "cf", 0, 3), ("level", "cq", 0, 0), ("key", "cq", 1, 3), ("side", "cq", 4, 3),
                ("hits_n", "cq", 7, 0))
 MAX_COLLIDE_WORLD = 64                      # rows a world with collisions (one wavefront's contact mask)
+# with the display orientation kept (Ball::CalculateYawPitchRoll, without dynamical orientation)
+YPR_FIELDS = (("yaw", "y", 0, 0), ("pitch", "y", 1, 0), ("roll", "y", 2, 0), ("old_yaw", "y", 3, 0),
+              ("old_pitch", "y", 4, 0), ("old_roll", "y", 5, 0), ("roll_speed", "y", 6, 0),
+              ("old_roll_speed", "y", 7, 0), ("yaw_delta", "y", 8, 0))
+YPR_PATHS = ("host", "cuda")                # the reference's names for where its orientation park evaluates them
 
 
 def f32(x: torch.Tensor) -> torch.Tensor:
@@ -106,6 +111,10 @@ def _field_property(name: str, blk: str, row: int, cols: int):
             base = self._cf if blk == "cf" else self._cq
             if base is None:
                 raise AttributeError(f"{name}: the park was made without collisions")
+        elif blk == "y":
+            base = self._y
+            if base is None:
+                raise AttributeError(f"{name}: the park was made without display_ypr")
         else:
             base = self._b.view(torch.bool)
         if cols:
@@ -123,20 +132,29 @@ def _field_property(name: str, blk: str, row: int, cols: int):
 
 class _Fields(type):
     def __new__(mcs, name, bases, ns):
-        for fname, blk, row, cols in FIELDS + COLL_FIELDS:
+        for fname, blk, row, cols in FIELDS + COLL_FIELDS + YPR_FIELDS:
             ns.setdefault(fname, _field_property(fname, blk, row, cols))
         return super().__new__(mcs, name, bases, ns)
 
 
 class VecModesPark(metaclass=_Fields):
     """n balls, one per row, in independent worlds, with destiny's flight modes; `evolve()` is one kernel launch a
-    tick. The reference's constructor (src/exact_ballpark/vec_modes.py:199-217)."""
+    tick. The reference's constructor (src/exact_ballpark/vec_modes.py:199-217).
+
+    `ypr` takes the reference's values ("host", "cuda": where its orientation park evaluates atan2, asin and sin); its
+    modes park keeps no yaw, pitch and roll, and neither does this one unless `display_ypr`: then FullBallpark's
+    display orientation without dynamical orientation (Ball::CalculateYawPitchRoll, src/exact_ballpark/full.py:
+    2043-2089) is kept in a block of its own, snapped when a free ball is loaded and when a velocity is set, evolved
+    at every live row's commit inside the tick's kernel, with glibc's functions on the device either way."""
 
     def __init__(self, n: int, friction: float = 1.0e6, tick_interval_ms: int = 1000, device="cuda",
                  dynamical_orientation: bool = False, use_new_orbit: bool = False, block: int = 256,
-                 ypr: str = "host", collisions: bool = False):
-        if ypr != "host":
-            raise ValueError('ypr must be "host" (the yaw, pitch and roll are display values, not kept here)')
+                 ypr: str = "host", collisions: bool = False, display_ypr: bool = False):
+        if ypr not in YPR_PATHS:
+            raise ValueError(f"ypr must be one of {YPR_PATHS}, not {ypr!r}")
+        if display_ypr and dynamical_orientation:
+            raise ValueError("display_ypr is the orientation without dynamical orientation")
+        self.ypr = ypr
         self.n = int(n)
         self.friction = float(friction)
         self.tick_interval = int(tick_interval_ms)
@@ -165,6 +183,8 @@ class VecModesPark(metaclass=_Fields):
         if self.collisions:
             self._cf = torch.zeros(N.NCF, self.stride, dtype=F64, device=self.dev)
             self._cq = torch.zeros(N.NCI, self.stride, dtype=I64, device=self.dev)
+        self.display_ypr = bool(display_ypr)
+        self._y = torch.zeros(N.NY, self.stride, dtype=F64, device=self.dev) if self.display_ypr else None
         self.current_time = 0
         self.refiled = torch.zeros(self.n, dtype=I64, device=self.dev)
         self.last_collision = -1.0
@@ -233,6 +253,7 @@ class VecModesPark(metaclass=_Fields):
         if self.collisions:
             s.coll_f64, s.coll_i64 = self._cf.data_ptr(), self._cq.data_ptr()
             s.hits, s.hits_cap = self._hits.data_ptr(), self._hits.shape[0]
+        s.ypr = self._y.data_ptr() if self.display_ypr else None
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -331,6 +352,9 @@ class VecModesPark(metaclass=_Fields):
         self._set_tiles()
         if self.dynamical_orientation:
             self._orient_setup(None, 1)
+        if self.display_ypr:                            # AddBall's snap of a free ball (full.py:1283-1288)
+            self._y.zero_()
+            self._ypr_snap(self.free)
         if self.collisions:
             self._collide_setup()
 
@@ -373,6 +397,26 @@ class VecModesPark(metaclass=_Fields):
             N.check(rc, "xb_collide_file")
         else:
             N.check(N.cpu().xb_cpu_collide_file(ctypes.byref(s), ptr, nr), "xb_cpu_collide_file")
+
+    def _ypr_snap(self, mask: torch.Tensor | None) -> None:
+        """xb_modes_ypr_snap on the rows in mask (None: every row): CalculateYawPitchRoll with snap."""
+        if not self.display_ypr or self.n == 0:
+            return
+        rows = None
+        if mask is not None:
+            rows = torch.nonzero(mask.to(self.dev).reshape(self.n)).reshape(-1).to(torch.int32)
+            if rows.numel() == 0:
+                return
+        nr = self.n if rows is None else rows.numel()
+        s = self.state()
+        ptr = None if rows is None else rows.data_ptr()
+        if self.dev.type == "cuda":
+            with torch.cuda.device(self.dev):
+                rc = N.hip().xb_modes_ypr_snap(ctypes.byref(s), ptr, nr,
+                                               torch.cuda.current_stream(self.dev).cuda_stream)
+            N.check(rc, "xb_modes_ypr_snap")
+        else:
+            N.check(N.cpu().xb_cpu_modes_ypr_snap(ctypes.byref(s), ptr, nr), "xb_cpu_modes_ypr_snap")
 
     @property
     def hits(self) -> torch.Tensor:
@@ -559,12 +603,15 @@ class VecModesPark(metaclass=_Fields):
         self._refile(ok)
 
     def set_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
-        """SetBallVelocity: the new and old velocity; with dynamical orientation the rotation snapped to it."""
+        """SetBallVelocity: the new and old velocity; with dynamical orientation the rotation snapped to it, with the
+        display orientation kept that snapped (full.py:1583-1593)."""
         v = v.to(device=self.dev, dtype=F64)
         self.vel = torch.where(mask[:, None], v, self.vel)
         self.old_vel = torch.where(mask[:, None], v, self.old_vel)
         if self.dynamical_orientation:
             self._orient_setup(mask, 0)
+        elif self.display_ypr:
+            self._ypr_snap(mask)
 
     # ---- dynamical orientation's setters (vec_orient.py:438-464) ----
 
@@ -679,7 +726,8 @@ class VecModesPark(metaclass=_Fields):
                 raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
 
     def _blocks(self) -> tuple:
-        return (self._f, self._q, self._b) + ((self._cf, self._cq, self._hits) if self.collisions else ())
+        return ((self._f, self._q, self._b) + ((self._cf, self._cq, self._hits) if self.collisions else ())
+                + ((self._y,) if self.display_ypr else ()))
 
     def _save(self) -> tuple:
         return tuple(b.clone() for b in self._blocks())

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 3
+#define XB_ABI_VERSION 4
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -72,6 +72,10 @@ typedef struct xb_modes_state {
     int64_t* hits;
     int32_t hits_cap;
     int32_t collisions;
+    /* If not NULL (and `dynamical` not set): the display orientation Ball::CalculateYawPitchRoll keeps, XB_NY float64
+       rows (yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta), evolved at every
+       live row's commit; snapped by xb_modes_ypr_snap. */
+    double* ypr;
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */
@@ -108,6 +112,12 @@ int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int6
    sphere reaches the next box) into coll_i64, as AddBall, SetBallRadius, SetMaxSpeed, SetBallPosition and fixing a
    ball file it. Replaces vec_collide_cuda.filing (src/exact_ballpark/vec_collide_cuda.cu:1214-1243, `filing_of`). */
 int xb_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, void* stream);
+
+/* The display orientation of the listed rows (rows NULL: rows 0 .. n_rows - 1) snapped to their state, as AddBall does
+   for a free ball and SetBallVelocity for any (Ball::CalculateYawPitchRoll with snap: no smoothing, the roll reset).
+   Needs state->ypr. Replaces the reference's `_yaw_pitch_roll(mask, snap=True)` (src/exact_ballpark/vec_orient.py:
+   479-535) and vec_ypr_cuda's device maths for it. */
+int xb_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, void* stream);
 
 /* One flight-mode command per listed row, applied in place between ticks: for each row r of rows[0 .. n_rows - 1]
    (rows NULL: rows 0 .. n_rows - 1), op[r] 0 none, 3 Stop, 1 GotoPoint(point[r][3]), 4 Follow and 5 Orbit (target row
@@ -204,6 +214,9 @@ int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, 
 
 /* xb_collide_file on a park in host memory (no stream). */
 int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_t n_rows);
+
+/* xb_modes_ypr_snap on a park in host memory (no stream). */
+int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows);
 
 /* xb_modes_commands on a park in host memory (no stream). */
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,

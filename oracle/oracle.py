@@ -68,6 +68,8 @@ def lib() -> ctypes.CDLL:
         L.or_overflow.argtypes = [vp]
         L.or_set_junk.argtypes = [vp, i64, i]
         L.or_get_collide.argtypes = [vp, i, pd, pq, pq]
+        L.or_park_set_ypr.argtypes = [vp, i]
+        L.or_get_ypr.argtypes = [vp, i, pd]
         _LIB = L
     return _LIB
 
@@ -76,13 +78,14 @@ class World:
     """One world of the oracle: add balls, command them, tick, read them back."""
 
     def __init__(self, friction: float, tick_ms: int, new_orbit: bool, start: int, dynamical: bool = False,
-                 collisions: bool = False):
+                 collisions: bool = False, ypr: bool = False):
         self.L = lib()
         self.p = self.L.or_park_new(float(friction), int(tick_ms), int(bool(new_orbit)), int(start))
         self.dynamical = bool(dynamical)
         self.collisions = bool(collisions)
         self.L.or_park_set_dynamical(self.p, int(self.dynamical))
         self.L.or_park_set_collisions(self.p, int(self.collisions))
+        self.L.or_park_set_ypr(self.p, int(bool(ypr)))
 
     def __del__(self):
         if getattr(self, "p", None):
@@ -146,6 +149,15 @@ class World:
         t = np.zeros((n, NT), dtype=np.float64)
         for k in range(n):
             self.L.or_get_turning(self.p, k, t[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return t
+
+    def ypr(self) -> np.ndarray:
+        """[n, 9] float64, balls by ascending id: yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed,
+        old_roll_speed, yaw_delta."""
+        n = self.L.or_count(self.p)
+        t = np.zeros((n, 9), dtype=np.float64)
+        for k in range(n):
+            self.L.or_get_ypr(self.p, k, t[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return t
 
     def collide(self) -> tuple:

@@ -62,6 +62,8 @@ typedef struct {
     int64_t key[3];
     int nh;
     int64_t hits[MAX_HITS];
+    /* the display orientation without dynamical orientation (full.py: yaw, pitch, roll_angle and the rest) */
+    double yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta;
 } OBall;
 
 typedef struct {
@@ -70,6 +72,7 @@ typedef struct {
     int use_new_orbit;
     int dynamical;                             /* FullBallpark's dynamical_orientation */
     int collisions;                            /* SpaceBallpark: the partition and the gradient method */
+    int ypr;                                   /* FullBallpark's yaw, pitch and roll without dynamical orientation */
     int overflow;                              /* a collision list past MAX_HITS */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
@@ -144,6 +147,7 @@ int or_park_size(void) { return (int)sizeof(OPark); }
 int or_ball_size(void) { return (int)sizeof(OBall); }
 
 static void file_ball(OPark* p, OBall* b);
+static void yaw_pitch_roll(OPark* p, OBall* b, int snap);
 
 static void refresh_time_factor(OPark* p, OBall* b) {     /* ref.py:82-87 */
     double m = b->mass * b->agility;
@@ -187,6 +191,7 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->rot_agility = (double)0.9f;
     refresh_time_factor(p, b);
     if (p->collisions) file_ball(p, b);                              /* space.py:1150 */
+    if (p->ypr && !p->dynamical && b->is_free) yaw_pitch_roll(p, b, 1);   /* full.py:1283-1288 */
     return 0;
 }
 
@@ -619,6 +624,7 @@ void or_set_velocity(OPark* p, int64_t id, double x, double y, double z) {
     if (!b) return;
     b->vel[0] = b->old_vel[0] = x, b->vel[1] = b->old_vel[1] = y, b->vel[2] = b->old_vel[2] = z;
     if (p->dynamical) snap_orientation(b);
+    else if (p->ypr) yaw_pitch_roll(p, b, 1);                           /* full.py:1583-1593 */
 }
 void or_set_max_angular_speed(OPark* p, int64_t id, double w) {
     w = F(w);
@@ -1470,6 +1476,75 @@ static void gradient(OPark* p, int bi) {                                /* space
     }
 }
 
+/* ------------------------------------------------------ the display orientation (full.py:893-905, 2043-2089) */
+
+#define HALF_PI 1.57079632679489655
+#define TWO_PI 6.2831853071795862
+#define ROLL_SPEED_ACCELERATION (-3.0)
+#define ROLL_SPEED_DECAY (-0.9)
+#define ROLL_ACCELERATION 1.0
+#define ROLL_DECAY (-0.5)
+
+static void modulo2pi(double a, double b, double* ao, double* bo) {     /* full.py:893-905 */
+    b = fmod(b, TWO_PI), a = fmod(a, TWO_PI);
+    double d0 = fabs(b - a), dp = fabs(b + TWO_PI - a), dm = fabs(b - TWO_PI - a);
+    *ao = a;
+    if (d0 < dp) *bo = (d0 < dm) ? b : b - TWO_PI;
+    else *bo = (dp < dm) ? b + TWO_PI : b - TWO_PI;
+}
+
+static double fmax_d(double a, double b) { return a < b ? b : a; }      /* std::max */
+static double fmin_d(double a, double b) { return b < a ? b : a; }      /* std::min */
+
+static void yaw_pitch_roll(OPark* p, OBall* b, int snap) {             /* full.py:2043-2089 */
+    b->old_pitch = b->pitch, b->old_yaw = b->yaw, b->old_roll = b->roll, b->old_roll_speed = b->roll_speed;
+    double mm = b->mass / p->friction;
+    mm = (mm != 0.0) ? 100.0 / mm : 100.0;
+    double vel2 = lensq(b->vel), d[3];
+    if (vel2 == 0.0) {
+        for (int i = 0; i < 3; ++i) d[i] = b->gp[i] - b->pos[i];
+    } else {
+        d[0] = d[1] = d[2] = 0.0;
+        if (b->mode == WARP) {
+            if (b->effect_stamp < 0) {
+                for (int i = 0; i < 3; ++i) d[i] = b->nh ? b->vel[i] : b->gp[i] - b->pos[i];
+            } else if (mm < 1.0 && p->current_time != b->effect_stamp) {
+                int64_t inner_i = (p->current_time - b->effect_stamp) * 2;
+                double inner = 15.0 < (double)inner_i ? 15.0 : (double)inner_i;
+                double x = mm * inner;
+                mm = 1.0 < x ? 1.0 : x;
+            }
+        } else {
+            for (int i = 0; i < 3; ++i) d[i] = b->vel[i];
+        }
+        if (b->mode != MISSILE && !snap) {
+            double g[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]}, off[3];
+            normalized(g, off);
+            double sc = sqrt(vel2) * 0.3;
+            for (int i = 0; i < 3; ++i) d[i] = d[i] + off[i] * sc;
+        }
+    }
+    b->yaw = xb_atan2(d[0], d[2]);
+    double r = length(d);
+    b->pitch = (r > 0.0) ? -xb_asin(fmin_d(1.0, fmax_d(-1.0, d[1] / r))) : 0.0;
+    modulo2pi(b->old_yaw, b->yaw, &b->old_yaw, &b->yaw);
+    modulo2pi(b->old_pitch, b->pitch, &b->old_pitch, &b->pitch);
+    if (snap) {
+        b->roll = b->old_roll = b->roll_speed = b->old_roll_speed = b->yaw_delta = 0.0;
+        return;
+    }
+    double tmp = 1.0 / (1.0 + mm * 0.5);
+    b->yaw = (b->old_yaw + mm * 0.5 * b->yaw) * tmp;
+    b->pitch = (b->old_pitch + mm * 0.5 * b->pitch) * tmp;
+    b->yaw_delta = xb_asin(xb_sin(b->yaw - b->old_yaw));
+    b->roll_speed = b->roll_speed + b->yaw_delta * ROLL_SPEED_ACCELERATION;
+    b->roll_speed = b->roll_speed + b->old_roll_speed * ROLL_SPEED_DECAY * p->dt;
+    b->roll_speed = fmax_d(-HALF_PI, fmin_d(HALF_PI, b->roll_speed));
+    b->roll = b->roll + b->roll_speed * ROLL_ACCELERATION * p->dt;
+    b->roll = b->roll + b->old_roll * ROLL_DECAY * p->dt;
+    b->roll = fmax_d(-HALF_PI, fmin_d(HALF_PI, b->roll));
+}
+
 /* SpaceBallpark's refiling (InsertBallInBoxes) where the reference calls it; a no-op without collisions. */
 static void refile(OPark* p, OBall* b) {
     if (p->collisions) file_ball(p, b);
@@ -1518,6 +1593,7 @@ void or_evolve(OPark* p) {                                              /* modes
             d[k] = b->pos[k] - b->old_pos[k];
         }
         if (p->dynamical) commit_rotation(p, b);
+        else if (p->ypr) yaw_pitch_roll(p, b, 0);                       /* full.py:2565-2567 */
         if (!(lensq(d) < 0.1)) refile(p, b);                            /* modes.py:737-743 */
     }
     p->current_time += 1;                                               /* (7) */
@@ -1557,6 +1633,17 @@ int or_get_followers(const OPark* p, int k, int64_t* out) {
 
 /* The collision park's switch (before any ball is added), space junk, the overflow flag. */
 void or_park_set_collisions(OPark* p, int on) { p->collisions = on; }
+void or_park_set_ypr(OPark* p, int on) { p->ypr = on; }
+
+/* Ball k's display orientation: yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed,
+   yaw_delta. */
+int or_get_ypr(const OPark* p, int k, double* d) {
+    if (k < 0 || k >= p->n) return -1;
+    const OBall* b = &p->balls[k];
+    d[0] = b->yaw, d[1] = b->pitch, d[2] = b->roll, d[3] = b->old_yaw, d[4] = b->old_pitch, d[5] = b->old_roll;
+    d[6] = b->roll_speed, d[7] = b->old_roll_speed, d[8] = b->yaw_delta;
+    return 0;
+}
 int or_overflow(const OPark* p) { return p->overflow; }
 void or_set_junk(OPark* p, int64_t id, int junk) {
     OBall* b = find(p, id);

@@ -24,6 +24,7 @@ D_COLS = list(range(26))                  # pos .. last_collision (the pad left 
 TURN_NAMES = ("rx", "ry", "rz", "rw", "orx", "ory", "orz", "orw", "wx", "wy", "wz", "owx", "owy", "owz", "tx", "ty",
               "tz", "rot_roll", "max_ang_vel", "rot_agility")
 Q_COLS = list(range(11))                  # id .. moribund
+YPR_NAMES = ("yaw", "pitch", "roll", "old_yaw", "old_pitch", "old_roll", "roll_speed", "old_roll_speed", "yaw_delta")
 
 
 def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
@@ -104,24 +105,29 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
             d = np.concatenate((d, np.stack([cf[0:3, r] for r in rows])), axis=1)
             qq = np.concatenate((qq, np.stack([cq[0:7, r] for r in rows])), axis=1)
             lists = [(lists[k], [int(x) for x in hits[:cq[7, r], r]]) for k, r in enumerate(rows)]
+        if park.display_ypr:                # the display orientation last
+            y = park._y[:, :n].cpu().numpy()
+            d = np.concatenate((d, np.stack([y[:, r] for r in rows])), axis=1)
         out[wi] = (d, qq, lists)
     return out
 
 
 def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bool = False,
-             general: bool = False, order=None) -> dict:
+             general: bool = False, order=None, initial: bool = False) -> dict:
     """{(world index, step): (d, q, followers)} after each recorded tick (`record`: a set of steps, None for all).
-    `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending)."""
+    `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending);
+    `initial`: also step 0, the state as loaded. A world flagged `ypr` keeps the display orientation."""
     groups = defaultdict(list)
     for wi in (order if order is not None else range(len(worlds))):
         w = worlds[wi]
         groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)),
-                bool(w.get("collisions", False)))].append(wi)
+                bool(w.get("collisions", False)), bool(w.get("ypr", False)))].append(wi)
     out = {}
-    for (friction, tick, new_orbit, dynamical, coll), members in groups.items():
+    for (friction, tick, new_orbit, dynamical, coll, ypr), members in groups.items():
         balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
-                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll)
+                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
+                            display_ypr=ypr)
         park.load([{"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                     "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
@@ -140,6 +146,9 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
             for c in worlds[wi]["cmds"]:
                 per_step[c[0]][wi].append(c)
         last = max(worlds[wi]["steps"] for wi in members)
+        if initial:
+            for wi, v in _snapshot(park, members, rows_of).items():
+                out[(wi, 0)] = v
         g = ModesGraph(park) if graph else None
         for s in range(1, last + 1):
             todo = per_step.get(s, {})
@@ -177,34 +186,44 @@ def _run_turning(w: dict):
         yield step, np.concatenate((d[:, D_COLS], o.turning()), axis=1), q, fol      # 46 columns
 
 
-def _run_collide(w: dict):
-    """O.run_world for a world with collisions: mLastC after the doubles, the filing after the ints, each ball's
-    (followers, collision list)."""
-    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], bool(w.get("dynamical")), True)
+def _run_flags(w: dict, initial: bool = False):
+    """O.run_world for a world with collisions or the display orientation: the turning state (with dynamical
+    orientation) and mLastC (with collisions) after the doubles, then the display orientation; the filing after the
+    ints; with collisions each ball's (followers, collision list)."""
+    dyn, coll, ypr = bool(w.get("dynamical")), bool(w.get("collisions")), bool(w.get("ypr"))
+    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], dyn, coll, ypr)
     for b in w["balls"]:
         o.add(b)
     by_step = defaultdict(list)
     for c in w["cmds"]:
         by_step[c[0]].append(c)
-    for step in range(1, w["steps"] + 1):
-        for c in by_step[step]:
-            o.command(c)
-        o.evolve()
+    for step in range(0 if initial else 1, w["steps"] + 1):
+        if step:
+            for c in by_step[step]:
+                o.command(c)
+            o.evolve()
         d, q, fol = o.state()
-        lc, filing, hits = o.collide()
-        parts = (d[:, D_COLS], o.turning(), lc) if w.get("dynamical") else (d[:, D_COLS], lc)
-        yield (step, np.concatenate(parts, axis=1), np.concatenate((q[:, Q_COLS], filing), axis=1),
-               list(zip(fol, hits)))
+        parts, qs, lists = [d[:, D_COLS]], [q[:, Q_COLS]], fol
+        if dyn:
+            parts.append(o.turning())
+        if coll:
+            lc, filing, hits = o.collide()
+            parts.append(lc)
+            qs.append(filing)
+            lists = [(list(f), list(h)) for f, h in zip(fol, hits)]
+        if ypr:
+            parts.append(o.ypr())
+        yield step, np.concatenate(parts, axis=1), np.concatenate(qs, axis=1), lists
 
 
-def run_oracle(worlds: list, indices, record=None) -> dict:
+def run_oracle(worlds: list, indices, record=None, initial: bool = False) -> dict:
     out = {}
     for wi in indices:
         w = worlds[wi]
-        if w.get("collisions"):
-            for step, d, q, lists in _run_collide(w):
+        if w.get("collisions") or w.get("ypr") or initial:
+            for step, d, q, lists in _run_flags(w, initial):
                 if record is None or step in record:
-                    out[(wi, step)] = (d, q, [(list(f), list(h)) for f, h in lists])
+                    out[(wi, step)] = (d, q, lists)
             continue
         runs = _run_turning(w) if w.get("dynamical") else O.run_world(w)
         for step, d, q, fol in runs:
@@ -238,9 +257,11 @@ def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
         if same_d.all() and same_q.all() and gf == wf:
             continue
         if len(bad) < limit:
+            w = worlds[key[0]]
             names_d = [O.D_NAMES[c] for c in D_COLS]
-            names_d += list(TURN_NAMES) if gd.shape[1] >= len(D_COLS) + len(TURN_NAMES) else []
-            names_d += ["cx", "cy", "cz"]
+            names_d += list(TURN_NAMES) if w.get("dynamical") else []
+            names_d += ["cx", "cy", "cz"] if w.get("collisions") else []
+            names_d += list(YPR_NAMES) if w.get("ypr") else []
             names_q = [O.Q_NAMES[c] for c in Q_COLS] + ["level", "kx", "ky", "kz", "sx", "sy", "sz"]
             diffs = [f"{names_d[c]}[{k}] {gd[k, c]!r} vs {wd[k, c]!r}" for k, c in zip(*np.nonzero(~same_d))][:6]
             diffs += [f"{names_q[c]}[{k}] {gq[k, c]} vs {wq[k, c]}" for k, c in zip(*np.nonzero(~same_q))][:6]
