@@ -70,6 +70,9 @@ def lib() -> ctypes.CDLL:
         L.or_get_collide.argtypes = [vp, i, pd, pq, pq]
         L.or_park_set_ypr.argtypes = [vp, i]
         L.or_get_ypr.argtypes = [vp, i, pd]
+        L.or_add_capsule.argtypes = [vp, i64, i64, pd, pd, d]
+        L.or_add_oriented_box.argtypes = [vp, i64, i64, pd, pd, pd, pd]
+        L.or_get_static_boxes.argtypes = [vp, i, pq]
         _LIB = L
     return _LIB
 
@@ -106,6 +109,27 @@ class World:
         if self.dynamical:                             # AddDynamicallyOrientedBall and the AddBall's snap
             self.L.or_add_turning(self.p, int(b["id"]), float(b.get("max_ang_vel", 1.0)),
                                   float(b.get("rot_agility", 0.9)))
+
+    def add_static(self, s: dict) -> None:
+        """AddCapsule ({"kind": "capsule", "id", "parent", "a", "b", "radius"}) or AddOrientedBox ({"kind": "box",
+        "id", "parent", "corner", "lx", "ly", "lz"}), filed at once."""
+        v = lambda x: (ctypes.c_double * 3)(*[float(c) for c in x])
+        if s["kind"] == "capsule":
+            rc = self.L.or_add_capsule(self.p, int(s["id"]), int(s["parent"]), v(s["a"]), v(s["b"]),
+                                       float(s["radius"]))
+        elif s["kind"] == "box":
+            rc = self.L.or_add_oriented_box(self.p, int(s["id"]), int(s["parent"]), v(s["corner"]), v(s["lx"]),
+                                            v(s["ly"]), v(s["lz"]))
+        else:
+            raise ValueError(f"unknown static collidable kind {s['kind']!r}")
+        if rc:
+            raise ValueError(f"static collidable {s['id']} refused")
+
+    def static_boxes(self, k: int) -> np.ndarray:
+        """[boxes, 3]: the keys static collidable k (in the order added) is filed in, sorted."""
+        out = np.zeros((128, 3), dtype=np.int64)
+        cnt = self.L.or_get_static_boxes(self.p, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        return out[:cnt]
 
     def command(self, c) -> None:
         _, op, a, b, x, y, z, rr, w = c

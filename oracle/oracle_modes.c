@@ -66,6 +66,23 @@ typedef struct {
     double yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta;
 } OBall;
 
+typedef struct { int level; int64_t k[3]; } OBox;
+
+#define MAX_STATICS 16                         /* a world's static collidables */
+#define MAX_SBOXES 128                         /* the boxes one is filed in; more sets the park's overflow flag */
+
+typedef struct {                               /* a capsule or an oriented box (full.py:650-705) */
+    int kind;                                  /* 0 capsule, 1 oriented box */
+    int64_t id, parent;
+    double pos[3];                             /* the centre */
+    /* capsule: the segment and its radius (a float) */
+    double a[3], b[3], ab[3], radius;
+    /* oriented box: BoxShape's corners A..H, its sides' normals, the edge vectors, the AABB, the bounding radius */
+    double corner[8][3], normal[6][3], local[3][3], lo[3], hi[3], bounding_radius_f;
+    int nbox;
+    OBox boxes[MAX_SBOXES];                    /* where InsertInBoxes filed it (one level) */
+} OStatic;
+
 typedef struct {
     double friction, dt;
     int64_t current_time;
@@ -76,6 +93,8 @@ typedef struct {
     int overflow;                              /* a collision list past MAX_HITS */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
+    int ns;
+    OStatic statics[MAX_STATICS];              /* FullBallpark's capsules and oriented boxes, in the order added */
 } OPark;
 
 /* ------------------------------------------------------------------------------------------ vector helpers */
@@ -1186,8 +1205,6 @@ static void file_ball(OPark* p, OBall* b) {
     if (level == 0 && b->radius * 2 > level_width(0)) p->overflow = 1;
 }
 
-typedef struct { int level; int64_t k[3]; } OBox;
-
 static int box_cmp(const OBox* a, const OBox* b) {                      /* the key tuple's order */
     for (int q = 0; q < 3; ++q)
         if (a->k[q] != b->k[q]) return a->k[q] < b->k[q] ? -1 : 1;
@@ -1221,10 +1238,11 @@ static OBox parent_of(OBox x) {
     return y;
 }
 
+#define MAX_PBOXES ((MAX_BALLS * 8 + MAX_STATICS * MAX_SBOXES) * NUM_LEVELS)
 typedef struct {                       /* the boxes that exist, and each one's search flag (Box::mHandled) */
     int n;
-    OBox box[MAX_BALLS * 8 * NUM_LEVELS];
-    int handled[MAX_BALLS * 8 * NUM_LEVELS];
+    OBox box[MAX_PBOXES];
+    int handled[MAX_PBOXES];
 } OBoxes;
 
 static int box_index(const OBoxes* U, OBox x) {
@@ -1233,23 +1251,38 @@ static int box_index(const OBoxes* U, OBox x) {
     return -1;
 }
 
+static void boxes_add(OBoxes* U, OBox x) {                              /* the box and its ancestors */
+    for (;;) {
+        if (box_index(U, x) < 0) {
+            U->box[U->n] = x;
+            U->handled[U->n++] = 0;
+        }
+        if (x.level == 0) break;
+        x = parent_of(x);
+    }
+}
+
 static void boxes_build(const OPark* p, OBoxes* U) {
     U->n = 0;
     for (int i = 0; i < p->n; ++i) {
         OBox bx[8];
         int c = ball_boxes(&p->balls[i], bx);
-        for (int j = 0; j < c; ++j) {
-            OBox x = bx[j];
-            for (;;) {
-                if (box_index(U, x) < 0) {
-                    U->box[U->n] = x;
-                    U->handled[U->n++] = 0;
-                }
-                if (x.level == 0) break;
-                x = parent_of(x);
-            }
-        }
+        for (int j = 0; j < c; ++j) boxes_add(U, bx[j]);
     }
+    for (int i = 0; i < p->ns; ++i)
+        for (int j = 0; j < p->statics[i].nbox; ++j) boxes_add(U, p->statics[i].boxes[j]);
+}
+
+static int static_in(const OStatic* s, OBox x) {
+    for (int j = 0; j < s->nbox; ++j)
+        if (box_eq(&s->boxes[j], &x)) return 1;
+    return 0;
+}
+
+static int box_static_count(const OPark* p, OBox x) {
+    int cnt = 0;
+    for (int i = 0; i < p->ns; ++i) cnt += static_in(&p->statics[i], x);
+    return cnt;
 }
 
 static int ball_in_box(const OBall* c, OBox x) {
@@ -1278,16 +1311,19 @@ static void explode_children(const OPark* p, const OBoxes* U, OBox x, int* out, 
     qsort(kids, nk, sizeof(OBox), box_qsort);
     for (int i = 0; i < nk; ++i) {
         explode_children(p, U, kids[i], out, n_out);
-        if (box_ball_count(p, kids[i]) > 0) out[(*n_out)++] = box_index(U, kids[i]);
+        if (box_ball_count(p, kids[i]) > 0 || box_static_count(p, kids[i]) > 0)
+            out[(*n_out)++] = box_index(U, kids[i]);
     }
 }
 
-/* GetNearbyBalls (space.py:819-885) for the gradient's search (collisionCase1: exclude_fixed, exclude_inert and
-   exclude_harmonics off, exclude_missiles and exclude_cloaked on), on a client's view: the ball's boxes by key; for
-   each its non-empty ancestors bottom-up until one already searched, its descendants with balls in post-order, then
-   the box itself if another ball is in it; then the balls of those boxes by id, each once. No ball here is a force
-   field (the harmonic filter passes every ball). Returns the candidates' indices in order. */
-static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out) {
+/* GetNearbyBalls (space.py:819-885; with the static collidables full.py:1115-1192) for the gradient's search
+   (collisionCase1: exclude_fixed, exclude_inert and exclude_harmonics off, exclude_missiles and exclude_cloaked on), on
+   a client's view: the ball's boxes by key; for each its non-empty ancestors bottom-up until one already searched, its
+   non-empty descendants in post-order, then the box itself if another ball or a static collidable is in it; then the
+   balls of those boxes by id, each once, and their static collidables by (parent id, id), each once. No ball here is a
+   force field (the harmonic filter passes every ball). Returns the ball candidates' indices in order; the static
+   candidates' indices go to `sout` (their count to *nsout). */
+static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out, int* sout, int* nsout) {
     const OBall* b = &p->balls[bi];
     boxes_build(p, U);
     OBox bx[8];
@@ -1302,19 +1338,20 @@ static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out) {
             x = parent_of(x);
             int xi = box_index(U, x);
             if (U->handled[xi]) break;
-            if (box_ball_count(p, x) > 0) {
+            if (box_ball_count(p, x) > 0 || box_static_count(p, x) > 0) {
                 sbox[ns++] = xi;
                 U->handled[xi] = 1;
             }
         }
         explode_children(p, U, bx[j], sbox, &ns);
-        if (box_ball_count(p, bx[j]) > 1) {
+        if (box_ball_count(p, bx[j]) > 1 || box_static_count(p, bx[j]) > 0) {
             sbox[ns++] = bj;
             U->handled[bj] = 1;
         }
     }
-    int handled[MAX_BALLS] = {0};
+    int handled[MAX_BALLS] = {0}, s_handled[MAX_STATICS] = {0};
     int cnt = 0;
+    *nsout = 0;
     for (int s = 0; s < ns; ++s) {
         OBox x = U->box[sbox[s]];
         U->handled[sbox[s]] = 0;
@@ -1326,6 +1363,19 @@ static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out) {
             if (o->mode == MISSILE) continue;
             handled[i] = 1;
             out[cnt++] = i;
+        }
+        for (;;) {                                                      /* its statics by (parent id, id) */
+            int pick = -1;
+            for (int i = 0; i < p->ns; ++i) {
+                const OStatic* t = &p->statics[i];
+                if (s_handled[i] || !static_in(t, x)) continue;
+                if (pick < 0 || t->parent < p->statics[pick].parent ||
+                    (t->parent == p->statics[pick].parent && t->id < p->statics[pick].id))
+                    pick = i;
+            }
+            if (pick < 0) break;
+            s_handled[pick] = 1;
+            sout[(*nsout)++] = pick;
         }
     }
     return cnt;
@@ -1461,12 +1511,15 @@ static void potential(OPark* p, OBall* me, OBall* other, int depth) {   /* space
     add_hit(p, me, other->id);
 }
 
+#include "oracle_statics.c"
+
 static OBoxes g_boxes;                                                  /* the walk's work space (one thread) */
 
-static void gradient(OPark* p, int bi) {                                /* space.py:1650-1661 */
+/* Gradient (space.py:1650-1661; with the static collidables full.py:2628-2644) */
+static void gradient(OPark* p, int bi) {
     OBall* b = &p->balls[bi];
-    int cand[MAX_BALLS];
-    int nc = nearby_balls(p, &g_boxes, bi, cand);
+    int cand[MAX_BALLS], scand[MAX_STATICS], nsc = 0;
+    int nc = nearby_balls(p, &g_boxes, bi, cand, scand, &nsc);
     for (int c = 0; c < nc; ++c) {
         OBall* n = &p->balls[cand[c]];
         if (((n->mode == MISSILE || n->mode == MUSHROOM) && b->id == n->owner_id) ||
@@ -1474,6 +1527,76 @@ static void gradient(OPark* p, int bi) {                                /* space
             continue;
         potential(p, b, n, 0);
     }
+    for (int c = 0; c < nsc; ++c) {
+        if (b->mode == MISSILE || b->mode == MUSHROOM) continue;
+        collide_with_ball(p, &p->statics[scand[c]], b);
+    }
+}
+
+/* AddCapsule (full.py:1361-1378, Capsule::Initialize :677-682): the radius as a float; filed at once. */
+int or_add_capsule(OPark* p, int64_t id, int64_t parent, const double* a, const double* b, double radius) {
+    if (p->ns >= MAX_STATICS) return -1;
+    OStatic* c = &p->statics[p->ns++];
+    memset(c, 0, sizeof(*c));
+    c->kind = 0, c->id = id, c->parent = parent;
+    copy3(a, c->a);
+    copy3(b, c->b);
+    c->radius = c_float(radius);
+    sub3(c->b, c->a, c->ab);
+    double t[3];
+    add3(c->a, c->b, t);
+    scale3(t, 0.5, c->pos);
+    capsule_file(p, c);
+    return 0;
+}
+
+/* AddOrientedBox (full.py:1380-1398; BoxShape::BoxShape :506-520, OrientedBox::Initialize :693-700): filed at once. */
+int or_add_oriented_box(OPark* p, int64_t id, int64_t parent, const double* corner, const double* lx,
+                        const double* ly, const double* lz) {
+    if (p->ns >= MAX_STATICS) return -1;
+    OStatic* s = &p->statics[p->ns++];
+    memset(s, 0, sizeof(*s));
+    s->kind = 1, s->id = id, s->parent = parent;
+    double(*C)[3] = s->corner;
+    copy3(corner, C[0]);
+    add3(C[0], lx, C[1]);
+    add3(C[0], ly, C[2]);
+    add3(C[0], lz, C[3]);
+    add3(C[1], ly, C[4]);
+    add3(C[2], lz, C[5]);
+    add3(C[1], lz, C[6]);
+    add3(C[4], lz, C[7]);
+    for (int side = 0; side < 6; ++side) {                              /* RectangleShape's normal */
+        const double *ra = C[SIDE_CORNERS[side][0]], *rb = C[SIDE_CORNERS[side][1]], *rc = C[SIDE_CORNERS[side][2]];
+        tri_normal(ra, rb, rc, s->normal[side]);
+    }
+    copy3(lx, s->local[0]);
+    copy3(ly, s->local[1]);
+    copy3(lz, s->local[2]);
+    double d[3];
+    sub3(C[7], C[0], d);
+    s->bounding_radius_f = c_float(length(d) / 2.0);
+    box_center(s, s->pos);
+    copy3(C[0], s->lo);
+    copy3(C[0], s->hi);
+    for (int i = 1; i < 8; ++i)                                         /* AABB::Update (full.py:627-632) */
+        for (int q = 0; q < 3; ++q) {
+            if (C[i][q] > s->hi[q]) s->hi[q] = C[i][q];
+            else if (C[i][q] < s->lo[q]) s->lo[q] = C[i][q];
+        }
+    obox_file(p, s);
+    return 0;
+}
+
+/* the static collidable k's filing: its level, then (kx, ky, kz) of each box, sorted; returns the box count */
+int or_get_static_boxes(const OPark* p, int k, int64_t* out) {
+    if (k < 0 || k >= p->ns) return -1;
+    const OStatic* s = &p->statics[k];
+    OBox bx[MAX_SBOXES];
+    memcpy(bx, s->boxes, sizeof(OBox) * s->nbox);
+    qsort(bx, s->nbox, sizeof(OBox), box_qsort);
+    for (int j = 0; j < s->nbox; ++j) out[3 * j] = bx[j].k[0], out[3 * j + 1] = bx[j].k[1], out[3 * j + 2] = bx[j].k[2];
+    return s->nbox;
 }
 
 /* ------------------------------------------------------ the display orientation (full.py:893-905, 2043-2089) */

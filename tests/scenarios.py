@@ -563,3 +563,268 @@ def do_collide_cases(seed: int, scale: int = 1) -> list:
                                       z=r.uniform(-math.pi, math.pi)))
         w["cmds"].sort(key=lambda c: c[0])
     return out
+
+
+def _rot_frame(r: random.Random) -> list:
+    """Three orthonormal axes: a random rotation, or the coordinate axes (an aligned box or capsule)."""
+    if r.random() < 0.25:
+        return [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]
+    u = _unit(r)
+    while True:
+        v = _unit(r)
+        d = sum(u[i] * v[i] for i in range(3))
+        v = [v[i] - d * u[i] for i in range(3)]
+        n = math.sqrt(sum(c * c for c in v))
+        if n > 1e-3:
+            v = [c / n for c in v]
+            break
+    w = [u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]]
+    return [u, v, w]
+
+
+def _capsule(r: random.Random, sid: int, parent: int, a: list, b: list, radius: float) -> dict:
+    return {"kind": "capsule", "id": sid, "parent": parent, "a": [float(x) for x in a], "b": [float(x) for x in b],
+            "radius": float(radius)}
+
+
+def _obox(r: random.Random, sid: int, parent: int, centre: list, half: list, frame: list) -> dict:
+    """An oriented box around `centre`: edge vectors 2 half[i] along the frame's axes (a skewed one sometimes)."""
+    lx, ly, lz = ([2.0 * half[i] * frame[i][q] for q in range(3)] for i in range(3))
+    if r.random() < 0.1:                           # skewed: an edge leaning into another
+        lz = [lz[q] + 0.3 * lx[q] for q in range(3)]
+    corner = [centre[q] - 0.5 * (lx[q] + ly[q] + lz[q]) for q in range(3)]
+    return {"kind": "box", "id": sid, "parent": parent, "corner": corner, "lx": lx, "ly": ly, "lz": lz}
+
+
+def static_cases(seed: int, scale: int = 1) -> list:
+    """The static collidable worlds (about 150 x scale) for seed: ships (free, massive) flown at capsules (through the
+    cylinder, at a cap, grazing, along the axis, from inside, at rest beside one, backing into one), at oriented boxes
+    (faces, edges, corners, from inside and at the centre, backing in, aligned, rotated, skewed, flat and wider than a
+    top box), through fields of capsules, boxes and fixed spheres, touching two at once, and on partition box faces at
+    several levels; degenerate capsules (zero length, no volume). Each world carries its capsules and boxes under
+    "statics" (AddCapsule's and AddOrientedBox's arguments, ids distinct within the world, parents shared sometimes)."""
+    r = random.Random(seed ^ 0x57A71C)
+    out: list = []
+
+    def world(kind, balls, statics, cmds, steps, *, tick=1000, friction=1.0e6):
+        out.append({"kind": kind, "friction": friction, "tick": tick, "steps": steps, "new_orbit": False,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": cmds, "collisions": True,
+                    "statics": statics})
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    def ship(bid, pos, **kw):
+        b = _coll_ball(r, bid, pos, 0.0, **kw)
+        b["pos"] = [float(x) for x in pos]
+        return b
+
+    def sids(n):
+        ids = r.sample(range(2000, 5000), n)
+        parents = [r.choice([0, 1, 7, r.randrange(1, 60)]) for _ in range(n)]
+        return ids, parents
+
+    def go(bid, tgt, step=1):
+        return _cmd(step, GOTO_OP, bid, x=tgt[0], y=tgt[1], z=tgt[2])
+
+    def along(p, u, d):
+        return [p[q] + u[q] * d for q in range(3)]
+
+    ticks = lambda: r.choice([1000, 1000, 500, 250, 2000])
+    fric = lambda: r.choice([1.0e6, 1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)])
+
+    for _ in range(40 * scale):                    # capsules: through, at a cap, grazing, along the axis, inside
+        c = centre()
+        fr = _rot_frame(r)
+        length = r.choice([r.uniform(200.0, 5000.0), 1024.0, 4096.0])
+        rad = r.choice([r.uniform(20.0, 400.0), r.uniform(400.0, 2000.0)])
+        axis = fr[0]
+        ca, cb = along(c, axis, -0.5 * length), along(c, axis, 0.5 * length)
+        (sid,), (par,) = sids(1)
+        flavour = r.choice(["through", "through", "cap", "graze", "axis", "inside", "rest", "backing"])
+        ids = _ids(r, r.randint(1, 3), big=r.random() < 0.2)
+        balls, cmds = [], []
+        for bid in ids:
+            side = fr[1] if r.random() < 0.5 else fr[2]
+            off = r.uniform(-0.5, 0.5) * length
+            if flavour == "through":
+                p0 = along(along(c, axis, off), side, -(rad + r.uniform(300.0, 3000.0)))
+                tgt = along(along(c, axis, off), side, rad + 5000.0)
+            elif flavour == "cap":
+                end = r.choice([-1.0, 1.0])
+                p0 = along(c, axis, end * (0.5 * length + rad + r.uniform(300.0, 3000.0)))
+                p0 = along(p0, side, r.uniform(-0.5, 0.5) * rad)
+                tgt = along(c, axis, -end * 0.5 * length)
+            elif flavour == "graze":
+                p0 = along(along(c, axis, off), side, -(rad + 2000.0))
+                tgt = along(along(along(c, axis, off), side, rad + 5000.0), fr[2] if side is fr[1] else fr[1],
+                            rad * r.uniform(0.8, 1.3))
+            elif flavour == "axis":
+                p0 = along(c, axis, -(0.5 * length + rad + r.uniform(100.0, 2000.0)))
+                tgt = along(c, axis, 0.5 * length + 8000.0)
+            elif flavour == "inside":
+                p0 = along(along(c, axis, off), side, r.choice([0.0, r.uniform(0.0, 0.9) * rad]))
+                tgt = along(p0, side, r.choice([-1.0, 1.0]) * r.uniform(0.0, 4000.0))
+            elif flavour == "rest":
+                p0 = along(along(c, axis, r.choice([off, 0.5 * length + rad * 0.5])), side,
+                           rad * r.uniform(0.5, 1.5))
+                tgt = None
+            else:                                  # backing: moving in, thrust pointed out
+                p0 = along(along(c, axis, off), side, rad + r.uniform(50.0, 400.0))
+                tgt = along(p0, side, 20000.0)
+            b = ship(bid, p0)
+            if flavour in ("rest",):
+                b["vel"] = [0.0, 0.0, 0.0]
+            elif flavour == "backing":
+                b["vel"] = [-side[q] * r.uniform(100.0, 400.0) for q in range(3)]
+            elif flavour == "axis" and fr[0] == [1.0, 0.0, 0.0]:
+                b["vel"] = [r.uniform(50.0, 300.0), 0.0, 0.0]
+            balls.append(b)
+            if tgt is not None:
+                cmds.append(go(bid, tgt, r.choice([1, 1, 2])))
+        statics = [_capsule(r, sid, par, ca, cb, rad)]
+        if r.random() < 0.15:                      # a degenerate one beside it: no length, or no volume
+            (s2,), (p2,) = sids(1)
+            s2 = sid + 1 if s2 == sid else s2
+            if r.random() < 0.5:
+                statics.append(_capsule(r, s2, p2, c, c, rad))
+            else:
+                statics.append(_capsule(r, s2, p2, ca, cb, 0.0))
+        world("capsule_" + flavour, balls, statics, cmds, r.randint(8, 30), tick=ticks(), friction=fric())
+
+    for _ in range(40 * scale):                    # oriented boxes: faces, edges, corners, inside, centre, backing
+        c = centre()
+        fr = _rot_frame(r)
+        half = [r.uniform(50.0, 1500.0) for _ in range(3)]
+        if r.random() < 0.15:
+            half[r.randrange(3)] = r.uniform(0.5, 5.0)            # flat
+        (sid,), (par,) = sids(1)
+        flavour = r.choice(["face", "face", "edge", "corner", "inside", "centre", "backing", "rest"])
+        ids = _ids(r, r.randint(1, 3), big=r.random() < 0.2)
+        balls, cmds = [], []
+        for bid in ids:
+            i = r.randrange(3)
+            sgn = r.choice([-1.0, 1.0])
+            if flavour == "face":
+                p0 = along(c, fr[i], sgn * (half[i] + r.uniform(300.0, 3000.0)))
+                p0 = along(p0, fr[(i + 1) % 3], r.uniform(-0.8, 0.8) * half[(i + 1) % 3])
+                tgt = along(c, fr[i], -sgn * 6000.0)
+            elif flavour == "edge":
+                j = (i + 1) % 3
+                edge = along(along(c, fr[i], sgn * half[i]), fr[j], r.choice([-1.0, 1.0]) * half[j])
+                p0 = along(along(edge, fr[i], sgn * 2000.0), fr[j], r.uniform(-300.0, 300.0))
+                tgt = along(c, fr[i], -sgn * 3000.0)
+            elif flavour == "corner":
+                cor = [c[q] + sum(r.choice([-1.0, 1.0]) * half[k] * fr[k][q] for k in range(3)) for q in range(3)]
+                d = [cor[q] - c[q] for q in range(3)]
+                p0 = [c[q] + d[q] * r.uniform(1.5, 4.0) for q in range(3)]
+                tgt = [c[q] - d[q] for q in range(3)]
+            elif flavour == "inside":
+                p0 = [c[q] + sum(r.uniform(-0.9, 0.9) * half[k] * fr[k][q] for k in range(3)) for q in range(3)]
+                tgt = along(p0, fr[i], sgn * r.uniform(0.0, 5000.0))
+            elif flavour == "centre":
+                p0 = list(c)
+                tgt = along(c, fr[i], sgn * 4000.0) if r.random() < 0.6 else None
+            elif flavour == "backing":
+                p0 = along(c, fr[i], sgn * (half[i] + r.uniform(50.0, 300.0)))
+                tgt = along(p0, fr[i], sgn * 20000.0)
+            else:
+                p0 = along(c, fr[i], sgn * (half[i] + r.uniform(-20.0, 100.0)))
+                tgt = None
+            b = ship(bid, p0)
+            if flavour in ("rest", "centre") and r.random() < 0.7:
+                b["vel"] = [0.0, 0.0, 0.0]
+            if flavour == "backing":
+                b["vel"] = [-sgn * fr[i][q] * r.uniform(100.0, 400.0) for q in range(3)]
+            balls.append(b)
+            if tgt is not None:
+                cmds.append(go(bid, tgt, r.choice([1, 1, 3])))
+        world("box_" + flavour, balls, [_obox(r, sid, par, c, half, fr)], cmds, r.randint(8, 30), tick=ticks(),
+              friction=fric())
+
+    for _ in range(5 * scale):                     # a box wider than a top box: filed in the 27 top boxes
+        c = centre()
+        half = [r.uniform(2.5e6, 4e6) for _ in range(3)]
+        (sid,), (par,) = sids(1)
+        bid = r.randrange(1, 1000)
+        p0 = along(c, [1.0, 0.0, 0.0], half[0] + 1500.0)
+        b = ship(bid, p0, vel=[-200.0, 0.0, 0.0])
+        world("box_huge", [b], [_obox(r, sid, par, c, half, _rot_frame(r))], [go(bid, c)], r.randint(5, 15))
+
+    for _ in range(30 * scale):                    # fields: capsules, boxes and fixed spheres, racers through them
+        c = centre()
+        n_st = r.randint(3, 8)
+        st_ids, parents = sids(n_st)
+        statics = []
+        for k in range(n_st):
+            p = [c[q] + r.uniform(-3000.0, 3000.0) for q in range(3)]
+            fr = _rot_frame(r)
+            if r.random() < 0.5:
+                ln, rad = r.uniform(100.0, 3000.0), r.uniform(30.0, 500.0)
+                statics.append(_capsule(r, st_ids[k], parents[k], along(p, fr[0], -0.5 * ln), along(p, fr[0], 0.5 * ln),
+                                        rad))
+            else:
+                statics.append(_obox(r, st_ids[k], parents[k], p, [r.uniform(30.0, 800.0) for _ in range(3)], fr))
+        ids = _ids(r, r.randint(2, 6), big=r.random() < 0.2)
+        balls, cmds = [], []
+        for k, bid in enumerate(ids):
+            if k >= 2 and r.random() < 0.3:
+                balls.append(_sphere(r, bid, c, 3000.0))
+                continue
+            u = _unit(r)
+            b = ship(bid, along(c, u, -r.uniform(4000.0, 7000.0)))
+            balls.append(b)
+            cmds.append(go(bid, along(c, u, 6000.0), r.choice([1, 1, 2])))
+            if r.random() < 0.3:
+                cmds.append(_cmd(r.randint(4, 15), STOP_OP, bid))
+        world("field", balls, statics, cmds, r.randint(10, 40), tick=ticks(), friction=fric())
+
+    for _ in range(20 * scale):                    # two or three at once: the walk's order and the equal-time rule
+        c = centre()
+        n_st = r.choice([2, 2, 3])
+        st_ids, parents = sids(n_st)
+        if r.random() < 0.4:
+            parents = [parents[0]] * n_st
+        u = _unit(r)
+        fr = _rot_frame(r)
+        statics = []
+        gap = r.uniform(0.0, 200.0)
+        for k in range(n_st):
+            sgn = 1.0 if k % 2 == 0 else -1.0
+            p = along(along(c, fr[1], sgn * (400.0 + gap)), fr[2], 300.0 * (k // 2))
+            if r.random() < 0.5:
+                statics.append(_capsule(r, st_ids[k], parents[k], along(p, fr[0], -1500.0), along(p, fr[0], 1500.0),
+                                        300.0))
+            else:
+                statics.append(_obox(r, st_ids[k], parents[k], along(p, fr[1], sgn * 300.0), [1500.0, 300.0, 300.0],
+                                     fr))
+        bid = r.randrange(1, 1000)
+        b = ship(bid, along(c, fr[0], -4000.0), radius=r.uniform(100.0 + gap, 400.0 + gap))
+        b["vel"] = [fr[0][q] * r.uniform(50.0, 300.0) for q in range(3)]
+        world("two_statics", [b], statics, [go(bid, along(c, fr[0], 6000.0))], r.randint(8, 25), tick=ticks())
+
+    for _ in range(15 * scale):                    # on partition box faces at several levels
+        lvl_w = [7864320.0 / 4 ** i for i in range(8)]
+        base = lvl_w[r.randint(4, 7)]
+        c = [round(r.uniform(-1e6, 1e6) / base) * base for _ in range(3)]
+        n_st = r.randint(2, 5)
+        st_ids, parents = sids(n_st)
+        statics = []
+        for k in range(n_st):
+            p = [c[q] + r.choice([-1, 1]) * r.uniform(0.0, 800.0) for q in range(3)]
+            fr = _rot_frame(r)
+            if r.random() < 0.5:
+                ln, rad = r.uniform(50.0, 4000.0), 10 ** r.uniform(0.5, 3.3)
+                statics.append(_capsule(r, st_ids[k], parents[k], along(p, fr[0], -0.5 * ln), along(p, fr[0], 0.5 * ln),
+                                        rad))
+            else:
+                statics.append(_obox(r, st_ids[k], parents[k], p, [10 ** r.uniform(0.5, 3.0) for _ in range(3)], fr))
+        ids = _ids(r, r.randint(1, 4), big=False)
+        balls, cmds = [], []
+        for bid in ids:
+            b = ship(bid, [c[q] + r.uniform(-3000.0, 3000.0) for q in range(3)],
+                     radius=10 ** r.uniform(0.0, 2.5))
+            balls.append(b)
+            cmds.append(go(bid, [c[q] + r.uniform(-3000.0, 3000.0) for q in range(3)], r.choice([1, 2])))
+        world("faces", balls, statics, cmds, r.randint(8, 30), tick=ticks())
+    return out
