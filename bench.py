@@ -11,6 +11,8 @@ Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
 - flight: 65,536 worlds x 4 ships, modes uniform.
 - turning: 65,536 worlds x 4 ships in 2 km clusters with dynamical orientation.
 - collide: 65,536 worlds x (4 ships + 4 fixed spheres) in 2 km clusters, the gradient collision method.
+- statics: the turning and collisions park with static collidables: 65,536 worlds x (4 ships + 4 fixed spheres) and 8
+  static collidables a world (capsules and oriented boxes), dynamical orientation, the gradient method.
 Each ragged workload (composed, ragged, profiles) is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
 row order, one tick a launch) and as the autotuned packed launch: whole worlds packed into tiles, with workgroup size,
 lane schedule, ticks per launch and world order (the row order, or the learned scheduler's regrouping, trained on the
@@ -46,6 +48,8 @@ BYTES_PER_SHIP_TICK_DO = 469       # with it: + 36 read (rot, ang_vel, max_ang_v
                                    # + 72 written (rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll)
 BYTES_PER_SHIP_TICK_COLL = 449     # with gradient collisions: + 28 read (the filing: key 24, level and sides 4)
                                    # + 60 written (mLastC 24, the collision list's length 8, the filing 28)
+BYTES_PER_SHIP_TICK_STATICS = 8    # with static collidables: the row's world's range (2 int32) read; the table is
+                                   # read only where a free ship's prediction reaches a static's test
 HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
 TRAFFIC_FILE = ROOT / "profiles" / "modes_tick_traffic.json"
 WORKLOADS = {
@@ -64,6 +68,11 @@ WORKLOADS = {
     "collide": "collision park: {worlds} worlds x (4 free massive ships in a 2 km cluster + 4 fixed massive spheres "
                "of radius 100-5000 m), every ship going to a point of its cluster; the gradient collision method "
                "(partition filing, walk-order contacts, Potential); no dynamical orientation or static collidables",
+    "statics": "turning and collisions park with static collidables: {worlds} worlds x (4 free massive ships in a 2 km "
+               "cluster + 4 fixed massive spheres of radius 100-5000 m) and 8 static collidables a world (capsules of "
+               "length 200-3000 m and radius 30-400 m, oriented boxes of half-extents 30-800 m, in turn, randomly "
+               "oriented within 2 km), dynamical orientation (rotations from yaw, pitch and roll uniform in +/-pi), "
+               "every ship going to a point of its cluster; the gradient collision method",
 }
 
 
@@ -131,7 +140,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=320)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--worlds", type=int, default=65536)
-    ap.add_argument("--workload", choices=("composed", "ragged", "profiles", "flight", "turning", "collide"),
+    ap.add_argument("--workload", choices=("composed", "ragged", "profiles", "flight", "turning", "collide", "statics"),
                     default="composed",
                     help="the primary line's workload (the others are measured beside it unless --only)")
     ap.add_argument("--only", action="store_true", help="measure the primary workload only")
@@ -175,6 +184,8 @@ def main() -> int:
             wl = WL.turning_park(1, a.worlds, 4)
         elif kind == "collide":
             wl = WL.collide_park(1, a.worlds, 4, 4)
+        elif kind == "statics":
+            wl = WL.collide_park(1, a.worlds, 4, 4, dynamical=True, statics=8)
         elif kind == "composed":
             wl = WL.composed_park(1, a.worlds)
         else:
@@ -238,9 +249,9 @@ def main() -> int:
         if shape["ticks"] > 1 and not a.no_single:
             one = measure(1, max(32, a.steps // 4), 4)
         n = wl["n"]
-        per_ship = {"turning": BYTES_PER_SHIP_TICK_DO, "collide": BYTES_PER_SHIP_TICK_COLL,
-                    "composed": BYTES_PER_SHIP_TICK_DO + BYTES_PER_SHIP_TICK_COLL - BYTES_PER_SHIP_TICK}.get(
-                        kind, BYTES_PER_SHIP_TICK)
+        both = BYTES_PER_SHIP_TICK_DO + BYTES_PER_SHIP_TICK_COLL - BYTES_PER_SHIP_TICK
+        per_ship = {"turning": BYTES_PER_SHIP_TICK_DO, "collide": BYTES_PER_SHIP_TICK_COLL, "composed": both,
+                    "statics": both + BYTES_PER_SHIP_TICK_STATICS}.get(kind, BYTES_PER_SHIP_TICK)
         res = {"workload": kind, "worlds": a.worlds, "ships": n, "shape": shape,
                "world_ticks_per_s": a.worlds / (m["ms_per_tick"] / 1000.0),
                "ship_ticks_per_s": n / (m["ms_per_tick"] / 1000.0), "ms_per_tick": m["ms_per_tick"],
@@ -295,11 +306,11 @@ def main() -> int:
     primary = tuned(a.workload)
     out_extra = {}
     if not a.only:
-        for kind in ("composed", "ragged", "profiles", "flight", "turning", "collide"):
+        for kind in ("composed", "ragged", "profiles", "flight", "turning", "collide", "statics"):
             if kind != a.workload:
                 other = tuned(kind)
                 out_extra[kind + "_park"] = {k: v for k, v in other.items() if not k.startswith("_")}
-                if kind in ("turning", "collide") and not a.no_cpu:
+                if kind in ("turning", "collide", "statics") and not a.no_cpu:
                     out_extra[kind + "_park"]["cpu_baseline"], out_extra[kind + "_park"]["exact"] = cpu_baseline(
                         other["_park"], other["_wl"], other["_ticks_done"], a.cpu_seconds / 2, max_worlds=1024)
                 del other

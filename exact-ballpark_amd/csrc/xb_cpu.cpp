@@ -35,6 +35,12 @@ XbPark host_park(const xb_modes_state& s) {
     P.cq = s.coll_i64;
     P.hits = s.hits;
     P.hits_cap = s.hits_cap;
+    P.statics = s.statics && s.st_f64 && s.st_range;
+    P.st_range = s.st_range;
+    P.st_f = s.st_f64;
+    P.st_q = s.st_i64;
+    P.st_keys = s.st_keys;
+    P.st_stride = s.st_stride;
     return P;
 }
 
@@ -247,6 +253,84 @@ int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_
         xb_coll_store(P, r, c);
     }
     return 0;
+}
+
+int xb_cpu_statics_setup(int64_t n, const int64_t* kind, const int64_t* id, const int64_t* parent,
+                         const double* geom, double* st_f64, int64_t* st_i64, int64_t* st_keys, int64_t st_stride,
+                         int32_t max_boxes) {
+    if (n < 0 || st_stride < n || max_boxes < 0 || (n > 0 && (!kind || !id || !parent || !geom || !st_f64 ||
+                                                               !st_i64 || (max_boxes > 0 && !st_keys))))
+        return -1;
+    int most = 0;
+    std::vector<int64_t> keys(3 * (size_t)(max_boxes > 0 ? max_boxes : 1));
+    for (int64_t k = 0; k < n; ++k) {
+        const double* g = geom + 12 * k;
+        double* f = st_f64;
+        auto put = [&](int row, XbV3 v) {
+            f[row * st_stride + k] = v.x, f[(row + 1) * st_stride + k] = v.y, f[(row + 2) * st_stride + k] = v.z;
+        };
+        for (int row = 0; row < XB_NSF; ++row) f[row * st_stride + k] = 0.0;
+        XbBoxSet set;
+        set.keys = keys.data(), set.cap = max_boxes, set.n = 0;
+        int level = 0, cnt;
+        if (kind[k] == XB_ST_CAPSULE) {                  /* Capsule::Initialize: the radius a float */
+            const XbV3 a = v3(g[0], g[1], g[2]), b = v3(g[3], g[4], g[5]);
+            const double radius = xb_f32(g[6]);
+            const XbV3 ab = v3_sub(b, a);
+            put(XB_SF_A, a), put(XB_SF_AB, ab), put(XB_SF_B, b);
+            f[XB_SF_RAD * st_stride + k] = radius;
+            f[XB_SF_ABAB * st_stride + k] = v3_dot(ab, ab);
+            const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+            cnt = xb_capsule_boxes(pa, pb, radius, &level, set);
+            if (cnt < 0) return -2;
+        } else if (kind[k] == XB_ST_BOX) {               /* BoxShape::BoxShape, OrientedBox::Initialize */
+            const XbV3 A = v3(g[0], g[1], g[2]), lx = v3(g[3], g[4], g[5]), ly = v3(g[6], g[7], g[8]);
+            const XbV3 lz = v3(g[9], g[10], g[11]);
+            XbV3 c[8];
+            c[0] = A, c[1] = v3_add(A, lx), c[2] = v3_add(A, ly), c[3] = v3_add(A, lz);
+            c[4] = v3_add(c[1], ly), c[5] = v3_add(c[2], lz), c[6] = v3_add(c[1], lz), c[7] = v3_add(c[4], lz);
+            for (int i = 0; i < 8; ++i) put(XB_SF_CORNER + 3 * i, c[i]);
+            for (int side = 0; side < 6; ++side) {
+                const XbV3 sa = c[xb_side_corner(side, 0)], sb = c[xb_side_corner(side, 1)];
+                const XbV3 sc = c[xb_side_corner(side, 2)], sd = c[xb_side_corner(side, 3)];
+                put(XB_SF_NORMAL + 3 * side, v3_unit(v3_cross(v3_sub(sb, sa), v3_sub(sc, sa))));
+                put(XB_SF_TNORMAL + 3 * side, v3_unit(v3_cross(v3_sub(sc, sa), v3_sub(sd, sa))));
+            }
+            const XbV3 centre = v3_add(A, v3_scale(v3_sub(c[7], A), 1.0 / 2.0));
+            put(XB_SF_CENTER, centre);
+            const XbV3 loc[3] = {lx, ly, lz};
+            for (int i = 0; i < 3; ++i) {
+                put(XB_SF_LOCAL + 3 * i, loc[i]);
+                put(XB_SF_UNIT + 3 * i, v3_unit(loc[i]));
+                f[(XB_SF_HALF + i) * st_stride + k] = v3_len(loc[i]) / 2.0;
+                f[(XB_SF_LENSQ + i) * st_stride + k] = v3_lensq(loc[i]);
+            }
+            double lo[3] = {A.x, A.y, A.z}, hi[3] = {A.x, A.y, A.z};
+            for (int i = 1; i < 8; ++i) {                /* AABB::Update */
+                const double v[3] = {c[i].x, c[i].y, c[i].z};
+                for (int q = 0; q < 3; ++q) {
+                    if (v[q] > hi[q]) hi[q] = v[q];
+                    else if (v[q] < lo[q]) lo[q] = v[q];
+                }
+            }
+            put(XB_SF_LO, v3(lo[0], lo[1], lo[2])), put(XB_SF_HI, v3(hi[0], hi[1], hi[2]));
+            const double brad_f = xb_f32(v3_len(v3_sub(c[7], A)) / 2.0);
+            const double pos[3] = {centre.x, centre.y, centre.z};
+            cnt = xb_obox_boxes(lo, hi, pos, brad_f, &level, set);
+        } else {
+            return -1;
+        }
+        st_i64[XB_SI_KIND * st_stride + k] = kind[k];
+        st_i64[XB_SI_ID * st_stride + k] = id[k];
+        st_i64[XB_SI_PARENT * st_stride + k] = parent[k];
+        st_i64[XB_SI_LEVEL * st_stride + k] = level;
+        st_i64[XB_SI_NBOX * st_stride + k] = cnt < max_boxes ? cnt : max_boxes;
+        if (cnt > most) most = cnt;
+        const int stored = cnt < max_boxes ? cnt : max_boxes;
+        for (int b = 0; b < stored; ++b)
+            for (int q = 0; q < 3; ++q) st_keys[(3 * b + q) * st_stride + k] = keys[3 * b + q];
+    }
+    return most;
 }
 
 int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows) {

@@ -40,6 +40,12 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.cq = s.coll_i64;
     P.hits = s.hits;
     P.hits_cap = s.hits_cap;
+    P.statics = s.statics && s.st_f64 && s.st_range;
+    P.st_range = s.st_range;
+    P.st_f = s.st_f64;
+    P.st_q = s.st_i64;
+    P.st_keys = s.st_keys;
+    P.st_stride = s.st_stride;
     return P;
 }
 

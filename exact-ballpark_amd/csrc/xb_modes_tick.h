@@ -101,6 +101,12 @@ struct XbPark {
     int64_t* cq;                   /* int64 filing and the collision list's length */
     int64_t* hits;                 /* the collision lists, hits_cap rows */
     int hits_cap;
+    int statics;                   /* the static collidables take part */
+    const int32_t* st_range;       /* each row's world's first static collidable and how many (2 rows) */
+    const double* st_f;            /* the static table (xb_statics.h) */
+    const int64_t* st_q;
+    const int64_t* st_keys;
+    int64_t st_stride;
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -726,6 +732,7 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
 }
 
 #include "xb_collide.h"
+#include "xb_statics.h"
 
 /* ------------------------------------------------------------------------------ the tick with collisions */
 
@@ -825,6 +832,14 @@ static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, Xb
             me.last_c = c.last_c, me.last_collision = s.last_collision;
             me.nh = c.nh, me.hits = P.hits + w.row, me.stride = P.stride, me.cap = P.hits_cap;
             xb_gradient(me, c.fil, c.junk, C, R.id, R.radius, j, wlo, whi, P);
+            if (P.statics && w.free_ && s.mode != XB_MODE_MISSILE && s.mode != XB_MODE_MUSHROOM) {
+                const int cnt = P.st_range[P.stride + w.row];
+                if (cnt > 0) {
+                    XbStatics S;
+                    S.f = P.st_f, S.q = P.st_q, S.keys = P.st_keys, S.stride = P.st_stride;
+                    xb_statics_gradient(me, c.fil, S, P.st_range[w.row], cnt, P);
+                }
+            }
             c.last_c = me.last_c;
             s.last_collision = me.last_collision;
             c.nh = me.nh;

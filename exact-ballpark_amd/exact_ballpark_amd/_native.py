@@ -16,10 +16,11 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 4
+ABI_VERSION = 5
 NF, NI, NB, NS = 46, 10, 6, 7
 NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 NY = 9                          # the display orientation block: yaw, pitch, roll, their old values, roll speed (2), yaw delta
+NSF, NSI = 93, 5                # the static collidables' table: geometry; kind, id, parent, level, box count
 
 
 class EpisodeRows(ctypes.Structure):
@@ -39,7 +40,9 @@ class ModesState(ctypes.Structure):
                 ("schedule", ctypes.c_int32), ("n_keys", ctypes.c_int32), ("keys", ctypes.c_void_p),
                 ("path", ctypes.c_void_p), ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("hits_cap", ctypes.c_int32), ("collisions", ctypes.c_int32),
-                ("ypr", ctypes.c_void_p)]
+                ("ypr", ctypes.c_void_p), ("st_f64", ctypes.c_void_p), ("st_i64", ctypes.c_void_p),
+                ("st_keys", ctypes.c_void_p), ("st_range", ctypes.c_void_p), ("st_stride", ctypes.c_int64),
+                ("st_boxes", ctypes.c_int32), ("statics", ctypes.c_int32)]
 
 
 _LIBS: dict = {}
@@ -93,6 +96,7 @@ def cpu() -> ctypes.CDLL:
         L.xb_cpu_rand_draw.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
         L.xb_cpu_collide_file.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64]
         L.xb_cpu_modes_ypr_snap.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64]
+        L.xb_cpu_statics_setup.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 7 + [ctypes.c_int64, ctypes.c_int32]
         L.xb_cpu_modes_orient_setup.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
                                                 ctypes.c_int]
         L.xb_cpu_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \

@@ -55,6 +55,7 @@ FIELD_NAMES = tuple(f[0] for f in FIELDS)
 COLL_FIELDS = (("last_c", "cf", 0, 3), ("level", "cq", 0, 0), ("key", "cq", 1, 3), ("side", "cq", 4, 3),
                ("hits_n", "cq", 7, 0))
 MAX_COLLIDE_WORLD = 64                      # rows a world with collisions (one wavefront's contact mask)
+MAX_WORLD_STATICS = 64                      # static collidables a world (one response mask)
 # with the display orientation kept (Ball::CalculateYawPitchRoll, without dynamical orientation)
 YPR_FIELDS = (("yaw", "y", 0, 0), ("pitch", "y", 1, 0), ("roll", "y", 2, 0), ("old_yaw", "y", 3, 0),
               ("old_pitch", "y", 4, 0), ("old_roll", "y", 5, 0), ("roll_speed", "y", 6, 0),
@@ -183,6 +184,7 @@ class VecModesPark(metaclass=_Fields):
         if self.collisions:
             self._cf = torch.zeros(N.NCF, self.stride, dtype=F64, device=self.dev)
             self._cq = torch.zeros(N.NCI, self.stride, dtype=I64, device=self.dev)
+        self._st = None                # the static collidables' table (load_statics): device tensors and the layout
         self.display_ypr = bool(display_ypr)
         self._y = torch.zeros(N.NY, self.stride, dtype=F64, device=self.dev) if self.display_ypr else None
         self.current_time = 0
@@ -254,6 +256,14 @@ class VecModesPark(metaclass=_Fields):
             s.coll_f64, s.coll_i64 = self._cf.data_ptr(), self._cq.data_ptr()
             s.hits, s.hits_cap = self._hits.data_ptr(), self._hits.shape[0]
         s.ypr = self._y.data_ptr() if self.display_ypr else None
+        st = self._st if self.collisions else None
+        s.statics = int(st is not None)
+        s.st_f64 = st["f"].data_ptr() if st else None
+        s.st_i64 = st["q"].data_ptr() if st else None
+        s.st_keys = st["keys"].data_ptr() if st else None
+        s.st_range = st["range"].data_ptr() if st else None
+        s.st_stride = st["stride"] if st else 0
+        s.st_boxes = st["boxes"] if st else 0
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -372,12 +382,109 @@ class VecModesPark(metaclass=_Fields):
         if self.tiles is None or biggest > MAX_COLLIDE_WORLD:
             raise ValueError(f"collisions need each world's rows consecutive and at most {MAX_COLLIDE_WORLD} rows a "
                              f"world, within a workgroup of {self.block}")
-        cap = 3 * (biggest - 1) + 1
+        cap = 3 * (biggest - 1) + 1 + (self._st["most"] if self._st else 0)     # and one entry a static collidable
         self._hits = torch.full((cap, self.stride), -1, dtype=I64, device=self.dev)
+        if self._st:
+            self._st_set_range()
         self._cf.zero_()
         self._cq.zero_()
         self._state = None
         self._refile(None)
+
+    # ---- static collidables (FullBallpark's capsules and oriented boxes; VecCollidePark's load_statics) ----
+
+    def load_statics(self, statics) -> None:
+        """Each world's static collidables, as AddCapsule and AddOrientedBox add them (a client's view), filed once:
+        `statics` a list indexed by world id, or a dict {world id: list}, of {"kind": "capsule", "id", "parent", "a",
+        "b", "radius"} (the radius a float) or {"kind": "box", "id", "parent", "corner", "lx", "ly", "lz"}; ids
+        distinct within a world, at most 64 a world (src/exact_ballpark/vec_collide.py:1329-1360). A park with
+        collisions only; they take part in every tick's gradient from now on (an empty `statics`: none)."""
+        items = statics.items() if isinstance(statics, dict) else enumerate(statics)
+        world, kind, ids, parents, geom = [], [], [], [], []
+        for w, recs in items:
+            for s in recs:
+                if s["kind"] == "capsule":
+                    g = [float(x) for x in s["a"]] + [float(x) for x in s["b"]] + [float(s["radius"])] + [0.0] * 5
+                    kind.append(0)
+                elif s["kind"] == "box":
+                    g = [float(x) for k in ("corner", "lx", "ly", "lz") for x in s[k]]
+                    kind.append(1)
+                else:
+                    raise ValueError(f"unknown static collidable kind {s['kind']!r}")
+                if len(g) != 12:
+                    raise ValueError(f"static collidable {s['id']}: three coordinates a point or vector")
+                world.append(int(w))
+                ids.append(int(s["id"]))
+                parents.append(int(s["parent"]))
+                geom.append(g)
+        self.load_statics_arrays(world=world, kind=kind, id=ids, parent=parents, geom=np.reshape(geom, (-1, 12)))
+
+    def load_statics_arrays(self, *, world, kind, id, parent, geom) -> None:
+        """`load_statics` from arrays: [S] world ids, kinds (0 capsule, 1 box), ids and parent ids, and [S, 12]
+        arguments (a capsule's A, B and radius, then zeros; a box's corner, lx, ly, lz); any order of worlds."""
+        if not self.collisions:
+            raise ValueError("static collidables take part in the gradient method: make the park with collisions=True")
+        world, kind = np.asarray(world, dtype=np.int64), np.asarray(kind, dtype=np.int64)
+        ids, par = np.asarray(id, dtype=np.int64), np.asarray(parent, dtype=np.int64)
+        geom = np.ascontiguousarray(np.asarray(geom, dtype=np.float64).reshape(-1, 12))
+        n = len(world)
+        if not (len(kind) == len(ids) == len(par) == len(geom) == n):
+            raise ValueError("static collidables: the arrays' lengths differ")
+        if n == 0:
+            self._st = None
+            self._collide_setup()
+            return
+        if ((kind != 0) & (kind != 1)).any():
+            raise ValueError(f"unknown static collidable kind {int(kind[(kind != 0) & (kind != 1)][0])}")
+        order = np.argsort(world, kind="stable")
+        world, kind, ids, par, geom = world[order], kind[order], ids[order], par[order], geom[order]
+        wid, first, count = np.unique(world, return_index=True, return_counts=True)
+        if int(count.max()) > MAX_WORLD_STATICS:
+            raise ValueError(f"a world has {int(count.max())} static collidables, at most {MAX_WORLD_STATICS}")
+        key = np.stack((world, ids), axis=1)
+        if len(np.unique(key, axis=0)) != n:
+            raise ValueError("static collidable ids must be distinct within a world")
+        stride = max(64, (n + 63) // 64 * 64)
+        f = np.zeros((N.NSF, stride), dtype=np.float64)
+        q = np.zeros((N.NSI, stride), dtype=np.int64)
+        boxes = 16
+        while True:
+            keys = np.zeros((3 * boxes, stride), dtype=np.int64)
+            most = N.cpu().xb_cpu_statics_setup(n, kind.ctypes.data, ids.ctypes.data, par.ctypes.data,
+                                                  geom.ctypes.data, f.ctypes.data, q.ctypes.data, keys.ctypes.data,
+                                                  stride, boxes)
+            if most == -2:
+                raise ValueError("Capsule::InsertInBoxes never ends (twice the radius is below half a float ulp of the "
+                                 "length left; Capsule.cpp:499, 578): Destiny's result is undefined")
+            if most < 0:
+                raise ValueError("xb_cpu_statics_setup refused the static collidables")
+            if most <= boxes:
+                break
+            boxes = most
+        t = lambda a: torch.from_numpy(a).to(self.dev)
+        self._st = {"f": t(f), "q": t(q), "keys": t(keys), "stride": stride, "boxes": boxes, "wid": wid,
+                    "first": first.astype(np.int32), "count": count.astype(np.int32), "most": int(count.max()),
+                    "range": None}
+        self._collide_setup()
+
+    def _st_set_range(self) -> None:
+        """Each row's world's first static collidable in the table and how many."""
+        w = self.world.cpu().numpy()
+        st = self._st
+        at = np.clip(np.searchsorted(st["wid"], w), 0, len(st["wid"]) - 1)
+        has = st["wid"][at] == w
+        rng = np.zeros((2, self.stride), dtype=np.int32)
+        rng[0, :self.n] = np.where(has, st["first"][at], 0)
+        rng[1, :self.n] = np.where(has, st["count"][at], 0)
+        st["range"] = torch.from_numpy(rng).to(self.dev)
+        self._state = None
+
+    def static_boxes(self, world: int, k: int) -> np.ndarray:
+        """[boxes, 3]: the keys world `world`'s k-th static collidable is filed in (at its level), sorted."""
+        i = int(self._st["first"][int(np.searchsorted(self._st["wid"], int(world)))]) + int(k)
+        nb = int(self._st["q"][4, i])
+        keys = self._st["keys"][:3 * nb, i].cpu().numpy().reshape(nb, 3)
+        return keys[np.lexsort(keys.T[::-1])]
 
     def _refile(self, mask: torch.Tensor | None) -> None:
         """InsertBallInBoxes for the rows in mask (None: every row): their filing from their current state."""

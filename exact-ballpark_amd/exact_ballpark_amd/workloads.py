@@ -89,14 +89,50 @@ def turning_park(seed: int, worlds: int, ships: int = 4) -> dict:
     return {"n": n, "worlds": worlds, "ships": ships, "rows": rows, "cmds": cmds, "ypr": ypr, "dynamical": True}
 
 
-def collide_park(seed: int, worlds: int, ships: int = 4, spheres: int = 4, dynamical: bool = False) -> dict:
+def _frames(rng: np.random.Generator, n: int) -> np.ndarray:
+    """[n, 3, 3]: n random orthonormal frames (rows the axes), from the QR of Gaussian matrices."""
+    q, r = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    return q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+
+
+def static_collidables(rng: np.random.Generator, centre: np.ndarray, per_world: int) -> dict:
+    """`per_world` static collidables a world around each world's centre ([W, 3]), capsules and oriented boxes in
+    turn: a capsule's axis of length U(200, 3000) m, radius U(30, 400) m; a box's half-extents U(30, 800) m; each in a
+    random orientation, its centre within 2 km of the world's. Ids 1001.., parent ids 0 or 1 (so a box's walk orders
+    them by parent, then id). As load_statics_arrays takes them."""
+    W = len(centre)
+    S = W * per_world
+    world = np.repeat(np.arange(W, dtype=np.int64), per_world)
+    j = np.tile(np.arange(per_world, dtype=np.int64), W)
+    kind = (j % 2).astype(np.int64)
+    mid = np.repeat(centre, per_world, axis=0) + rng.uniform(-2000.0, 2000.0, (S, 3))
+    fr = _frames(rng, S)
+    length = rng.uniform(200.0, 3000.0, S)
+    radius = rng.uniform(30.0, 400.0, S)
+    half = rng.uniform(30.0, 800.0, (S, 3))
+    geom = np.zeros((S, 12))
+    a = mid - 0.5 * length[:, None] * fr[:, 0]
+    b = mid + 0.5 * length[:, None] * fr[:, 0]
+    cap = kind == 0
+    geom[cap, 0:3], geom[cap, 3:6], geom[cap, 6] = a[cap], b[cap], radius[cap]
+    edges = 2.0 * half[:, :, None] * fr                           # [S, 3 edges, 3]
+    corner = mid - 0.5 * edges.sum(axis=1)
+    box = ~cap
+    geom[box, 0:3] = corner[box]
+    geom[box, 3:12] = edges[box].reshape(-1, 9)
+    return {"world": world, "kind": kind, "id": 1001 + j, "parent": rng.integers(0, 2, S).astype(np.int64),
+            "geom": geom}
+
+
+def collide_park(seed: int, worlds: int, ships: int = 4, spheres: int = 4, dynamical: bool = False,
+                 statics: int = 0) -> dict:
     """The turning-and-collisions park's bodies (the gradient method): each world's ships in a 2 km cluster around a
     centre uniform in a +/-1e6 m cube, at rest, mass U(1e6, 1e8) kg, agility U(0.1, 5), maximum speed U(100, 3000)
     m/s, radius U(20, 500) m, massive; and `spheres` fixed, massive spheres of radius U(100, 5000) m in a 4 km cluster
     around the same centre. At tick 1 each ship goes to a point of its world's cluster (GotoPoint), so ships cross,
     meet each other and hit or start inside the spheres. With `dynamical`, the turning values and rotations of
     turning_park (maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), RotationQuaternion(yaw, pitch,
-    roll) of angles U(-pi, pi))."""
+    roll) of angles U(-pi, pi)). With `statics`, that many static collidables a world (static_collidables)."""
     rng = np.random.default_rng(seed)
     per = ships + spheres
     n = worlds * per
@@ -125,8 +161,9 @@ def collide_park(seed: int, worlds: int, ships: int = 4, spheres: int = 4, dynam
     cmds = {"op": np.where(ship, GOTO_OP, 0).astype(np.int64), "point": centre + rng.uniform(-1000.0, 1000.0, (n, 3)),
             "target": np.arange(n, dtype=np.int64), "range": np.zeros(n), "min_range": np.full(n, 20000.0),
             "warp_factor": np.full(n, 20, dtype=np.int64)}
+    st = static_collidables(rng, centre[::per], statics) if statics else None
     return {"n": n, "worlds": worlds, "ships": per, "rows": rows, "cmds": cmds, "ypr": ypr, "dynamical": dynamical,
-            "collisions": True}
+            "collisions": True, "statics": st}
 
 
 def load_flight_park(park, wl: dict) -> None:
@@ -145,6 +182,8 @@ def load_flight_park(park, wl: dict) -> None:
                      max_ang_vel=t(r["max_ang_vel"]) if "max_ang_vel" in r else None,
                      rot_agility=t(r["rot_agility"]) if "rot_agility" in r else None,
                      junk=t(r["junk"], torch.bool) if "junk" in r else None)
+    if wl.get("statics") is not None:
+        park.load_statics_arrays(**wl["statics"])
     if wl.get("ypr") is not None:
         park.set_rotation_ypr(torch.ones(wl["n"], dtype=torch.bool, device=dev), t(wl["ypr"]))
     apply_commands(park, wl["cmds"])
@@ -203,9 +242,21 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
             cmds.append((1, o, int(r["id"][k]), tgt, p[0], p[1], p[2],
                          float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
                          int(c["warp_factor"][k])))
+    statics = []
+    st = wl.get("statics")
+    if st is not None:
+        wid = int(r["world"][rows[0]])
+        for k in np.flatnonzero(st["world"] == wid):
+            g = [float(v) for v in st["geom"][k]]
+            if int(st["kind"][k]) == 0:
+                statics.append({"kind": "capsule", "id": int(st["id"][k]), "parent": int(st["parent"][k]),
+                                "a": g[0:3], "b": g[3:6], "radius": g[6]})
+            else:
+                statics.append({"kind": "box", "id": int(st["id"][k]), "parent": int(st["parent"][k]),
+                                "corner": g[0:3], "lx": g[3:6], "ly": g[6:9], "lz": g[9:12]})
     return {"kind": "flight", "friction": 1.0e6, "tick": 1000, "steps": steps, "new_orbit": False, "start": 0,
             "balls": balls, "cmds": cmds, "dynamical": bool(wl.get("dynamical", False)),
-            "collisions": bool(wl.get("collisions", False))}
+            "collisions": bool(wl.get("collisions", False)), "statics": statics}
 
 
 def select_worlds(wl: dict, order) -> dict:
@@ -219,7 +270,12 @@ def select_worlds(wl: dict, order) -> dict:
     r = {k: v[rows] for k, v in wl["rows"].items()}
     c = {k: v[rows] for k, v in wl["cmds"].items()}
     c["target"] = new_row[c["target"]]
-    return {"n": int(rows.size), "worlds": int(order.size), "ships": s, "rows": r, "cmds": c}
+    out = {"n": int(rows.size), "worlds": int(order.size), "ships": s, "rows": r, "cmds": c}
+    st = wl.get("statics")
+    if st is not None:                                  # the selected worlds' static collidables (by world id)
+        keep = np.isin(st["world"], np.unique(r["world"]))
+        out["statics"] = {k: v[keep] for k, v in st.items()}
+    return out
 
 
 def truncated_geometric_p(mean: float, top: int) -> float:

@@ -221,6 +221,8 @@ def run_world(w: dict, record_every_step: bool = True):
               w.get("collisions", False))
     for b in w["balls"]:
         o.add(b)
+    for st in w.get("statics", []):
+        o.add_static(st)
     for step in range(1, w["steps"] + 1):
         for c in w["cmds"]:
             if c[0] == step:
@@ -252,6 +254,8 @@ def time_worlds(worlds: list, ticks: int) -> tuple:
                   w.get("collisions", False))
         for b in w["balls"]:
             o.add(b)
+        for st in w.get("statics", []):
+            o.add_static(st)
         for c in w["cmds"]:
             if c[0] == 1:
                 o.command(c)

@@ -134,6 +134,9 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                     "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
                     "moribund": b["moribund"], "mode": b["mode"], "max_ang_vel": b.get("max_ang_vel", 1.0),
                     "rot_agility": b.get("rot_agility", 0.9), "junk": b.get("junk", False)} for wi, b in balls])
+        statics = {wi: worlds[wi]["statics"] for wi in members if worlds[wi].get("statics")}
+        if statics:
+            park.load_statics(statics)
         if general:
             park.tiles = None
             park._state = None
@@ -194,6 +197,8 @@ def _run_flags(w: dict, initial: bool = False):
     o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], dyn, coll, ypr)
     for b in w["balls"]:
         o.add(b)
+    for s in w.get("statics", []):
+        o.add_static(s)
     by_step = defaultdict(list)
     for c in w["cmds"]:
         by_step[c[0]].append(c)
