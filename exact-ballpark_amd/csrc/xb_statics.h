@@ -484,6 +484,16 @@ static inline XB_HD int64_t xb_static_key(const XbStatics& S, int64_t k, const X
     return best;
 }
 
+/* Whether static collidable k comes before static collidable l in the ball's walk: by walk key, then parent id, then
+   id. */
+static inline XB_HD bool xb_static_before(const XbStatics& S, int64_t k, int64_t l, const XbFiling& mf) {
+    const int64_t kk = xb_static_key(S, k, mf), kl = xb_static_key(S, l, mf);
+    if (kk != kl) return kk < kl;
+    const int64_t pk = XB_SQ(S, XB_SI_PARENT, k), pl = XB_SQ(S, XB_SI_PARENT, l);
+    if (pk != pl) return pk < pl;
+    return XB_SQ(S, XB_SI_ID, k) < XB_SQ(S, XB_SI_ID, l);
+}
+
 /* Gradient's static collidables for a free ball (`me` after its Potential with the balls), its world's statics
    [first, first + count): the responses found, then applied in (walk key, parent id, id) order. */
 static inline XB_HD void xb_statics_gradient(XbCollMe& me, const XbFiling& mf, const XbStatics& S, int64_t first,
@@ -498,15 +508,15 @@ static inline XB_HD void xb_statics_gradient(XbCollMe& me, const XbFiling& mf, c
             resp |= 1ULL << i;
     }
     while (resp) {
-        int pick = -1;
-        int64_t pk = 0, pp = 0, pid = 0;
-        for (uint64_t a = resp; a; a &= a - 1) {
+        /* Only the index is carried through the search; the tie-breaks are read again from the table. (Carrying the
+           key, parent and id beside the index in one conditional update was compiled for gfx950 so that the index
+           kept an earlier candidate while the id moved on.) */
+        int pick = __builtin_ctzll(resp);
+        for (uint64_t a = resp & (resp - 1); a; a &= a - 1) {
             const int i = __builtin_ctzll(a);
-            const int64_t key = xb_static_key(S, first + i, mf);
-            const int64_t par = XB_SQ(S, XB_SI_PARENT, first + i), id = XB_SQ(S, XB_SI_ID, first + i);
-            if (pick < 0 || key < pk || (key == pk && (par < pp || (par == pp && id < pid))))
-                pick = i, pk = key, pp = par, pid = id;
+            if (xb_static_before(S, first + i, first + pick, mf)) pick = i;
         }
+        const int64_t pid = XB_SQ(S, XB_SI_ID, first + pick);
         resp &= ~(1ULL << pick);
         XbStaticResp r;
         xb_static_response(S, first + pick, me, p0, p1, vp1, P, &r);

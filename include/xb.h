@@ -7,8 +7,8 @@
  * Return codes: 0 success; -1 refused before any launch (a bad block size, size or argument; see xb_error_string);
  * a positive value is the hipError_t of the launch.
  *
- * Each entry point names the reference interface it replaces (the ctypes-bound functions of the reference's
- * src/exact_ballpark/*_cuda.cu); the reference-side bindings are in INTEGRATION.md.
+ * Each entry point names the reference interface it replaces (the ctypes-bound functions of the reference's CUDA
+ * sources, src/exact_ballpark/vec_*_cuda.cu) and where it is declared there.
  */
 #ifndef XB_H
 #define XB_H

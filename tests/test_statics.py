@@ -34,8 +34,9 @@ def compare_golden(got, golden, kept, do=False):
             same_q = gq[:, Q_GOLDEN] == wq
             hits = [h for _, h in gl]
             if not (same_d.all() and same_q.all() and hits == wl):
-                cols = sorted(set(np.nonzero(~same_d)[1].tolist()))
-                bad.append((wi, w["kind"], step, cols[:6], hits != wl))
+                diff = sorted(set(np.nonzero(~same_d)[1].tolist()))
+                qdiff = sorted(set(np.nonzero(~same_q)[1].tolist()))
+                bad.append((wi, w["kind"], step, diff[:6], qdiff[:4], hits if hits != wl else None, wl))
     return bad
 
 
