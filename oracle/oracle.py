@@ -16,6 +16,7 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
 VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, YPR_OP = 8, 9, 10, 11, 12, 13
+LAUNCH_OP, MFOLLOW_OP = 14, 15
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -73,6 +74,8 @@ def lib() -> ctypes.CDLL:
         L.or_add_capsule.argtypes = [vp, i64, i64, pd, pd, d]
         L.or_add_oriented_box.argtypes = [vp, i64, i64, pd, pd, pd, pd]
         L.or_get_static_boxes.argtypes = [vp, i, pq]
+        L.or_launch_missile.argtypes = [vp, i64, i64, i64, i, i]
+        L.or_missile_follow.argtypes = [vp, i64, i64, i64]
         _LIB = L
     return _LIB
 
@@ -160,6 +163,10 @@ class World:
             L.or_set_rot_agility(p, a, rr)
         elif op == YPR_OP:
             L.or_set_rotation_ypr(p, a, x, y, z)
+        elif op == LAUNCH_OP:
+            L.or_launch_missile(p, a, b, w, int(x), int(y))
+        elif op == MFOLLOW_OP:
+            L.or_missile_follow(p, a, b, w)
         else:
             raise ValueError(op)
 

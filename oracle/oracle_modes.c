@@ -85,6 +85,7 @@ typedef struct {                               /* a capsule or an oriented box (
 
 typedef struct {
     double friction, dt;
+    int tick_ms;                               /* the tick interval, milliseconds */
     int64_t current_time;
     int use_new_orbit;
     int dynamical;                             /* FullBallpark's dynamical_orientation */
@@ -157,6 +158,7 @@ OPark* or_park_new(double friction, int tick_interval_ms, int use_new_orbit, int
     OPark* p = (OPark*)calloc(1, sizeof(OPark));
     p->friction = friction;
     p->dt = tick_interval_ms * 0.001;
+    p->tick_ms = tick_interval_ms;
     p->use_new_orbit = use_new_orbit;
     p->current_time = current_time;
     return p;
@@ -219,6 +221,7 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
    for + - * / and sqrt of floats that is the float operation). Vectors, quaternions and matrices are carbonengine's
    (full.py:179-325). The float transcendentals are glibc's (the product's maths headers, pinned on their own). */
 
+static double ieee_div(double a, double b) { return a / b; }           /* modes.py:99-105: C's division */
 static double F(double x) { return (double)(float)x; }
 static double fdiv_(double a, double b) { return F(a / b); }
 static double fmax_(double a, double b) { return a < b ? b : a; }                 /* std::max */
@@ -847,6 +850,127 @@ void or_entity_warp_in(OPark* p, int64_t id, double x, double y, double z, int64
 
 /* --------------------------------------------------------------------------------------------- behaviours */
 
+/* GotoThrust toward `target` (full.py:2320-2331): tapered within a tick's distance unless a missile's */
+static void goto_thrust_to(const OPark* p, const OBall* b, const double target[3], int missile, double out[3]) {
+    double a[3] = {target[0] - b->pos[0], target[1] - b->pos[1], target[2] - b->pos[2]};
+    double length2 = lensq(a);
+    double dist = b->sf * b->max_vel * p->dt;
+    double max_thrust = p->friction * b->sf * b->max_vel / (b->mass * b->agility);
+    double unit[3];
+    normalized(a, unit);
+    double scale = max_thrust;
+    if (!missile && length2 < dist * dist) {
+        double coff = length2 / (dist * dist);
+        scale = max_thrust * coff * coff;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = unit[i] * scale;
+}
+
+#define MISSILE_STRAIGHT_MS 800                                   /* full.py:84 */
+#define MISSILE_LAUNCH_SPEED 150.0                                /* full.py:85 */
+
+/* EvolveMissile (full.py:2337-2356): straight on for 800 ms, then at full thrust toward the target's position
+   predicted dist / maxVel ahead, offset by the follow range */
+static void evolve_missile(OPark* p, OBall* b, double out[3]) {
+    if ((p->current_time - b->effect_stamp) * p->tick_ms <= MISSILE_STRAIGHT_MS) {
+        goto_thrust_to(p, b, b->gp, 0, out);
+        return;
+    }
+    OBall* o = find(p, b->follow_ptr);
+    double op[3] = {o->pos[0], o->pos[1], o->pos[2]}, delta[3], target[3];
+    for (int i = 0; i < 3; ++i) delta[i] = b->pos[i] - op[i];
+    double dist = length(delta);
+    double ct = ieee_div(dist, b->max_vel);
+    for (int i = 0; i < 3; ++i) target[i] = op[i] + o->vel[i] * ct;
+    double r = b->follow_range + b->radius + o->radius;
+    if (dist == 0.0) {
+        target[0] = op[0] + 1.0 * r, target[1] = op[1] + 0.0 * r, target[2] = op[2] + 0.0 * r;
+    } else {
+        double inv = ieee_div(1.0, dist);
+        for (int i = 0; i < 3; ++i) target[i] = target[i] + (delta[i] * r) * inv;
+    }
+    b->gp[0] = target[0], b->gp[1] = target[1], b->gp[2] = target[2];
+    goto_thrust_to(p, b, target, 1, out);
+}
+
+/* Ball::GetRotatedVector (full.py:2112-2117): by the rotation with dynamical orientation, else by RotationQuaternion of
+   the display yaw, pitch and roll (floats) */
+static void rotated_vector(const OPark* p, const OBall* b, const double v[3], double out[3]) {
+    double q[4], m[4][4];
+    if (p->dynamical) {
+        memcpy(q, b->rot, sizeof(q));
+    } else {
+        double yaw = F(b->yaw), pitch = F(b->pitch), roll = F(b->roll);
+        double halfpitch = F(pitch * 0.5), cp = cosf_(halfpitch), sp = sinf_(halfpitch);
+        double halfyaw = F(yaw * 0.5), cy = cosf_(halfyaw), sy = sinf_(halfyaw);
+        double halfroll = F(roll * 0.5), cr = cosf_(halfroll), sr = sinf_(halfroll);
+        q[0] = F(F(F(cr * sp) * cy) + F(F(sr * cp) * sy)), q[1] = F(F(F(cr * cp) * sy) - F(F(sr * sp) * cy));
+        q[2] = F(F(F(sr * cp) * cy) - F(F(cr * sp) * sy)), q[3] = F(F(F(cr * cp) * cy) + F(F(sr * sp) * sy));
+    }
+    m_rotation_qf(q, m);
+    transform_coordf(v, m, out);
+}
+
+/* MissileFollow (full.py:1679-1706): stopped, then aimed at the target (follow range minus both radii, a float),
+   straight on for 800 ms toward 1e16 m along its velocity, at full speed fraction */
+void or_missile_follow(OPark* p, int64_t src, int64_t dst, int64_t owner) {
+    if (dst < 0 || src == dst || src == owner || dst == owner) return;
+    OBall *m = find(p, src), *t = find(p, dst);
+    if (!m || !t || t->is_moribund) return;
+    stop_ball(p, m);
+    if (m->bubble != t->bubble) return;
+    if (t->cloaked) return;
+    m->follow_id = dst;
+    m->follow_ptr = dst;
+    m->follow_range = -F(m->radius + t->radius);
+    m->owner_id = owner;
+    m->effect_stamp = p->current_time;
+    double u[3];
+    normalized(m->vel, u);
+    for (int i = 0; i < 3; ++i) m->gp[i] = u[i] * 1.e16;
+    m->sf = 1.0;
+    m->mode = MISSILE;
+    fol_add(t, m->id);
+}
+
+/* LaunchMissile (full.py:1708-1736): from the launcher (|owner|), aimed at the target or along the launcher's velocity
+   (its heading backwards when at rest) at max(150, its speed); massive if asked; SetBallPosition (re-filed),
+   SetBallVelocity (snapped); then MissileFollow, or GotoDirection without a target */
+void or_launch_missile(OPark* p, int64_t src, int64_t dst, int64_t owner, int aimed, int massive) {
+    if (dst < 0) return;
+    OBall* l = find(p, owner < 0 ? -owner : owner);
+    if (!l) return;
+    OBall* t = dst != owner ? find(p, dst) : NULL;
+    double max_velocity = 0.0, ps[3] = {l->pos[0], l->pos[1], l->pos[2]}, v0[3];
+    if (t && aimed == 1 && !t->cloaked) {
+        double d[3] = {t->pos[0] - ps[0], t->pos[1] - ps[1], t->pos[2] - ps[2]};
+        normalized(d, v0);
+    } else {
+        double vs[3] = {l->vel[0], l->vel[1], l->vel[2]}, dir[3];
+        max_velocity = l->max_vel;
+        normalized(vs, dir);
+        if (lensq(dir) == 0.0) {
+            const double back[3] = {0.0, 0.0, -1.0};
+            rotated_vector(p, l, back, dir);
+        }
+        double sp = fmax_(MISSILE_LAUNCH_SPEED, max_velocity);
+        for (int i = 0; i < 3; ++i) v0[i] = vs[i] + dir[i] * sp;
+    }
+    OBall* m = find(p, src);
+    if (m && massive) m->is_massive = 1;
+    if (m) {                                                            /* SetBallPosition (space.py:1207-1214) */
+        for (int i = 0; i < 3; ++i) m->pos[i] = m->old_pos[i] = ps[i];
+        if (p->collisions) file_ball(p, m);
+    }
+    or_set_velocity(p, src, v0[0], v0[1], v0[2]);
+    if (t && t->cloaked) return;
+    if (!t) {
+        if (max_velocity > 0.0 && m) goto_direction_ball(p, m, v0);
+        return;
+    }
+    or_missile_follow(p, src, dst, owner);
+}
+
 static void goto_thrust(const OPark* p, const OBall* b, double out[3]) {   /* ref.py:151-160 */
     double a[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]};
     double length2 = lensq(a);
@@ -1064,6 +1188,8 @@ static void behaviour(OPark* p, OBall* b) {                              /* mode
         }
     } else if (m == GOTO) {
         goto_thrust(p, b, a);
+    } else if (m == MISSILE) {
+        evolve_missile(p, b, a);
     } else if (m == FOLLOW) {
         evolve_follow(p, b, a);
     } else if (m == ORBIT) {
@@ -1166,7 +1292,7 @@ static int64_t to_int64_(double x) {                                    /* space
 }
 static int64_t clamp_key(int64_t k, int64_t n) { return k < 0 ? 0 : k >= n ? n - 1 : k; }
 
-static double ieee_div(double a, double b) { return a / b; }           /* modes.py:99-105: C's division */
+
 
 static double inflated_radius(const OPark* p, const OBall* b) {        /* space.py:710-719 */
     if (!b->is_free) return b->radius;
@@ -1323,7 +1449,7 @@ static void explode_children(const OPark* p, const OBoxes* U, OBox x, int* out, 
    balls of those boxes by id, each once, and their static collidables by (parent id, id), each once. No ball here is a
    force field (the harmonic filter passes every ball). Returns the ball candidates' indices in order; the static
    candidates' indices go to `sout` (their count to *nsout). */
-static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out, int* sout, int* nsout) {
+static int nearby_balls(const OPark* p, OBoxes* U, int bi, int exclude_missiles, int* out, int* sout, int* nsout) {
     const OBall* b = &p->balls[bi];
     boxes_build(p, U);
     OBox bx[8];
@@ -1360,7 +1486,7 @@ static int nearby_balls(const OPark* p, OBoxes* U, int bi, int* out, int* sout, 
             if (!ball_in_box(o, x) || handled[i] || i == bi) continue;
             if (o->is_moribund) continue;
             if (o->cloaked || !o->is_massive) continue;
-            if (o->mode == MISSILE) continue;
+            if (exclude_missiles && o->mode == MISSILE) continue;
             handled[i] = 1;
             out[cnt++] = i;
         }
@@ -1515,14 +1641,22 @@ static void potential(OPark* p, OBall* me, OBall* other, int depth) {   /* space
 
 static OBoxes g_boxes;                                                  /* the walk's work space (one thread) */
 
-/* Gradient (space.py:1650-1661; with the static collidables full.py:2628-2644) */
+/* Gradient (space.py:1650-1661; with the static collidables and the missiles full.py:1194-1211, 2628-2644): a normal
+   missile's one candidate is its target (unfiltered, in its bubble); a defender missile (negative owner) sees
+   GetNearbyBalls with the missiles kept; every other ball GetNearbyBalls without missiles. */
 static void gradient(OPark* p, int bi) {
     OBall* b = &p->balls[bi];
-    int cand[MAX_BALLS], scand[MAX_STATICS], nsc = 0;
-    int nc = nearby_balls(p, &g_boxes, bi, cand, scand, &nsc);
+    int cand[MAX_BALLS], scand[MAX_STATICS], nsc = 0, nc = 0;
+    if (b->mode == MISSILE && b->owner_id >= 0) {
+        OBall* t = find(p, b->follow_ptr);
+        if (t && t->bubble == b->bubble) cand[nc++] = (int)(t - p->balls);
+    } else {
+        nc = nearby_balls(p, &g_boxes, bi, b->mode != MISSILE, cand, scand, &nsc);
+    }
     for (int c = 0; c < nc; ++c) {
         OBall* n = &p->balls[cand[c]];
-        if (((n->mode == MISSILE || n->mode == MUSHROOM) && b->id == n->owner_id) ||
+        if ((b->mode == MISSILE && (n->id == b->owner_id || (n->mode == MINIBALL && n->owner_id == b->owner_id))) ||
+            ((n->mode == MISSILE || n->mode == MUSHROOM) && b->id == n->owner_id) ||
             (n->mode == MISSILE && b->id == n->follow_id) || (n->is_junk && !b->is_junk))
             continue;
         potential(p, b, n, 0);

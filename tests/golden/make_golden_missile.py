@@ -1,0 +1,113 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/missile_golden.npz: the reference's own scalar ballpark with missiles
+(src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
+gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
+ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
+mLastCollision, the filing, the collision list in order and the display yaw, pitch and roll. With `--do` it writes
+missile_do_golden.npz: the same worlds with dynamical orientation (`do_missile_cases`), the turning state in place of
+the yaw, pitch and roll.
+
+Run only where the reference is mounted (/root/reference); the fixture it writes is committed and nothing else reads
+the reference. The setup of make_golden_collide.py and make_golden_ypr.py is reused: the reference's glibc 2.34
+aarch64 transcriptions in place of the C library, the math.fma shim.
+"""
+
+from __future__ import annotations
+
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+import make_golden_ypr as Y  # noqa: E402  (installs the redirections)
+
+C = Y.C
+S = C.S
+full = C.full
+
+
+def _apply(park, c) -> None:
+    _, op, a, b, x, y, z, rr, w = c
+    if op == S.LAUNCH_OP:
+        park.launch_missile(a, b, w, int(x), int(y))
+    elif op == S.MFOLLOW_OP:
+        park.missile_follow(a, b, w)
+    else:
+        C._apply(park, c)
+
+
+def _state(b, do: bool) -> tuple:
+    return C._state(b, True) if do else Y._state(b)
+
+
+def run_world(w: dict) -> list:
+    do = bool(w.get("dynamical"))
+    park = full.FullBallpark(friction=w["friction"], tick_interval_ms=w["tick"], use_new_orbit=w["new_orbit"],
+                             dynamical_orientation=do)
+    park.current_time = w["start"]
+    for b in w["balls"]:
+        kw = {"max_ang_vel": b["max_ang_vel"], "rot_agility": b["rot_agility"]} if do else {}
+        park.add_ball(b["id"], b["mass"], b["radius"], b["max_vel"], is_free=b["free"], is_massive=b["massive"],
+                      is_interactive=b["interactive"], is_space_junk=b.get("junk", False), x=b["pos"][0],
+                      y=b["pos"][1], z=b["pos"][2], vx=b["vel"][0], vy=b["vel"][1], vz=b["vel"][2],
+                      agility=b["agility"], speed_fraction=b["sf"], **kw)
+        if b["cloaked"]:
+            park.set_cloaked(b["id"], b["cloaked"])
+        park.balls[b["id"]].bubble = b["bubble"]
+    order = sorted(b["id"] for b in w["balls"])
+    out = []
+    for k, bid in enumerate(order):
+        d, q = _state(park.balls[bid], do)
+        out.append((0, k, d, q, list(park.balls[bid].collisions)))
+    for step in range(1, w["steps"] + 1):
+        for c in w["cmds"]:
+            if c[0] == step:
+                _apply(park, c)
+        park.evolve()
+        for k, bid in enumerate(order):
+            d, q = _state(park.balls[bid], do)
+            out.append((step, k, d, q, list(park.balls[bid].collisions)))
+    return out
+
+
+def main(seed: int = 1, do: bool = False) -> None:
+    name = "missile_do_golden.npz" if do else "missile_golden.npz"
+    ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
+    if not do:
+        for w in ws:
+            w["ypr"] = True
+    kept, skipped = [], []
+    rec_world, rec_step, rec_ball, rec_d, rec_q, hit_off, hit_ids = [], [], [], [], [], [0], []
+    for wi, w in enumerate(ws):
+        try:
+            rows = run_world(w)
+        except (ZeroDivisionError, ValueError, OverflowError, C.space.DestinyUndefined, NotImplementedError) as e:
+            skipped.append([wi, type(e).__name__, str(e)[:80]])
+            continue
+        kept.append(wi)
+        for step, k, d, q, hits in rows:
+            rec_world.append(wi)
+            rec_step.append(step)
+            rec_ball.append(k)
+            rec_d.append(d)
+            rec_q.append(q)
+            hit_ids.extend(hits)
+            hit_off.append(len(hit_ids))
+    d = np.array(rec_d, dtype=np.float64).view(np.uint64)
+    names = C.DOUBLES + (C.TURNING if do else Y.YPR)
+    np.savez_compressed(HERE / name, seed=np.int64(seed), worlds=np.array(json.dumps(ws)),
+                        kept=np.array(kept, dtype=np.int64), skipped=np.array(json.dumps(skipped)),
+                        world=np.array(rec_world, dtype=np.int32), step=np.array(rec_step, dtype=np.int32),
+                        ball=np.array(rec_ball, dtype=np.int32), d=d, q=np.array(rec_q, dtype=np.int64),
+                        hit_off=np.array(hit_off, dtype=np.int64), hit_ids=np.array(hit_ids, dtype=np.int64),
+                        doubles=np.array(names), ints=np.array(C.INTS))
+    n_hits = sum(1 for i in range(1, len(hit_off)) if hit_off[i] > hit_off[i - 1])
+    print(f"{name}: {len(kept)} worlds kept, {len(skipped)} skipped {skipped[:5]}, {len(rec_d)} records, "
+          f"{n_hits} with collisions", file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main(int(sys.argv[1]) if len(sys.argv) > 1 and sys.argv[1] != "--do" else 1, do="--do" in sys.argv)

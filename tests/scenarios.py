@@ -828,3 +828,149 @@ def static_cases(seed: int, scale: int = 1) -> list:
             cmds.append(go(bid, [c[q] + r.uniform(-3000.0, 3000.0) for q in range(3)], r.choice([1, 2])))
         world("faces", balls, statics, cmds, r.randint(8, 30), tick=ticks())
     return out
+
+
+LAUNCH_OP = 14                  # LaunchMissile(missile a, target b, owner w, aimed x, massive y)
+MFOLLOW_OP = 15                 # MissileFollow(missile a, target b, owner w)
+
+
+def _launch(step, missile, target, owner, aimed, massive) -> tuple:
+    return _cmd(step, LAUNCH_OP, missile, target, x=float(aimed), y=float(massive), w=owner)
+
+
+def missile_cases(seed: int, scale: int = 1) -> list:
+    """The missile worlds (about 150 x scale) for seed, with collisions: ships in a cluster (some flying GOTO, some at
+    rest, some fixed spheres among them) and missile balls launched at them (LaunchMissile: aimed or along the
+    launcher's velocity, from a launcher at rest along its heading backwards; massive or not; a defender's negative
+    owner; no target, a cloaked target, a target in another bubble, the owner as target), straight on for 800 ms then in
+    pursuit, hitting their targets (the response to its own target not kept, its id listed) and whatever is in the way;
+    targets warping away (the missile released, flying on non-massive); missiles at missiles; MissileFollow given
+    directly; tick intervals from 250 ms to 2 s (300, 400 and 800 among them: the straight flight's end on a tick and
+    between ticks)."""
+    r = random.Random(seed ^ 0x3155113)
+    out: list = []
+
+    def world(kind, balls, cmds, steps, *, tick=1000, friction=1.0e6):
+        cmds.sort(key=lambda c: c[0])
+        out.append({"kind": kind, "friction": friction, "tick": tick, "steps": steps, "new_orbit": False,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": cmds, "collisions": True})
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    def missile_ball(bid, c, **kw):
+        b = _coll_ball(r, bid, c, 3000.0, radius=r.uniform(1.0, 20.0), mass=10 ** r.uniform(3.0, 5.0),
+                       max_vel=r.uniform(500.0, 3000.0), agility=r.uniform(0.2, 2.0), sf=1.0,
+                       massive=r.random() < 0.3, vel=[0.0, 0.0, 0.0])
+        b.update(kw)
+        return b
+
+    for _ in range(110 * scale):
+        c = centre()
+        n_ships = r.randint(2, 6)
+        n_mis = r.randint(1, 4)
+        ids = _ids(r, n_ships + n_mis, big=r.random() < 0.2)
+        ships, missiles = ids[:n_ships], ids[n_ships:]
+        balls, cmds = [], []
+        spread = r.choice([2000.0, 8000.0, 20000.0])
+        for k, bid in enumerate(ships):
+            if k >= 2 and r.random() < 0.2:
+                balls.append(_sphere(r, bid, c, spread))
+                continue
+            b = _coll_ball(r, bid, c, spread, max_vel=r.uniform(50.0, 400.0))
+            balls.append(b)
+            if r.random() < 0.6:
+                p = [c[i] + r.uniform(-spread, spread) for i in range(3)]
+                cmds.append(_cmd(1, GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        for bid in missiles:
+            balls.append(missile_ball(bid, c))
+        free_ships = [b["id"] for b in balls if b["id"] in ships and b["free"]]
+        kinds = []
+        for m in missiles:
+            launcher = r.choice(free_ships or ships)
+            x = r.random()
+            if x < 0.55:
+                target = r.choice([s for s in ships if s != launcher] or ships)
+                kind = "launch"
+            elif x < 0.65:
+                target = r.choice([mm for mm in missiles if mm != m] or ships)
+                kind = "at_missile"
+            elif x < 0.72:
+                target = launcher
+                kind = "no_target"
+            elif x < 0.77:
+                target = -1
+                kind = "negative_target"
+            else:
+                target = r.choice(ships)
+                kind = "launch"
+            owner = -launcher if r.random() < 0.25 else launcher
+            if owner < 0:
+                kind = "defender"
+            aimed = 1 if r.random() < 0.5 else 0
+            massive = 1 if r.random() < 0.7 else 0
+            step = r.choice([1, 1, 2, 3, 5])
+            cmds.append(_launch(step, m, target, owner, aimed, massive))
+            kinds.append(kind)
+        x = r.random()
+        if x < 0.12:                                    # a target warps away: its missiles flying on
+            t = r.choice(ships)
+            p = [c[i] + r.uniform(-1e9, 1e9) for i in range(3)]
+            cmds.append(_cmd(r.randint(3, 8), WARP_OP, t, x=p[0], y=p[1], z=p[2], rr=20000.0, w=20))
+            kinds.append("warp_away")
+        elif x < 0.2:                                   # a cloaked target, or another bubble's
+            t = r.choice(ships)
+            b = next(bb for bb in balls if bb["id"] == t)
+            if r.random() < 0.5:
+                b["cloaked"] = 1
+                kinds.append("cloaked_target")
+            else:
+                b["bubble"] = 3
+                kinds.append("other_bubble")
+        elif x < 0.3:                                   # MissileFollow given directly to a moving missile
+            m = r.choice(missiles)
+            t = r.choice(ships)
+            cmds.append(_cmd(r.randint(2, 6), MFOLLOW_OP, m, t, w=r.choice(ships)))
+            kinds.append("missile_follow")
+        world("missile_" + (kinds[0] if kinds else "none"), balls, cmds, r.randint(10, 45),
+              tick=r.choice([1000, 1000, 500, 250, 2000, 300, 400, 800]),
+              friction=r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]))
+
+    for _ in range(25 * scale):                        # launchers at rest: along the heading backwards
+        c = centre()
+        ids = _ids(r, r.randint(3, 5), big=False)
+        launcher = ids[0]
+        balls = [_coll_ball(r, launcher, c, 0.0, vel=[0.0, 0.0, 0.0])]
+        balls.append(_coll_ball(r, ids[1], c, 5000.0))
+        cmds = []
+        for m in ids[2:]:
+            balls.append(missile_ball(m, c))
+            cmds.append(_launch(1, m, r.choice([ids[1], launcher]), launcher, 0, r.choice([0, 1])))
+        world("missile_still_launcher", balls, cmds, r.randint(8, 30))
+
+    for _ in range(15 * scale):                        # defenders among missiles
+        c = centre()
+        ids = _ids(r, 6, big=False)
+        a, b2 = ids[0], ids[1]
+        balls = [_coll_ball(r, a, c, 3000.0), _coll_ball(r, b2, c, 3000.0)]
+        cmds = []
+        for k, m in enumerate(ids[2:]):
+            balls.append(missile_ball(m, c, massive=True))
+            if k < 2:
+                cmds.append(_launch(1, m, b2, a, 1, 1))          # attackers at b2
+            else:
+                cmds.append(_launch(2, m, ids[2 + k - 2], -b2, 1, 1))   # b2's defenders at the attackers
+        world("missile_defenders", balls, cmds, r.randint(10, 30), tick=r.choice([1000, 250]))
+    return out
+
+
+def do_missile_cases(seed: int, scale: int = 1) -> list:
+    """missile_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
+    r = random.Random(seed ^ 0xD15)
+    out = missile_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+    return out
