@@ -2,9 +2,10 @@
 """Benchmark: bit-exact world-ticks/s of the batched destiny tick on one MI355X.
 
 Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
-- composed (the primary line): the ragged composed park without missiles or static collidables, 65,536 worlds with
-  massive ships per world geometric on 1..32 (mean 6), modes uniform over stop, goto, follow, orbit and warp given at
-  tick 1, dynamical orientation (rotations from yaw, pitch and roll) and the gradient collision method.
+- composed (the primary line): the ragged composed park, 65,536 worlds with massive ships per world geometric on
+  1..32 (mean 6) and 0..8 missiles a world (uniform) launched at tick 1 by a ship of the world at another, modes
+  uniform over stop, goto, follow, orbit and warp given at tick 1, dynamical orientation (rotations from yaw, pitch and
+  roll) and the gradient collision method (a missile's one candidate its target).
 - ragged: the same worlds as a flight-modes park (no orientation, no collisions).
 - profiles: the same ragged park with world activity profiles (quiet, travelling, in combat, mixed), the worlds
   differing in cost from one another as a server's do: the learned scheduler's workload.
@@ -62,9 +63,10 @@ WORKLOADS = {
     "turning": "turning park: {worlds} worlds x 4 ships with dynamical orientation, each world a 2 km cluster, "
                "rotations from yaw, pitch and roll uniform in +/-pi, every ship going to a point of its cluster "
                "(70%), orbiting (15%) or following (15%); no collisions",
-    "composed": "ragged composed park without missiles or static collidables: {worlds} worlds, ships per world "
-                "geometric on 1..32 (mean 6), massive, modes uniform over stop/goto/follow/orbit/warp, dynamical "
-                "orientation (rotations from yaw, pitch and roll uniform in +/-pi), gradient collisions",
+    "composed": "ragged composed park: {worlds} worlds, massive ships per world geometric on 1..32 (mean 6), 0-8 "
+                "missiles per world (uniform) launched at tick 1 at another ship (aimed half the time, massive, one "
+                "in ten a defender), modes uniform over stop/goto/follow/orbit/warp, dynamical orientation (rotations "
+                "from yaw, pitch and roll uniform in +/-pi), gradient collisions; no static collidables",
     "collide": "collision park: {worlds} worlds x (4 free massive ships in a 2 km cluster + 4 fixed massive spheres "
                "of radius 100-5000 m), every ship going to a point of its cluster; the gradient collision method "
                "(partition filing, walk-order contacts, Potential); no dynamical orientation or static collidables",
@@ -187,7 +189,7 @@ def main() -> int:
         elif kind == "statics":
             wl = WL.collide_park(1, a.worlds, 4, 4, dynamical=True, statics=8)
         elif kind == "composed":
-            wl = WL.composed_park(1, a.worlds)
+            wl = WL.composed_park(1, a.worlds, missiles=8)
         else:
             wl = WL.ragged_park(1, a.worlds, mix="profiles" if kind == "profiles" else "uniform")
         park = VecModesPark(wl["n"], device="cuda", block=256, dynamical_orientation=bool(wl.get("dynamical")),
@@ -281,13 +283,14 @@ def main() -> int:
             res["learned_scheduler"] = {"training": training, "orders": cmp,
                                         "gain_over_rows": cmp["rows"]["ms_per_tick"] / cmp["learned"]["ms_per_tick"]}
         if not a.no_padded and kind in ("ragged", "profiles", "composed"):
-            pad = WL.padded(wl, 32)
+            width = max(32, int(wl["counts"].max()))
+            pad = WL.padded(wl, width)
             pp = VecModesPark(pad["n"], device="cuda", block=256, dynamical_orientation=bool(pad.get("dynamical")),
                               collisions=bool(pad.get("collisions")))
             WL.load_flight_park(pp, pad)
             mp = _measure(pp, 1, max(32, a.steps // 4), 4)
-            res["padded_baseline"] = {"layout": "every world padded to 32 rows, lanes in row order, worlds in row "
-                                                "order, one tick a launch",
+            res["padded_baseline"] = {"layout": f"every world padded to {width} rows (32 ships, and the missiles), "
+                                                "lanes in row order, worlds in row order, one tick a launch",
                                       "world_ticks_per_s": a.worlds / (mp["ms_per_tick"] / 1000.0),
                                       "ms_per_tick": mp["ms_per_tick"],
                                       "speedup": mp["ms_per_tick"] / m["ms_per_tick"]}

@@ -204,7 +204,7 @@ static inline XB_HD double xb_collide_two_spheres(XbV3 p0, XbV3 p1, XbV3 q0, XbV
 struct XbCollMe {
     XbV3 pos, vel, g;              /* start of tick (after the behaviours), mLastG */
     double m, mass, tf, radius;    /* mass x agility, the base mass, the decay factor */
-    int64_t id, mode, follow_id;
+    int64_t id, mode, follow_id, owner_id;
     XbV3 last_c;
     double last_collision;
     int64_t nh;                    /* the collision list's length */
@@ -305,7 +305,7 @@ struct XbCollRecs {
     double *px, *py, *pz, *vx, *vy, *vz, *gx, *gy, *gz;   /* position, velocity, mLastG */
     double *ex, *ey, *ez;                                 /* the tick's straight prediction (depth 0) */
     double *mass, *m, *tf;                                /* base mass; mass x agility (1e34 if fixed); decay */
-    int64_t *kx, *ky, *kz, *owner;                        /* the filing's key; the owner id */
+    int64_t *kx, *ky, *kz, *owner, *fid;                  /* the filing's key; the owner id; the follow id */
     int8_t *level, *mx, *my, *mz;                         /* the filing's level and sides */
     uint8_t* flags;                                       /* XB_CF_* */
 };
@@ -315,6 +315,7 @@ struct XbCollRecs {
 #define XB_CF_SKIP 8               /* never a candidate: moribund, cloaked, a missile */
 #define XB_CF_JUNK 16
 #define XB_CF_MUSHROOM 32
+#define XB_CF_MISSILE 64           /* a candidate of defender missiles only */
 
 static inline XB_HD XbFiling xb_rec_filing(const XbCollRecs& C, int j) {
     XbFiling f;
@@ -341,16 +342,28 @@ static inline XB_HD XbCollOther xb_rec_other(const XbCollRecs& C, const int64_t*
     return o;
 }
 
-/* Whether the ball at place k is one of the gradient's candidates of `me` (at place j): live, massive (or dropped out
-   of warp before me in id order), uncloaked, not a missile, not my mushroom, junk only to junk. */
-static inline XB_HD bool xb_coll_candidate(const XbCollRecs& C, const int64_t* ids, int j, int k, bool me_junk) {
+/* Gradient's skips for the pair (me, the ball at place k) (full.py:2633-2638): my own missile or mushroom, a missile
+   at me, junk unless I am junk; for a missile, its owner (the signed owner id: a defender's launcher is not skipped). */
+static inline XB_HD bool xb_coll_skipped(const XbCollRecs& C, const int64_t* ids, int j, int k, bool me_junk,
+                                         bool me_missile, int64_t me_owner) {
+    const uint8_t f = C.flags[k];
+    if ((f & (XB_CF_MUSHROOM | XB_CF_MISSILE)) && C.owner[k] == ids[j]) return true;
+    if ((f & XB_CF_MISSILE) && C.fid[k] == ids[j]) return true;
+    if ((f & XB_CF_JUNK) && !me_junk) return true;
+    return me_missile && ids[k] == me_owner;
+}
+
+/* Whether the ball at place k is one of the gradient's candidates of `me` (at place j) among GetNearbyBalls' (not a
+   normal missile's): live, massive (or dropped out of warp before me in id order), uncloaked, not a missile unless
+   `me` is a defender missile, and not skipped. */
+static inline XB_HD bool xb_coll_candidate(const XbCollRecs& C, const int64_t* ids, int j, int k, bool me_junk,
+                                           bool me_defender, int64_t me_owner) {
     if (k == j) return false;
     const uint8_t f = C.flags[k];
     if (f & XB_CF_SKIP) return false;
+    if ((f & XB_CF_MISSILE) && !me_defender) return false;
     if (!(f & XB_CF_MASSIVE) && !((f & XB_CF_DROPPED) && ids[k] < ids[j])) return false;
-    if ((f & XB_CF_MUSHROOM) && C.owner[k] == ids[j]) return false;
-    if ((f & XB_CF_JUNK) && !me_junk) return false;
-    return true;
+    return !xb_coll_skipped(C, ids, j, k, me_junk, me_defender, me_owner);
 }
 
 /* Gradient (space.py:1650-1661) for the ball at place j of a group whose world occupies places [wlo, whi) (at most 64
@@ -360,9 +373,10 @@ static inline XB_HD void xb_gradient(XbCollMe& me, const XbFiling& mf, bool me_j
                                      const int64_t* ids, const double* radius, int j, int wlo, int whi,
                                      const XbPark& P) {
     const XbV3 me_end = v3(C.ex[j], C.ey[j], C.ez[j]);
+    const bool defender = me.mode == XB_MODE_MISSILE;     /* a normal missile's gradient is xb_missile_gradient */
     uint64_t active = 0;
     for (int k = wlo; k < whi; ++k) {
-        if (!xb_coll_candidate(C, ids, j, k, me_junk)) continue;
+        if (!xb_coll_candidate(C, ids, j, k, me_junk, defender, me.owner_id)) continue;
         if (xb_order_key(mf, xb_rec_filing(C, k)) == XB_NOT_RELATED) continue;
         const double cr = me.radius + radius[k];
         const double s = xb_collide_two_spheres(me.pos, me_end, v3(C.px[k], C.py[k], C.pz[k]),
@@ -381,6 +395,16 @@ static inline XB_HD void xb_gradient(XbCollMe& me, const XbFiling& mf, bool me_j
         const XbCollOther o = xb_rec_other(C, ids, radius, pick);
         xb_potential<0>(me, o, me.pos, o.pos, P);
     }
+}
+
+/* A normal missile's Gradient (Partition::GetCollisionCandidates, full.py:1202-1209): its one candidate is its target
+   at place tp, whatever its flags and boxes (its bubble is the missile's: MissileFollow refuses another's, and no
+   bubble changes in this park), less Gradient's skips. */
+static inline XB_HD void xb_missile_gradient(XbCollMe& me, bool me_junk, const XbCollRecs& C, const int64_t* ids,
+                                             const double* radius, int j, int tp, const XbPark& P) {
+    if (tp < 0 || tp == j || xb_coll_skipped(C, ids, j, tp, me_junk, true, me.owner_id)) return;
+    const XbCollOther o = xb_rec_other(C, ids, radius, tp);
+    xb_potential<0>(me, o, me.pos, o.pos, P);
 }
 
 #endif

@@ -41,6 +41,7 @@ XbPark host_park(const xb_modes_state& s) {
     P.st_q = s.st_i64;
     P.st_keys = s.st_keys;
     P.st_stride = s.st_stride;
+    P.tick_ms = (int64_t)(s.dt * 1000.0 + 0.5);
     return P;
 }
 
@@ -74,14 +75,14 @@ struct Scratch {
     void size_coll(int m) {
         if ((int)cflags.size() >= m) return;
         cd.assign((size_t)15 * m, 0.0);
-        cq.assign((size_t)4 * m, 0);
+        cq.assign((size_t)5 * m, 0);
         cb.assign((size_t)4 * m, 0);
         cflags.assign(m, 0);
         double* d0 = cd.data();
         C.px = d0, C.py = d0 + m, C.pz = d0 + 2 * m, C.vx = d0 + 3 * m, C.vy = d0 + 4 * m, C.vz = d0 + 5 * m;
         C.gx = d0 + 6 * m, C.gy = d0 + 7 * m, C.gz = d0 + 8 * m, C.ex = d0 + 9 * m, C.ey = d0 + 10 * m;
         C.ez = d0 + 11 * m, C.mass = d0 + 12 * m, C.m = d0 + 13 * m, C.tf = d0 + 14 * m;
-        C.kx = cq.data(), C.ky = C.kx + m, C.kz = C.ky + m, C.owner = C.kz + m;
+        C.kx = cq.data(), C.ky = C.kx + m, C.kz = C.ky + m, C.owner = C.kz + m, C.fid = C.owner + m;
         C.level = cb.data(), C.mx = C.level + m, C.my = C.mx + m, C.mz = C.my + m;
         C.flags = cflags.data();
     }

@@ -46,6 +46,7 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.st_q = s.st_i64;
     P.st_keys = s.st_keys;
     P.st_stride = s.st_stride;
+    P.tick_ms = (int64_t)(s.dt * 1000.0 + 0.5);
     return P;
 }
 
@@ -160,7 +161,7 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
 }
 
 // The collision records of a tile in LDS, after the tick's own records (XB_LDS_BYTES_PER_ROW a lane).
-constexpr int XB_COLL_LDS_BYTES_PER_ROW = 15 * 8 + 4 * 8 + 4 + 1;
+constexpr int XB_COLL_LDS_BYTES_PER_ROW = 15 * 8 + 5 * 8 + 4 + 1;
 
 __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
     XbCollRecs C;
@@ -169,8 +170,8 @@ __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
     C.gx = d + 6 * nb, C.gy = d + 7 * nb, C.gz = d + 8 * nb, C.ex = d + 9 * nb, C.ey = d + 10 * nb;
     C.ez = d + 11 * nb, C.mass = d + 12 * nb, C.m = d + 13 * nb, C.tf = d + 14 * nb;
     int64_t* q = reinterpret_cast<int64_t*>(d + 15 * nb);
-    C.kx = q, C.ky = q + nb, C.kz = q + 2 * nb, C.owner = q + 3 * nb;
-    int8_t* b = reinterpret_cast<int8_t*>(q + 4 * nb);
+    C.kx = q, C.ky = q + nb, C.kz = q + 2 * nb, C.owner = q + 3 * nb, C.fid = q + 4 * nb;
+    int8_t* b = reinterpret_cast<int8_t*>(q + 5 * nb);
     C.level = b, C.mx = b + nb, C.my = b + 2 * nb, C.mz = b + 3 * nb;
     C.flags = reinterpret_cast<uint8_t*>(b + 4 * nb);
     return C;

@@ -107,6 +107,7 @@ struct XbPark {
     const int64_t* st_q;
     const int64_t* st_keys;
     int64_t st_stride;
+    int64_t tick_ms;               /* the tick interval in milliseconds (a missile's straight flight) */
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -141,19 +142,40 @@ struct XbShip {
 
 /* ------------------------------------------------------------------------------------------- modes */
 
-/* GotoThrust (ref.py:151-160): toward the target at maxThrust, tapered by (L^2 / D^2)^2 within one tick's distance */
-static inline XB_HD XbV3 xb_goto_thrust(const XbShip& s, XbV3 target, const XbPark& P) {
+/* GotoThrust (ref.py:151-160; full.py:2320-2331): toward the target at maxThrust, tapered by (L^2 / D^2)^2 within one
+   tick's distance unless a missile's */
+static inline XB_HD XbV3 xb_goto_thrust(const XbShip& s, XbV3 target, const XbPark& P, bool missile = false) {
     XbV3 a = v3_sub(target, s.pos);
     double length2 = v3_lensq(a);
     double dist = s.sf * s.max_vel * P.dt;
     double max_thrust = P.friction * s.sf * s.max_vel / (s.mass * s.agility);
     XbV3 unit = v3_unit(a);
     double dd = dist * dist;
-    if (length2 < dd) {
+    if (!missile && length2 < dd) {
         double coff = length2 / dd;
         return v3_scale(unit, max_thrust * coff * coff);
     }
     return v3_scale(unit, max_thrust);
+}
+
+#define XB_MISSILE_STRAIGHT_MS 800                /* Ballpark.cpp:974 (full.py:84) */
+
+/* EvolveMissile (full.py:2337-2356): straight on toward mGoto while (clock - launch) x tick <= 800 ms; then at full
+   thrust, untapered, toward the target's position predicted dist / maxVel ahead along its velocity (as this ball's turn
+   sees it), offset by followRange + both radii back toward the missile (mGoto kept). */
+static inline XB_HD XbV3 xb_missile(XbShip& s, bool target_ok, XbV3 op, XbV3 ov, double other_radius,
+                                    const XbPark& P) {
+    if ((s.clock - s.effect_stamp) * P.tick_ms <= XB_MISSILE_STRAIGHT_MS) return xb_goto_thrust(s, s.gp, P);
+    if (!target_ok) return v3(0.0, 0.0, 0.0);
+    const XbV3 delta = v3_sub(s.pos, op);
+    const double dist = v3_len(delta);
+    const double ct = dist / s.max_vel;
+    XbV3 target = v3_add(op, v3_scale(ov, ct));
+    const double r = s.follow_range + s.radius + other_radius;
+    if (dist == 0.0) target = v3(op.x + 1.0 * r, op.y + 0.0 * r, op.z + 0.0 * r);
+    else target = v3_add(target, v3_scale(v3_scale(delta, r), 1.0 / dist));
+    s.gp = target;
+    return xb_goto_thrust(s, target, P, true);
 }
 
 /* EvolveFollow (modes.py:428-442): the point at followRange + both radii from the target's tick-start position */
@@ -399,7 +421,8 @@ static inline XB_HD void xb_clear_follow(XbShip& s) {
    by GotoDirection along its velocity (refused, so left as it is, when the velocity or the point is not finite), any
    other member stops (Stop(Ball*) clears the follow fields of a FOLLOW or ORBIT ball). Returns the mode it leaves. */
 static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
-    if (s.interactive && s.mode == XB_MODE_ORBIT) {
+    if (s.mode == XB_MODE_MISSILE || (s.interactive && s.mode == XB_MODE_ORBIT)) {
+        if (apply && s.mode == XB_MODE_MISSILE) s.massive = false;    /* a missile flies on, non-massive */
         if (!v3_finite(s.vel)) return s.mode;
         XbV3 p = v3_add(s.pos, v3_scale(v3_unit(s.vel), XB_DIR_DIST));
         if (!v3_finite(p)) return s.mode;
@@ -643,6 +666,16 @@ static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, 
         } else if (s.mode == XB_MODE_GOTO) {
             a = xb_goto_thrust(s, s.gp, P);
             w.path = XB_PATH_GOTO;
+        } else if (s.mode == XB_MODE_MISSILE) {
+            XbV3 op = v3(0.0, 0.0, 0.0), ov = op;
+            double orad = 0.0;
+            if (target_ok) {
+                op = v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]);
+                ov = v3(R.v0x[tp], (R.id[tp] < s.id) ? R.vy_after[tp] : R.v0y[tp], R.v0z[tp]);
+                orad = R.radius[tp];
+            }
+            a = xb_missile(s, target_ok, op, ov, orad, P);
+            w.path = XB_PATH_FOLLOW;
         } else if ((s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT) && !target_ok) {
         } else if (s.mode == XB_MODE_FOLLOW) {
             a = xb_follow(s, v3(R.p0x[tp], R.p0y[tp], R.p0z[tp]), R.radius[tp], P);
@@ -784,11 +817,13 @@ static inline XB_HD void xb_coll_record(const XbRow& w, const XbCollRow& c, cons
     C.ex[j] = e.x, C.ey[j] = e.y, C.ez[j] = e.z;
     C.mass[j] = s.mass, C.m[j] = m, C.tf[j] = s.tf;
     C.owner[j] = s.owner_id;
+    C.fid[j] = s.follow_id;
     xb_rec_put_filing(C, j, c.fil);
     uint8_t f = 0;
     if (s.massive) f |= XB_CF_MASSIVE;
     if (w.free_) f |= XB_CF_FREE;
-    if (w.moribund || c.cloaked != 0 || s.mode == XB_MODE_MISSILE) f |= XB_CF_SKIP;
+    if (w.moribund || c.cloaked != 0) f |= XB_CF_SKIP;
+    if (s.mode == XB_MODE_MISSILE) f |= XB_CF_MISSILE;
     if (c.junk) f |= XB_CF_JUNK;
     if (s.mode == XB_MODE_MUSHROOM) f |= XB_CF_MUSHROOM;
     C.flags[j] = f;
@@ -828,10 +863,15 @@ static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, Xb
             XbCollMe me;
             me.pos = s.pos, me.vel = s.vel, me.g = s.last_g;
             me.m = s.mass * s.agility, me.mass = s.mass, me.tf = s.tf, me.radius = s.radius;
-            me.id = s.id, me.mode = s.mode, me.follow_id = s.follow_id;
+            me.id = s.id, me.mode = s.mode, me.follow_id = s.follow_id, me.owner_id = s.owner_id;
             me.last_c = c.last_c, me.last_collision = s.last_collision;
             me.nh = c.nh, me.hits = P.hits + w.row, me.stride = P.stride, me.cap = P.hits_cap;
-            xb_gradient(me, c.fil, c.junk, C, R.id, R.radius, j, wlo, whi, P);
+            if (s.mode == XB_MODE_MISSILE && s.owner_id >= 0) {
+                const int64_t tp = s.follow_ptr >= 0 ? (int64_t)j + (s.follow_ptr - w.row) : -1;
+                xb_missile_gradient(me, c.junk, C, R.id, R.radius, j, (tp >= wlo && tp < whi) ? (int)tp : -1, P);
+            } else {
+                xb_gradient(me, c.fil, c.junk, C, R.id, R.radius, j, wlo, whi, P);
+            }
             if (P.statics && w.free_ && s.mode != XB_MODE_MISSILE && s.mode != XB_MODE_MUSHROOM) {
                 const int cnt = P.st_range[P.stride + w.row];
                 if (cnt > 0) {

@@ -34,6 +34,7 @@ MIN_WARP_DISTANCE_SQ = 10000000000.0        # WarpTo's short warp (modes.py:78)
 MAX_CATCH_UP_TICKS = 5                      # OnTick's resynchronisation (full.py:92)
 TROLL_MIN_DELAY = 1                         # SetBallTroll (vec_modes.py:90)
 DIR_DIST = 1.0e17                           # GotoDirection's point
+MISSILE_LAUNCH_SPEED = 150.0                # LaunchMissile's least speed (full.py:85)
 
 # (name, block, first block row, columns; 0 for a [n] field)
 FIELDS = (
@@ -100,6 +101,30 @@ def rotation_ypr(yaw: torch.Tensor, pitch: torch.Tensor, roll: torch.Tensor) -> 
     z = f32(f32(f32(sr * cp) * cy) - f32(f32(cr * sp) * sy))
     w = f32(f32(f32(cr * cp) * cy) + f32(f32(sr * sp) * sy))
     return torch.stack((x, y, z, w), dim=1)
+
+
+def rotated_vector(q: torch.Tensor, vec) -> torch.Tensor:
+    """Ball::GetRotatedVector's TransformCoord(vec, RotationQuaternion matrix of q) for [n, 4] float quaternions held in
+    doubles (full.py:283-324): the matrix's entries and the row vector's products and sums as float operations, divided
+    by the homogeneous norm (1 here)."""
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    m00 = f32(1.0 - f32(2.0 * f32(f32(y * y) + f32(z * z))))
+    m01 = f32(2.0 * f32(f32(x * y) + f32(z * w)))
+    m02 = f32(2.0 * f32(f32(x * z) - f32(y * w)))
+    m10 = f32(2.0 * f32(f32(x * y) - f32(z * w)))
+    m11 = f32(1.0 - f32(2.0 * f32(f32(x * x) + f32(z * z))))
+    m12 = f32(2.0 * f32(f32(y * z) + f32(x * w)))
+    m20 = f32(2.0 * f32(f32(x * z) + f32(y * w)))
+    m21 = f32(2.0 * f32(f32(y * z) - f32(x * w)))
+    m22 = f32(1.0 - f32(2.0 * f32(f32(x * x) + f32(y * y))))
+    vx, vy, vz = (float(c) for c in vec)
+    zero = torch.zeros_like(x)
+    norm = f32(f32(f32(f32(zero * vx) + f32(zero * vy)) + f32(zero * vz)) + 1.0)
+    cols = []
+    for a, b, c in ((m00, m10, m20), (m01, m11, m21), (m02, m12, m22)):
+        cols.append(f32(f32(f32(f32(f32(a * vx) + f32(b * vy)) + f32(c * vz)) + 0.0) / norm))
+    out = torch.stack(cols, dim=1)
+    return torch.where((norm != 0.0)[:, None], out, 0.0)
 
 
 def _field_property(name: str, blk: str, row: int, cols: int):
@@ -350,6 +375,7 @@ class VecModesPark(metaclass=_Fields):
             self.max_ang_vel = 1.0
             self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
         self.ids, self.world, self.clock, self.mode = c(ids, I64), c(world, I64), c(time, I64), c(mode, I64)
+        self._index = None                            # lookup's (world, id) -> row map, rebuilt on demand
         self.free, self.massive = c(free, torch.bool), c(massive, torch.bool)
         self.interactive, self.moribund = c(interactive, torch.bool), c(moribund, torch.bool)
         self.cloaked, self.bubble = c(cloaked, I64), c(bubble, I64)
@@ -544,7 +570,8 @@ class VecModesPark(metaclass=_Fields):
     # ---- the follower bookkeeping: Stop(Ball*) (Ballpark.cpp:4578; vec_modes.py:286-311) ----
 
     def _following(self) -> torch.Tensor:
-        return (self.mode == FOLLOW) | (self.mode == ORBIT)
+        """The follow modes whose fields Stop(Ball*) clears (FOLLOW, ORBIT, MISSILE, FORMATION)."""
+        return (self.mode == FOLLOW) | (self.mode == ORBIT) | (self.mode == MISSILE) | (self.mode == FORMATION)
 
     def _stop_rows(self, mask: torch.Tensor) -> None:
         follower = mask & self._following()
@@ -603,6 +630,87 @@ class VecModesPark(metaclass=_Fields):
 
     def orbit(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor) -> None:
         self._follow(mask, target, range_, ORBIT)
+
+    # ---- missiles (VecCombatPark's API, src/exact_ballpark/vec_combat.py:339-396; full.py:1679-1736) ----
+
+    def lookup(self, ball: torch.Tensor) -> torch.Tensor:
+        """[n] the row of Destiny id ball[row] in row's world, or -1 (VecCombatPark.lookup, vec_combat.py:217-223)."""
+        if getattr(self, "_index", None) is None:
+            w, ids = self.world.cpu().tolist(), self.ids.cpu().tolist()
+            self._index = {(a, b): r for r, (a, b) in enumerate(zip(w, ids))}
+        w = self.world.cpu().tolist()
+        want = ball.to(I64).cpu().tolist()
+        rows = [self._index.get((a, b), -1) for a, b in zip(w, want)]
+        return torch.tensor(rows, dtype=I64, device=self.dev)
+
+    def missile_follow(self, mask: torch.Tensor, target: torch.Tensor, owner: torch.Tensor) -> None:
+        """MissileFollow for the missiles in `mask` (target and owner Destiny ids; a negative owner: a defender): refused
+        for a negative target, the missile, its owner or a moribund target as target; stopped; then, unless the target is
+        in another bubble or cloaked, aimed at it (follow range minus both radii, a float), straight on for 800 ms
+        toward 1e16 m along its velocity, at full speed fraction."""
+        dst = target.to(device=self.dev, dtype=I64)
+        own = owner.to(device=self.dev, dtype=I64)
+        trow = self.lookup(dst)
+        tsafe = trow.clamp(min=0)
+        src = self.ids
+        ok = (mask.to(self.dev) & (dst >= 0) & (src != dst) & (src != own) & (dst != own) & (trow >= 0)
+              & ~self.moribund[tsafe])
+        self._stop_rows(ok)
+        ok = ok & (self.bubble == self.bubble[tsafe]) & (self.cloaked[tsafe] == 0)
+        self.follow_id = torch.where(ok, dst, self.follow_id)
+        self.follow_ptr = torch.where(ok, trow, self.follow_ptr)
+        self.follow_range = torch.where(ok, -f32(self.radius + self.radius[tsafe]), self.follow_range)
+        self.owner_id = torch.where(ok, own, self.owner_id)
+        self.effect_stamp = torch.where(ok, self.clock, self.effect_stamp)
+        self.goto = torch.where(ok[:, None], _unit(self.vel) * 1.0e16, self.goto)
+        self.speed_fraction = torch.where(ok, 1.0, self.speed_fraction)
+        self.mode = torch.where(ok, MISSILE, self.mode)
+
+    def launch_missile(self, mask: torch.Tensor, target: torch.Tensor, owner: torch.Tensor, aimed: torch.Tensor,
+                       massive: torch.Tensor) -> None:
+        """LaunchMissile for the missile balls in `mask` (target and owner Destiny ids, the launcher |owner|; aimed and
+        massive the thunk's chars): from the launcher's position, aimed at the target (unit speed) or along the
+        launcher's velocity (its heading backwards when at rest) at max(150, its maximum speed); massive if asked;
+        SetBallPosition (re-filed) and SetBallVelocity (snapped); then MissileFollow, or GotoDirection without a
+        target. At rest without dynamical orientation the heading is the display orientation's (display_ypr)."""
+        ch = lambda t: ((t.to(device=self.dev, dtype=I64) + 128) & 0xFF) - 128
+        dst = target.to(device=self.dev, dtype=I64)
+        own = owner.to(device=self.dev, dtype=I64)
+        aimed, massive = ch(aimed), ch(massive)
+        go = mask.to(self.dev) & (dst >= 0)
+        lrow = self.lookup(own.abs())
+        go = go & (lrow >= 0)
+        lsafe = lrow.clamp(min=0)
+        trow = torch.where(dst != own, self.lookup(dst), -1)
+        has_t = trow >= 0
+        tsafe = trow.clamp(min=0)
+        ps = self.pos[lsafe]
+        aim = has_t & (aimed == 1) & (self.cloaked[tsafe] == 0)
+        v_aim = _unit(self.pos[tsafe] - ps)
+        vs = self.vel[lsafe]
+        max_velocity = torch.where(aim, 0.0, self.max_vel[lsafe])
+        direction = _unit(vs)
+        still = _dot(direction, direction) == 0.0
+        need = go & ~aim & still
+        if bool(need.any()):
+            if self.dynamical_orientation:
+                q = self.rot[lsafe]
+            elif self.display_ypr:
+                q = rotation_ypr(self.yaw[lsafe], self.pitch[lsafe], self.roll[lsafe])
+            else:
+                raise ValueError("a launcher at rest aims by its heading: without dynamical orientation, that is the "
+                                 "display orientation (make the park with display_ypr=True)")
+            direction = torch.where(still[:, None], rotated_vector(q, (0.0, 0.0, -1.0)), direction)
+        speed = torch.where(MISSILE_LAUNCH_SPEED < max_velocity, max_velocity, MISSILE_LAUNCH_SPEED)
+        v0 = torch.where(aim[:, None], v_aim, vs + direction * speed[:, None])
+        self.massive = torch.where(go & (massive != 0), True, self.massive)
+        self.pos = torch.where(go[:, None], ps, self.pos)                    # SetBallPosition
+        self.old_pos = torch.where(go[:, None], ps, self.old_pos)
+        self._refile(go)
+        self.set_velocity(go, v0)
+        cloaked_t = has_t & (self.cloaked[tsafe] != 0)
+        self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
+        self.missile_follow(go & has_t & ~cloaked_t, dst, own)
 
     def warp_to(self, mask: torch.Tensor, point: torch.Tensor, min_range: torch.Tensor,
                 warp_factor: torch.Tensor) -> None:

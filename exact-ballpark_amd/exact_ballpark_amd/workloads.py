@@ -15,6 +15,7 @@ import torch
 
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP = 1, 2, 3, 4, 5, 6
 YPR_OP = 13                                        # SetBallRotation(RotationQuaternion(yaw, pitch, roll))
+LAUNCH_OP = 14                                     # LaunchMissile(missile, target id, owner id, aimed, massive)
 
 
 def _ball_points(rng: np.random.Generator, n: int, radius: float) -> np.ndarray:
@@ -187,6 +188,20 @@ def load_flight_park(park, wl: dict) -> None:
     if wl.get("ypr") is not None:
         park.set_rotation_ypr(torch.ones(wl["n"], dtype=torch.bool, device=dev), t(wl["ypr"]))
     apply_commands(park, wl["cmds"])
+    launch_missiles(park, wl)
+
+
+def launch_missiles(park, wl: dict) -> None:
+    """The workload's tick-1 missile launches (LaunchMissile through the park's API), if any."""
+    L = wl.get("launch")
+    if L is None or len(L["row"]) == 0:
+        return
+    dev = park.dev
+    full = lambda v: torch.zeros(park.n, dtype=torch.int64).index_put_((torch.as_tensor(L["row"]),),
+                                                                        torch.as_tensor(v, dtype=torch.int64)).to(dev)
+    mask = torch.zeros(park.n, dtype=torch.bool)
+    mask[torch.as_tensor(L["row"])] = True
+    park.launch_missile(mask.to(dev), full(L["target"]), full(L["owner"]), full(L["aimed"]), full(L["massive"]))
 
 
 def apply_commands(park, c: dict, mask=None) -> None:
@@ -242,6 +257,11 @@ def world_scenario(wl: dict, wi: int, steps: int) -> dict:
             cmds.append((1, o, int(r["id"][k]), tgt, p[0], p[1], p[2],
                          float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
                          int(c["warp_factor"][k])))
+    L = wl.get("launch")
+    if L is not None:                                    # then the launches, in row order
+        for i in np.flatnonzero((L["row"] >= rows.start) & (L["row"] < rows.stop)):
+            cmds.append((1, LAUNCH_OP, int(r["id"][L["row"][i]]), int(L["target"][i]), float(L["aimed"][i]),
+                         float(L["massive"][i]), 0.0, 0.0, int(L["owner"][i])))
     statics = []
     st = wl.get("statics")
     if st is not None:
@@ -347,10 +367,73 @@ def ragged_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int 
             "mix": mix}
 
 
-def composed_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32, mix: str = "uniform") -> dict:
-    """The ragged composed park without missiles or static collidables: ragged_park's worlds (massive ships), with
-    dynamical orientation (maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), each ship rotated by
-    RotationQuaternion(yaw, pitch, roll) of angles U(-pi, pi)) and the gradient collision method."""
+def _with_missiles(wl: dict, rng: np.random.Generator, most: int) -> dict:
+    """The ragged workload with U{0..most} missile balls a world after its ships (ids after the ships'): radius
+    U(1, 20) m, mass U(1e3, 1e5) kg, maximum speed U(500, 3000) m/s, agility U(0.2, 2), free, at rest, STOP; each
+    launched at tick 1 (LaunchMissile, after the ships' commands) by a ship of its world at another (a lone ship's
+    missile has no target and flies straight on), aimed half the time, massive, one in ten as a defender (a negative
+    owner). Every follow target's row renumbered."""
+    W, counts, start = wl["worlds"], wl["counts"], wl["start"]
+    extra = rng.integers(0, most + 1, W)
+    new_counts = counts + extra
+    new_start = np.concatenate(([0], np.cumsum(new_counts)[:-1]))
+    n_old, n = wl["n"], int(new_counts.sum())
+    ship_dest = np.repeat(new_start, counts) + (np.arange(n_old) - np.repeat(start, counts))
+    mis_world = np.repeat(np.arange(W), extra)
+    mis_k = np.arange(int(extra.sum())) - np.repeat(np.concatenate(([0], np.cumsum(extra)[:-1])), extra)
+    mis_dest = new_start[mis_world] + counts[mis_world] + mis_k
+    m = len(mis_dest)
+    rows = {}
+    for key, v in wl["rows"].items():
+        out = np.zeros((n,) + v.shape[1:], dtype=v.dtype)
+        out[ship_dest] = v
+        rows[key] = out
+    rows["id"][mis_dest] = counts[mis_world] + 1 + mis_k
+    rows["world"][mis_dest] = mis_world
+    launcher = new_start[mis_world] + (rng.uniform(0.0, 1.0, m) * counts[mis_world]).astype(np.int64)
+    other = (rng.uniform(0.0, 1.0, m) * np.maximum(counts[mis_world] - 1, 1)).astype(np.int64)
+    own = launcher - new_start[mis_world]
+    other = np.where(other >= own, other + 1, other)
+    other = np.minimum(other, counts[mis_world] - 1)
+    lid = rows["id"][launcher]
+    tid = np.where(counts[mis_world] > 1, other + 1, lid)
+    rows["mass"][mis_dest] = rng.uniform(1e3, 1e5, m)
+    rows["radius"][mis_dest] = rng.uniform(1.0, 20.0, m)
+    rows["max_vel"][mis_dest] = rng.uniform(500.0, 3000.0, m)
+    rows["agility"][mis_dest] = rng.uniform(0.2, 2.0, m)
+    rows["speed_fraction"][mis_dest] = 1.0
+    rows["pos"][mis_dest] = rows["pos"][launcher]
+    rows["free"][mis_dest] = True
+    rows["mode"][mis_dest] = 2
+    for key in ("max_ang_vel", "rot_agility"):
+        if key in rows:
+            rows[key][mis_dest] = rng.uniform(0.2, 2.0, m) if key == "max_ang_vel" else rng.uniform(0.3, 3.0, m)
+    cmds = {}
+    for key, v in wl["cmds"].items():
+        out = np.zeros((n,) + v.shape[1:], dtype=v.dtype)
+        out[ship_dest] = v
+        cmds[key] = out
+    cmds["target"][ship_dest] = ship_dest[wl["cmds"]["target"]]
+    owner = np.where(rng.uniform(0.0, 1.0, m) < 0.1, -lid, lid)
+    launch = {"row": mis_dest.astype(np.int64), "target": tid.astype(np.int64), "owner": owner.astype(np.int64),
+              "aimed": (rng.uniform(0.0, 1.0, m) < 0.5).astype(np.int64), "massive": np.ones(m, dtype=np.int64)}
+    out = dict(wl)
+    ypr = wl.get("ypr")
+    if ypr is not None:
+        y = np.zeros((n, 3))
+        y[ship_dest] = ypr
+        y[mis_dest] = rng.uniform(-np.pi, np.pi, (m, 3))
+        out["ypr"] = y
+    out.update(n=n, counts=new_counts, start=new_start, rows=rows, cmds=cmds, launch=launch)
+    return out
+
+
+def composed_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: int = 32, mix: str = "uniform",
+                  missiles: int = 0) -> dict:
+    """The ragged composed park without static collidables: ragged_park's worlds (massive ships), with dynamical
+    orientation (maximum angular speed U(0.2, 2) rad/s, rotational agility U(0.3, 3), each ship rotated by
+    RotationQuaternion(yaw, pitch, roll) of angles U(-pi, pi)) and the gradient collision method; with `missiles`,
+    U{0..missiles} missiles a world launched at tick 1 (_with_missiles)."""
     wl = ragged_park(seed, worlds, mean_ships, max_ships, mix)
     rng = np.random.default_rng(seed + 0x5EED)
     n = wl["n"]
@@ -358,6 +441,8 @@ def composed_park(seed: int, worlds: int, mean_ships: float = 6.0, max_ships: in
     wl["rows"]["rot_agility"] = rng.uniform(0.3, 3.0, n)
     wl["rows"]["junk"] = np.zeros(n, dtype=bool)
     wl.update(ypr=rng.uniform(-np.pi, np.pi, (n, 3)), dynamical=True, collisions=True)
+    if missiles:
+        wl = _with_missiles(wl, np.random.default_rng(seed + 0x315), missiles)
     return wl
 
 
@@ -398,6 +483,9 @@ def padded(wl: dict, width: int) -> dict:
     if wl.get("ypr") is not None:
         ypr = np.zeros((n, 3))
         ypr[dest] = wl["ypr"]
+    launch = None
+    if wl.get("launch") is not None:
+        launch = dict(wl["launch"], row=dest[wl["launch"]["row"]])
     return {"n": n, "worlds": W, "ships": width, "counts": counts, "start": np.arange(W) * width, "rows": rows,
             "cmds": cmds, "real": real, "dest": dest, "ypr": ypr, "dynamical": wl.get("dynamical", False),
-            "collisions": wl.get("collisions", False)}
+            "collisions": wl.get("collisions", False), "launch": launch}

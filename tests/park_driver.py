@@ -16,7 +16,7 @@ import torch
 
 import oracle as O
 import scenarios as S
-from exact_ballpark_amd.vec_modes import FOLLOW, ORBIT, ModesGraph, VecModesPark
+from exact_ballpark_amd.vec_modes import FOLLOW, FORMATION, MISSILE, ORBIT, ModesGraph, VecModesPark
 
 F64, I64 = torch.float64, torch.int64
 # the oracle's state columns the park is compared on
@@ -34,15 +34,17 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
     xyz = torch.zeros(n, 3, dtype=F64)
     rr = torch.zeros(n, dtype=F64)
     w = torch.zeros(n, dtype=I64)
+    tid = torch.zeros(n, dtype=I64)                 # the target's Destiny id (the missile commands take ids)
     for wi, (_, _, a, b, x, y, z, r, ww) in items:
         row = index.get((wi, a))
         if row is None:
             continue
         mask[row] = True
         target[row] = index.get((wi, b), -1)
+        tid[row] = b
         xyz[row] = torch.tensor((x, y, z), dtype=F64)
         rr[row], w[row] = r, ww
-    mask, target, xyz, rr, w = (t.to(dev) for t in (mask, target, xyz, rr, w))
+    mask, target, xyz, rr, w, tid = (t.to(dev) for t in (mask, target, xyz, rr, w, tid))
     if op == S.GOTO_OP:
         park.goto_point(mask, xyz)
     elif op == S.DIR_OP:
@@ -69,6 +71,10 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.set_rot_agility(mask, rr)
     elif op == S.YPR_OP:
         park.set_rotation_ypr(mask, xyz)
+    elif op == S.LAUNCH_OP:
+        park.launch_missile(mask, tid, w, xyz[:, 0].to(I64), xyz[:, 1].to(I64))
+    elif op == S.MFOLLOW_OP:
+        park.missile_follow(mask, tid, w)
     else:
         raise ValueError(op)
 
@@ -83,7 +89,7 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
     mode, fptr = q[1], q[5]
     fol = defaultdict(list)
     for row in range(n):
-        if mode[row] in (FOLLOW, ORBIT) and fptr[row] >= 0:
+        if mode[row] in (FOLLOW, ORBIT, MISSILE, FORMATION) and fptr[row] >= 0:
             fol[int(fptr[row])].append(int(ids[row]))
     out = {}
     for wi in members:
@@ -275,4 +281,7 @@ def compare(got: dict, want: dict, worlds: list, limit: int = 5) -> list:
             bad.append(f"world {key[0]} ({worlds[key[0]]['kind']}) step {key[1]}: " + "; ".join(diffs))
         else:
             bad.append(f"{key}")
+            if len(bad) > 40:
+                bad.append("...")
+                break
     return bad

@@ -113,10 +113,10 @@ def test_collision_worlds_with_orientation_match_oracle():
 
 
 def test_composed_park_host_oracle_and_padded():
-    """The ragged composed park (ragged worlds of 1..32 massive ships, dynamical orientation, collisions): the device
-    against the host build after 20 ticks, sampled worlds against the oracle, and the padded-to-32 layout's real rows
-    equal to the packed layout's (follow pointers mapped through the padding)."""
-    wl = WL.composed_park(5, 4096)
+    """The ragged composed park (ragged worlds of 1..32 massive ships and 0..8 missiles launched at tick 1, dynamical
+    orientation, collisions): the device against the host build after 20 ticks, sampled worlds against the oracle, and
+    the padded layout's real rows equal to the packed layout's (follow pointers mapped through the padding)."""
+    wl = WL.composed_park(5, 4096, missiles=8)
     g, h = _load(wl, DEV), _load(wl, "cpu")
     for _ in range(20):
         g.evolve()
@@ -127,7 +127,7 @@ def test_composed_park_host_oracle_and_padded():
         w = D.run_oracle([sc], [0], record={20})[(0, 20)]
         got = D._snapshot(g, [wi], {wi: list(WL.world_rows(wl, wi))})[wi]
         assert D._same_bits(got[0], w[0]).all() and (got[1] == w[1]).all() and got[2] == w[2], wi
-    pad = WL.padded(wl, 32)
+    pad = WL.padded(wl, max(32, int(wl["counts"].max())))
     p = _load(pad, DEV)
     for _ in range(20):
         p.evolve()
