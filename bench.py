@@ -78,6 +78,14 @@ WORKLOADS = {
 }
 
 
+def _kernel_which(park) -> int:
+    """xb_modes_info's index of the tiled kernel the park's tick launches."""
+    if park.collisions and park._st is not None:
+        return 9 if park.dynamical_orientation else 8
+    return {(False, False): 0, (True, False): 3, (False, True): 4, (True, True): 5}[
+        (bool(park.dynamical_orientation), bool(park.collisions))]
+
+
 def _measure(park, ticks_per_launch: int, steps: int, warmup: int) -> dict:
     """Times `steps` ticks as graph replays of several launches of `ticks_per_launch` ticks each (about 64 ticks a
     replay, so the launches run back to back and the events time the kernel), then the latency of single launches."""
@@ -328,8 +336,7 @@ def main() -> int:
         "latency_ms": {"tick_p50": p["tick_latency_ms"]["p50"], "tick_p99": p["tick_latency_ms"]["p99"],
                        "launch_p50": p["launch_latency_ms"]["p50"], "launch_p99": p["launch_latency_ms"]["p99"]},
         "roofline": p["roofline"], "single_tick": p["single_tick"],
-        "kernel": N.kernel_info({(False, False): 0, (True, False): 3, (False, True): 4, (True, True): 5}[
-            (bool(p["_park"].dynamical_orientation), bool(p["_park"].collisions))]),
+        "kernel": N.kernel_info(_kernel_which(p["_park"])),
     }
     for key in ("learned_scheduler", "padded_baseline"):
         if key in p:

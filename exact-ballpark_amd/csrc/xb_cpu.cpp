@@ -139,11 +139,11 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
         }
         for (int64_t k = 0; k < cnt; ++k) {
             if (ypr)
-                xb_coll_phase34<DO, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k],
-                                          whi[k], P, rot1[k], w1[k], &ys[k]);
+                xb_coll_phase34<DO, true, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k,
+                                                wlo[k], whi[k], P, rot1[k], w1[k], &ys[k]);
             else
-                xb_coll_phase34<DO, false>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k, wlo[k],
-                                           whi[k], P, rot1[k], w1[k]);
+                xb_coll_phase34<DO, false, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k,
+                                                 wlo[k], whi[k], P, rot1[k], w1[k]);
             xb_tick_tail(rows[k]);
         }
     }

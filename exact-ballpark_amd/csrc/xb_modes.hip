@@ -181,7 +181,7 @@ __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
 // the reference's order needs (modes.py:702-745): (A, B) the tick-start records; (1) the behaviours, then each row's
 // view for the gradients; (3a) the rows in warp proper, whose drop-out makes them massive for the balls after them;
 // (3b) each massive row's gradient over its world's rows, its end state; (4) the commit and the refiling.
-template <bool DO, bool YPR>
+template <bool DO, bool YPR, bool ST>
 __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     extern __shared__ double xb_lds[];
     const int nb = blockDim.x;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
         if (warped) xb_coll_phase3_warp(w, c, C, j, P, &p, &v);
         __syncthreads();
         if (on) {
-            xb_coll_phase34<DO, YPR>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1, &y);
+            xb_coll_phase34<DO, YPR, ST>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1, &y);
             xb_tick_tail(w);
         }
         __syncthreads();
@@ -298,11 +298,25 @@ xb_modes_tiled_turning(xb_modes_state s) {
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
 xb_collide_tiled_flight(xb_modes_state s) {
-    xb_collide_tiled_body<false, false>(s);
+    xb_collide_tiled_body<false, false, false>(s);
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
 xb_collide_tiled_turning(xb_modes_state s) {
-    xb_collide_tiled_body<true, false>(s);
+    xb_collide_tiled_body<true, false, false>(s);
+}
+// With static collidables: the statics' responses compiled in (kept out of the other kernels: their working set
+// costs registers and scratch every tick).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_flight_st(xb_modes_state s) {
+    xb_collide_tiled_body<false, false, true>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_turning_st(xb_modes_state s) {
+    xb_collide_tiled_body<true, false, true>(s);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
+xb_collide_tiled_flight_ypr_st(xb_modes_state s) {
+    xb_collide_tiled_body<false, true, true>(s);
 }
 // The flight tick keeping the display orientation (without dynamical orientation only).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_FLIGHT)))
@@ -311,7 +325,7 @@ xb_modes_tiled_flight_ypr(xb_modes_state s) {
 }
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XB_WAVES_COLLIDE)))
 xb_collide_tiled_flight_ypr(xb_modes_state s) {
-    xb_collide_tiled_body<false, true>(s);
+    xb_collide_tiled_body<false, true, false>(s);
 }
 
 __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) {
@@ -473,9 +487,18 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
             (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
             return -1;
         const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
-        if (dyn) xb_collide_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
-        else if (ypr) xb_collide_tiled_flight_ypr<<<state->n_tiles, block, lds, st>>>(*state);
-        else xb_collide_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
+        const bool stat = state->statics && state->st_f64 && state->st_range;
+        if (stat && (!state->st_i64 || !state->st_keys || state->st_stride <= 0)) return -1;
+        if (dyn) {
+            if (stat) xb_collide_tiled_turning_st<<<state->n_tiles, block, lds, st>>>(*state);
+            else xb_collide_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
+        } else if (ypr) {
+            if (stat) xb_collide_tiled_flight_ypr_st<<<state->n_tiles, block, lds, st>>>(*state);
+            else xb_collide_tiled_flight_ypr<<<state->n_tiles, block, lds, st>>>(*state);
+        } else {
+            if (stat) xb_collide_tiled_flight_st<<<state->n_tiles, block, lds, st>>>(*state);
+            else xb_collide_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);
+        }
         return xb_launch_rc();
     }
     if (state->tiles) {
@@ -557,6 +580,8 @@ int xb_modes_info(int which, int64_t* out) {
     else if (which == 5) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_turning));
     else if (which == 6) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_tiled_flight_ypr));
     else if (which == 7) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_flight_ypr));
+    else if (which == 8) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_flight_st));
+    else if (which == 9) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_collide_tiled_turning_st));
     else if (which == 2) e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&xb_modes_step_a));
     else return -1;
     if (e != hipSuccess) return (int)e;

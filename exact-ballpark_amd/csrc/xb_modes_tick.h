@@ -851,7 +851,7 @@ static inline XB_HD void xb_coll_phase3_warp(XbRow& w, XbCollRow& c, const XbCol
 
 /* Phase 3's gradient and end state for a row not in warp proper, then phase 4 and the refiling of a row that moved
    at least sqrt(0.1) m (modes.py:717-743). The row's world is at places [wlo, whi) of the group. */
-template <bool DO, bool YPR = false>
+template <bool DO, bool YPR = false, bool ST = false>
 static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, XbV3 p, XbV3 v, const XbCollRecs& C,
                                          const XbRecords& R, int j, int wlo, int whi, const XbPark& P, XoQ rot1,
                                          XoV3 w1, XbYpr* y = nullptr) {
@@ -872,7 +872,7 @@ static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, Xb
             } else {
                 xb_gradient(me, c.fil, c.junk, C, R.id, R.radius, j, wlo, whi, P);
             }
-            if (P.statics && w.free_ && s.mode != XB_MODE_MISSILE && s.mode != XB_MODE_MUSHROOM) {
+            if (ST && P.statics && w.free_ && s.mode != XB_MODE_MISSILE && s.mode != XB_MODE_MUSHROOM) {
                 const int cnt = P.st_range[P.stride + w.row];
                 if (cnt > 0) {
                     XbStatics S;

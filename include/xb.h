@@ -142,7 +142,8 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
                       const int64_t* warp_factor, void* stream);
 
 /* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass, 3 tiled with
-   dynamical orientation, 4 tiled with collisions, 5 with both;
+   dynamical orientation, 4 tiled with collisions, 5 with both, 6 tiled keeping the display orientation, 7 that with
+   collisions, 8 tiled with collisions and static collidables, 9 that with dynamical orientation;
    out[0..3] = vector registers a lane (hipFuncAttributes numRegs), private (scratch) bytes a lane, static LDS bytes,
    maximum threads a workgroup. Replaces xbpm_info (src/exact_ballpark/vec_modes_cuda.cu:766). */
 int xb_modes_info(int which, int64_t* out);
