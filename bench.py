@@ -139,9 +139,17 @@ def _roofline(n_ships: int, m: dict, ticks_per_launch: int, traffic, per_ship: i
 
 
 def _traffic(key: str):
-    if TRAFFIC_FILE.exists():
-        return json.loads(TRAFFIC_FILE.read_text()).get(key)
-    return None
+    """The measured HBM bytes a launch for workload_order_tTicks (profiles/modes_tick_traffic.json), or, when that
+    world order was not profiled, the same kernel and ticks a launch in the other order (the order changes which worlds
+    share a workgroup, not the bytes a row moves); None when neither was."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    table = json.loads(TRAFFIC_FILE.read_text())
+    if key in table:
+        return table[key]
+    kind, order, ticks = key.split("_")
+    other = "rows" if order == "learned" else "learned"
+    return table.get(f"{kind}_{other}_{ticks}")
 
 
 def main() -> int:

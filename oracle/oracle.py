@@ -17,6 +17,7 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 5, 6, 7
 VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, YPR_OP = 8, 9, 10, 11, 12, 13
 LAUNCH_OP, MFOLLOW_OP = 14, 15
+SETFORM_OP, FFOLLOW_OP = 16, 17
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -76,6 +77,9 @@ def lib() -> ctypes.CDLL:
         L.or_get_static_boxes.argtypes = [vp, i, pq]
         L.or_launch_missile.argtypes = [vp, i64, i64, i64, i, i]
         L.or_missile_follow.argtypes = [vp, i64, i64, i64]
+        L.or_load_formations.argtypes = [vp, i, ctypes.POINTER(ctypes.c_int), pd]
+        L.or_set_ball_formation.argtypes = [vp, i64, i]
+        L.or_formation_follow.argtypes = [vp, i64, i64, i]
         _LIB = L
     return _LIB
 
@@ -128,6 +132,14 @@ class World:
         if rc:
             raise ValueError(f"static collidable {s['id']} refused")
 
+    def load_formations(self, formations: list) -> None:
+        """LoadFormations: each formation's slot offsets ([x, y, z] each)."""
+        counts = (ctypes.c_int * max(1, len(formations)))(*[len(f) for f in formations])
+        flat = [float(c) for f in formations for v in f for c in v]
+        offs = (ctypes.c_double * max(1, len(flat)))(*flat)
+        if self.L.or_load_formations(self.p, len(formations), counts, offs):
+            raise ValueError("formations refused (at most 8 of at most 32 slots)")
+
     def static_boxes(self, k: int) -> np.ndarray:
         """[boxes, 3]: the keys static collidable k (in the order added) is filed in, sorted."""
         out = np.zeros((128, 3), dtype=np.int64)
@@ -167,6 +179,10 @@ class World:
             L.or_launch_missile(p, a, b, w, int(x), int(y))
         elif op == MFOLLOW_OP:
             L.or_missile_follow(p, a, b, w)
+        elif op == SETFORM_OP:
+            L.or_set_ball_formation(p, a, w)
+        elif op == FFOLLOW_OP:
+            L.or_formation_follow(p, a, b, w)
         else:
             raise ValueError(op)
 
@@ -230,6 +246,8 @@ def run_world(w: dict, record_every_step: bool = True):
         o.add(b)
     for st in w.get("statics", []):
         o.add_static(st)
+    if w.get("formations"):
+        o.load_formations(w["formations"])
     for step in range(1, w["steps"] + 1):
         for c in w["cmds"]:
             if c[0] == step:
@@ -263,6 +281,8 @@ def time_worlds(worlds: list, ticks: int) -> tuple:
             o.add(b)
         for st in w.get("statics", []):
             o.add_static(st)
+        if w.get("formations"):
+            o.load_formations(w["formations"])
         for c in w["cmds"]:
             if c[0] == 1:
                 o.command(c)

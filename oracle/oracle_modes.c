@@ -64,11 +64,14 @@ typedef struct {
     int64_t hits[MAX_HITS];
     /* the display orientation without dynamical orientation (full.py: yaw, pitch, roll_angle and the rest) */
     double yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta;
+    int formation_id;                          /* a leader's formation (-1: none; a char) */
 } OBall;
 
 typedef struct { int level; int64_t k[3]; } OBox;
 
 #define MAX_STATICS 16                         /* a world's static collidables */
+#define MAX_FORMATIONS 8
+#define MAX_FORM_SLOTS 32
 #define MAX_SBOXES 128                         /* the boxes one is filed in; more sets the park's overflow flag */
 
 typedef struct {                               /* a capsule or an oriented box (full.py:650-705) */
@@ -96,6 +99,9 @@ typedef struct {
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
     int ns;
     OStatic statics[MAX_STATICS];              /* FullBallpark's capsules and oriented boxes, in the order added */
+    int nform;                                 /* the formations (LoadFormations): each one's slot offsets */
+    int form_len[MAX_FORMATIONS];
+    double form[MAX_FORMATIONS][MAX_FORM_SLOTS][3];
 } OPark;
 
 /* ------------------------------------------------------------------------------------------ vector helpers */
@@ -208,6 +214,7 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->is_free = is_free, b->is_massive = is_massive, b->is_interactive = is_interactive;
     b->cloaked = cloaked, b->bubble = bubble, b->is_moribund = is_moribund;
     b->rot[3] = b->old_rot[3] = 1.0;                                 /* Ball::Ball: identity, at rest, 1 rad/s, 0.9 */
+    b->formation_id = -1;
     b->max_ang_vel = 1.0;
     b->rot_agility = (double)0.9f;
     refresh_time_factor(p, b);
@@ -971,6 +978,95 @@ void or_launch_missile(OPark* p, int64_t src, int64_t dst, int64_t owner, int ai
     or_missile_follow(p, src, dst, owner);
 }
 
+/* EvolveFormation (full.py:2359-2382): the leader's slot offset, scaled to both radii plus its length, as a float
+   vector turned by the leader's rotation; GotoThrust toward the leader's position plus it (mGoto kept). No thrust when
+   the leader has no formation or the slot is past its end. A negative slot is Destiny's undefined behaviour: the
+   park's overflow flag. */
+static void evolve_formation(OPark* p, OBall* b, double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    OBall* o = find(p, b->follow_ptr);
+    if (!o || o->formation_id < 0 || o->formation_id > p->nform - 1) return;
+    int slot = (int)(int8_t)(b->effect_stamp & 0xFF);
+    const int fid = o->formation_id;
+    if (slot > p->form_len[fid] - 1) return;
+    if (slot < 0) {
+        p->overflow = 1;
+        return;
+    }
+    const double* offset = p->form[fid][slot];
+    double len = length(offset);
+    if (len == 0.0) len = 1.0;
+    double s = b->radius + o->radius + len, v[3], t[3];
+    double inv = ieee_div(1.0, len);
+    for (int i = 0; i < 3; ++i) t[i] = F((offset[i] * s) * inv);     /* v3_of(d_div(d_scale(offset, s), length)) */
+    rotated_vector(p, o, t, v);
+    double op[3];
+    for (int i = 0; i < 3; ++i) op[i] = o->pos[i] + v[i];
+    b->gp[0] = op[0], b->gp[1] = op[1], b->gp[2] = op[2];
+    goto_thrust_to(p, b, op, 0, out);
+}
+
+/* LoadFormations (full.py:1859-1861): the slot offsets as doubles; n formations, counts[k] slots each, offsets
+   3 doubles a slot in order */
+int or_load_formations(OPark* p, int n, const int* counts, const double* offsets) {
+    if (n < 0 || n > MAX_FORMATIONS) return -1;
+    int at = 0;
+    for (int k = 0; k < n; ++k) {
+        if (counts[k] < 0 || counts[k] > MAX_FORM_SLOTS) return -1;
+        p->form_len[k] = counts[k];
+        for (int j = 0; j < counts[k]; ++j, ++at)
+            for (int q = 0; q < 3; ++q) p->form[k][j][q] = offsets[3 * at + q];
+    }
+    p->nform = n;
+    return 0;
+}
+
+/* SetBallFormation (full.py:1863-1897, a client's view): leaving a formation (-1) stops every follower in it
+   (StopAllFollowers: missiles and interactive orbiters fly on); a smaller formation stops the FORMATION followers in
+   removed slots. Reading past the table is Destiny's undefined behaviour: the overflow flag. */
+void or_set_ball_formation(OPark* p, int64_t id, int formation_id) {
+    int fid = (int)(int8_t)(formation_id & 0xFF);
+    OBall* b = find(p, id);
+    if (!b || fid > p->nform - 1) return;
+    if (b->formation_id >= 0) {
+        if (fid == -1) {
+            stop_all_followers(p, b);
+        } else if (fid != b->formation_id) {
+            if (fid < 0 || b->formation_id > p->nform - 1) {
+                p->overflow = 1;
+                return;
+            }
+            int new_len = p->form_len[fid];
+            if (p->form_len[b->formation_id] > new_len) {
+                for (int i = 0; i < p->n; ++i) {
+                    OBall* f = &p->balls[i];
+                    if (f->mode == FORMATION && f->follow_ptr == b->id && f->follow_id == b->id &&
+                        f->effect_stamp > new_len - 1)
+                        stop_ball(p, f);
+                }
+            }
+        }
+    }
+    b->formation_id = fid;
+}
+
+/* FormationFollow (full.py:1899-1918): into slot `slot` (a char) of the leader's formation */
+void or_formation_follow(OPark* p, int64_t src, int64_t dst, int slot_in) {
+    int slot = (int)(int8_t)(slot_in & 0xFF);
+    OBall* b = find(p, src);
+    if (!b || src == dst) return;
+    OBall* d = find(p, dst);
+    if (!d || d->formation_id < 0 || d->formation_id > p->nform - 1) return;
+    if (slot > p->form_len[d->formation_id] - 1) return;
+    stop_ball(p, b);
+    if (b->bubble != d->bubble || d->cloaked) return;
+    b->follow_id = dst;
+    b->follow_ptr = dst;
+    b->effect_stamp = slot;
+    b->mode = FORMATION;
+    fol_add(d, b->id);
+}
+
 static void goto_thrust(const OPark* p, const OBall* b, double out[3]) {   /* ref.py:151-160 */
     double a[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]};
     double length2 = lensq(a);
@@ -1190,6 +1286,8 @@ static void behaviour(OPark* p, OBall* b) {                              /* mode
         goto_thrust(p, b, a);
     } else if (m == MISSILE) {
         evolve_missile(p, b, a);
+    } else if (m == FORMATION) {
+        evolve_formation(p, b, a);
     } else if (m == FOLLOW) {
         evolve_follow(p, b, a);
     } else if (m == ORBIT) {

@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/missile_golden.npz: the reference's own scalar ballpark with missiles
+"""Writes tests/golden/missile_golden.npz (with `--formations`, formation_golden.npz: `formation_cases`, LoadFormations,
+SetBallFormation and FormationFollow): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -35,6 +36,10 @@ def _apply(park, c) -> None:
         park.launch_missile(a, b, w, int(x), int(y))
     elif op == S.MFOLLOW_OP:
         park.missile_follow(a, b, w)
+    elif op == S.SETFORM_OP:
+        park.set_ball_formation(a, w)
+    elif op == S.FFOLLOW_OP:
+        park.formation_follow(a, b, w)
     else:
         C._apply(park, c)
 
@@ -57,6 +62,8 @@ def run_world(w: dict) -> list:
         if b["cloaked"]:
             park.set_cloaked(b["id"], b["cloaked"])
         park.balls[b["id"]].bubble = b["bubble"]
+    if w.get("formations"):
+        park.load_formations(w["formations"])
     order = sorted(b["id"] for b in w["balls"])
     out = []
     for k, bid in enumerate(order):
@@ -73,9 +80,12 @@ def run_world(w: dict) -> list:
     return out
 
 
-def main(seed: int = 1, do: bool = False) -> None:
-    name = "missile_do_golden.npz" if do else "missile_golden.npz"
-    ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
+def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
+    name = f"{what}_do_golden.npz" if do else f"{what}_golden.npz"
+    if what == "formation":
+        ws = S.do_formation_cases(seed) if do else S.formation_cases(seed)
+    else:
+        ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
     if not do:
         for w in ws:
             w["ypr"] = True
@@ -110,4 +120,6 @@ def main(seed: int = 1, do: bool = False) -> None:
 
 
 if __name__ == "__main__":
-    main(int(sys.argv[1]) if len(sys.argv) > 1 and sys.argv[1] != "--do" else 1, do="--do" in sys.argv)
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    main(int(args[0]) if args else 1, do="--do" in sys.argv,
+         what="formation" if "--formations" in sys.argv else "missile")

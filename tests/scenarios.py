@@ -974,3 +974,92 @@ def do_missile_cases(seed: int, scale: int = 1) -> list:
             b["max_ang_vel"] = r.uniform(0.2, 2.0)
             b["rot_agility"] = r.uniform(0.3, 3.0)
     return out
+
+
+SETFORM_OP = 16                 # SetBallFormation(leader a, formation w)
+FFOLLOW_OP = 17                 # FormationFollow(follower a, leader b, slot w)
+
+
+def formation_cases(seed: int, scale: int = 1) -> list:
+    """The formation worlds (about 120 x scale) for seed, with collisions and the display yaw, pitch and roll (the
+    leader's rotation turns the slot offsets: without dynamical orientation RotationQuaternion of its display
+    orientation): 1-3 formations of 1-12 slots (offsets within 3 km, a zero offset among them); a leader flying GOTO,
+    orbiting or at rest, its followers put in slots (FormationFollow) at tick 1 and later; the leader moved to a smaller
+    formation (the followers in removed slots stop), out of its formation (every follower stops), to a formation past
+    the table (refused); a follower into a slot past the formation (refused), a leader in another bubble or cloaked
+    (refused after the Stop), a follower leaving for a missile or GOTO."""
+    r = random.Random(seed ^ 0xF0F0)
+    out: list = []
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    for _ in range(120 * scale):
+        c = centre()
+        forms = []
+        for _ in range(r.randint(1, 3)):
+            n = r.randint(1, 12)
+            offs = [[r.uniform(-3000.0, 3000.0) for _ in range(3)] for _ in range(n)]
+            if r.random() < 0.2:
+                offs[r.randrange(n)] = [0.0, 0.0, 0.0]
+            forms.append(offs)
+        n_ships = r.randint(2, 7)
+        ids = _ids(r, n_ships, big=r.random() < 0.2)
+        leader = ids[0]
+        balls = [_coll_ball(r, bid, c, 4000.0, massive=r.random() < 0.8, max_vel=r.uniform(50.0, 600.0))
+                 for bid in ids]
+        cmds = []
+        fid = r.randrange(len(forms))
+        cmds.append(_cmd(1, SETFORM_OP, leader, w=fid))
+        x = r.random()
+        if x < 0.5:
+            p = [c[i] + r.uniform(-20000.0, 20000.0) for i in range(3)]
+            cmds.append(_cmd(1, GOTO_OP, leader, x=p[0], y=p[1], z=p[2]))
+        elif x < 0.7 and n_ships > 2:
+            cmds.append(_cmd(1, ORBIT_OP, leader, ids[-1], rr=r.uniform(500.0, 5000.0)))
+        for k, f in enumerate(ids[1:]):
+            if x >= 0.7 and k == n_ships - 2:
+                break
+            slot = r.choice([k % max(1, len(forms[fid])), r.randrange(len(forms[fid]) + 2), -1 if r.random() < 0.05 else 0])
+            cmds.append(_cmd(r.choice([1, 1, 2, 4]), FFOLLOW_OP, f, leader, w=slot))
+        kind = "formation"
+        y = r.random()
+        if y < 0.15:
+            cmds.append(_cmd(r.randint(5, 12), SETFORM_OP, leader, w=-1))
+            kind = "formation_leave"
+        elif y < 0.35:
+            cmds.append(_cmd(r.randint(4, 12), SETFORM_OP, leader, w=r.randrange(len(forms))))
+            kind = "formation_change"
+        elif y < 0.42:
+            cmds.append(_cmd(r.randint(4, 12), SETFORM_OP, leader, w=len(forms) + r.randint(0, 3)))
+            kind = "formation_past_table"
+        elif y < 0.5:
+            b = next(bb for bb in balls if bb["id"] == leader)
+            if r.random() < 0.5:
+                b["cloaked"] = 1
+                kind = "formation_cloaked_leader"
+            else:
+                b["bubble"] = 3
+                kind = "formation_other_bubble"
+        elif y < 0.6 and n_ships > 2:
+            f = ids[1]
+            p = [c[i] + r.uniform(-9000.0, 9000.0) for i in range(3)]
+            cmds.append(_cmd(r.randint(5, 10), GOTO_OP, f, x=p[0], y=p[1], z=p[2]))
+            kind = "formation_follower_leaves"
+        out.append({"kind": kind, "friction": r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]),
+                    "tick": r.choice([1000, 1000, 500, 2000]), "steps": r.randint(10, 40), "new_orbit": r.random() < 0.5,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]),
+                    "collisions": True, "formations": forms})
+    return out
+
+
+def do_formation_cases(seed: int, scale: int = 1) -> list:
+    """formation_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
+    r = random.Random(seed ^ 0xF0D0)
+    out = formation_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+    return out

@@ -19,6 +19,8 @@ def run(w, do=False):
     o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], do, True, not do)
     for b in w["balls"]:
         o.add(b)
+    if w.get("formations"):
+        o.load_formations(w["formations"])
     by_step = defaultdict(list)
     for c in w["cmds"]:
         by_step[c[0]].append(c)
