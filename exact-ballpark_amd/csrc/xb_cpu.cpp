@@ -42,6 +42,10 @@ XbPark host_park(const xb_modes_state& s) {
     P.st_keys = s.st_keys;
     P.st_stride = s.st_stride;
     P.tick_ms = (int64_t)(s.dt * 1000.0 + 0.5);
+    P.form = s.form;
+    P.form_len = s.form_len;
+    P.n_form = s.form ? s.n_form : 0;
+    P.form_stride = s.form_stride;
     return P;
 }
 
@@ -49,7 +53,18 @@ struct Scratch {
     std::vector<double> d;
     std::vector<int64_t> id;
     std::vector<uint8_t> act;
+    std::vector<float> fq;
+    std::vector<int8_t> ffid;
     XbRecords R;
+    /* the formation records, when the park has formations */
+    void size_form(int m, bool on) {
+        if (!on) {
+            R.fq = nullptr, R.ffid = nullptr;
+            return;
+        }
+        if ((int)ffid.size() < m) fq.assign((size_t)4 * m, 0.0f), ffid.assign(m, -1);
+        R.fq = fq.data(), R.ffid = ffid.data();
+    }
     void size(int m) {
         if ((int)act.size() >= m) return;
         d.assign((size_t)8 * m, 0.0);
@@ -95,6 +110,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
                     int64_t cnt) {
     S.size((int)cnt);
     S.size_coll((int)cnt);
+    S.size_form((int)cnt, P.n_form > 0);
     rows.resize((size_t)cnt);
     std::vector<XbCollRow> cr((size_t)cnt);
     std::vector<int> wlo((size_t)cnt), whi((size_t)cnt);
@@ -119,7 +135,10 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
     }
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
-        for (int64_t k = 0; k < cnt; ++k) xb_record_a(rows[k], S.R, (int)k);
+        for (int64_t k = 0; k < cnt; ++k) {
+            xb_record_a(rows[k], S.R, (int)k);
+            if (S.R.ffid) xb_record_form<DO>(rows[k], S.R, (int)k, ypr ? &ys[k] : nullptr);
+        }
         for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
         for (int64_t k = 0; k < cnt; ++k) {
             rot1[k] = xo_q(0.0f, 0.0f, 0.0f, 1.0f);
@@ -159,6 +178,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
 void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector<XbRow>& rows, int64_t lo,
                int64_t cnt) {
     S.size((int)cnt);
+    S.size_form((int)cnt, P.n_form > 0);
     rows.resize((size_t)cnt);
     const bool dyn = s.dynamical != 0;
     const bool ypr = !dyn && s.ypr;
@@ -170,7 +190,13 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
     }
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
-        for (int64_t k = 0; k < cnt; ++k) xb_record_a(rows[k], S.R, (int)k);
+        for (int64_t k = 0; k < cnt; ++k) {
+            xb_record_a(rows[k], S.R, (int)k);
+            if (S.R.ffid) {
+                if (dyn) xb_record_form<true>(rows[k], S.R, (int)k, nullptr);
+                else xb_record_form<false>(rows[k], S.R, (int)k, ypr ? &ys[k] : nullptr);
+            }
+        }
         for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
         for (int64_t k = 0; k < cnt; ++k) {
             if (dyn) xb_tick_row<true>(rows[k], S.R, (int)k, 0, (int)cnt, P);
@@ -198,6 +224,9 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
     if (!state || state->n < 0 || state->stride < state->n) return -1;
     if (state->n == 0) return 0;
     if (state->dynamical && state->ypr) return -1;
+    if (state->form && state->n_form > 0 &&
+        (!state->form_len || state->form_stride <= 0 || !state->tiles || (!state->dynamical && !state->ypr)))
+        return -1;
     const XbPark P = host_park(*state);
     if (state->collisions) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 || state->hits_cap < 0 ||

@@ -55,7 +55,8 @@
 #define XB_B_INTERACTIVE 3
 #define XB_B_SENSOR 4
 #define XB_B_JUNK 5         /* space junk (collides only with space junk) */
-#define XB_NB 6
+#define XB_B_FORMATION 6    /* a leader's formation: a char (-1 none), two's complement */
+#define XB_NB 7
 
 /* the general path's scratch (caller-owned): float64 rows pos0 (3), vel0 (3), vy_after; then one uint8 row act */
 #define XB_S_POS0 0

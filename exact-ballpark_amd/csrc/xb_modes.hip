@@ -47,10 +47,26 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.st_keys = s.st_keys;
     P.st_stride = s.st_stride;
     P.tick_ms = (int64_t)(s.dt * 1000.0 + 0.5);
+    P.form = s.form;
+    P.form_len = s.form_len;
+    P.n_form = s.form ? s.n_form : 0;
+    P.form_stride = s.form_stride;
     return P;
 }
 
 constexpr int XB_LDS_BYTES_PER_ROW = 9 * 8 + 4 + 1;     // the records, the lane schedule (int32), act
+constexpr int XB_FORM_LDS_BYTES_PER_ROW = 4 * 4 + 1;   // with formations: a leader's rotation (4 floats), its formation
+
+// The formation records after a kernel's other LDS (`used` bytes a lane), when the park has formations.
+__device__ __forceinline__ void xb_form_lds(XbRecords& R, const XbPark& P, int nb, int used) {
+    R.fq = nullptr;
+    R.ffid = nullptr;
+    if (P.n_form <= 0) return;
+    extern __shared__ double xb_lds[];
+    char* base = reinterpret_cast<char*>(xb_lds) + (size_t)used * nb;
+    R.fq = reinterpret_cast<float*>(base);
+    R.ffid = reinterpret_cast<int8_t*>(R.fq + 4 * nb);
+}
 
 // Waves a SIMD each tiled kernel is built for. A row's whole state lives in registers across a launch's ticks, and
 // with dynamical orientation or collisions the compiler would take every register a lane has: one wave a SIMD, so a
@@ -133,6 +149,7 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
     int* perm = reinterpret_cast<int*>(R.id + nb);
     R.act = reinterpret_cast<uint8_t*>(perm + nb);
     const XbPark P = xb_park(s);
+    xb_form_lds(R, P, nb, XB_LDS_BYTES_PER_ROW);
     const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
     const bool on = j < cnt;
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
@@ -145,7 +162,10 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
         if (YPR) xb_ypr_load(s.ypr, s.stride, r, y);
     }
     for (int t = 0; t < ticks; ++t) {
-        if (on) xb_record_a(w, R, j);
+        if (on) {
+            xb_record_a(w, R, j);
+            if (R.ffid) xb_record_form<DO>(w, R, j, YPR ? &y : nullptr);
+        }
         __syncthreads();
         if (on) xb_record_b(w, R, j, 0, cnt, P);
         __syncthreads();
@@ -202,6 +222,7 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     R.act = reinterpret_cast<uint8_t*>(perm + nb);
     const XbCollRecs C = xb_coll_lds(xb_lds + (XB_LDS_BYTES_PER_ROW * nb) / 8, nb);
     const XbPark P = xb_park(s);
+    xb_form_lds(R, P, nb, XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
     const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
     const bool on = j < cnt;
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
@@ -224,7 +245,10 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     }
     __syncthreads();
     for (int t = 0; t < ticks; ++t) {
-        if (on) xb_record_a(w, R, j);
+        if (on) {
+            xb_record_a(w, R, j);
+            if (R.ffid) xb_record_form<DO>(w, R, j, YPR ? &y : nullptr);
+        }
         __syncthreads();
         if (on) xb_record_b(w, R, j, 0, cnt, P);
         __syncthreads();
@@ -340,6 +364,8 @@ __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) 
     R.radius = s.f64 + XB_F_RADIUS * s.stride;
     R.id = s.i64 + XB_I_IDS * s.stride;
     R.act = s.scratch_u8;
+    R.fq = nullptr;                   // the general launch has no formations (refused at the entry point)
+    R.ffid = nullptr;
     return R;
 }
 
@@ -482,11 +508,14 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     hipStream_t st = (hipStream_t)stream;
     if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
     if (dyn && ypr) return -1;      // dynamical orientation keeps the rotation, not the display orientation
+    const bool forms = state->form && state->n_form > 0;
+    if (forms && (!state->form_len || state->form_stride <= 0 || !state->tiles || (!dyn && !ypr))) return -1;
+    const size_t form_lds = forms ? (size_t)block * XB_FORM_LDS_BYTES_PER_ROW : 0;
     if (state->collisions) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 ||
             (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
             return -1;
-        const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
+        const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW) + form_lds;
         const bool stat = state->statics && state->st_f64 && state->st_range;
         if (stat && (!state->st_i64 || !state->st_keys || state->st_stride <= 0)) return -1;
         if (dyn) {
@@ -503,7 +532,7 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     }
     if (state->tiles) {
         if (state->n_tiles <= 0) return -1;
-        const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW;
+        const size_t lds = (size_t)block * XB_LDS_BYTES_PER_ROW + form_lds;
         if (dyn) xb_modes_tiled_turning<<<state->n_tiles, block, lds, st>>>(*state);
         else if (ypr) xb_modes_tiled_flight_ypr<<<state->n_tiles, block, lds, st>>>(*state);
         else xb_modes_tiled_flight<<<state->n_tiles, block, lds, st>>>(*state);

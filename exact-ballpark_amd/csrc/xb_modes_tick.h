@@ -108,6 +108,9 @@ struct XbPark {
     const int64_t* st_keys;
     int64_t st_stride;
     int64_t tick_ms;               /* the tick interval in milliseconds (a missile's straight flight) */
+    const double* form;            /* the formations' slot offsets (3 doubles a slot, form_stride slots a formation) */
+    const int32_t* form_len;
+    int n_form, form_stride;
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -116,6 +119,8 @@ struct XbRecords {
     double *p0x, *p0y, *p0z, *v0x, *v0y, *v0z, *vy_after;
     int64_t* id;
     double* radius;
+    float* fq;                     /* with formations: each leader's rotation at the tick's start (4 floats a place) */
+    int8_t* ffid;                  /* and its formation (-1 none) */
 };
 
 static inline XB_HD XbV3 xb_ld3(const XbPark& P, int f, int64_t r) {
@@ -136,6 +141,7 @@ struct XbShip {
     double mass, tf, last_collision, radius, max_vel, agility, sf, follow_range;
     int64_t id, mode, effect_stamp, follow_id, owner_id, follow_ptr, clock;
     bool massive, interactive, sensor;
+    int formation_id;              /* a leader's formation (-1 none) */
 };
 
 #include "xb_orient.h"
@@ -531,6 +537,7 @@ static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     s.massive = XB_B(P, XB_B_MASSIVE, r) != 0;
     s.interactive = XB_B(P, XB_B_INTERACTIVE, r) != 0;
     s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
+    s.formation_id = (int)(int8_t)XB_B(P, XB_B_FORMATION, r);
     w.path = XB_PATH_IDLE;
     w.row = r;
     w.turned = false;
@@ -585,6 +592,8 @@ static inline XB_HD void xb_turn_store(const XbPark& P, int64_t r, const XbTurn&
 
 static inline XB_HD bool xb_row_live(const XbRow& w) { return w.free_ && !w.moribund; }
 
+#include "xb_ypr.h"
+
 /* Step A for the row at place j of its group */
 static inline XB_HD void xb_record_a(const XbRow& w, const XbRecords& R, int j) {
     const XbShip& s = w.s;
@@ -595,6 +604,43 @@ static inline XB_HD void xb_record_a(const XbRow& w, const XbRecords& R, int j) 
     R.v0x[j] = s.vel.x, R.v0y[j] = s.vel.y, R.v0z[j] = s.vel.z;
     R.id[j] = s.id;
     R.radius[j] = s.radius;
+}
+
+/* Step A with formations: the row's formation and, for a leader, its rotation at the tick's start: with dynamical
+   orientation its own, else RotationQuaternion of its display yaw, pitch and roll as floats (`y`; Ball::
+   GetRotatedVector, full.py:2112-2117). */
+template <bool DO>
+static inline XB_HD void xb_record_form(const XbRow& w, const XbRecords& R, int j, const XbYpr* y) {
+    const int fid = w.s.formation_id;
+    R.ffid[j] = (int8_t)fid;
+    if (fid < 0) return;
+    XoQ q = xo_q(0.0f, 0.0f, 0.0f, 1.0f);
+    if (DO) q = w.t.rot;
+    else if (y) q = xo_rotation_ypr((float)y->yaw, (float)y->pitch, (float)y->roll);
+    R.fq[4 * j] = q.x, R.fq[4 * j + 1] = q.y, R.fq[4 * j + 2] = q.z, R.fq[4 * j + 3] = q.w;
+}
+
+/* EvolveFormation (full.py:2359-2382) for a FORMATION row whose leader is at place tp: the leader's slot offset
+   (slot: the row's effect stamp as a char), scaled to both radii plus its length, as a float vector turned by the
+   leader's rotation, added to the leader's position; GotoThrust toward it (mGoto kept). No thrust without the leader's
+   formation or past its slots. */
+static inline XB_HD XbV3 xb_formation(XbShip& s, const XbRecords& R, int tp, const XbPark& P) {
+    const int fid = R.ffid[tp];
+    if (fid < 0 || fid > P.n_form - 1) return v3(0.0, 0.0, 0.0);
+    const int slot = (int)(int8_t)(s.effect_stamp & 0xFF);
+    if (slot > P.form_len[fid] - 1 || slot < 0) return v3(0.0, 0.0, 0.0);
+    const double* o = P.form + 3 * ((int64_t)fid * P.form_stride + slot);
+    const XbV3 offset = v3(o[0], o[1], o[2]);
+    double len = v3_len(offset);
+    if (len == 0.0) len = 1.0;
+    const double sc = s.radius + R.radius[tp] + len;
+    const double inv = 1.0 / len;
+    const XoV3 t = xo_v((float)((offset.x * sc) * inv), (float)((offset.y * sc) * inv), (float)((offset.z * sc) * inv));
+    const XoQ q = xo_q(R.fq[4 * tp], R.fq[4 * tp + 1], R.fq[4 * tp + 2], R.fq[4 * tp + 3]);
+    const XoV3 v = xo_transform(t, xo_matrix(q));
+    const XbV3 op = v3(R.p0x[tp] + (double)v.x, R.p0y[tp] + (double)v.y, R.p0z[tp] + (double)v.z);
+    s.gp = op;
+    return xb_goto_thrust(s, op, P);
 }
 
 /* The place of the row's follow target in its group (a world's rows keep their order and spacing in the group), or -1
@@ -628,7 +674,6 @@ static inline XB_HD void xb_record_b(const XbRow& w, const XbRecords& R, int j, 
 
 /* Step C: the row's whole tick (phase 1, phases 3 and 4, its clock); the group's places are [lo, hi). Other rows are
    read only through the records. */
-#include "xb_ypr.h"
 
 /* Phase 1 of the row at place j (Ballpark::Evolve's behaviours, modes.py:707-716): a release when the target starts
    warp proper, before or after this ball's own turn by their ids; the behaviour (mLastG; with dynamical orientation
@@ -666,6 +711,9 @@ static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, 
         } else if (s.mode == XB_MODE_GOTO) {
             a = xb_goto_thrust(s, s.gp, P);
             w.path = XB_PATH_GOTO;
+        } else if (s.mode == XB_MODE_FORMATION) {
+            if (target_ok && R.ffid) a = xb_formation(s, R, (int)tp, P);
+            w.path = XB_PATH_FOLLOW;
         } else if (s.mode == XB_MODE_MISSILE) {
             XbV3 op = v3(0.0, 0.0, 0.0), ov = op;
             double orad = 0.0;

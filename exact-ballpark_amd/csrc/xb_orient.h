@@ -93,6 +93,16 @@ static inline XB_HD XoV3 xo_transform(XoV3 v, const XoM& m) {    // TransformCoo
 }
 static inline XB_HD XoV3 xo_heading(XoQ rot) { return xo_transform(xo_v(0.0f, 0.0f, 1.0f), xo_matrix(rot)); }
 
+// RotationQuaternion(yaw, pitch, roll) of floats (DirectXMath's XMQuaternionRotationRollPitchYaw, scalar path;
+// full.py:268-280): the half angles' glibc cosf and sinf, the four sums of products left to right.
+static inline XB_HD XoQ xo_rotation_ypr(float yaw, float pitch, float roll) {
+    const float hp = pitch * 0.5f, hy = yaw * 0.5f, hr = roll * 0.5f;
+    const float cp = xb_cosf(hp), sp = xb_sinf(hp), cy = xb_cosf(hy), sy = xb_sinf(hy);
+    const float cr = xb_cosf(hr), sr = xb_sinf(hr);
+    return xo_q((cr * sp) * cy + (sr * cp) * sy, (cr * cp) * sy - (sr * sp) * cy, (sr * cp) * cy - (cr * sp) * sy,
+                (cr * cp) * cy + (sr * sp) * sy);
+}
+
 /* ---- the searches (full.py:2133-2290) ---- */
 
 struct XoShip {

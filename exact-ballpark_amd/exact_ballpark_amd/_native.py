@@ -16,8 +16,8 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 5
-NF, NI, NB, NS = 46, 10, 6, 7
+ABI_VERSION = 6
+NF, NI, NB, NS = 46, 10, 7, 7
 NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 NY = 9                          # the display orientation block: yaw, pitch, roll, their old values, roll speed (2), yaw delta
 NSF, NSI = 93, 5                # the static collidables' table: geometry; kind, id, parent, level, box count
@@ -42,7 +42,8 @@ class ModesState(ctypes.Structure):
                 ("hits", ctypes.c_void_p), ("hits_cap", ctypes.c_int32), ("collisions", ctypes.c_int32),
                 ("ypr", ctypes.c_void_p), ("st_f64", ctypes.c_void_p), ("st_i64", ctypes.c_void_p),
                 ("st_keys", ctypes.c_void_p), ("st_range", ctypes.c_void_p), ("st_stride", ctypes.c_int64),
-                ("st_boxes", ctypes.c_int32), ("statics", ctypes.c_int32)]
+                ("st_boxes", ctypes.c_int32), ("statics", ctypes.c_int32), ("form", ctypes.c_void_p),
+                ("form_len", ctypes.c_void_p), ("n_form", ctypes.c_int32), ("form_stride", ctypes.c_int32)]
 
 
 _LIBS: dict = {}

@@ -48,7 +48,7 @@ FIELDS = (
     ("owner_id", "q", 4, 0), ("follow_ptr", "q", 5, 0), ("clock", "q", 6, 0), ("world", "q", 7, 0),
     ("cloaked", "q", 8, 0), ("bubble", "q", 9, 0),
     ("free", "b", 0, 0), ("moribund", "b", 1, 0), ("massive", "b", 2, 0), ("interactive", "b", 3, 0),
-    ("sensor", "b", 4, 0), ("junk", "b", 5, 0),
+    ("sensor", "b", 4, 0), ("junk", "b", 5, 0), ("formation_id", "c", 6, 0),
 )
 FIELD_NAMES = tuple(f[0] for f in FIELDS)
 # with the gradient collision method: mLastC, the filing (the level and key of the ball's box, its side on each axis),
@@ -141,6 +141,8 @@ def _field_property(name: str, blk: str, row: int, cols: int):
             base = self._y
             if base is None:
                 raise AttributeError(f"{name}: the park was made without display_ypr")
+        elif blk == "c":
+            base = self._b.view(torch.int8)
         else:
             base = self._b.view(torch.bool)
         if cols:
@@ -223,6 +225,7 @@ class VecModesPark(metaclass=_Fields):
         self.rot_agility = float(f32(torch.tensor(0.9, dtype=F64)))
         self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
         self.old_rot = self.rot.clone()
+        self._forms = None             # load_formations' table (formations, slots, lengths) on the device
         self.tiles = None
         self.rowmap = None             # [n] int32: the tiled launch's rows in a regrouped world order (None: row order)
         self._state = None
@@ -289,6 +292,11 @@ class VecModesPark(metaclass=_Fields):
         s.st_range = st["range"].data_ptr() if st else None
         s.st_stride = st["stride"] if st else 0
         s.st_boxes = st["boxes"] if st else 0
+        fm = self._forms
+        s.form = fm["table"].data_ptr() if fm else None
+        s.form_len = fm["len"].data_ptr() if fm else None
+        s.n_form = fm["n"] if fm else 0
+        s.form_stride = fm["stride"] if fm else 0
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -385,6 +393,7 @@ class VecModesPark(metaclass=_Fields):
         self.follow_range = 10.0
         self.last_collision = -1.0
         self.junk = False if junk is None else c(junk, torch.bool)
+        self.formation_id = -1
         self._set_tiles()
         if self.dynamical_orientation:
             self._orient_setup(None, 1)
@@ -712,6 +721,79 @@ class VecModesPark(metaclass=_Fields):
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
 
+    # ---- formations (VecCombatPark's API, src/exact_ballpark/vec_combat.py:400-457; full.py:1859-1918) ----
+
+    def load_formations(self, formations: list) -> None:
+        """LoadFormations: the park's formations, each a list of slot offsets [x, y, z] (doubles). Without dynamical
+        orientation a leader's rotation is its display orientation's: the park needs display_ypr=True."""
+        f = [[[float(c) for c in v] for v in vectors] for vectors in formations]
+        stride = max([len(v) for v in f] + [1])
+        table = torch.zeros(max(1, len(f)), stride, 3, dtype=F64)
+        for i, vectors in enumerate(f):
+            if vectors:
+                table[i, :len(vectors)] = torch.tensor(vectors, dtype=F64)
+        lens = torch.tensor([len(v) for v in f] or [0], dtype=torch.int32)
+        self._forms = {"table": table.to(self.dev).contiguous(), "len": lens.to(self.dev), "n": len(f),
+                       "stride": stride, "lens": [len(v) for v in f]}
+        self._state = None
+
+    def _formation_len(self, fid: torch.Tensor) -> torch.Tensor:
+        lens = torch.tensor(self._forms["lens"] or [0], dtype=I64, device=self.dev)
+        return lens[fid.clamp(0, max(0, self._forms["n"] - 1))]
+
+    def set_ball_formation(self, mask: torch.Tensor, formation: torch.Tensor) -> None:
+        """SetBallFormation (a client's view) for the balls in `mask` (the formation a char): leaving a formation (-1)
+        releases its followers (StopAllFollowers); a smaller one stops the FORMATION followers in the slots it lacks;
+        a formation past the table is refused. A change from a formation gone past the table, or to a negative one, is
+        Destiny's undefined behaviour (Ballpark.cpp:6696): refused with an error."""
+        if self._forms is None:
+            raise ValueError("set_ball_formation: load_formations first")
+        fid = ((formation.to(device=self.dev, dtype=I64) + 128) & 0xFF) - 128
+        F = self._forms["n"]
+        ok = mask.to(self.dev) & ~(fid > F - 1)
+        cur = self.formation_id.to(I64)
+        inside = ok & (cur >= 0)
+        leave = inside & (fid == -1)
+        change = inside & (fid != -1) & (fid != cur)
+        if bool((change & ((fid < 0) | (cur > F - 1))).any()):
+            raise ValueError("SetBallFormation reads mFormations out of its range: Destiny's result is undefined")
+        self._release_followers(leave)
+        new_len = self._formation_len(fid)
+        shrink = change & (self._formation_len(cur) > new_len)
+        safe = torch.where(self.follow_ptr >= 0, self.follow_ptr, 0)
+        f = ((self.mode == FORMATION) & (self.follow_ptr >= 0) & shrink[safe]
+             & (self.follow_id == self.ids[safe]) & (self.effect_stamp > new_len[safe] - 1))
+        self._stop_rows(f)
+        self.formation_id = torch.where(ok, fid, cur).to(torch.int8)
+
+    def formation_follow(self, mask: torch.Tensor, leader: torch.Tensor, slot: torch.Tensor) -> None:
+        """FormationFollow for the balls in `mask`: leader is each row's leader's Destiny id, slot a char; refused
+        without the leader's formation or past its slots; stopped; then, unless the leader is in another bubble or
+        cloaked, in the slot. A negative slot is Destiny's undefined behaviour (EvolveFormation reads before the
+        formation, Ballpark.cpp:1044): refused with an error."""
+        if self._forms is None:
+            raise ValueError("formation_follow: load_formations first")
+        if not self.dynamical_orientation and not self.display_ypr:
+            raise ValueError("formations turn by the leader's rotation: without dynamical orientation that is the "
+                             "display orientation (make the park with display_ypr=True)")
+        if self.tiles is None:
+            raise ValueError("formations need the tiled launch (each world's rows consecutive, in one workgroup)")
+        slot = ((slot.to(device=self.dev, dtype=I64) + 128) & 0xFF) - 128
+        dst = leader.to(device=self.dev, dtype=I64)
+        lrow = self.lookup(dst)
+        lsafe = lrow.clamp(min=0)
+        fid = self.formation_id.to(I64)[lsafe]
+        ok = (mask.to(self.dev) & (self.ids != dst) & (lrow >= 0) & (fid >= 0) & ~(fid > self._forms["n"] - 1)
+              & ~(slot > self._formation_len(fid) - 1))
+        if bool((ok & (slot < 0)).any()):
+            raise ValueError("FormationFollow into a negative slot: EvolveFormation's result is undefined")
+        self._stop_rows(ok)
+        ok = ok & (self.bubble == self.bubble[lsafe]) & (self.cloaked[lsafe] == 0)
+        self.follow_id = torch.where(ok, dst, self.follow_id)
+        self.follow_ptr = torch.where(ok, lrow, self.follow_ptr)
+        self.effect_stamp = torch.where(ok, slot, self.effect_stamp)
+        self.mode = torch.where(ok, FORMATION, self.mode)
+
     def warp_to(self, mask: torch.Tensor, point: torch.Tensor, min_range: torch.Tensor,
                 warp_factor: torch.Tensor) -> None:
         point = point.to(device=self.dev, dtype=F64)
@@ -760,10 +842,14 @@ class VecModesPark(metaclass=_Fields):
             N.check(rc, "xb_cpu_modes_commands")
 
     def _release_followers(self, warping: torch.Tensor) -> None:
+        """StopAllFollowers (modes.py:410-426): missiles (non-massive from then on) and interactive orbiters fly on
+        along their velocities, the other followers stop."""
         f = self._members(warping)
-        orbiter = f & self.interactive & (self.mode == ORBIT)
-        self._stop_rows(f & ~orbiter)
-        self.goto_direction(orbiter, self.vel.clone())
+        missile = f & (self.mode == MISSILE)
+        fly = missile | (f & self.interactive & (self.mode == ORBIT))
+        self.massive = torch.where(missile, False, self.massive)
+        self._stop_rows(f & ~fly)
+        self.goto_direction(fly, self.vel.clone())
 
     def entity_warp_in(self, mask: torch.Tensor, point: torch.Tensor, warp_factor: torch.Tensor) -> None:
         point = point.to(device=self.dev, dtype=F64)

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 5
+#define XB_ABI_VERSION 6
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -88,6 +88,14 @@ typedef struct xb_modes_state {
     int64_t st_stride;
     int32_t st_boxes;
     int32_t statics;
+    /* The park's formations (LoadFormations), when n_form > 0 (tiled shape only; without dynamical orientation the
+       leaders' rotations are their display orientation's: `ypr` needed): form holds each formation's slot offsets,
+       formation f's slot k at form[3 (f form_stride + k) ..], form_len[f] its slot count. A leader's formation is u8
+       row XB_B_FORMATION (a char). */
+    const double* form;
+    const int32_t* form_len;
+    int32_t n_form;
+    int32_t form_stride;
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */

@@ -75,6 +75,10 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.launch_missile(mask, tid, w, xyz[:, 0].to(I64), xyz[:, 1].to(I64))
     elif op == S.MFOLLOW_OP:
         park.missile_follow(mask, tid, w)
+    elif op == S.SETFORM_OP:
+        park.set_ball_formation(mask, w)
+    elif op == S.FFOLLOW_OP:
+        park.formation_follow(mask, tid, w)
     else:
         raise ValueError(op)
 
@@ -127,9 +131,10 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
     for wi in (order if order is not None else range(len(worlds))):
         w = worlds[wi]
         groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)),
-                bool(w.get("collisions", False)), bool(w.get("ypr", False)))].append(wi)
+                bool(w.get("collisions", False)), bool(w.get("ypr", False)),
+                repr(w.get("formations")))].append(wi)       # one formation table a park, as LoadFormations keeps
     out = {}
-    for (friction, tick, new_orbit, dynamical, coll, ypr), members in groups.items():
+    for (friction, tick, new_orbit, dynamical, coll, ypr, _), members in groups.items():
         balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
                             use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
@@ -143,6 +148,8 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
         statics = {wi: worlds[wi]["statics"] for wi in members if worlds[wi].get("statics")}
         if statics:
             park.load_statics(statics)
+        if worlds[members[0]].get("formations") is not None:
+            park.load_formations(worlds[members[0]]["formations"])
         if general:
             park.tiles = None
             park._state = None
@@ -205,6 +212,8 @@ def _run_flags(w: dict, initial: bool = False):
         o.add(b)
     for s in w.get("statics", []):
         o.add_static(s)
+    if w.get("formations"):
+        o.load_formations(w["formations"])
     by_step = defaultdict(list)
     for c in w["cmds"]:
         by_step[c[0]].append(c)
