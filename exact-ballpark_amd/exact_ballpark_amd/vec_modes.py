@@ -721,6 +721,25 @@ class VecModesPark(metaclass=_Fields):
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
 
+    # ---- cloaking (VecCombatPark's API, src/exact_ballpark/vec_combat.py:497-519; full.py:1744-1780) ----
+
+    def cloak(self, mask: torch.Tensor, mode: torch.Tensor, range_: torch.Tensor | None = None) -> None:
+        """CloakBall for the balls in `mask`, a client's view (no proximity sensor: `range_` is the master's and
+        changes nothing here): refused unless the mode, a signed char, is positive; the followers released
+        (StopAllFollowers); cloaked with the mode's byte; non-massive."""
+        mode = ((mode.to(device=self.dev, dtype=I64) + 128) & 0xFF) - 128
+        ok = mask.to(self.dev) & (mode > 0)
+        self._release_followers(ok)
+        self.cloaked = torch.where(ok, mode & 0xFF, self.cloaked)
+        self.massive = torch.where(ok, False, self.massive)
+
+    def uncloak(self, mask: torch.Tensor) -> None:
+        """UncloakBall for the balls in `mask`, a client's view: uncloaked; massive again unless in warp proper."""
+        ok = mask.to(self.dev)
+        self.cloaked = torch.where(ok, 0, self.cloaked)
+        warping = (self.mode == WARP) & (self.effect_stamp >= 0)
+        self.massive = torch.where(ok & ~warping, True, self.massive)
+
     # ---- formations (VecCombatPark's API, src/exact_ballpark/vec_combat.py:400-457; full.py:1859-1918) ----
 
     def load_formations(self, formations: list) -> None:

@@ -18,6 +18,7 @@ GOTO_OP, DIR_OP, STOP_OP, FOLLOW_OP, ORBIT_OP, WARP_OP, WARPIN_OP = 1, 2, 3, 4, 
 VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, YPR_OP = 8, 9, 10, 11, 12, 13
 LAUNCH_OP, MFOLLOW_OP = 14, 15
 SETFORM_OP, FFOLLOW_OP = 16, 17
+CLOAK_OP, UNCLOAK_OP = 18, 19
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -80,6 +81,8 @@ def lib() -> ctypes.CDLL:
         L.or_load_formations.argtypes = [vp, i, ctypes.POINTER(ctypes.c_int), pd]
         L.or_set_ball_formation.argtypes = [vp, i64, i]
         L.or_formation_follow.argtypes = [vp, i64, i64, i]
+        L.or_cloak_ball.argtypes = [vp, i64, i]
+        L.or_uncloak_ball.argtypes = [vp, i64]
         _LIB = L
     return _LIB
 
@@ -183,6 +186,10 @@ class World:
             L.or_set_ball_formation(p, a, w)
         elif op == FFOLLOW_OP:
             L.or_formation_follow(p, a, b, w)
+        elif op == CLOAK_OP:
+            L.or_cloak_ball(p, a, w)
+        elif op == UNCLOAK_OP:
+            L.or_uncloak_ball(p, a)
         else:
             raise ValueError(op)
 

@@ -1067,6 +1067,25 @@ void or_formation_follow(OPark* p, int64_t src, int64_t dst, int slot_in) {
     fol_add(d, b->id);
 }
 
+/* CloakBall (full.py:1744-1758), a client's view (no proximity sensor): refused unless the mode, a signed char, is
+   positive; the followers released (StopAllFollowers); cloaked with the mode's byte; non-massive. */
+void or_cloak_ball(OPark* p, int64_t id, int mode_in) {
+    int mode = (int)(int8_t)(mode_in & 0xFF);
+    OBall* b = find(p, id);
+    if (!b || mode <= 0) return;
+    stop_all_followers(p, b);
+    b->cloaked = mode & 0xFF;
+    b->is_massive = 0;
+}
+
+/* UncloakBall (full.py:1760-1780), a client's view: uncloaked; massive again unless in warp proper. */
+void or_uncloak_ball(OPark* p, int64_t id) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    b->cloaked = 0;
+    if (!(b->mode == WARP && !(b->effect_stamp < 0))) b->is_massive = 1;
+}
+
 static void goto_thrust(const OPark* p, const OBall* b, double out[3]) {   /* ref.py:151-160 */
     double a[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]};
     double length2 = lensq(a);

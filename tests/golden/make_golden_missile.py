@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Writes tests/golden/missile_golden.npz (with `--formations`, formation_golden.npz: `formation_cases`, LoadFormations,
-SetBallFormation and FormationFollow): the reference's own scalar ballpark with missiles
+SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -40,6 +40,10 @@ def _apply(park, c) -> None:
         park.set_ball_formation(a, w)
     elif op == S.FFOLLOW_OP:
         park.formation_follow(a, b, w)
+    elif op == S.CLOAK_OP:
+        park.cloak_ball(a, w)
+    elif op == S.UNCLOAK_OP:
+        park.uncloak_ball(a)
     else:
         C._apply(park, c)
 
@@ -84,6 +88,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
     name = f"{what}_do_golden.npz" if do else f"{what}_golden.npz"
     if what == "formation":
         ws = S.do_formation_cases(seed) if do else S.formation_cases(seed)
+    elif what == "cloak":
+        ws = S.do_cloak_cases(seed) if do else S.cloak_cases(seed)
     else:
         ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
     if not do:
@@ -122,4 +128,4 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     main(int(args[0]) if args else 1, do="--do" in sys.argv,
-         what="formation" if "--formations" in sys.argv else "missile")
+         what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv else "missile")

@@ -79,6 +79,10 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.set_ball_formation(mask, w)
     elif op == S.FFOLLOW_OP:
         park.formation_follow(mask, tid, w)
+    elif op == S.CLOAK_OP:
+        park.cloak(mask, w)
+    elif op == S.UNCLOAK_OP:
+        park.uncloak(mask)
     else:
         raise ValueError(op)
 

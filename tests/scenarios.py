@@ -1063,3 +1063,97 @@ def do_formation_cases(seed: int, scale: int = 1) -> list:
             b["max_ang_vel"] = r.uniform(0.2, 2.0)
             b["rot_agility"] = r.uniform(0.3, 3.0)
     return out
+
+
+CLOAK_OP = 18                   # CloakBall(ball a, mode w: a signed char)
+UNCLOAK_OP = 19                 # UncloakBall(ball a)
+
+
+def cloak_cases(seed: int, scale: int = 1) -> list:
+    """The cloaking worlds (about 120 x scale) for seed, with collisions: a ball cloaking (CloakBall with modes 1-3, a
+    refused 0, -1 or 200: a signed char) among ships and fixed spheres, its followers released (FOLLOW and non-interactive
+    ORBIT stopped, interactive orbiters, missiles at it and its formation's followers as each is), then uncloaking
+    (massive again, also a ball that was not massive before; not massive while in warp proper); commands at the
+    cloaked ball after it cloaked (Follow, Orbit, LaunchMissile, FormationFollow: refused or flying straight)."""
+    r = random.Random(seed ^ 0xC10A)
+    out: list = []
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    for _ in range(120 * scale):
+        c = centre()
+        n = r.randint(3, 7)
+        ids = _ids(r, n, big=r.random() < 0.2)
+        cl = ids[0]
+        spread = r.choice([1500.0, 4000.0, 10000.0])
+        balls = [_coll_ball(r, cl, c, spread, massive=r.random() < 0.8, max_vel=r.uniform(50.0, 400.0))]
+        cmds, forms = [], None
+        if r.random() < 0.3:
+            forms = [[[r.uniform(-2000.0, 2000.0) for _ in range(3)] for _ in range(r.randint(1, 6))]]
+            cmds.append(_cmd(1, SETFORM_OP, cl, w=0))
+        for k, bid in enumerate(ids[1:]):
+            x = r.random()
+            if x < 0.15 and k >= 1:
+                balls.append(_sphere(r, bid, c, spread))
+                continue
+            if x < 0.35:                                # a missile at the cloaker
+                balls.append(_coll_ball(r, bid, c, spread, radius=r.uniform(1.0, 20.0), mass=10 ** r.uniform(3, 5),
+                                        max_vel=r.uniform(500.0, 2000.0), sf=1.0, vel=[0.0, 0.0, 0.0]))
+                owner = ids[1] if ids[1] != bid else cl
+                cmds.append(_launch(r.choice([1, 2]), bid, cl, owner, r.choice([0, 1]), r.choice([0, 1])))
+                continue
+            balls.append(_coll_ball(r, bid, c, spread, max_vel=r.uniform(50.0, 400.0)))
+            y = r.random()
+            if y < 0.35:
+                cmds.append(_cmd(1, FOLLOW_OP, bid, cl, rr=r.uniform(200.0, 3000.0)))
+            elif y < 0.7:
+                cmds.append(_cmd(1, ORBIT_OP, bid, cl, rr=r.uniform(500.0, 4000.0)))
+            elif forms is not None:
+                cmds.append(_cmd(r.choice([1, 2]), FFOLLOW_OP, bid, cl, w=r.randrange(len(forms[0]))))
+        x = r.random()
+        if x < 0.3:
+            p = [c[i] + r.uniform(-20000.0, 20000.0) for i in range(3)]
+            cmds.append(_cmd(1, GOTO_OP, cl, x=p[0], y=p[1], z=p[2]))
+        elif x < 0.45:                                  # warping away: uncloaked in warp proper or before
+            p = [c[i] + r.uniform(-1e9, 1e9) for i in range(3)]
+            cmds.append(_cmd(1, WARP_OP, cl, x=p[0], y=p[1], z=p[2], rr=20000.0, w=20))
+        t_cloak = r.randint(2, 8)
+        mode = r.choice([1, 1, 1, 2, 3, 0, -1, 200])
+        cmds.append(_cmd(t_cloak, CLOAK_OP, cl, w=mode))
+        kind = "cloak" if 0 < mode < 128 else "cloak_refused"
+        if r.random() < 0.7:
+            cmds.append(_cmd(t_cloak + r.randint(1, 12), UNCLOAK_OP, cl))
+            kind += "_uncloak"
+        if r.random() < 0.3 and n > 2:                 # a command at the cloaked ball
+            f = ids[-1]
+            z = r.random()
+            if z < 0.4:
+                cmds.append(_cmd(t_cloak + 1, FOLLOW_OP, f, cl, rr=1000.0))
+            elif z < 0.7:
+                cmds.append(_cmd(t_cloak + 1, ORBIT_OP, f, cl, rr=2000.0))
+            elif forms is not None:
+                cmds.append(_cmd(t_cloak + 1, FFOLLOW_OP, f, cl, w=0))
+            kind += "_after"
+        if r.random() < 0.15:                           # a second cloak while cloaked
+            cmds.append(_cmd(t_cloak + 1, CLOAK_OP, cl, w=2))
+        w = {"kind": kind, "friction": r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]),
+             "tick": r.choice([1000, 1000, 500, 2000]), "steps": r.randint(12, 36), "new_orbit": r.random() < 0.5,
+             "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]),
+             "collisions": True}
+        if forms is not None:
+            w["formations"] = forms
+        out.append(w)
+    return out
+
+
+def do_cloak_cases(seed: int, scale: int = 1) -> list:
+    """cloak_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
+    r = random.Random(seed ^ 0xC1D0)
+    out = cloak_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+    return out
