@@ -19,6 +19,7 @@ VEL_OP, ROT_OP, ANGVEL_OP, MAXANG_OP, ROTAG_OP, YPR_OP = 8, 9, 10, 11, 12, 13
 LAUNCH_OP, MFOLLOW_OP = 14, 15
 SETFORM_OP, FFOLLOW_OP = 16, 17
 CLOAK_OP, UNCLOAK_OP = 18, 19
+MUSHROOM_OP, FREE_OP = 20, 21
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -83,6 +84,8 @@ def lib() -> ctypes.CDLL:
         L.or_formation_follow.argtypes = [vp, i64, i64, i]
         L.or_cloak_ball.argtypes = [vp, i64, i]
         L.or_uncloak_ball.argtypes = [vp, i64]
+        L.or_add_mushroom.argtypes = [vp, i64, d, d]
+        L.or_set_free.argtypes = [vp, i64, i]
         _LIB = L
     return _LIB
 
@@ -190,6 +193,10 @@ class World:
             L.or_cloak_ball(p, a, w)
         elif op == UNCLOAK_OP:
             L.or_uncloak_ball(p, a)
+        elif op == MUSHROOM_OP:
+            L.or_add_mushroom(p, a, rr, x)
+        elif op == FREE_OP:
+            L.or_set_free(p, a, w)
         else:
             raise ValueError(op)
 

@@ -174,6 +174,7 @@ int or_park_size(void) { return (int)sizeof(OPark); }
 int or_ball_size(void) { return (int)sizeof(OBall); }
 
 static void file_ball(OPark* p, OBall* b);
+static void refile(OPark* p, OBall* b);
 static void yaw_pitch_roll(OPark* p, OBall* b, int snap);
 
 static void refresh_time_factor(OPark* p, OBall* b) {     /* ref.py:82-87 */
@@ -1086,6 +1087,77 @@ void or_uncloak_ball(OPark* p, int64_t id) {
     if (!(b->mode == WARP && !(b->effect_stamp < 0))) b->is_massive = 1;
 }
 
+/* AddMushroom (full.py:1826-1846), a client's view: refused for a range (a float) or a lifetime not above 0, or a
+   missing owner; else ball 0 (a client's id) as AddBall adds it (full.py:1230-1282): mass 1e20f, radius 0, maximum
+   speed 0, fixed, massive, not interactive, at the owner's position, at rest, agility and speed fraction 1, no bubble;
+   with dynamical orientation AddDynamicallyOrientedBall's turning part without a snap (full.py:1290-1306): identity,
+   at rest, maximum angular speed 0, rotation agility 1. Then its range (follow_range), lifetime (goto x), owner, the
+   clock and MUSHROOM. A ball 0 already in the park (AddBall's "not created" path) is not restated: the overflow
+   flag. */
+void or_add_mushroom(OPark* p, int64_t owner_id, double range, double life) {
+    range = c_float(range);
+    if (range <= 0.0 || life <= 0) return;
+    OBall* o = find(p, owner_id);
+    if (!o) return;
+    if (find(p, 0)) {
+        p->overflow = 1;
+        return;
+    }
+    double pos[3] = {o->pos[0], o->pos[1], o->pos[2]}, vel[3] = {0.0, 0.0, 0.0};
+    if (or_add_ball(p, 0, (double)1.0e20f, 0.0, 0.0, pos, vel, 1.0, 1.0, 0, 1, 0, 0, -1, 0, STOP)) {
+        p->overflow = 1;
+        return;
+    }
+    OBall* m = find(p, 0);
+    if (p->dynamical) {
+        double w[3] = {0.0, 0.0, 0.0}, back[4];
+        m->rot[0] = m->rot[1] = m->rot[2] = 0.0, m->rot[3] = 1.0;
+        memcpy(m->ang_vel, w, sizeof(w));
+        q_rotationf(w, F(-F(p->dt) * v3_lenf(w)), back);
+        q_mulf(m->rot, back, m->old_rot);
+        memcpy(m->old_ang_vel, w, sizeof(w));
+        m->max_ang_vel = 0.0;
+        m->rot_agility = 1.0;
+    }
+    m->follow_range = range;
+    m->gp[0] = life;
+    m->owner_id = owner_id;
+    m->effect_stamp = p->current_time;
+    m->mode = MUSHROOM;
+}
+
+/* SetBallFree (full.py:1555-1572) without minis: freeing; fixing stops the ball, zeroes its velocity (SetBallVelocity),
+   re-files it and clears mLastG. */
+void or_set_free(OPark* p, int64_t id, int flag) {
+    OBall* b = find(p, id);
+    if (!b || (flag != 0) == (b->is_free != 0)) return;
+    b->is_free = flag != 0;
+    if (b->is_free) return;
+    stop_ball(p, b);
+    or_set_velocity(p, id, 0.0, 0.0, 0.0);
+    if (p->collisions) refile(p, b);
+    b->last_g[0] = b->last_g[1] = b->last_g[2] = 0.0;
+}
+
+/* A mushroom's turn in Evolve's last phase (full.py:2545-2555): the fraction of its lifetime passed, in float; growing,
+   mRadius = range x powf(fraction, 0.25f) (the C library's powf: the reference's own glibc transcription agrees with
+   it on every float) and re-filed; grown, RemoveBall inside Evolve (full.py:1633-1657): stopped, its followers
+   released, moribund, non-massive, stamped. Either way no swap: the integrated state stays in the old members. */
+static void evolve_mushroom(OPark* p, OBall* b) {
+    float elapsed = (float)(double)((p->current_time - b->effect_stamp) * (int64_t)p->tick_ms);
+    float fraction = (elapsed * 0.001f) / (float)b->gp[0];
+    if ((double)fraction < 1.0) {
+        b->radius = (double)(float)(b->follow_range * (double)powf(fraction, 0.25f));
+        if (p->collisions) refile(p, b);
+    } else {
+        stop_ball(p, b);
+        stop_all_followers(p, b);
+        b->is_moribund = 1;
+        b->is_massive = 0;
+        b->effect_stamp = p->current_time;
+    }
+}
+
 static void goto_thrust(const OPark* p, const OBall* b, double out[3]) {   /* ref.py:151-160 */
     double a[3] = {b->gp[0] - b->pos[0], b->gp[1] - b->pos[1], b->gp[2] - b->pos[2]};
     double length2 = lensq(a);
@@ -1960,6 +2032,17 @@ void or_evolve(OPark* p) {                                              /* modes
     for (int i = 0; i < p->n; ++i) {                                    /* (4) commit */
         OBall* b = &p->balls[i];
         if (!live[i]) continue;
+        if (b->mode == MUSHROOM) {
+            for (int k = 0; k < 3; ++k) b->old_pos[k] = np[i][k], b->old_vel[k] = nv[i][k];
+            if (p->dynamical) {
+                double rot[4], w[3];
+                apply_torque(p, b->rot, b->ang_vel, b->torque, b->rot_roll, tau_of(p, b), rot, w);
+                memcpy(b->old_rot, rot, sizeof(rot));
+                memcpy(b->old_ang_vel, w, sizeof(w));
+            }
+            evolve_mushroom(p, b);
+            continue;
+        }
         double d[3];
         for (int k = 0; k < 3; ++k) {
             b->old_pos[k] = b->pos[k], b->old_vel[k] = b->vel[k];

@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Writes tests/golden/missile_golden.npz (with `--formations`, formation_golden.npz: `formation_cases`, LoadFormations,
-SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall): the reference's own scalar ballpark with missiles
+SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall;
+with `--mushrooms`, mushroom_golden.npz: `mushroom_cases`, AddMushroom and SetBallFree, ball 0 recorded once added): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -28,6 +29,9 @@ import make_golden_ypr as Y  # noqa: E402  (installs the redirections)
 C = Y.C
 S = C.S
 full = C.full
+import exact_ballpark.dlibm_powf as dlibm_powf  # noqa: E402
+
+full._powf = dlibm_powf.powf                        # the mushrooms' growth: the reference's glibc powf transcription
 
 
 def _apply(park, c) -> None:
@@ -44,6 +48,10 @@ def _apply(park, c) -> None:
         park.cloak_ball(a, w)
     elif op == S.UNCLOAK_OP:
         park.uncloak_ball(a)
+    elif op == S.MUSHROOM_OP:
+        park.add_mushroom(a, rr, x)
+    elif op == S.FREE_OP:
+        park.set_free(a, bool(w))
     else:
         C._apply(park, c)
 
@@ -68,7 +76,8 @@ def run_world(w: dict) -> list:
         park.balls[b["id"]].bubble = b["bubble"]
     if w.get("formations"):
         park.load_formations(w["formations"])
-    order = sorted(b["id"] for b in w["balls"])
+    known = {b["id"] for b in w["balls"]} | {0}        # the world's balls and a client's mushroom (ball 0)
+    order = sorted(bid for bid in park.balls if bid in known)
     out = []
     for k, bid in enumerate(order):
         d, q = _state(park.balls[bid], do)
@@ -78,6 +87,7 @@ def run_world(w: dict) -> list:
             if c[0] == step:
                 _apply(park, c)
         park.evolve()
+        order = sorted(bid for bid in park.balls if bid in known)
         for k, bid in enumerate(order):
             d, q = _state(park.balls[bid], do)
             out.append((step, k, d, q, list(park.balls[bid].collisions)))
@@ -90,6 +100,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
         ws = S.do_formation_cases(seed) if do else S.formation_cases(seed)
     elif what == "cloak":
         ws = S.do_cloak_cases(seed) if do else S.cloak_cases(seed)
+    elif what == "mushroom":
+        ws = S.do_mushroom_cases(seed) if do else S.mushroom_cases(seed)
     else:
         ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
     if not do:
@@ -128,4 +140,5 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     main(int(args[0]) if args else 1, do="--do" in sys.argv,
-         what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv else "missile")
+         what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv
+         else "mushroom" if "--mushrooms" in sys.argv else "missile")

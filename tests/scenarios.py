@@ -1157,3 +1157,78 @@ def do_cloak_cases(seed: int, scale: int = 1) -> list:
             b["max_ang_vel"] = r.uniform(0.2, 2.0)
             b["rot_agility"] = r.uniform(0.3, 3.0)
     return out
+
+
+MUSHROOM_OP = 20                # AddMushroom(owner a, range rr, lifetime x): ball 0 on a client
+FREE_OP = 21                    # SetBallFree(ball a, flag w)
+
+
+def mushroom_cases(seed: int, scale: int = 1) -> list:
+    """The mushroom worlds (about 120 x scale) for seed, with collisions: AddMushroom at a ship (ball 0, fixed, massive,
+    radius 0; refused for a range or lifetime not above 0 or a missing owner), freed (SetBallFree) so it grows at its
+    turn (range x powf(fraction, 0.25f)) until its lifetime has passed and it is removed (moribund, non-massive), or
+    left fixed, or fixed again while growing; ships and fixed spheres around it flying into and through it (its owner
+    passing through: the owner's skip), at rest inside it; lifetimes of 1-30 ticks at 250 ms to 2 s."""
+    r = random.Random(seed ^ 0x5A00)
+    out: list = []
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    for _ in range(120 * scale):
+        c = centre()
+        n = r.randint(2, 6)
+        ids = _ids(r, n, big=r.random() < 0.2)
+        spread = r.choice([800.0, 2500.0, 6000.0])
+        balls, cmds = [], []
+        for k, bid in enumerate(ids):
+            if k >= 2 and r.random() < 0.2:
+                balls.append(_sphere(r, bid, c, spread))
+                continue
+            balls.append(_coll_ball(r, bid, c, spread, max_vel=r.uniform(50.0, 500.0)))
+            if r.random() < 0.6:
+                p = [c[i] + r.uniform(-spread, spread) for i in range(3)]
+                cmds.append(_cmd(r.randint(1, 3), GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        owner = r.choice([b["id"] for b in balls if b["free"]] or ids)
+        tick = r.choice([1000, 1000, 500, 250, 2000])
+        life = r.choice([r.uniform(0.5, 30.0) * tick / 1000.0, float(r.randint(1, 20)) * tick / 1000.0])
+        rng = r.choice([r.uniform(200.0, 6000.0), r.uniform(10.0, 300.0)])
+        kind = "mushroom"
+        x = r.random()
+        if x < 0.06:
+            rng = r.choice([0.0, -50.0])
+            kind = "mushroom_refused"
+        elif x < 0.1:
+            life = r.choice([0.0, -1.0])
+            kind = "mushroom_refused"
+        elif x < 0.13:
+            owner = 999_999_999_999
+            kind = "mushroom_refused"
+        t_add = r.choice([1, 1, 2, 4])
+        cmds.append(_cmd(t_add, MUSHROOM_OP, owner, x=life, rr=rng))
+        y = r.random()
+        if y < 0.8:
+            t_free = t_add + r.choice([0, 0, 1, 3])
+            cmds.append(_cmd(t_free, FREE_OP, 0, w=1))
+            if r.random() < 0.12:
+                cmds.append(_cmd(t_free + r.randint(1, 4), FREE_OP, 0, w=0))
+                kind += "_refixed"
+            else:
+                kind += "_freed"
+        steps = min(45, t_add + int(life * 1000.0 / tick) + r.randint(2, 8))
+        out.append({"kind": kind, "friction": r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]), "tick": tick,
+                    "steps": max(steps, t_add + 2), "new_orbit": r.random() < 0.5, "start": r.randrange(0, 2_000_000),
+                    "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]), "collisions": True})
+    return out
+
+
+def do_mushroom_cases(seed: int, scale: int = 1) -> list:
+    """mushroom_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
+    r = random.Random(seed ^ 0x5AD0)
+    out = mushroom_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+    return out
