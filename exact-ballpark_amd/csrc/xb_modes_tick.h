@@ -483,6 +483,7 @@ struct XbRow {
     bool turned;                   /* the turning state was committed (stored back then) */
     XbV3 old_pos, old_vel;
     bool free_, moribund;
+    bool grown;                    /* a mushroom's turn changed its radius or removed it (stored back then) */
     uint8_t path;                  /* the path class its last tick executed (XB_PATH_*) */
     int64_t row;                   /* its physical row */
 };
@@ -541,9 +542,11 @@ static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     w.path = XB_PATH_IDLE;
     w.row = r;
     w.turned = false;
+    w.grown = false;
 }
 
-/* Every field the tick may change (radius, mass, agility, maximum speed, decay, ids, flags moribund: never) */
+/* Every field the tick may change (mass, agility, maximum speed, decay, ids: never; the radius and the moribund flag
+   only at a mushroom's turn) */
 static inline XB_HD void xb_row_store(const XbPark& P, const XbRow& w) {
     const int64_t r = w.row;
     const XbShip& s = w.s;
@@ -566,6 +569,10 @@ static inline XB_HD void xb_row_store(const XbPark& P, const XbRow& w) {
     XB_B(P, XB_B_MASSIVE, r) = s.massive ? 1 : 0;
     XB_B(P, XB_B_INTERACTIVE, r) = s.interactive ? 1 : 0;
     XB_B(P, XB_B_SENSOR, r) = s.sensor ? 1 : 0;
+    if (w.grown) {
+        XB_F(P, XB_F_RADIUS, r) = s.radius;
+        XB_B(P, XB_B_MORIBUND, r) = w.moribund ? 1 : 0;
+    }
 }
 
 /* The turning state (dynamical orientation): floats held in the float64 block rows */
@@ -776,6 +783,36 @@ static inline XB_HD void xb_phase4(XbRow& w, XbV3 p, XbV3 v, XoQ rot1, XoV3 w1) 
     }
 }
 
+/* A mushroom's turn in the last phase (full.py:2545-2555), in place of the swap: the end-of-tick state stays in the old
+   members (with dynamical orientation the rotation a tick on in the old rotation, as the turn before the swap leaves
+   it, full.py:2526-2532); the fraction of its lifetime passed, (float)((now - start) tick) x 0.001f / (float)lifetime
+   in float; growing, mRadius = range x powf(fraction, 0.25f) (glibc's powf) and true: to be re-filed; its lifetime
+   passed, RemoveBall inside Evolve (full.py:1633-1657): stopped, moribund, non-massive, stamped with the clock (its
+   followers would be released: there are none, the park refuses to make a mushroom a target). */
+template <bool DO>
+static inline XB_HD bool xb_mushroom_turn(XbRow& w, XbV3 p, XbV3 v, XoQ rot1, XoV3 w1, const XbPark& P) {
+    XbShip& s = w.s;
+    w.old_pos = p;
+    w.old_vel = v;
+    if (DO) {
+        w.t.old_rot = rot1;
+        w.t.old_w = w1;
+        w.turned = true;
+    }
+    w.grown = true;
+    const float elapsed = (float)(double)((s.clock - s.effect_stamp) * (int64_t)P.tick_ms);
+    const float fraction = (elapsed * 0.001f) / (float)s.gp.x;
+    if ((double)fraction < 1.0) {
+        s.radius = (double)(float)(s.follow_range * (double)xb_powf(fraction, 0.25f));
+        return true;
+    }
+    s.mode = XB_MODE_STOP;
+    w.moribund = true;
+    s.massive = false;
+    s.effect_stamp = s.clock;
+    return false;
+}
+
 /* The end of a row's tick: PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937: fixed, at rest,
    RIGID), and the world's clock. */
 static inline XB_HD void xb_tick_tail(XbRow& w) {
@@ -806,8 +843,12 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
         XbV3 a = v3(s.last_g.x + 0.0, s.last_g.y + 0.0, s.last_g.z + 0.0);     /* mLastG + mLastC (zero) */
         if (s.mode == XB_MODE_WARP && s.effect_stamp >= 0) xb_phase3_warp(w, a, P, &p, &v);
         else xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
-        xb_phase4<DO>(w, p, v, rot1, w1);
-        if (YPR && !DO) xb_yaw_pitch_roll(*y, s, false, s.clock, false, P);
+        if (s.mode == XB_MODE_MUSHROOM) {
+            xb_mushroom_turn<DO>(w, p, v, rot1, w1, P);
+        } else {
+            xb_phase4<DO>(w, p, v, rot1, w1);
+            if (YPR && !DO) xb_yaw_pitch_roll(*y, s, false, s.clock, false, P);
+        }
     }
     xb_tick_tail(w);
 }
@@ -935,6 +976,10 @@ static inline XB_HD void xb_coll_phase34(XbRow& w, XbCollRow& c, bool warped, Xb
         p = s.pos, v = s.vel;
         const XbV3 a = v3(s.last_g.x + c.last_c.x, s.last_g.y + c.last_c.y, s.last_g.z + c.last_c.z);
         xb_integrate(&p, &v, a, s.mass * s.agility, s.tf, P);
+    }
+    if (s.mode == XB_MODE_MUSHROOM) {
+        if (xb_mushroom_turn<DO>(w, p, v, rot1, w1, P)) xb_file(s, w.free_, P, c.fil);
+        return;
     }
     xb_phase4<DO>(w, p, v, rot1, w1);
     if (YPR && !DO) xb_yaw_pitch_roll(*y, s, c.nh > 0, s.clock, false, P);

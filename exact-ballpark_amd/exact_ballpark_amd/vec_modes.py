@@ -25,7 +25,7 @@ import torch
 from . import _native as N
 
 F64, I64, U8 = torch.float64, torch.int64, torch.uint8
-GOTO, FOLLOW, STOP, WARP, ORBIT, MISSILE = 0, 1, 2, 3, 4, 5
+GOTO, FOLLOW, STOP, WARP, ORBIT, MISSILE, MUSHROOM = 0, 1, 2, 3, 4, 5, 6
 BOID, TROLL, MINIBALL, FIELD, RIGID, FORMATION = 7, 8, 9, 10, 11, 12
 INERT_MODES = (BOID, MINIBALL, FIELD, RIGID)
 LOADABLE = (STOP,) + INERT_MODES
@@ -56,6 +56,8 @@ FIELD_NAMES = tuple(f[0] for f in FIELDS)
 COLL_FIELDS = (("last_c", "cf", 0, 3), ("level", "cq", 0, 0), ("key", "cq", 1, 3), ("side", "cq", 4, 3),
                ("hits_n", "cq", 7, 0))
 MAX_COLLIDE_WORLD = 64                      # rows a world with collisions (one wavefront's contact mask)
+EMPTY_ID = -(2 ** 62)                       # an empty row's id (a row kept for a ball added mid-run: a mushroom)
+MUSHROOM_MASS = float(np.float32(1.0e20))   # AddMushroom's mass, the float literal 1.0e20f (Ballpark.cpp:3742)
 MAX_WORLD_STATICS = 64                      # static collidables a world (one response mask)
 # with the display orientation kept (Ball::CalculateYawPitchRoll, without dynamical orientation)
 YPR_FIELDS = (("yaw", "y", 0, 0), ("pitch", "y", 1, 0), ("roll", "y", 2, 0), ("old_yaw", "y", 3, 0),
@@ -394,6 +396,9 @@ class VecModesPark(metaclass=_Fields):
         self.last_collision = -1.0
         self.junk = False if junk is None else c(junk, torch.bool)
         self.formation_id = -1
+        empty = self.ids == EMPTY_ID
+        if bool((empty & (self.free | self.massive | ~self.moribund)).any()):
+            raise ValueError("an empty row (id EMPTY_ID) is moribund, fixed and not massive: use empty_row()")
         self._set_tiles()
         if self.dynamical_orientation:
             self._orient_setup(None, 1)
@@ -620,6 +625,12 @@ class VecModesPark(metaclass=_Fields):
         ok = mask & torch.isfinite(d).all(dim=1)
         self.goto_point(ok, self.pos + _unit(d) * DIR_DIST)
 
+    def _refuse_mushroom_targets(self, ok: torch.Tensor, trow: torch.Tensor) -> None:
+        """A mushroom as a target (its followers would be released at its removal inside the tick, before or after
+        their own turns): not batched, an error."""
+        if bool((ok & (self.mode[trow] == MUSHROOM)).any()):
+            raise ValueError("a mushroom as a follow, orbit, missile or formation target is not supported")
+
     def _follow(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor, mode: int) -> None:
         rng = f32(range_.to(device=self.dev, dtype=F64))
         target = target.to(device=self.dev, dtype=I64)
@@ -628,6 +639,7 @@ class VecModesPark(metaclass=_Fields):
         rows = torch.arange(self.n, device=self.dev)
         ok = (mask & torch.isfinite(rng) & inside & (safe != rows) & (self.world[safe] == self.world)
               & ~self.moribund[safe] & (self.bubble[safe] == self.bubble) & (self.cloaked[safe] == 0))
+        self._refuse_mushroom_targets(ok, safe)
         self._stop_rows(ok)
         self.follow_id = torch.where(ok, self.ids[safe], self.follow_id)
         self.follow_ptr = torch.where(ok, safe, self.follow_ptr)
@@ -664,6 +676,7 @@ class VecModesPark(metaclass=_Fields):
         src = self.ids
         ok = (mask.to(self.dev) & (dst >= 0) & (src != dst) & (src != own) & (dst != own) & (trow >= 0)
               & ~self.moribund[tsafe])
+        self._refuse_mushroom_targets(ok, tsafe)
         self._stop_rows(ok)
         ok = ok & (self.bubble == self.bubble[tsafe]) & (self.cloaked[tsafe] == 0)
         self.follow_id = torch.where(ok, dst, self.follow_id)
@@ -693,6 +706,7 @@ class VecModesPark(metaclass=_Fields):
         trow = torch.where(dst != own, self.lookup(dst), -1)
         has_t = trow >= 0
         tsafe = trow.clamp(min=0)
+        self._refuse_mushroom_targets(go & has_t, tsafe)
         ps = self.pos[lsafe]
         aim = has_t & (aimed == 1) & (self.cloaked[tsafe] == 0)
         v_aim = _unit(self.pos[tsafe] - ps)
@@ -720,6 +734,87 @@ class VecModesPark(metaclass=_Fields):
         cloaked_t = has_t & (self.cloaked[tsafe] != 0)
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
+
+    # ---- mushrooms (VecCombatPark's API, src/exact_ballpark/vec_combat.py:540-613; full.py:1826-1846) ----
+
+    @staticmethod
+    def empty_row(world: int, time: int = 0) -> dict:
+        """A `load` row kept empty for a ball added mid-run (add_mushroom): no ball (id EMPTY_ID), moribund, fixed,
+        not massive, at the origin; the tick skips it and no ball meets it."""
+        return {"id": EMPTY_ID, "world": world, "time": time, "mass": 0.0, "radius": 0.0, "max_vel": 0.0,
+                "pos": [0.0, 0.0, 0.0], "vel": [0.0, 0.0, 0.0], "agility": 1.0, "speed_fraction": 1.0, "free": False,
+                "massive": False, "interactive": False, "cloaked": 0, "bubble": -1, "moribund": True, "mode": STOP}
+
+    def add_mushroom(self, mask: torch.Tensor, owner: torch.Tensor, range_: torch.Tensor, life: torch.Tensor) -> None:
+        """AddMushroom into the empty rows in `mask`, a client's view (owner a Destiny id in the row's world, the range
+        a float, the lifetime a double): refused (the row left empty) for a range or lifetime not above 0 or a missing
+        owner; else ball 0 as AddBall adds it (mass 1e20f, radius 0, maximum speed 0, fixed, massive, not
+        interactive, at the owner's position, at rest, agility and speed fraction 1, no bubble; with dynamical
+        orientation maximum angular speed 0 and rotation agility 1, not snapped), then its range (follow_range),
+        lifetime (goto x), owner, the clock, MUSHROOM; filed. Freed (set_free), it grows at its turn in each tick and
+        is removed there when its lifetime has passed. A second ball 0 in a world (AddBall's "not created" path) is
+        refused with an error."""
+        mask = mask.to(self.dev)
+        if bool((mask & (self.ids != EMPTY_ID)).any()):
+            raise ValueError("add_mushroom: a row is not empty")
+        rng = f32(range_.to(device=self.dev, dtype=F64))
+        life = life.to(device=self.dev, dtype=F64)
+        orow = self.lookup(owner.to(device=self.dev, dtype=I64))
+        ok = mask & (rng > 0.0) & (life > 0.0) & (orow >= 0)
+        if not bool(ok.any()):
+            return
+        if bool((ok & (self.lookup(torch.zeros(self.n, dtype=I64, device=self.dev)) >= 0)).any()):
+            raise ValueError("add_mushroom: ball 0 is in the world already (AddBall's not-created path)")
+        if int(torch.bincount(self.world[ok]).max()) > 1:
+            raise ValueError("add_mushroom: two mushrooms in one world")
+        osafe = orow.clamp(min=0)
+        where = lambda new, old: torch.where(ok.reshape(-1, *([1] * (old.dim() - 1))), new, old)
+        zero3 = torch.zeros_like(self.pos)
+        self.ids = torch.where(ok, 0, self.ids)
+        self._index = None
+        self.mass = torch.where(ok, MUSHROOM_MASS, self.mass)
+        self.radius = torch.where(ok, 0.0, self.radius)
+        self.max_vel = torch.where(ok, 0.0, self.max_vel)
+        self.agility = torch.where(ok, 1.0, self.agility)
+        self.speed_fraction = torch.where(ok, 1.0, self.speed_fraction)
+        self.pos = where(self.pos[osafe], self.pos)
+        self.old_pos = where(self.pos[osafe], self.old_pos)
+        self.vel = where(zero3, self.vel)
+        self.old_vel = where(zero3, self.old_vel)
+        g = zero3.clone()
+        g[:, 0] = life
+        self.goto = where(g, self.goto)
+        self.last_g = where(zero3, self.last_g)
+        self.time_factor = torch.where(ok, self._decay(self.mass * self.agility), self.time_factor)
+        self.free = torch.where(ok, False, self.free)
+        self.massive = torch.where(ok, True, self.massive)
+        self.interactive = torch.where(ok, False, self.interactive)
+        self.moribund = torch.where(ok, False, self.moribund)
+        self.junk = torch.where(ok, False, self.junk)
+        self.sensor = torch.where(ok, False, self.sensor)
+        self.cloaked = torch.where(ok, 0, self.cloaked)
+        self.bubble = torch.where(ok, -1, self.bubble)
+        self.formation_id = torch.where(ok, -1, self.formation_id.to(I64)).to(torch.int8)
+        self.follow_range = torch.where(ok, rng, self.follow_range)
+        self.owner_id = torch.where(ok, owner.to(device=self.dev, dtype=I64), self.owner_id)
+        self.follow_id = torch.where(ok, 0, self.follow_id)
+        self.follow_ptr = torch.where(ok, -1, self.follow_ptr)
+        self.effect_stamp = torch.where(ok, self.clock, self.effect_stamp)
+        self.last_collision = torch.where(ok, -1.0, self.last_collision)
+        self.mode = torch.where(ok, MUSHROOM, self.mode)
+        if self.dynamical_orientation:
+            ident = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
+            self.rot = where(ident, self.rot)
+            self.ang_vel = where(zero3, self.ang_vel)
+            self.torque = where(zero3, self.torque)
+            self.rot_roll = torch.where(ok, 0.0, self.rot_roll)
+            self.max_ang_vel = torch.where(ok, 0.0, self.max_ang_vel)
+            self.rot_agility = torch.where(ok, 1.0, self.rot_agility)
+            self._orient_setup(ok, 1)              # the old rotation (at rest: nothing to snap)
+        if self.collisions:
+            self.last_c = where(zero3, self.last_c)
+            self._refile(ok)
+        self._state = None
 
     # ---- cloaking (VecCombatPark's API, src/exact_ballpark/vec_combat.py:497-519; full.py:1744-1780) ----
 
@@ -806,6 +901,7 @@ class VecModesPark(metaclass=_Fields):
               & ~(slot > self._formation_len(fid) - 1))
         if bool((ok & (slot < 0)).any()):
             raise ValueError("FormationFollow into a negative slot: EvolveFormation's result is undefined")
+        self._refuse_mushroom_targets(ok, lsafe)
         self._stop_rows(ok)
         ok = ok & (self.bubble == self.bubble[lsafe]) & (self.cloaked[lsafe] == 0)
         self.follow_id = torch.where(ok, dst, self.follow_id)
@@ -993,9 +1089,9 @@ class VecModesPark(metaclass=_Fields):
         self.old_ang_vel = torch.where(ok[:, None], 0.0, self.old_ang_vel)
 
     def set_free(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
-        """SetBallFree (vec_modes.py:438-451)."""
+        """SetBallFree (vec_modes.py:438-451; full.py:1555-1572 without minis); an empty row is no ball: unchanged."""
         flag = flag.to(self.dev) != 0
-        change = mask & (flag != self.free)
+        change = mask.to(self.dev) & (flag != self.free) & (self.ids != EMPTY_ID)
         fix = change & ~flag
         self.free = torch.where(change, flag, self.free)
         self._stop_rows(fix)

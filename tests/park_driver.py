@@ -16,7 +16,7 @@ import torch
 
 import oracle as O
 import scenarios as S
-from exact_ballpark_amd.vec_modes import FOLLOW, FORMATION, MISSILE, ORBIT, ModesGraph, VecModesPark
+from exact_ballpark_amd.vec_modes import EMPTY_ID, FOLLOW, FORMATION, MISSILE, ORBIT, ModesGraph, VecModesPark
 
 F64, I64 = torch.float64, torch.int64
 # the oracle's state columns the park is compared on
@@ -36,7 +36,9 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
     w = torch.zeros(n, dtype=I64)
     tid = torch.zeros(n, dtype=I64)                 # the target's Destiny id (the missile commands take ids)
     for wi, (_, _, a, b, x, y, z, r, ww) in items:
-        row = index.get((wi, a))
+        row = index.get((wi, "empty" if op == S.MUSHROOM_OP else a))
+        if op == S.MUSHROOM_OP:
+            b = a
         if row is None:
             continue
         mask[row] = True
@@ -79,6 +81,10 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.set_ball_formation(mask, w)
     elif op == S.FFOLLOW_OP:
         park.formation_follow(mask, tid, w)
+    elif op == S.MUSHROOM_OP:                       # into the world's empty row, the owner a Destiny id
+        park.add_mushroom(mask, tid, rr, xyz[:, 0])
+    elif op == S.FREE_OP:
+        park.set_free(mask, w)
     elif op == S.CLOAK_OP:
         park.cloak(mask, w)
     elif op == S.UNCLOAK_OP:
@@ -101,7 +107,7 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
             fol[int(fptr[row])].append(int(ids[row]))
     out = {}
     for wi in members:
-        rows = sorted(rows_of[wi], key=lambda r: ids[r])
+        rows = sorted((r for r in rows_of[wi] if ids[r] != EMPTY_ID), key=lambda r: ids[r])
         d = np.stack([np.concatenate((f[0:3, r], f[3:6, r], f[12:15, r], f[15:18, r], f[6:9, r], f[9:12, r],
                                       [f[18, r], f[21, r], f[22, r], f[23, r], f[24, r], f[25, r], f[19, r],
                                        f[20, r]])) for r in rows])
@@ -139,11 +145,14 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                 repr(w.get("formations")))].append(wi)       # one formation table a park, as LoadFormations keeps
     out = {}
     for (friction, tick, new_orbit, dynamical, coll, ypr, _), members in groups.items():
-        balls = [(wi, b) for wi in members for b in worlds[wi]["balls"]]
+        balls = [(wi, b) for wi in members
+                 for b in worlds[wi]["balls"] + ([None] if any(c[1] == S.MUSHROOM_OP for c in worlds[wi]["cmds"])
+                                                 else [])]     # a world's empty row for its mushroom
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
                             use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
                             display_ypr=ypr)
-        park.load([{"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
+        park.load([VecModesPark.empty_row(wi, worlds[wi]["start"]) if b is None else
+                   {"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                     "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
                     "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
@@ -157,7 +166,8 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
         if general:
             park.tiles = None
             park._state = None
-        index = {(wi, b["id"]): i for i, (wi, b) in enumerate(balls)}
+        index = {(wi, b["id"] if b is not None else "empty"): i for i, (wi, b) in enumerate(balls)}
+        index.update({(wi, 0): i for i, (wi, b) in enumerate(balls) if b is None})    # ball 0 once added
         rows_of = defaultdict(list)
         for i, (wi, _) in enumerate(balls):
             rows_of[wi].append(i)

@@ -1,0 +1,31 @@
+"""Mushrooms on gfx950 (the tick kernels' mushroom turn: growth by glibc's powf and the re-filing, the removal; a
+ball's own mushroom skipped), bit for bit (two NaNs equal): against the reference's own vectors
+(tests/golden/mushroom_golden.npz with the display yaw, pitch and roll, mushroom_do_golden.npz with dynamical
+orientation): every ball as added and after every tick, with mLastC, the filing and the collision lists, in workgroups
+of 256 and 64 rows."""
+
+import pytest
+
+import golden_fixture as G
+import park_driver as D
+from test_statics import compare_golden
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+@pytest.mark.parametrize("block", [256, 64])
+def test_park_matches_reference_vectors(block):
+    golden = G.load("mushroom_golden.npz")
+    kept = golden["kept"]
+    got = D.run_park(golden["worlds"], DEV, order=kept, initial=True, block=block)
+    bad = compare_golden(got, golden, kept)
+    assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"
+
+
+def test_park_with_orientation_matches_reference_vectors():
+    golden = G.load("mushroom_do_golden.npz")
+    kept = golden["kept"]
+    got = D.run_park(golden["worlds"], DEV, order=kept, initial=True, block=128)
+    bad = compare_golden(got, golden, kept, do=True)
+    assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"
