@@ -308,7 +308,11 @@ struct XbCollRecs {
     int64_t *kx, *ky, *kz, *owner, *fid;                  /* the filing's key; the owner id; the follow id */
     int8_t *level, *mx, *my, *mz;                         /* the filing's level and sides */
     uint8_t* flags;                                       /* XB_CF_* */
+    uint8_t* kind;                                        /* XB_CK_*: a force field, a miniball */
+    int64_t *fh, *fc, *fa;                                /* with force fields: harmonic, corporation, alliance */
 };
+#define XB_CK_FIELD 1
+#define XB_CK_MINIBALL 2
 #define XB_CF_MASSIVE 1            /* a candidate of every ball */
 #define XB_CF_DROPPED 2            /* dropped out of warp in this tick's phase 3: a candidate of the balls after it */
 #define XB_CF_FREE 4
@@ -343,14 +347,31 @@ static inline XB_HD XbCollOther xb_rec_other(const XbCollRecs& C, const int64_t*
 }
 
 /* Gradient's skips for the pair (me, the ball at place k) (full.py:2633-2638): my own missile or mushroom, a missile
-   at me, junk unless I am junk; for a missile, its owner (the signed owner id: a defender's launcher is not skipped). */
+   at me, junk unless I am junk; for a missile, its owner (the signed owner id: a defender's launcher is not skipped)
+   and its owner's miniballs. */
 static inline XB_HD bool xb_coll_skipped(const XbCollRecs& C, const int64_t* ids, int j, int k, bool me_junk,
                                          bool me_missile, int64_t me_owner) {
     const uint8_t f = C.flags[k];
     if ((f & (XB_CF_MUSHROOM | XB_CF_MISSILE)) && C.owner[k] == ids[j]) return true;
     if ((f & XB_CF_MISSILE) && C.fid[k] == ids[j]) return true;
     if ((f & XB_CF_JUNK) && !me_junk) return true;
-    return me_missile && ids[k] == me_owner;
+    return me_missile && (ids[k] == me_owner || ((C.kind[k] & XB_CK_MINIBALL) && C.owner[k] == me_owner));
+}
+
+/* GetNearbyBalls' force field filter (full.py:1162-1174): a ball in mode FIELD is no candidate of `me` (at place j)
+   when my harmonic is -2 (the pass), or its harmonic, corporation or alliance (not -1) is mine. */
+static inline XB_HD bool xb_field_passed(const XbCollRecs& C, int j, int k) {
+    if (!(C.kind[k] & XB_CK_FIELD) || !C.fh) return false;
+    if (C.fh[j] == -2) return true;
+    if (C.fh[k] != -1 && C.fh[k] == C.fh[j]) return true;
+    if (C.fc[k] != -1 && C.fc[k] == C.fc[j]) return true;
+    return C.fa[k] != -1 && C.fa[k] == C.fa[j];
+}
+
+/* BallSortComparer (full.py:888-890): within a box two local (negative) ids by their owners first, otherwise by id */
+static inline XB_HD bool xb_ball_before(int64_t a, int64_t a_owner, int64_t b, int64_t b_owner) {
+    if (a < 0 && b < 0) return a_owner < b_owner || (a_owner == b_owner && a < b);
+    return a < b;
 }
 
 /* Whether the ball at place k is one of the gradient's candidates of `me` (at place j) among GetNearbyBalls' (not a
@@ -363,6 +384,7 @@ static inline XB_HD bool xb_coll_candidate(const XbCollRecs& C, const int64_t* i
     if (f & XB_CF_SKIP) return false;
     if ((f & XB_CF_MISSILE) && !me_defender) return false;
     if (!(f & XB_CF_MASSIVE) && !((f & XB_CF_DROPPED) && ids[k] < ids[j])) return false;
+    if (xb_field_passed(C, j, k)) return false;
     return !xb_coll_skipped(C, ids, j, k, me_junk, me_defender, me_owner);
 }
 
@@ -383,13 +405,18 @@ static inline XB_HD void xb_gradient(XbCollMe& me, const XbFiling& mf, bool me_j
                                                 v3(C.ex[k], C.ey[k], C.ez[k]), cr);
         if (s != -1.0) active |= 1ULL << (k - wlo);
     }
-    while (active) {                                    /* the contacts in (walk key, id) order */
+    while (active) {                        /* the contacts in (walk key, BallSortComparer) order */
         int pick = -1;
-        int64_t pk = 0, pid = 0;
+        int64_t pk = 0;
         for (uint64_t a = active; a; a &= a - 1) {
             const int k = wlo + __builtin_ctzll(a);
             const int64_t key = xb_order_key(mf, xb_rec_filing(C, k));
-            if (pick < 0 || key < pk || (key == pk && ids[k] < pid)) pick = k, pk = key, pid = ids[k];
+            bool take = pick < 0 || key < pk;
+            if (!take && key == pk) take = xb_ball_before(ids[k], C.owner[k], ids[pick], C.owner[pick]);
+            if (take) {
+                pick = k;
+                pk = key;
+            }
         }
         active &= ~(1ULL << (pick - wlo));
         const XbCollOther o = xb_rec_other(C, ids, radius, pick);

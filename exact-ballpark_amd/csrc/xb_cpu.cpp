@@ -46,6 +46,7 @@ XbPark host_park(const xb_modes_state& s) {
     P.form_len = s.form_len;
     P.n_form = s.form ? s.n_form : 0;
     P.form_stride = s.form_stride;
+    P.fld = s.fld;
     return P;
 }
 
@@ -85,7 +86,8 @@ struct Scratch {
     std::vector<double> cd;
     std::vector<int64_t> cq;
     std::vector<int8_t> cb;
-    std::vector<uint8_t> cflags;
+    std::vector<uint8_t> cflags, ckind;
+    std::vector<int64_t> cfld;
     XbCollRecs C;
     void size_coll(int m) {
         if ((int)cflags.size() >= m) return;
@@ -100,6 +102,15 @@ struct Scratch {
         C.kx = cq.data(), C.ky = C.kx + m, C.kz = C.ky + m, C.owner = C.kz + m, C.fid = C.owner + m;
         C.level = cb.data(), C.mx = C.level + m, C.my = C.mx + m, C.mz = C.my + m;
         C.flags = cflags.data();
+        ckind.assign(m, 0);
+        C.kind = ckind.data();
+        cfld.assign((size_t)3 * m, -1);
+        C.fh = C.fc = C.fa = nullptr;
+    }
+    void size_fields(bool on) {                 // after size_coll: the force fields' keys, when the park has them
+        C.fh = on ? cfld.data() : nullptr;
+        C.fc = on ? cfld.data() + cflags.size() : nullptr;
+        C.fa = on ? cfld.data() + 2 * cflags.size() : nullptr;
     }
 };
 
@@ -110,6 +121,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
                     int64_t cnt) {
     S.size((int)cnt);
     S.size_coll((int)cnt);
+    S.size_fields(P.fld != nullptr);
     S.size_form((int)cnt, P.n_form > 0);
     rows.resize((size_t)cnt);
     std::vector<XbCollRow> cr((size_t)cnt);

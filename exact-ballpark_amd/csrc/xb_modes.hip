@@ -51,6 +51,7 @@ __device__ __forceinline__ XbPark xb_park(const xb_modes_state& s) {
     P.form_len = s.form_len;
     P.n_form = s.form ? s.n_form : 0;
     P.form_stride = s.form_stride;
+    P.fld = s.fld;
     return P;
 }
 
@@ -181,7 +182,8 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
 }
 
 // The collision records of a tile in LDS, after the tick's own records (XB_LDS_BYTES_PER_ROW a lane).
-constexpr int XB_COLL_LDS_BYTES_PER_ROW = 15 * 8 + 5 * 8 + 4 + 1;
+constexpr int XB_COLL_LDS_BYTES_PER_ROW = 15 * 8 + 5 * 8 + 4 + 1 + 1;
+constexpr int XB_FIELD_LDS_BYTES_PER_ROW = 3 * 8;      // with force fields: harmonic, corporation, alliance
 
 __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
     XbCollRecs C;
@@ -194,7 +196,18 @@ __device__ __forceinline__ XbCollRecs xb_coll_lds(double* base, int nb) {
     int8_t* b = reinterpret_cast<int8_t*>(q + 5 * nb);
     C.level = b, C.mx = b + nb, C.my = b + 2 * nb, C.mz = b + 3 * nb;
     C.flags = reinterpret_cast<uint8_t*>(b + 4 * nb);
+    C.kind = C.flags + nb;
+    C.fh = C.fc = C.fa = nullptr;
     return C;
+}
+
+// The force fields' keys after a kernel's other LDS (`used` bytes a lane, a multiple of 8 bytes for 64 lanes), when
+// the park has them.
+__device__ __forceinline__ void xb_field_lds(XbCollRecs& C, const XbPark& P, int nb, int used) {
+    if (!P.fld) return;
+    extern __shared__ double xb_lds[];
+    int64_t* q = reinterpret_cast<int64_t*>(reinterpret_cast<char*>(xb_lds) + (size_t)used * nb);
+    C.fh = q, C.fc = q + nb, C.fa = q + 2 * nb;
 }
 
 // The tiled tick with the gradient collision method: the phases of Ballpark::Evolve with a barrier between them, as
@@ -220,9 +233,11 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     R.id = reinterpret_cast<int64_t*>(R.radius + nb);
     int* perm = reinterpret_cast<int*>(R.id + nb);
     R.act = reinterpret_cast<uint8_t*>(perm + nb);
-    const XbCollRecs C = xb_coll_lds(xb_lds + (XB_LDS_BYTES_PER_ROW * nb) / 8, nb);
+    XbCollRecs C = xb_coll_lds(xb_lds + (XB_LDS_BYTES_PER_ROW * nb) / 8, nb);
     const XbPark P = xb_park(s);
     xb_form_lds(R, P, nb, XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW);
+    xb_field_lds(C, P, nb, XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW +
+                               (P.n_form > 0 ? XB_FORM_LDS_BYTES_PER_ROW : 0));
     const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
     const bool on = j < cnt;
     const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
@@ -515,7 +530,8 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 ||
             (state->hits_cap > 0 && !state->hits) || state->hits_cap < 0)
             return -1;
-        const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW) + form_lds;
+        const size_t lds = (size_t)block * (XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW) + form_lds +
+                           (state->fld ? (size_t)block * XB_FIELD_LDS_BYTES_PER_ROW : 0);
         const bool stat = state->statics && state->st_f64 && state->st_range;
         if (stat && (!state->st_i64 || !state->st_keys || state->st_stride <= 0)) return -1;
         if (dyn) {

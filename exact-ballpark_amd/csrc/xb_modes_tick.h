@@ -111,6 +111,7 @@ struct XbPark {
     const double* form;            /* the formations' slot offsets (3 doubles a slot, form_stride slots a formation) */
     const int32_t* form_len;
     int n_form, form_stride;
+    const int64_t* fld;            /* the force fields' keys [3][stride] (harmonic, corporation, alliance), or null */
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -907,6 +908,10 @@ static inline XB_HD void xb_coll_record(const XbRow& w, const XbCollRow& c, cons
     C.mass[j] = s.mass, C.m[j] = m, C.tf[j] = s.tf;
     C.owner[j] = s.owner_id;
     C.fid[j] = s.follow_id;
+    C.kind[j] = (uint8_t)((s.mode == XB_MODE_FIELD ? XB_CK_FIELD : 0) | (s.mode == XB_MODE_MINIBALL ? XB_CK_MINIBALL : 0));
+    if (C.fh) {
+        C.fh[j] = P.fld[w.row], C.fc[j] = P.fld[P.stride + w.row], C.fa[j] = P.fld[2 * P.stride + w.row];
+    }
     xb_rec_put_filing(C, j, c.fil);
     uint8_t f = 0;
     if (s.massive) f |= XB_CF_MASSIVE;

@@ -56,6 +56,7 @@ FIELD_NAMES = tuple(f[0] for f in FIELDS)
 COLL_FIELDS = (("last_c", "cf", 0, 3), ("level", "cq", 0, 0), ("key", "cq", 1, 3), ("side", "cq", 4, 3),
                ("hits_n", "cq", 7, 0))
 MAX_COLLIDE_WORLD = 64                      # rows a world with collisions (one wavefront's contact mask)
+LOCAL_BALLS = -1073741824                   # DSTLOCALBALLS: a client's local ids count down from it
 EMPTY_ID = -(2 ** 62)                       # an empty row's id (a row kept for a ball added mid-run: a mushroom)
 MUSHROOM_MASS = float(np.float32(1.0e20))   # AddMushroom's mass, the float literal 1.0e20f (Ballpark.cpp:3742)
 MAX_WORLD_STATICS = 64                      # static collidables a world (one response mask)
@@ -228,6 +229,7 @@ class VecModesPark(metaclass=_Fields):
         self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
         self.old_rot = self.rot.clone()
         self._forms = None             # load_formations' table (formations, slots, lengths) on the device
+        self._fld = None               # the force fields' keys [3, stride] (harmonic, corporation, alliance) once set
         self.tiles = None
         self.rowmap = None             # [n] int32: the tiled launch's rows in a regrouped world order (None: row order)
         self._state = None
@@ -299,6 +301,7 @@ class VecModesPark(metaclass=_Fields):
         s.form_len = fm["len"].data_ptr() if fm else None
         s.n_form = fm["n"] if fm else 0
         s.form_stride = fm["stride"] if fm else 0
+        s.fld = self._fld.data_ptr() if self._fld is not None else None
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -343,6 +346,8 @@ class VecModesPark(metaclass=_Fields):
             max_ang_vel=t([float(x) for x in g("max_ang_vel", 1.0)]),
             rot_agility=t([float(x) for x in g("rot_agility", 0.9)]),
             junk=t([bool(x) for x in g("junk", False)], torch.bool))
+        if any("owner" in r for r in rows):             # a miniball's parent (mini_rows)
+            self.owner_id = t([int(x) for x in g("owner", 0)], I64).to(self.dev)
 
     def load_arrays(self, *, ids, world, time, mass, radius, max_vel, pos, vel, agility, speed_fraction, free,
                     massive, interactive, cloaked, bubble, moribund, mode, max_ang_vel=None, rot_agility=None,
@@ -364,7 +369,8 @@ class VecModesPark(metaclass=_Fields):
         x = f32(c(max_vel))
         self.max_vel = torch.where(x < 0.0, 0.0, x)
         x = f32(c(agility))
-        self.agility = torch.where(x <= 0.0, 1.0, x)
+        mini = c(mode, I64) == MINIBALL                 # AddMiniball's own values, without AddBall's clamps
+        self.agility = torch.where((x <= 0.0) & ~mini, 1.0, x)
         x = f32(c(speed_fraction))
         self.speed_fraction = torch.where(x < 0.0, 0.0, x)
         p, v = c(pos), c(vel)
@@ -400,8 +406,9 @@ class VecModesPark(metaclass=_Fields):
         if bool((empty & (self.free | self.massive | ~self.moribund)).any()):
             raise ValueError("an empty row (id EMPTY_ID) is moribund, fixed and not massive: use empty_row()")
         self._set_tiles()
-        if self.dynamical_orientation:
-            self._orient_setup(None, 1)
+        self._fld = None
+        if self.dynamical_orientation:                  # a miniball keeps the identity, old and new (full.py:1350)
+            self._orient_setup(None if not bool(mini.any()) else ~mini, 1)
         if self.display_ypr:                            # AddBall's snap of a free ball (full.py:1283-1288)
             self._y.zero_()
             self._ypr_snap(self.free)
@@ -734,6 +741,61 @@ class VecModesPark(metaclass=_Fields):
         cloaked_t = has_t & (self.cloaked[tsafe] != 0)
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
+
+    # ---- minis and force fields (VecCollidePark's, src/exact_ballpark/vec_collide.py:364-420, 748-770) ----
+
+    @staticmethod
+    def mini_rows(balls: list, minis: list, world: int, time: int = 0) -> tuple:
+        """A world's minis as a client's Destiny makes them for fixed parents at the load (make_minis,
+        vec_collide.py:372; full.py:1336-1357, 1476-1536), each taking the next local id (LOCAL_BALLS - 1, - 2, ... in
+        the order of `minis`; a free or missing parent's take none): (the miniballs' `load` rows, the minicapsules'
+        and miniboxes' `load_statics` records). `balls` are the world's `load` rows, `minis` records {"kind": "ball",
+        "parent", "offset", "radius"}, {"kind": "capsule", "parent", "a", "b", "radius"} (offsets from the parent)
+        or {"kind": "box", "parent", "corner", "lx", "ly", "lz"} (the corner an offset). A miniball: fixed, massive,
+        not interactive, MINIBALL, the parent's mass, the radius a float, maximum speed, agility and speed fraction 0,
+        at the parent's position plus its offset, owned by the parent."""
+        by_id = {int(b["id"]): b for b in balls}
+        local, rows, statics = LOCAL_BALLS, [], []
+        for m in minis:
+            par = by_id.get(int(m["parent"]))
+            if par is None or par.get("free", True):
+                continue
+            local -= 1
+            pp = [float(c) for c in par["pos"]]
+            add = lambda o: [pp[i] + float(o[i]) for i in range(3)]
+            if m["kind"] == "ball":
+                rows.append({"id": local, "world": world, "time": time, "mass": float(par["mass"]),
+                             "radius": float(m["radius"]), "max_vel": 0.0, "pos": add(m["offset"]),
+                             "vel": [0.0, 0.0, 0.0], "agility": 0.0, "speed_fraction": 0.0, "free": False,
+                             "massive": True, "interactive": False, "cloaked": 0, "bubble": -1, "moribund": False,
+                             "mode": MINIBALL, "owner": int(par["id"])})
+            elif m["kind"] == "capsule":
+                statics.append({"kind": "capsule", "id": local, "parent": int(par["id"]), "a": add(m["a"]),
+                                "b": add(m["b"]), "radius": float(m["radius"])})
+            else:
+                statics.append({"kind": "box", "id": local, "parent": int(par["id"]), "corner": add(m["corner"]),
+                                "lx": list(m["lx"]), "ly": list(m["ly"]), "lz": list(m["lz"])})
+        return rows, statics
+
+    def set_harmonic(self, mask: torch.Tensor, harmonic: torch.Tensor, corporation: torch.Tensor,
+                     alliance: torch.Tensor, is_field: torch.Tensor) -> None:
+        """SetBallHarmonic (space.py:1272-1282) for the balls in `mask`: the harmonic, corporation and alliance; a
+        force field is stopped and set to FIELD; clearing a field stops it. The first call gives the park its keys
+        (-1 for every ball before)."""
+        mask = mask.to(self.dev)
+        if self._fld is None:
+            self._fld = torch.full((3, self.stride), -1, dtype=I64, device=self.dev)
+            self._state = None
+        for k, v in enumerate((harmonic, corporation, alliance)):
+            self._fld[k, :self.n] = torch.where(mask, v.to(device=self.dev, dtype=I64), self._fld[k, :self.n])
+        field = is_field.to(self.dev) != 0
+        self._stop_rows(mask & (field | (self.mode == FIELD)))
+        self.mode = torch.where(mask & field, FIELD, self.mode)
+
+    @property
+    def harmonic(self) -> torch.Tensor:
+        return self._fld[0, :self.n] if self._fld is not None else torch.full((self.n,), -1, dtype=I64,
+                                                                               device=self.dev)
 
     # ---- mushrooms (VecCombatPark's API, src/exact_ballpark/vec_combat.py:540-613; full.py:1826-1846) ----
 

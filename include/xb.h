@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 6
+#define XB_ABI_VERSION 7
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -96,6 +96,10 @@ typedef struct xb_modes_state {
     const int32_t* form_len;
     int32_t n_form;
     int32_t form_stride;
+    /* The force fields' keys (SetBallHarmonic), or NULL for none: int64 [3][stride], each row's harmonic, corporation
+       and alliance (-1 for none). A candidate in mode FIELD is left out of the gradient when the responder's harmonic
+       is -2 or one of its keys (not -1) is the responder's. */
+    const int64_t* fld;
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */

@@ -20,6 +20,7 @@ LAUNCH_OP, MFOLLOW_OP = 14, 15
 SETFORM_OP, FFOLLOW_OP = 16, 17
 CLOAK_OP, UNCLOAK_OP = 18, 19
 MUSHROOM_OP, FREE_OP = 20, 21
+HARMONIC_OP = 22
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -86,6 +87,12 @@ def lib() -> ctypes.CDLL:
         L.or_uncloak_ball.argtypes = [vp, i64]
         L.or_add_mushroom.argtypes = [vp, i64, d, d]
         L.or_set_free.argtypes = [vp, i64, i]
+        L.or_add_miniball.argtypes = [vp, i64, pd, d]
+        L.or_add_mini_capsule.argtypes = [vp, i64, pd, pd, d]
+        L.or_add_mini_box.argtypes = [vp, i64, pd, pd, pd, pd]
+        for f in (L.or_add_miniball, L.or_add_mini_capsule, L.or_add_mini_box):
+            f.restype = i64
+        L.or_set_harmonic.argtypes = [vp, i64, i64, i64, i64, i]
         _LIB = L
     return _LIB
 
@@ -122,6 +129,18 @@ class World:
         if self.dynamical:                             # AddDynamicallyOrientedBall and the AddBall's snap
             self.L.or_add_turning(self.p, int(b["id"]), float(b.get("max_ang_vel", 1.0)),
                                   float(b.get("rot_agility", 0.9)))
+
+    def add_mini(self, m: dict) -> None:
+        """A fixed parent's mini, made at once and taking the next local id: {"kind": "ball", "parent", "offset",
+        "radius"}, {"kind": "capsule", "parent", "a", "b", "radius"} (offsets from the parent) or {"kind": "box",
+        "parent", "corner", "lx", "ly", "lz"} (the corner an offset)."""
+        v = lambda x: (ctypes.c_double * 3)(*[float(c) for c in x])
+        if m["kind"] == "ball":
+            self.L.or_add_miniball(self.p, int(m["parent"]), v(m["offset"]), float(m["radius"]))
+        elif m["kind"] == "capsule":
+            self.L.or_add_mini_capsule(self.p, int(m["parent"]), v(m["a"]), v(m["b"]), float(m["radius"]))
+        else:
+            self.L.or_add_mini_box(self.p, int(m["parent"]), v(m["corner"]), v(m["lx"]), v(m["ly"]), v(m["lz"]))
 
     def add_static(self, s: dict) -> None:
         """AddCapsule ({"kind": "capsule", "id", "parent", "a", "b", "radius"}) or AddOrientedBox ({"kind": "box",
@@ -197,6 +216,8 @@ class World:
             L.or_add_mushroom(p, a, rr, x)
         elif op == FREE_OP:
             L.or_set_free(p, a, w)
+        elif op == HARMONIC_OP:                 # harmonic w, corporation b, alliance x, a force field if y
+            L.or_set_harmonic(p, a, w, b, int(x), int(y))
         else:
             raise ValueError(op)
 
@@ -260,6 +281,8 @@ def run_world(w: dict, record_every_step: bool = True):
         o.add(b)
     for st in w.get("statics", []):
         o.add_static(st)
+    for mini in w.get("minis", []):
+        o.add_mini(mini)
     if w.get("formations"):
         o.load_formations(w["formations"])
     for step in range(1, w["steps"] + 1):
@@ -295,6 +318,8 @@ def time_worlds(worlds: list, ticks: int) -> tuple:
             o.add(b)
         for st in w.get("statics", []):
             o.add_static(st)
+        for mini in w.get("minis", []):
+            o.add_mini(mini)
         if w.get("formations"):
             o.load_formations(w["formations"])
         for c in w["cmds"]:

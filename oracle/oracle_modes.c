@@ -52,6 +52,7 @@ typedef struct {
     int64_t follow_ptr;                        /* the target's id; -1 for none (modes.py: None) */
     int is_free, is_massive, is_interactive, is_moribund, sensor;
     int64_t cloaked, bubble;
+    int64_t harmonic, corporation, alliance;   /* SetBallHarmonic's (-1 at AddBall) */
     int nf;
     int64_t followers[MAX_FOLLOWERS];
     /* dynamical orientation (full.py): floats held in double */
@@ -95,6 +96,7 @@ typedef struct {
     int collisions;                            /* SpaceBallpark: the partition and the gradient method */
     int ypr;                                   /* FullBallpark's yaw, pitch and roll without dynamical orientation */
     int overflow;                              /* a collision list past MAX_HITS */
+    int64_t local_hi;                          /* GetIDForCollisionObject(-1)'s last local id on a client */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
     int ns;
@@ -216,6 +218,7 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->cloaked = cloaked, b->bubble = bubble, b->is_moribund = is_moribund;
     b->rot[3] = b->old_rot[3] = 1.0;                                 /* Ball::Ball: identity, at rest, 1 rad/s, 0.9 */
     b->formation_id = -1;
+    b->harmonic = b->corporation = b->alliance = -1;
     b->max_ang_vel = 1.0;
     b->rot_agility = (double)0.9f;
     refresh_time_factor(p, b);
@@ -1638,6 +1641,12 @@ static void explode_children(const OPark* p, const OBoxes* U, OBox x, int* out, 
    balls of those boxes by id, each once, and their static collidables by (parent id, id), each once. No ball here is a
    force field (the harmonic filter passes every ball). Returns the ball candidates' indices in order; the static
    candidates' indices go to `sout` (their count to *nsout). */
+/* BallSortComparer (full.py:888-890): two local (negative) ids by their owners first, otherwise by id. */
+static int ball_sort_before(const OBall* a, const OBall* b) {
+    if (a->id < 0 && b->id < 0) return a->owner_id < b->owner_id || (a->owner_id == b->owner_id && a->id < b->id);
+    return a->id < b->id;
+}
+
 static int nearby_balls(const OPark* p, OBoxes* U, int bi, int exclude_missiles, int* out, int* sout, int* nsout) {
     const OBall* b = &p->balls[bi];
     boxes_build(p, U);
@@ -1667,14 +1676,32 @@ static int nearby_balls(const OPark* p, OBoxes* U, int bi, int exclude_missiles,
     int handled[MAX_BALLS] = {0}, s_handled[MAX_STATICS] = {0};
     int cnt = 0;
     *nsout = 0;
+    const OBall* me = &p->balls[bi];
     for (int s = 0; s < ns; ++s) {
         OBox x = U->box[sbox[s]];
         U->handled[sbox[s]] = 0;
-        for (int i = 0; i < p->n; ++i) {                                /* balls kept by id: the box's order */
+        int inbox[MAX_BALLS], nb = 0;
+        for (int i = 0; i < p->n; ++i)                                  /* the box's balls, BallSortComparer's order */
+            if (ball_in_box(&p->balls[i], x)) {
+                int k = nb++;
+                while (k > 0 && ball_sort_before(&p->balls[i], &p->balls[inbox[k - 1]])) {
+                    inbox[k] = inbox[k - 1];
+                    --k;
+                }
+                inbox[k] = i;
+            }
+        for (int q = 0; q < nb; ++q) {
+            const int i = inbox[q];
             const OBall* o = &p->balls[i];
-            if (!ball_in_box(o, x) || handled[i] || i == bi) continue;
+            if (handled[i] || i == bi) continue;
             if (o->is_moribund) continue;
             if (o->cloaked || !o->is_massive) continue;
+            if (o->mode == FIELD) {                                     /* full.py:1162-1174 */
+                if (me->harmonic == -2) continue;
+                if (o->harmonic != -1 && o->harmonic == me->harmonic) continue;
+                if (o->corporation != -1 && o->corporation == me->corporation) continue;
+                if (o->alliance != -1 && o->alliance == me->alliance) continue;
+            }
             if (exclude_missiles && o->mode == MISSILE) continue;
             handled[i] = 1;
             out[cnt++] = i;
@@ -2139,4 +2166,74 @@ int or_maths(int func, int64_t n, const double* x, double* out) {
 void or_evolve_all(OPark** parks, int np, int ticks) {
     for (int i = 0; i < np; ++i)
         for (int t = 0; t < ticks; ++t) or_evolve(parks[i]);
+}
+
+/* ------------------------------------------------------------------------ minis and force fields (COL-713, MOV-737) */
+
+static int64_t next_local_id(OPark* p) {                /* GetIDForCollisionObject(-1) on a client (space.py:1083-1095) */
+    if (p->local_hi == 0) p->local_hi = -1073741824;    /* DSTLOCALBALLS */
+    p->local_hi -= 1;
+    return p->local_hi;
+}
+
+/* Ball::AddMiniBall through the thunker and Ballpark::AddMiniball for a fixed parent (full.py:1336-1357, 1476-1491): a
+   fixed, massive, non-interactive ball at the parent's position plus the offset, the parent's mass, the radius a
+   float, maximum speed, agility and speed fraction 0, owned by the parent, MINIBALL, the next local id; nothing for a
+   free or missing parent. Returns the id (0 for none). */
+int64_t or_add_miniball(OPark* p, int64_t parent, const double* off, double radius) {
+    OBall* o = find(p, parent);
+    if (!o || o->is_free) return 0;
+    radius = c_float(radius);
+    const int64_t id = next_local_id(p);
+    double pos[3] = {o->pos[0] + off[0], o->pos[1] + off[1], o->pos[2] + off[2]}, vel[3] = {0.0, 0.0, 0.0};
+    const double mass = o->mass;
+    if (or_add_ball(p, id, mass, radius, 0.0, pos, vel, 1.0, 0.0, 0, 1, 0, 0, -1, 0, STOP)) {
+        p->overflow = 1;
+        return 0;
+    }
+    OBall* m = find(p, id);
+    m->max_vel = 0.0, m->agility = 0.0, m->sf = 0.0;
+    refresh_time_factor(p, m);
+    m->owner_id = parent;
+    m->mode = MINIBALL;
+    return id;
+}
+
+/* PyAddMiniCapsule and AddCapsule(-1, parent, ...) for a fixed parent (full.py:1493-1512): the ends the parent's
+   position plus the offsets, the next local id. */
+int64_t or_add_mini_capsule(OPark* p, int64_t parent, const double* a, const double* b, double radius) {
+    OBall* o = find(p, parent);
+    if (!o || o->is_free) return 0;
+    const int64_t id = next_local_id(p);
+    double pa[3], pb[3];
+    for (int i = 0; i < 3; ++i) pa[i] = o->pos[i] + a[i], pb[i] = o->pos[i] + b[i];
+    if (or_add_capsule(p, id, parent, pa, pb, radius)) p->overflow = 1;
+    return id;
+}
+
+/* Ball::AddMiniBox and AddOrientedBox(-1, parent, ...) for a fixed parent (full.py:1514-1536): the corner the
+   parent's position plus its offset, the next local id. */
+int64_t or_add_mini_box(OPark* p, int64_t parent, const double* corner, const double* lx, const double* ly,
+                        const double* lz) {
+    OBall* o = find(p, parent);
+    if (!o || o->is_free) return 0;
+    const int64_t id = next_local_id(p);
+    double c[3];
+    for (int i = 0; i < 3; ++i) c[i] = corner[i] + o->pos[i];
+    if (or_add_oriented_box(p, id, parent, c, lx, ly, lz)) p->overflow = 1;
+    return id;
+}
+
+/* SetBallHarmonic (space.py:1272-1282): the harmonic, corporation and alliance; a force field stopped and FIELD;
+   clearing a field stops it. */
+void or_set_harmonic(OPark* p, int64_t id, int64_t harmonic, int64_t corporation, int64_t alliance, int is_field) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    b->harmonic = harmonic, b->corporation = corporation, b->alliance = alliance;
+    if (is_field) {
+        stop_ball(p, b);
+        b->mode = FIELD;
+    } else if (b->mode == FIELD) {
+        stop_ball(p, b);
+    }
 }

@@ -1,7 +1,9 @@
 #!/usr/bin/env python3
 """Writes tests/golden/missile_golden.npz (with `--formations`, formation_golden.npz: `formation_cases`, LoadFormations,
 SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall;
-with `--mushrooms`, mushroom_golden.npz: `mushroom_cases`, AddMushroom and SetBallFree, ball 0 recorded once added): the reference's own scalar ballpark with missiles
+with `--mushrooms`, mushroom_golden.npz: `mushroom_cases`, AddMushroom and SetBallFree, ball 0 recorded once added;
+with `--minis`, mini_golden.npz: `mini_cases`, a fixed parent's miniballs, minicapsules and miniboxes, and force fields
+by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -52,6 +54,8 @@ def _apply(park, c) -> None:
         park.add_mushroom(a, rr, x)
     elif op == S.FREE_OP:
         park.set_free(a, bool(w))
+    elif op == S.HARMONIC_OP:
+        park.set_harmonic(a, w, b, int(x), bool(y))
     else:
         C._apply(park, c)
 
@@ -74,10 +78,16 @@ def run_world(w: dict) -> list:
         if b["cloaked"]:
             park.set_cloaked(b["id"], b["cloaked"])
         park.balls[b["id"]].bubble = b["bubble"]
+    for m in w.get("minis", []):                   # each taking the next local id, in this order
+        if m["kind"] == "ball":
+            park.add_mini_ball(m["parent"], *m["offset"], m["radius"])
+        elif m["kind"] == "capsule":
+            park.add_mini_capsule(m["parent"], *m["a"], *m["b"], m["radius"])
+        else:
+            park.add_mini_box(m["parent"], *m["corner"], *m["lx"], *m["ly"], *m["lz"])
     if w.get("formations"):
         park.load_formations(w["formations"])
-    known = {b["id"] for b in w["balls"]} | {0}        # the world's balls and a client's mushroom (ball 0)
-    order = sorted(bid for bid in park.balls if bid in known)
+    order = sorted(park.balls)                      # the world's balls, its miniballs, a client's mushroom (ball 0)
     out = []
     for k, bid in enumerate(order):
         d, q = _state(park.balls[bid], do)
@@ -87,7 +97,7 @@ def run_world(w: dict) -> list:
             if c[0] == step:
                 _apply(park, c)
         park.evolve()
-        order = sorted(bid for bid in park.balls if bid in known)
+        order = sorted(park.balls)
         for k, bid in enumerate(order):
             d, q = _state(park.balls[bid], do)
             out.append((step, k, d, q, list(park.balls[bid].collisions)))
@@ -102,6 +112,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
         ws = S.do_cloak_cases(seed) if do else S.cloak_cases(seed)
     elif what == "mushroom":
         ws = S.do_mushroom_cases(seed) if do else S.mushroom_cases(seed)
+    elif what == "mini":
+        ws = S.do_mini_cases(seed) if do else S.mini_cases(seed)
     else:
         ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
     if not do:
@@ -141,4 +153,4 @@ if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     main(int(args[0]) if args else 1, do="--do" in sys.argv,
          what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv
-         else "mushroom" if "--mushrooms" in sys.argv else "missile")
+         else "mushroom" if "--mushrooms" in sys.argv else "mini" if "--minis" in sys.argv else "missile")

@@ -85,6 +85,8 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.add_mushroom(mask, tid, rr, xyz[:, 0])
     elif op == S.FREE_OP:
         park.set_free(mask, w)
+    elif op == S.HARMONIC_OP:                       # harmonic w, corporation b, alliance x, a field if y
+        park.set_harmonic(mask, w, tid, xyz[:, 0].to(I64), xyz[:, 1])
     elif op == S.CLOAK_OP:
         park.cloak(mask, w)
     elif op == S.UNCLOAK_OP:
@@ -145,20 +147,24 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                 repr(w.get("formations")))].append(wi)       # one formation table a park, as LoadFormations keeps
     out = {}
     for (friction, tick, new_orbit, dynamical, coll, ypr, _), members in groups.items():
+        minis = {wi: VecModesPark.mini_rows(worlds[wi]["balls"], worlds[wi]["minis"], wi, worlds[wi]["start"])
+                 for wi in members if worlds[wi].get("minis")}
         balls = [(wi, b) for wi in members
-                 for b in worlds[wi]["balls"] + ([None] if any(c[1] == S.MUSHROOM_OP for c in worlds[wi]["cmds"])
-                                                 else [])]     # a world's empty row for its mushroom
+                 for b in worlds[wi]["balls"] + [{"_row": m, "id": m["id"]} for m in minis.get(wi, ([], []))[0]]
+                 + ([None] if any(c[1] == S.MUSHROOM_OP for c in worlds[wi]["cmds"])
+                    else [])]     # the world's miniballs; its empty row for a mushroom
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
                             use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
                             display_ypr=ypr)
-        park.load([VecModesPark.empty_row(wi, worlds[wi]["start"]) if b is None else
+        park.load([VecModesPark.empty_row(wi, worlds[wi]["start"]) if b is None else b["_row"] if "_row" in b else
                    {"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                     "speed_fraction": b["sf"], "free": b["free"], "massive": b["massive"],
                     "interactive": b["interactive"], "cloaked": b["cloaked"], "bubble": b["bubble"],
                     "moribund": b["moribund"], "mode": b["mode"], "max_ang_vel": b.get("max_ang_vel", 1.0),
                     "rot_agility": b.get("rot_agility", 0.9), "junk": b.get("junk", False)} for wi, b in balls])
-        statics = {wi: worlds[wi]["statics"] for wi in members if worlds[wi].get("statics")}
+        statics = {wi: worlds[wi].get("statics", []) + minis.get(wi, ([], []))[1] for wi in members
+                   if worlds[wi].get("statics") or minis.get(wi, ([], []))[1]}
         if statics:
             park.load_statics(statics)
         if worlds[members[0]].get("formations") is not None:
@@ -226,6 +232,8 @@ def _run_flags(w: dict, initial: bool = False):
         o.add(b)
     for s in w.get("statics", []):
         o.add_static(s)
+    for mini in w.get("minis", []):
+        o.add_mini(mini)
     if w.get("formations"):
         o.load_formations(w["formations"])
     by_step = defaultdict(list)

@@ -1232,3 +1232,136 @@ def do_mushroom_cases(seed: int, scale: int = 1) -> list:
             b["max_ang_vel"] = r.uniform(0.2, 2.0)
             b["rot_agility"] = r.uniform(0.3, 3.0)
     return out
+
+
+HARMONIC_OP = 22                # SetBallHarmonic(ball a, harmonic w, corporation b, alliance int(x), a field if y)
+
+
+def mini_cases(seed: int, scale: int = 1) -> list:
+    """The minis and force field worlds (about 140 x scale) for seed, with collisions. Minis: fixed parents (spheres)
+    with 1-4 miniballs (offsets within twice the parent's radius), 0-2 minicapsules and 0-2 miniboxes (COL-713: each
+    the next local id, a miniball fixed and massive with the parent's mass, MINIBALL, owned by the parent), a free
+    parent's minis refused; ships flying through them and at rest among them (several local balls in one box: the
+    owners' order first), missiles launched from a parent (its own miniballs skipped). Force fields: a ball made a
+    field (SetBallHarmonic: stopped, FIELD), ships with its harmonic, corporation or alliance passing through it, with
+    harmonic -2 passing every field, others meeting it; a field cleared (stopped) mid-run."""
+    r = random.Random(seed ^ 0x313)
+    out: list = []
+
+    def centre():
+        return [r.uniform(-1e6, 1e6) for _ in range(3)]
+
+    def off(scale_):
+        return [r.uniform(-scale_, scale_) for _ in range(3)]
+
+    for _ in range(80 * scale):                          # minis
+        c = centre()
+        n = r.randint(3, 6)
+        ids = _ids(r, n, big=r.random() < 0.2)
+        spread = r.choice([1500.0, 4000.0])
+        balls, cmds, minis = [], [], []
+        n_par = r.randint(1, 2)
+        twin = n_par == 2 and r.random() < 0.4           # two parents at one place: their miniballs share boxes
+        for k, bid in enumerate(ids):
+            if k < n_par:
+                balls.append(_sphere(r, bid, c, 30.0 if twin else spread * 0.5))
+                if twin:
+                    balls[-1]["radius"] = r.uniform(10.0, 50.0)
+                continue
+            if twin:                                    # ships close by, flying at the twins' centre
+                balls.append(_coll_ball(r, bid, c, 700.0, max_vel=r.uniform(100.0, 300.0), radius=r.uniform(5.0, 60.0)))
+                cmds.append(_cmd(1, GOTO_OP, bid, x=c[0], y=c[1], z=c[2]))
+                continue
+            balls.append(_coll_ball(r, bid, c, spread, max_vel=r.uniform(50.0, 500.0)))
+            if r.random() < 0.7:
+                p = [c[i] + r.uniform(-spread, spread) for i in range(3)]
+                cmds.append(_cmd(r.randint(1, 3), GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        parents = [b for b in balls if not b["free"]]
+        if r.random() < 0.1:                            # a free parent: its minis refused
+            parents.append(r.choice([b for b in balls if b["free"]]))
+        for par in parents:
+            rad = par["radius"]
+            for _ in range(r.randint(2, 4) if twin else r.randint(1, 4)):
+                minis.append({"kind": "ball", "parent": par["id"],
+                              "offset": off(40.0) if twin else off(2.0 * rad),
+                              "radius": r.uniform(90.0, 110.0) if twin else r.uniform(10.0, max(20.0, rad))})
+            for _ in range(r.choice([0, 0, 1, 2])):
+                minis.append({"kind": "capsule", "parent": par["id"], "a": off(1.5 * rad), "b": off(1.5 * rad),
+                              "radius": r.uniform(5.0, 0.5 * rad + 10.0)})
+            for _ in range(r.choice([0, 0, 1, 2])):
+                lx = off(r.uniform(50.0, rad + 100.0))
+                minis.append({"kind": "box", "parent": par["id"], "corner": off(rad), "lx": lx,
+                              "ly": off(r.uniform(50.0, rad + 100.0)), "lz": off(r.uniform(50.0, rad + 100.0))})
+        r.shuffle(minis)
+        kind = "minis_twin" if twin else "minis"
+        local, mini_ids = -1073741824, []                 # the local ids the minis take (a fixed parent's only)
+        fixed = {b["id"] for b in balls if not b["free"]}
+        for m in minis:
+            if m["parent"] in fixed:
+                local -= 1
+                if m["kind"] == "ball":
+                    mini_ids.append((m["parent"], local))
+        if r.random() < 0.35 and len(balls) > n_par:    # a missile from a parent at a ship, or at its own miniball
+            par = parents[0]
+            mid = max(ids) + 1
+            balls.append(_coll_ball(r, mid, c, spread, radius=r.uniform(1.0, 20.0), mass=10 ** r.uniform(3, 5),
+                                    max_vel=r.uniform(500.0, 2000.0), sf=1.0, vel=[0.0, 0.0, 0.0]))
+            tgt = r.choice([b["id"] for b in balls if b["free"] and b["id"] != mid])
+            own = [i for p_, i in mini_ids if p_ == par["id"]]
+            kind = "minis_missile"
+            if own and r.random() < 0.5:
+                tgt = r.choice(own)
+                balls[-1]["bubble"] = -1                # a miniball's bubble: MissileFollow takes it as target
+                kind = "minis_missile_at_own"
+            cmds.append(_launch(r.choice([1, 2]), mid, tgt, par["id"], r.choice([0, 1]), 1))
+        out.append({"kind": kind, "friction": r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]),
+                    "tick": r.choice([1000, 1000, 500, 2000]), "steps": r.randint(8, 30), "new_orbit": False,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]),
+                    "collisions": True, "minis": minis})
+
+    for _ in range(60 * scale):                          # force fields
+        c = centre()
+        n = r.randint(3, 7)
+        ids = _ids(r, n, big=r.random() < 0.2)
+        field = ids[0]
+        balls = [_sphere(r, field, c, 0.0) if r.random() < 0.7 else _coll_ball(r, field, c, 0.0)]
+        balls[0]["radius"] = r.uniform(500.0, 5000.0)
+        h, corp, ally = r.randint(1, 5), r.randint(100, 105), r.choice([-1, r.randint(200, 203)])
+        cmds = [_cmd(1, HARMONIC_OP, field, corp, x=ally, y=1, w=h)]
+        for bid in ids[1:]:
+            b = _coll_ball(r, bid, c, 3.0 * balls[0]["radius"], max_vel=r.uniform(100.0, 600.0))
+            balls.append(b)
+            x = r.random()
+            if x < 0.2:
+                cmds.append(_cmd(1, HARMONIC_OP, bid, r.randint(100, 105), x=-1, y=0, w=h))
+            elif x < 0.35:
+                cmds.append(_cmd(1, HARMONIC_OP, bid, corp, x=-1, y=0, w=r.randint(1, 5)))
+            elif x < 0.45:
+                cmds.append(_cmd(1, HARMONIC_OP, bid, r.randint(100, 105), x=ally, y=0, w=r.randint(1, 5)))
+            elif x < 0.55:
+                cmds.append(_cmd(1, HARMONIC_OP, bid, -1, x=-1, y=0, w=-2))
+            elif x < 0.75:
+                cmds.append(_cmd(1, HARMONIC_OP, bid, r.randint(100, 105), x=-1, y=0, w=r.randint(1, 5)))
+            p = [c[i] + r.uniform(-0.3, 0.3) * balls[0]["radius"] for i in range(3)]
+            cmds.append(_cmd(r.randint(1, 3), GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        kind = "field"
+        if r.random() < 0.2:
+            cmds.append(_cmd(r.randint(4, 10), HARMONIC_OP, field, corp, x=ally, y=0, w=h))
+            kind = "field_cleared"
+        out.append({"kind": kind, "friction": r.choice([1.0e6, 1.0e6, 10 ** r.uniform(4.0, 7.0)]),
+                    "tick": r.choice([1000, 1000, 500, 2000]), "steps": r.randint(10, 35), "new_orbit": False,
+                    "start": r.randrange(0, 2_000_000), "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]),
+                    "collisions": True})
+    return out
+
+
+def do_mini_cases(seed: int, scale: int = 1) -> list:
+    """mini_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
+    r = random.Random(seed ^ 0x31D)
+    out = mini_cases(seed, scale)
+    for w in out:
+        w["dynamical"] = True
+        for b in w["balls"]:
+            b["max_ang_vel"] = r.uniform(0.2, 2.0)
+            b["rot_agility"] = r.uniform(0.3, 3.0)
+    return out

@@ -1,0 +1,47 @@
+"""The oracle's minis and force fields (COL-713, MOV-737): a fixed parent's miniballs (local ids, fixed and massive
+with the parent's mass, MINIBALL, owned by the parent), minicapsules and miniboxes (static collidables with the
+parent); GetNearbyBalls' BallSortComparer order within a box (two local ids by their owners first); a missile skipping
+its owner's miniballs; SetBallHarmonic and the candidates' force field filter (harmonic -2 passes, or the field's
+harmonic, corporation or alliance the ball's). Pinned to the reference: every ball of the mini and field worlds
+(tests/scenarios.py `mini_cases`) as added and after every tick, bit for bit with mLastC, the filing, the collision
+lists and the display yaw, pitch and roll, against the vectors the reference's own FullBallpark wrote
+(tests/golden/make_golden_missile.py --minis); and with dynamical orientation (mini_do_golden.npz)."""
+
+import pytest
+
+import golden_fixture as G
+from test_oracle_missile_golden import check
+
+MODE, FIELD, MINIBALL = 1, 10, 9
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return G.load("mini_golden.npz")
+
+
+def test_oracle_minis_and_fields_match_reference(golden):
+    bad = check(golden, False)
+    assert not bad, f"{len(bad)} worlds differ, first {bad[:6]}"
+
+
+def test_oracle_minis_and_fields_with_orientation_match_reference():
+    bad = check(G.load("mini_do_golden.npz"), True)
+    assert not bad, f"{len(bad)} worlds differ, first {bad[:6]}"
+
+
+def test_fixture_covers_minis_and_fields(golden):
+    """Miniballs met by ships, two local balls in one list, fields passed (a ball inside a field's radius that does
+    not list it) and met."""
+    met_mini = two_local = field_met = 0
+    for (wi, step), (d, q, hits) in golden["records"].items():
+        ids = q[:, 0]
+        modes = dict(zip(ids.tolist(), q[:, MODE].tolist()))
+        for h in hits:
+            locals_ = [x for x in h if x < 0]
+            met_mini += int(bool(locals_))
+            two_local += int(len(locals_) >= 2)
+            field_met += int(any(modes.get(x) == FIELD for x in h))
+    kinds = {golden["worlds"][wi]["kind"] for wi in golden["kept"]}
+    assert met_mini >= 100 and two_local >= 20 and field_met >= 50, (met_mini, two_local, field_met)
+    assert {"minis", "minis_missile", "field", "field_cleared"} <= kinds

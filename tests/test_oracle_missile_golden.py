@@ -19,6 +19,8 @@ def run(w, do=False):
     o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], do, True, not do)
     for b in w["balls"]:
         o.add(b)
+    for mini in w.get("minis", []):
+        o.add_mini(mini)
     if w.get("formations"):
         o.load_formations(w["formations"])
     by_step = defaultdict(list)
