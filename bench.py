@@ -355,6 +355,8 @@ def main() -> int:
         out["serving"], srv = serving(a)
     if not a.no_cpu:
         out["cpu_baseline"], out["exact"] = cpu_baseline(p["_park"], p["_wl"], p["_ticks_done"], a.cpu_seconds)
+        if a.workload == "composed":
+            out["host_batched"] = host_batched(a.cpu_seconds / 2)
         if srv is not None:
             out["serving"]["cpu_baseline"], out["serving"]["exact"] = cpu_baseline_serving(srv)
     print(json.dumps(out), flush=True)
@@ -403,6 +405,34 @@ def serving(a) -> tuple:
                                 "max": int(wait.max())}}
     print(f"# serving: {st}", file=sys.stderr, flush=True)
     return out, srv
+
+
+def host_batched(seconds: float, worlds: int = 4096, ticks: int = 64) -> dict:
+    """The CPU batched path (the tick kernels' source built for the host, libxb_cpu.so, OpenMP over tiles) on the
+    composed workload's generator at `worlds` worlds (seed 1, 0-8 missiles a world): the same `ticks` ticks after one
+    warm-up tick (fewer when one core would pass `seconds`), on every host core the OpenMP runtime is given and on
+    one core."""
+    import os
+
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    wl = WL.composed_park(1, worlds, missiles=8)
+    out = {"workload": f"ragged composed park, {worlds} worlds ({wl['n']} rows), seed 1, 0-8 missiles a world"}
+    for label, threads in (("one_core", 1), ("all_cores", 0)):     # one core first: it sets the ticks
+        park = VecModesPark(wl["n"], device="cpu", block=64, dynamical_orientation=True, collisions=True)
+        WL.load_flight_park(park, wl)
+        park.cpu_threads = threads
+        park.evolve()
+        n, t0 = 0, time.perf_counter()
+        while n < ticks and (threads != 1 or time.perf_counter() - t0 < seconds):
+            park.evolve()
+            n += 1
+        ticks = n
+        dt = time.perf_counter() - t0
+        cores = 1 if threads == 1 else int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+        out[label] = {"value": worlds * n / dt, "unit": "world-ticks/s", "cores": cores, "ticks": n}
+    print(f"# host batched: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline_serving(srv) -> tuple:

@@ -1199,7 +1199,7 @@ class VecModesPark(metaclass=_Fields):
                 rc = N.hip().xb_modes_tick(ctypes.byref(s), torch.cuda.current_stream(self.dev).cuda_stream, b)
             N.check(rc, "xb_modes_tick")
         else:
-            rc = N.cpu().xb_cpu_modes_tick(ctypes.byref(s), 0)
+            rc = N.cpu().xb_cpu_modes_tick(ctypes.byref(s), int(getattr(self, "cpu_threads", 0)))
             if rc:
                 raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
 
