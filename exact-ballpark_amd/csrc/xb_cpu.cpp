@@ -230,9 +230,17 @@ extern "C" {
 
 int xb_cpu_abi(void) { return XB_ABI_VERSION; }
 
+// OpenMP's default team size, read once before any call sets a size of its own (a call's `threads` applies to that
+// call only).
+static int default_threads() {
+    static const int n = omp_get_max_threads();
+    return n;
+}
+
 // One tick of the park in host memory; tiles as xb_modes_tick's (NULL: the whole park is one group). threads <= 0:
 // OpenMP's default.
 int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
+    const int team = threads > 0 ? threads : default_threads();
     if (!state || state->n < 0 || state->stride < state->n) return -1;
     if (state->n == 0) return 0;
     if (state->dynamical && state->ypr) return -1;
@@ -246,8 +254,7 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
             return -1;
         const int nt = state->n_tiles;
         const bool dyn = state->dynamical != 0;
-        if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel
+#pragma omp parallel num_threads(team)
         {
             Scratch S;
             std::vector<XbRow> rows;
@@ -267,8 +274,7 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
         return 0;
     }
     const int nt = state->n_tiles;
-    if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel
+#pragma omp parallel num_threads(team)
     {
         Scratch S;
         std::vector<XbRow> rows;
