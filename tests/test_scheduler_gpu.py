@@ -5,7 +5,7 @@
   restatement, within stated float32 tolerances; the backward pass repeated is bit-identical (no atomics);
 - one Adam step against the oracle's, from the same gradient;
 - training on the park's own recorded ticks lowers the loss and beats the majority bucket;
-- the learned regrouping ticks every world to the same bits as the unsorted order (the order is the judge);
+- the learned regrouping ticks every world to the same bits as the unsorted order (the unsorted order is the reference);
 - the trained network's regrouping lowers the modeled launch cost and the measured time per tick against the unsorted
   order on the same worlds (measured figures printed; the modeled one asserted).
 """
