@@ -742,6 +742,32 @@ class VecModesPark(metaclass=_Fields):
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
 
+    # ---- collision events ----
+
+    def collision_events(self) -> list:
+        """Ball::DispatchCollisions of the last tick (full.py:2571-2583; COL-719), a client's view: for each live
+        missile that moved at least sqrt(0.1) m in it, a DoCollision event per entry of its collision list, as
+        (world, time, missile id, collider id, mLastC x, y, z), worlds and balls by ascending id, each list in its
+        order. Read after each one-tick launch: a launch of several ticks leaves its last tick's only."""
+        if not self.collisions or self.n == 0:
+            return []
+        d = self.pos - self.old_pos
+        sq = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        rows = torch.nonzero(self.free & ~self.moribund & (self.mode == MISSILE) & ~(sq < 0.1)
+                             & (self.hits_n > 0)).reshape(-1)
+        if rows.numel() == 0:
+            return []
+        r = rows.cpu().tolist()
+        world, ids, clock = (x[rows].cpu().tolist() for x in (self.world, self.ids, self.clock))
+        nh = self.hits_n[rows].cpu().tolist()
+        lc = self.last_c[rows].cpu().tolist()
+        hits = self._hits[:, rows].cpu().tolist()
+        out = []
+        for k in sorted(range(len(r)), key=lambda k: (world[k], ids[k])):
+            for h in range(nh[k]):
+                out.append((world[k], clock[k] - 1, ids[k], hits[h][k], *lc[k]))
+        return out
+
     # ---- minis and force fields (VecCollidePark's, src/exact_ballpark/vec_collide.py:364-420, 748-770) ----
 
     @staticmethod

@@ -93,6 +93,7 @@ def lib() -> ctypes.CDLL:
         for f in (L.or_add_miniball, L.or_add_mini_capsule, L.or_add_mini_box):
             f.restype = i64
         L.or_set_harmonic.argtypes = [vp, i64, i64, i64, i64, i]
+        L.or_take_events.argtypes = [vp, pq, pd, i]
         _LIB = L
     return _LIB
 
@@ -129,6 +130,13 @@ class World:
         if self.dynamical:                             # AddDynamicallyOrientedBall and the AddBall's snap
             self.L.or_add_turning(self.p, int(b["id"]), float(b.get("max_ang_vel", 1.0)),
                                   float(b.get("rot_agility", 0.9)))
+
+    def take_events(self) -> list:
+        """The DoCollision events since the last call: (time, missile id, collider id, mLastC x, y, z)."""
+        q = (ctypes.c_int64 * (3 * 512))()
+        c = (ctypes.c_double * (3 * 512))()
+        n = self.L.or_take_events(self.p, q, c, 512)
+        return [(q[3 * i], q[3 * i + 1], q[3 * i + 2], c[3 * i], c[3 * i + 1], c[3 * i + 2]) for i in range(n)]
 
     def add_mini(self, m: dict) -> None:
         """A fixed parent's mini, made at once and taking the next local id: {"kind": "ball", "parent", "offset",

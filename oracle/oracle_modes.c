@@ -37,6 +37,7 @@ enum { GOTO = 0, FOLLOW = 1, STOP = 2, WARP = 3, ORBIT = 4, MISSILE = 5, MUSHROO
 #define STOP_DAMP 0.9345794392523364485981308411215   /* ref.py:37 */
 #define MAX_FOLLOWERS 64
 #define MAX_BALLS 256
+#define MAX_EVENTS 512
 #define MAX_HITS 64                            /* a tick's collision list; more sets the park's overflow flag */
 #define NUM_LEVELS 8                           /* space.py:69-77 */
 #define BIGBOX (65536.0 * 480.0 * 0.25)
@@ -97,6 +98,9 @@ typedef struct {
     int ypr;                                   /* FullBallpark's yaw, pitch and roll without dynamical orientation */
     int overflow;                              /* a collision list past MAX_HITS */
     int64_t local_hi;                          /* GetIDForCollisionObject(-1)'s last local id on a client */
+    int n_ev;                                  /* DoCollision events since the last read (at most MAX_EVENTS) */
+    int64_t ev_q[MAX_EVENTS][3];               /* time, missile id, collider id */
+    double ev_c[MAX_EVENTS][3];                /* the missile's mLastC */
     int n;
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
     int ns;
@@ -2078,7 +2082,20 @@ void or_evolve(OPark* p) {                                              /* modes
         }
         if (p->dynamical) commit_rotation(p, b);
         else if (p->ypr) yaw_pitch_roll(p, b, 0);                       /* full.py:2565-2567 */
-        if (!(lensq(d) < 0.1)) refile(p, b);                            /* modes.py:737-743 */
+        if (lensq(d) < 0.1) continue;                                   /* modes.py:737-743 */
+        refile(p, b);
+        if (b->mode == MISSILE) {                       /* Ball::DispatchCollisions (full.py:2571-2583) */
+            for (int h = 0; h < b->nh; ++h) {
+                if (p->n_ev >= MAX_EVENTS) {
+                    p->overflow = 1;
+                    break;
+                }
+                int64_t* q = p->ev_q[p->n_ev];
+                q[0] = p->current_time, q[1] = b->id, q[2] = b->hits[h];
+                for (int k = 0; k < 3; ++k) p->ev_c[p->n_ev][k] = b->last_c[k];
+                p->n_ev++;
+            }
+        }
     }
     p->current_time += 1;                                               /* (7) */
 }
@@ -2236,4 +2253,14 @@ void or_set_harmonic(OPark* p, int64_t id, int64_t harmonic, int64_t corporation
     } else if (b->mode == FIELD) {
         stop_ball(p, b);
     }
+}
+
+/* The DoCollision events since the last call (time, missile id, collider id; its mLastC), in Evolve's order; the
+   buffer is emptied. Returns the count. */
+int or_take_events(OPark* p, int64_t* q, double* c, int max) {
+    const int n = p->n_ev < max ? p->n_ev : max;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) q[3 * i + k] = p->ev_q[i][k], c[3 * i + k] = p->ev_c[i][k];
+    p->n_ev = 0;
+    return n;
 }

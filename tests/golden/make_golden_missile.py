@@ -88,7 +88,8 @@ def run_world(w: dict) -> list:
     if w.get("formations"):
         park.load_formations(w["formations"])
     order = sorted(park.balls)                      # the world's balls, its miniballs, a client's mushroom (ball 0)
-    out = []
+    out, events = [], []
+    park.events.clear()
     for k, bid in enumerate(order):
         d, q = _state(park.balls[bid], do)
         out.append((0, k, d, q, list(park.balls[bid].collisions)))
@@ -97,11 +98,13 @@ def run_world(w: dict) -> list:
             if c[0] == step:
                 _apply(park, c)
         park.evolve()
+        events.extend(e for e in park.events if e[1] == "DoCollision")
+        park.events.clear()
         order = sorted(park.balls)
         for k, bid in enumerate(order):
             d, q = _state(park.balls[bid], do)
             out.append((step, k, d, q, list(park.balls[bid].collisions)))
-    return out
+    return out, events
 
 
 def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
@@ -121,13 +124,18 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
             w["ypr"] = True
     kept, skipped = [], []
     rec_world, rec_step, rec_ball, rec_d, rec_q, hit_off, hit_ids = [], [], [], [], [], [0], []
+    ev_world, ev_q, ev_c = [], [], []                # DoCollision: (time, missile id, collider id), mLastC
     for wi, w in enumerate(ws):
         try:
-            rows = run_world(w)
+            rows, evs = run_world(w)
         except (ZeroDivisionError, ValueError, OverflowError, C.space.DestinyUndefined, NotImplementedError) as e:
             skipped.append([wi, type(e).__name__, str(e)[:80]])
             continue
         kept.append(wi)
+        for e in evs:
+            ev_world.append(wi)
+            ev_q.append([e[0], e[2], e[3]])
+            ev_c.append([e[4], e[5], e[6]])
         for step, k, d, q, hits in rows:
             rec_world.append(wi)
             rec_step.append(step)
@@ -143,8 +151,11 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
                         world=np.array(rec_world, dtype=np.int32), step=np.array(rec_step, dtype=np.int32),
                         ball=np.array(rec_ball, dtype=np.int32), d=d, q=np.array(rec_q, dtype=np.int64),
                         hit_off=np.array(hit_off, dtype=np.int64), hit_ids=np.array(hit_ids, dtype=np.int64),
-                        doubles=np.array(names), ints=np.array(C.INTS))
+                        doubles=np.array(names), ints=np.array(C.INTS),
+                        ev_world=np.array(ev_world, dtype=np.int32), ev_q=np.array(ev_q, dtype=np.int64).reshape(-1, 3),
+                        ev_c=np.array(ev_c, dtype=np.float64).reshape(-1, 3).view(np.uint64))
     n_hits = sum(1 for i in range(1, len(hit_off)) if hit_off[i] > hit_off[i - 1])
+    print(f"{name}: {len(ev_world)} DoCollision events", file=sys.stderr)
     print(f"{name}: {len(kept)} worlds kept, {len(skipped)} skipped {skipped[:5]}, {len(rec_d)} records, "
           f"{n_hits} with collisions", file=sys.stderr)
 

@@ -135,7 +135,7 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
 
 
 def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bool = False,
-             general: bool = False, order=None, initial: bool = False) -> dict:
+             general: bool = False, order=None, initial: bool = False, events: dict | None = None) -> dict:
     """{(world index, step): (d, q, followers)} after each recorded tick (`record`: a set of steps, None for all).
     `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending);
     `initial`: also step 0, the state as loaded. A world flagged `ypr` keeps the display orientation."""
@@ -199,6 +199,10 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                 g.tick()
             else:
                 park.evolve()
+            if events is not None:                  # {world index: [(time, missile, collider, c x, y, z)]}
+                for e in park.collision_events():
+                    if s <= worlds[e[0]]["steps"]:     # a world past its own last step ticks on unrecorded
+                        events.setdefault(e[0], []).append(tuple(e[1:]))
             live = [wi for wi in members if s <= worlds[wi]["steps"] and (record is None or s in record)]
             if live:
                 for wi, v in _snapshot(park, live, rows_of).items():
