@@ -742,6 +742,28 @@ class VecModesPark(metaclass=_Fields):
         self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
         self.missile_follow(go & has_t & ~cloaked_t, dst, own)
 
+    # ---- removal ----
+
+    def remove(self, mask: torch.Tensor) -> None:
+        """RemoveBall between ticks without a delay (full.py:1633-1675; VecCombatPark.remove, vec_combat.py:520), a
+        client's view, for the balls in `mask`: stopped, their followers released (missiles and interactive orbiters
+        fly on), then their rows emptied (no ball: the tick skips them, no ball meets them; add_mushroom may take
+        them). A fixed ball's minis are not removed with it: refused with an error."""
+        mask = mask.to(self.dev) & (self.ids != EMPTY_ID)
+        if not bool(mask.any()):
+            return
+        if bool((mask & ~self.free & (torch.isin(self.ids, self.owner_id[self.mode == MINIBALL]))).any()):
+            raise ValueError("remove: a parent with minis (its minis' removal is not batched)")
+        self._stop_rows(mask)
+        self._release_followers(mask)
+        self.ids = torch.where(mask, EMPTY_ID, self.ids)
+        self._index = None
+        self.moribund = torch.where(mask, True, self.moribund)
+        self.massive = torch.where(mask, False, self.massive)
+        self.free = torch.where(mask, False, self.free)
+        self.mode = torch.where(mask, STOP, self.mode)
+        self._state = None
+
     # ---- collision events ----
 
     def collision_events(self) -> list:

@@ -21,6 +21,7 @@ SETFORM_OP, FFOLLOW_OP = 16, 17
 CLOAK_OP, UNCLOAK_OP = 18, 19
 MUSHROOM_OP, FREE_OP = 20, 21
 HARMONIC_OP = 22
+REMOVE_OP = 23
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -94,6 +95,7 @@ def lib() -> ctypes.CDLL:
             f.restype = i64
         L.or_set_harmonic.argtypes = [vp, i64, i64, i64, i64, i]
         L.or_take_events.argtypes = [vp, pq, pd, i]
+        L.or_remove_ball.argtypes = [vp, i64]
         _LIB = L
     return _LIB
 
@@ -224,6 +226,8 @@ class World:
             L.or_add_mushroom(p, a, rr, x)
         elif op == FREE_OP:
             L.or_set_free(p, a, w)
+        elif op == REMOVE_OP:
+            L.or_remove_ball(p, a)
         elif op == HARMONIC_OP:                 # harmonic w, corporation b, alliance x, a force field if y
             L.or_set_harmonic(p, a, w, b, int(x), int(y))
         else:

@@ -2264,3 +2264,15 @@ int or_take_events(OPark* p, int64_t* q, double* c, int max) {
     p->n_ev = 0;
     return n;
 }
+
+/* RemoveBall outside Evolve without a delay (full.py:1633-1675), a client's view, for a ball without minis: stopped,
+   its followers released (StopAllFollowers), then out of the park (and so of the partition's boxes). */
+void or_remove_ball(OPark* p, int64_t id) {
+    OBall* b = find(p, id);
+    if (!b) return;
+    stop_ball(p, b);
+    stop_all_followers(p, b);
+    const int k = (int)(b - p->balls);
+    for (int i = k; i + 1 < p->n; ++i) p->balls[i] = p->balls[i + 1];
+    p->n--;
+}

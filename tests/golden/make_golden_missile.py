@@ -3,7 +3,8 @@
 SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall;
 with `--mushrooms`, mushroom_golden.npz: `mushroom_cases`, AddMushroom and SetBallFree, ball 0 recorded once added;
 with `--minis`, mini_golden.npz: `mini_cases`, a fixed parent's miniballs, minicapsules and miniboxes, and force fields
-by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids): the reference's own scalar ballpark with missiles
+by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids; with `--remove`,
+remove_golden.npz: `remove_cases`, RemoveBall between ticks, a removed ball out of the records from then on): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -56,6 +57,8 @@ def _apply(park, c) -> None:
         park.set_free(a, bool(w))
     elif op == S.HARMONIC_OP:
         park.set_harmonic(a, w, b, int(x), bool(y))
+    elif op == S.REMOVE_OP:
+        park.remove_ball(a)
     else:
         C._apply(park, c)
 
@@ -115,6 +118,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
         ws = S.do_cloak_cases(seed) if do else S.cloak_cases(seed)
     elif what == "mushroom":
         ws = S.do_mushroom_cases(seed) if do else S.mushroom_cases(seed)
+    elif what == "remove":
+        ws = S.remove_cases(seed)
     elif what == "mini":
         ws = S.do_mini_cases(seed) if do else S.mini_cases(seed)
     else:
@@ -164,4 +169,5 @@ if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     main(int(args[0]) if args else 1, do="--do" in sys.argv,
          what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv
-         else "mushroom" if "--mushrooms" in sys.argv else "mini" if "--minis" in sys.argv else "missile")
+         else "mushroom" if "--mushrooms" in sys.argv else "mini" if "--minis" in sys.argv
+         else "remove" if "--remove" in sys.argv else "missile")

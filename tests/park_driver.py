@@ -85,6 +85,8 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.add_mushroom(mask, tid, rr, xyz[:, 0])
     elif op == S.FREE_OP:
         park.set_free(mask, w)
+    elif op == S.REMOVE_OP:
+        park.remove(mask)
     elif op == S.HARMONIC_OP:                       # harmonic w, corporation b, alliance x, a field if y
         park.set_harmonic(mask, w, tid, xyz[:, 0].to(I64), xyz[:, 1])
     elif op == S.CLOAK_OP:

@@ -1365,3 +1365,18 @@ def do_mini_cases(seed: int, scale: int = 1) -> list:
             b["max_ang_vel"] = r.uniform(0.2, 2.0)
             b["rot_agility"] = r.uniform(0.3, 3.0)
     return out
+
+
+REMOVE_OP = 23                  # RemoveBall(ball a) between ticks, without a delay
+
+
+def remove_cases(seed: int, scale: int = 1) -> list:
+    """The removal worlds (about 120 x scale) for seed: cloak_cases' worlds with the cloak replaced by RemoveBall of
+    the same ball (its followers released: FOLLOW and non-interactive ORBIT stopped, interactive orbiters, missiles
+    at it and its formation's followers as each is; the ball out of every later candidate list), the uncloak by a
+    second removal (nothing), and the commands at it afterwards refused."""
+    out = cloak_cases(seed ^ 0x7E, scale)
+    for w in out:
+        w["cmds"] = [(c[0], REMOVE_OP) + tuple(c[2:]) if c[1] in (CLOAK_OP, UNCLOAK_OP) else c for c in w["cmds"]]
+        w["kind"] = w["kind"].replace("cloak_refused", "remove").replace("uncloak", "again").replace("cloak", "remove")
+    return out
