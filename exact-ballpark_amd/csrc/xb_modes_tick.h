@@ -426,7 +426,8 @@ static inline XB_HD void xb_clear_follow(XbShip& s) {
 
 /* The release of one follower of a ball that starts warp proper (modes.py:410-426): an interactive orbiter flies on
    by GotoDirection along its velocity (refused, so left as it is, when the velocity or the point is not finite), any
-   other member stops (Stop(Ball*) clears the follow fields of a FOLLOW or ORBIT ball). Returns the mode it leaves. */
+   other member stops (Stop(Ball*) clears the follow fields of a FOLLOW, ORBIT or FORMATION ball). Returns the mode it
+   leaves. */
 static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
     if (s.mode == XB_MODE_MISSILE || (s.interactive && s.mode == XB_MODE_ORBIT)) {
         if (apply && s.mode == XB_MODE_MISSILE) s.massive = false;    /* a missile flies on, non-massive */
@@ -442,7 +443,7 @@ static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
         return XB_MODE_GOTO;
     }
     if (apply) {
-        if (s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT) xb_clear_follow(s);
+        if (s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT || s.mode == XB_MODE_FORMATION) xb_clear_follow(s);
         s.mode = XB_MODE_STOP;
     }
     return XB_MODE_STOP;

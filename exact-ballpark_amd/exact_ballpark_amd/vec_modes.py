@@ -402,6 +402,8 @@ class VecModesPark(metaclass=_Fields):
         self.last_collision = -1.0
         self.junk = False if junk is None else c(junk, torch.bool)
         self.formation_id = -1
+        self._dying = torch.zeros(self.n, dtype=torch.bool, device=dev)    # removed with a delay: the moribund set
+        self._mush = torch.zeros(self.n, dtype=torch.bool, device=dev)     # mushrooms (in the set once removed)
         empty = self.ids == EMPTY_ID
         if bool((empty & (self.free | self.massive | ~self.moribund)).any()):
             raise ValueError("an empty row (id EMPTY_ID) is moribund, fixed and not massive: use empty_row()")
@@ -744,14 +746,28 @@ class VecModesPark(metaclass=_Fields):
 
     # ---- removal ----
 
-    def remove(self, mask: torch.Tensor) -> None:
-        """RemoveBall between ticks without a delay (full.py:1633-1675; VecCombatPark.remove, vec_combat.py:520), a
-        client's view, for the balls in `mask`: stopped, their followers released (missiles and interactive orbiters
-        fly on), then their rows emptied (no ball: the tick skips them, no ball meets them; add_mushroom may take
-        them). A fixed ball's minis are not removed with it: refused with an error."""
+    def remove(self, mask: torch.Tensor, delay: torch.Tensor | None = None) -> None:
+        """RemoveBall between ticks (full.py:1633-1675; VecCombatPark.remove, vec_combat.py:520), a client's view,
+        for the balls in `mask`: stopped, their followers released (missiles and interactive orbiters fly on); without
+        a delay their rows emptied (no ball: the tick skips them, no ball meets them; add_mushroom may take them);
+        with a delay of d > 0 ticks moribund and non-massive, stamped with the clock plus d, until
+        bring_out_the_deads removes them. A fixed ball's minis are not removed with it: refused with an error."""
         mask = mask.to(self.dev) & (self.ids != EMPTY_ID)
         if not bool(mask.any()):
             return
+        if delay is not None:
+            d = delay.to(device=self.dev, dtype=I64)
+            later = mask & (d > 0)
+            if bool(later.any()):
+                self._stop_rows(later)
+                self._release_followers(later)
+                self.moribund = torch.where(later, True, self.moribund)
+                self.massive = torch.where(later, False, self.massive)
+                self.effect_stamp = torch.where(later, self.clock + d, self.effect_stamp)
+                self._dying = self._dying | later
+                mask = mask & ~later
+                if not bool(mask.any()):
+                    return
         if bool((mask & ~self.free & (torch.isin(self.ids, self.owner_id[self.mode == MINIBALL]))).any()):
             raise ValueError("remove: a parent with minis (its minis' removal is not batched)")
         self._stop_rows(mask)
@@ -762,7 +778,46 @@ class VecModesPark(metaclass=_Fields):
         self.massive = torch.where(mask, False, self.massive)
         self.free = torch.where(mask, False, self.free)
         self.mode = torch.where(mask, STOP, self.mode)
+        self._dying = self._dying & ~mask
         self._state = None
+
+    def bring_out_the_deads(self, worlds=None) -> None:
+        """BringOutTheDeads between ticks (space.py:1319-1341), as OnTick calls it, in the given worlds (default
+        every world): over each world's moribund set (the balls removed with a delay, a mushroom removed in a tick)
+        by ascending id, a ball no longer moribund leaves the set, one whose stamp has passed is removed, and up to 7
+        more within 2 ticks of theirs."""
+        dying = self._dying | (self._mush & self.moribund & (self.ids != EMPTY_ID))
+        if not bool(dying.any()):
+            return
+        rows = torch.nonzero(dying).reshape(-1).cpu().tolist()
+        world, ids, stamp, clock, mor = (x.cpu().tolist() for x in (self.world, self.ids, self.effect_stamp,
+                                                                    self.clock, self.moribund))
+        keep = None if worlds is None else set(int(w) for w in worlds)
+        by_world: dict = {}
+        for r in rows:
+            if keep is None or world[r] in keep:
+                by_world.setdefault(world[r], []).append(r)
+        due, left = [], []
+        for w, rs in by_world.items():
+            kill = 0
+            for r in sorted(rs, key=lambda r: ids[r]):
+                if not mor[r]:
+                    left.append(r)
+                elif stamp[r] - clock[r] < 0:
+                    due.append(r)
+                elif stamp[r] - clock[r] - 2 < 0 and kill < 7:
+                    due.append(r)
+                    kill += 1
+        if left:
+            out = torch.zeros(self.n, dtype=torch.bool, device=self.dev)
+            out[left] = True
+            self._dying = self._dying & ~out
+            self._mush = self._mush & ~out
+        if due:
+            m = torch.zeros(self.n, dtype=torch.bool, device=self.dev)
+            m[due] = True
+            self._mush = self._mush & ~m
+            self.remove(m)
 
     # ---- collision events ----
 
@@ -912,6 +967,7 @@ class VecModesPark(metaclass=_Fields):
         self.effect_stamp = torch.where(ok, self.clock, self.effect_stamp)
         self.last_collision = torch.where(ok, -1.0, self.last_collision)
         self.mode = torch.where(ok, MUSHROOM, self.mode)
+        self._mush = self._mush | ok
         if self.dynamical_orientation:
             ident = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
             self.rot = where(ident, self.rot)

@@ -22,6 +22,7 @@ CLOAK_OP, UNCLOAK_OP = 18, 19
 MUSHROOM_OP, FREE_OP = 20, 21
 HARMONIC_OP = 22
 REMOVE_OP = 23
+BRING_OP = 24
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -96,6 +97,8 @@ def lib() -> ctypes.CDLL:
         L.or_set_harmonic.argtypes = [vp, i64, i64, i64, i64, i]
         L.or_take_events.argtypes = [vp, pq, pd, i]
         L.or_remove_ball.argtypes = [vp, i64]
+        L.or_remove_ball_delayed.argtypes = [vp, i64, i64]
+        L.or_bring_out_the_deads.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -226,8 +229,10 @@ class World:
             L.or_add_mushroom(p, a, rr, x)
         elif op == FREE_OP:
             L.or_set_free(p, a, w)
-        elif op == REMOVE_OP:
-            L.or_remove_ball(p, a)
+        elif op == REMOVE_OP:                   # a delay of w ticks (0: at once)
+            L.or_remove_ball_delayed(p, a, w)
+        elif op == BRING_OP:
+            L.or_bring_out_the_deads(p)
         elif op == HARMONIC_OP:                 # harmonic w, corporation b, alliance x, a force field if y
             L.or_set_harmonic(p, a, w, b, int(x), int(y))
         else:

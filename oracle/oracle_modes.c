@@ -52,6 +52,7 @@ typedef struct {
     int64_t mode, follow_id, owner_id, effect_stamp;
     int64_t follow_ptr;                        /* the target's id; -1 for none (modes.py: None) */
     int is_free, is_massive, is_interactive, is_moribund, sensor;
+    int dying;                                 /* in the park's moribund set (removed with a delay or in Evolve) */
     int64_t cloaked, bubble;
     int64_t harmonic, corporation, alliance;   /* SetBallHarmonic's (-1 at AddBall) */
     int nf;
@@ -1160,6 +1161,7 @@ static void evolve_mushroom(OPark* p, OBall* b) {
         stop_ball(p, b);
         stop_all_followers(p, b);
         b->is_moribund = 1;
+        b->dying = 1;
         b->is_massive = 0;
         b->effect_stamp = p->current_time;
     }
@@ -2275,4 +2277,43 @@ void or_remove_ball(OPark* p, int64_t id) {
     const int k = (int)(b - p->balls);
     for (int i = k; i + 1 < p->n; ++i) p->balls[i] = p->balls[i + 1];
     p->n--;
+}
+
+/* RemoveBall between ticks with a delay (full.py:1633-1657), a client's view: stopped, its followers released,
+   moribund and non-massive, stamped with the clock plus the delay, in the moribund set; 0 removes it at once. */
+void or_remove_ball_delayed(OPark* p, int64_t id, int64_t delay) {
+    if (delay <= 0) {
+        or_remove_ball(p, id);
+        return;
+    }
+    OBall* b = find(p, id);
+    if (!b) return;
+    stop_ball(p, b);
+    stop_all_followers(p, b);
+    b->is_moribund = 1;
+    b->is_massive = 0;
+    b->effect_stamp = p->current_time + delay;
+    b->dying = 1;
+}
+
+/* BringOutTheDeads between ticks (space.py:1319-1341): over the moribund set by ascending id, a ball no longer
+   moribund leaves the set; one whose stamp has passed is removed, and up to 7 more within 2 ticks of theirs. */
+void or_bring_out_the_deads(OPark* p) {
+    int64_t due[MAX_BALLS];
+    int nd = 0, kill = 0;
+    for (int i = 0; i < p->n; ++i) {
+        OBall* b = &p->balls[i];
+        if (!b->dying) continue;
+        if (!b->is_moribund) {
+            b->dying = 0;
+            continue;
+        }
+        if (b->effect_stamp - p->current_time < 0) {
+            due[nd++] = b->id;
+        } else if (b->effect_stamp - p->current_time - 2 < 0 && kill < 7) {
+            due[nd++] = b->id;
+            kill++;
+        }
+    }
+    for (int i = 0; i < nd; ++i) or_remove_ball(p, due[i]);
 }

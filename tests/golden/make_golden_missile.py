@@ -58,7 +58,9 @@ def _apply(park, c) -> None:
     elif op == S.HARMONIC_OP:
         park.set_harmonic(a, w, b, int(x), bool(y))
     elif op == S.REMOVE_OP:
-        park.remove_ball(a)
+        park.remove_ball(a, w)
+    elif op == S.BRING_OP:
+        park.bring_out_the_deads()
     else:
         C._apply(park, c)
 
@@ -119,7 +121,7 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
     elif what == "mushroom":
         ws = S.do_mushroom_cases(seed) if do else S.mushroom_cases(seed)
     elif what == "remove":
-        ws = S.remove_cases(seed)
+        ws = S.remove_cases(seed) + S.delayed_remove_cases(seed)
     elif what == "mini":
         ws = S.do_mini_cases(seed) if do else S.mini_cases(seed)
     else:

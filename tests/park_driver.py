@@ -28,6 +28,9 @@ YPR_NAMES = ("yaw", "pitch", "roll", "old_yaw", "old_pitch", "old_roll", "roll_s
 
 
 def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
+    if op == S.BRING_OP:                            # BringOutTheDeads in each listed world
+        park.bring_out_the_deads(worlds=[wi for wi, _ in items])
+        return
     n, dev = park.n, park.dev
     mask = torch.zeros(n, dtype=torch.bool)
     target = torch.full((n,), -1, dtype=I64)
@@ -85,8 +88,8 @@ def _apply(park: VecModesPark, op: int, items: list, index: dict) -> None:
         park.add_mushroom(mask, tid, rr, xyz[:, 0])
     elif op == S.FREE_OP:
         park.set_free(mask, w)
-    elif op == S.REMOVE_OP:
-        park.remove(mask)
+    elif op == S.REMOVE_OP:                         # a delay of w ticks (0: at once)
+        park.remove(mask, w)
     elif op == S.HARMONIC_OP:                       # harmonic w, corporation b, alliance x, a field if y
         park.set_harmonic(mask, w, tid, xyz[:, 0].to(I64), xyz[:, 1])
     elif op == S.CLOAK_OP:

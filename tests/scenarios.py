@@ -1377,6 +1377,27 @@ def remove_cases(seed: int, scale: int = 1) -> list:
     second removal (nothing), and the commands at it afterwards refused."""
     out = cloak_cases(seed ^ 0x7E, scale)
     for w in out:
-        w["cmds"] = [(c[0], REMOVE_OP) + tuple(c[2:]) if c[1] in (CLOAK_OP, UNCLOAK_OP) else c for c in w["cmds"]]
+        w["cmds"] = [(c[0], REMOVE_OP) + tuple(c[2:8]) + (0,) if c[1] in (CLOAK_OP, UNCLOAK_OP) else c
+                     for c in w["cmds"]]
         w["kind"] = w["kind"].replace("cloak_refused", "remove").replace("uncloak", "again").replace("cloak", "remove")
+    return out
+
+
+BRING_OP = 24                   # BringOutTheDeads, as OnTick calls it before a tick
+
+
+def delayed_remove_cases(seed: int, scale: int = 1) -> list:
+    """The delayed removal worlds (about 60 x scale): cloak_cases' worlds with the cloak replaced by RemoveBall of the
+    same ball with a delay of 1-6 ticks (moribund and non-massive until BringOutTheDeads removes it, the stamp passed
+    or within two ticks), BringOutTheDeads before every tick; a second removal meanwhile."""
+    r = random.Random(seed ^ 0xDEAD)
+    out = cloak_cases(seed ^ 0xD1, scale)[:60 * scale]
+    for w in out:
+        cmds = [(s, BRING_OP, 0, 0, 0.0, 0.0, 0.0, 0.0, 0) for s in range(1, w["steps"] + 1)]
+        for c in w["cmds"]:
+            if c[1] in (CLOAK_OP, UNCLOAK_OP):
+                c = (c[0], REMOVE_OP) + tuple(c[2:8]) + (r.choice([1, 2, 3, 6]),)
+            cmds.append(c)
+        w["cmds"] = sorted(cmds, key=lambda q: (q[0], q[1] != BRING_OP))
+        w["kind"] = "delayed_remove"
     return out
