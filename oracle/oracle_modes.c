@@ -99,7 +99,7 @@ typedef struct {
     int ypr;                                   /* FullBallpark's yaw, pitch and roll without dynamical orientation */
     int overflow;                              /* a collision list past MAX_HITS */
     int64_t local_hi;                          /* GetIDForCollisionObject(-1)'s last local id on a client */
-    int n_ev;                                  /* DoCollision events since the last read (at most MAX_EVENTS) */
+    int n_ev;                                  /* DoCollision events since the last read (the first MAX_EVENTS) */
     int64_t ev_q[MAX_EVENTS][3];               /* time, missile id, collider id */
     double ev_c[MAX_EVENTS][3];                /* the missile's mLastC */
     int n;
@@ -2088,10 +2088,7 @@ void or_evolve(OPark* p) {                                              /* modes
         refile(p, b);
         if (b->mode == MISSILE) {                       /* Ball::DispatchCollisions (full.py:2571-2583) */
             for (int h = 0; h < b->nh; ++h) {
-                if (p->n_ev >= MAX_EVENTS) {
-                    p->overflow = 1;
-                    break;
-                }
+                if (p->n_ev >= MAX_EVENTS) break;       /* unread events: the oldest are kept, no more */
                 int64_t* q = p->ev_q[p->n_ev];
                 q[0] = p->current_time, q[1] = b->id, q[2] = b->hits[h];
                 for (int k = 0; k < 3; ++k) p->ev_c[p->n_ev][k] = b->last_c[k];
