@@ -78,6 +78,11 @@ WORKLOADS = {
 }
 
 
+KERNEL_NAMES = {0: "xb_modes_tiled_flight", 3: "xb_modes_tiled_turning", 4: "xb_collide_tiled_flight",
+                5: "xb_collide_tiled_turning", 8: "xb_collide_tiled_flight (static collidables)",
+                9: "xb_collide_tiled_turning (static collidables)"}
+
+
 def _kernel_which(park) -> int:
     """xb_modes_info's index of the tiled kernel the park's tick launches."""
     if park.collisions and park._st is not None:
@@ -175,6 +180,7 @@ def main() -> int:
     ap.add_argument("--arrival-rate", type=float, default=5000.0, help="episodes arriving per serving step (1 s)")
     ap.add_argument("--serve-check", type=int, default=48, help="finished episodes checked against the oracle")
     ap.add_argument("--no-single", action="store_true", help="skip the single-tick measurement (for profiling)")
+    ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
     a = ap.parse_args()
@@ -312,6 +318,7 @@ def main() -> int:
                                       "speedup": mp["ms_per_tick"] / m["ms_per_tick"]}
             del pp
         res["_ticks_done"] = at_snap + last[0]
+        res["_ticks_timed"] = m["ticks"]
         res["_park"], res["_wl"] = park, wl
         return res
 
@@ -336,7 +343,7 @@ def main() -> int:
     p = primary
     out = {
         "metric": METRIC, "value": p["world_ticks_per_s"], "unit": "world-ticks/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": p["ms_per_tick"], "higher_is_better": True,
+        "steps": p["_ticks_timed"], "warmup": a.warmup, "ms_per_step": p["ms_per_tick"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": WORKLOADS[p["workload"]].format(worlds=a.worlds),
                    "worlds": a.worlds, "ships": p["ships"], "seed": 1, "shape": p["shape"]},
@@ -344,7 +351,7 @@ def main() -> int:
         "latency_ms": {"tick_p50": p["tick_latency_ms"]["p50"], "tick_p99": p["tick_latency_ms"]["p99"],
                        "launch_p50": p["launch_latency_ms"]["p50"], "launch_p99": p["launch_latency_ms"]["p99"]},
         "roofline": p["roofline"], "single_tick": p["single_tick"],
-        "kernel": N.kernel_info(_kernel_which(p["_park"])),
+        "kernel": dict(N.kernel_info(_kernel_which(p["_park"])), name=KERNEL_NAMES[_kernel_which(p["_park"])]),
     }
     for key in ("learned_scheduler", "padded_baseline"):
         if key in p:
@@ -359,8 +366,52 @@ def main() -> int:
             out["host_batched"] = host_batched(a.cpu_seconds / 2)
         if srv is not None:
             out["serving"]["cpu_baseline"], out["serving"]["exact"] = cpu_baseline_serving(srv)
-    print(json.dumps(out), flush=True)
+    Path(a.detail).parent.mkdir(parents=True, exist_ok=True)
+    Path(a.detail).write_text(json.dumps(out, indent=1))
+    for key, v in out.items():                       # one short line a park, before the headline
+        if key.endswith("_park") or key == "serving":
+            print("# " + json.dumps(_park_summary(key, v)), file=sys.stderr, flush=True)
+    print(json.dumps(_headline(out, p, a)), flush=True)
     return 0
+
+
+def _park_summary(key: str, v: dict) -> dict:
+    keep = ("world_ticks_per_s", "ship_ticks_per_s", "ms_per_tick", "shape", "exact", "step_latency_ms",
+            "episodes_done")
+    s = {"park": key}
+    s.update({k: v[k] for k in keep if k in v})
+    if "single_tick" in v:
+        s["single_tick_hbm_frac"] = v["single_tick"]["roofline"]["frac"]
+    return s
+
+
+def _headline(out: dict, p: dict, a) -> dict:
+    """The line the run is judged on (the last stdout line, well under 2 KB): the primary park's rate, the single-tick
+    launch's HBM roofline (the multi-tick launch keeps rows in registers between ticks, so its per-tick bytes are not
+    an HBM figure), the CPU baseline and the bit-exactness check; everything else is in the detail file."""
+    one = p["single_tick"]["roofline"]
+    head = {k: out[k] for k in ("metric", "value", "unit", "n_gpus")}
+    head.update({
+        "steps": p["_ticks_timed"], "warmup": a.warmup, "ms_per_step": out["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS[p["workload"]].format(worlds=a.worlds), "worlds": a.worlds,
+                   "rows": p["ships"], "shape": "{block}/{schedule}/{ticks}/{order}".format(**p["shape"])},
+        "ship_ticks_per_s": out["ship_ticks_per_s"],
+        "tick_latency_ms": {"p50": out["latency_ms"]["tick_p50"], "p99": out["latency_ms"]["tick_p99"]},
+        "roofline": {"bound": "hbm", "achieved": one["achieved"], "peak": one["peak"], "unit": "GB/s",
+                     "frac": one["frac"], "traffic": None, "kernel": out["kernel"].get("name"),
+                     "ticks_per_launch": 1, "launch_us": one["launch_us"],
+                     "bytes_per_launch": one["bytes_per_launch_algorithmic"]},
+    })
+    if "cpu_baseline" in out:
+        head["cpu_baseline"] = out["cpu_baseline"]
+        head["exact"] = out["exact"]
+    if "padded_baseline" in out:
+        head["padded_speedup"] = out["padded_baseline"]["speedup"]
+    if "learned_scheduler" in out:
+        head["learned_gain"] = out["learned_scheduler"]["gain_over_rows"]
+    head["detail"] = a.detail
+    return head
 
 
 def serving(a) -> tuple:
