@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #define XB_HD __device__
 #define XB_TABLE __constant__
 #include "xb_modes_tick.h"
@@ -493,6 +495,36 @@ int xb_launch_rc() {
 
 bool xb_block_ok(int block) { return block > 0 && block <= 256 && block % 64 == 0; }
 
+// The allocation-free tick's scratch: the general launch's records (XB_NS float64 rows, then one uint8 row) for up to
+// XB_NOALLOC_MAX_ROWS rows, in one array of the library's own on each device that loads it (about 15 MB, held until
+// the process ends). Host code never passes the array's name to a kernel: the name is a host-side stand-in, so the
+// device address of each device's copy is taken once per device (hipGetSymbolAddress, with that device current) and
+// kept in scratch_ptr.
+constexpr size_t XB_NOALLOC_F64_BYTES = (size_t)XB_NS * sizeof(double) * XB_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) unsigned char scratch[XB_NOALLOC_F64_BYTES + XB_NOALLOC_MAX_ROWS];
+
+constexpr int XB_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[XB_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`: its address looked up on the first call on that device, then reused.
+int xb_scratch_of_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= XB_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return -1;
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -501,6 +533,9 @@ int xb_abi(void) { return XB_ABI_VERSION; }
 
 const char* xb_error_string(int rc) {
     if (rc == 0) return "success";
+    if (rc == XB_ERR_TOO_MANY_ROWS)
+        return "refused before launch: the park's stride is above XB_NOALLOC_MAX_ROWS, the most rows the "
+               "allocation-free tick's scratch holds";
     if (rc < 0) return "refused before launch (block size, sizes or arguments)";
     return hipGetErrorString((hipError_t)rc);
 }
@@ -566,6 +601,18 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     else if (ypr) xb_modes_step_c<false, true><<<grid, block, 0, st>>>(*state);
     else xb_modes_step_c<false, false><<<grid, block, 0, st>>>(*state);
     return xb_launch_rc();
+}
+
+int xb_modes_tick_noalloc(const xb_modes_state* state, void* stream, int block) {
+    if (!state || state->tiles || state->collisions || state->ticks > 1 || state->schedule) return -1;
+    if (state->stride > XB_NOALLOC_MAX_ROWS) return XB_ERR_TOO_MANY_ROWS;
+    void* base = nullptr;
+    const int rc = xb_scratch_of_current_device(&base);
+    if (rc) return rc;
+    xb_modes_state s = *state;
+    s.scratch_f64 = (double*)base;                                   // XB_NS rows of s.stride doubles
+    s.scratch_u8 = (uint8_t*)base + XB_NOALLOC_F64_BYTES;            // one row of s.stride bytes
+    return xb_modes_tick(&s, stream, block);
 }
 
 int xb_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, int what, void* stream) {

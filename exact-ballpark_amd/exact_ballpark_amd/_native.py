@@ -21,6 +21,8 @@ NF, NI, NB, NS = 46, 10, 7, 7
 NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 NY = 9                          # the display orientation block: yaw, pitch, roll, their old values, roll speed (2), yaw delta
 NSF, NSI = 93, 5                # the static collidables' table: geometry; kind, id, parent, level, box count
+NOALLOC_MAX_ROWS = 1 << 18      # the most rows (stride) xb_modes_tick_noalloc takes
+ERR_TOO_MANY_ROWS = -2          # its refusal of a larger park
 
 
 class EpisodeRows(ctypes.Structure):
@@ -70,6 +72,7 @@ def hip() -> ctypes.CDLL:
         L.xb_error_string.argtypes = [ctypes.c_int]
         L.xb_layout.argtypes = [ctypes.POINTER(ctypes.c_int64)]
         L.xb_modes_tick.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int]
+        L.xb_modes_tick_noalloc.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int]
         L.xb_maths_eval.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_int]
         L.xb_modes_info.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]

@@ -116,6 +116,22 @@ int xb_layout(int64_t* out);
    null stream). block: a multiple of 64, at most 256. Replaces xbpm_tick (src/exact_ballpark/vec_modes_cuda.cu:754). */
 int xb_modes_tick(const xb_modes_state* state, void* stream, int block);
 
+/* ---- the allocation-free tick ---- */
+
+#define XB_NOALLOC_MAX_ROWS (1 << 18)   /* the most rows (the state's stride) xb_modes_tick_noalloc takes */
+#define XB_ERR_TOO_MANY_ROWS (-2)       /* its refusal of a larger park (before any launch) */
+
+/* No counterpart in the reference: for callers that cannot call hipMalloc (code under HIP graph capture, tight
+   serving loops, harnesses that forbid allocation). xb_modes_tick's general shape (state->tiles NULL: any row order,
+   one tick, three launches) with its records kept in a static device array of the library's own (about 15 MB on each
+   device the library runs on, held until the process ends) in place of the caller's scratch_f64 and scratch_u8, which
+   are ignored. Allocates nothing: no hipMalloc, hipFree or allocating library call. Refuses a tiled or collision
+   state, ticks > 1 or a lane schedule (-1), and a stride above XB_NOALLOC_MAX_ROWS (XB_ERR_TOO_MANY_ROWS). Each
+   launch writes every scratch word the tick reads before reading it, so nothing needs clearing between calls; but
+   every call on one device shares that device's one array, so calls that overlap on one device (other streams or
+   threads) race. */
+int xb_modes_tick_noalloc(const xb_modes_state* state, void* stream, int block);
+
 /* The tick's double functions alone, out[i] = f(x[i]) (func 0 sin, 1 cos, 2 log, 3 exp, 4 sincos's sine, 5 sincos's
    cosine, 6 IEEE sqrt, 7 IEEE division x[i] / x[n + i] with out of n elements; on floats held in doubles 8 expf, 9
    logf, 10 sinf, 11 cosf, 12 acosf, 13 atanf, 14 atan2f(x[i], x[n + i]); 15 the double pow(x[i], x[n + i]); 16 the
