@@ -755,6 +755,8 @@ class VecModesPark(metaclass=_Fields):
         mask = mask.to(self.dev) & (self.ids != EMPTY_ID)
         if not bool(mask.any()):
             return
+        if bool((mask & ~self.free & (torch.isin(self.ids, self.owner_id[self.mode == MINIBALL]))).any()):
+            raise ValueError("remove: a parent with minis (its minis' removal is not batched)")
         if delay is not None:
             d = delay.to(device=self.dev, dtype=I64)
             later = mask & (d > 0)
@@ -768,8 +770,6 @@ class VecModesPark(metaclass=_Fields):
                 mask = mask & ~later
                 if not bool(mask.any()):
                     return
-        if bool((mask & ~self.free & (torch.isin(self.ids, self.owner_id[self.mode == MINIBALL]))).any()):
-            raise ValueError("remove: a parent with minis (its minis' removal is not batched)")
         self._stop_rows(mask)
         self._release_followers(mask)
         self.ids = torch.where(mask, EMPTY_ID, self.ids)
@@ -830,8 +830,11 @@ class VecModesPark(metaclass=_Fields):
             return []
         d = self.pos - self.old_pos
         sq = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
-        rows = torch.nonzero(self.free & ~self.moribund & (self.mode == MISSILE) & ~(sq < 0.1)
-                             & (self.hits_n > 0)).reshape(-1)
+        sel = self.free & ~self.moribund & (self.mode == MISSILE) & ~(sq < 0.1) & (self.hits_n > 0)
+        ticked = getattr(self, "_ticked", None)
+        if ticked is not None and ticked[0] == self.current_time:
+            sel = sel & ticked[1]                  # a world the last (masked) tick left out posts no event
+        rows = torch.nonzero(sel).reshape(-1)
         if rows.numel() == 0:
             return []
         r = rows.cpu().tolist()
@@ -1334,6 +1337,8 @@ class VecModesPark(metaclass=_Fields):
         else:
             self._tick()
         self.current_time += 1
+        self._ticked = None if active is None else (self.current_time,
+                                                    active.to(device=self.dev, dtype=torch.bool).reshape(self.n))
 
     def evolve_ticks(self, ticks: int) -> None:
         """`ticks` ticks of every world with no command between them: one launch when the park is tiled (each lane
@@ -1389,8 +1394,10 @@ class VecModesPark(metaclass=_Fields):
 class ModesGraph:
     """A VecModesPark's tick captured once as a HIP graph on the park's own state blocks, so each replay is one tick
     (the reference's ModesGraph, src/exact_ballpark/vec_modes_cuda.py:371-432); with `ticks` ticks a launch and
-    `launches` launches a replay, one replay runs ticks x launches ticks back to back. Commands and setters write into the
-    blocks, so nothing needs re-binding between replays. With `masked=True` the capture holds OnTick's per-world mask
+    `launches` launches a replay, one replay runs ticks x launches ticks back to back. Commands and setters that write
+    into the blocks need no re-binding between replays; a call that replaces a buffer the launches point at (the first
+    set_harmonic, load_formations, load_statics, record_paths, a regrouping, new tiles) makes tick() raise: capture
+    again. With `masked=True` the capture holds OnTick's per-world mask
     as a device buffer: the blocks saved, the tick, the rows outside the mask put back; `tick(active)` writes the mask
     and replays."""
 
@@ -1424,6 +1431,7 @@ class ModesGraph:
             if masked:
                 for s, b in zip(self.saved, park._blocks()):
                     b.copy_(torch.where(self.mask[None, :], b, s))
+        self.captured = self._pointers()
 
     def tick(self, active: torch.Tensor | None = None) -> None:
         if active is not None and not self.masked:
@@ -1433,5 +1441,17 @@ class ModesGraph:
                 self.mask.fill_(True)
             else:
                 self.mask[:self.park.n].copy_(active.to(device=self.park.dev, dtype=torch.bool).reshape(-1))
+        if self._pointers() != self.captured:
+            raise RuntimeError("a block, table or buffer of the park was replaced since the capture (a first "
+                               "set_harmonic, load_formations, load_statics, record_paths, a regrouping or new tiles): "
+                               "capture the graph again")
         self.graph.replay()
         self.park.current_time += self.ticks * self.launches
+        self.park._ticked = None if not self.masked or active is None else (
+            self.park.current_time, self.mask[:self.park.n].clone())
+
+    def _pointers(self) -> tuple:
+        """Every pointer of the park's state the capture froze into its launches."""
+        s = self.park.state()
+        return tuple(getattr(s, name) for name, typ in N.ModesState._fields_ if typ is ctypes.c_void_p) + (
+            s.n_tiles, s.hits_cap, s.n_form)
