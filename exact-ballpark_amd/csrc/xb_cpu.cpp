@@ -399,6 +399,44 @@ int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int6
     return 0;
 }
 
+static bool command_args_ok(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const xb_command* cmds) {
+    return state && state->f64 && state->i64 && state->u8 && state->n >= 0 && state->stride >= state->n &&
+           n_rows >= 0 && cmds && (rows || n_rows <= state->n);
+}
+
+int xb_cpu_modes_command_check(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                               int32_t* status) {
+    if (!command_args_ok(state, rows, n_rows, cmds) || !status) return -1;
+    const XbPark P = host_park(*state);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r >= 0 && r < state->n) *status |= xb_cmd_gather(P, state->n, r, cmds[j], state->ypr);
+    }
+    return 0;
+}
+
+int xb_cpu_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                               uint8_t* mark) {
+    if (!command_args_ok(state, rows, n_rows, cmds)) return -1;
+    const XbPark P = host_park(*state);
+    const int64_t n = state->n;
+    for (int64_t j = 0; j < n_rows; ++j) {                  /* gather: every read of another row first */
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r >= 0 && r < n) xb_cmd_gather(P, n, r, cmds[j], state->ypr);
+    }
+    for (int64_t j = 0; j < n_rows; ++j) {                  /* apply: each row's own */
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r >= 0 && r < n && xb_cmd_run(P, r, cmds[j], state->ypr) && mark) mark[r] = 1;
+    }
+    if (!mark) return 0;
+    for (int64_t r = 0; r < n; ++r) xb_cmd_release(P, n, r, mark);
+    for (int64_t j = 0; j < n_rows; ++j) {
+        const int64_t r = rows ? (int64_t)rows[j] : j;
+        if (r >= 0 && r < n) mark[r] = 0;
+    }
+    return 0;
+}
+
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
                           const double* point, const int64_t* target, const double* range, const double* min_range,
                           const int64_t* warp_factor) {

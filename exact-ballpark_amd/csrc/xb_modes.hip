@@ -460,6 +460,40 @@ __global__ void xb_commands_kernel(xb_modes_state s, const int32_t* __restrict__
     xb_cmd_apply(xb_park(s), s.n, r, op, point, target, range, min_range, warp_factor);
 }
 
+// The command batch's steps, one lane per listed row (rows NULL: rows 0 .. n_rows - 1), or per park row (release).
+__global__ void xb_command_gather_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows,
+                                         xb_command* __restrict__ cmds, int32_t* __restrict__ status) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    const int refuse = xb_cmd_gather(xb_park(s), s.n, r, cmds[j], s.ypr);
+    if (refuse && status) atomicOr(status, refuse);
+}
+
+__global__ void xb_command_run_kernel(xb_modes_state s, const int32_t* __restrict__ rows, int64_t n_rows,
+                                      xb_command* __restrict__ cmds, uint8_t* __restrict__ mark) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r < 0 || r >= s.n) return;
+    if (xb_cmd_run(xb_park(s), r, cmds[j], s.ypr) && mark) mark[r] = 1;
+}
+
+__global__ void xb_command_release_kernel(xb_modes_state s, const uint8_t* __restrict__ mark) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= s.n) return;
+    xb_cmd_release(xb_park(s), s.n, r, mark);
+}
+
+__global__ void xb_command_unmark_kernel(int64_t n, const int32_t* __restrict__ rows, int64_t n_rows,
+                                         uint8_t* __restrict__ mark) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_rows) return;
+    const int64_t r = rows ? (int64_t)rows[j] : j;
+    if (r >= 0 && r < n) mark[r] = 0;
+}
+
 __global__ void xb_maths_kernel(int func, int64_t n, const double* __restrict__ x, double* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -652,6 +686,40 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
     if (n_rows == 0) return 0;
     xb_commands_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(
         *state, rows, n_rows, op, point, target, range, min_range, warp_factor);
+    return xb_launch_rc();
+}
+
+static bool xb_command_args_ok(const xb_modes_state* state, const int32_t* rows, int64_t n_rows,
+                               const xb_command* cmds) {
+    return state && state->f64 && state->i64 && state->u8 && state->n >= 0 && state->stride >= state->n &&
+           n_rows >= 0 && cmds && (rows || n_rows <= state->n);
+}
+
+int xb_modes_command_check(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                           int32_t* status, void* stream) {
+    if (!xb_command_args_ok(state, rows, n_rows, cmds) || !status) return -1;
+    if (n_rows == 0) return 0;
+    xb_command_gather_kernel<<<(unsigned)((n_rows + 255) / 256), 256, 0, (hipStream_t)stream>>>(*state, rows, n_rows,
+                                                                                                 cmds, status);
+    return xb_launch_rc();
+}
+
+int xb_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                           uint8_t* mark, void* stream) {
+    if (!xb_command_args_ok(state, rows, n_rows, cmds)) return -1;
+    if (n_rows == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned grid = (unsigned)((n_rows + 255) / 256);
+    xb_command_gather_kernel<<<grid, 256, 0, st>>>(*state, rows, n_rows, cmds, nullptr);
+    int rc = xb_launch_rc();
+    if (rc) return rc;
+    xb_command_run_kernel<<<grid, 256, 0, st>>>(*state, rows, n_rows, cmds, mark);
+    rc = xb_launch_rc();
+    if (rc || !mark) return rc;
+    xb_command_release_kernel<<<(unsigned)((state->n + 255) / 256), 256, 0, st>>>(*state, mark);
+    rc = xb_launch_rc();
+    if (rc) return rc;
+    xb_command_unmark_kernel<<<grid, 256, 0, st>>>(state->n, rows, n_rows, mark);
     return xb_launch_rc();
 }
 

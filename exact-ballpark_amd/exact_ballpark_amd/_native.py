@@ -84,6 +84,10 @@ def hip() -> ctypes.CDLL:
                                             ctypes.c_void_p]
         L.xb_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
             [ctypes.c_void_p] * 7
+        L.xb_modes_command_check.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_modes_command_apply.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _check_layout(L, "libxb_hip.so")
         _LIBS["hip"] = L
     return _LIBS["hip"]
@@ -106,6 +110,10 @@ def cpu() -> ctypes.CDLL:
                                                 ctypes.c_int]
         L.xb_cpu_modes_commands.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64] + \
             [ctypes.c_void_p] * 6
+        L.xb_cpu_modes_command_check.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_void_p, ctypes.c_void_p]
+        L.xb_cpu_modes_command_apply.argtypes = [ctypes.POINTER(ModesState), ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_void_p, ctypes.c_void_p]
         L.xb_cpu_pack_tiles.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_int64)]
         _LIBS["cpu"] = L

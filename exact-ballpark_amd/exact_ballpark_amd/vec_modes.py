@@ -29,12 +29,7 @@ GOTO, FOLLOW, STOP, WARP, ORBIT, MISSILE, MUSHROOM = 0, 1, 2, 3, 4, 5, 6
 BOID, TROLL, MINIBALL, FIELD, RIGID, FORMATION = 7, 8, 9, 10, 11, 12
 INERT_MODES = (BOID, MINIBALL, FIELD, RIGID)
 LOADABLE = (STOP,) + INERT_MODES
-AU = 0.1495978707e12
-MIN_WARP_DISTANCE_SQ = 10000000000.0        # WarpTo's short warp (modes.py:78)
 MAX_CATCH_UP_TICKS = 5                      # OnTick's resynchronisation (full.py:92)
-TROLL_MIN_DELAY = 1                         # SetBallTroll (vec_modes.py:90)
-DIR_DIST = 1.0e17                           # GotoDirection's point
-MISSILE_LAUNCH_SPEED = 150.0                # LaunchMissile's least speed (full.py:85)
 
 # (name, block, first block row, columns; 0 for a [n] field)
 FIELDS = (
@@ -67,6 +62,29 @@ YPR_FIELDS = (("yaw", "y", 0, 0), ("pitch", "y", 1, 0), ("roll", "y", 2, 0), ("o
 YPR_PATHS = ("host", "cuda")                # the reference's names for where its orientation park evaluates them
 
 
+# the command interpreter's ops (exact-ballpark_amd/csrc/xb_commands.h, XB_CMD_*) and its record (include/xb.h's
+# xb_command: 29 words of 8 bytes; word 28 the gathered state bits, XB_ACCEPTED among them)
+CMD = {"goto": 1, "goto_direction": 2, "stop": 3, "follow": 4, "orbit": 5, "warp": 6, "warp_in": 7,
+       "stop_release": 8, "release": 9, "set_mass": 10, "set_agility": 11, "set_max_vel": 12,
+       "set_speed_fraction": 13, "set_radius": 14, "set_velocity": 15, "set_max_ang_speed": 16,
+       "set_rot_agility": 17, "set_ang_vel": 18, "set_rotation": 19, "set_rotation_ypr": 20, "set_free": 21,
+       "set_rigid": 22, "set_boid": 23, "set_interactive": 24, "set_troll": 25, "missile_follow": 26,
+       "launch_missile": 27}
+CMD_WORDS = 29
+CMD_STATE_WORD = 28
+CMD_ACCEPTED = 32
+REFUSALS = {1: "a mushroom as a follow, orbit, missile or formation target is not supported",
+            2: "a launcher at rest aims by its heading: without dynamical orientation, that is the display "
+               "orientation (make the park with display_ypr=True)"}
+
+
+def _refusal(status: int) -> None:
+    """Raises the gather step's refusal, if any (its status bits)."""
+    for bit, msg in REFUSALS.items():
+        if status & bit:
+            raise ValueError(msg)
+
+
 def f32(x: torch.Tensor) -> torch.Tensor:
     """A double rounded to the nearest float and back (a C++ float member; out of range an infinity)."""
     return x.to(torch.float32).to(F64)
@@ -80,13 +98,6 @@ def _sqrt(x: torch.Tensor) -> torch.Tensor:
     """IEEE square root, correctly rounded, by the library's own kernel (or its host build): torch's CPU float64 sqrt
     is not correctly rounded in every case."""
     return N.maths("sqrt", x)
-
-
-def _unit(d: torch.Tensor) -> torch.Tensor:
-    """Vector3d::Normalize on rows: times the reciprocal of the length; a zero row stays as it is."""
-    norm = _sqrt(_dot(d, d))
-    inv = torch.ones_like(norm) / torch.where(norm == 0.0, 1.0, norm)
-    return torch.where((norm == 0.0)[:, None], d, d * inv[:, None])
 
 
 def rotation_ypr(yaw: torch.Tensor, pitch: torch.Tensor, roll: torch.Tensor) -> torch.Tensor:
@@ -180,12 +191,13 @@ class VecModesPark(metaclass=_Fields):
 
     def __init__(self, n: int, friction: float = 1.0e6, tick_interval_ms: int = 1000, device="cuda",
                  dynamical_orientation: bool = False, use_new_orbit: bool = False, block: int = 256,
-                 ypr: str = "host", collisions: bool = False, display_ypr: bool = False):
+                 ypr: str = "host", collisions: bool = False, display_ypr: bool = False, master: bool = False):
         if ypr not in YPR_PATHS:
             raise ValueError(f"ypr must be one of {YPR_PATHS}, not {ypr!r}")
         if display_ypr and dynamical_orientation:
             raise ValueError("display_ypr is the orientation without dynamical orientation")
         self.ypr = ypr
+        self.master = bool(master)        # the server's view (OnTick's resynchronisation); a client's otherwise
         self.n = int(n)
         self.friction = float(friction)
         self.tick_interval = int(tick_interval_ms)
@@ -590,49 +602,79 @@ class VecModesPark(metaclass=_Fields):
             raise RuntimeError("a collision list past its room")
         return [[int(x) for x in h[r, :k]] for r, k in enumerate(n)]
 
-    # ---- the follower bookkeeping: Stop(Ball*) (Ballpark.cpp:4578; vec_modes.py:286-311) ----
+    # ---- commands and setters: one record a listed row, run by xb_modes_command_apply ----
 
-    def _following(self) -> torch.Tensor:
-        """The follow modes whose fields Stop(Ball*) clears (FOLLOW, ORBIT, MISSILE, FORMATION)."""
-        return (self.mode == FOLLOW) | (self.mode == ORBIT) | (self.mode == MISSILE) | (self.mode == FORMATION)
+    def _command(self, op: str, mask: torch.Tensor, *, flag=None, ints=(), x=None, check: bool = False,
+                 release: bool = False) -> torch.Tensor | None:
+        """Runs command `op` (CMD) on the rows in `mask` [n] in one batch: each row's flag [n], integer arguments
+        (tensors [n]) and float arguments x [n, k] into its record (include/xb.h's xb_command); `check`: the gather
+        step first, its refusals raised before anything changes; `release`: StopAllFollowers for the rows the
+        command marks. Returns the records (None when no row is listed)."""
+        if self.n == 0:
+            return None
+        d = self.dev
+        rows = torch.nonzero(mask.to(d).reshape(self.n)).reshape(-1)
+        m = rows.numel()
+        if m == 0:
+            return None
+        rec = torch.zeros(m, CMD_WORDS, dtype=I64, device=d)
+        rec[:, 0] = CMD[op]
+        if flag is not None:
+            rec[:, 1] = flag.to(device=d, dtype=I64).reshape(self.n)[rows]
+        for k, v in enumerate(ints):
+            rec[:, 2 + k] = v.to(device=d, dtype=I64).reshape(self.n)[rows]
+        if x is not None:
+            xv = x.to(device=d, dtype=F64).reshape(self.n, -1)[rows]
+            rec.view(F64)[:, 5:5 + xv.shape[1]] = xv
+        rows = rows.to(torch.int32)
+        s = self.state()
+        mark = None
+        if release:
+            if getattr(self, "_mark", None) is None:
+                self._mark = torch.zeros(self.stride, dtype=U8, device=d)
+            mark = self._mark.data_ptr()
+        if d.type == "cuda":
+            with torch.cuda.device(d):
+                stream = torch.cuda.current_stream(d).cuda_stream
+                if check:
+                    status = torch.zeros(1, dtype=torch.int32, device=d)
+                    N.check(N.hip().xb_modes_command_check(ctypes.byref(s), rows.data_ptr(), m, rec.data_ptr(),
+                                                           status.data_ptr(), stream), "xb_modes_command_check")
+                    _refusal(int(status.item()))
+                N.check(N.hip().xb_modes_command_apply(ctypes.byref(s), rows.data_ptr(), m, rec.data_ptr(), mark,
+                                                       stream), "xb_modes_command_apply")
+        else:
+            if check:
+                status = ctypes.c_int32(0)
+                N.check(N.cpu().xb_cpu_modes_command_check(ctypes.byref(s), rows.data_ptr(), m, rec.data_ptr(),
+                                                           ctypes.byref(status)), "xb_cpu_modes_command_check")
+                _refusal(status.value)
+            N.check(N.cpu().xb_cpu_modes_command_apply(ctypes.byref(s), rows.data_ptr(), m, rec.data_ptr(), mark),
+                    "xb_cpu_modes_command_apply")
+        return rec
 
     def _stop_rows(self, mask: torch.Tensor) -> None:
-        follower = mask & self._following()
-        self.effect_stamp = torch.where(follower, 0, self.effect_stamp)
-        self.follow_ptr = torch.where(follower, -1, self.follow_ptr)
-        self.follow_id = torch.where(follower, 0, self.follow_id)
-        self.owner_id = torch.where(follower, 0, self.owner_id)
-        self.follow_range = torch.where(follower, 0.0, self.follow_range)
-        self.mode = torch.where(mask, STOP, self.mode)
+        """Stop(Ball*) on the rows in mask: a follower forgets its target; the mode becomes STOP."""
+        self._command("stop", mask)
 
-    def _members(self, rows: torch.Tensor) -> torch.Tensor:
-        """The balls in the follower set of a row in `rows` as StopAllFollowers acts on it: follow_ptr names the row
-        and the follow id is its id."""
-        fp = self.follow_ptr
-        safe = torch.where(fp >= 0, fp, 0)
-        return (fp >= 0) & (self.follow_id == self.ids[safe]) & rows[safe]
+    def _release_followers(self, mask: torch.Tensor) -> None:
+        """StopAllFollowers of the balls in mask: their missiles (no longer massive) and interactive orbiters fly on
+        along their velocities, their other followers stop."""
+        self._command("release", mask, release=True)
 
     def followers(self, row: int) -> list:
-        sel = (self.follow_ptr == row) & (self.follow_id == self.ids[row])
-        return sorted(self.ids[sel].tolist())
-
-    # ---- commands (vec_modes.py:315-417) ----
+        """The ids of the balls following the ball of `row` (their follow pointer at the row, naming it by id)."""
+        fol = torch.nonzero((self.follow_ptr == int(row)) & (self.follow_id == self.ids[int(row)])).reshape(-1)
+        return sorted(int(i) for i in self.ids[fol].tolist())
 
     def stop(self, mask: torch.Tensor) -> None:
-        self._stop_rows(mask & (self.mode != STOP))
+        self._command("stop", mask)
 
     def goto_point(self, mask: torch.Tensor, point: torch.Tensor) -> None:
-        point = point.to(device=self.dev, dtype=F64)
-        ok = mask & torch.isfinite(point).all(dim=1)
-        self._stop_rows(ok)
-        self.goto = torch.where(ok[:, None], point, self.goto)
-        self.speed_fraction = torch.where(ok & (self.speed_fraction == 0.0), 1.0, self.speed_fraction)
-        self.mode = torch.where(ok, GOTO, self.mode)
+        self._command("goto", mask, x=point)
 
     def goto_direction(self, mask: torch.Tensor, direction: torch.Tensor) -> None:
-        d = direction.to(device=self.dev, dtype=F64)
-        ok = mask & torch.isfinite(d).all(dim=1)
-        self.goto_point(ok, self.pos + _unit(d) * DIR_DIST)
+        self._command("goto_direction", mask, x=direction)
 
     def _refuse_mushroom_targets(self, ok: torch.Tensor, trow: torch.Tensor) -> None:
         """A mushroom as a target (its followers would be released at its removal inside the tick, before or after
@@ -640,26 +682,11 @@ class VecModesPark(metaclass=_Fields):
         if bool((ok & (self.mode[trow] == MUSHROOM)).any()):
             raise ValueError("a mushroom as a follow, orbit, missile or formation target is not supported")
 
-    def _follow(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor, mode: int) -> None:
-        rng = f32(range_.to(device=self.dev, dtype=F64))
-        target = target.to(device=self.dev, dtype=I64)
-        inside = (target >= 0) & (target < self.n)
-        safe = torch.where(inside, target, 0)
-        rows = torch.arange(self.n, device=self.dev)
-        ok = (mask & torch.isfinite(rng) & inside & (safe != rows) & (self.world[safe] == self.world)
-              & ~self.moribund[safe] & (self.bubble[safe] == self.bubble) & (self.cloaked[safe] == 0))
-        self._refuse_mushroom_targets(ok, safe)
-        self._stop_rows(ok)
-        self.follow_id = torch.where(ok, self.ids[safe], self.follow_id)
-        self.follow_ptr = torch.where(ok, safe, self.follow_ptr)
-        self.follow_range = torch.where(ok, rng, self.follow_range)
-        self.mode = torch.where(ok, mode, self.mode)
-
     def follow(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor) -> None:
-        self._follow(mask, target, range_, FOLLOW)
+        self._command("follow", mask, ints=(target,), x=range_, check=True)
 
     def orbit(self, mask: torch.Tensor, target: torch.Tensor, range_: torch.Tensor) -> None:
-        self._follow(mask, target, range_, ORBIT)
+        self._command("orbit", mask, ints=(target,), x=range_, check=True)
 
     # ---- missiles (VecCombatPark's API, src/exact_ballpark/vec_combat.py:339-396; full.py:1679-1736) ----
 
@@ -679,23 +706,7 @@ class VecModesPark(metaclass=_Fields):
         in another bubble or cloaked, aimed at it (follow range minus both radii, a float), straight on for 800 ms
         toward 1e16 m along its velocity, at full speed fraction."""
         dst = target.to(device=self.dev, dtype=I64)
-        own = owner.to(device=self.dev, dtype=I64)
-        trow = self.lookup(dst)
-        tsafe = trow.clamp(min=0)
-        src = self.ids
-        ok = (mask.to(self.dev) & (dst >= 0) & (src != dst) & (src != own) & (dst != own) & (trow >= 0)
-              & ~self.moribund[tsafe])
-        self._refuse_mushroom_targets(ok, tsafe)
-        self._stop_rows(ok)
-        ok = ok & (self.bubble == self.bubble[tsafe]) & (self.cloaked[tsafe] == 0)
-        self.follow_id = torch.where(ok, dst, self.follow_id)
-        self.follow_ptr = torch.where(ok, trow, self.follow_ptr)
-        self.follow_range = torch.where(ok, -f32(self.radius + self.radius[tsafe]), self.follow_range)
-        self.owner_id = torch.where(ok, own, self.owner_id)
-        self.effect_stamp = torch.where(ok, self.clock, self.effect_stamp)
-        self.goto = torch.where(ok[:, None], _unit(self.vel) * 1.0e16, self.goto)
-        self.speed_fraction = torch.where(ok, 1.0, self.speed_fraction)
-        self.mode = torch.where(ok, MISSILE, self.mode)
+        self._command("missile_follow", mask, ints=(self.lookup(dst), dst, owner), check=True)
 
     def launch_missile(self, mask: torch.Tensor, target: torch.Tensor, owner: torch.Tensor, aimed: torch.Tensor,
                        massive: torch.Tensor) -> None:
@@ -704,45 +715,13 @@ class VecModesPark(metaclass=_Fields):
         launcher's velocity (its heading backwards when at rest) at max(150, its maximum speed); massive if asked;
         SetBallPosition (re-filed) and SetBallVelocity (snapped); then MissileFollow, or GotoDirection without a
         target. At rest without dynamical orientation the heading is the display orientation's (display_ypr)."""
-        ch = lambda t: ((t.to(device=self.dev, dtype=I64) + 128) & 0xFF) - 128
         dst = target.to(device=self.dev, dtype=I64)
         own = owner.to(device=self.dev, dtype=I64)
-        aimed, massive = ch(aimed), ch(massive)
-        go = mask.to(self.dev) & (dst >= 0)
-        lrow = self.lookup(own.abs())
-        go = go & (lrow >= 0)
-        lsafe = lrow.clamp(min=0)
-        trow = torch.where(dst != own, self.lookup(dst), -1)
-        has_t = trow >= 0
-        tsafe = trow.clamp(min=0)
-        self._refuse_mushroom_targets(go & has_t, tsafe)
-        ps = self.pos[lsafe]
-        aim = has_t & (aimed == 1) & (self.cloaked[tsafe] == 0)
-        v_aim = _unit(self.pos[tsafe] - ps)
-        vs = self.vel[lsafe]
-        max_velocity = torch.where(aim, 0.0, self.max_vel[lsafe])
-        direction = _unit(vs)
-        still = _dot(direction, direction) == 0.0
-        need = go & ~aim & still
-        if bool(need.any()):
-            if self.dynamical_orientation:
-                q = self.rot[lsafe]
-            elif self.display_ypr:
-                q = rotation_ypr(self.yaw[lsafe], self.pitch[lsafe], self.roll[lsafe])
-            else:
-                raise ValueError("a launcher at rest aims by its heading: without dynamical orientation, that is the "
-                                 "display orientation (make the park with display_ypr=True)")
-            direction = torch.where(still[:, None], rotated_vector(q, (0.0, 0.0, -1.0)), direction)
-        speed = torch.where(MISSILE_LAUNCH_SPEED < max_velocity, max_velocity, MISSILE_LAUNCH_SPEED)
-        v0 = torch.where(aim[:, None], v_aim, vs + direction * speed[:, None])
-        self.massive = torch.where(go & (massive != 0), True, self.massive)
-        self.pos = torch.where(go[:, None], ps, self.pos)                    # SetBallPosition
-        self.old_pos = torch.where(go[:, None], ps, self.old_pos)
-        self._refile(go)
-        self.set_velocity(go, v0)
-        cloaked_t = has_t & (self.cloaked[tsafe] != 0)
-        self.goto_direction(go & ~has_t & (max_velocity > 0.0), v0)
-        self.missile_follow(go & has_t & ~cloaked_t, dst, own)
+        launcher = torch.where(dst >= 0, self.lookup(own.abs()), -1)
+        tgt = torch.where(dst != own, self.lookup(dst), -1)
+        flag = (aimed.to(device=self.dev, dtype=I64) & 0xFF) | ((massive.to(device=self.dev, dtype=I64) & 0xFF) << 8)
+        self._command("launch_missile", mask.to(self.dev) & (launcher >= 0), flag=flag, ints=(launcher, tgt, own),
+                      check=True)
 
     # ---- removal ----
 
@@ -761,8 +740,7 @@ class VecModesPark(metaclass=_Fields):
             d = delay.to(device=self.dev, dtype=I64)
             later = mask & (d > 0)
             if bool(later.any()):
-                self._stop_rows(later)
-                self._release_followers(later)
+                self._command("stop_release", later, release=True)
                 self.moribund = torch.where(later, True, self.moribund)
                 self.massive = torch.where(later, False, self.massive)
                 self.effect_stamp = torch.where(later, self.clock + d, self.effect_stamp)
@@ -770,8 +748,7 @@ class VecModesPark(metaclass=_Fields):
                 mask = mask & ~later
                 if not bool(mask.any()):
                     return
-        self._stop_rows(mask)
-        self._release_followers(mask)
+        self._command("stop_release", mask, release=True)
         self.ids = torch.where(mask, EMPTY_ID, self.ids)
         self._index = None
         self.moribund = torch.where(mask, True, self.moribund)
@@ -1080,23 +1057,9 @@ class VecModesPark(metaclass=_Fields):
 
     def warp_to(self, mask: torch.Tensor, point: torch.Tensor, min_range: torch.Tensor,
                 warp_factor: torch.Tensor) -> None:
-        point = point.to(device=self.dev, dtype=F64)
-        mr = min_range.to(device=self.dev, dtype=F64)
-        wf = warp_factor.to(device=self.dev, dtype=I64)
-        ok = mask & torch.isfinite(point).all(dim=1)
-        mr = torch.where(mr < 0.0, 0.0, mr)
-        wf = torch.where(wf <= 0, 1, wf)
-        d = self.pos - point
-        short = ok & (_dot(d, d) < MIN_WARP_DISTANCE_SQ)
-        self.goto_point(short, point)
-        ok = ok & ~short & ~((self.mode == WARP) & (self.goto == point).all(dim=1))
-        self._stop_rows(ok)
-        self.goto = torch.where(ok[:, None], point, self.goto)
-        self.effect_stamp = torch.where(ok, -1, self.effect_stamp)
-        self.follow_range = torch.where(ok, 0.0, self.follow_range)
-        self.follow_id = torch.where(ok, mr.contiguous().view(I64), self.follow_id)
-        self.owner_id = torch.where(ok, wf, self.owner_id)
-        self.mode = torch.where(ok, WARP, self.mode)
+        x = torch.cat((point.to(device=self.dev, dtype=F64).reshape(self.n, 3),
+                       min_range.to(device=self.dev, dtype=F64).reshape(self.n, 1)), dim=1)
+        self._command("warp", mask, ints=(warp_factor,), x=x)
 
     def commands(self, op: torch.Tensor, point: torch.Tensor, target: torch.Tensor, range_: torch.Tensor,
                  min_range: torch.Tensor, warp_factor: torch.Tensor, rows: torch.Tensor | None = None) -> None:
@@ -1125,78 +1088,37 @@ class VecModesPark(metaclass=_Fields):
             rc = N.cpu().xb_cpu_modes_commands(ctypes.byref(s), None if rows is None else rows.data_ptr(), nr, *ptrs)
             N.check(rc, "xb_cpu_modes_commands")
 
-    def _release_followers(self, warping: torch.Tensor) -> None:
-        """StopAllFollowers (modes.py:410-426): missiles (non-massive from then on) and interactive orbiters fly on
-        along their velocities, the other followers stop."""
-        f = self._members(warping)
-        missile = f & (self.mode == MISSILE)
-        fly = missile | (f & self.interactive & (self.mode == ORBIT))
-        self.massive = torch.where(missile, False, self.massive)
-        self._stop_rows(f & ~fly)
-        self.goto_direction(fly, self.vel.clone())
-
     def entity_warp_in(self, mask: torch.Tensor, point: torch.Tensor, warp_factor: torch.Tensor) -> None:
-        point = point.to(device=self.dev, dtype=F64)
-        self.warp_to(mask, point, torch.zeros_like(point[:, 0]), warp_factor)
-        go = mask & (self.mode != GOTO)
-        self._release_followers(go)
-        delta = _unit(self.pos - self.goto)
-        mr = self.follow_id.contiguous().view(F64)
-        dst = self.goto + delta * mr[:, None]
-        self.goto = torch.where(go[:, None], dst, self.goto)
-        self.massive = torch.where(go, False, self.massive)
-        self.sensor = torch.where(go, False, self.sensor)
-        self.vel = torch.where(go[:, None], _unit(point) * AU, self.vel)
-        if self.dynamical_orientation:                  # SnapOrientation to the new velocity (full.py:1574-1581)
-            self._orient_setup(go, 0)
-        self.effect_stamp = torch.where(go, torch.clamp(self.clock - 5, min=0), self.effect_stamp)
-        back = self.pos - self.goto
-        self.last_collision = torch.where(go, _sqrt(_dot(back, back)), self.last_collision)
+        self._command("warp_in", mask, ints=(warp_factor,), x=point, release=True)
 
     # ---- setters (vec_orient.py:403-466; SetBallVelocity without dynamical orientation, vec_modes.py:421-428) ----
 
     def set_mass(self, mask: torch.Tensor, mass: torch.Tensor) -> None:
-        m = mass.to(device=self.dev, dtype=F64)
-        ok = mask & ~(m <= 0.0)
-        self.mass = torch.where(ok, m, self.mass)
-        self.time_factor = torch.where(ok, self._decay(self.mass * self.agility), self.time_factor)
+        self._command("set_mass", mask, x=mass)
 
     def set_agility(self, mask: torch.Tensor, agility: torch.Tensor) -> None:
-        a = f32(agility.to(device=self.dev, dtype=F64))
-        ok = mask & ~(a <= 0.0)
-        self.agility = torch.where(ok, a, self.agility)
-        self.time_factor = torch.where(ok, self._decay(self.mass * self.agility), self.time_factor)
+        self._command("set_agility", mask, x=agility)
+
+    def _count_refiles(self, rec: torch.Tensor | None, mask: torch.Tensor) -> None:
+        """`refiled`: the SetMaxSpeed and SetBallRadius calls that took their value, one a row."""
+        if rec is None:
+            return
+        rows = torch.nonzero(mask.to(self.dev).reshape(self.n)).reshape(-1)
+        self.refiled.index_add_(0, rows, ((rec[:, CMD_STATE_WORD] & CMD_ACCEPTED) != 0).to(I64))
 
     def set_max_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
-        v = f32(v.to(device=self.dev, dtype=F64))
-        ok = mask & ~(v < 0.0)
-        self.max_vel = torch.where(ok, v, self.max_vel)
-        self.refiled += ok.to(I64)
-        self._refile(ok)
+        self._count_refiles(self._command("set_max_vel", mask, x=v), mask)
 
     def set_speed_fraction(self, mask: torch.Tensor, s: torch.Tensor) -> None:
-        s = f32(s.to(device=self.dev, dtype=F64))
-        ok = mask & torch.isfinite(s)
-        s = torch.where(s < 0.0, 0.0, torch.where(s > 1.0, 1.0, s))
-        self.speed_fraction = torch.where(ok, s, self.speed_fraction)
+        self._command("set_speed_fraction", mask, x=s)
 
     def set_radius(self, mask: torch.Tensor, r: torch.Tensor) -> None:
-        r = f32(r.to(device=self.dev, dtype=F64))
-        ok = mask & ~(r < 0.0)
-        self.radius = torch.where(ok, r, self.radius)
-        self.refiled += ok.to(I64)
-        self._refile(ok)
+        self._count_refiles(self._command("set_radius", mask, x=r), mask)
 
     def set_velocity(self, mask: torch.Tensor, v: torch.Tensor) -> None:
         """SetBallVelocity: the new and old velocity; with dynamical orientation the rotation snapped to it, with the
         display orientation kept that snapped (full.py:1583-1593)."""
-        v = v.to(device=self.dev, dtype=F64)
-        self.vel = torch.where(mask[:, None], v, self.vel)
-        self.old_vel = torch.where(mask[:, None], v, self.old_vel)
-        if self.dynamical_orientation:
-            self._orient_setup(mask, 0)
-        elif self.display_ypr:
-            self._ypr_snap(mask)
+        self._command("set_velocity", mask, x=v)
 
     # ---- dynamical orientation's setters (vec_orient.py:438-464) ----
 
@@ -1223,71 +1145,40 @@ class VecModesPark(metaclass=_Fields):
 
     def set_max_angular_speed(self, mask: torch.Tensor, w: torch.Tensor) -> None:
         """SetMaxAngularSpeed: the float ignored if negative."""
-        w = f32(w.to(device=self.dev, dtype=F64))
-        self.max_ang_vel = torch.where(mask & ~(w < 0.0), w, self.max_ang_vel)
+        self._command("set_max_ang_speed", mask, x=w)
 
     def set_rot_agility(self, mask: torch.Tensor, a: torch.Tensor) -> None:
-        """SetBallAngularAgility: the float ignored unless positive."""
-        a = f32(a.to(device=self.dev, dtype=F64))
-        self.rot_agility = torch.where(mask & ~(a <= 0.0), a, self.rot_agility)
+        """SetBallAngularAgility: ignored unless positive (a float)."""
+        self._command("set_rot_agility", mask, x=a)
 
     def set_angular_velocity(self, mask: torch.Tensor, w: torch.Tensor) -> None:
         """SetBallAngularVelocity: the new and old angular velocity, as floats."""
-        w = f32(w.to(device=self.dev, dtype=F64))
-        self.ang_vel = torch.where(mask[:, None], w, self.ang_vel)
-        self.old_ang_vel = torch.where(mask[:, None], w, self.old_ang_vel)
+        self._command("set_ang_vel", mask, x=w)
 
     def set_rotation_ypr(self, mask: torch.Tensor, ypr: torch.Tensor) -> None:
         """SetBallRotation(RotationQuaternion(yaw, pitch, roll)) with ypr [n, 3] (yaw, pitch, roll; each as a float)."""
-        ypr = ypr.to(device=self.dev, dtype=F64)
-        self.set_rotation(mask, rotation_ypr(ypr[:, 0], ypr[:, 1], ypr[:, 2]))
+        self._command("set_rotation_ypr", mask, x=ypr)
 
     def set_rotation(self, mask: torch.Tensor, q: torch.Tensor) -> None:
-        """SetBallRotation: the floats normalised when longer than 1e-20 (otherwise ignored), the rotation stopped.
-        Each float operation as a double operation rounded to float (the same for + - * / sqrt of floats)."""
-        r = f32(q.to(device=self.dev, dtype=F64))
-        x, y, z, w = r.unbind(1)
-        sq = f32(f32(f32(f32(x * x) + f32(y * y)) + f32(z * z)) + f32(w * w))
-        length = f32(_sqrt(sq))
-        ok = mask & (length > 1e-20)
-        d = f32(torch.ones_like(length) / torch.where(ok, length, 1.0))
-        r = f32(r * d[:, None])
-        self.rot = torch.where(ok[:, None], r, self.rot)
-        self.old_rot = torch.where(ok[:, None], r, self.old_rot)
-        self.ang_vel = torch.where(ok[:, None], 0.0, self.ang_vel)
-        self.old_ang_vel = torch.where(ok[:, None], 0.0, self.old_ang_vel)
+        """SetBallRotation: the float quaternion [n, 4] normalised when longer than 1e-20 (otherwise ignored), the
+        rotation stopped."""
+        self._command("set_rotation", mask, x=q)
 
     def set_free(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
         """SetBallFree (vec_modes.py:438-451; full.py:1555-1572 without minis); an empty row is no ball: unchanged."""
-        flag = flag.to(self.dev) != 0
-        change = mask.to(self.dev) & (flag != self.free) & (self.ids != EMPTY_ID)
-        fix = change & ~flag
-        self.free = torch.where(change, flag, self.free)
-        self._stop_rows(fix)
-        self.set_velocity(fix, torch.zeros_like(self.vel))
-        self._refile(fix)
-        self.last_g = torch.where(fix[:, None], 0.0, self.last_g)
+        self._command("set_free", mask, flag=(flag.to(self.dev) != 0))
 
     def set_ball_rigid(self, mask: torch.Tensor) -> None:
-        self._stop_rows(mask)
-        self.mode = torch.where(mask, RIGID, self.mode)
+        self._command("set_rigid", mask)
 
     def set_boid(self, mask: torch.Tensor, on: torch.Tensor) -> None:
-        on = on.to(self.dev) != 0
-        self.mode = torch.where(mask & on, BOID, self.mode)
-        self._stop_rows(mask & ~on)
+        self._command("set_boid", mask, flag=(on.to(self.dev) != 0))
 
     def set_interactive(self, mask: torch.Tensor, flag: torch.Tensor) -> None:
-        self.interactive = torch.where(mask, flag.to(self.dev) != 0, self.interactive)
+        self._command("set_interactive", mask, flag=(flag.to(self.dev) != 0))
 
     def set_ball_troll(self, mask: torch.Tensor, delay: torch.Tensor) -> None:
-        d = delay.to(device=self.dev, dtype=I64)
-        d = torch.where(d < TROLL_MIN_DELAY, TROLL_MIN_DELAY, d)
-        self._stop_rows(mask)
-        self.set_free(mask, torch.ones_like(mask))
-        self.set_interactive(mask, torch.ones_like(mask))
-        self.effect_stamp = torch.where(mask, self.clock + d, self.effect_stamp)
-        self.mode = torch.where(mask, TROLL, self.mode)
+        self._command("set_troll", mask, ints=(delay,))
 
     # ---- one tick (Ballpark::Evolve; vec_modes.py:943-960) ----
 
@@ -1352,42 +1243,53 @@ class VecModesPark(metaclass=_Fields):
             raise ValueError("evolve_torch is the CPU park's tick; a GPU park ticks with evolve()")
         self.evolve(active=active)
 
-    # ---- OnTick (vec_orient.py:834-879) ----
+    # ---- OnTick (Ballpark::OnTick; full.py:2590-2620, vec_orient.py:834-879) ----
 
-    def _world_split(self, x: torch.Tensor) -> torch.Tensor:
-        order = torch.argsort(self.world, stable=True)
-        w, v = self.world[order], x[order]
-        return ((w[1:] == w[:-1]) & (v[1:] != v[:-1])).any()
+    def _world_times_disagree(self, sim: torch.Tensor) -> bool:
+        """Whether two rows of one world hold different simulation times (OnTick runs whole worlds)."""
+        _, wk = torch.unique(self.world, return_inverse=True)
+        lo = torch.full((int(wk.max()) + 1,), torch.iinfo(torch.int64).max, dtype=I64, device=self.dev)
+        hi = torch.full_like(lo, torch.iinfo(torch.int64).min)
+        lo.scatter_reduce_(0, wk, sim, reduce="amin")
+        hi.scatter_reduce_(0, wk, sim, reduce="amax")
+        return bool((lo != hi).any())
 
     def on_tick(self, sim_time, tick=None, after=None) -> list:
-        """Ballpark::OnTick for every world at its own simulation time (100 ns units, a world's rows sharing it): less
-        than one whole interval since the world's `time`: nothing; a world's first call evolves once; every other
-        world runs its whole intervals under OnTick's mask, the remainder carried. Returns the masks of the ticks run."""
+        """Ballpark::OnTick for every world at its own simulation time (`sim_time` [n] in 100 ns units, a world's rows
+        sharing it): BringOutTheDeads first, in every world; then per world the intervals it is due, an interval being
+        the tick interval x 10,000: none when less than one has passed since the world's `time` (its time kept); one
+        on its first call (its time then `sim_time`); on a master's park (`master=True`) one when more than
+        MAX_CATCH_UP_TICKS are due (the backlog dropped); otherwise every whole interval, the remainder carried in its
+        time. Round j ticks the worlds due more than j intervals together (`tick(rows)`, by default
+        `evolve(active=rows)`, `tick(None)` when every row is due); `after(rows)` follows each round. Returns each
+        round's row mask."""
         n, d = self.n, self.dev
-        if getattr(self, "time", None) is None:
+        if getattr(self, "time", None) is None:          # Be::Time mTime and mFirstTime, as a new ballpark holds them
             self.time = torch.zeros(n, dtype=I64, device=d)
             self.first_time = torch.ones(n, dtype=torch.bool, device=d)
         sim = torch.as_tensor(sim_time, dtype=I64).to(d).reshape(n)
-        interval = int(round(self.dt * 1000.0)) * 10000
-        last = sim - self.time
-        go = ~(last < interval)
-        k = torch.where(go, torch.div(last, interval, rounding_mode="floor"), 0)
-        first = go & self.first_time
-        rest = go & ~first
-        top = torch.maximum(torch.where(rest, k, 0).max(), first.any().to(I64)) if n else torch.zeros((), dtype=I64)
-        top, split = torch.stack((top.to(I64), self._world_split(sim).to(I64))).tolist() if n else (0, 0)
-        if split:
+        if n and self._world_times_disagree(sim):
             raise ValueError("on_tick: a world's rows hold different simulation times")
+        self.bring_out_the_deads()
+        step = int(self.tick_interval) * 10000
+        behind = sim - self.time
+        due = behind >= step
+        whole = torch.where(due, torch.div(behind, step, rounding_mode="floor"), 0)
+        fresh = due & self.first_time                   # a first call evolves once
+        resync = (whole > MAX_CATCH_UP_TICKS) if self.master else torch.zeros_like(due)   # a master drops a backlog
+        rounds = torch.where(fresh | resync, 1, whole)
+        carry = torch.where(fresh | resync, 0, behind - whole * step)
+        total = int(rounds.max()) if n else 0
         tick = tick or (lambda rows: self.evolve(active=rows))
         masks = []
-        for j in range(top):
-            m = (rest & (k > j)) | (first & (j == 0))
-            tick(None if bool(m.all()) else m)
+        for j in range(total):
+            rows = rounds > j
+            tick(None if bool(rows.all()) else rows)
             if after is not None:
-                after(m)
-            masks.append(m)
-        self.first_time = self.first_time & ~first
-        self.time = torch.where(go, sim - torch.where(rest, last - k * interval, 0), self.time)
+                after(rows)
+            masks.append(rows)
+        self.first_time = self.first_time & ~fresh
+        self.time = torch.where(due, sim - carry, self.time)
         return masks
 
 

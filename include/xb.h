@@ -169,6 +169,35 @@ int xb_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t 
                       const double* point, const int64_t* target, const double* range, const double* min_range,
                       const int64_t* warp_factor, void* stream);
 
+/* One listed row's command or setter between ticks (exact-ballpark_amd/csrc/xb_commands.h numbers the ops and gives
+   each one's arguments): the caller fills op, flag, i and x and leaves the rest zero; the gather step of
+   xb_modes_command_check and xb_modes_command_apply fills the rest from other rows (a target's, a launcher's). Every
+   word is 8 bytes. */
+typedef struct xb_command {
+    int64_t op, flag;
+    int64_t i[3];
+    double x[4];
+    double l_pos[3], l_vel[3], l_max_vel, l_rot[4];     /* gathered: the launcher */
+    double t_pos[3], t_radius;                          /* gathered: the target */
+    int64_t t_id, t_world, t_bubble, t_cloaked, t_state;
+} xb_command;
+
+/* The gather step alone for rows[j] with cmds[j] (j < n_rows): fills each record from the other rows it names and ORs
+   into *status (device memory, cleared by the caller) the refusals the batch would meet (1 a mushroom as a follow,
+   orbit or missile target; 2 a missile launcher at rest without dynamical or display orientation); writes nothing to
+   the park. The reference refuses these in its masked commands before any change (vec_combat.py:339-398). */
+int xb_modes_command_check(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                           int32_t* status, void* stream);
+
+/* The batch: gather, then each listed row's command on its own row (other rows read only through its record, so the
+   result is that of reading every row before writing any); then, when `mark` is not NULL (n bytes, all zero on entry
+   and on return), StopAllFollowers for the balls an EntityWarpIn or a removal's stop marked, over every row of the
+   park. One row listed twice is a race. Replaces the reference's masked commands and setters on device tensors
+   (src/exact_ballpark/vec_modes.py:286-483, vec_orient.py:403-478, vec_combat.py:339-398) and the server kernel's
+   per-row commands (vec_server_cuda.cu:117-186). */
+int xb_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                           uint8_t* mark, void* stream);
+
 /* Kernel attributes as loaded on the current device: which 0 tiled, 1 general main, 2 general prepass, 3 tiled with
    dynamical orientation, 4 tiled with collisions, 5 with both, 6 tiled keeping the display orientation, 7 that with
    collisions, 8 tiled with collisions and static collidables, 9 that with dynamical orientation;
@@ -270,6 +299,12 @@ int xb_cpu_statics_setup(int64_t n, const int64_t* kind, const int64_t* id, cons
 
 /* xb_modes_ypr_snap on a park in host memory (no stream). */
 int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int64_t n_rows);
+
+/* xb_modes_command_check and xb_modes_command_apply on a park in host memory (no stream; status in host memory). */
+int xb_cpu_modes_command_check(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                               int32_t* status);
+int xb_cpu_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
+                               uint8_t* mark);
 
 /* xb_modes_commands on a park in host memory (no stream). */
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,

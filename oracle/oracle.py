@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
         L.or_warp_to.argtypes = [vp, i64, d, d, d, d, i64]
         L.or_entity_warp_in.argtypes = [vp, i64, d, d, d, i64]
         L.or_evolve.argtypes = [vp]
+        L.or_on_tick.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
+        L.or_on_tick.restype = ctypes.c_int
         L.or_count.argtypes = [vp]
         L.or_get_ball.argtypes = [vp, i, pd, pq]
         L.or_get_followers.argtypes = [vp, i, pq]
@@ -240,6 +242,10 @@ class World:
 
     def evolve(self) -> None:
         self.L.or_evolve(self.p)
+
+    def on_tick(self, sim_time: int, master: bool = False) -> int:
+        """Ballpark::OnTick at sim_time (100 ns units): the ticks it ran."""
+        return int(self.L.or_on_tick(self.p, int(sim_time), int(bool(master))))
 
     def turning(self) -> np.ndarray:
         """[n, 20] float64, balls by ascending id: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel,

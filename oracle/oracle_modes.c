@@ -106,6 +106,8 @@ typedef struct {
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
     int ns;
     OStatic statics[MAX_STATICS];              /* FullBallpark's capsules and oriented boxes, in the order added */
+    int64_t time;                              /* OnTick's Be::Time mTime (full.py:2590-2620) */
+    int ticked_once;                           /* not mFirstTime: OnTick has evolved this park */
     int nform;                                 /* the formations (LoadFormations): each one's slot offsets */
     int form_len[MAX_FORMATIONS];
     double form[MAX_FORMATIONS][MAX_FORM_SLOTS][3];
@@ -2313,4 +2315,31 @@ void or_bring_out_the_deads(OPark* p) {
         }
     }
     for (int i = 0; i < nd; ++i) or_remove_ball(p, due[i]);
+}
+
+/* Ballpark::OnTick (full.py:2590-2620): BringOutTheDeads; nothing before one whole interval (tick x 10,000 in 100 ns
+   units) since mTime; on a master a backlog of more than MAX_CATCH_UP_TICKS (5) intervals is one; the first call
+   evolves once and takes the time as it is; otherwise one Evolve an interval, the remainder kept in mTime. Returns the
+   ticks run. */
+int or_on_tick(OPark* p, int64_t sim_time, int master) {
+    int64_t last = sim_time - p->time;
+    or_bring_out_the_deads(p);
+    const int64_t interval = (int64_t)p->tick_ms * 10000;
+    if (last < interval) return 0;
+    if (master && last / interval > 5) last = interval;
+    int ran = 0;
+    if (!p->ticked_once) {
+        or_evolve(p);
+        ran = 1;
+        last = 0;
+        p->ticked_once = 1;
+    } else {
+        for (int64_t k = last / interval; k > 0; --k) {
+            or_evolve(p);
+            ++ran;
+            last -= interval;
+        }
+    }
+    p->time = sim_time - last;
+    return ran;
 }

@@ -140,7 +140,8 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
 
 
 def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bool = False,
-             general: bool = False, order=None, initial: bool = False, events: dict | None = None) -> dict:
+             general: bool = False, order=None, initial: bool = False, events: dict | None = None,
+             ontick: bool = False) -> dict:
     """{(world index, step): (d, q, followers)} after each recorded tick (`record`: a set of steps, None for all).
     `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending);
     `initial`: also step 0, the state as loaded. A world flagged `ypr` keeps the display orientation."""
@@ -149,9 +150,9 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
         w = worlds[wi]
         groups[(w["friction"], w["tick"], bool(w["new_orbit"]), bool(w.get("dynamical", False)),
                 bool(w.get("collisions", False)), bool(w.get("ypr", False)),
-                repr(w.get("formations")))].append(wi)       # one formation table a park, as LoadFormations keeps
+                repr(w.get("formations")), bool(w.get("master", False)))].append(wi)   # one formation table a park
     out = {}
-    for (friction, tick, new_orbit, dynamical, coll, ypr, _), members in groups.items():
+    for (friction, tick, new_orbit, dynamical, coll, ypr, _, master), members in groups.items():
         minis = {wi: VecModesPark.mini_rows(worlds[wi]["balls"], worlds[wi]["minis"], wi, worlds[wi]["start"])
                  for wi in members if worlds[wi].get("minis")}
         balls = [(wi, b) for wi in members
@@ -160,7 +161,7 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                     else [])]     # the world's miniballs; its empty row for a mushroom
         park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
                             use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
-                            display_ypr=ypr)
+                            display_ypr=ypr, master=master)
         park.load([VecModesPark.empty_row(wi, worlds[wi]["start"]) if b is None else b["_row"] if "_row" in b else
                    {"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
@@ -200,7 +201,10 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                         by_op[cmds[j][1]].append((wi, cmds[j]))
                 for op, items in by_op.items():
                     _apply(park, op, items, index)
-            if g is not None:
+            if ontick:                              # Ballpark::OnTick at each world's s-th simulation time
+                sim = [worlds[wi]["calls"][min(s, len(worlds[wi]["calls"])) - 1] for wi, _ in balls]
+                park.on_tick(torch.tensor(sim, dtype=torch.int64))
+            elif g is not None:
                 g.tick()
             else:
                 park.evolve()
@@ -252,7 +256,10 @@ def _run_flags(w: dict, initial: bool = False):
         if step:
             for c in by_step[step]:
                 o.command(c)
-            o.evolve()
+            if "calls" in w:                        # an OnTick world: its step-th simulation time
+                o.on_tick(w["calls"][step - 1], w.get("master", False))
+            else:
+                o.evolve()
         d, q, fol = o.state()
         parts, qs, lists = [d[:, D_COLS]], [q[:, Q_COLS]], fol
         if dyn:

@@ -1401,3 +1401,82 @@ def delayed_remove_cases(seed: int, scale: int = 1) -> list:
         w["cmds"] = sorted(cmds, key=lambda q: (q[0], q[1] != BRING_OP))
         w["kind"] = "delayed_remove"
     return out
+
+
+def ontick_cases(seed: int, scale: int = 1) -> list:
+    """OnTick worlds (about 70 x scale): delayed_remove_cases' worlds driven by Ballpark::OnTick alone (no
+    BringOutTheDeads of their own: OnTick's first step), each with a schedule of simulation times (`calls`, 100 ns
+    units from 0) whose gaps are drawn from fractions of the tick interval: too early (0.3, 0.999), one interval, a
+    remainder carried (1.5, 2.7, 3.2), a backlog (6.4, 7, 11); a third of the worlds on a master (`master`: every
+    ball interactive, so its one bubble is never dead; a backlog above five intervals runs one); and worlds where more
+    than seven balls fall due within two ticks at once (BringOutTheDeads' buffer of 7). A command of tick s is given
+    before the call whose intervals reach tick s (its `cmds`' first element is that call's index, from 1)."""
+    r = random.Random(seed ^ 0x07C1)
+    base = delayed_remove_cases(seed ^ 0x0711, scale)[:60 * scale]
+    gaps = (0.3, 0.999, 1.0, 1.0, 1.0, 1.5, 2.0, 2.7, 3.2, 6.4, 7.0, 11.0)
+    out = []
+    for k, w in enumerate(base + _buffer_worlds(r, base[: 10 * scale])):
+        w = dict(w)
+        w["master"] = k % 3 == 0
+        if w["master"]:
+            w["balls"] = [dict(b, interactive=True) for b in w["balls"]]
+        interval = w["tick"] * 10000
+        calls, ticks_after, t, done, first = [], [], 0, 0, True
+        while done < w["steps"] and len(calls) < 48:
+            t += int(round(r.choice(gaps) * interval))
+            calls.append(t)
+            # the intervals this call runs, as OnTick counts them (time: the start of the world's carried remainder)
+            last = t - w.setdefault("_time", 0)
+            if last >= interval:
+                whole = last // interval
+                if first or (w["master"] and whole > 5):
+                    n_run, w["_time"] = 1, t
+                else:
+                    n_run, w["_time"] = whole, t - (last - whole * interval)
+                first = False
+                done += n_run
+            ticks_after.append(done)
+        del w["_time"]
+        cmds = []
+        for c in w["cmds"]:
+            if c[1] == BRING_OP:
+                continue
+            before = [i for i, d in enumerate(ticks_after) if d >= c[0]]
+            if before:
+                cmds.append((before[0] + 1,) + tuple(c[1:]))
+        w["cmds"] = sorted(cmds, key=lambda q: q[0])
+        w["calls"] = calls
+        w["steps"] = len(calls)
+        w["kind"] = ("master_" if w["master"] else "") + w["kind"] + "_ontick"
+        out.append(w)
+    return out
+
+
+def _buffer_worlds(r: random.Random, worlds: list) -> list:
+    """Worlds of 10-14 balls (a world's balls copied at new ids and offsets), nine of them removed with a delay of 2
+    at one tick and two more with a delay of 1: more than 7 due within two ticks at the next calls."""
+    out = []
+    for w in worlds:
+        w = dict(w)
+        balls = [dict(b) for b in w["balls"]]
+        ids = {b["id"] for b in balls}
+        k, want = 0, 12 + r.randrange(3)
+        while len(balls) < want:
+            src = dict(w["balls"][k % len(w["balls"])])
+            nid = max(ids) + 1 + r.randrange(1000)
+            ids.add(nid)
+            src["id"] = nid
+            src["pos"] = [p + r.uniform(-3000.0, 3000.0) for p in src["pos"]]
+            src["mode"] = 2
+            balls.append(src)
+            k += 1
+        w["balls"] = balls
+        step = 1 + r.randrange(max(1, w["steps"] // 2))
+        picks = r.sample(sorted(ids), 11)
+        cmds = [c for c in w["cmds"] if c[1] == BRING_OP]
+        for j, bid in enumerate(picks):
+            cmds.append((step, REMOVE_OP, bid, 0, 0.0, 0.0, 0.0, 0.0, 2 if j < 9 else 1))
+        w["cmds"] = sorted(cmds, key=lambda q: (q[0], q[1] != BRING_OP))
+        w["kind"] = "remove_buffer"
+        out.append(w)
+    return out

@@ -31,7 +31,10 @@ def run(w, do=False):
         if step:
             for c in by_step[step]:
                 o.command(c)
-            o.evolve()
+            if "calls" in w:                        # an OnTick world: its step-th simulation time
+                o.on_tick(w["calls"][step - 1], w.get("master", False))
+            else:
+                o.evolve()
         d, q, _ = o.state()
         lc, filing, hits = o.collide()
         tail = o.turning() if do else o.ypr()
