@@ -23,6 +23,8 @@ MUSHROOM_OP, FREE_OP = 20, 21
 HARMONIC_OP = 22
 REMOVE_OP = 23
 BRING_OP = 24
+MASTER_OPS = {30: 1, 31: 2, 32: 3, 33: 4, 34: 5, 35: 6, 36: 7, 37: 8}   # keep alive .. CopyBubbles (oracle_master.c)
+EVENT_NAMES = {1: "DoProximity", 2: "DoRange", 3: "DoTargetTracking", 4: "DoFinalCleanup"}
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
 ND, NQ = 27, 12
 D_NAMES = ("px", "py", "pz", "vx", "vy", "vz", "gx", "gy", "gz", "ax", "ay", "az", "opx", "opy", "opz", "ovx", "ovy",
@@ -101,6 +103,22 @@ def lib() -> ctypes.CDLL:
         L.or_remove_ball.argtypes = [vp, i64]
         L.or_remove_ball_delayed.argtypes = [vp, i64, i64]
         L.or_bring_out_the_deads.argtypes = [vp]
+        L.or_park_set_master.argtypes = [vp, ctypes.c_uint32]
+        L.or_master_op.argtypes = [vp, i, i64, i64, d, d, i64]
+        for name in ("or_master_bubbles", "or_master_tops", "or_master_dicts", "or_master_keep_balls",
+                     "or_master_remote"):
+            getattr(L, name).argtypes = [vp, pq]
+        L.or_master_pairs.argtypes = [vp, i, pq]
+        L.or_master_umap.argtypes = [vp, i, pq]
+        L.or_master_bubble_id.restype = i64
+        L.or_master_bubble_id.argtypes = [vp]
+        L.or_master_events.argtypes = [vp, pq, i]
+        L.or_master_visibility.restype = i64
+        L.or_master_visibility.argtypes = [vp, i64, i64]
+        L.or_master_range.argtypes = [vp, i, i64, d, d, d, d, i, pq, pd]
+        L.or_master_bubble_at.restype = i64
+        L.or_master_bubble_at.argtypes = [vp, d, d, d, ctypes.POINTER(ctypes.c_int)]
+        L.or_master_overflow.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -237,8 +255,83 @@ class World:
             L.or_bring_out_the_deads(p)
         elif op == HARMONIC_OP:                 # harmonic w, corporation b, alliance x, a force field if y
             L.or_set_harmonic(p, a, w, b, int(x), int(y))
+        elif op in MASTER_OPS:                  # the master's space layer (tests/scenarios.py's space ops)
+            L.or_master_op(p, MASTER_OPS[op], a, b, rr, x, w)
         else:
             raise ValueError(op)
+
+    def set_master(self, seed: int) -> None:
+        """The server's view: the partition's tops, bubbles and sensors, srand(seed) for its shuffled sensors."""
+        self.L.or_park_set_master(self.p, int(seed) & 0xFFFFFFFF)
+
+    def master_state(self, queries: dict) -> dict:
+        """The master's structures and queries, as tests/golden/make_golden_space.py records the reference's."""
+        L, p = self.L, self.p
+        if L.or_master_overflow(p):
+            raise OverflowError("a master table past the oracle's bound, or a ball wider than a top box")
+        buf = (ctypes.c_int64 * 65536)()
+        dbuf = (ctypes.c_double * 1024)()
+        n = L.or_count(p)
+        d, q, _ = self.state()
+        ids = [int(x) for x in q[:, 0]]
+        L.or_master_bubbles(p, buf)
+        bub = [int(buf[2 * i]) for i in range(n)]
+        old = [int(buf[2 * i + 1]) for i in range(n)]
+        nt = L.or_master_tops(p, buf)
+        tops = sorted([int(buf[4 * i + k]) for k in range(4)] for i in range(nt))
+        L.or_master_dicts(p, buf)
+        dicts, k = [], 1
+        for _ in range(int(buf[0])):
+            b, c = int(buf[k]), int(buf[k + 1])
+            k += 2
+            dicts.append([b, [[int(buf[k + 2 * j]), int(buf[k + 2 * j + 1])] for j in range(c)]])
+            k += 2 * c
+
+        def pairs(which):
+            m = L.or_master_pairs(p, which, buf)
+            out = {}
+            for i in range(m):
+                out.setdefault(int(buf[2 * i]), []).append(int(buf[2 * i + 1]))
+            return sorted([bb, sorted(v)] for bb, v in out.items())
+        inter, keep = pairs(0), pairs(1)
+        nk = L.or_master_keep_balls(p, buf)
+        keep_balls = sorted(int(buf[i]) for i in range(nk))
+        m = L.or_master_umap(p, -1, buf)
+        prox, prox_bkt = [int(buf[i]) for i in range(m)], [int(buf[m]), int(buf[m + 1])]
+        sensors = []
+        for i in range(n):
+            m = L.or_master_umap(p, i, buf)
+            sensors.append([ids[i], [int(buf[j]) for j in range(m)], int(buf[m]), int(buf[m + 1])])
+        ne = L.or_master_events(p, buf, 16384)
+        events = []
+        for i in range(ne):
+            kind, bid, a, b = (int(buf[4 * i + j]) for j in range(4))
+            name = EVENT_NAMES[kind]
+            events.append([name, bid, a, b] if kind == 1 else [name, bid, a] if kind in (2, 3) else [name, bid])
+        R = float(queries["range"])
+        vis = [int(L.or_master_visibility(p, ids[0], j)) for j in ids[1:]] if ids else []
+        rng, rngd = [], []
+        for i in ids:
+            c = L.or_master_range(p, 1, i, 0.0, 0.0, 0.0, R, 0, buf, dbuf)
+            rng.append([int(buf[j]) for j in range(max(c, 0))])
+            rngd.append([[float(dbuf[j]).hex(), int(buf[j])] for j in range(max(c, 0))])
+        if ids:
+            pt = [float(d[0, k] + queries["point_offset"][k]) for k in range(3)]
+            c = L.or_master_range(p, 0, 0, pt[0], pt[1], pt[2], R, 0, buf, dbuf)
+            point = [int(buf[j]) for j in range(max(c, 0))]
+        else:
+            pt, point = [0.0, 0.0, 0.0], []
+        at = []
+        for i in range(n):
+            found = ctypes.c_int(0)
+            v = L.or_master_bubble_at(p, float(d[i, 0]), float(d[i, 1]), float(d[i, 2]), ctypes.byref(found))
+            at.append(int(v) if found.value else None)
+        nr = L.or_master_remote(p, buf)
+        return {"ids": ids, "bubble": bub, "old_bubble": old, "tops": tops, "dicts": dicts, "inter": inter,
+                "keep": keep, "keep_balls": keep_balls, "prox": prox, "prox_bkt": prox_bkt, "sensors": sensors,
+                "bubble_id": int(L.or_master_bubble_id(p)), "events": events, "vis": vis, "range": rng,
+                "range_dist": rngd, "point": [c.hex() for c in pt], "point_ids": point, "at": at,
+                "remote": [int(buf[i]) for i in range(nr)]}
 
     def evolve(self) -> None:
         self.L.or_evolve(self.p)

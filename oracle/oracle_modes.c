@@ -68,6 +68,16 @@ typedef struct {
     /* the display orientation without dynamical orientation (full.py: yaw, pitch, roll_angle and the rest) */
     double yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta;
     int formation_id;                          /* a leader's formation (-1: none; a char) */
+    /* the master's (oracle_master.c): the ball dictionary's place, the old bubble, the keep-alive mark, the proximity
+       sensor, the notification range and target tracking */
+    int64_t seq, old_bubble;
+    int has_prox;
+    double s_range, s_period, s_elapsed;
+    int s_only;
+    struct { int n; int64_t key[64]; uint64_t bkt, next; } s_balls;
+    int64_t trk_id;                            /* the tracked ball's id, 0: none */
+    double trk_range, ntf_range;
+    int trk_in, ntf_in;
 } OBall;
 
 typedef struct { int level; int64_t k[3]; } OBox;
@@ -106,6 +116,8 @@ typedef struct {
     OBall balls[MAX_BALLS];                    /* kept sorted by id */
     int ns;
     OStatic statics[MAX_STATICS];              /* FullBallpark's capsules and oriented boxes, in the order added */
+    struct OMaster* m;                         /* the master's partition tops, bubbles and sensors (NULL: a client) */
+    int64_t next_seq;                          /* the ball dictionary's next place */
     int64_t time;                              /* OnTick's Be::Time mTime (full.py:2590-2620) */
     int ticked_once;                           /* not mFirstTime: OnTick has evolved this park */
     int nform;                                 /* the formations (LoadFormations): each one's slot offsets */
@@ -178,12 +190,19 @@ OPark* or_park_new(double friction, int tick_interval_ms, int use_new_orbit, int
     p->current_time = current_time;
     return p;
 }
-void or_park_free(OPark* p) { free(p); }
+void or_park_free(OPark* p) {
+    if (p) free(p->m);
+    free(p);
+}
 int or_park_size(void) { return (int)sizeof(OPark); }
 int or_ball_size(void) { return (int)sizeof(OBall); }
 
 static void file_ball(OPark* p, OBall* b);
 static void refile(OPark* p, OBall* b);
+static void master_insert_ball(OPark* p, OBall* b);
+static int master_dead(const OPark* p, const OBall* b);
+static void master_proximities(OPark* p);
+static void master_removed(OPark* p, OBall* b);
 static void yaw_pitch_roll(OPark* p, OBall* b, int snap);
 
 static void refresh_time_factor(OPark* p, OBall* b) {     /* ref.py:82-87 */
@@ -228,8 +247,13 @@ int or_add_ball(OPark* p, int64_t id, double mass, double radius, double max_vel
     b->harmonic = b->corporation = b->alliance = -1;
     b->max_ang_vel = 1.0;
     b->rot_agility = (double)0.9f;
+    b->seq = p->next_seq++;
+    b->old_bubble = -1;
+    b->s_period = 2.0;
+    b->s_balls.bkt = 1;
     refresh_time_factor(p, b);
-    if (p->collisions) file_ball(p, b);                              /* space.py:1150 */
+    if (p->m) master_insert_ball(p, b);                              /* space.py:1150 (the master's filing) */
+    else if (p->collisions) file_ball(p, b);
     if (p->ypr && !p->dynamical && b->is_free) yaw_pitch_roll(p, b, 1);   /* full.py:1283-1288 */
     return 0;
 }
@@ -2028,7 +2052,8 @@ static void yaw_pitch_roll(OPark* p, OBall* b, int snap) {             /* full.p
 
 /* SpaceBallpark's refiling (InsertBallInBoxes) where the reference calls it; a no-op without collisions. */
 static void refile(OPark* p, OBall* b) {
-    if (p->collisions) file_ball(p, b);
+    if (p->m) master_insert_ball(p, b);
+    else if (p->collisions) file_ball(p, b);
 }
 
 void or_evolve(OPark* p) {                                              /* modes.py:702-745 */
@@ -2036,7 +2061,7 @@ void or_evolve(OPark* p) {                                              /* modes
     int live[MAX_BALLS];
     for (int i = 0; i < p->n; ++i) {                                    /* (1) behaviours, ascending id */
         OBall* b = &p->balls[i];
-        live[i] = b->is_free && !b->is_moribund;
+        live[i] = b->is_free && !b->is_moribund && !(p->m && master_dead(p, b));   /* TIME-714 on a master */
         if (live[i]) {
             b->nh = 0;                                                  /* space.py:1994-1997 */
             behaviour(p, b);
@@ -2098,6 +2123,7 @@ void or_evolve(OPark* p) {                                              /* modes
             }
         }
     }
+    if (p->m) master_proximities(p);                                    /* (6) HandleProximities */
     p->current_time += 1;                                               /* (7) */
 }
 
@@ -2273,6 +2299,7 @@ void or_remove_ball(OPark* p, int64_t id) {
     if (!b) return;
     stop_ball(p, b);
     stop_all_followers(p, b);
+    if (p->m) master_removed(p, b);                                     /* its boxes, bubble and sensors */
     const int k = (int)(b - p->balls);
     for (int i = k; i + 1 < p->n; ++i) p->balls[i] = p->balls[i + 1];
     p->n--;
@@ -2343,3 +2370,5 @@ int or_on_tick(OPark* p, int64_t sim_time, int master) {
     p->time = sim_time - last;
     return ran;
 }
+
+#include "oracle_master.c"

@@ -1480,3 +1480,142 @@ def _buffer_worlds(r: random.Random, worlds: list) -> list:
         w["kind"] = "remove_buffer"
         out.append(w)
     return out
+
+
+# ---- the master's space layer: partition tops, bubbles, sensors (space.py's SpaceBallpark, is_master) ----
+
+TOP_WIDTH = 7864320.0           # a top box's width (space.py:71-72): the top faces sit at its multiples
+KEEPALIVE_OP = 30               # the Python layer's bubbleKeepAliveBalls.add(ball a)
+SENSOR_OP = 31                  # AddProximitySensor(ball a, range rr, period x, shuffle w & 1, only interactives w & 2)
+UNSENSOR_OP = 32                # RemoveProximitySensor(ball a)
+NOTIFY_OP = 33                  # SetNotificationRange(ball a, range rr)
+TRACK_OP = 34                   # SetTargetTracking(ball a, target b, range rr)
+INTERACTIVE_OP = 35             # SetBallInteractive(ball a, flag w)
+INIT_BUBBLES_OP = 36            # InitializeBubbles (a 0) or UpdateBallBubble(ball a)
+COPY_BUBBLES_OP = 37            # CopyBubbles
+
+
+def space_cases(seed: int, scale: int = 1) -> list:
+    """The master's space worlds (about 150 x scale): 2-12 balls in 1-3 clusters placed on top-box faces, edges and
+    corners and inside top boxes (clusters 0-30,000 km apart, spread 0-2,000 km), not massive (no collision
+    gradient), some fixed, half interactive (a cluster of none: a dead bubble, its balls skipped by the tick), some
+    kept alive; proximity sensors (range 1-250 km, period 1-5 s, some shuffled by the world's glibc rand(), some
+    sensing interactives only) added at the load and mid-run and removed; notification ranges on GOTO and FOLLOW
+    balls; target tracking; modes moving balls across top faces (GOTO at up to 3 km/s across a face, FOLLOW, ORBIT),
+    warps to the other clusters (dropping out into their bubbles); SetBallInteractive, RemoveBall, InitializeBubbles,
+    UpdateBallBubble and CopyBubbles between ticks; 20-50 ticks. Each world is queried after each tick (`queries`)."""
+    r = random.Random(seed ^ 0x5BACE)
+    out = []
+    for wi in range(150 * scale):
+        crowd = wi % 10 == 9                        # 16-30 balls, most sensing: the maps grow past 13 buckets
+        n = r.randrange(16, 31) if crowd else r.randrange(2, 13)
+        nclus = 1 if crowd else r.choice([1, 2, 2, 3, 3])
+        centres = []
+        for _ in range(nclus):
+            base = [r.randrange(-3, 4) * TOP_WIDTH for _ in range(3)]
+            place = r.choice(["face", "edge", "corner", "inside", "inside"])
+            off = {"face": [r.uniform(0.2, 0.8), 0.0, r.uniform(0.2, 0.8)],
+                   "edge": [0.0, 0.0, r.uniform(0.2, 0.8)], "corner": [0.0, 0.0, 0.0],
+                   "inside": [r.uniform(0.1, 0.9) for _ in range(3)]}[place]
+            centres.append([b + o * TOP_WIDTH for b, o in zip(base, off)])
+        spread = r.choice([2e3, 2e4]) if crowd else r.choice([0.0, 2e4, 2e5, 2e6])
+        balls, ids = [], r.sample(range(1, 5000), n)
+        dead_cluster = r.randrange(nclus) if r.random() < 0.3 else -1
+        for k in range(n):
+            c = k % nclus
+            pos = [x + r.gauss(0.0, spread) + r.uniform(-3e3, 3e3) for x in centres[c]]
+            inter = (r.random() < 0.5) and c != dead_cluster
+            b = _ball(r, ids[k], pos=pos, massive=False, interactive=inter, bubble=-1, cloaked=0,
+                      free=r.random() < 0.85, vel=[0.0, 0.0, 0.0],
+                      max_vel=r.choice([r.uniform(100.0, 3000.0), r.uniform(10.0, 500.0)]))
+            b["cluster"] = c
+            balls.append(b)
+        steps = r.randrange(20, 51)
+        cmds = []
+        byc = {}
+        for b in balls:
+            byc.setdefault(b["cluster"], []).append(b)
+        for b in balls:
+            if r.random() < 0.15:
+                cmds.append(_cmd(0, KEEPALIVE_OP, b["id"]))
+            if r.random() < (0.85 if crowd else 0.45):
+                s0 = 0 if r.random() < 0.6 else r.randrange(1, steps)
+                rng = r.uniform(5e4, 2.5e5) if crowd else r.choice([r.uniform(1e3, 2.5e5), r.uniform(1e3, 2e4),
+                                                                    r.uniform(5e4, 2.5e5)])
+                cmds.append(_cmd(s0, SENSOR_OP, b["id"], rr=rng, x=r.choice([1.0, 2.0, 2.0, 3.5, 5.0]),
+                                 w=(1 if r.random() < 0.4 else 0) | (2 if r.random() < 0.3 else 0)))
+                if r.random() < 0.2:
+                    cmds.append(_cmd(r.randrange(s0 + 1, steps + 1), UNSENSOR_OP, b["id"]))
+            mates = [o for o in byc[b["cluster"]] if o is not b]
+            kind = r.choice(["stop", "goto", "cross", "cross", "follow", "orbit", "warp", "warp"])
+            s1 = r.choice([1, 1, r.randrange(1, max(2, steps // 3))])
+            if kind in ("cross", "warp"):           # agile enough to cross a face, or align, within the run
+                b["mass"] = 10 ** r.uniform(6.0, 7.0)
+                b["agility"] = r.uniform(0.05, 0.5)
+                b["free"] = True
+            if kind == "cross":                     # a few km from the nearest top face on x, heading across it
+                face = round(b["pos"][0] / TOP_WIDTH) * TOP_WIDTH
+                side = r.choice([-1.0, 1.0])
+                b["pos"][0] = face - side * r.uniform(2e3, 3e4)
+                b["max_vel"] = r.uniform(1000.0, 3000.0)
+            if kind == "goto":
+                cmds.append(_cmd(s1, GOTO_OP, b["id"], x=b["pos"][0] + r.uniform(-2e5, 2e5),
+                                 y=b["pos"][1] + r.uniform(-2e5, 2e5), z=b["pos"][2] + r.uniform(-2e5, 2e5)))
+            elif kind == "cross":                   # toward the nearest top face on x: crossing it within the run
+                cmds.append(_cmd(s1, GOTO_OP, b["id"], x=face + side * 5e5, y=b["pos"][1], z=b["pos"][2]))
+            elif kind in ("follow", "orbit") and mates:
+                t = r.choice(mates)
+                cmds.append(_cmd(s1, FOLLOW_OP if kind == "follow" else ORBIT_OP, b["id"], t["id"],
+                                 rr=r.uniform(500.0, 20000.0)))
+            elif kind == "warp" and nclus > 1:
+                other = centres[(b["cluster"] + 1) % nclus]
+                cmds.append(_cmd(s1, WARP_OP, b["id"], x=other[0] + r.uniform(-1e4, 1e4),
+                                 y=other[1] + r.uniform(-1e4, 1e4), z=other[2], rr=r.choice([0.0, 20000.0]), w=20))
+            if r.random() < 0.25:
+                cmds.append(_cmd(r.randrange(0, steps), NOTIFY_OP, b["id"], rr=r.uniform(1e3, 1e5)))
+            if r.random() < 0.2 and mates:
+                t = r.choice(mates + [b])
+                cmds.append(_cmd(r.randrange(0, steps), TRACK_OP, b["id"], t["id"],
+                                 rr=r.choice([0.0, r.uniform(1e3, 3e5)])))
+            if r.random() < 0.15:
+                cmds.append(_cmd(r.randrange(1, steps), INTERACTIVE_OP, b["id"], w=int(r.random() < 0.5)))
+        if r.random() < 0.3:
+            cmds.append(_cmd(r.randrange(1, steps), REMOVE_OP, r.choice(balls)["id"]))
+        if r.random() < 0.4:
+            cmds.append(_cmd(r.randrange(1, steps), INIT_BUBBLES_OP, 0 if r.random() < 0.7 else r.choice(balls)["id"]))
+        for s in range(1, steps + 1):
+            if r.random() < 0.3:
+                cmds.append(_cmd(s, COPY_BUBBLES_OP, 0))
+        for b in balls:
+            del b["cluster"]
+        if wi % 15 == 7:                            # remote followers: a target crossing into another bubble
+            balls, cmds, steps = _remote_world(r)
+        out.append({"kind": "space", "friction": 1.0e6, "tick": r.choice([1000, 1000, 500]), "steps": steps,
+                    "new_orbit": r.random() < 0.3, "start": 0, "master": True, "seed": r.randrange(1, 2 ** 31),
+                    "balls": balls, "cmds": sorted(cmds, key=lambda c: c[0]),
+                    "queries": {"range": r.choice([5e3, 5e4, 2e5]), "point_offset": [r.uniform(-1e5, 1e5)
+                                                                                     for _ in range(3)]}})
+    return out
+
+
+def _remote_world(r: random.Random) -> tuple:
+    """Clusters two top boxes apart on x (bubbles 0 and 1); a target of the upper one starts just inside its top's
+    lower face and flies down across it into the empty top between them, whose first bubbled neighbour
+    (NearbyBubbles' i-j-k order: i = -1 first) is the lower cluster's: the target takes bubble 0 while its follower,
+    a few hundred metres behind, is still in bubble 1 (a remote follower until it crosses too)."""
+    base = [r.randrange(-3, 2) * TOP_WIDTH, r.randrange(-3, 3) * TOP_WIDTH + 0.5 * TOP_WIDTH,
+            r.randrange(-3, 3) * TOP_WIDTH + 0.5 * TOP_WIDTH]
+    lower = [base[0] + 0.5 * TOP_WIDTH, base[1], base[2]]
+    face = base[0] + 2 * TOP_WIDTH
+    balls = [_ball(r, 101 + k, pos=[lower[0] + r.uniform(-1e4, 1e4), lower[1] + r.uniform(-1e4, 1e4), lower[2]],
+                   massive=False, interactive=True, bubble=-1, cloaked=0, vel=[0.0, 0.0, 0.0]) for k in range(2)]
+    t = _ball(r, 201, pos=[face + r.uniform(3e3, 2e4), base[1], base[2]], massive=False, interactive=True, bubble=-1,
+              cloaked=0, vel=[0.0, 0.0, 0.0], mass=10 ** r.uniform(6.0, 7.0), agility=r.uniform(0.05, 0.5),
+              max_vel=r.uniform(1500.0, 3000.0), free=True)
+    f = _ball(r, 202, pos=[t["pos"][0] + r.uniform(200.0, 800.0), base[1] + 100.0, base[2]], massive=False,
+              interactive=True, bubble=-1, cloaked=0, vel=[0.0, 0.0, 0.0], mass=10 ** r.uniform(6.0, 7.0),
+              agility=r.uniform(0.05, 0.5), max_vel=r.uniform(1500.0, 3000.0), free=True)
+    balls += [t, f]
+    cmds = [_cmd(1, GOTO_OP, 201, x=face - 4e5, y=base[1], z=base[2]),
+            _cmd(1, FOLLOW_OP, 202, 201, rr=r.uniform(100.0, 600.0))]
+    return balls, cmds, r.randrange(25, 45)
