@@ -15,40 +15,10 @@
 #include "xb_commands.h"
 #include "xb_rand.h"
 #include "../../include/xb.h"
+#include "xb_park.h"
+#include "xb_space.h"
 
 namespace {
-
-XbPark host_park(const xb_modes_state& s) {
-    XbPark P;
-    P.f = s.f64;
-    P.q = s.i64;
-    P.b = s.u8;
-    P.stride = s.stride;
-    P.friction = s.friction;
-    P.dt = s.dt;
-    P.k2 = s.friction * s.friction;
-    P.ook2 = 1.0 / P.k2;
-    P.ook = 1.0 / s.friction;
-    P.new_orbit = s.new_orbit;
-    P.dynamical = s.dynamical;
-    P.cf = s.coll_f64;
-    P.cq = s.coll_i64;
-    P.hits = s.hits;
-    P.hits_cap = s.hits_cap;
-    P.statics = s.statics && s.st_f64 && s.st_range;
-    P.st_range = s.st_range;
-    P.st_f = s.st_f64;
-    P.st_q = s.st_i64;
-    P.st_keys = s.st_keys;
-    P.st_stride = s.st_stride;
-    P.tick_ms = (int64_t)(s.dt * 1000.0 + 0.5);
-    P.form = s.form;
-    P.form_len = s.form_len;
-    P.n_form = s.form ? s.n_form : 0;
-    P.form_stride = s.form_stride;
-    P.fld = s.fld;
-    return P;
-}
 
 struct Scratch {
     std::vector<double> d;
@@ -247,7 +217,7 @@ int xb_cpu_modes_tick(const xb_modes_state* state, int threads) {
     if (state->form && state->n_form > 0 &&
         (!state->form_len || state->form_stride <= 0 || !state->tiles || (!state->dynamical && !state->ypr)))
         return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     if (state->collisions) {
         if (!state->tiles || state->n_tiles <= 0 || !state->coll_f64 || !state->coll_i64 || state->hits_cap < 0 ||
             (state->hits_cap > 0 && !state->hits))
@@ -289,7 +259,7 @@ int xb_cpu_collide_file(const xb_modes_state* state, const int32_t* rows, int64_
     if (!state || !state->f64 || !state->i64 || !state->u8 || !state->coll_f64 || !state->coll_i64 || state->n < 0 ||
         state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
         return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r < 0 || r >= state->n) continue;
@@ -385,7 +355,7 @@ int xb_cpu_modes_ypr_snap(const xb_modes_state* state, const int32_t* rows, int6
     if (!state || !state->f64 || !state->i64 || !state->u8 || !state->ypr || state->n < 0 ||
         state->stride < state->n || n_rows < 0 || (!rows && n_rows > state->n))
         return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r < 0 || r >= state->n) continue;
@@ -407,7 +377,7 @@ static bool command_args_ok(const xb_modes_state* state, const int32_t* rows, in
 int xb_cpu_modes_command_check(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
                                int32_t* status) {
     if (!command_args_ok(state, rows, n_rows, cmds) || !status) return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r >= 0 && r < state->n) *status |= xb_cmd_gather(P, state->n, r, cmds[j], state->ypr);
@@ -418,7 +388,7 @@ int xb_cpu_modes_command_check(const xb_modes_state* state, const int32_t* rows,
 int xb_cpu_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, xb_command* cmds,
                                uint8_t* mark) {
     if (!command_args_ok(state, rows, n_rows, cmds)) return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     const int64_t n = state->n;
     for (int64_t j = 0; j < n_rows; ++j) {                  /* gather: every read of another row first */
         const int64_t r = rows ? (int64_t)rows[j] : j;
@@ -437,13 +407,49 @@ int xb_cpu_modes_command_apply(const xb_modes_state* state, const int32_t* rows,
     return 0;
 }
 
+static bool space_args_ok(const xb_modes_state* park, const xb_space_state* space) {
+    return park && park->f64 && park->i64 && park->u8 && space && space->W >= 0 && space->N >= 1 && space->N <= 64 &&
+           space->rows && space->by_id && space->ids;
+}
+
+int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, int64_t K, const int64_t* code,
+                      const int64_t* a, const int64_t* b, const double* f) {
+    if (!space_args_ok(park, space) || K < 0 || (K > 0 && (!code || !a || !b || !f))) return -1;
+    const XbPark P = xb_park(*park);
+    const int64_t W = space->W;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t w = 0; w < W; ++w) {
+        XsCtx c{space, P, w, W, space->N};
+        for (int64_t k = 0; k < K; ++k) {
+            const double fk[4] = {f[(0 * K + k) * W + w], f[(1 * K + k) * W + w], f[(2 * K + k) * W + w],
+                                  f[(3 * K + k) * W + w]};
+            xs_apply(c, code[k * W + w], a[k * W + w], b[k * W + w], fk);
+        }
+    }
+    return 0;
+}
+
+int xb_cpu_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
+                       const int64_t* b, const double* f, int64_t* out, double* outf) {
+    if (!space_args_ok(park, space) || !a || !b || !f || !out) return -1;
+    const XbPark P = xb_park(*park);
+    const int64_t W = space->W;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t w = 0; w < W; ++w) {
+        XsCtx c{space, P, w, W, space->N};
+        const double fw[5] = {f[0 * W + w], f[1 * W + w], f[2 * W + w], f[3 * W + w], f[4 * W + w]};
+        xs_query(c, kind, a[w], b[w], fw, out, outf);
+    }
+    return 0;
+}
+
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
                           const double* point, const int64_t* target, const double* range, const double* min_range,
                           const int64_t* warp_factor) {
     if (!state || !state->f64 || !state->i64 || !state->u8 || state->n < 0 || state->stride < state->n || n_rows < 0 ||
         !op || !point || !target || !range || !min_range || !warp_factor || (!rows && n_rows > state->n))
         return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r >= 0 && r < state->n) xb_cmd_apply(P, state->n, r, op, point, target, range, min_range, warp_factor);
@@ -455,7 +461,7 @@ int xb_cpu_modes_orient_setup(const xb_modes_state* state, const int32_t* rows, 
     if (!state || !state->f64 || state->n < 0 || state->stride < state->n || n_rows < 0 || what < 0 || what > 1 ||
         (!rows && n_rows > state->n))
         return -1;
-    const XbPark P = host_park(*state);
+    const XbPark P = xb_park(*state);
     for (int64_t j = 0; j < n_rows; ++j) {
         const int64_t r = rows ? (int64_t)rows[j] : j;
         if (r < 0 || r >= state->n) continue;

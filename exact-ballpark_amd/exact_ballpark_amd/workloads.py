@@ -489,3 +489,136 @@ def padded(wl: dict, width: int) -> dict:
     return {"n": n, "worlds": W, "ships": width, "counts": counts, "start": np.arange(W) * width, "rows": rows,
             "cmds": cmds, "real": real, "dest": dest, "ypr": ypr, "dynamical": wl.get("dynamical", False),
             "collisions": wl.get("collisions", False), "launch": launch}
+
+
+def partition_park(seed: int, worlds: int, min_ships: int = 4, max_ships: int = 64) -> dict:
+    """The partition park (the master's space layer over the flight modes): `worlds` worlds of U{min_ships..max_ships}
+    ships, each world's ships in U{1..16} Gaussian clusters (centres uniform in a +/-1e9 m cube, offsets N(0, 5e4^2)
+    per axis), so that clusters in different top boxes form different bubbles. Per ship as the flight park's (mass
+    U(1e6, 1e8) kg, agility U(0.1, 5), maximum speed U(100, 3000) m/s, radius U(20, 500) m), not massive (no collision
+    contact: the layer runs over a client park without the gradient), interactive with probability 1/2. A proximity
+    sensor on half the ships (range U(1e3, 2.5e5) m, period U(1, 5) s, shuffled by the world's rand() half the time,
+    sensing interactive balls only one time in five). At tick 1 modes uniform over stop, goto (a point within 200 km of
+    the ship), follow and orbit (another ship of its cluster, range U(500, 20000) m; refused across bubbles) and warp (to
+    another cluster's centre, the thunker's defaults). Rows world-major; each world's slots in row order."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(min_ships, max_ships + 1, worlds)
+    n = int(counts.sum())
+    start = np.concatenate(([0], np.cumsum(counts)[:-1])).astype(np.int64)
+    world = np.repeat(np.arange(worlds, dtype=np.int64), counts)
+    local = np.arange(n) - start[world]
+    nclus = rng.integers(1, 17, worlds)
+    centres = rng.uniform(-1e9, 1e9, (worlds, 16, 3))
+    clus = (rng.uniform(0.0, 1.0, n) * nclus[world]).astype(np.int64)
+    pos = centres[world, clus] + rng.normal(0.0, 5e4, (n, 3))
+    rows = {
+        "id": (local + 1).astype(np.int64), "world": world, "time": np.zeros(n, dtype=np.int64),
+        "mass": rng.uniform(1e6, 1e8, n), "radius": rng.uniform(20.0, 500.0, n),
+        "max_vel": rng.uniform(100.0, 3000.0, n), "agility": rng.uniform(0.1, 5.0, n),
+        "speed_fraction": np.ones(n), "pos": pos, "vel": np.zeros((n, 3)),
+        "free": np.ones(n, dtype=bool), "massive": np.zeros(n, dtype=bool),
+        "interactive": rng.uniform(0.0, 1.0, n) < 0.5, "moribund": np.zeros(n, dtype=bool),
+        "cloaked": np.zeros(n, dtype=np.int64), "bubble": np.full(n, -1, dtype=np.int64),
+        "mode": np.full(n, 2, dtype=np.int64),
+    }
+    kind = rng.integers(0, 5, n)                        # 0 stop, 1 goto, 2 follow, 3 orbit, 4 warp
+    # another ship of the same cluster, else none
+    order = np.lexsort((local, clus, world))            # rows grouped by (world, cluster)
+    key = world[order] * 16 + clus[order]
+    first = np.concatenate(([0], np.flatnonzero(key[1:] != key[:-1]) + 1))
+    size = np.diff(np.concatenate((first, [n])))
+    grp = np.repeat(np.arange(first.size), size)
+    pick = rng.integers(0, np.maximum(size[grp] - 1, 1))
+    pos_in = np.arange(n) - first[grp]
+    pick = np.where(pick >= pos_in, pick + 1, pick)
+    mate = np.full(n, -1, dtype=np.int64)
+    ok = size[grp] > 1
+    mate[order[ok]] = order[first[grp[ok]] + pick[ok]]
+    other_clus = (clus + 1 + rng.integers(0, 15, n)) % np.maximum(nclus[world], 1)
+    warp_to = centres[world, other_clus]
+    point = np.where((kind == 4)[:, None], warp_to, pos + rng.uniform(-2e5, 2e5, (n, 3)))
+    op = np.select([kind == 1, (kind == 2) & (mate >= 0), (kind == 3) & (mate >= 0), (kind == 4) & (nclus[world] > 1)],
+                   [GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP], 0)
+    cmds = {"op": op.astype(np.int64), "point": point, "target": np.maximum(mate, 0), "range":
+            rng.uniform(500.0, 20000.0, n), "min_range": np.full(n, 20000.0),
+            "warp_factor": np.full(n, 20, dtype=np.int64)}
+    sensor = rng.uniform(0.0, 1.0, n) < 0.5
+    sensors = {"row": np.flatnonzero(sensor), "range": rng.uniform(1e3, 2.5e5, n)[sensor],
+               "period": rng.uniform(1.0, 5.0, n)[sensor], "shuffle": (rng.uniform(0.0, 1.0, n) < 0.5)[sensor],
+               "only": (rng.uniform(0.0, 1.0, n) < 0.2)[sensor]}
+    return {"n": n, "worlds": worlds, "ships": None, "counts": counts, "start": start, "rows": rows, "cmds": cmds,
+            "sensors": sensors, "seeds": rng.integers(1, 2 ** 31, worlds), "partition": True}
+
+
+def space_rows(wl: dict) -> torch.Tensor:
+    """[W, N] the park row of each world's slots (N the widest world), -1 past a world's ships."""
+    W = wl["worlds"]
+    width = int(wl["counts"].max())
+    rows = np.full((W, width), -1, dtype=np.int64)
+    k = np.arange(width)[None, :]
+    rows = np.where(k < wl["counts"][:, None], wl["start"][:, None] + k, -1)
+    return torch.as_tensor(rows, dtype=torch.int64)
+
+
+def load_partition_park(park, wl: dict, space_cls):
+    """Loads a partition park: the rows, the master's layer over them (each world seeded with its own srand seed) and
+    its AddBall filings, the sensors (one tape op a world at a time, in row order), then the tick-1 commands. Returns
+    the layer."""
+    r = wl["rows"]
+    dev = park.dev
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt).to(dev)
+    park.load_arrays(ids=t(r["id"], torch.int64), world=t(r["world"], torch.int64), time=t(r["time"], torch.int64),
+                     mass=t(r["mass"]), radius=t(r["radius"]), max_vel=t(r["max_vel"]), pos=t(r["pos"]),
+                     vel=t(r["vel"]), agility=t(r["agility"]), speed_fraction=t(r["speed_fraction"]),
+                     free=t(r["free"], torch.bool), massive=t(r["massive"], torch.bool),
+                     interactive=t(r["interactive"], torch.bool), cloaked=t(r["cloaked"], torch.int64),
+                     bubble=t(r["bubble"], torch.int64), moribund=t(r["moribund"], torch.bool),
+                     mode=t(r["mode"], torch.int64))
+    space = space_cls(park, space_rows(wl), seed=torch.as_tensor(wl["seeds"], dtype=torch.int64))
+    space.load()
+    s = wl["sensors"]
+    W = wl["worlds"]
+    world = r["world"][s["row"]]
+    slot = s["row"] - wl["start"][world]
+    per = np.bincount(world, minlength=W)
+    K = int(per.max()) if len(world) else 0
+    if K:
+        code = np.zeros((W, K), dtype=np.int64)
+        a = np.full((W, K), -1, dtype=np.int64)
+        b = np.zeros((W, K), dtype=np.int64)
+        f = np.zeros((W, K, 4))
+        col = np.arange(len(world)) - np.concatenate(([0], np.cumsum(per)[:-1]))[world]
+        code[world, col] = 7                              # ADD_SENSOR
+        a[world, col] = slot
+        b[world, col] = s["only"].astype(np.int64)
+        f[world, col, 0], f[world, col, 1], f[world, col, 2] = s["range"], s["period"], s["shuffle"]
+        space.tape(torch.as_tensor(code), torch.as_tensor(a), torch.as_tensor(b), torch.as_tensor(f))
+    apply_commands(park, wl["cmds"])
+    return space
+
+
+def partition_scenario(wl: dict, wi: int, steps: int) -> dict:
+    """World wi of a partition park as a space scenario (tests/scenarios.py's space worlds' form), for the oracle."""
+    r, c, s = wl["rows"], wl["cmds"], wl["sensors"]
+    rows = world_rows(wl, wi)
+    balls = [{"id": int(r["id"][k]), "mass": float(r["mass"][k]), "radius": float(r["radius"][k]),
+              "max_vel": float(r["max_vel"][k]), "agility": float(r["agility"][k]),
+              "sf": float(r["speed_fraction"][k]), "pos": [float(v) for v in r["pos"][k]], "vel": [0.0, 0.0, 0.0],
+              "free": True, "massive": False, "interactive": bool(r["interactive"][k]), "cloaked": 0, "bubble": -1,
+              "moribund": False, "mode": 2} for k in rows]
+    cmds = []
+    for i in np.flatnonzero((s["row"] >= rows.start) & (s["row"] < rows.stop)):
+        cmds.append((0, 31, int(r["id"][s["row"][i]]), 0, float(s["period"][i]), 0.0, 0.0, float(s["range"][i]),
+                     int(s["shuffle"][i]) | (2 if s["only"][i] else 0)))
+    for o in (GOTO_OP, FOLLOW_OP, ORBIT_OP, WARP_OP):        # apply_commands' order
+        for k in rows:
+            if int(c["op"][k]) != o:
+                continue
+            p = [float(v) for v in c["point"][k]]
+            tgt = int(r["id"][int(c["target"][k])])
+            cmds.append((0, o, int(r["id"][k]), tgt, p[0], p[1], p[2],
+                         float(c["min_range"][k]) if o == WARP_OP else float(c["range"][k]),
+                         int(c["warp_factor"][k])))
+    return {"kind": "partition", "friction": 1.0e6, "tick": 1000, "steps": steps, "new_orbit": False, "start": 0,
+            "master": True, "seed": int(wl["seeds"][wi]), "balls": balls, "cmds": cmds,
+            "queries": {"range": 5e4, "point_offset": [0.0, 0.0, 0.0]}}

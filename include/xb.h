@@ -205,6 +205,83 @@ int xb_modes_command_apply(const xb_modes_state* state, const int32_t* rows, int
    maximum threads a workgroup. Replaces xbpm_info (src/exact_ballpark/vec_modes_cuda.cu:766). */
 int xb_modes_info(int which, int64_t* out);
 
+/* ---- the master's space layer (the partition's tops and bubbles, the sensors, the queries) ---- */
+
+/* W worlds of N slots (N <= 64), each world's tables world-interleaved: entry k of world w's table at [k * W + w]
+   (k counting the table's rows; a per-slot table [q][N][W] has entry (q, j) at [(q * N + j) * W + w]). Replaces the
+   reference's `struct Space` of [W, ...] tensors (src/exact_ballpark/vec_space.cu:171-237). Room per world: T tops,
+   E bubble dictionary entries, D dictionaries, I pairs of each set, M events; a world past a room is flagged in
+   wflags (and left outside the layer's domain). The caller zeroes every table but the following, set at creation:
+   rows and by_id (as given), level -1, bubble and old_bubble -1, trk -1, s_period 2, s_bkt and p_bkt 1, rnd each
+   world's glibc srand state (xb_rand_seed's). */
+typedef struct xb_space_state {
+    int64_t W, N, T, E, D, I, M;
+    const int32_t* rows;      /* [N][W] slot j's park row (-1: none), the slots in the order the world adds them */
+    const int32_t* by_id;     /* [N][W] the slots by ascending id, -1 after the last */
+    int64_t* ids;             /* [N][W] each slot's ball id, from its ADD */
+    int8_t* level;            /* [N][W] the filing's level, -1 for none */
+    int64_t* key;             /* [3][N][W] */
+    int8_t* mod;              /* [3][N][W] */
+    int64_t* bubble;          /* [N][W] */
+    int64_t* old_bubble;      /* [N][W] */
+    int64_t* seq;             /* [N][W] the ball dictionary's place */
+    uint8_t* sflags;          /* [N][W] in the park, dead, in warp proper, proximity, interactives only, in range,
+                                 in tracking range, kept alive (xb_space.h's XS_* bits) */
+    float* s_range;           /* [N][W] the proximity sensor's range (a float) */
+    double* s_period;         /* [N][W] */
+    double* s_elapsed;        /* [N][W] */
+    int32_t* trk;             /* [N][W] the tracked slot, -1: none */
+    double* trk_range;        /* [N][W] */
+    double* ntf_range;        /* [N][W] */
+    double* trk_pos;          /* [3][N][W] a removed slot's last position (the tracking pointer outlives it) */
+    int8_t* s_keys;           /* [N][N][W] slot j's sensor list, key i at [(i N + j) W + w], unordered_map order */
+    int32_t* s_len;           /* [N][W] */
+    int32_t* s_bkt;           /* [N][W] its bucket count */
+    int32_t* s_next;          /* [N][W] its rehash policy's next resize */
+    int32_t* tn;              /* [W] the tops that exist */
+    int32_t* top_key;         /* [3][T][W] */
+    int64_t* top_bub;         /* [T][W] */
+    int32_t* top_ref;         /* [T][W] the filings' boxes under it (a top with none is released) */
+    int32_t* en;              /* [W] the bubble dictionaries' entries, in insertion order */
+    int64_t* e_bub;           /* [E][W] */
+    int8_t* e_slot;           /* [E][W] */
+    int8_t* e_act;            /* [E][W] +1 added, 0 kept, -1 removed */
+    int32_t* dn;              /* [W] the dictionaries, in insertion order */
+    int64_t* d_bub;           /* [D][W] */
+    int32_t* in_;             /* [W] bubbleInteractives as (bubble, slot) pairs */
+    int64_t* i_bub;           /* [I][W] */
+    int8_t* i_slot;           /* [I][W] */
+    int32_t* kn;              /* [W] bubbleKeepAlives as pairs */
+    int64_t* k_bub;           /* [I][W] */
+    int8_t* k_slot;           /* [I][W] */
+    int8_t* p_keys;           /* [N][W] the proximity balls, unordered_map order */
+    int32_t* p_len;           /* [W] */
+    int32_t* p_bkt;           /* [W] */
+    int32_t* p_next;          /* [W] */
+    int64_t* bubble_id;       /* [W] mBubbleId */
+    int64_t* seq_next;        /* [W] */
+    int32_t* evn;             /* [W] events since the last CLEAR_EVENTS */
+    int64_t* ev;              /* [4][M][W] kind (1 DoProximity, 2 DoRange, 3 DoTargetTracking, 4 DoFinalCleanup),
+                                 ball id, a, b */
+    int32_t* rnd;             /* [33][W] glibc's TYPE_3 table, then its front and rear indices */
+    int64_t* wflags;          /* [W] rooms passed, refusals (XS_F_*) */
+} xb_space_state;
+
+/* The layout as built: out[0] = sizeof(xb_space_state), out[1] = the op count, out[2] = the largest N. */
+int xb_space_layout(int64_t* out);
+
+/* Per world, K operations in order (op k of world w: code, a (a slot) and b (an int) at [k * W + w], its floats f at
+   [(q * K + k) * W + w], q < 4), one lane a world, over the park `park` (the tick state the slots' rows index: the
+   layer reads and writes its rows' bubble, interactive, sensor and, around the park's tick, moribund fields). The ops
+   are xb_space.h's XS_OP_*. Replaces xbps_tape_cuda (src/exact_ballpark/vec_space.cu:1871). */
+int xb_space_tape(const xb_modes_state* park, const xb_space_state* space, int64_t K, const int64_t* code,
+                  const int64_t* a, const int64_t* b, const double* f, void* stream);
+
+/* One query a world (kind XS_Q_*, a and b at [w], f [5][W]): out [1 + N][W] (word k of world w at [k * W + w]), outf
+   [N][W] (the range queries' squared distances; may be NULL). Replaces xbps_query_cuda (vec_space.cu:1895). */
+int xb_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
+                   const int64_t* b, const double* f, int64_t* out, double* outf, void* stream);
+
 /* ---- the learned scheduler (no counterpart in the reference: it orders work, never changes a result) ---- */
 
 #define XB_MLP_IN 12
@@ -310,6 +387,12 @@ int xb_cpu_modes_command_apply(const xb_modes_state* state, const int32_t* rows,
 int xb_cpu_modes_commands(const xb_modes_state* state, const int32_t* rows, int64_t n_rows, const uint8_t* op,
                           const double* point, const int64_t* target, const double* range, const double* min_range,
                           const int64_t* warp_factor);
+
+/* xb_space_tape and xb_space_query on a park and layer in host memory (no stream). */
+int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, int64_t K, const int64_t* code,
+                      const int64_t* a, const int64_t* b, const double* f);
+int xb_cpu_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
+                       const int64_t* b, const double* f, int64_t* out, double* outf);
 
 /* xb_rand_seed and xb_rand_draw on the host (no stream). */
 int xb_cpu_rand_seed(const uint32_t* seeds, int64_t n_worlds, int32_t* states);
