@@ -12,6 +12,9 @@ Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
 - flight: 65,536 worlds x 4 ships, modes uniform.
 - turning: 65,536 worlds x 4 ships in 2 km clusters with dynamical orientation.
 - collide: 65,536 worlds x (4 ships + 4 fixed spheres) in 2 km clusters, the gradient collision method.
+- partition: the master's space layer over the flight park: 65,536 worlds of 4..64 ships in Gaussian clusters,
+  proximity sensors on half the ships, modes uniform; each tick the dead bubbles, the park's tick, the re-filings and
+  HandleProximities as one graph replay (measured beside the others, with its own CPU baseline and exactness check).
 - statics: the turning and collisions park with static collidables: 65,536 worlds x (4 ships + 4 fixed spheres) and 8
   static collidables a world (capsules and oriented boxes), dynamical orientation, the gradient method.
 Each ragged workload (composed, ragged, profiles) is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
@@ -70,6 +73,10 @@ WORKLOADS = {
     "collide": "collision park: {worlds} worlds x (4 free massive ships in a 2 km cluster + 4 fixed massive spheres "
                "of radius 100-5000 m), every ship going to a point of its cluster; the gradient collision method "
                "(partition filing, walk-order contacts, Potential); no dynamical orientation or static collidables",
+    "partition": "partition park (the master's space layer over the flight park): {worlds} worlds of 4-64 ships in "
+                 "1-16 Gaussian clusters (sigma 50 km) a world, centres in a 2e9 m cube, half the ships interactive, "
+                 "a proximity sensor on half (range 1-250 km, period 1-5 s, half shuffled), modes uniform over "
+                 "stop/goto/follow/orbit/warp (follow and orbit within the cluster, warp to another cluster)",
     "statics": "turning and collisions park with static collidables: {worlds} worlds x (4 free massive ships in a 2 km "
                "cluster + 4 fixed massive spheres of radius 100-5000 m) and 8 static collidables a world (capsules of "
                "length 200-3000 m and radius 30-400 m, oriented boxes of half-extents 30-800 m, in turn, randomly "
@@ -180,6 +187,8 @@ def main() -> int:
     ap.add_argument("--arrival-rate", type=float, default=5000.0, help="episodes arriving per serving step (1 s)")
     ap.add_argument("--serve-check", type=int, default=48, help="finished episodes checked against the oracle")
     ap.add_argument("--no-single", action="store_true", help="skip the single-tick measurement (for profiling)")
+    ap.add_argument("--no-partition", dest="partition", action="store_false",
+                    help="leave out the partition park's line (the master's space layer)")
     ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -358,6 +367,8 @@ def main() -> int:
             out[key] = p[key]
     out.update(out_extra)
     srv = None
+    if not a.only and a.partition:
+        out["partition_park"] = partition(a)
     if a.episodes:
         out["serving"], srv = serving(a)
     if not a.no_cpu:
@@ -412,6 +423,91 @@ def _headline(out: dict, p: dict, a) -> dict:
         head["learned_gain"] = out["learned_scheduler"]["gain_over_rows"]
     head["detail"] = a.detail
     return head
+
+
+def partition(a) -> dict:
+    """The partition park's line: the master's space layer over the flight park (WORKLOADS["partition"]), each tick the
+    dead bubbles' marking, the park's tick, the re-filings and HandleProximities as one HIP graph replay (SpaceGraph),
+    then the tick's sensor events drained (a fill of the per-world event counts). The CPU baseline is the oracle's
+    master world on one core over a sample of the same worlds for the same ticks; their final states (every ball's
+    flight state, filing and display orientation, and the master's structures) are compared bit for bit."""
+    import numpy as np
+    import torch
+
+    import golden_fixture as G
+    import oracle as O
+    import park_driver as D
+    import space_driver as SD
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    from exact_ballpark_amd.vec_space import SpaceGraph, VecSpace
+
+    wl = WL.partition_park(1, a.worlds)
+    park = VecModesPark(wl["n"], device="cuda", block=256, display_ypr=True, master=True)
+    room = 64 * 65                          # a tick's events a world at most: each sensor entering or leaving each ball
+    space = WL.load_partition_park(park, wl, lambda *x, **k: VecSpace(*x, events=room, **k))
+    g = SpaceGraph(space)
+    done = 0
+
+    def ticks(k):
+        nonlocal done
+        for _ in range(k):
+            g.tick()
+            space.clear_events()
+        done += k
+
+    ticks(max(4, a.warmup // 4))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    K = max(8, a.steps // 4)
+    start.record(stream)
+    ticks(K)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / K
+    lat = []
+    for _ in range(50):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ticks(1)
+        e1.record(stream)
+        lat.append((e0, e1))
+    torch.cuda.synchronize()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in lat])
+    space.check()
+    res = {"workload": WORKLOADS["partition"].format(worlds=a.worlds), "worlds": a.worlds, "ships": wl["n"],
+           "world_ticks_per_s": a.worlds / (ms / 1000.0), "ship_ticks_per_s": wl["n"] / (ms / 1000.0),
+           "ms_per_tick": ms, "ticks_timed": K,
+           "tick_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}
+    print(f"# partition: {res}", file=sys.stderr, flush=True)
+    if a.no_cpu:
+        return res
+    W = a.worlds
+    probe = [WL.partition_scenario(wl, wi, done) for wi in range(min(32, W))]
+    dt, _ = O.time_space_worlds(probe, done)
+    m = int(max(32, min(W, 4096, a.cpu_seconds / max(dt / len(probe), 1e-9))))
+    idx = np.linspace(0, W - 1, m).astype(int).tolist()
+    print(f"# partition cpu baseline: the oracle's master on {m} worlds x {done} ticks", file=sys.stderr, flush=True)
+    dt, objs = O.time_space_worlds([WL.partition_scenario(wl, wi, done) for wi in idx], done)
+    bad = 0
+    keys = ("ids", "bubble", "old_bubble", "tops", "dicts", "inter", "keep", "keep_balls", "prox", "prox_bkt",
+            "sensors", "bubble_id")
+    for o, wi in zip(objs, idx):
+        d, q, _ = o.state()
+        lc, filing, _ = o.collide()
+        od = np.concatenate([d[:, :26], lc, o.ypr()], axis=1)
+        oq = np.concatenate([q[:, G.Q_FROM_ORACLE], filing], axis=1)
+        gd, gq = SD._flight(park, space, wi, list(WL.world_rows(wl, wi)))
+        want = o.master_state({"range": 5e4, "point_offset": [0.0, 0.0, 0.0]})
+        got = SD.structures(space, wi)
+        same = od.shape == gd.shape and D._same_bits(od, gd).all() and (oq == gq).all()
+        bad += not (same and all(got[k] == want[k] for k in keys))
+    res["cpu_baseline"] = {"value": m * done / dt, "unit": "world-ticks/s", "cores": 1, "kind": "port",
+                           "sample": f"{m} of the partition park's worlds (evenly spaced) x {done} ticks, the "
+                                     "oracle's master world in its C loop, 1 thread"}
+    res["exact"] = {"worlds_checked": m, "ticks": done, "worlds_differing": bad, "bit_exact": bad == 0}
+    return res
 
 
 def serving(a) -> tuple:

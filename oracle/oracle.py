@@ -422,6 +422,27 @@ def maths(func: str, x: np.ndarray) -> np.ndarray:
     return out
 
 
+def time_space_worlds(worlds: list, ticks: int) -> tuple:
+    """As time_worlds for space worlds (the master's view: the partition, bubbles and sensors, each world srand'ed
+    with its seed): every world's balls added and its step-0 commands given, then `ticks` ticks of all of them in one
+    C loop on one core; returns (seconds, the worlds' oracle objects)."""
+    import time
+    objs = []
+    for w in worlds:
+        o = World(w["friction"], w["tick"], w["new_orbit"], w["start"], False, True, True)
+        o.set_master(w["seed"])
+        for b in w["balls"]:
+            o.add(b)
+        for c in w["cmds"]:
+            if c[0] == 0:
+                o.command(c)
+        objs.append(o)
+    arr = (ctypes.c_void_p * len(objs))(*[o.p for o in objs])
+    t0 = time.perf_counter()
+    lib().or_evolve_all(arr, len(objs), int(ticks))
+    return time.perf_counter() - t0, objs
+
+
 def time_worlds(worlds: list, ticks: int) -> tuple:
     """Builds every world (balls, tick-1 commands applied), then times `ticks` ticks of all of them in one C loop on
     one core; returns (seconds, the worlds' oracle objects for checking)."""

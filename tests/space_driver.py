@@ -111,6 +111,30 @@ def _flight(park, space, wi: int, rows: list) -> tuple:
     return d, np.concatenate([qq, level[:, None], key, mod], axis=1)
 
 
+def structures(space, wi: int, live=None, ids_all=None, bub=None, old=None) -> dict:
+    """World wi's master structures (the record's keys but the events and the queries); `live` its live slots by
+    ascending id, the other arguments the layer's tables on the host (read here when not given)."""
+    if ids_all is None:
+        ids_all = space.st["ids"][:, wi:wi + 1].cpu().numpy()
+        bub = space.st["bubble"][:, wi:wi + 1].cpu().numpy()
+        old = space.st["old_bubble"][:, wi:wi + 1].cpu().numpy()
+        fl = space.st["sflags"][:, wi].cpu().numpy()
+        live = sorted((j for j in range(space.N) if fl[j] & 1), key=lambda j: ids_all[j, 0])
+        col = 0
+    else:
+        col = wi
+    return {
+        "ids": [int(ids_all[j, col]) for j in live], "bubble": [int(bub[j, col]) for j in live],
+        "old_bubble": [int(old[j, col]) for j in live],
+        "tops": sorted(list(k) + [v] for k, v in space.tops(wi).items()), "dicts": space.dicts(wi),
+        "inter": sorted([k, sorted(v)] for k, v in space.interactives(wi).items()),
+        "keep": sorted([k, sorted(v)] for k, v in space.keep_alives(wi).items()),
+        "keep_balls": space.keep_alive_balls(wi), "prox": space.proximity_order(wi),
+        "prox_bkt": list(space.proximity_buckets(wi)),
+        "sensors": [[int(ids_all[j, col]), space.sensor_list(wi, j), *space.sensor_buckets(wi, j)] for j in live],
+        "bubble_id": int(space.st["bubble_id"][wi])}
+
+
 def master_states(park, space, worlds: list, slot_of=None) -> list:
     """Each world's master record, as make_golden_space.py's master_state writes the reference's."""
     W, Nn = space.W, space.N
@@ -160,17 +184,8 @@ def master_states(park, space, worlds: list, slot_of=None) -> list:
     pout, rout = pout.cpu(), rout.cpu()
     out = []
     for wi in range(W):
-        ids = [int(ids_all[j, wi]) for j in live[wi]]
-        prox_bkt = list(space.proximity_buckets(wi))
-        out.append({
-            "ids": ids, "bubble": [int(bub[j, wi]) for j in live[wi]], "old_bubble": [int(old[j, wi]) for j in live[wi]],
-            "tops": sorted(list(k) + [v] for k, v in space.tops(wi).items()), "dicts": space.dicts(wi),
-            "inter": sorted([k, sorted(v)] for k, v in space.interactives(wi).items()),
-            "keep": sorted([k, sorted(v)] for k, v in space.keep_alives(wi).items()),
-            "keep_balls": space.keep_alive_balls(wi), "prox": space.proximity_order(wi), "prox_bkt": prox_bkt,
-            "sensors": [[int(ids_all[j, wi]), space.sensor_list(wi, j), *space.sensor_buckets(wi, j)]
-                        for j in live[wi]],
-            "bubble_id": int(space.st["bubble_id"][wi]),
+        out.append(structures(space, wi, live[wi], ids_all, bub, old))
+        out[-1].update({
             "events": [list(e) for e in space.events(wi)],
             "vis": vis[wi], "range": rng[wi], "range_dist": rngd[wi],
             "point": [float(pt[wi, q]).hex() for q in range(3)] if live[wi] else [0.0.hex()] * 3,
