@@ -189,6 +189,8 @@ def main() -> int:
     ap.add_argument("--no-single", action="store_true", help="skip the single-tick measurement (for profiling)")
     ap.add_argument("--no-partition", dest="partition", action="store_false",
                     help="leave out the partition park's line (the master's space layer)")
+    ap.add_argument("--partition-only", action="store_true",
+                    help="measure the partition park's line alone (its JSON the last line; for profiling)")
     ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -209,6 +211,9 @@ def main() -> int:
     from exact_ballpark_amd.vec_modes import VecModesPark
 
     torch.cuda.set_device(0)
+    if a.partition_only:
+        print(json.dumps(partition(a)), flush=True)
+        return 0
 
     def build(kind: str):
         if kind == "flight":
