@@ -3,7 +3,9 @@
 SetBallFormation and FormationFollow; with `--cloak`, cloak_golden.npz: `cloak_cases`, CloakBall and UncloakBall;
 with `--mushrooms`, mushroom_golden.npz: `mushroom_cases`, AddMushroom and SetBallFree, ball 0 recorded once added;
 with `--minis`, mini_golden.npz: `mini_cases`, a fixed parent's miniballs, minicapsules and miniboxes, and force fields
-by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids; with `--remove`,
+by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids; with `--mini-own`,
+mini_own_golden.npz: `mini_own_cases`, defender missiles through the miniballs of a parent whose id is their owner's
+(Gradient's skip) or another's; with `--remove`,
 remove_golden.npz: `remove_cases`, RemoveBall between ticks, a removed ball out of the records from then on): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
@@ -124,6 +126,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
         ws = S.remove_cases(seed) + S.delayed_remove_cases(seed)
     elif what == "mini":
         ws = S.do_mini_cases(seed) if do else S.mini_cases(seed)
+    elif what == "mini_own":
+        ws = S.mini_own_cases(seed)
     else:
         ws = S.do_missile_cases(seed) if do else S.missile_cases(seed)
     if not do:
@@ -171,5 +175,6 @@ if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     main(int(args[0]) if args else 1, do="--do" in sys.argv,
          what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv
-         else "mushroom" if "--mushrooms" in sys.argv else "mini" if "--minis" in sys.argv
+         else "mushroom" if "--mushrooms" in sys.argv else "mini_own" if "--mini-own" in sys.argv
+         else "mini" if "--minis" in sys.argv
          else "remove" if "--remove" in sys.argv else "missile")

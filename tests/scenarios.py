@@ -1355,6 +1355,47 @@ def mini_cases(seed: int, scale: int = 1) -> list:
     return out
 
 
+def mini_own_cases(seed: int, scale: int = 1) -> list:
+    """Defender missiles through a fixed parent's miniballs (about 40 x scale worlds). Gradient skips, for a missile,
+    its owner and every miniball whose owner is its owner (full.py:2633-2634). A normal missile never meets a
+    miniball (its one candidate is its target, and a negative target is refused), so only a defender, whose owner is
+    negative, can: a parent with the negative id -L is its owner when ship L launches it as a defender (owner -L).
+    Each world: the non-massive launcher L, a fixed parent with id -L (the skip: the defender passes through the parent
+    and its miniballs) or another negative id (no skip: the defender meets them), its 2-4 miniballs strung along the
+    line from the launcher to the target (each beside the line by at most 20 m), the target ship at rest beyond the
+    parent, and the massive defender launched aimed at it at tick 1 or 2."""
+    r = random.Random(seed ^ 0x0E1)
+    out: list = []
+    for i in range(40 * scale):
+        c = [r.uniform(-1e6, 1e6) for _ in range(3)]
+        u = [r.gauss(0.0, 1.0) for _ in range(3)]
+        norm = math.sqrt(sum(x * x for x in u))
+        u = [x / norm for x in u]
+        dist = r.uniform(1500.0, 4000.0)
+        ids = _ids(r, 3, big=False)
+        launcher_id, target_id, missile_id = ids
+        own = i % 2 == 0
+        parent_id = -launcher_id if own else -(launcher_id + r.randrange(1, 50))
+        launcher = _coll_ball(r, launcher_id, c, 0.0, massive=False, vel=[0.0, 0.0, 0.0])
+        parent = _sphere(r, parent_id, [c[q] + u[q] * dist for q in range(3)], 0.0)
+        parent["radius"] = r.uniform(20.0, 120.0)
+        target = _coll_ball(r, target_id, [c[q] + u[q] * 2.0 * dist for q in range(3)], 0.0, vel=[0.0, 0.0, 0.0])
+        missile = _coll_ball(r, missile_id, c, 0.0, radius=r.uniform(1.0, 20.0), mass=10 ** r.uniform(3, 5),
+                             max_vel=r.uniform(500.0, 2000.0), sf=1.0, vel=[0.0, 0.0, 0.0])
+        minis, back = [], parent["radius"] + 80.0
+        for _ in range(r.randint(2, 4)):
+            minis.append({"kind": "ball", "parent": parent_id,
+                          "offset": [-u[q] * back + r.uniform(-20.0, 20.0) for q in range(3)],
+                          "radius": r.uniform(40.0, 120.0)})
+            back += r.uniform(150.0, 300.0)
+        cmds = [_launch(r.choice([1, 2]), missile_id, target_id, -launcher_id, 1, 1)]
+        out.append({"kind": "defender_own_minis" if own else "defender_other_minis", "friction": 1.0e6,
+                    "tick": r.choice([250, 500, 1000]), "steps": r.randint(10, 25), "new_orbit": False,
+                    "start": r.randrange(0, 2_000_000), "balls": [launcher, parent, target, missile], "cmds": cmds,
+                    "collisions": True, "minis": minis})
+    return out
+
+
 def do_mini_cases(seed: int, scale: int = 1) -> list:
     """mini_cases' worlds with dynamical orientation (turning values drawn as in do_collide_cases)."""
     r = random.Random(seed ^ 0x31D)

@@ -39,3 +39,13 @@ def test_worlds_match_oracle_reversed():
     got = D.run_park(ws, "cpu", order=idx, block=64)
     bad = D.compare(got, D.run_oracle(ws, idx), ws)
     assert not bad, f"{len(bad)} differ: {bad}"
+
+
+@pytest.mark.parametrize("block", [256, 64])
+def test_defenders_through_minis_match_reference_vectors(block):
+    """Defender missiles through the miniballs of a parent whose id is their owner's (skipped) or another's (met)."""
+    golden = G.load("mini_own_golden.npz")
+    kept = golden["kept"]
+    got = D.run_park(golden["worlds"], "cpu", order=kept, initial=True, block=block)
+    bad = compare_golden(got, golden, kept)
+    assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"

@@ -29,3 +29,13 @@ def test_park_with_orientation_matches_reference_vectors():
     got = D.run_park(golden["worlds"], DEV, order=kept, initial=True, block=128)
     bad = compare_golden(got, golden, kept, do=True)
     assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"
+
+
+@pytest.mark.parametrize("block", [256, 64])
+def test_defenders_through_minis_match_reference_vectors(block):
+    """Defender missiles through the miniballs of a parent whose id is their owner's (skipped) or another's (met)."""
+    golden = G.load("mini_own_golden.npz")
+    kept = golden["kept"]
+    got = D.run_park(golden["worlds"], DEV, order=kept, initial=True, block=block)
+    bad = compare_golden(got, golden, kept)
+    assert not bad, f"{len(bad)} (world, step) records differ, first {bad[:6]}"

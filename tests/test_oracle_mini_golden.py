@@ -5,7 +5,11 @@ its owner's miniballs; SetBallHarmonic and the candidates' force field filter (h
 harmonic, corporation or alliance the ball's). Pinned to the reference: every ball of the mini and field worlds
 (tests/scenarios.py `mini_cases`) as added and after every tick, bit for bit with mLastC, the filing, the collision
 lists and the display yaw, pitch and roll, against the vectors the reference's own FullBallpark wrote
-(tests/golden/make_golden_missile.py --minis); and with dynamical orientation (mini_do_golden.npz)."""
+(tests/golden/make_golden_missile.py --minis); and with dynamical orientation (mini_do_golden.npz). The skip of a
+missile's owner's miniballs on the defender worlds of mini_own_golden.npz (`mini_own_cases`, --mini-own): a defender
+whose owner is its parent's negative id passes through the miniballs, another parent's it meets."""
+
+import numpy as np
 
 import pytest
 
@@ -45,3 +49,36 @@ def test_fixture_covers_minis_and_fields(golden):
     kinds = {golden["worlds"][wi]["kind"] for wi in golden["kept"]}
     assert met_mini >= 100 and two_local >= 20 and field_met >= 50, (met_mini, two_local, field_met)
     assert {"minis", "minis_missile", "field", "field_cleared"} <= kinds
+
+
+def test_oracle_defenders_through_minis_match_reference():
+    bad = check(G.load("mini_own_golden.npz"), False)
+    assert not bad, f"{len(bad)} worlds differ, first {bad[:6]}"
+
+
+def test_fixture_reaches_the_owner_miniball_skip():
+    """In every world whose parent is the defender's owner, the defender overlaps one of the parent's miniballs at
+    some tick and never lists one (nor the parent); where the parent is another's, it lists them."""
+    g = G.load("mini_own_golden.npz")
+    own = other = 0
+    for wi in g["kept"]:
+        w = g["worlds"][wi]
+        missile, parent = w["balls"][3], w["balls"][1]
+        overlapped, listed = False, 0
+        for step in range(w["steps"] + 1):
+            d, q, hits = g["records"][(wi, step)]
+            ids = q[:, 0].tolist()
+            k = ids.index(missile["id"])
+            listed += sum(1 for x in hits[k] if x < -1_000_000 or x == parent["id"])
+            minis = [m for m in w["minis"] if m["kind"] == "ball"]
+            local = sorted((i for i in ids if i < -1_000_000), reverse=True)
+            for m, mid in zip(minis, local):
+                j = ids.index(mid)
+                if np.linalg.norm(d[k, 0:3] - d[j, 0:3]) < m["radius"] + missile["radius"]:
+                    overlapped = True
+        if w["kind"] == "defender_own_minis":
+            assert listed == 0, wi
+            own += overlapped
+        else:
+            other += listed > 0
+    assert own >= 15 and other >= 15, (own, other)
