@@ -437,11 +437,9 @@ static inline XB_HD bool xb_cmd_run(const XbPark& P, int64_t r, XbCommand& c, do
     return false;
 }
 
-/* StopAllFollowers' part on one follower (row r, following a marked ball): a missile (no longer massive) and an
-   interactive orbiter fly on along their velocity; any other follower stops. */
-static inline XB_HD void xb_cmd_release(const XbPark& P, int64_t n, int64_t r, const uint8_t* mark) {
-    const int64_t t = XB_Q(P, XB_I_FOLLOW_PTR, r);
-    if (t < 0 || t >= n || !mark[t] || XB_Q(P, XB_I_FOLLOW_ID, r) != XB_Q(P, XB_I_IDS, t)) return;
+/* StopAllFollowers' part on one follower (row r): a missile (no longer massive) and an interactive orbiter fly on along
+   their velocity; any other follower stops. */
+static inline XB_HD void xb_row_released(const XbPark& P, int64_t r) {
     const int64_t mode = XB_Q(P, XB_I_MODE, r);
     const bool missile = mode == XB_MODE_MISSILE;
     if (missile) XB_B(P, XB_B_MASSIVE, r) = 0;
@@ -449,6 +447,13 @@ static inline XB_HD void xb_cmd_release(const XbPark& P, int64_t n, int64_t r, c
         xb_row_goto_direction(P, r, xb_ld3(P, XB_F_VEL, r));
     else
         xb_row_stop(P, r);
+}
+
+/* StopAllFollowers over the park: row r is released when its follow target is marked and still names it by id. */
+static inline XB_HD void xb_cmd_release(const XbPark& P, int64_t n, int64_t r, const uint8_t* mark) {
+    const int64_t t = XB_Q(P, XB_I_FOLLOW_PTR, r);
+    if (t < 0 || t >= n || !mark[t] || XB_Q(P, XB_I_FOLLOW_ID, r) != XB_Q(P, XB_I_IDS, t)) return;
+    xb_row_released(P, r);
 }
 
 /* The flight commands of the serving loop's batch (op, point, target row, range, minimum range, warp factor at row

@@ -17,6 +17,7 @@
 #include "../../include/xb.h"
 #include "xb_park.h"
 #include "xb_space.h"
+#include "xb_server.h"
 
 namespace {
 
@@ -145,7 +146,10 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
             else
                 xb_coll_phase34<DO, false, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k,
                                                  wlo[k], whi[k], P, rot1[k], w1[k]);
-            xb_tick_tail(rows[k]);
+            if (xb_tick_tail(rows[k])) {            /* a petrified troll: SetBallFree(false)'s snap and filing */
+                if (ypr) xb_yaw_pitch_roll(ys[k], rows[k].s, false, rows[k].s.clock, true, P);
+                xb_file(rows[k].s, rows[k].free_, P, cr[k].fil);
+            }
         }
     }
     for (int64_t k = 0; k < cnt; ++k) {
@@ -426,6 +430,16 @@ int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, i
             xs_apply(c, code[k * W + w], a[k * W + w], b[k * W + w], fk);
         }
     }
+    return 0;
+}
+
+int xb_cpu_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q,
+                           int phase) {
+    if (!xsv_args_ok(park, srv, q, phase)) return -1;
+    const XbPark P = xb_park(*park);
+    const int64_t W = srv->W;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t w = 0; w < W; ++w) xsv_world(P, park->n, park->ypr, *srv, *q, phase, w);
     return 0;
 }
 

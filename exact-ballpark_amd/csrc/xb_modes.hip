@@ -255,7 +255,10 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
         __syncthreads();
         if (on) {
             xb_coll_phase34<DO, YPR, ST>(w, c, warped, p, v, C, R, j, wlo, whi, P, rot1, w1, &y);
-            xb_tick_tail(w);
+            if (xb_tick_tail(w)) {                  /* a petrified troll: SetBallFree(false)'s snap and filing */
+                if (YPR && !DO) xb_yaw_pitch_roll(y, w.s, false, w.s.clock, true, P);
+                xb_file(w.s, w.free_, P, c.fil);
+            }
         }
         __syncthreads();
     }

@@ -815,10 +815,12 @@ static inline XB_HD bool xb_mushroom_turn(XbRow& w, XbV3 p, XbV3 v, XoQ rot1, Xo
     return false;
 }
 
-/* The end of a row's tick: PetrifyTroll of a troll due (src/exact_ballpark/vec_modes.py:927-937: fixed, at rest,
-   RIGID), and the world's clock. */
-static inline XB_HD void xb_tick_tail(XbRow& w) {
+/* The end of a row's tick: PetrifyTroll of a troll due (full.py:1800-1807: SetBallFree(false), so fixed, at rest,
+   mLastG cleared; SetBallInteractive(false); RIGID), and the world's clock. Returns whether the row was petrified:
+   the caller then does SetBallFree's rest, the display orientation snapped to the rest velocity and the filing. */
+static inline XB_HD bool xb_tick_tail(XbRow& w) {
     XbShip& s = w.s;
+    bool petrified = false;
     if (xb_row_live(w) && s.mode == XB_MODE_TROLL && !(s.effect_stamp > s.clock)) {
         s.vel = v3(0.0, 0.0, 0.0);
         w.old_vel = s.vel;
@@ -826,8 +828,10 @@ static inline XB_HD void xb_tick_tail(XbRow& w) {
         w.free_ = false;
         s.interactive = false;
         s.mode = XB_MODE_RIGID;
+        petrified = true;
     }
     s.clock = s.clock + 1;
+    return petrified;
 }
 
 /* One tick of the row at place j of its group [lo, hi), without collisions (mLastC stays zero). With YPR (and without
@@ -852,7 +856,7 @@ static inline XB_HD void xb_tick_row(XbRow& w, const XbRecords& R, int j, int lo
             if (YPR && !DO) xb_yaw_pitch_roll(*y, s, false, s.clock, false, P);
         }
     }
-    xb_tick_tail(w);
+    if (xb_tick_tail(w) && YPR && !DO) xb_yaw_pitch_roll(*y, s, false, s.clock, true, P);
 }
 
 #include "xb_collide.h"

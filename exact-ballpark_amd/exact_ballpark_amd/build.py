@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 CPU_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3", "-fopenmp",
              "-Wall", "-Wno-unused-function"]
-HIP_SOURCES = ["xb_modes.hip", "xb_sched.hip", "xb_serve.hip", "xb_rand.hip", "xb_space.hip"]
+HIP_SOURCES = ["xb_modes.hip", "xb_sched.hip", "xb_serve.hip", "xb_rand.hip", "xb_space.hip", "xb_server.hip"]
 BANNED_SOURCE = re.compile(r"__ocml_|__builtin_amdgcn_(rcp|rsq|sqrt|exp|log|sin|cos)|\b__f?(exp|log|sin|cos)f?\(")
 FAST_FLAGS = re.compile(r"\b(fadd|fsub|fmul|fdiv|frem|fneg|fcmp|call)\b[^\n]*\b(fast|contract|afn|reassoc|nnan|ninf|nsz|arcp)\b")
 

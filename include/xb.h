@@ -282,6 +282,66 @@ int xb_space_tape(const xb_modes_state* park, const xb_space_state* space, int64
 int xb_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
                    const int64_t* b, const double* f, int64_t* out, double* outf, void* stream);
 
+/* ---- the server layer's pre-tick (Destiny's Actions queue, Ticker, bubble update and send flags) ---- */
+
+/* The server layer's own state for W worlds of one bubble each, over a park (the world tables world-interleaved: entry
+   k of world w at [k * W + w]; the per-ball tables by park row). Replaces the reference's `struct Srv`
+   (src/exact_ballpark/vec_server_cuda.cu:83-91). */
+typedef struct xb_server_state {
+    int64_t W, N;                 /* worlds; the most balls a world holds */
+    const int32_t* rows;          /* [N][W] the park rows of each world's balls in the order they were added, -1 after */
+    const int64_t* pilot;         /* [n] the pilot flying each row's ship, -1 for none */
+    int64_t* stamp_for_system;    /* [W] the stamp the next queued actions get (Actions._stamp_for_system) */
+    int64_t* stamp;               /* [W] written: the stamp of the actions this pre-tick applied */
+    int64_t* clock;               /* [W] written: the park's time at this pre-tick */
+    int64_t* bubble_time;         /* [W] BubbleUpdater's last park time (-1 before the first) */
+    uint8_t* settled;             /* [n] the ball's old bubble is its bubble (AdditionsAndDeletions has seen it) */
+    uint8_t* fresh;               /* [n] a user ball getting every uncloaked ball of its bubble this tick */
+    uint8_t* listed;              /* [n] the ball is in those lists (uncloaked at the bubble update) */
+    uint8_t* has_character;       /* [n] written: the row's pilot gets a singlecast this tick */
+    uint8_t* gets_bubble;         /* [n] written: the row's pilot gets the bubble's narrowcast this tick */
+    int64_t* wflags;              /* [W] OR-ed: 1 a command the batched park refuses (a mushroom as a target) */
+    int32_t master;               /* the server's view: a normal cloak adds the ball's proximity sensor */
+    int32_t snapshot;             /* write the stream records (rec_add, rec_unc) the decode of the sends needs */
+} xb_server_state;
+
+/* One tick's queue: J calls, each for every world ([J][W], call j of world w at [j * W + w]; f [J][W][4]). kind[j] is
+   the call's action (the reference's KINDS numbering, src/exact_ballpark/vec_server.py:120-125); mask bit 0: pushed
+   for the world, bit 1: already tested by an undo (bit 0 is then its queued flag). Replaces `struct Queue`
+   (vec_server_cuda.cu:70-80). */
+typedef struct xb_server_queue {
+    int64_t J;
+    const int8_t* kind;           /* [J] */
+    const uint8_t* mask;          /* [J][W] */
+    const int64_t* ball;          /* [J][W] Destiny ids */
+    const int64_t* target;        /* [J][W] FollowBall's and Orbit's target ids */
+    const double* f;              /* [J][W][4] the float arguments */
+    const int64_t* n;             /* [J][W] the int argument */
+    const int64_t* loc_in;        /* [J][W] the local argument (SetBallPosition's is_local_ball) */
+    const uint8_t* on;            /* [W] the worlds this pre-tick runs for (OnTick's mask); NULL: every world */
+    uint8_t* ok;                  /* [J][W] written: queued (pushed and not a duplicate of the world's last) */
+    int64_t* row;                 /* [J][W] written: the ball's park row, -1 for none */
+    int64_t* trow;                /* [J][W] written: the target's park row, -1 for none */
+    int64_t* loc;                 /* [J][W] written: the local argument as queued (CloakBall's bubble, 0 or -1) */
+    int8_t* route;                /* [J][W] written: 0 sent nowhere, 1 to the bubble, 2 to the ball's character */
+    uint8_t* notice;              /* [J][n] written for UncloakBall calls: the rows whose pilot gets AddBallsToPark */
+    int64_t* rec_add;             /* [N][W][XB_SREC_WORDS] with snapshot: a world's balls at its bubble update */
+    int64_t* rec_unc;             /* [J][W][XB_SREC_WORDS] with snapshot: an uncloaked ball as its UncloakBall left it */
+} xb_server_queue;
+
+#define XB_SREC_WORDS 34          /* a stream record (exact-ballpark_amd/csrc/xb_server.h numbers its words) */
+
+/* The pre-tick of every world whose `on` byte is set, one lane a world: phase 0 the stamps, the queue in order (the
+   duplicate test against the world's last queued call, each ball's and target's row, every call but CloakBall applied
+   to the park by the park's own commands and setters), the bubble update, the history (CloakBall applied there; each
+   call's route; UncloakBall's notices) and the send flags; phase 1 the send flags alone (after the park's tick, which
+   can change a ball's interactivity); phase 2 the queue's test alone (each call's queued flag, rows and local
+   argument, as phase 0 would find them; nothing else written: the pending queries' view of the queue). Worlds whose
+   `on` byte is clear are not touched and their flags are cleared (phases 0 and 1).
+   Replaces xbpv_pre_tick (src/exact_ballpark/vec_server_cuda.cu:615). */
+int xb_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q, int phase,
+                       void* stream);
+
 /* ---- the learned scheduler (no counterpart in the reference: it orders work, never changes a result) ---- */
 
 #define XB_MLP_IN 12
@@ -393,6 +453,10 @@ int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, i
                       const int64_t* a, const int64_t* b, const double* f);
 int xb_cpu_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
                        const int64_t* b, const double* f, int64_t* out, double* outf);
+
+/* xb_server_pre_tick on a park and server layer in host memory (no stream). */
+int xb_cpu_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q,
+                           int phase);
 
 /* xb_rand_seed and xb_rand_draw on the host (no stream). */
 int xb_cpu_rand_seed(const uint32_t* seeds, int64_t n_worlds, int32_t* states);
