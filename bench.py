@@ -15,6 +15,10 @@ Workloads (synthetic, seeded; exact_ballpark_amd.workloads):
 - partition: the master's space layer over the flight park: 65,536 worlds of 4..64 ships in Gaussian clusters,
   proximity sensors on half the ships, modes uniform; each tick the dead bubbles, the park's tick, the re-filings and
   HandleProximities as one graph replay (measured beside the others, with its own CPU baseline and exactness check).
+- server: the server layer over the flight park: 65,536 worlds x 4 ships, every ship interactive and flown by a pilot,
+  each tick a GotoDirection and a SetSpeedFraction queued for every ship (8 calls a world), the pre-tick, the park's
+  tick and the post-tick as one HIP graph replay (ServerGraph; measured beside the others, checked against the host
+  build on a sample of its worlds).
 - statics: the turning and collisions park with static collidables: 65,536 worlds x (4 ships + 4 fixed spheres) and 8
   static collidables a world (capsules and oriented boxes), dynamical orientation, the gradient method.
 Each ragged workload (composed, ragged, profiles) is measured as the plain padded-to-32 launch (every world padded to 32 rows, lanes and worlds in
@@ -77,6 +81,10 @@ WORKLOADS = {
                  "1-16 Gaussian clusters (sigma 50 km) a world, centres in a 2e9 m cube, half the ships interactive, "
                  "a proximity sensor on half (range 1-250 km, period 1-5 s, half shuffled), modes uniform over "
                  "stop/goto/follow/orbit/warp (follow and orbit within the cluster, warp to another cluster)",
+    "server": "server layer over the flight park: {worlds} worlds x 4 ships 4 km apart, every ship interactive and "
+              "piloted, each tick GotoDirection (a random unit vector) and SetSpeedFraction (uniform 0-1) queued for "
+              "every ship, then the pre-tick (duplicate test, calls applied in queue order, bubble update, routes), "
+              "the park's tick and the send flags",
     "statics": "turning and collisions park with static collidables: {worlds} worlds x (4 free massive ships in a 2 km "
                "cluster + 4 fixed massive spheres of radius 100-5000 m) and 8 static collidables a world (capsules of "
                "length 200-3000 m and radius 30-400 m, oriented boxes of half-extents 30-800 m, in turn, randomly "
@@ -191,6 +199,10 @@ def main() -> int:
                     help="leave out the partition park's line (the master's space layer)")
     ap.add_argument("--partition-only", action="store_true",
                     help="measure the partition park's line alone (its JSON the last line; for profiling)")
+    ap.add_argument("--no-server", dest="server", action="store_false",
+                    help="leave out the server layer's line")
+    ap.add_argument("--server-only", action="store_true",
+                    help="measure the server layer's line alone (its JSON the last line; for profiling)")
     ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -213,6 +225,9 @@ def main() -> int:
     torch.cuda.set_device(0)
     if a.partition_only:
         print(json.dumps(partition(a)), flush=True)
+        return 0
+    if a.server_only:
+        print(json.dumps(server_park(a)), flush=True)
         return 0
 
     def build(kind: str):
@@ -374,6 +389,8 @@ def main() -> int:
     srv = None
     if not a.only and a.partition:
         out["partition_park"] = partition(a)
+    if not a.only and a.server:
+        out["server_park"] = server_park(a)
     if a.episodes:
         out["serving"], srv = serving(a)
     if not a.no_cpu:
@@ -512,6 +529,116 @@ def partition(a) -> dict:
                            "sample": f"{m} of the partition park's worlds (evenly spaced) x {done} ticks, the "
                                      "oracle's master world in its C loop, 1 thread"}
     res["exact"] = {"worlds_checked": m, "ticks": done, "worlds_differing": bad, "bit_exact": bad == 0}
+    return res
+
+
+def _server_world_rows(W: int, seed: int = 7) -> tuple:
+    """The server line's ships (4 a world, 4 km around a centre in a 2e5 m cube) and pilots."""
+    import random
+    r = random.Random(seed)
+    rows = []
+    for w in range(W):
+        c = [r.uniform(-1e5, 1e5) for _ in range(3)]
+        for k in range(4):
+            rows.append({"id": k + 1, "world": w, "time": 0, "mass": 10 ** r.uniform(6.0, 8.0),
+                         "radius": r.uniform(20.0, 80.0), "max_vel": r.uniform(100.0, 400.0),
+                         "pos": [x + r.uniform(-4e3, 4e3) for x in c], "vel": [0.0, 0.0, 0.0],
+                         "agility": r.uniform(0.3, 1.5), "speed_fraction": 1.0, "interactive": True})
+    return rows, [k for _ in range(W) for k in range(4)]
+
+
+def server_park(a) -> dict:
+    """The server layer's line (WORKLOADS["server"]): each tick 8 calls a world queued through a ServerGraph's inputs,
+    then one replay (pre-tick kernel, the park's tick, the flags kernel). The same calls run eagerly on the host build
+    for a sample of the worlds, whose final park state is compared bit for bit (a world's result does not depend on
+    the others)."""
+    import numpy as np
+    import torch
+
+    from exact_ballpark_amd import vec_server as VS
+    from exact_ballpark_amd.vec_modes import VecModesPark
+
+    W = a.worlds
+    rows, pilots = _server_world_rows(W)
+    park = VecModesPark(len(rows), device="cuda", block=256, master=True)
+    park.load(rows)
+    srv = VS.VecServer(VS.ModesAdapter(park), pilots)
+    schedule = [k for _ in range(4) for k in ("GotoDirection", "SetSpeedFraction")]
+    g = VS.ServerGraph(srv, schedule)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    ticks_inputs = []
+    for t in range(4):                       # four input sets, cycled: the calls change from tick to tick
+        ins = []
+        for k in range(4):
+            d = (torch.rand(W, 3, generator=gen, dtype=torch.float64) * 2 - 1)
+            sfr = torch.rand(W, 1, generator=gen, dtype=torch.float64)
+            ins.append((d, sfr))
+        ticks_inputs.append(ins)
+    dev_inputs = [[(d.cuda(), s.cuda()) for d, s in ins] for ins in ticks_inputs]
+    for j in range(8):
+        g.inputs[j]["mask"].fill_(1)
+        g.inputs[j]["ball"].fill_(j // 2 + 1)
+    done = 0
+
+    def tick():
+        nonlocal done
+        for k, (d, sfr) in enumerate(dev_inputs[done % 4]):
+            g.inputs[2 * k]["f"][:, :3].copy_(d)
+            g.inputs[2 * k + 1]["f"][:, :1].copy_(sfr)
+        g.tick()
+        done += 1
+
+    for _ in range(max(4, a.warmup // 4)):
+        tick()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    K = max(16, a.steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        tick()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    lat = []
+    for _ in range(50):
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        tick()
+        s1.record(stream)
+        lat.append((s0, s1))
+    torch.cuda.synchronize()
+    lat = np.array([x.elapsed_time(y) for x, y in lat])
+    srv.check()
+    res = {"workload": WORKLOADS["server"].format(worlds=W), "worlds": W, "ships": len(rows),
+           "world_ticks_per_s": W / (ms / 1000.0), "ship_ticks_per_s": len(rows) / (ms / 1000.0), "ms_per_tick": ms,
+           "ticks_timed": K, "calls_per_tick": 8 * W,
+           "tick_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}
+    print(f"# server: {res}", file=sys.stderr, flush=True)
+    if a.no_cpu:
+        return res
+    m = min(W, 2048)                          # the host build's eager ticks on the first m worlds
+    cp = VecModesPark(4 * m, device="cpu", master=True)
+    cp.load(rows[:4 * m])
+    cs = VS.VecServer(VS.ModesAdapter(cp), pilots[:4 * m])
+    t0 = time.perf_counter()
+    for t in range(done):
+        for k, (d, sfr) in enumerate(ticks_inputs[t % 4]):
+            mask = torch.ones(m, dtype=torch.bool)
+            ball = torch.full((m,), k + 1, dtype=torch.int64)
+            cs.push("GotoDirection", mask, ball, f=d[:m])
+            cs.push("SetSpeedFraction", mask, ball, f=sfr[:m])
+        cs.pre_tick(snapshot=False)
+        cs.evolve()
+        cs.post_tick()
+    dt = time.perf_counter() - t0
+    gp = [blk[:, :4 * m].cpu() for blk in park._blocks()]
+    same = all(torch.equal(x[:, :4 * m], y) for x, y in zip([b.cpu() for b in cp._blocks()], gp))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"value": m * done / dt, "unit": "world-ticks/s", "cores": cores,
+                           "kind": "port", "sample": f"the first {m} worlds x {done} ticks on the host build of the "
+                                                     f"same kernel source (eager pushes, OpenMP over worlds)"}
+    res["exact"] = {"worlds_checked": m, "ticks": done, "bit_exact": bool(same)}
     return res
 
 

@@ -433,6 +433,11 @@ int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, i
     return 0;
 }
 
+int xb_cpu_server_layout(int64_t* out) {
+    xsv_layout(out);
+    return 0;
+}
+
 int xb_cpu_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q,
                            int phase) {
     if (!xsv_args_ok(park, srv, q, phase)) return -1;

@@ -18,6 +18,8 @@
 // Include after xb_commands.h (the interpreter) and include/xb.h.
 #pragma once
 
+#include <stddef.h>
+
 #define XB_SV_NONE 0
 #define XB_SV_BUBBLE 1
 #define XB_SV_CHARACTER 2
@@ -314,6 +316,11 @@ static inline XB_HD void xsv_pass2(const XsvCtx& c) {
     const XbPark& P = c.P;
     const int64_t w = c.w, W = c.W;
     const bool live = xsv_live(c);
+    for (int64_t k = 0; k < c.N; ++k) {        /* a character's history starts with its bubble additions */
+        const int64_t x = xsv_row(c, k);
+        if (x >= 0) S.history[x] = S.fresh[x];
+    }
+    bool bubble = false;
     for (int64_t j = 0; j < Q.J; ++j) {
         const int64_t i = j * W + w;
         const int kind = Q.kind[j];
@@ -331,7 +338,9 @@ static inline XB_HD void xsv_pass2(const XsvCtx& c) {
             for (int64_t k = 0; k < c.N; ++k) {
                 const int64_t x = xsv_row(c, k);
                 if (x < 0) continue;
-                Q.notice[j * c.n + x] = valid && live && XB_B(P, XB_B_INTERACTIVE, x) && x != r && S.pilot[x] >= 0;
+                const bool to = valid && live && XB_B(P, XB_B_INTERACTIVE, x) && x != r && S.pilot[x] >= 0;
+                Q.notice[j * c.n + x] = to;
+                if (to) S.history[x] = 1;
             }
             if (S.snapshot && Q.rec_unc && valid) xsv_record(c, r, Q.rec_unc + i * XB_SREC_WORDS);
         } else if (kind == XB_K_SET_BALL_POSITION) {   /* a local ball's move is sent nowhere */
@@ -340,30 +349,23 @@ static inline XB_HD void xsv_pass2(const XsvCtx& c) {
             route = xsv_route_of(c, r);
         }
         Q.route[i] = route;
+        if (route == XB_SV_BUBBLE) bubble = true;
+        if (route == XB_SV_CHARACTER) S.history[r] = 1;
     }
+    S.bubble_history[w] = bubble;
 }
 
 /* ParkUpdateBatcher.send_batch's split (server.py:947-1001): a pilot gets a singlecast when its character has history
-   (the bubble additions, a call routed to it, an uncloak's notice) and the bubble's narrowcast when the bubble has
-   history and its ship is interactive. */
+   (the bubble additions, a call routed to it, an uncloak's notice: `history`, from the pre-tick) and the bubble's
+   narrowcast when the bubble has history and its ship is interactive (now: after the park's tick). */
 static inline XB_HD void xsv_flags(const XsvCtx& c) {
     const xb_server_state& S = *c.S;
-    const xb_server_queue& Q = *c.Q;
-    const int64_t w = c.w, W = c.W;
-    bool bubble = false;
-    for (int64_t j = 0; j < Q.J; ++j)
-        if (Q.route[j * W + w] == XB_SV_BUBBLE) bubble = true;
+    const bool bubble = S.bubble_history[c.w] != 0;
     for (int64_t k = 0; k < c.N; ++k) {
         const int64_t x = xsv_row(c, k);
         if (x < 0) continue;
-        bool has = S.fresh[x] != 0;
-        for (int64_t j = 0; j < Q.J; ++j) {
-            const int64_t i = j * W + w;
-            if (Q.ok[i] && Q.row[i] == x && Q.route[i] == XB_SV_CHARACTER) has = true;
-            if (Q.kind[j] == XB_K_UNCLOAK_BALL && Q.notice[j * c.n + x]) has = true;
-        }
         const bool pilot = S.pilot[x] >= 0;
-        S.has_character[x] = has && pilot;
+        S.has_character[x] = S.history[x] && pilot;
         S.gets_bubble[x] = bubble && XB_B(c.P, XB_B_INTERACTIVE, x) && pilot;
     }
 }
@@ -402,11 +404,20 @@ static inline bool xsv_args_ok(const xb_modes_state* park, const xb_server_state
     if (!park || !park->f64 || !park->i64 || !park->u8 || !s || !q || phase < 0 || phase > 2) return false;
     if (s->W < 0 || s->N < 0 || (s->W > 0 && (!s->rows || !s->pilot || !s->stamp_for_system || !s->stamp ||
                                                !s->clock || !s->bubble_time || !s->settled || !s->fresh ||
-                                               !s->listed || !s->has_character || !s->gets_bubble || !s->wflags)))
+                                               !s->listed || !s->has_character || !s->gets_bubble || !s->wflags ||
+                                               !s->history || !s->bubble_history)))
         return false;
     if (q->J < 0 || q->J > 4096) return false;
     if (q->J > 0 && (!q->kind || !q->mask || !q->ball || !q->target || !q->f || !q->n || !q->loc_in || !q->ok ||
                      !q->row || !q->trow || !q->loc || !q->route || !q->notice))
         return false;
     return true;
+}
+
+static inline void xsv_layout(int64_t* out) {
+    out[0] = (int64_t)sizeof(xb_server_state);
+    out[1] = (int64_t)offsetof(xb_server_state, wflags);
+    out[2] = (int64_t)sizeof(xb_server_queue);
+    out[3] = (int64_t)offsetof(xb_server_queue, rec_unc);
+    out[4] = XB_SREC_WORDS;
 }

@@ -29,6 +29,11 @@ __global__ void __launch_bounds__(64) xb_server_kernel(xb_modes_state park, xb_s
 
 extern "C" {
 
+int xb_server_layout(int64_t* out) {
+    xsv_layout(out);
+    return 0;
+}
+
 int xb_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q, int phase,
                        void* stream) {
     if (!xsv_args_ok(park, srv, q, phase)) return -1;

@@ -890,6 +890,49 @@ class VecModesPark(metaclass=_Fields):
                 "pos": [0.0, 0.0, 0.0], "vel": [0.0, 0.0, 0.0], "agility": 1.0, "speed_fraction": 1.0, "free": False,
                 "massive": False, "interactive": False, "cloaked": 0, "bubble": -1, "moribund": True, "mode": STOP}
 
+    def add_balls(self, mask: torch.Tensor, rows: list) -> None:
+        """The Python AddBall of new balls mid-run into the empty rows in `mask` (VecCombatPark.add_balls,
+        vec_combat.py:225-266; PyAddBall, Thunkers.cpp:150): one dict a masked row, in row order, with `load`'s keys
+        but the world and the clock, which are the row's. Each row is made as `load` makes a ball (AddBall's clamps and
+        defaults, the decay factor, STOP, with dynamical orientation the rotation snapped, the display orientation of a
+        free ball snapped) in a park of the new balls alone, and its blocks' columns copied in; a collision park files
+        it. A row that is not empty, or an id its world already holds, is refused before anything changes."""
+        idx = torch.nonzero(mask.to(self.dev).reshape(self.n)).reshape(-1)
+        k = idx.numel()
+        if k != len(rows):
+            raise ValueError(f"add_balls: {len(rows)} balls for {k} rows")
+        if k == 0:
+            return
+        if bool((self.ids[idx] != EMPTY_ID).any()):
+            raise ValueError("add_balls: a row is not empty")
+        worlds, clocks = self.world[idx].cpu().tolist(), self.clock[idx].cpu().tolist()
+        taken = set(zip(self.world.cpu().tolist(), self.ids.cpu().tolist()))
+        for w, r in zip(worlds, rows):
+            if (w, int(r["id"])) in taken or int(r["id"]) == EMPTY_ID:
+                raise ValueError(f"add_balls: ball {r['id']} is in world {w} already")
+            taken.add((w, int(r["id"])))
+        sub = VecModesPark(k, self.friction, self.tick_interval, self.dev, self.dynamical_orientation,
+                           self.use_new_orbit, block=self.block, display_ypr=self.display_ypr)
+        sub.load([{**r, "world": w, "time": t} for r, w, t in zip(rows, worlds, clocks)])
+        keep_bubble = not all("bubble" in r for r in rows)     # the empty row's bubble otherwise (its world's)
+        bubble = self._q[9, idx].clone()
+        self._f[:, idx] = sub._f[:, :k]
+        self._q[:, idx] = sub._q[:, :k]
+        self._b[:, idx] = sub._b[:, :k]
+        if keep_bubble:
+            self._q[9, idx] = bubble
+        if self.display_ypr:
+            self._y[:, idx] = sub._y[:, :k]
+        if self._fld is not None:
+            self._fld[:, idx] = -1
+        self._dying[idx] = False
+        self._mush[idx] = False
+        self._index = None
+        if self.collisions:
+            self._cf[:, idx] = 0.0
+            self._cq[7, idx] = 0                       # the collision list's length
+            self._refile(mask)
+
     def add_mushroom(self, mask: torch.Tensor, owner: torch.Tensor, range_: torch.Tensor, life: torch.Tensor) -> None:
         """AddMushroom into the empty rows in `mask`, a client's view (owner a Destiny id in the row's world, the range
         a float, the lifetime a double): refused (the row left empty) for a range or lifetime not above 0 or a missing

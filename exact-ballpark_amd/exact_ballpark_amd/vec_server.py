@@ -76,7 +76,8 @@ class ServerState(ctypes.Structure):
     """include/xb.h's xb_server_state."""
     _fields_ = [("W", ctypes.c_int64), ("N", ctypes.c_int64)] + [
         (n, ctypes.c_void_p) for n in ("rows", "pilot", "stamp_for_system", "stamp", "clock", "bubble_time",
-                                       "settled", "fresh", "listed", "has_character", "gets_bubble", "wflags")] + [
+                                       "settled", "fresh", "listed", "has_character", "gets_bubble", "history",
+                                       "bubble_history", "wflags")] + [
         ("master", ctypes.c_int32), ("snapshot", ctypes.c_int32)]
 
 
@@ -88,13 +89,22 @@ class ServerQueue(ctypes.Structure):
 
 
 def _lib(dev: torch.device) -> ctypes.CDLL:
+    """The library for the park's device, its server entry points bound and its struct layout checked."""
     L = N.hip() if dev.type == "cuda" else N.cpu()
     if not getattr(L, "_server_bound", False):
         vp = ctypes.c_void_p
         if dev.type == "cuda":
             L.xb_server_pre_tick.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+            layout = L.xb_server_layout
         else:
             L.xb_cpu_server_pre_tick.argtypes = [vp, vp, vp, ctypes.c_int]
+            layout = L.xb_cpu_server_layout
+        out = (ctypes.c_int64 * 5)()
+        layout(out)
+        want = (ctypes.sizeof(ServerState), ServerState.wflags.offset, ctypes.sizeof(ServerQueue),
+                ServerQueue.rec_unc.offset, SREC_WORDS)
+        if tuple(out) != want:
+            raise N.NativeError(f"the server layer's layout {tuple(out)} is not the loader's {want}")
         L._server_bound = True
     return L
 
@@ -243,6 +253,8 @@ class VecServer:
         self.listed = torch.zeros(self.R, dtype=U8, device=d)
         self.has_character = torch.zeros(self.R, dtype=U8, device=d)
         self.gets_bubble = torch.zeros(self.R, dtype=U8, device=d)
+        self.history = torch.zeros(self.R, dtype=U8, device=d)
+        self.bubble_history = torch.zeros(self.W, dtype=U8, device=d)
         self.stamp = torch.zeros(self.W, dtype=I64, device=d)
         self.clock_out = torch.zeros(self.W, dtype=I64, device=d)
         self.wflags = torch.zeros(self.W, dtype=I64, device=d)
@@ -263,7 +275,8 @@ class VecServer:
         for name, t in (("stamp_for_system", self.stamp_for_system), ("stamp", self.stamp),
                         ("clock", self.clock_out), ("bubble_time", self.bubble_time), ("settled", self.settled),
                         ("fresh", self.fresh), ("listed", self.listed), ("has_character", self.has_character),
-                        ("gets_bubble", self.gets_bubble), ("wflags", self.wflags)):
+                        ("gets_bubble", self.gets_bubble), ("wflags", self.wflags), ("history", self.history),
+                        ("bubble_history", self.bubble_history)):
             setattr(s, name, t.data_ptr())
         s.master, s.snapshot = int(self.a.master), int(bool(snapshot))
         return s
@@ -615,7 +628,7 @@ class ServerGraph:
     def _server_tensors(self) -> tuple:
         S = self.server
         return (S.stamp_for_system, S.bubble_time, S.settled, S.fresh, S.listed, S.has_character, S.gets_bubble,
-                S.stamp, S.clock_out, S.wflags)
+                S.stamp, S.clock_out, S.wflags, S.history, S.bubble_history)
 
     def _pointers(self) -> tuple:
         s = self.server.a.park.state()
@@ -635,14 +648,15 @@ class ServerGraph:
             raise RuntimeError("the park replaced a buffer the graph's launches point at: capture again")
         self.graph.replay()
         S.a.park.current_time += 1
+        q = self.q                               # the kernel writes 0 or 1: the bytes viewed as bools, no launch
         for j, s in enumerate(self.slots):
-            s.mask = self.q["mask"][j] != 0
-            s.ok, s.row, s.trow, s.loc = self.q["ok"][j] != 0, self.q["row"][j], self.q["trow"][j], self.q["loc"][j]
-        S.batch = Batch(stamp=S.stamp, clock=S.clock_out, slots=self.slots, route=self.q["route"].t(),
-                        notices=[(j, self.q["notice"][j] != 0, None) for j, s in enumerate(self.slots)
+            s.mask = q["mask"][j].view(BOOL)
+            s.ok, s.row, s.trow, s.loc = q["ok"][j].view(BOOL), q["row"][j], q["trow"][j], q["loc"][j]
+        S.batch = Batch(stamp=S.stamp, clock=S.clock_out, slots=self.slots, route=q["route"].t(),
+                        notices=[(j, q["notice"][j].view(BOOL), None) for j, s in enumerate(self.slots)
                                  if s.kind == UNCLOAK],
-                        additions=(S.fresh != 0, S.listed != 0, None), gets_bubble=S.gets_bubble != 0,
-                        has_character=S.has_character != 0)
+                        additions=(S.fresh.view(BOOL), S.listed.view(BOOL), None),
+                        gets_bubble=S.gets_bubble.view(BOOL), has_character=S.has_character.view(BOOL))
         return S.batch
 
 

@@ -300,6 +300,9 @@ typedef struct xb_server_state {
     uint8_t* listed;              /* [n] the ball is in those lists (uncloaked at the bubble update) */
     uint8_t* has_character;       /* [n] written: the row's pilot gets a singlecast this tick */
     uint8_t* gets_bubble;         /* [n] written: the row's pilot gets the bubble's narrowcast this tick */
+    uint8_t* history;             /* [n] written: the row's character has history this tick (additions, a call routed
+                                     to it, an uncloak's notice), before the pilot gate */
+    uint8_t* bubble_history;      /* [W] written: the world's bubble has history this tick */
     int64_t* wflags;              /* [W] OR-ed: 1 a command the batched park refuses (a mushroom as a target) */
     int32_t master;               /* the server's view: a normal cloak adds the ball's proximity sensor */
     int32_t snapshot;             /* write the stream records (rec_add, rec_unc) the decode of the sends needs */
@@ -330,6 +333,11 @@ typedef struct xb_server_queue {
 } xb_server_queue;
 
 #define XB_SREC_WORDS 34          /* a stream record (exact-ballpark_amd/csrc/xb_server.h numbers its words) */
+
+/* The layout as built, for the loader's check: out[0] = sizeof(xb_server_state), out[1] its `wflags` offset, out[2] =
+   sizeof(xb_server_queue), out[3] its `rec_unc` offset, out[4] = XB_SREC_WORDS. Plays the part of the reference's ABI
+   check at load (src/exact_ballpark/vec_server_cuda.py's xbpv_abi). */
+int xb_server_layout(int64_t* out);
 
 /* The pre-tick of every world whose `on` byte is set, one lane a world: phase 0 the stamps, the queue in order (the
    duplicate test against the world's last queued call, each ball's and target's row, every call but CloakBall applied
@@ -454,7 +462,8 @@ int xb_cpu_space_tape(const xb_modes_state* park, const xb_space_state* space, i
 int xb_cpu_space_query(const xb_modes_state* park, const xb_space_state* space, int64_t kind, const int64_t* a,
                        const int64_t* b, const double* f, int64_t* out, double* outf);
 
-/* xb_server_pre_tick on a park and server layer in host memory (no stream). */
+/* xb_server_layout and xb_server_pre_tick on a park and server layer in host memory (no stream). */
+int xb_cpu_server_layout(int64_t* out);
 int xb_cpu_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q,
                            int phase);
 

@@ -141,10 +141,11 @@ def _snapshot(park: VecModesPark, members: list, rows_of: dict) -> dict:
 
 def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bool = False,
              general: bool = False, order=None, initial: bool = False, events: dict | None = None,
-             ontick: bool = False) -> dict:
+             ontick: bool = False, make=None) -> dict:
     """{(world index, step): (d, q, followers)} after each recorded tick (`record`: a set of steps, None for all).
     `general`: force the general (untiled) launch; `order`: the world indices in row order (default ascending);
-    `initial`: also step 0, the state as loaded. A world flagged `ypr` keeps the display orientation."""
+    `initial`: also step 0, the state as loaded. A world flagged `ypr` keeps the display orientation. `make(n,
+    friction, tick, device, new_orbit, block, dynamical, collisions, ypr, master)`: the park (default VecModesPark)."""
     groups = defaultdict(list)
     for wi in (order if order is not None else range(len(worlds))):
         w = worlds[wi]
@@ -159,9 +160,12 @@ def run_park(worlds: list, device: str, block: int = 256, record=None, graph: bo
                  for b in worlds[wi]["balls"] + [{"_row": m, "id": m["id"]} for m in minis.get(wi, ([], []))[0]]
                  + ([None] if any(c[1] == S.MUSHROOM_OP for c in worlds[wi]["cmds"])
                     else [])]     # the world's miniballs; its empty row for a mushroom
-        park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
-                            use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical, collisions=coll,
-                            display_ypr=ypr, master=master)
+        if make is None:
+            park = VecModesPark(len(balls), friction=friction, tick_interval_ms=tick, device=device,
+                                use_new_orbit=new_orbit, block=block, dynamical_orientation=dynamical,
+                                collisions=coll, display_ypr=ypr, master=master)
+        else:
+            park = make(len(balls), friction, tick, device, new_orbit, block, dynamical, coll, ypr, master)
         park.load([VecModesPark.empty_row(wi, worlds[wi]["start"]) if b is None else b["_row"] if "_row" in b else
                    {"id": b["id"], "world": wi, "time": worlds[wi]["start"], "mass": b["mass"], "radius": b["radius"],
                     "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
