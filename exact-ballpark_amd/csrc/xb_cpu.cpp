@@ -444,7 +444,7 @@ int xb_cpu_server_pre_tick(const xb_modes_state* park, const xb_server_state* sr
     const XbPark P = xb_park(*park);
     const int64_t W = srv->W;
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t w = 0; w < W; ++w) xsv_world(P, park->n, park->ypr, *srv, *q, phase, w);
+    for (int64_t w = 0; w < W; ++w) xsv_world(P, park->n, park->ypr, *srv, *q, srv->space, phase, w);
     return 0;
 }
 

@@ -1,10 +1,12 @@
 // SPDX-License-Identifier: MIT
 //
 // The server layer's pre-tick, one lane a world, run by the host build and the gfx950 kernels alike: Destiny's
-// net/server for a world of one bubble (Actions' queue with its duplicate test, the Ticker's tick, BubbleUpdater's
-// additions, the history's routes, ParkUpdateBatcher's send flags), as the reference's batched layer does it
-// (src/exact_ballpark/vec_server.py:660-1066; its kernel vec_server_cuda.cu:445-595; the scalar layer server.py:397-1143
-// with Destiny's _actions.py, _baseticker.py, _ticker.py, _bubbleupdater.py and _parkupdatebatcher.py).
+// net/server (Actions' queue with its duplicate test, the Ticker's tick, BubbleUpdater's additions, the history's
+// routes, ParkUpdateBatcher's send flags), as the reference's batched layers do it for a world of one bubble
+// (src/exact_ballpark/vec_server.py:660-1066; its kernel vec_server_cuda.cu:445-595) and, over the master's partition
+// (xb_space.h), for a world of several bubbles (vec_server_space.py:225-661; its kernel vec_server_space_cuda.cu:
+// 355-427); the scalar layer is server.py:397-1143 with Destiny's _actions.py, _baseticker.py, _ticker.py,
+// _bubbleupdater.py and _parkupdatebatcher.py.
 //
 // A world's calls run in queue order on its own lane, each applied to the park at once by the park's own commands and
 // setters (xb_commands.h), so a call reads the rows the calls before it wrote, as Destiny's Ticker applies them one by
@@ -87,17 +89,51 @@ struct XsvCtx {
     const xb_server_state* S;
     const xb_server_queue* Q;
     int64_t w, W, N;
+    XsCtx sp;                  /* the partition (sp.s null: one bubble a world) */
 };
 
 static inline XB_HD int64_t xsv_row(const XsvCtx& c, int64_t k) { return c.S->rows[k * c.W + c.w]; }
 
-/* The world's row of the ball with Destiny id `id`, or -1 (ids are distinct within a world). */
-static inline XB_HD int64_t xsv_lookup(const XsvCtx& c, int64_t id) {
+/* The world's slot of the ball with Destiny id `id`, or -1 (ids are distinct within a world). */
+static inline XB_HD int xsv_slot_of(const XsvCtx& c, int64_t id) {
     for (int64_t k = 0; k < c.N; ++k) {
         const int64_t x = xsv_row(c, k);
-        if (x >= 0 && XB_Q(c.P, XB_I_IDS, x) == id) return x;
+        if (x >= 0 && XB_Q(c.P, XB_I_IDS, x) == id) return (int)k;
     }
     return -1;
+}
+
+/* The world's row of the ball with Destiny id `id`, or -1. */
+static inline XB_HD int64_t xsv_lookup(const XsvCtx& c, int64_t id) {
+    const int k = xsv_slot_of(c, id);
+    return k >= 0 ? xsv_row(c, k) : -1;
+}
+
+/* the slot of a row of this world, -1 for none */
+static inline XB_HD int xsv_slot_of_row(const XsvCtx& c, int64_t r) {
+    for (int64_t k = 0; k < c.N; ++k)
+        if (xsv_row(c, k) == r) return (int)k;
+    return -1;
+}
+
+/* ---- the partition's sets, as the server reads them ---- */
+
+static inline XB_HD bool xsv_in_ia(const XsvCtx& c, int64_t bub, int k) {   /* slot k in bubbleInteractives[bub] */
+    if (bub == -1) return false;
+    for (int i = 0; i < XS_WORLD(c.sp, in_); ++i) {
+        const int64_t a = (int64_t)i * c.W + c.w;
+        if (c.sp.s->i_bub[a] == bub && c.sp.s->i_slot[a] == k) return true;
+    }
+    return false;
+}
+
+static inline XB_HD int64_t xsv_ia_bubble(const XsvCtx& c, int k) {   /* the set holding slot k (the last), or -1 */
+    int64_t b = -1;
+    for (int i = 0; i < XS_WORLD(c.sp, in_); ++i) {
+        const int64_t a = (int64_t)i * c.W + c.w;
+        if (c.sp.s->i_slot[a] == k) b = c.sp.s->i_bub[a];
+    }
+    return b;
 }
 
 /* StopAllFollowers of row r, over the world's rows. */
@@ -115,8 +151,9 @@ static inline XB_HD void xsv_release(const XsvCtx& c, int64_t r) {
 static inline XB_HD void xsv_cloak(const XsvCtx& c, int64_t r, int64_t mode_in, double range) {
     const int mode = (int)(int8_t)(uint8_t)(mode_in & 0xFF);
     if (r < 0 || mode <= 0) return;
-    if (c.S->master && mode == XB_DST_NORMAL_CLOAK && !(xb_f32(XB_F(c.P, XB_F_RADIUS, r) + xb_f32(range)) < 0.0))
-        XB_B(c.P, XB_B_SENSOR, r) = 1;
+    if (c.S->master && !c.sp.s && mode == XB_DST_NORMAL_CLOAK &&
+        !(xb_f32(XB_F(c.P, XB_F_RADIUS, r) + xb_f32(range)) < 0.0))
+        XB_B(c.P, XB_B_SENSOR, r) = 1;           /* (with the partition its CLOAK_SENSOR op adds the sensor) */
     xsv_release(c, r);
     XB_Q(c.P, XB_I_CLOAKED, r) = mode & 0xFF;
     XB_B(c.P, XB_B_MASSIVE, r) = 0;
@@ -126,7 +163,7 @@ static inline XB_HD void xsv_cloak(const XsvCtx& c, int64_t r, int64_t mode_in, 
    warp proper. */
 static inline XB_HD void xsv_uncloak(const XsvCtx& c, int64_t r) {
     if (r < 0) return;
-    if (c.S->master && XB_Q(c.P, XB_I_CLOAKED, r) == XB_DST_NORMAL_CLOAK) XB_B(c.P, XB_B_SENSOR, r) = 0;
+    if (c.S->master && !c.sp.s && XB_Q(c.P, XB_I_CLOAKED, r) == XB_DST_NORMAL_CLOAK) XB_B(c.P, XB_B_SENSOR, r) = 0;
     XB_Q(c.P, XB_I_CLOAKED, r) = 0;
     const bool warping = XB_Q(c.P, XB_I_MODE, r) == XB_MODE_WARP && XB_Q(c.P, XB_I_EFFECT_STAMP, r) >= 0;
     if (!warping) XB_B(c.P, XB_B_MASSIVE, r) = 1;
@@ -176,6 +213,28 @@ static inline XB_HD void xsv_apply(const XsvCtx& c, int kind, int64_t r, int64_t
     if (xb_cmd_run(c.P, r, cmd, c.ypr)) xsv_release(c, r);
 }
 
+/* A queued call but CloakBall on slot k (row r) with the partition: the park's part, then the partition's (the
+   re-filing of SetBallPosition, of SetBallRadius and SetMaxSpeed when accepted and of SetBallFree fixing a free ball;
+   SetBallInteractive's sets in place of the park's flag; UncloakBall's sensors before the park's part). */
+static inline XB_HD void xsv_apply_space(const XsvCtx& c, int kind, int k, int64_t r, int64_t tr, const double* f,
+                                         int64_t nn) {
+    const double z[4] = {0.0, 0.0, 0.0, 0.0};
+    if (kind == XB_K_SET_BALL_INTERACTIVE) {
+        xs_apply(c.sp, XS_OP_SET_INTERACTIVE, k, nn != 0, z);
+        return;
+    }
+    if (kind == XB_K_UNCLOAK_BALL) {
+        xs_apply(c.sp, XS_OP_UNCLOAK, k, 0, z);
+        xsv_uncloak(c, r);
+        return;
+    }
+    bool refile = kind == XB_K_SET_BALL_POSITION;
+    if (kind == XB_K_SET_BALL_RADIUS || kind == XB_K_SET_MAX_SPEED) refile = !(xb_f32(f[0]) < 0.0);
+    if (kind == XB_K_SET_BALL_FREE) refile = nn == 0 && XB_B(c.P, XB_B_FREE, r) != 0;
+    xsv_apply(c, kind, r, tr, f, nn);
+    if (refile) xs_apply(c.sp, XS_OP_REFILE, k, 0, z);
+}
+
 /* The fields WriteBallToStream writes of row x, into record `rec`. */
 static inline XB_HD void xsv_record(const XsvCtx& c, int64_t x, int64_t* rec) {
     const XbPark& P = c.P;
@@ -212,25 +271,47 @@ static inline XB_HD void xsv_record(const XsvCtx& c, int64_t x, int64_t* rec) {
 /* The queue in order: each call tested against the world's last queued call (Actions._add_to_system_history's
    duplicate test, server.py:413-425, as the pre-tick meets the calls), its rows looked up, and, with `apply`, every
    queued call but CloakBall applied to the park (Ticker._apply_park_parent_methods_except_cloak_ball). */
-static inline XB_HD void xsv_queue(const XsvCtx& c, bool apply) {
+static inline XB_HD void xsv_queue(const XsvCtx& c, bool apply, int64_t J) {
     const xb_server_state& S = *c.S;
     const xb_server_queue& Q = *c.Q;
     const int64_t w = c.w, W = c.W;
+    const bool space = c.sp.s != nullptr;
+    /* CloakBall's bubble as queued (Actions.cloak_ball reads it at the call: its old bubble while the two differ,
+       server.py:551-561; one bubble: 0, or -1 while its old bubble is -1), from the state the pre-tick starts from,
+       which is the state at the call unless the park ticked in between (mask bit 2: the host froze it in loc_in) */
+    for (int64_t j = 0; j < J; ++j) {
+        if (Q.kind[j] != XB_K_CLOAK_BALL) continue;
+        const int64_t i = j * W + w;
+        if (Q.mask[i] & 4) {
+            Q.loc[i] = Q.loc_in[i];
+            continue;
+        }
+        const int k = xsv_slot_of(c, Q.ball[i]);
+        int64_t loc = -1;
+        if (k >= 0 && space) {
+            const int64_t nb = XS_SLOT(c.sp, bubble, k), ob = XS_SLOT(c.sp, old_bubble, k);
+            loc = nb != ob ? ob : nb;
+        } else if (k >= 0) {
+            loc = S.settled[xsv_row(c, k)] ? 0 : -1;
+        }
+        Q.loc[i] = loc;
+    }
     bool has_last = false;
     int64_t lk = 0, lb = 0, lt = 0, ln = 0, ll = 0;
     double lf[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t j = 0; j < Q.J; ++j) {
+    for (int64_t j = 0; j < J; ++j) {
         const int64_t i = j * W + w;
         const int kind = Q.kind[j];
         const int64_t b = Q.ball[i], t = Q.target[i], nn = Q.n[i];
         const double* f = Q.f + 4 * i;
-        const int64_t r = xsv_lookup(c, b);
+        const int k = xsv_slot_of(c, b);
+        const int64_t r = k >= 0 ? xsv_row(c, k) : -1;
         const int64_t tr = (kind == XB_K_FOLLOW_BALL || kind == XB_K_ORBIT) ? xsv_lookup(c, t) : -1;
         const uint8_t mv = Q.mask[i];
         bool pushed = (mv & 1) != 0;
         int64_t loc = Q.loc_in[i];
-        if (kind == XB_K_CLOAK_BALL) {          /* its bubble as queued: 0, or -1 before its old bubble is its bubble */
-            loc = (r >= 0 && S.settled[r]) ? 0 : -1;
+        if (kind == XB_K_CLOAK_BALL) {
+            loc = Q.loc[i];
             pushed = pushed && r >= 0;          /* Actions.cloak_ball queues nothing for a ball not in the park */
         }
         const bool tested = (mv & 2) != 0;      /* an undo's outcome: queued as bit 0 says */
@@ -245,8 +326,115 @@ static inline XB_HD void xsv_queue(const XsvCtx& c, bool apply) {
         has_last = true;
         lk = kind, lb = b, lt = t, ln = nn, ll = loc;
         for (int k = 0; k < 4; ++k) lf[k] = f[k];
-        if (apply && kind != XB_K_CLOAK_BALL && r >= 0) xsv_apply(c, kind, r, tr, f, nn);
+        if (!apply || kind == XB_K_CLOAK_BALL || r < 0) continue;
+        if (space)
+            xsv_apply_space(c, kind, k, r, tr, f, nn);
+        else
+            xsv_apply(c, kind, r, tr, f, nn);
     }
+}
+
+/* Actions.clean_up_followers (server.py:570-573) at the pre-tick's start, with the partition: a Stop queued behind
+   the pushed calls for every free follower of a ball missing or in another bubble (GetRemoteFollowers, ascending
+   id), into the queue's slots J_user .. J - 1. */
+static inline XB_HD void xsv_clean_up(const XsvCtx& c) {
+    const xb_server_queue& Q = *c.Q;
+    int64_t m = Q.J_user;
+    for (int64_t q = 0; q < c.N; ++q) {
+        const int j = c.sp.s->by_id[XS_AT(c.sp, q)];
+        if (j < 0 || !xs_in(c.sp, j)) continue;
+        const int64_t r = xs_row(c.sp, j);
+        if (!XB_B(c.P, XB_B_FREE, r) || !xb_follows(XB_Q(c.P, XB_I_MODE, r))) continue;
+        const int t = xs_slot_of_id(c.sp, XB_Q(c.P, XB_I_FOLLOW_ID, r));
+        if (t >= 0 && XS_SLOT(c.sp, bubble, t) == XS_SLOT(c.sp, bubble, j)) continue;
+        if (m >= Q.J) break;
+        const int64_t i = m * c.W + c.w;
+        Q.ball[i] = xs_id(c.sp, j), Q.mask[i] = 1;
+        ++m;
+    }
+    for (; m < Q.J; ++m) Q.mask[m * c.W + c.w] = 0, Q.ball[m * c.W + c.w] = 0;
+}
+
+/* Ballpark::PopulateAddDelForBubble (server.py:1775-1793): the bubble's dictionary in order, its additions (+1) and
+   removals (-1); no cloaked addition, no removal of a cloaked ball still in the park. */
+static inline XB_HD void xsv_populate(const XsvCtx& c, int64_t bub, int64_t x) {
+    const xb_server_state& S = *c.S;
+    for (int e = 0; e < XS_WORLD(c.sp, en); ++e) {
+        const int64_t a = (int64_t)e * c.W + c.w;
+        if (c.sp.s->e_bub[a] != bub) continue;
+        const int j = c.sp.s->e_slot[a];
+        const int act = c.sp.s->e_act[a];
+        const bool here = xs_in(c.sp, j);
+        const bool cl = here && XB_Q(c.P, XB_I_CLOAKED, xs_row(c.sp, j)) != 0;
+        if (act != -1 ? (!here || cl) : cl) continue;
+        if (act == 1) {
+            if (S.ad_add) S.ad_add[(int64_t)S.ad_addn[x] * c.n + x] = (int8_t)j;
+            S.ad_addn[x]++;
+        } else if (act == -1) {
+            if (S.ad_del) S.ad_del[(int64_t)S.ad_deln[x] * c.n + x] = (int8_t)j;
+            S.ad_deln[x]++;
+        }
+    }
+}
+
+/* Ballpark::PyAdditionsAndDeletions (server.py:1807-1889) and CopyBubbles: per user (interactive) ball its lists, its
+   bubble's when it stayed in a bubble with a dictionary (1 when the bubble has a pre-built list, an interactive ball,
+   2 otherwise), else its own across its new and old bubbles (3); then its old bubble becomes its bubble. */
+static inline XB_HD void xsv_additions_space(const XsvCtx& c) {
+    const xb_server_state& S = *c.S;
+    for (int64_t k = 0; k < c.N; ++k) {
+        const int64_t x = xsv_row(c, k);
+        if (x < 0) continue;
+        S.ad_kind[x] = 0, S.ad_addn[x] = 0, S.ad_deln[x] = 0;
+    }
+    for (int p = 0; p < XS_WORLD(c.sp, in_); ++p) {
+        const int u = c.sp.s->i_slot[(int64_t)p * c.W + c.w];
+        for (int p2 = 0; p2 < p; ++p2)                  /* a ball in two interactive sets: outside the domain */
+            if (c.sp.s->i_slot[(int64_t)p2 * c.W + c.w] == u) xs_flag(c.sp, XS_F_SETS);
+        if (!xs_in(c.sp, u)) continue;
+        const int64_t x = xs_row(c.sp, u);
+        const int64_t nb = XS_SLOT(c.sp, bubble, u), ob = XS_SLOT(c.sp, old_bubble, u);
+        const bool dict_b = nb != -1 && xs_dict_has(c.sp, nb), dict_o = ob != -1 && xs_dict_has(c.sp, ob);
+        S.ad_addn[x] = 0, S.ad_deln[x] = 0;
+        if (nb == ob && dict_b) {
+            S.ad_kind[x] = xs_pair_count(c.sp, 0, nb) > 0 ? 1 : 2;
+            xsv_populate(c, nb, x);
+        } else {
+            S.ad_kind[x] = 3;
+            for (int pass = 0; pass < 2; ++pass) {   /* the new bubble's entries, then the old bubble's */
+                const int64_t bub = pass == 0 ? nb : ob;
+                if (pass == 0 ? !dict_b : !dict_o) continue;
+                for (int e = 0; e < XS_WORLD(c.sp, en); ++e) {
+                    const int64_t a = (int64_t)e * c.W + c.w;
+                    if (c.sp.s->e_bub[a] != bub) continue;
+                    const int j = c.sp.s->e_slot[a];
+                    const int act = c.sp.s->e_act[a];
+                    const bool here = xs_in(c.sp, j);
+                    const bool cl = here && XB_Q(c.P, XB_I_CLOAKED, xs_row(c.sp, j)) != 0;
+                    bool add = false, del = false;
+                    if (pass == 0) {
+                        if (cl) continue;
+                        const bool was_in = dict_o && xs_entry_find(c.sp, ob, j) >= 0;
+                        add = act >= 0 && !was_in;
+                        del = act == -1 && (was_in || !dict_o);
+                    } else {
+                        if (act != -1 ? (!here || cl) : cl) continue;
+                        const bool is_in = dict_b && xs_entry_find(c.sp, nb, j) >= 0;
+                        del = !is_in && act <= 0;
+                    }
+                    if (add) {
+                        if (S.ad_add) S.ad_add[(int64_t)S.ad_addn[x] * c.n + x] = (int8_t)j;
+                        S.ad_addn[x]++;
+                    } else if (del) {
+                        if (S.ad_del) S.ad_del[(int64_t)S.ad_deln[x] * c.n + x] = (int8_t)j;
+                        S.ad_deln[x]++;
+                    }
+                }
+            }
+        }
+        XS_SLOT(c.sp, old_bubble, u) = nb;
+    }
+    xs_copy_bubbles(c.sp);
 }
 
 /* The stamps, the queue and the bubble update (Ticker.tick up to _update_bubbles, server.py:656-776). */
@@ -260,7 +448,30 @@ static inline XB_HD void xsv_pass1(const XsvCtx& c) {
     S.stamp[w] = S.stamp_for_system[w];
     S.clock[w] = clock;
     S.stamp_for_system[w] = clock + 1;
-    xsv_queue(c, true);
+    if (c.sp.s) {
+        xsv_clean_up(c);
+        xsv_queue(c, true, Q.J);
+        /* BubbleUpdater over the partition, once per park time (its lists kept otherwise) */
+        if (clock != S.bubble_time[w]) xsv_additions_space(c);
+        S.bubble_time[w] = clock;
+        bool adds = false;
+        for (int64_t k = 0; k < c.N; ++k) {
+            const int64_t x = xsv_row(c, k);
+            if (x >= 0 && S.ad_kind[x] != 0 && S.ad_addn[x] > 0) adds = true;
+        }
+        if (S.snapshot && Q.rec_add && adds) {
+            for (int64_t k = 0; k < c.N; ++k) {
+                const int64_t x = xsv_row(c, k);
+                int64_t* rec = Q.rec_add + (k * W + w) * XB_SREC_WORDS;
+                if (x >= 0)
+                    xsv_record(c, x, rec);
+                else
+                    rec[XB_SR_FLAGS] = 0;
+            }
+        }
+        return;
+    }
+    xsv_queue(c, true, Q.J);
     /* BubbleUpdater: AdditionsAndDeletions once per park time (Thunkers.cpp:2570-2895 for one bubble): a user ball
        whose old bubble is not yet its bubble gets every uncloaked ball, itself included, when there is one */
     bool any_listed = false;
@@ -355,6 +566,85 @@ static inline XB_HD void xsv_pass2(const XsvCtx& c) {
     S.bubble_history[w] = bubble;
 }
 
+/* The history in queue order over the partition (Ticker._apply_history with SpaceServer's routes,
+   vec_server_space.py:478-520): CloakBall to the bubble it was queued in, then its sensor and the park's cloak;
+   UncloakBall to its own character and its AddBallsToPark to the other piloted balls of its bubble's interactive set;
+   any other call to its ball's bubble when that bubble has a dictionary and an interactive ball (a local
+   SetBallPosition nowhere), or to the ball's character while it is cloaked. A character's history starts with its
+   bubble update's lists. */
+static inline XB_HD void xsv_pass2_space(const XsvCtx& c) {
+    const xb_server_state& S = *c.S;
+    const xb_server_queue& Q = *c.Q;
+    const XbPark& P = c.P;
+    const int64_t w = c.w, W = c.W;
+    for (int64_t k = 0; k < c.N; ++k) {
+        const int64_t x = xsv_row(c, k);
+        if (x >= 0) S.history[x] = S.ad_kind[x] != 0 && (S.ad_addn[x] > 0 || S.ad_deln[x] > 0);
+    }
+    for (int64_t j = 0; j < Q.J; ++j) {
+        const int64_t i = j * W + w;
+        const int kind = Q.kind[j];
+        const bool ok = Q.ok[i] != 0;
+        const int64_t r = Q.row[i];
+        const bool valid = ok && r >= 0;
+        const int k = valid ? xsv_slot_of_row(c, r) : -1;
+        int8_t route = XB_SV_NONE;
+        int64_t rb = -1;
+        if (kind == XB_K_CLOAK_BALL) {
+            if (ok) route = XB_SV_BUBBLE, rb = Q.loc[i];
+            if (valid) {
+                const double fc[4] = {Q.f[4 * i], 0.0, 0.0, 0.0};
+                xs_apply(c.sp, XS_OP_CLOAK_SENSOR, k, Q.n[i], fc);
+                xsv_cloak(c, r, Q.n[i], Q.f[4 * i]);
+            }
+        } else if (kind == XB_K_UNCLOAK_BALL) {
+            const int64_t bub = valid ? XS_SLOT(c.sp, bubble, k) : -1;
+            if (valid && xsv_in_ia(c, bub, k) && S.pilot[r] >= 0) route = XB_SV_CHARACTER;
+            for (int64_t q = 0; q < c.N; ++q) {
+                const int64_t x = xsv_row(c, q);
+                if (x < 0) continue;
+                const bool to = valid && x != r && S.pilot[x] >= 0 && xsv_in_ia(c, bub, (int)q);
+                Q.notice[j * c.n + x] = to;
+                if (to) S.history[x] = 1;
+            }
+            if (S.snapshot && Q.rec_unc && valid) xsv_record(c, r, Q.rec_unc + i * XB_SREC_WORDS);
+        } else if (valid && Q.ball[i] > 0) {
+            const int64_t bub = XS_SLOT(c.sp, bubble, k);
+            bool live = bub != -1 && xs_dict_has(c.sp, bub) && xs_pair_count(c.sp, 0, bub) > 0;
+            if (kind == XB_K_SET_BALL_POSITION) live = live && Q.loc[i] == 0;
+            if (live) {
+                route = xsv_route_of(c, r);
+                if (route == XB_SV_BUBBLE) rb = bub;
+            }
+        }
+        Q.route[i] = route;
+        Q.rbubble[i] = rb;
+        if (route == XB_SV_CHARACTER) S.history[r] = 1;
+    }
+}
+
+/* send_batch over the partition (vec_server_space.py:522-553): a pilot gets its ship's bubble's narrowcast when its
+   ship is in a bubble's interactive set (now: after the park's tick) and a call went to that bubble, and a singlecast
+   when its character has history. */
+static inline XB_HD void xsv_flags_space(const XsvCtx& c) {
+    const xb_server_state& S = *c.S;
+    const xb_server_queue& Q = *c.Q;
+    for (int64_t k = 0; k < c.N; ++k) {
+        const int64_t x = xsv_row(c, k);
+        if (x < 0) continue;
+        const int64_t ib = xsv_ia_bubble(c, (int)k);
+        bool hist = false;
+        for (int64_t j = 0; ib != -1 && j < Q.J && !hist; ++j) {
+            const int64_t i = j * c.W + c.w;
+            hist = Q.route[i] == XB_SV_BUBBLE && Q.rbubble[i] == ib;
+        }
+        const bool pilot = S.pilot[x] >= 0;
+        S.has_character[x] = S.history[x] && pilot;
+        S.gets_bubble[x] = hist && pilot;
+        S.narrow[x] = hist && pilot ? ib : -1;
+    }
+}
+
 /* ParkUpdateBatcher.send_batch's split (server.py:947-1001): a pilot gets a singlecast when its character has history
    (the bubble additions, a call routed to it, an uncloak's notice: `history`, from the pre-tick) and the bubble's
    narrowcast when the bubble has history and its ship is interactive (now: after the park's tick). */
@@ -377,15 +667,16 @@ static inline XB_HD void xsv_quiet(const XsvCtx& c) {
         if (x < 0) continue;
         c.S->has_character[x] = 0;
         c.S->gets_bubble[x] = 0;
+        if (c.S->narrow) c.S->narrow[x] = -1;
     }
 }
 
 /* World w's part of xb_server_pre_tick. */
 static inline XB_HD void xsv_world(const XbPark& P, int64_t n, double* ypr, const xb_server_state& S,
-                                   const xb_server_queue& Q, int phase, int64_t w) {
-    const XsvCtx c{P, n, ypr, &S, &Q, w, S.W, S.N};
+                                   const xb_server_queue& Q, const xb_space_state* sp, int phase, int64_t w) {
+    const XsvCtx c{P, n, ypr, &S, &Q, w, S.W, S.N, XsCtx{sp, P, w, S.W, S.N}};
     if (phase == 2) {                          /* the test alone: the queued flags and rows, nothing applied */
-        xsv_queue(c, false);
+        xsv_queue(c, false, Q.J_user);
         return;
     }
     if (Q.on && !Q.on[w]) {
@@ -394,9 +685,15 @@ static inline XB_HD void xsv_world(const XbPark& P, int64_t n, double* ypr, cons
     }
     if (phase == 0) {
         xsv_pass1(c);
-        xsv_pass2(c);
+        if (sp)
+            xsv_pass2_space(c);
+        else
+            xsv_pass2(c);
     }
-    xsv_flags(c);
+    if (sp)
+        xsv_flags_space(c);
+    else
+        xsv_flags(c);
 }
 
 static inline bool xsv_args_ok(const xb_modes_state* park, const xb_server_state* s, const xb_server_queue* q,
@@ -407,17 +704,22 @@ static inline bool xsv_args_ok(const xb_modes_state* park, const xb_server_state
                                                !s->listed || !s->has_character || !s->gets_bubble || !s->wflags ||
                                                !s->history || !s->bubble_history)))
         return false;
-    if (q->J < 0 || q->J > 4096) return false;
+    if (q->J < 0 || q->J > 4096 || q->J_user < 0 || q->J_user > q->J) return false;
     if (q->J > 0 && (!q->kind || !q->mask || !q->ball || !q->target || !q->f || !q->n || !q->loc_in || !q->ok ||
                      !q->row || !q->trow || !q->loc || !q->route || !q->notice))
         return false;
+    if (!s->space) return q->J_user == q->J;
+    const xb_space_state* sp = s->space;          /* the partition: the same worlds and slots, its lists' tables */
+    if (sp->W != s->W || sp->N != s->N || s->N > 64 || !sp->rows || !sp->by_id || !sp->ids) return false;
+    if (s->W > 0 && (!s->ad_kind || !s->ad_addn || !s->ad_deln || !s->narrow)) return false;
+    if (q->J > 0 && !q->rbubble) return false;
     return true;
 }
 
 static inline void xsv_layout(int64_t* out) {
     out[0] = (int64_t)sizeof(xb_server_state);
-    out[1] = (int64_t)offsetof(xb_server_state, wflags);
+    out[1] = (int64_t)offsetof(xb_server_state, narrow);
     out[2] = (int64_t)sizeof(xb_server_queue);
-    out[3] = (int64_t)offsetof(xb_server_queue, rec_unc);
+    out[3] = (int64_t)offsetof(xb_server_queue, rbubble);
     out[4] = XB_SREC_WORDS;
 }

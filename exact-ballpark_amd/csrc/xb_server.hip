@@ -2,7 +2,8 @@
 //
 // The server layer's pre-tick on gfx950 (xb_server.h): one lane a world, a wavefront's 64 lanes 64 consecutive worlds,
 // so each table entry of a call (its mask, ball, arguments, queued flag, route) is one contiguous read or write across
-// the wavefront; the park's rows a world's calls touch are that world's own.
+// the wavefront; the park's rows a world's calls touch are that world's own, and so are its entries of the
+// partition's tables (the same world-interleaved layout) when the server runs over the partition.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -12,17 +13,19 @@
 #include "xb_modes_tick.h"
 #include "xb_maths_float.h"
 #include "xb_commands.h"
+#include "xb_rand.h"
 #include "../../include/xb.h"
 #include "xb_park.h"
+#include "xb_space.h"
 #include "xb_server.h"
 
 namespace {
 
 __global__ void __launch_bounds__(64) xb_server_kernel(xb_modes_state park, xb_server_state srv, xb_server_queue q,
-                                                       int phase) {
+                                                       xb_space_state sp, int phase) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= srv.W) return;
-    xsv_world(xb_park(park), park.n, park.ypr, srv, q, phase, w);
+    xsv_world(xb_park(park), park.n, park.ypr, srv, q, srv.space ? &sp : nullptr, phase, w);
 }
 
 }  // namespace
@@ -39,7 +42,9 @@ int xb_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, c
     if (!xsv_args_ok(park, srv, q, phase)) return -1;
     if (srv->W == 0) return 0;
     const unsigned grid = (unsigned)((srv->W + 63) / 64);
-    xb_server_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(*park, *srv, *q, phase);
+    xb_space_state sp = {};                       /* the partition's tables by value (srv->space is a host address) */
+    if (srv->space) sp = *srv->space;
+    xb_server_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(*park, *srv, *q, sp, phase);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : (int)e;
 }

@@ -39,7 +39,9 @@
 #define XS_OP_AFTER_EVOLVE 13           /* after the park's tick: the drop-outs' and commits' re-filings, proximities */
 #define XS_OP_CLEAR_EVENTS 14
 #define XS_OP_CLOAK_SENSOR 15           /* CloakBall's sensor on the master: mode b (a char), range f[0] */
-#define XS_OPS 16
+#define XS_OP_UNCLOAK 16                /* UncloakBall's part on the master, before the park's: the normal cloak's sensor
+                                           removed; unless in warp proper, slot a out of its bubble's sensor lists */
+#define XS_OPS 17
 
 #define XS_Q_VISIBILITY 1               /* CheckVisibility(slot a, slot b): out[0] the occluder's id, 0 or -2 */
 #define XS_Q_RANGE_BALL 2               /* GetBallIdsInRange(slot a, range f[3], cloaked b): out[0] n (-1: no such ball),
@@ -906,6 +908,25 @@ static inline XB_HD void xs_remove_ball(const XsCtx& c, int j) {        /* space
     XS_SLOT(c, sflags, j) &= (uint8_t)~XS_IN;
 }
 
+/* UncloakBall on the master (full.py:1760-1780), the layer's part: a normal cloak's proximity sensor removed
+   (RemoveProximitySensor); unless the ball is in warp proper, it leaves the sensor list of every live proximity ball of
+   its bubble. The park's part (the cloak byte, massive again) follows. */
+static inline XB_HD void xs_uncloak(const XsCtx& c, int j) {
+    const int64_t r = xs_row(c, j);
+    if (XB_Q(c.P, XB_I_CLOAKED, r) == 1) {
+        XB_B(c.P, XB_B_SENSOR, r) = 0;
+        xs_proximity_status(c, j);
+    }
+    if (XB_Q(c.P, XB_I_MODE, r) == XB_MODE_WARP && XB_Q(c.P, XB_I_EFFECT_STAMP, r) >= 0) return;
+    const int64_t bub = XS_SLOT(c, bubble, j);
+    const XsMap prox = xs_prox_map(c);
+    for (int i = 0; i < *prox.len; ++i) {
+        const int k = prox.keys[(int64_t)i * prox.stride];
+        if (XS_SLOT(c, bubble, k) != bub || xs_moribund(c, k) || !(XS_SLOT(c, sflags, k) & XS_PROX)) continue;
+        xs_map_erase(xs_sensor_map(c, k), j);
+    }
+}
+
 static inline XB_HD void xs_set_interactive(const XsCtx& c, int j, bool on) {
     const int64_t r = xs_row(c, j);
     const bool was = XB_B(c.P, XB_B_INTERACTIVE, r) != 0;
@@ -1030,6 +1051,9 @@ static inline XB_HD void xs_apply(const XsCtx& c, int64_t code, int64_t a, int64
         case XS_OP_CLOAK_SENSOR:
             if (slot && xs_in(c, j) && (int)(int8_t)(uint8_t)(b & 0xFF) == 1)
                 xs_add_sensor(c, j, f[0], XS_CLOAK_SENSOR_PERIOD, false, false);
+            break;
+        case XS_OP_UNCLOAK:
+            if (slot && xs_in(c, j)) xs_uncloak(c, j);
             break;
         default: xs_flag(c, XS_F_BAD_OP); break;
     }

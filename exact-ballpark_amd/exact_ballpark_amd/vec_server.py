@@ -78,14 +78,15 @@ class ServerState(ctypes.Structure):
         (n, ctypes.c_void_p) for n in ("rows", "pilot", "stamp_for_system", "stamp", "clock", "bubble_time",
                                        "settled", "fresh", "listed", "has_character", "gets_bubble", "history",
                                        "bubble_history", "wflags")] + [
-        ("master", ctypes.c_int32), ("snapshot", ctypes.c_int32)]
+        ("master", ctypes.c_int32), ("snapshot", ctypes.c_int32)] + [
+        (n, ctypes.c_void_p) for n in ("space", "ad_kind", "ad_addn", "ad_deln", "ad_add", "ad_del", "narrow")]
 
 
 class ServerQueue(ctypes.Structure):
     """include/xb.h's xb_server_queue."""
-    _fields_ = [("J", ctypes.c_int64)] + [
+    _fields_ = [("J", ctypes.c_int64), ("J_user", ctypes.c_int64)] + [
         (n, ctypes.c_void_p) for n in ("kind", "mask", "ball", "target", "f", "n", "loc_in", "on", "ok", "row",
-                                       "trow", "loc", "route", "notice", "rec_add", "rec_unc")]
+                                       "trow", "loc", "route", "notice", "rec_add", "rec_unc", "rbubble")]
 
 
 def _lib(dev: torch.device) -> ctypes.CDLL:
@@ -101,8 +102,8 @@ def _lib(dev: torch.device) -> ctypes.CDLL:
             layout = L.xb_cpu_server_layout
         out = (ctypes.c_int64 * 5)()
         layout(out)
-        want = (ctypes.sizeof(ServerState), ServerState.wflags.offset, ctypes.sizeof(ServerQueue),
-                ServerQueue.rec_unc.offset, SREC_WORDS)
+        want = (ctypes.sizeof(ServerState), ServerState.narrow.offset, ctypes.sizeof(ServerQueue),
+                ServerQueue.rbubble.offset, SREC_WORDS)
         if tuple(out) != want:
             raise N.NativeError(f"the server layer's layout {tuple(out)} is not the loader's {want}")
         L._server_bound = True
@@ -207,6 +208,7 @@ class Slot:
     n: torch.Tensor                     # [W] the int argument
     loc: torch.Tensor                   # [W] the local argument
     resolved: bool = False              # tested by an undo: `mask` is its queued flag
+    frozen: bool = False                # `loc` holds CloakBall's bubble as read at the call (over the partition)
     ok: torch.Tensor | None = None
     row: torch.Tensor | None = None
     trow: torch.Tensor | None = None
@@ -312,9 +314,10 @@ class VecServer:
     def _queue_struct(q: dict) -> ServerQueue:
         s = ServerQueue()
         s.J = q["J"]
+        s.J_user = q.get("J_user", q["J"])
         for name in ("kind", "mask", "ball", "target", "f", "n", "loc_in", "on", "ok", "row", "trow", "loc", "route",
-                     "notice", "rec_add", "rec_unc"):
-            setattr(s, name, _ptr(q[name]))
+                     "notice", "rec_add", "rec_unc", "rbubble"):
+            setattr(s, name, _ptr(q.get(name)))
         return s
 
     def _launch(self, q: dict, phase: int, snapshot: bool = False) -> None:

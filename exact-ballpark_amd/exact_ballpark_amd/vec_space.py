@@ -33,7 +33,7 @@ I64, F64, I32, I8, U8 = torch.int64, torch.float64, torch.int32, torch.int8, tor
 MAX_SLOTS = 64
 OPS = ("NONE", "ADD", "REFILE", "INIT_BUBBLES", "COPY_BUBBLES", "KEEP_ALIVE", "SET_INTERACTIVE", "ADD_SENSOR",
        "REMOVE_SENSOR", "NOTIFY_RANGE", "TRACK", "REMOVE_BALL", "DEAD", "AFTER_EVOLVE", "CLEAR_EVENTS",
-       "CLOAK_SENSOR")
+       "CLOAK_SENSOR", "UNCLOAK")
 OP = {name: k for k, name in enumerate(OPS)}
 QUERIES = ("NONE", "VISIBILITY", "RANGE_BALL", "RANGE_POINT", "BUBBLE_AT", "REMOTE_FOLLOWERS")
 Q = {name: k for k, name in enumerate(QUERIES)}
@@ -98,8 +98,9 @@ class VecSpace:
             raise ValueError("VecSpace runs over a VecModesPark (or its ModesView)")
         if lib is not None:
             raise ValueError("lib: the layer binds the park's own library")
-        if hits or server:
-            raise ValueError("the branch counters and the server's bubble lists are not kept by this layer")
+        if hits:
+            raise ValueError("the branch counters are not kept by this layer")
+        self.server = bool(server)          # the server's bubble update's lists are the server layer's own tables
         self.view = view if isinstance(view, ModesView) else ModesView(park)
         self.park = park
         self.dev = park.dev

@@ -306,6 +306,19 @@ typedef struct xb_server_state {
     int64_t* wflags;              /* [W] OR-ed: 1 a command the batched park refuses (a mushroom as a target) */
     int32_t master;               /* the server's view: a normal cloak adds the ball's proximity sensor */
     int32_t snapshot;             /* write the stream records (rec_add, rec_unc) the decode of the sends needs */
+    /* The master's partition (the partition server; NULL: one bubble a world): the space layer's state over the same
+       park and the same worlds (its `rows` the server's). With it the pre-tick also queues the follower clean-up's
+       Stops, runs each call's part in the partition (re-filings, the interactive sets, the cloak's and uncloak's
+       sensors), AdditionsAndDeletions across old and new bubbles in place of the one-bubble additions, and routes
+       every event to its ball's bubble; the flags give each pilot the narrowcast of its ship's bubble. */
+    const xb_space_state* space;
+    int8_t* ad_kind;              /* [n] the bubble update's lists per row: 0 none, 1 its bubble's, 2 its own in a
+                                     bubble of no pre-built list, 3 its own across its old and new bubbles */
+    int32_t* ad_addn;             /* [n] */
+    int32_t* ad_deln;             /* [n] */
+    int8_t* ad_add;               /* [N][n] the additions as world slots, in the dictionaries' order (NULL: counts) */
+    int8_t* ad_del;               /* [N][n] the deletions */
+    int64_t* narrow;              /* [n] written: the bubble whose narrowcast the row's pilot gets, -1 none */
 } xb_server_state;
 
 /* One tick's queue: J calls, each for every world ([J][W], call j of world w at [j * W + w]; f [J][W][4]). kind[j] is
@@ -314,9 +327,11 @@ typedef struct xb_server_state {
    (vec_server_cuda.cu:70-80). */
 typedef struct xb_server_queue {
     int64_t J;
+    int64_t J_user;               /* with a partition, the pushed calls: slots J_user .. J - 1 (kind Stop) take the
+                                     follower clean-up's Stops, which the kernel writes (ball, mask); else J */
     const int8_t* kind;           /* [J] */
-    const uint8_t* mask;          /* [J][W] */
-    const int64_t* ball;          /* [J][W] Destiny ids */
+    uint8_t* mask;                /* [J][W] */
+    int64_t* ball;                /* [J][W] Destiny ids */
     const int64_t* target;        /* [J][W] FollowBall's and Orbit's target ids */
     const double* f;              /* [J][W][4] the float arguments */
     const int64_t* n;             /* [J][W] the int argument */
@@ -330,12 +345,13 @@ typedef struct xb_server_queue {
     uint8_t* notice;              /* [J][n] written for UncloakBall calls: the rows whose pilot gets AddBallsToPark */
     int64_t* rec_add;             /* [N][W][XB_SREC_WORDS] with snapshot: a world's balls at its bubble update */
     int64_t* rec_unc;             /* [J][W][XB_SREC_WORDS] with snapshot: an uncloaked ball as its UncloakBall left it */
+    int64_t* rbubble;             /* [J][W] written with a partition: the bubble a call routed to the bubble goes to */
 } xb_server_queue;
 
 #define XB_SREC_WORDS 34          /* a stream record (exact-ballpark_amd/csrc/xb_server.h numbers its words) */
 
-/* The layout as built, for the loader's check: out[0] = sizeof(xb_server_state), out[1] its `wflags` offset, out[2] =
-   sizeof(xb_server_queue), out[3] its `rec_unc` offset, out[4] = XB_SREC_WORDS. Plays the part of the reference's ABI
+/* The layout as built, for the loader's check: out[0] = sizeof(xb_server_state), out[1] its `narrow` offset, out[2] =
+   sizeof(xb_server_queue), out[3] its `rbubble` offset, out[4] = XB_SREC_WORDS. Plays the part of the reference's ABI
    check at load (src/exact_ballpark/vec_server_cuda.py's xbpv_abi). */
 int xb_server_layout(int64_t* out);
 
@@ -345,8 +361,13 @@ int xb_server_layout(int64_t* out);
    call's route; UncloakBall's notices) and the send flags; phase 1 the send flags alone (after the park's tick, which
    can change a ball's interactivity); phase 2 the queue's test alone (each call's queued flag, rows and local
    argument, as phase 0 would find them; nothing else written: the pending queries' view of the queue). Worlds whose
-   `on` byte is clear are not touched and their flags are cleared (phases 0 and 1).
-   Replaces xbpv_pre_tick (src/exact_ballpark/vec_server_cuda.cu:615). */
+   `on` byte is clear are not touched and their flags are cleared (phases 0 and 1). With the partition (srv->space),
+   phase 0 also writes the follower clean-up's Stops into slots J_user .. J - 1, applies each call's part in the
+   partition, runs AdditionsAndDeletions across old and new bubbles (ad_*), writes each BUBBLE route's bubble (rbubble)
+   and routes by bubble; the flags give each pilot its ship's bubble's narrowcast (narrow). The partition's tables are
+   read and written as xb_space_tape's are, on the same stream.
+   Replaces xbpv_pre_tick (src/exact_ballpark/vec_server_cuda.cu:615) and xbpq_pre_tick
+   (vec_server_space_cuda.cu:399). */
 int xb_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, const xb_server_queue* q, int phase,
                        void* stream);
 

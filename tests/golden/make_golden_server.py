@@ -1,9 +1,11 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/server_golden.npz (with `--do`, server_do_golden.npz): the reference's own server layer
+"""Writes tests/golden/server_golden.npz (with `--do`, server_do_golden.npz; with `--space`, server_space_golden.npz
+over worlds of several bubbles): the reference's own server layer
 (src/exact_ballpark/server.py: Destiny's Actions, Ticker, BubbleUpdater and ParkUpdateBatcher on a master
 DestinyPark over its FullBallpark, with src/exact_ballpark/client.py's PilotInterests and RecordingNetwork), driven as
 Destiny's own test helper drives it: each tick the world's calls in order, the Ticker's tick, the park's Evolve, the
-batcher's send_batch. The worlds are tests/server_scenarios.py's `server_cases`. Recorded, as bits: every ball's
+batcher's send_batch. The worlds are tests/server_scenarios.py's `server_cases` (`space_server_cases`). Recorded, as
+bits: every ball's
 state after every tick (the flight fields, the display yaw, pitch and roll or with `--do` the turning state, the
 filing, the flags, the cloak byte, the bubble and old bubble), every update each client received at each tick (its
 kind, method, state, wait flag and count, the ball streams as bytes), and each pending query's answer.
@@ -50,7 +52,7 @@ def _per_client(sent: list) -> dict:
     return {str(c): v for c, v in sorted(out.items())}
 
 
-def run_world(w: dict) -> tuple:
+def run_world(w: dict, space: bool = False) -> tuple:
     do = bool(w["do"])
     if do:
         srv.enable_dynamical_orientation()
@@ -83,7 +85,7 @@ def run_world(w: dict) -> tuple:
         park.Evolve()
         batcher.send_batch()
         sends.append(SS.canon(_per_client(network.take())))
-        if len({b.bubble for b in lane.balls.values()}) != 1:
+        if not space and len({b.bubble for b in lane.balls.values()}) != 1:
             raise ValueError("a ball left the world's bubble")
         for k, bid in enumerate(order):
             d, q = _state(lane.balls[bid], do)
@@ -94,15 +96,16 @@ def run_world(w: dict) -> tuple:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--do", action="store_true")
+    ap.add_argument("--space", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--worlds", type=int, default=90)
     a = ap.parse_args()
-    ws = SS.server_cases(a.seed, a.worlds, do=a.do)
+    ws = SS.space_server_cases(a.seed, a.worlds) if a.space else SS.server_cases(a.seed, a.worlds, do=a.do)
     kept, skipped = [], []
     rw, rt, rb, rd, rq, sends, answers = [], [], [], [], [], [], []
     for wi, w in enumerate(ws):
         try:
-            states, snd, ans = run_world(w)
+            states, snd, ans = run_world(w, a.space)
         except (ZeroDivisionError, ValueError, OverflowError, C.space.DestinyUndefined, NotImplementedError) as e:
             skipped.append([wi, type(e).__name__, str(e)[:80]])
             continue
@@ -111,7 +114,7 @@ def main() -> None:
             rw.append(wi), rt.append(t), rb.append(k), rd.append(d), rq.append(q)
         sends.append(snd)
         answers.append(ans)
-    name = "server_do_golden.npz" if a.do else "server_golden.npz"
+    name = "server_space_golden.npz" if a.space else "server_do_golden.npz" if a.do else "server_golden.npz"
     np.savez_compressed(HERE / name, world=np.array(rw, dtype=np.int32), step=np.array(rt, dtype=np.int32),
                         ball=np.array(rb, dtype=np.int32), d=np.array(rd, dtype=np.float64).view(np.int64),
                         q=np.array(rq, dtype=np.int64), kept=np.array(kept, dtype=np.int32), seed=np.int64(a.seed),

@@ -18,6 +18,7 @@ import torch
 
 import server_scenarios as SS
 from exact_ballpark_amd import vec_server as VS
+from exact_ballpark_amd import vec_server_space as VSS
 from exact_ballpark_amd.vec_modes import VecModesPark
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -26,10 +27,12 @@ F64, I64 = torch.float64, torch.int64
 # gradient's), then the display orientation or the turning state; of its ints: id .. sensor, interactive, free, cloak
 D_FLIGHT = list(range(26))
 Q_COLS = [0, 1, 2, 3, 4, 5, 6, 7, 15, 16, 17]
+Q_BUBBLES = [18, 19]                # the bubble and old bubble (compared over the partition)
 
 
-def load_golden(do: bool) -> dict:
-    z = np.load(GOLDEN / ("server_do_golden.npz" if do else "server_golden.npz"), allow_pickle=False)
+def load_golden(do: bool, space: bool = False) -> dict:
+    name = "server_space_golden.npz" if space else "server_do_golden.npz" if do else "server_golden.npz"
+    z = np.load(GOLDEN / name, allow_pickle=False)
     worlds = json.loads(str(z["worlds"]))
     kept = [int(k) for k in z["kept"]]
     d, q = z["d"].view(np.float64), z["q"]
@@ -41,9 +44,10 @@ def load_golden(do: bool) -> dict:
             "answers": json.loads(str(z["answers"]))}
 
 
-def build(worlds: list, device: str) -> tuple:
+def build(worlds: list, device: str, space: bool = False) -> tuple:
     """(VecServer, rows [world][ball index]) for the worlds on one VecModesPark (master, display orientation without
-    dynamical orientation), each world's balls in AddBall's order."""
+    dynamical orientation), each world's balls in AddBall's order; with `space` a SpaceServer over the master's
+    partition."""
     do = bool(worlds[0]["do"])
     rows, index = [], []
     for wi, w in enumerate(worlds):
@@ -53,7 +57,8 @@ def build(worlds: list, device: str) -> tuple:
             rows.append({"id": b["id"], "world": wi, "time": 0, "mass": b["mass"], "radius": b["radius"],
                          "max_vel": b["max_vel"], "pos": b["pos"], "vel": b["vel"], "agility": b["agility"],
                          "speed_fraction": b["sf"], "free": True, "massive": b["massive"],
-                         "interactive": b["interactive"], "cloaked": 0, "bubble": 0, "moribund": False,
+                         "interactive": b["interactive"], "cloaked": 0, "bubble": -1 if space else 0,
+                         "moribund": False,
                          "max_ang_vel": b["max_ang_vel"], "rot_agility": b["rot_agility"]})
     park = VecModesPark(len(rows), device=device, dynamical_orientation=do, display_ypr=not do, master=True)
     park.load(rows)
@@ -63,6 +68,8 @@ def build(worlds: list, device: str) -> tuple:
             p = w["pilots"].get(str(b["id"]), w["pilots"].get(b["id"]))
             if p is not None:
                 pilots[index[wi][k]] = int(p)
+    if space:
+        return VSS.SpaceServer(VSS.SpaceModesAdapter(park), pilots), index
     return VS.VecServer(VS.ModesAdapter(park), pilots), index
 
 
@@ -150,7 +157,8 @@ def apply_tick(server: VS.VecServer, worlds: list, t: int) -> list:
 
 def sends_of(server: VS.VecServer) -> list:
     """Each world's sends of the last tick, canonical as the fixture holds them."""
-    return [SS.canon({str(c): v for c, v in sorted(per.items())}) for per in VS.decode(server)]
+    dec = VSS.decode if isinstance(server, VSS.SpaceServer) else VS.decode
+    return [SS.canon({str(c): v for c, v in sorted(per.items())}) for per in dec(server)]
 
 
 def park_rows(server: VS.VecServer, worlds: list, index: list) -> dict:
@@ -171,9 +179,12 @@ def park_rows(server: VS.VecServer, worlds: list, index: list) -> dict:
     ids = p.ids.cpu()
     fp = p.follow_ptr.cpu()
     fid = torch.where(fp >= 0, ids[fp.clamp(min=0)], torch.full_like(fp, -1))
-    qq = torch.stack([ids, p.mode.cpu(), p.effect_stamp.cpu(), p.follow_id.cpu(), p.owner_id.cpu(), fid,
-                      p.massive.cpu().to(I64), p.sensor.cpu().to(I64), p.interactive.cpu().to(I64),
-                      p.free.cpu().to(I64), p.cloaked.cpu()], dim=1).numpy()
+    cols_q = [ids, p.mode.cpu(), p.effect_stamp.cpu(), p.follow_id.cpu(), p.owner_id.cpu(), fid,
+              p.massive.cpu().to(I64), p.sensor.cpu().to(I64), p.interactive.cpu().to(I64), p.free.cpu().to(I64),
+              p.cloaked.cpu()]
+    if isinstance(server, VSS.SpaceServer):
+        cols_q += [server.bubbles().cpu(), server.old_bubbles().cpu()]
+    qq = torch.stack(cols_q, dim=1).numpy()
     return {(wi, k): (dd[r], qq[r]) for wi, rs in enumerate(index) for k, r in enumerate(rs)}
 
 
@@ -181,10 +192,11 @@ def _bits(x: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(x, dtype=np.float64).view(np.int64)
 
 
-def compare_golden(do: bool, device: str, limit: int | None = None, reverse: bool = False) -> dict:
+def compare_golden(do: bool, device: str, limit: int | None = None, reverse: bool = False,
+                   space: bool = False) -> dict:
     """The batched layer on the fixture's kept worlds (the first `limit`; in reverse order with `reverse`) against the
-    fixture: rows, sends and answers that differ."""
-    g = load_golden(do)
+    fixture: rows, sends and answers that differ (`space`: the partition server on server_space_golden.npz)."""
+    g = load_golden(do, space)
     pos = list(range(len(g["kept"])))[:limit] if limit else list(range(len(g["kept"])))
     if reverse:
         pos = pos[::-1]
@@ -192,8 +204,9 @@ def compare_golden(do: bool, device: str, limit: int | None = None, reverse: boo
     worlds = [g["worlds"][k] for k in kept]
     for w in worlds:
         w["ticks"] = [[(e[0], e[1], tuple(e[2])) for e in tick] for tick in w["ticks"]]
-    server, index = build(worlds, device)
+    server, index = build(worlds, device, space)
     bad_rows, bad_sends, bad_answers, compared = [], [], [], 0
+    qcols = Q_COLS + (Q_BUBBLES if space else [])
     T = len(worlds[0]["ticks"])
     extra = list(range(29, 29 + (20 if do else 9)))
     for t in range(T):
@@ -212,7 +225,7 @@ def compare_golden(do: bool, device: str, limit: int | None = None, reverse: boo
                 wd, wq = g["states"][(k, t + 1)][b]
                 gd, gq = got[(i, b)]
                 want_d = np.concatenate([wd[D_FLIGHT], wd[extra]])
-                want_q = wq[Q_COLS]
+                want_q = wq[qcols]
                 compared += 1
                 if not (np.array_equal(_bits(want_d), _bits(gd)) and np.array_equal(want_q, gq)):
                     cols = [c for c in range(len(want_d)) if _bits(want_d[c:c + 1])[0] != _bits(gd[c:c + 1])[0]]
