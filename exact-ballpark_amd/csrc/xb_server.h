@@ -355,6 +355,17 @@ static inline XB_HD void xsv_clean_up(const XsvCtx& c) {
     for (; m < Q.J; ++m) Q.mask[m * c.W + c.w] = 0, Q.ball[m * c.W + c.w] = 0;
 }
 
+/* slot j appended to row x's list (a table of [N][n] slots and its count); a list never holds more than the world's N
+   slots (each dictionary lists a slot once, and a slot is in the new bubble's entries or the old's), checked anyway */
+static inline XB_HD void xsv_list_put(const XsvCtx& c, int8_t* tab, int32_t* cnt, int64_t x, int j) {
+    if (cnt[x] >= c.N) {
+        xs_flag(c.sp, XS_F_ENTRIES);
+        return;
+    }
+    if (tab) tab[(int64_t)cnt[x] * c.n + x] = (int8_t)j;
+    cnt[x]++;
+}
+
 /* Ballpark::PopulateAddDelForBubble (server.py:1775-1793): the bubble's dictionary in order, its additions (+1) and
    removals (-1); no cloaked addition, no removal of a cloaked ball still in the park. */
 static inline XB_HD void xsv_populate(const XsvCtx& c, int64_t bub, int64_t x) {
@@ -367,13 +378,10 @@ static inline XB_HD void xsv_populate(const XsvCtx& c, int64_t bub, int64_t x) {
         const bool here = xs_in(c.sp, j);
         const bool cl = here && XB_Q(c.P, XB_I_CLOAKED, xs_row(c.sp, j)) != 0;
         if (act != -1 ? (!here || cl) : cl) continue;
-        if (act == 1) {
-            if (S.ad_add) S.ad_add[(int64_t)S.ad_addn[x] * c.n + x] = (int8_t)j;
-            S.ad_addn[x]++;
-        } else if (act == -1) {
-            if (S.ad_del) S.ad_del[(int64_t)S.ad_deln[x] * c.n + x] = (int8_t)j;
-            S.ad_deln[x]++;
-        }
+        if (act == 1)
+            xsv_list_put(c, S.ad_add, S.ad_addn, x, j);
+        else if (act == -1)
+            xsv_list_put(c, S.ad_del, S.ad_deln, x, j);
     }
 }
 
@@ -422,13 +430,10 @@ static inline XB_HD void xsv_additions_space(const XsvCtx& c) {
                         const bool is_in = dict_b && xs_entry_find(c.sp, nb, j) >= 0;
                         del = !is_in && act <= 0;
                     }
-                    if (add) {
-                        if (S.ad_add) S.ad_add[(int64_t)S.ad_addn[x] * c.n + x] = (int8_t)j;
-                        S.ad_addn[x]++;
-                    } else if (del) {
-                        if (S.ad_del) S.ad_del[(int64_t)S.ad_deln[x] * c.n + x] = (int8_t)j;
-                        S.ad_deln[x]++;
-                    }
+                    if (add)
+                        xsv_list_put(c, S.ad_add, S.ad_addn, x, j);
+                    else if (del)
+                        xsv_list_put(c, S.ad_del, S.ad_deln, x, j);
                 }
             }
         }

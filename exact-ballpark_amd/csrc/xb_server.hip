@@ -21,11 +21,20 @@
 
 namespace {
 
+/* worlds of one bubble: the partition's code folds away (its context null), so this kernel keeps its own registers */
 __global__ void __launch_bounds__(64) xb_server_kernel(xb_modes_state park, xb_server_state srv, xb_server_queue q,
-                                                       xb_space_state sp, int phase) {
+                                                       int phase) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= srv.W) return;
-    xsv_world(xb_park(park), park.n, park.ypr, srv, q, srv.space ? &sp : nullptr, phase, w);
+    xsv_world(xb_park(park), park.n, park.ypr, srv, q, nullptr, phase, w);
+}
+
+/* over the partition: its tables by value beside the server's */
+__global__ void __launch_bounds__(64) xb_server_space_kernel(xb_modes_state park, xb_server_state srv,
+                                                             xb_server_queue q, xb_space_state sp, int phase) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= srv.W) return;
+    xsv_world(xb_park(park), park.n, park.ypr, srv, q, &sp, phase, w);
 }
 
 }  // namespace
@@ -42,9 +51,10 @@ int xb_server_pre_tick(const xb_modes_state* park, const xb_server_state* srv, c
     if (!xsv_args_ok(park, srv, q, phase)) return -1;
     if (srv->W == 0) return 0;
     const unsigned grid = (unsigned)((srv->W + 63) / 64);
-    xb_space_state sp = {};                       /* the partition's tables by value (srv->space is a host address) */
-    if (srv->space) sp = *srv->space;
-    xb_server_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(*park, *srv, *q, sp, phase);
+    if (srv->space)                               /* the partition's tables by value (srv->space is a host address) */
+        xb_server_space_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(*park, *srv, *q, *srv->space, phase);
+    else
+        xb_server_kernel<<<grid, 64, 0, (hipStream_t)stream>>>(*park, *srv, *q, phase);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : (int)e;
 }

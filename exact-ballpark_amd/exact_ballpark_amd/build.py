@@ -77,15 +77,28 @@ def check_isa(src: Path) -> dict:
     return {"kernels": asm.count(".amdhsa_kernel ")}
 
 
+def _jobs() -> int:
+    return max(1, min(8, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+
+
 def build_hip(check: bool = True) -> Path:
+    """Each source compiled to its own object (host and gfx950 code) and the checks' IR and ISA, all in parallel, then
+    one link."""
+    from concurrent.futures import ThreadPoolExecutor
     LIB.mkdir(exist_ok=True)
     out = LIB / "libxb_hip.so"
-    srcs = [str(CSRC / s) for s in HIP_SOURCES]
-    _run(["hipcc", f"--offload-arch={ARCH}", *HIP_FLAGS, "-shared", *srcs, "-o", str(out)], cwd=CSRC)
+    objs = [LIB / (Path(s).stem + ".o") for s in HIP_SOURCES]
+    steps = [lambda s=s, o=o: _run(["hipcc", f"--offload-arch={ARCH}", *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(o)],
+                                   cwd=CSRC) for s, o in zip(HIP_SOURCES, objs)]
     if check:
-        for s in HIP_SOURCES:
-            check_ir(CSRC / s)
-            check_isa(CSRC / s)
+        steps += [lambda s=s: check_ir(CSRC / s) for s in HIP_SOURCES]
+        steps += [lambda s=s: check_isa(CSRC / s) for s in HIP_SOURCES]
+    with ThreadPoolExecutor(_jobs()) as ex:
+        for f in [ex.submit(st) for st in steps]:
+            f.result()
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", *map(str, objs), "-o", str(out)], cwd=CSRC)
+    for o in objs:
+        o.unlink()
     return out
 
 
