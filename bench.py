@@ -85,6 +85,16 @@ WORKLOADS = {
               "piloted, each tick GotoDirection (a random unit vector) and SetSpeedFraction (uniform 0-1) queued for "
               "every ship, then the pre-tick (duplicate test, calls applied in queue order, bubble update, routes), "
               "the park's tick and the send flags",
+    "space_server": "partition server (the server layer over the master's space layer): the partition park's {worlds} "
+                    "worlds of 4-64 ships in 1-16 clusters (their bubbles), sensors and tick-1 modes, every "
+                    "interactive ship piloted; each tick GotoDirection and SetSpeedFraction queued for ships 1-4 of "
+                    "every world, then the pre-tick (follower clean-up, calls applied with their partition parts, "
+                    "AdditionsAndDeletions across bubbles, routes per bubble), the partition's tick and the send flags",
+    "space_probe": "partition probe (the reference's published space shape): {worlds} worlds x 4 ships around a top "
+                   "box's face (one bubble a world), each ship interactive with a 5 km proximity sensor of a 1 s "
+                   "period, flying along x under GotoDirection (re-filed every tick); the partition's tick as a "
+                   "SpaceGraph replay, and the server over it (every ship piloted, GotoDirection and SetSpeedFraction "
+                   "queued for each ship each tick) as a SpaceServerGraph replay",
     "statics": "turning and collisions park with static collidables: {worlds} worlds x (4 free massive ships in a 2 km "
                "cluster + 4 fixed massive spheres of radius 100-5000 m) and 8 static collidables a world (capsules of "
                "length 200-3000 m and radius 30-400 m, oriented boxes of half-extents 30-800 m, in turn, randomly "
@@ -203,6 +213,14 @@ def main() -> int:
                     help="leave out the server layer's line")
     ap.add_argument("--server-only", action="store_true",
                     help="measure the server layer's line alone (its JSON the last line; for profiling)")
+    ap.add_argument("--no-space-server", dest="space_server", action="store_false",
+                    help="leave out the partition server's line")
+    ap.add_argument("--space-server-only", action="store_true",
+                    help="measure the partition server's line alone (its JSON the last line; for profiling)")
+    ap.add_argument("--no-space-probe", dest="space_probe", action="store_false",
+                    help="leave out the partition probe's line (the reference's published space shape)")
+    ap.add_argument("--space-probe-only", action="store_true",
+                    help="measure the partition probe's line alone (its JSON the last line; for profiling)")
     ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -228,6 +246,12 @@ def main() -> int:
         return 0
     if a.server_only:
         print(json.dumps(server_park(a)), flush=True)
+        return 0
+    if a.space_server_only:
+        print(json.dumps(space_server_park(a)), flush=True)
+        return 0
+    if a.space_probe_only:
+        print(json.dumps(space_probe(a)), flush=True)
         return 0
 
     def build(kind: str):
@@ -391,6 +415,10 @@ def main() -> int:
         out["partition_park"] = partition(a)
     if not a.only and a.server:
         out["server_park"] = server_park(a)
+    if not a.only and a.space_server:
+        out["space_server_park"] = space_server_park(a)
+    if not a.only and a.space_probe:
+        out["space_probe_park"] = space_probe(a)
     if a.episodes:
         out["serving"], srv = serving(a)
     if not a.no_cpu:
@@ -639,6 +667,257 @@ def server_park(a) -> dict:
                            "kind": "port", "sample": f"the first {m} worlds x {done} ticks on the host build of the "
                                                      f"same kernel source (eager pushes, OpenMP over worlds)"}
     res["exact"] = {"worlds_checked": m, "ticks": done, "bit_exact": bool(same)}
+    return res
+
+
+def space_server_park(a) -> dict:
+    """The partition server's line (WORKLOADS["space_server"]): each tick 8 calls a world queued through a
+    SpaceServerGraph's inputs, then one replay (the pre-tick kernel over the partition, the partition's tick: dead
+    bubbles, the park's kernel, the re-filings and HandleProximities; the flags kernel). The same calls run eagerly on
+    the host build for a sample of the worlds (a park and partition of their own), whose final park state and
+    partition tables are compared bit for bit."""
+    import numpy as np
+    import torch
+
+    from exact_ballpark_amd import vec_server_space as VSS
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    from exact_ballpark_amd.vec_space import VecSpace
+
+    W = a.worlds
+    room = 64 * 65
+
+    def make(wl, device):
+        park = VecModesPark(wl["n"], device=device, block=256, display_ypr=True, master=True)
+        space = WL.load_partition_park(park, wl, lambda *x, **k: VecSpace(*x, events=room, **k))
+        inter = wl["rows"]["interactive"]
+        pilots = np.where(inter, wl["rows"]["id"], -1)
+        srv = VSS.SpaceServer(VSS.SpaceModesAdapter(park), pilots.tolist(), space=space)
+        return park, space, srv
+
+    wl = WL.partition_park(1, W)
+    park, space, srv = make(wl, "cuda")
+    schedule = [k for _ in range(4) for k in ("GotoDirection", "SetSpeedFraction")]
+    g = VSS.SpaceServerGraph(srv, schedule)
+    gen = torch.Generator(device="cpu").manual_seed(12)
+    ticks_inputs = []
+    for t in range(4):
+        ins = []
+        for k in range(4):
+            d = (torch.rand(W, 3, generator=gen, dtype=torch.float64) * 2 - 1)
+            sfr = torch.rand(W, 1, generator=gen, dtype=torch.float64)
+            ins.append((d, sfr))
+        ticks_inputs.append(ins)
+    dev_inputs = [[(d.cuda(), s.cuda()) for d, s in ins] for ins in ticks_inputs]
+    for j in range(8):
+        g.inputs[j]["mask"].fill_(1)
+        g.inputs[j]["ball"].fill_(j // 2 + 1)
+    done = 0
+
+    def tick():
+        nonlocal done
+        for k, (d, sfr) in enumerate(dev_inputs[done % 4]):
+            g.inputs[2 * k]["f"][:, :3].copy_(d)
+            g.inputs[2 * k + 1]["f"][:, :1].copy_(sfr)
+        g.tick()
+        space.clear_events()
+        done += 1
+
+    for _ in range(max(4, a.warmup // 4)):
+        tick()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    K = max(16, a.steps // 4)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        tick()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    lat = []
+    for _ in range(50):
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        tick()
+        s1.record(stream)
+        lat.append((s0, s1))
+    torch.cuda.synchronize()
+    lat = np.array([x.elapsed_time(y) for x, y in lat])
+    srv.check()
+    res = {"workload": WORKLOADS["space_server"].format(worlds=W), "worlds": W, "ships": wl["n"],
+           "world_ticks_per_s": W / (ms / 1000.0), "ship_ticks_per_s": wl["n"] / (ms / 1000.0), "ms_per_tick": ms,
+           "ticks_timed": K, "calls_per_tick": 8 * W, "cleanup_stops": srv.cleanup_stops,
+           "tick_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}}
+    print(f"# space server: {res}", file=sys.stderr, flush=True)
+    if a.no_cpu:
+        return res
+    m = min(W, 512)                            # the host build's eager ticks on the first m worlds
+    cwl = WL.partition_park(1, W)
+    sub = WL.take_worlds(cwl, m)
+    cp, cspace, cs = make(sub, "cpu")
+    t0 = time.perf_counter()
+    for t in range(done):
+        for k, (d, sfr) in enumerate(ticks_inputs[t % 4]):
+            mask = torch.ones(m, dtype=torch.bool)
+            ball = torch.full((m,), k + 1, dtype=torch.int64)
+            cs.push("GotoDirection", mask, ball, f=d[:m])
+            cs.push("SetSpeedFraction", mask, ball, f=sfr[:m])
+        cs.pre_tick(snapshot=False)
+        cs.evolve()
+        cs.post_tick()
+        cspace.clear_events()
+    dt = time.perf_counter() - t0
+    n = sub["n"]
+    same = all(torch.equal(x[:, :n], y[:, :n].cpu()) for x, y in zip(cp._blocks(), park._blocks()))
+    tables = cspace.N == space.N                # (the sample's widest world sets its tables' rows)
+    if tables:
+        same = same and all(torch.equal(v, space.st[k][..., :m].cpu()) for k, v in cspace.st.items()
+                            if k not in ("rows", "by_id"))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"value": m * done / dt, "unit": "world-ticks/s", "cores": cores,
+                           "kind": "port", "sample": f"the first {m} worlds x {done} ticks on the host build of the "
+                                                     f"same kernel source (eager pushes, OpenMP over worlds)"}
+    res["exact"] = {"worlds_checked": m, "ticks": done, "bit_exact": bool(same), "partition_tables_compared": tables}
+    return res
+
+
+def _probe_rows(W: int) -> list:
+    """The partition probe's ships: per world 4 around the face between two top boxes (the first deep in the left
+    top, so its bubble comes first and the right top takes its id), 8 km apart at most, interactive."""
+    import numpy as np
+    g = np.random.default_rng(1)
+    top = 65536.0 * 480.0 * 0.25
+    off = (g.random((W, 4, 3)) - 0.5) * 8000.0
+    rows = []
+    for w in range(W):
+        for k in range(4):
+            pos = [top + off[w, k, 0], 0.3 * top + off[w, k, 1], -0.3 * top + off[w, k, 2]]
+            if k == 0:
+                pos[0] -= 0.5 * top
+            rows.append({"id": 101 + k, "world": w, "time": 0, "mass": 1e7, "radius": 50.0, "max_vel": 300.0,
+                         "pos": [float(x) for x in pos], "vel": [0.0, 0.0, 0.0], "agility": 0.5,
+                         "speed_fraction": 1.0, "free": True, "massive": False, "interactive": True, "cloaked": 0,
+                         "bubble": -1, "moribund": False})
+    return rows
+
+
+def space_probe(a) -> dict:
+    """The partition probe's line (WORKLOADS["space_probe"]): the shape of the reference's published space and
+    space-server figures (65,536 worlds x 4 ships), the partition's tick as SpaceGraph replays and the server over it
+    as SpaceServerGraph replays; each checked bit for bit against the host build on the first worlds."""
+    import torch
+
+    from exact_ballpark_amd import vec_server_space as VSS
+    from exact_ballpark_amd.vec_modes import VecModesPark
+    from exact_ballpark_amd.vec_space import SpaceGraph, VecSpace
+
+    W = a.worlds
+    rows = _probe_rows(W)
+
+    def make(W_, device, server):
+        p = VecModesPark(4 * W_, device=device, block=256, display_ypr=True, master=True)
+        p.load(rows[:4 * W_])
+        sp = VecSpace(p, torch.arange(4 * W_).reshape(W_, 4))
+        sp.load()
+        d = torch.tensor([[1.0, 0.2, 0.1] if k % 2 else [-1.0, -0.1, 0.2] for _ in range(W_) for k in range(4)],
+                         dtype=torch.float64, device=p.dev)
+        p.goto_direction(torch.ones(p.n, dtype=torch.bool, device=p.dev), d)
+        code = torch.full((W_, 4), 7, dtype=torch.int64, device=p.dev)           # ADD_SENSOR: 5 km, 1 s
+        slots = torch.arange(4, dtype=torch.int64, device=p.dev).expand(W_, 4).contiguous()
+        f = torch.zeros(W_, 4, 4, dtype=torch.float64, device=p.dev)
+        f[:, :, 0], f[:, :, 1] = 5000.0, 1.0
+        sp.tape(code, slots, torch.zeros_like(code), f)
+        srv = VSS.SpaceServer(VSS.SpaceModesAdapter(p), [k % 4 for k in range(p.n)], space=sp) if server else None
+        return p, sp, srv
+
+    def timed(tick, K):
+        for _ in range(max(4, a.warmup // 4)):
+            tick()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(K):
+            tick()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / K
+
+    K = max(16, a.steps // 4)
+    res = {"workload": WORKLOADS["space_probe"].format(worlds=W), "worlds": W, "ships": 4 * W}
+    # the partition alone
+    p, sp, _ = make(W, "cuda", False)
+    g = SpaceGraph(sp)
+    n_part = [0]
+
+    def part_tick():
+        g.tick()
+        sp.clear_events()
+        n_part[0] += 1
+
+    ms = timed(part_tick, K)
+    res["partition"] = {"world_ticks_per_s": W / (ms / 1000.0), "ms_per_tick": ms, "ticks_timed": K,
+                        "reference_published": {"world_ticks_per_s": 80.4e6, "source": "docs/benchmarks.md:271 "
+                                                "(the reference's SpaceGraph, 65,536 x 4, GH200)"}}
+    sp.check()
+    keep = (p, sp)
+    # the server over it
+    p2, sp2, srv = make(W, "cuda", True)
+    schedule = [k for _ in range(4) for k in ("GotoDirection", "SetSpeedFraction")]
+    gs = VSS.SpaceServerGraph(srv, schedule)
+    gen = torch.Generator(device="cpu").manual_seed(2)
+    d = (torch.rand(W, 4, 3, generator=gen) - 0.5).to(torch.float64)
+    sfr = torch.rand(W, 4, generator=gen).to(torch.float64)
+    for k in range(4):
+        for j, val in ((2 * k, d[:, k]), (2 * k + 1, sfr[:, k:k + 1])):
+            gs.inputs[j]["mask"].fill_(1)
+            gs.inputs[j]["ball"].fill_(101 + k)
+            gs.inputs[j]["f"][:, :val.shape[1]].copy_(val.cuda())
+    n_srv = [0]
+
+    def srv_tick():
+        gs.tick()
+        sp2.clear_events()
+        n_srv[0] += 1
+
+    ms2 = timed(srv_tick, K)
+    srv.check()
+    res["server"] = {"world_ticks_per_s": W / (ms2 / 1000.0), "ms_per_tick": ms2, "ticks_timed": K,
+                     "calls_per_tick": 8 * W,
+                     "reference_published": {"world_ticks_per_s": 68.7e6, "source": "docs/benchmarks.md:311 "
+                                             "(the reference's SpaceServer as a CUDA graph, 65,536 x 4, GH200)"}}
+    print(f"# space probe: {res}", file=sys.stderr, flush=True)
+    if a.no_cpu:
+        return res
+    m = min(W, 4096)                          # the host build on the first m worlds, the same ticks
+    cp, csp, _ = make(m, "cpu", False)
+    t0 = time.perf_counter()
+    for _ in range(n_part[0]):
+        csp.tick(cp)
+        csp.clear_events()
+    dt = time.perf_counter() - t0
+    same = all(torch.equal(x[:, :4 * m], y[:, :4 * m].cpu()) for x, y in zip(cp._blocks(), keep[0]._blocks()))
+    same = same and all(torch.equal(v, keep[1].st[k][..., :m].cpu()) for k, v in csp.st.items())
+    cq, csp2, csrv = make(m, "cpu", True)
+    for _ in range(n_srv[0]):
+        for k in range(4):
+            mask = torch.ones(m, dtype=torch.bool)
+            ball = torch.full((m,), 101 + k, dtype=torch.int64)
+            csrv.push("GotoDirection", mask, ball, f=d[:m, k])
+            csrv.push("SetSpeedFraction", mask, ball, f=sfr[:m, k:k + 1])
+        csrv.pre_tick(snapshot=False)
+        csrv.evolve()
+        csrv.post_tick()
+        csp2.clear_events()
+    same2 = all(torch.equal(x[:, :4 * m], y[:, :4 * m].cpu()) for x, y in zip(cq._blocks(), p2._blocks()))
+    same2 = same2 and all(torch.equal(v, sp2.st[k][..., :m].cpu()) for k, v in csp2.st.items())
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"value": m * n_part[0] / dt, "unit": "world-ticks/s", "cores": cores, "kind": "port",
+                           "sample": f"the partition's tick on the first {m} worlds x {n_part[0]} ticks on the host "
+                                     f"build of the same kernel source (OpenMP over worlds)"}
+    res["exact"] = {"worlds_checked": m, "partition_ticks": n_part[0], "server_ticks": n_srv[0],
+                    "bit_exact": bool(same and same2)}
     return res
 
 

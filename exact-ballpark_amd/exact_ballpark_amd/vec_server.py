@@ -655,12 +655,16 @@ class ServerGraph:
         for j, s in enumerate(self.slots):
             s.mask = q["mask"][j].view(BOOL)
             s.ok, s.row, s.trow, s.loc = q["ok"][j].view(BOOL), q["row"][j], q["trow"][j], q["loc"][j]
-        S.batch = Batch(stamp=S.stamp, clock=S.clock_out, slots=self.slots, route=q["route"].t(),
-                        notices=[(j, q["notice"][j].view(BOOL), None) for j, s in enumerate(self.slots)
-                                 if s.kind == UNCLOAK],
-                        additions=(S.fresh.view(BOOL), S.listed.view(BOOL), None),
-                        gets_bubble=S.gets_bubble.view(BOOL), has_character=S.has_character.view(BOOL))
+        S.batch = self._batch()
         return S.batch
+
+    def _batch(self) -> Batch:
+        S, q = self.server, self.q
+        return Batch(stamp=S.stamp, clock=S.clock_out, slots=self.slots, route=q["route"].t(),
+                     notices=[(j, q["notice"][j].view(BOOL), None) for j, s in enumerate(self.slots)
+                              if s.kind == UNCLOAK],
+                     additions=(S.fresh.view(BOOL), S.listed.view(BOOL), None),
+                     gets_bubble=S.gets_bubble.view(BOOL), has_character=S.has_character.view(BOOL))
 
 
 # ===================================================================================== what each client holds

@@ -81,7 +81,7 @@ class SpaceBatch(VS.Batch):
 
 class SpaceServer(VS.VecServer):
     """VecServer over the master's partition (several bubbles a world). `space`: the VecSpace over the adapter's park
-    (by default `space_for(adapter)`), loaded here. `kernel`: the reference's choice of path; both are this layer's
+    (by default `space_for(adapter)`), loaded here unless its caller has loaded it. `kernel`: the reference's choice of path; both are this layer's
     one kernel (the GPU's for a GPU park, the host build's for a CPU park)."""
 
     def __init__(self, adapter: VS.ModesAdapter, pilots=None, space: SP.VecSpace | None = None,
@@ -94,7 +94,8 @@ class SpaceServer(VS.VecServer):
                 not torch.equal(sp.rows.cpu(), adapter.rows_of_world.cpu()):
             raise ValueError("the partition must cover the adapter's park, worlds and rows (space_for)")
         self.space = sp
-        sp.load()
+        if not sp.loaded:                     # (a layer loaded with its sensors by its caller is taken as it is)
+            sp.load()
         R, N, d = self.R, adapter.N, self.dev
         self.ad_kind = torch.zeros(R, dtype=I8, device=d)
         self.ad_addn = torch.zeros(R, dtype=I32, device=d)
@@ -221,6 +222,51 @@ class SpaceServer(VS.VecServer):
         out = torch.full((self.R + 1,), -1, dtype=I64, device=self.dev)
         out.scatter_(0, self.space._flat, self.space.st["old_bubble"].t().reshape(-1))
         return out[:self.R]
+
+
+# ===================================================================================== a tick as a HIP graph
+
+class SpaceServerGraph(VS.ServerGraph):
+    """A SpaceServer's tick with a fixed schedule of pushes captured once as a HIP graph: the pre-tick (the follower
+    clean-up included), the partition's tick (dead bubbles, the park's kernel, the re-filings and HandleProximities)
+    and the post-tick (the reference's SpaceServerGraph, vec_server_space_cuda.py:483-541). As ServerGraph: before
+    each replay the caller writes each push's arguments into `inputs[j]`; `tick()` replays and returns the SpaceBatch
+    (views of the fixed buffers; no stream records). The partition's tables are fixed buffers too."""
+
+    def __init__(self, server: SpaceServer, schedule: list, warm: int = 1):
+        if not isinstance(server, SpaceServer):
+            raise TypeError("SpaceServerGraph takes a SpaceServer")
+        super().__init__(server, schedule, warm)
+        q, J = self.q, len(self.schedule)
+        self.cleanup = [VS.Slot(STOP, q["mask"][j].view(BOOL), q["ball"][j], q["target"][j], q["f"][j], q["n"][j],
+                                q["loc_in"][j]) for j in range(J, q["J"])]
+
+    def _server_tensors(self) -> tuple:
+        S = self.server
+        return super()._server_tensors() + (S.ad_kind, S.ad_addn, S.ad_deln, S.ad_add, S.ad_del, S.narrow,
+                                            S._stops) + tuple(S.space.st.values())
+
+    def _body(self) -> None:
+        S, q = self.server, self.q
+        S._launch(q, 0)
+        S._stops += (q["mask"][q["J_user"]:] & 1).sum(dim=0).to(I64)
+        S.space.tick(S.a.park)
+        S._launch(q, 1)
+
+    def _batch(self) -> SpaceBatch:
+        S, q = self.server, self.q
+        J = len(self.schedule)
+        for k, s in enumerate(self.cleanup):
+            j = J + k
+            s.ok, s.row, s.trow, s.loc = q["ok"][j].view(BOOL), q["row"][j], q["trow"][j], q["loc"][j]
+        slots = self.slots + self.cleanup
+        return SpaceBatch(stamp=S.stamp, clock=S.clock_out, slots=slots, route=q["route"].t(),
+                          notices=[(j, q["notice"][j].view(BOOL), None) for j, s in enumerate(slots)
+                                   if s.kind == VS.UNCLOAK],
+                          additions=None, gets_bubble=S.gets_bubble.view(BOOL),
+                          has_character=S.has_character.view(BOOL), rbubble=q["rbubble"].t(),
+                          ad={"kind": S.ad_kind, "addn": S.ad_addn, "deln": S.ad_deln, "add": S.ad_add,
+                              "dele": S.ad_del}, ad_records=None, narrow=S.narrow)
 
 
 # ================================================================================ decoding (the host, the tests)

@@ -181,6 +181,7 @@ class VecSpace:
         t["rnd"] = self._srand(seeds)
         self.st = t
         self._state = None
+        self.loaded = False                 # load() has filed the balls
         self._flat = torch.where(self.rows >= 0, self.rows, park.n).flatten()   # slot -> row (past R: none)
 
     def _srand(self, seeds: torch.Tensor) -> torch.Tensor:
@@ -282,6 +283,7 @@ class VecSpace:
         code = torch.where(self.rows >= 0, OP["ADD"], OP["NONE"]).to(I64)
         slots = torch.arange(self.N, dtype=I64, device=self.dev).expand(self.W, self.N).contiguous()
         self.tape(code, slots)
+        self.loaded = True
 
     def dead_rows(self, worlds: torch.Tensor | None = None) -> torch.Tensor:
         """Phase 1's dead bubbles (TIME-714): their live rows flagged moribund for the park's tick, and [R] of them

@@ -550,6 +550,19 @@ def partition_park(seed: int, worlds: int, min_ships: int = 4, max_ships: int = 
             "sensors": sensors, "seeds": rng.integers(1, 2 ** 31, worlds), "partition": True}
 
 
+def take_worlds(wl: dict, m: int) -> dict:
+    """The first m worlds of a partition park, as partition_park lays them out (rows, commands and sensors of those
+    worlds; the commands' targets and the sensors' rows index rows within a world's own, so they carry over)."""
+    m = min(int(m), int(wl["worlds"]))
+    n = int(wl["start"][m - 1] + wl["counts"][m - 1]) if m else 0
+    rows = {k: v[:n] for k, v in wl["rows"].items()}
+    cmds = {k: v[:n] for k, v in wl["cmds"].items()}
+    keep = wl["sensors"]["row"] < n
+    sensors = {k: v[keep] for k, v in wl["sensors"].items()}
+    return dict(wl, n=n, worlds=m, counts=wl["counts"][:m], start=wl["start"][:m], rows=rows, cmds=cmds,
+                sensors=sensors, seeds=wl["seeds"][:m])
+
+
 def space_rows(wl: dict) -> torch.Tensor:
     """[W, N] the park row of each world's slots (N the widest world), -1 past a world's ships."""
     W = wl["worlds"]

@@ -239,10 +239,11 @@ def compare_golden(do: bool, device: str, limit: int | None = None, reverse: boo
 del math, struct
 
 
-def run_streams(worlds: list, device: str, snapshot: bool = False) -> list:
+def run_streams(worlds: list, device: str, snapshot: bool = False, space: bool = False) -> list:
     """The worlds' streams tick by tick on one park; after each tick (the park's blocks, the calls' queued flags and
-    routes, the send flags, the query answers) on the host."""
-    server, _ = build(worlds, device)
+    routes, the send flags, the query answers; over the partition also each route's bubble, the narrowcast bubbles,
+    the bubble update's lists and the partition's tables) on the host."""
+    server, _ = build(worlds, device, space)
     out = []
     for t in range(len(worlds[0]["ticks"])):
         answers = apply_tick(server, worlds, t)
@@ -255,6 +256,9 @@ def run_streams(worlds: list, device: str, snapshot: bool = False) -> list:
                     "ok": [s.ok.cpu().clone() for s in b.slots], "route": b.route.cpu().clone(),
                     "has": b.has_character.cpu().clone(), "gets": b.gets_bubble.cpu().clone(),
                     "answers": json.dumps(answers)})
+        if space:
+            out[-1]["blocks"] += [b.rbubble.cpu().clone(), b.narrow.cpu().clone()] + \
+                [v.cpu().clone() for v in b.ad.values()] + [v.cpu().clone() for v in server.space.st.values()]
     server.check()
     return out
 
