@@ -299,7 +299,8 @@ def main() -> int:
                      "autotuned": False}
         else:
             r = AT.tune(park, blocks=(a.block,) if a.block else (64, 128, 256),
-                        schedules=(a.schedule,) if a.schedule >= 0 else (0, 1),
+                        schedules=(a.schedule,) if a.schedule >= 0 else
+                        ((0, 1, 3) if park.dynamical_orientation else (0, 1)),
                         ticks=(a.ticks_per_launch,) if a.ticks_per_launch else (1, 4, 8, 16), orders=orders)
             shape = {"block": r["block"], "schedule": r["schedule"], "ticks": r["ticks"], "order": r["order"],
                      "autotuned": True, "trials_ms_per_tick": r["trials"]}
@@ -307,6 +308,7 @@ def main() -> int:
               f"{shape['ticks']}/{shape['order']}", file=sys.stderr, flush=True)
         snap, at_snap = park._save(), done          # every measurement below starts from this state
         path0 = park.path.clone() if park.path is not None else None
+        iters0 = park.iters.clone() if park.iters is not None else None
         last = [0]
 
         def restore():
@@ -314,6 +316,8 @@ def main() -> int:
                 blk.copy_(old)
             if path0 is not None:
                 park.path.copy_(path0)              # the recorded paths too: the learned order is predicted from them
+            if iters0 is not None:
+                park.iters.copy_(iters0)            # and the search steps: schedule 3 places lanes by them
 
         def measure(ticks_per_launch, steps, warmup):
             restore()

@@ -158,6 +158,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
         xb_coll_store(P, rows[k].row, cr[k]);
         if (ypr) xb_ypr_store(s.ypr, s.stride, rows[k].row, ys[k]);
         if (s.path) s.path[rows[k].row] = rows[k].path;
+        if (s.iters) s.iters[rows[k].row] = rows[k].iters;
     }
 }
 
@@ -195,6 +196,7 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
         if (dyn && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
         if (ypr) xb_ypr_store(s.ypr, s.stride, rows[k].row, ys[k]);
         if (s.path) s.path[rows[k].row] = rows[k].path;
+        if (s.iters) s.iters[rows[k].row] = rows[k].iters;
     }
 }
 

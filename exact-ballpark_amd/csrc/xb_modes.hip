@@ -59,6 +59,17 @@ __device__ __forceinline__ void xb_form_lds(XbRecords& R, const XbPark& P, int n
 constexpr int XB_MAX_WAVES = 4;                          // 256 lanes
 constexpr int XB_KEY_BUCKETS = 16;
 
+// A row's orientation searches' steps in its last tick as one of 16 buckets, two a power of two (0; 1; 2, 3; 4-5, 6-7;
+// ...; from 128 all in the last): lanes of similar search lengths share a wavefront, whose search loop runs as long as
+// its longest lane's.
+__device__ __forceinline__ int xb_steps_bucket(int v) {
+    if (v <= 0) return 0;
+    const int m = 31 - __clz(v);
+    const int half = m >= 1 ? (v >> (m - 1)) & 1 : 0;
+    const int b = 1 + 2 * m + half;
+    return b < XB_KEY_BUCKETS - 1 ? b : XB_KEY_BUCKETS - 1;
+}
+
 // The place (row of the tile) lane j runs: j itself in row order; otherwise the lanes sorted by their rows' keys, a
 // stable counting sort by ballot (a lane's rank among equal keys: its wave's lower lanes, then the lower waves), so
 // the assignment is deterministic. Places past the tile's row count take the last bucket.
@@ -71,6 +82,8 @@ __device__ __forceinline__ int xb_schedule_place(const xb_modes_state& s, const 
         const int64_t r = s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j);
         if (s.schedule == 2) {
             key = s.keys[r];
+        } else if (s.schedule == 3) {
+            key = xb_steps_bucket(s.iters[r]);
         } else {
             const bool live = XB_B(P, XB_B_FREE, r) && !XB_B(P, XB_B_MORIBUND, r);
             key = xb_branch_key(XB_Q(P, XB_I_MODE, r), XB_Q(P, XB_I_EFFECT_STAMP, r), live, P.new_orbit);
@@ -149,6 +162,7 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
         if (DO && w.turned) xb_turn_store(P, r, w.t);
         if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
         if (s.path) s.path[r] = w.path;
+        if (s.iters) s.iters[r] = w.iters;
     }
 }
 
@@ -268,6 +282,7 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
         xb_coll_store(P, r, c);
         if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
         if (s.path) s.path[r] = w.path;
+        if (s.iters) s.iters[r] = w.iters;
     }
 }
 
@@ -396,6 +411,7 @@ __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     if (DO && w.turned) xb_turn_store(P, r, w.t);
     if (YPR) xb_ypr_store(s.ypr, s.stride, r, y);
     if (s.path) s.path[r] = w.path;
+    if (s.iters) s.iters[r] = w.iters;
 }
 
 // Dynamical orientation's set-up of the listed rows (rows NULL: rows 0 .. n_rows - 1): what 0 SnapOrientation to the
@@ -562,7 +578,9 @@ int xb_modes_tick(const xb_modes_state* state, void* stream, int block) {
     const bool dyn = state->dynamical != 0;
     const bool ypr = state->ypr != nullptr;
     hipStream_t st = (hipStream_t)stream;
-    if (state->schedule < 0 || state->schedule > 2 || (state->schedule == 2 && !state->keys)) return -1;
+    if (state->schedule < 0 || state->schedule > 3 || (state->schedule == 2 && !state->keys) ||
+        (state->schedule == 3 && !state->iters))
+        return -1;
     if (dyn && ypr) return -1;      // dynamical orientation keeps the rotation, not the display orientation
     const bool forms = state->form && state->n_form > 0;
     if (forms && (!state->form_len || state->form_stride <= 0 || !state->tiles || (!dyn && !ypr))) return -1;

@@ -487,6 +487,7 @@ struct XbRow {
     bool free_, moribund;
     bool grown;                    /* a mushroom's turn changed its radius or removed it (stored back then) */
     uint8_t path;                  /* the path class its last tick executed (XB_PATH_*) */
+    uint8_t iters;                 /* its orientation searches' steps in its last tick, at most 255 */
     int64_t row;                   /* its physical row */
 };
 
@@ -542,6 +543,7 @@ static inline XB_HD void xb_row_load(const XbPark& P, int64_t r, XbRow& w) {
     s.sensor = XB_B(P, XB_B_SENSOR, r) != 0;
     s.formation_id = (int)(int8_t)XB_B(P, XB_B_FORMATION, r);
     w.path = XB_PATH_IDLE;
+    w.iters = 0;
     w.row = r;
     w.turned = false;
     w.grown = false;
@@ -749,7 +751,11 @@ static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, 
         } else if (s.mode == XB_MODE_STOP) {
             if (!DO) s.vel.y = (s.vel.y - XB_STOP_DECAY * s.vel.y) * XB_STOP_DAMP;
         }
-        if (DO) a = xo_orient(s, w.t, a, P.friction, P.dt, &rot1, &w1);
+        if (DO) {
+            int it = 0;
+            a = xo_orient(s, w.t, a, P.friction, P.dt, &rot1, &w1, &it);
+            w.iters = (uint8_t)(it < 255 ? it : 255);
+        }
         s.last_g = a;
     }
     if (rel_after) xb_release(s, true);

@@ -138,8 +138,8 @@ static inline XB_HD void xo_n2_alpha2_f(const XoShip& k, float omega0, int* n2, 
     }
 }
 
-/* GotoTorqueBreaks (full.py:2155-2222): the heading torque toward `direction` */
-static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ rot, XoV3 w) {
+/* GotoTorqueBreaks (full.py:2155-2222): the heading torque toward `direction`; `iters` counts the search's steps */
+static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ rot, XoV3 w, int* iters) {
     const XoV3 zero = xo_v(0.0f, 0.0f, 0.0f);
     const float max_alpha = k.max_ang_vel * k.t_scale;
     if (max_alpha <= 0.0f) return zero;
@@ -186,6 +186,7 @@ static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ r
             if (xo_len(alpha1) < max_alpha) break;
             n1 += 1;
         }
+        *iters += n1;
         const int n2_old = n2;
         const XoV3 omega1 = xo_add(xo_mul(alpha1, tau * (1.0f - kappa1)), xo_mul(omega0, kappa1));
         xo_n2_alpha2(k, omega1, xo_len(omega1), &n2, &kappa2, &alpha2);
@@ -196,8 +197,9 @@ static inline XB_HD XoV3 xo_torque_breaks(const XoShip& k, XoV3 direction, XoQ r
     return zero;
 }
 
-/* GotoRollBreaks (full.py:2224-2290): the roll torque, into strong turns, otherwise upright */
-static inline XB_HD float xo_roll_breaks(const XoShip& k, XoV3 heading_torque, XoQ rot, XoV3 w) {
+/* GotoRollBreaks (full.py:2224-2290): the roll torque, into strong turns, otherwise upright; `iters` counts the search's
+   steps */
+static inline XB_HD float xo_roll_breaks(const XoShip& k, XoV3 heading_torque, XoQ rot, XoV3 w, int* iters) {
     const float max_alpha = k.t_scale * k.max_ang_vel;
     if (max_alpha <= 0.0f) return 0.0f;
     const XoM mat = xo_matrix(xo_conj(rot));
@@ -247,6 +249,7 @@ static inline XB_HD float xo_roll_breaks(const XoShip& k, XoV3 heading_torque, X
             if (xo_abs(alpha1) < max_alpha) break;
             n1 += 1;
         }
+        *iters += n1;
         const int n2_old = n2;
         const float omega1 = (alpha1 * tau) * (1.0f - kappa1) + omega_roll * kappa1;
         xo_n2_alpha2_f(k, omega1, &n2, &kappa2, &alpha2);
@@ -288,9 +291,10 @@ static inline XB_HD void xo_apply_torque(XoQ rotation, XoV3 angvel, XoV3 torque,
 /* EvolveBehaviorForBall's dynamical-orientation branch (full.py:2459-2485): above 1 % of the maximum acceleration the
    torque and roll are searched and the thrust laid along the heading a tick on, scaled by cos^2 of its angle (zero
    at 90 degrees or more); below it the thrust is kept, a ball in warp proper turning toward its exit point. The
-   rotation and angular velocity a tick on (the same ApplyTorque the tick's commit makes) come back in rot1, w1. */
+   rotation and angular velocity a tick on (the same ApplyTorque the tick's commit makes) come back in rot1, w1; the
+   two searches' steps in *iters (the tick's cost the scheduler learns, never a result). */
 static inline XB_HD XbV3 xo_orient(const XbShip& s, XbTurn& t, XbV3 a, double friction, double dt, XoQ* rot1,
-                                   XoV3* w1) {
+                                   XoV3* w1, int* iters) {
     const float accel = (float)v3_len(a);
     const float ff = (float)friction;
     const float fmass = (float)s.mass, ag = (float)s.agility, sf = (float)s.sf, mv = (float)s.max_vel;
@@ -304,10 +308,10 @@ static inline XB_HD XbV3 xo_orient(const XbShip& s, XbTurn& t, XbV3 a, double fr
     k.tf = xb_expf(-k.fdt * k.t_scale);
     const XoV3 a32 = xo_of(a);
     XoV3 torque = xo_v(0.0f, 0.0f, 0.0f);
-    if (thrust) torque = xo_torque_breaks(k, a32, t.rot, t.w);
+    if (thrust) torque = xo_torque_breaks(k, a32, t.rot, t.w, iters);
     else if (s.mode == XB_MODE_WARP && !(s.effect_stamp < 0))
-        torque = xo_torque_breaks(k, xo_of(v3_sub(s.gp, s.pos)), t.rot, t.w);
-    const float roll = xo_roll_breaks(k, torque, t.rot, t.w);
+        torque = xo_torque_breaks(k, xo_of(v3_sub(s.gp, s.pos)), t.rot, t.w, iters);
+    const float roll = xo_roll_breaks(k, torque, t.rot, t.w, iters);
     const float tau = ((fmass * ag) * t.rot_agility) / ff;
     xo_apply_torque(t.rot, t.w, torque, roll, tau, k.fdt, rot1, w1);
     t.torque = torque;

@@ -16,7 +16,7 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 7
+ABI_VERSION = 8
 NF, NI, NB, NS = 46, 10, 7, 7
 NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 NY = 9                          # the display orientation block: yaw, pitch, roll, their old values, roll speed (2), yaw delta
@@ -40,7 +40,7 @@ class ModesState(ctypes.Structure):
                 ("dt", ctypes.c_double), ("ffriction", ctypes.c_float), ("fdt", ctypes.c_float),
                 ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32),
                 ("schedule", ctypes.c_int32), ("n_keys", ctypes.c_int32), ("keys", ctypes.c_void_p),
-                ("path", ctypes.c_void_p), ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
+                ("path", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("hits_cap", ctypes.c_int32), ("collisions", ctypes.c_int32),
                 ("ypr", ctypes.c_void_p), ("st_f64", ctypes.c_void_p), ("st_i64", ctypes.c_void_p),
                 ("st_keys", ctypes.c_void_p), ("st_range", ctypes.c_void_p), ("st_stride", ctypes.c_int64),

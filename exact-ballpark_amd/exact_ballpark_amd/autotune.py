@@ -29,6 +29,7 @@ def time_shape(park: VecModesPark, block: int, schedule: int, ticks: int, reps: 
     reps = reps or max(3, 32 // ticks)
     saved = park._save()
     path0 = park.path.clone() if park.path is not None else None
+    iters0 = park.iters.clone() if park.iters is not None else None
     t0, keys0 = park.current_time, park.keys
     old = (park.block, park.schedule)
     park.block = block
@@ -54,6 +55,8 @@ def time_shape(park: VecModesPark, block: int, schedule: int, ticks: int, reps: 
         _restore(park, saved)
         if path0 is not None:
             park.path.copy_(path0)
+        if iters0 is not None:
+            park.iters.copy_(iters0)
         park.current_time, park.keys = t0, keys0
         park.block, park.schedule = old
         park._set_tiles()

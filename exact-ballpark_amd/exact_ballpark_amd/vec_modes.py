@@ -245,9 +245,11 @@ class VecModesPark(metaclass=_Fields):
         self.tiles = None
         self.rowmap = None             # [n] int32: the tiled launch's rows in a regrouped world order (None: row order)
         self._state = None
-        self.schedule = 0              # lanes of a tile: 0 row order, 1 the built-in branch key, 2 `keys`
+        self.schedule = 0              # lanes of a tile: 0 row order, 1 the built-in branch key, 2 `keys`, 3 the
+                                       # rows' last search steps (`iters`, record_paths)
         self.keys = None               # [n] uint8 buckets for schedule 2 (the learned scheduler's)
         self.path = None               # [n] uint8: each row's executed path class, when recorded
+        self.iters = None              # [n] uint8: each row's orientation searches' steps, when recorded
         self._set_tiles()
 
     # ---- the state as the kernel takes it ----
@@ -295,6 +297,7 @@ class VecModesPark(metaclass=_Fields):
         s.n_keys = 16
         s.keys = self.keys.data_ptr() if (self.keys is not None and s.schedule == 2) else None
         s.path = self.path.data_ptr() if self.path is not None else None
+        s.iters = self.iters.data_ptr() if self.iters is not None else None
         s.collisions = int(self.collisions)
         if self.collisions:
             s.coll_f64, s.coll_i64 = self._cf.data_ptr(), self._cq.data_ptr()
@@ -317,8 +320,11 @@ class VecModesPark(metaclass=_Fields):
         return s
 
     def record_paths(self, on: bool = True) -> None:
-        """Record each row's executed path class (XB_PATH_*: 0 idle .. 7 warp edge) in `path` at every tick."""
+        """Record at every tick each row's executed path class (XB_PATH_*: 0 idle .. 7 warp edge) in `path` and its
+        orientation searches' steps (GotoTorqueBreaks' and GotoRollBreaks' inner iterations, at most 255) in `iters`:
+        what its tick cost, never a result."""
         self.path = torch.zeros(self.n, dtype=U8, device=self.dev) if on else None
+        self.iters = torch.zeros(self.n, dtype=U8, device=self.dev) if on else None
         self._state = None
 
     def row_tensors(self) -> list:

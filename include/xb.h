@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 7
+#define XB_ABI_VERSION 8
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -57,10 +57,14 @@ typedef struct xb_modes_state {
                                       doubles; STOP without its damping; the new orbit's normal from the heading */
     int32_t ticks;                 /* ticks per launch (tiled shape only; 0 or 1: one) */
     int32_t schedule;              /* lanes of a tile: 0 in row order; 1 sorted by the built-in branch key;
-                                      2 sorted by keys[row] (tiled shape only) */
+                                      2 sorted by keys[row]; 3 sorted by the orientation searches' steps each row took
+                                      in its last recorded tick (iters[row], bucketed) (tiled shape only) */
     int32_t n_keys;                /* the keys' bucket count, at most 16 */
     const uint8_t* keys;           /* schedule 2: each row's bucket, below n_keys */
     uint8_t* path;                 /* if not NULL: each row's executed path class in its last tick (0..7) */
+    uint8_t* iters;                /* if not NULL: each row's orientation searches' steps in its last tick (GotoTorque-
+                                      Breaks' and GotoRollBreaks' inner iterations, at most 255; 0 without dynamical
+                                      orientation): a cost counter for the scheduler, never a result */
     /* The gradient collision method (SpaceBallpark's partition and Gradient, a client's view), tiled shape only, on
        when `collisions` is set: each world at most 64 rows, consecutive in the tile. coll_f64 holds XB_NCF rows
        (mLastC), coll_i64 XB_NCI rows (the filing: level, key x3, side x3; the tick's collision list's length), hits

@@ -5,6 +5,8 @@
   by branch key, 16 ticks a launch, worlds regrouped by the learned scheduler trained on the park's own ticks) against
   the plain launch (workgroups of 256 rows in row order, one tick a launch) over the whole park, and sampled worlds
   against the oracle;
+- the composed park with the lanes placed by each row's last orientation search steps (schedule 3) against the plain
+  launch;
 - the flight-modes park of 1,024 worlds x 4 ships with the new orbit law (use_new_orbit) for 100 ticks against the
   oracle, the old law's run beside it in tests/test_modes_gpu.py.
 """
@@ -61,6 +63,27 @@ def test_composed_full_size_headline_shape_equals_plain_and_oracle():
         want = D.run_oracle([sc], [0], record={ticks})[(0, ticks)]
         got = D._snapshot(shaped, [wi], {wi: list(WL.world_rows(wl, wi))})[wi]
         assert D._same_bits(got[0], want[0]).all() and (got[1] == want[1]).all() and got[2] == want[2], wi
+
+
+def test_composed_search_steps_schedule_equals_plain():
+    """Lanes placed by each row's last orientation search steps (schedule 3, the engine's own recorded counters), 16
+    ticks a launch, against the plain launch: the same bits."""
+    wl = WL.composed_park(2, 16384, missiles=8)
+    plain, shaped = _load(wl, block=256), _load(wl, block=128)
+    shaped.record_paths(True)
+    for _ in range(3):
+        plain.evolve()
+        shaped.evolve()
+    assert int(shaped.iters.max()) > 0
+    shaped.schedule = 3
+    print(f"composed park, {wl['n']} rows: lanes by search steps, 2 x 16 ticks a launch", flush=True)
+    for _ in range(2):
+        shaped.evolve_ticks(16)
+    for _ in range(32):
+        plain.evolve()
+    torch.cuda.synchronize()
+    for k, (x, y) in enumerate(zip(_blocks(plain), _blocks(shaped))):
+        assert torch.equal(x.view(torch.uint8), y.view(torch.uint8)), f"block {k} differs"
 
 
 def test_flight_park_new_orbit_matches_oracle():
