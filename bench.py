@@ -95,6 +95,9 @@ WORKLOADS = {
                    "period, flying along x under GotoDirection (re-filed every tick); the partition's tick as a "
                    "SpaceGraph replay, and the server over it (every ship piloted, GotoDirection and SetSpeedFraction "
                    "queued for each ship each tick) as a SpaceServerGraph replay",
+    "masked": "OnTick's masked tick on the flight park: {worlds} worlds x 4 ships, modes uniform over "
+              "stop/goto/follow/orbit/warp, half the worlds ticking each tick (alternate halves), one masked ModesGraph "
+              "replay a tick (the kernels skip the rows outside the mask)",
     "statics": "turning and collisions park with static collidables: {worlds} worlds x (4 free massive ships in a 2 km "
                "cluster + 4 fixed massive spheres of radius 100-5000 m) and 8 static collidables a world (capsules of "
                "length 200-3000 m and radius 30-400 m, oriented boxes of half-extents 30-800 m, in turn, randomly "
@@ -221,6 +224,10 @@ def main() -> int:
                     help="leave out the partition probe's line (the reference's published space shape)")
     ap.add_argument("--space-probe-only", action="store_true",
                     help="measure the partition probe's line alone (its JSON the last line; for profiling)")
+    ap.add_argument("--no-masked", dest="masked", action="store_false",
+                    help="leave out OnTick's masked tick line")
+    ap.add_argument("--masked-only", action="store_true",
+                    help="measure OnTick's masked tick line alone (its JSON the last line; for profiling)")
     ap.add_argument("--detail", default="bench_detail.json", help="where the full results go (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also copy this many MiB device to device 8 times (a kernel of known traffic, for profiling)")
@@ -252,6 +259,9 @@ def main() -> int:
         return 0
     if a.space_probe_only:
         print(json.dumps(space_probe(a)), flush=True)
+        return 0
+    if a.masked_only:
+        print(json.dumps(masked_park(a)), flush=True)
         return 0
 
     def build(kind: str):
@@ -423,6 +433,8 @@ def main() -> int:
         out["space_server_park"] = space_server_park(a)
     if not a.only and a.space_probe:
         out["space_probe_park"] = space_probe(a)
+    if not a.only and a.masked:
+        out["masked_park"] = masked_park(a)
     if a.episodes:
         out["serving"], srv = serving(a)
     if not a.no_cpu:
@@ -783,6 +795,74 @@ def space_server_park(a) -> dict:
                            "kind": "port", "sample": f"the first {m} worlds x {done} ticks on the host build of the "
                                                      f"same kernel source (eager pushes, OpenMP over worlds)"}
     res["exact"] = {"worlds_checked": m, "ticks": done, "bit_exact": bool(same), "partition_tables_compared": tables}
+    return res
+
+
+def masked_park(a) -> dict:
+    """OnTick's masked tick (WORKLOADS["masked"]): the flight park as masked ModesGraph replays, half the worlds a
+    tick, against the same park's unmasked graph; the host build of the same source runs the same masks on the first
+    worlds, compared bit for bit."""
+    import torch
+
+    from exact_ballpark_amd import workloads as WL
+    from exact_ballpark_amd.vec_modes import ModesGraph, VecModesPark
+
+    W = a.worlds
+    wl = WL.flight_park(1, W, 4)
+    park = VecModesPark(wl["n"], device="cuda", block=256)
+    WL.load_flight_park(park, wl)
+    halves = [park.world % 2 == k for k in (0, 1)]
+    gm = ModesGraph(park, masked=True)
+    done = [0]
+
+    def masked_tick():
+        gm.tick(halves[done[0] % 2])
+        done[0] += 1
+
+    def timed(fn, K):
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(K):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / K
+
+    K = max(64, a.steps)
+    ms = timed(masked_tick, K)
+    ref = VecModesPark(wl["n"], device="cuda", block=256)
+    WL.load_flight_park(ref, wl)
+    gu = ModesGraph(ref)
+    ms_full = timed(gu.tick, K)
+    res = {"workload": WORKLOADS["masked"].format(worlds=W), "worlds": W, "ships": wl["n"],
+           "world_ticks_per_s": W / (ms / 1000.0), "ms_per_tick": ms, "ticks_timed": K,
+           "worlds_ticked_per_s": W / 2 / (ms / 1000.0), "unmasked_ms_per_tick": ms_full,
+           "vs_unmasked": ms_full / ms,
+           "reference_published": {"world_ticks_per_s": 26.5e6, "source": "docs/benchmarks.md:78 (the reference's "
+                                   "masked flight-modes graph, half the worlds, 65,536 x 4, GH200; its worlds are a "
+                                   "leader, a follower, an orbiter and a warper)"}}
+    print(f"# masked: {res}", file=sys.stderr, flush=True)
+    if a.no_cpu:
+        return res
+    m = min(W, 4096)
+    sub = WL.flight_park(1, W, 4)
+    cp = VecModesPark(4 * m, device="cpu", block=256)
+    WL.load_flight_park(cp, WL.select_worlds(sub, list(range(m))))
+    ch = [cp.world % 2 == k for k in (0, 1)]
+    t0 = time.perf_counter()
+    for t in range(done[0]):
+        cp.evolve(active=ch[t % 2])
+    dt = time.perf_counter() - t0
+    same = all(torch.equal(x[:, :4 * m], y[:, :4 * m].cpu()) for x, y in zip(cp._blocks(), park._blocks()))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res["cpu_baseline"] = {"value": m * done[0] / dt, "unit": "world-ticks/s", "cores": cores, "kind": "port",
+                           "sample": f"the first {m} worlds x {done[0]} masked ticks on the host build of the same "
+                                     f"kernel source"}
+    res["exact"] = {"worlds_checked": m, "ticks": done[0], "bit_exact": bool(same)}
     return res
 
 

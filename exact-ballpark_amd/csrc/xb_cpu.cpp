@@ -109,6 +109,8 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
         xb_coll_load(P, rows[k].row, cr[k]);
         if (ypr) xb_ypr_load(s.ypr, s.stride, rows[k].row, ys[k]);
     }
+    std::vector<char> on((size_t)cnt);               /* OnTick's mask: a row outside it keeps every field */
+    for (int64_t k = 0; k < cnt; ++k) on[k] = !s.active || s.active[rows[k].row];
     for (int64_t k = 0; k < cnt; ++k) {
         const int64_t wk = XB_Q(P, XB_I_WORLD, rows[k].row);
         int a = (int)k, b = (int)k + 1;
@@ -119,11 +121,14 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             xb_record_a(rows[k], S.R, (int)k);
             if (S.R.ffid) xb_record_form<DO>(rows[k], S.R, (int)k, ypr ? &ys[k] : nullptr);
         }
-        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        for (int64_t k = 0; k < cnt; ++k)
+            if (on[k]) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             rot1[k] = xo_q(0.0f, 0.0f, 0.0f, 1.0f);
             w1[k] = xo_v(0.0f, 0.0f, 0.0f);
             xb_phase1<DO>(rows[k], S.R, (int)k, 0, (int)cnt, P, rot1[k], w1[k]);
@@ -134,12 +139,14 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
             xb_coll_record(rows[k], cr[k], S.C, (int)k, P);
         }
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             const XbShip& sh = rows[k].s;
             warped[k] = xb_row_live(rows[k]) && sh.mode == XB_MODE_WARP && sh.effect_stamp >= 0;
             pe[k] = v3(0.0, 0.0, 0.0), ve[k] = pe[k];
             if (warped[k]) xb_coll_phase3_warp(rows[k], cr[k], S.C, (int)k, P, &pe[k], &ve[k]);
         }
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             if (ypr)
                 xb_coll_phase34<DO, true, true>(rows[k], cr[k], warped[k] != 0, pe[k], ve[k], S.C, S.R, (int)k,
                                                 wlo[k], whi[k], P, rot1[k], w1[k], &ys[k]);
@@ -153,6 +160,7 @@ void run_group_coll(const XbPark& P, const xb_modes_state& s, Scratch& S, std::v
         }
     }
     for (int64_t k = 0; k < cnt; ++k) {
+        if (!on[k]) continue;
         xb_row_store(P, rows[k]);
         if (DO && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
         xb_coll_store(P, rows[k].row, cr[k]);
@@ -175,23 +183,29 @@ void run_group(const XbPark& P, const xb_modes_state& s, Scratch& S, std::vector
         if (dyn) xb_turn_load(P, rows[k].row, rows[k].t);
         if (ypr) xb_ypr_load(s.ypr, s.stride, rows[k].row, ys[k]);
     }
+    std::vector<char> on((size_t)cnt);               /* OnTick's mask: a row outside it keeps every field */
+    for (int64_t k = 0; k < cnt; ++k) on[k] = !s.active || s.active[rows[k].row];
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     for (int t = 0; t < ticks; ++t) {
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             xb_record_a(rows[k], S.R, (int)k);
             if (S.R.ffid) {
                 if (dyn) xb_record_form<true>(rows[k], S.R, (int)k, nullptr);
                 else xb_record_form<false>(rows[k], S.R, (int)k, ypr ? &ys[k] : nullptr);
             }
         }
-        for (int64_t k = 0; k < cnt; ++k) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
+        for (int64_t k = 0; k < cnt; ++k)
+            if (on[k]) xb_record_b(rows[k], S.R, (int)k, 0, (int)cnt, P);
         for (int64_t k = 0; k < cnt; ++k) {
+            if (!on[k]) continue;
             if (dyn) xb_tick_row<true>(rows[k], S.R, (int)k, 0, (int)cnt, P);
             else if (ypr) xb_tick_row<false, true>(rows[k], S.R, (int)k, 0, (int)cnt, P, &ys[k]);
             else xb_tick_row<false>(rows[k], S.R, (int)k, 0, (int)cnt, P);
         }
     }
     for (int64_t k = 0; k < cnt; ++k) {
+        if (!on[k]) continue;
         xb_row_store(P, rows[k]);
         if (dyn && rows[k].turned) xb_turn_store(P, rows[k].row, rows[k].t);
         if (ypr) xb_ypr_store(s.ypr, s.stride, rows[k].row, ys[k]);

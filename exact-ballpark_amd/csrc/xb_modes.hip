@@ -136,8 +136,8 @@ __device__ __forceinline__ void xb_modes_tiled_body(const xb_modes_state& s) {
     const XbPark P = xb_park(s);
     xb_form_lds(R, P, nb, XB_LDS_BYTES_PER_ROW);
     const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
-    const bool on = j < cnt;
-    const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    const int64_t r = j < cnt ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    const bool on = j < cnt && (!s.active || s.active[r]);     /* a row outside OnTick's mask: nothing */
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     XbRow w;
     XbYpr y;
@@ -224,8 +224,8 @@ __device__ __forceinline__ void xb_collide_tiled_body(const xb_modes_state& s) {
     xb_field_lds(C, P, nb, XB_LDS_BYTES_PER_ROW + XB_COLL_LDS_BYTES_PER_ROW +
                                (P.n_form > 0 ? XB_FORM_LDS_BYTES_PER_ROW : 0));
     const int j = s.schedule ? xb_schedule_place(s, P, (int)threadIdx.x, cnt, lo, perm) : (int)threadIdx.x;
-    const bool on = j < cnt;
-    const int64_t r = on ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    const int64_t r = j < cnt ? (s.rowmap ? (int64_t)s.rowmap[lo + j] : (int64_t)(lo + j)) : 0;
+    const bool on = j < cnt && (!s.active || s.active[r]);     /* a row outside OnTick's mask: nothing */
     const int ticks = s.ticks > 0 ? s.ticks : 1;
     XbRow w;
     XbCollRow c;
@@ -375,7 +375,7 @@ __device__ __forceinline__ XbRecords xb_global_records(const xb_modes_state& s) 
 
 __global__ void __launch_bounds__(256) xb_modes_step_a(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.n) return;
+    if (r >= s.n || (s.active && !s.active[r])) return;
     XbRecords R = xb_global_records(s);
     const XbPark P = xb_park(s);
     XbRow w;
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(256) xb_modes_step_a(xb_modes_state s) {
 
 __global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.n) return;
+    if (r >= s.n || (s.active && !s.active[r])) return;
     XbRow w;
     xb_row_load(xb_park(s), r, w);
     xb_record_b(w, xb_global_records(s), (int)r, 0, (int)s.n, xb_park(s));
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(256) xb_modes_step_b(xb_modes_state s) {
 template <bool DO, bool YPR>
 __global__ void __launch_bounds__(256) xb_modes_step_c(xb_modes_state s) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.n) return;
+    if (r >= s.n || (s.active && !s.active[r])) return;
     const XbPark P = xb_park(s);
     XbRow w;
     xb_row_load(P, r, w);

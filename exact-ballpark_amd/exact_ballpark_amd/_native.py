@@ -40,7 +40,8 @@ class ModesState(ctypes.Structure):
                 ("dt", ctypes.c_double), ("ffriction", ctypes.c_float), ("fdt", ctypes.c_float),
                 ("dynamical", ctypes.c_int32), ("ticks", ctypes.c_int32),
                 ("schedule", ctypes.c_int32), ("n_keys", ctypes.c_int32), ("keys", ctypes.c_void_p),
-                ("path", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
+                ("path", ctypes.c_void_p), ("iters", ctypes.c_void_p), ("active", ctypes.c_void_p),
+                ("coll_f64", ctypes.c_void_p), ("coll_i64", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("hits_cap", ctypes.c_int32), ("collisions", ctypes.c_int32),
                 ("ypr", ctypes.c_void_p), ("st_f64", ctypes.c_void_p), ("st_i64", ctypes.c_void_p),
                 ("st_keys", ctypes.c_void_p), ("st_range", ctypes.c_void_p), ("st_stride", ctypes.c_int64),

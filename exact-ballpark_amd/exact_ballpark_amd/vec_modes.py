@@ -1231,24 +1231,30 @@ class VecModesPark(metaclass=_Fields):
 
     # ---- one tick (Ballpark::Evolve; vec_modes.py:943-960) ----
 
-    def _tick(self, block: int | None = None, ticks: int = 1) -> None:
+    def _tick(self, block: int | None = None, ticks: int = 1, active: torch.Tensor | None = None) -> None:
+        """`ticks` ticks in one launch; `active` (a bool or uint8 device tensor of at least n rows, kept alive by the
+        caller until the launch has run): OnTick's mask, the rows outside it skipped by the kernels."""
         s = self.state()
         if self.n == 0:
             return
         b = self.block if block is None else int(block)
         if ticks > 1 and self.tiles is None:
             for _ in range(ticks):
-                self._tick(block)
+                self._tick(block, 1, active)
             return
         s.ticks = int(ticks)
-        if self.dev.type == "cuda":
-            with torch.cuda.device(self.dev):
-                rc = N.hip().xb_modes_tick(ctypes.byref(s), torch.cuda.current_stream(self.dev).cuda_stream, b)
-            N.check(rc, "xb_modes_tick")
-        else:
-            rc = N.cpu().xb_cpu_modes_tick(ctypes.byref(s), int(getattr(self, "cpu_threads", 0)))
-            if rc:
-                raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
+        s.active = active.data_ptr() if active is not None else None
+        try:
+            if self.dev.type == "cuda":
+                with torch.cuda.device(self.dev):
+                    rc = N.hip().xb_modes_tick(ctypes.byref(s), torch.cuda.current_stream(self.dev).cuda_stream, b)
+                N.check(rc, "xb_modes_tick")
+            else:
+                rc = N.cpu().xb_cpu_modes_tick(ctypes.byref(s), int(getattr(self, "cpu_threads", 0)))
+                if rc:
+                    raise RuntimeError(f"xb_cpu_modes_tick failed ({rc})")
+        finally:
+            s.active = None
 
     def _blocks(self) -> tuple:
         return ((self._f, self._q, self._b) + ((self._cf, self._cq, self._hits) if self.collisions else ())
@@ -1267,13 +1273,12 @@ class VecModesPark(metaclass=_Fields):
     def evolve(self, lib=None, active: torch.Tensor | None = None) -> None:
         """One tick of every world, in place, no host synchronisation. `active` [n]: OnTick's per-world mask (a world's
         rows together): only those rows tick; every other row keeps every field bit for bit (the reference's
-        masked_tick, vec_orient.py:217-254: the tick reads only rows of each row's own world)."""
+        masked_tick, vec_orient.py:217-254, which ticks every row and puts the others back; here the kernels skip the
+        rows outside the mask: the tick reads only rows of each row's own world)."""
         if lib is not None:
             raise ValueError("lib: this park binds its own library")
         if active is not None:
-            saved = self._save()
-            self._tick()
-            self._put_back(saved, active)
+            self._tick(active=active.to(device=self.dev, dtype=torch.bool).reshape(self.n).contiguous())
         else:
             self._tick()
         self.current_time += 1
@@ -1348,9 +1353,8 @@ class ModesGraph:
     `launches` launches a replay, one replay runs ticks x launches ticks back to back. Commands and setters that write
     into the blocks need no re-binding between replays; a call that replaces a buffer the launches point at (the first
     set_harmonic, load_formations, load_statics, record_paths, a regrouping, new tiles) makes tick() raise: capture
-    again. With `masked=True` the capture holds OnTick's per-world mask
-    as a device buffer: the blocks saved, the tick, the rows outside the mask put back; `tick(active)` writes the mask
-    and replays."""
+    again. With `masked=True` the capture holds OnTick's per-world mask as a device buffer the kernels read (the rows
+    outside it skipped); `tick(active)` writes the mask and replays."""
 
     def __init__(self, park: VecModesPark, block: int | None = None, lib=None, masked: bool = False,
                  ticks: int = 1, launches: int = 1):
@@ -1366,22 +1370,16 @@ class ModesGraph:
         if self.launches < 1 or self.ticks < 1:
             raise ValueError("ticks and launches are at least 1")
         self.mask = torch.ones(park.stride, dtype=torch.bool, device=park.dev)
-        self.saved = [torch.empty_like(b) for b in park._blocks()] if masked else None
+        act = self.mask if masked else None
         before = park._save()                            # warm-up: the module loaded before the capture,
-        park._tick(self.block, self.ticks)               # the park's state put back after it
+        park._tick(self.block, self.ticks, act)          # the park's state put back after it
         for b, old in zip(park._blocks(), before):
             b.copy_(old)
         torch.cuda.synchronize(park.dev)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            if masked:
-                for s, b in zip(self.saved, park._blocks()):
-                    s.copy_(b)
             for _ in range(self.launches):
-                park._tick(self.block, self.ticks)
-            if masked:
-                for s, b in zip(self.saved, park._blocks()):
-                    b.copy_(torch.where(self.mask[None, :], b, s))
+                park._tick(self.block, self.ticks, act)
         self.captured = self._pointers()
 
     def tick(self, active: torch.Tensor | None = None) -> None:

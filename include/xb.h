@@ -65,6 +65,8 @@ typedef struct xb_modes_state {
     uint8_t* iters;                /* if not NULL: each row's orientation searches' steps in its last tick (GotoTorque-
                                       Breaks' and GotoRollBreaks' inner iterations, at most 255; 0 without dynamical
                                       orientation): a cost counter for the scheduler, never a result */
+    const uint8_t* active;         /* if not NULL: OnTick's mask, each row's byte (a world's rows alike): a row whose
+                                      byte is 0 does not tick and keeps every field bit for bit */
     /* The gradient collision method (SpaceBallpark's partition and Gradient, a client's view), tiled shape only, on
        when `collisions` is set: each world at most 64 rows, consecutive in the tile. coll_f64 holds XB_NCF rows
        (mLastC), coll_i64 XB_NCI rows (the filing: level, key x3, side x3; the tick's collision list's length), hits
