@@ -161,6 +161,8 @@ class World:
         q = (ctypes.c_int64 * (3 * 512))()
         c = (ctypes.c_double * (3 * 512))()
         n = self.L.or_take_events(self.p, q, c, 512)
+        if n < 0:
+            raise OverflowError("the oracle left DoCollision events out (more than 512 unread): read them each tick")
         return [(q[3 * i], q[3 * i + 1], q[3 * i + 2], c[3 * i], c[3 * i + 1], c[3 * i + 2]) for i in range(n)]
 
     def add_mini(self, m: dict) -> None:

@@ -110,6 +110,7 @@ typedef struct {
     int overflow;                              /* a collision list past MAX_HITS */
     int64_t local_hi;                          /* GetIDForCollisionObject(-1)'s last local id on a client */
     int n_ev;                                  /* DoCollision events since the last read (the first MAX_EVENTS) */
+    int64_t ev_dropped;                        /* those past MAX_EVENTS since the last read (not kept) */
     int64_t ev_q[MAX_EVENTS][3];               /* time, missile id, collider id */
     double ev_c[MAX_EVENTS][3];                /* the missile's mLastC */
     int n;
@@ -2115,7 +2116,10 @@ void or_evolve(OPark* p) {                                              /* modes
         refile(p, b);
         if (b->mode == MISSILE) {                       /* Ball::DispatchCollisions (full.py:2571-2583) */
             for (int h = 0; h < b->nh; ++h) {
-                if (p->n_ev >= MAX_EVENTS) break;       /* unread events: the oldest are kept, no more */
+                if (p->n_ev >= MAX_EVENTS) {            /* unread events: the oldest are kept, the rest counted */
+                    p->ev_dropped++;
+                    continue;
+                }
                 int64_t* q = p->ev_q[p->n_ev];
                 q[0] = p->current_time, q[1] = b->id, q[2] = b->hits[h];
                 for (int k = 0; k < 3; ++k) p->ev_c[p->n_ev][k] = b->last_c[k];
@@ -2283,13 +2287,16 @@ void or_set_harmonic(OPark* p, int64_t id, int64_t harmonic, int64_t corporation
 }
 
 /* The DoCollision events since the last call (time, missile id, collider id; its mLastC), in Evolve's order; the
-   buffer is emptied. Returns the count. */
+   buffer is emptied. Returns the count, or -1 when events were left out since the last call (more than MAX_EVENTS
+   unread, or more than `max`): a reader that takes them gets all of them or an error. */
 int or_take_events(OPark* p, int64_t* q, double* c, int max) {
     const int n = p->n_ev < max ? p->n_ev : max;
+    const int lost = p->ev_dropped > 0 || p->n_ev > max;
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) q[3 * i + k] = p->ev_q[i][k], c[3 * i + k] = p->ev_c[i][k];
     p->n_ev = 0;
-    return n;
+    p->ev_dropped = 0;
+    return lost ? -1 : n;
 }
 
 /* RemoveBall outside Evolve without a delay (full.py:1633-1675), a client's view, for a ball without minis: stopped,

@@ -59,3 +59,25 @@ def test_park_events_match_reference(name, do):
     D.run_park(golden["worlds"], "cpu", order=golden["kept"], block=128, events=got)
     bad = [wi for wi in golden["kept"] if not same(got.get(wi, []), want.get(wi, []))]
     assert not bad, f"{len(bad)} worlds differ, first {bad[:6]}"
+
+
+def test_oracle_flags_events_it_left_out():
+    """A reader of the oracle's events gets every one or an error: a world whose run piles up more events than a
+    read takes (here a read of one at a time) makes take_events raise instead of truncating."""
+    import ctypes
+    golden, want = G.load("missile_golden.npz"), golden_events("missile_golden.npz")
+    wi = next(w for w in golden["kept"] if len(want.get(w, [])) >= 2)
+    w = golden["worlds"][wi]
+    o = O.World(w["friction"], w["tick"], w["new_orbit"], w["start"], False, True, True)
+    for b in w["balls"]:
+        o.add(b)
+    by_step = defaultdict(list)
+    for cmd in w["cmds"]:
+        by_step[cmd[0]].append(cmd)
+    for step in range(1, w["steps"] + 1):
+        for cmd in by_step[step]:
+            o.command(cmd)
+        o.evolve()
+    q, c = (ctypes.c_int64 * 3)(), (ctypes.c_double * 3)()
+    assert o.L.or_take_events(o.p, q, c, 1) == -1          # more unread than the read takes
+    assert o.take_events() == []                           # the buffer emptied, the flag cleared
