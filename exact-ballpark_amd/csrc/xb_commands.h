@@ -148,6 +148,8 @@ static inline XB_HD int xb_cmd_gather(const XbPark& P, int64_t n, int64_t r, XbC
 
 /* Stop(Ball*): a follower forgets its target (stamp, pointer, ids, range); the mode becomes STOP. */
 static inline XB_HD void xb_row_stop(const XbPark& P, int64_t r) {
+    if (XB_Q(P, XB_I_MODE, r) == XB_MODE_FORMATION)               /* the master's FreeFormationSlot first */
+        xb_free_formation_slot(P, XB_Q(P, XB_I_FOLLOW_PTR, r), XB_Q(P, XB_I_EFFECT_STAMP, r));
     if (xb_follows(XB_Q(P, XB_I_MODE, r))) {
         XB_Q(P, XB_I_EFFECT_STAMP, r) = 0;
         XB_Q(P, XB_I_FOLLOW_PTR, r) = -1;

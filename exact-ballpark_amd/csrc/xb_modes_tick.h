@@ -112,6 +112,7 @@ struct XbPark {
     const int32_t* form_len;
     int n_form, form_stride;
     const int64_t* fld;            /* the force fields' keys [3][stride] (harmonic, corporation, alliance), or null */
+    int64_t* fslots;               /* the master's formation reservations [stride] (a leader's bits), or null */
 };
 
 /* A row's tick-start records, as the others read them in phase 1 (indexed by the row's place in its group). */
@@ -424,11 +425,30 @@ static inline XB_HD void xb_clear_follow(XbShip& s) {
     s.follow_range = 0.0;
 }
 
+/* FreeFormationSlot((size_t) mEffectStamp) of a FORMATION follower's Stop(Ball*) on the master (Ballpark.cpp:4629-4633,
+   full.py:1850-1857; the reference's kernel vec_combat_cuda.cu:200-224): its slot's bit cleared in its leader's
+   reservations, nothing without the leader's loaded formation or for a slot past its end (an unsigned comparison).
+   Followers of one leader may stop in one launch: the clear is an atomic AND, which commutes. A slot of 16 or more
+   (Destiny's bitset<16> throws) is refused before it can be reached: FormationFollow on the master takes slots 0-15. */
+static inline XB_HD void xb_free_formation_slot(const XbPark& P, int64_t leader, int64_t slot) {
+    if (!P.fslots || leader < 0) return;
+    const int fid = (int)(int8_t)P.b[XB_B_FORMATION * P.stride + leader];
+    if (fid < 0 || fid > P.n_form - 1) return;
+    const uint64_t s = (uint64_t)slot;
+    if (s > (uint64_t)(int64_t)(P.form_len[fid] - 1) || s >= 16) return;
+    const uint64_t keep = ~((uint64_t)1 << s);
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAnd(reinterpret_cast<unsigned long long*>(P.fslots + leader), (unsigned long long)keep);
+#else
+    __atomic_fetch_and(reinterpret_cast<uint64_t*>(P.fslots + leader), keep, __ATOMIC_RELAXED);
+#endif
+}
+
 /* The release of one follower of a ball that starts warp proper (modes.py:410-426): an interactive orbiter flies on
    by GotoDirection along its velocity (refused, so left as it is, when the velocity or the point is not finite), any
    other member stops (Stop(Ball*) clears the follow fields of a FOLLOW, ORBIT or FORMATION ball). Returns the mode it
    leaves. */
-static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
+static inline XB_HD int64_t xb_release(XbShip& s, bool apply, const XbPark* P = nullptr) {
     if (s.mode == XB_MODE_MISSILE || (s.interactive && s.mode == XB_MODE_ORBIT)) {
         if (apply && s.mode == XB_MODE_MISSILE) s.massive = false;    /* a missile flies on, non-massive */
         if (!v3_finite(s.vel)) return s.mode;
@@ -443,6 +463,7 @@ static inline XB_HD int64_t xb_release(XbShip& s, bool apply) {
         return XB_MODE_GOTO;
     }
     if (apply) {
+        if (P && s.mode == XB_MODE_FORMATION) xb_free_formation_slot(*P, s.follow_ptr, s.effect_stamp);
         if (s.mode == XB_MODE_FOLLOW || s.mode == XB_MODE_ORBIT || s.mode == XB_MODE_FORMATION) xb_clear_follow(s);
         s.mode = XB_MODE_STOP;
     }
@@ -701,7 +722,7 @@ static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, 
         if (R.id[tj] < s.id) rel_before = true;
         else rel_after = true;
     }
-    if (rel_before) xb_release(s, true);
+    if (rel_before) xb_release(s, true, &P);
     w.path = live ? XB_PATH_COAST : XB_PATH_IDLE;
     if (live) {
         XbV3 a = v3(0.0, 0.0, 0.0);
@@ -758,7 +779,7 @@ static inline XB_HD void xb_phase1(XbRow& w, const XbRecords& R, int j, int lo, 
         }
         s.last_g = a;
     }
-    if (rel_after) xb_release(s, true);
+    if (rel_after) xb_release(s, true, &P);
 }
 
 /* Phase 3 of a row in warp proper (modes.py:724-732): its warp path; dropping out (TRV-714), the drop-out point

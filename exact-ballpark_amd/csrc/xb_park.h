@@ -34,5 +34,6 @@ static inline XB_HD XbPark xb_park(const xb_modes_state& s) {
     P.n_form = s.form ? s.n_form : 0;
     P.form_stride = s.form_stride;
     P.fld = s.fld;
+    P.fslots = P.n_form > 0 ? s.form_slots : nullptr;
     return P;
 }

@@ -16,7 +16,7 @@ from pathlib import Path
 import torch  # noqa: F401  (first: its HIP runtime is the one the library binds)
 
 LIB_DIR = Path(__file__).resolve().parent.parent / "lib"
-ABI_VERSION = 8
+ABI_VERSION = 9
 NF, NI, NB, NS = 46, 10, 7, 7
 NCF, NCI = 3, 8                 # the collision blocks: mLastC; level, key (3), side (3), the list's length
 NY = 9                          # the display orientation block: yaw, pitch, roll, their old values, roll speed (2), yaw delta
@@ -47,7 +47,7 @@ class ModesState(ctypes.Structure):
                 ("st_keys", ctypes.c_void_p), ("st_range", ctypes.c_void_p), ("st_stride", ctypes.c_int64),
                 ("st_boxes", ctypes.c_int32), ("statics", ctypes.c_int32), ("form", ctypes.c_void_p),
                 ("form_len", ctypes.c_void_p), ("n_form", ctypes.c_int32), ("form_stride", ctypes.c_int32),
-                ("fld", ctypes.c_void_p)]
+                ("fld", ctypes.c_void_p), ("form_slots", ctypes.c_void_p)]
 
 
 _LIBS: dict = {}

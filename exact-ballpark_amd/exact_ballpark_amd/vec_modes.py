@@ -30,6 +30,7 @@ BOID, TROLL, MINIBALL, FIELD, RIGID, FORMATION = 7, 8, 9, 10, 11, 12
 INERT_MODES = (BOID, MINIBALL, FIELD, RIGID)
 LOADABLE = (STOP,) + INERT_MODES
 MAX_CATCH_UP_TICKS = 5                      # OnTick's resynchronisation (full.py:92)
+FORMATION_MAX_SLOTS = 16                    # a leader's reservations, a std::bitset<16> (Ball.h)
 
 # (name, block, first block row, columns; 0 for a [n] field)
 FIELDS = (
@@ -241,6 +242,7 @@ class VecModesPark(metaclass=_Fields):
         self.rot = torch.tensor([0.0, 0.0, 0.0, 1.0], dtype=F64, device=self.dev).expand(self.n, 4)
         self.old_rot = self.rot.clone()
         self._forms = None             # load_formations' table (formations, slots, lengths) on the device
+        self._fslots = torch.zeros(self.n, dtype=I64, device=self.dev)   # each ball's reserved formation slots
         self._fld = None               # the force fields' keys [3, stride] (harmonic, corporation, alliance) once set
         self.tiles = None
         self.rowmap = None             # [n] int32: the tiled launch's rows in a regrouped world order (None: row order)
@@ -317,6 +319,7 @@ class VecModesPark(metaclass=_Fields):
         s.n_form = fm["n"] if fm else 0
         s.form_stride = fm["stride"] if fm else 0
         s.fld = self._fld.data_ptr() if self._fld is not None else None
+        s.form_slots = self._fslots.data_ptr() if (fm and self.master) else None
         return s
 
     def record_paths(self, on: bool = True) -> None:
@@ -427,6 +430,7 @@ class VecModesPark(metaclass=_Fields):
             raise ValueError("an empty row (id EMPTY_ID) is moribund, fixed and not massive: use empty_row()")
         self._set_tiles()
         self._fld = None
+        self._fslots.zero_()
         if self.dynamical_orientation:                  # a miniball keeps the identity, old and new (full.py:1350)
             self._orient_setup(None if not bool(mini.any()) else ~mini, 1)
         if self.display_ypr:                            # AddBall's snap of a free ball (full.py:1283-1288)
@@ -761,6 +765,7 @@ class VecModesPark(metaclass=_Fields):
         self.massive = torch.where(mask, False, self.massive)
         self.free = torch.where(mask, False, self.free)
         self.mode = torch.where(mask, STOP, self.mode)
+        self._fslots.masked_fill_(mask, 0)              # the ball is gone with its reservations
         self._dying = self._dying & ~mask
         self._state = None
 
@@ -1067,6 +1072,7 @@ class VecModesPark(metaclass=_Fields):
         if bool((change & ((fid < 0) | (cur > F - 1))).any()):
             raise ValueError("SetBallFormation reads mFormations out of its range: Destiny's result is undefined")
         self._release_followers(leave)
+        self._fslots.masked_fill_(leave, 0)             # leaving a formation clears every reservation
         new_len = self._formation_len(fid)
         shrink = change & (self._formation_len(cur) > new_len)
         safe = torch.where(self.follow_ptr >= 0, self.follow_ptr, 0)
@@ -1074,6 +1080,49 @@ class VecModesPark(metaclass=_Fields):
              & (self.follow_id == self.ids[safe]) & (self.effect_stamp > new_len[safe] - 1))
         self._stop_rows(f)
         self.formation_id = torch.where(ok, fid, cur).to(torch.int8)
+
+    @property
+    def formation_slots(self) -> torch.Tensor:
+        """[n] int64: each ball's reserved formation slots (Ball::mFormationSlots, a std::bitset<16>): set by
+        reserve_formation_slot, cleared by free_formation_slot, by leaving its formation and, on the master, by the
+        Stop(Ball*) of a FORMATION follower in that slot (its commands' and StopAllFollowers inside the tick)."""
+        return self._fslots
+
+    def _slot_leader(self, row: int, what: str) -> tuple:
+        fid = int(self.formation_id[int(row)])
+        if self._forms is None or fid == -1 or fid > self._forms["n"] - 1:
+            return -1, 0
+        if fid < 0:
+            raise ValueError(f"{what} reads mFormations at a negative index: Destiny's result is undefined")
+        return fid, self._forms["lens"][fid]
+
+    def reserve_formation_slot(self, row: int) -> int:
+        """ReserveFormationSlot (Ball.cpp:2275; full.py:1920-1936) of the ball in `row`: the first slot of its
+        formation (of at most 16) not reserved, now reserved, or -1 (no formation, or more than 16 slots, or all
+        taken). One ball on the host, as the reference's VecCombatPark does it (vec_combat.py:458-476)."""
+        fid, n = self._slot_leader(row, "ReserveFormationSlot")
+        if fid < 0 or n > FORMATION_MAX_SLOTS:
+            return -1
+        bits = int(self._fslots[int(row)])
+        for i in range(n):
+            if not bits >> i & 1:
+                self._fslots[int(row)] = bits | (1 << i)
+                return i
+        return -1
+
+    def free_formation_slot(self, row: int, slot: int) -> None:
+        """FreeFormationSlot (Ball.cpp:2324; full.py:1938-1954) of the ball in `row`, with a size_t slot: nothing
+        without a formation or for a slot past its end; a slot of 16 or more in a longer formation is Destiny's
+        bitset<16> throwing: refused with an error."""
+        fid, n = self._slot_leader(row, "FreeFormationSlot")
+        if fid < 0:
+            return
+        s = int(slot) & 0xFFFFFFFFFFFFFFFF
+        if s > ((n - 1) & 0xFFFFFFFFFFFFFFFF):
+            return
+        if s >= FORMATION_MAX_SLOTS:
+            raise ValueError("FreeFormationSlot of a slot past 15: Destiny's bitset<16> throws")
+        self._fslots[int(row)] = int(self._fslots[int(row)]) & ~(1 << s)
 
     def formation_follow(self, mask: torch.Tensor, leader: torch.Tensor, slot: torch.Tensor) -> None:
         """FormationFollow for the balls in `mask`: leader is each row's leader's Destiny id, slot a char; refused
@@ -1096,6 +1145,9 @@ class VecModesPark(metaclass=_Fields):
               & ~(slot > self._formation_len(fid) - 1))
         if bool((ok & (slot < 0)).any()):
             raise ValueError("FormationFollow into a negative slot: EvolveFormation's result is undefined")
+        if self.master and bool((ok & (slot >= FORMATION_MAX_SLOTS)).any()):
+            raise ValueError("FormationFollow into a slot past 15 on the master: the follower's Stop would free it in "
+                             "Destiny's bitset<16>, which throws")
         self._refuse_mushroom_targets(ok, lsafe)
         self._stop_rows(ok)
         ok = ok & (self.bubble == self.bubble[lsafe]) & (self.cloaked[lsafe] == 0)

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define XB_ABI_VERSION 8
+#define XB_ABI_VERSION 9
 
 /* A flight-modes park in device (or, for libxb_cpu.so, host) memory: the structure-of-arrays blocks laid out as
    exact-ballpark_amd/csrc/xb_layout.h defines them (field f of row r at block[f * stride + r]). Replaces the
@@ -106,6 +106,11 @@ typedef struct xb_modes_state {
        and alliance (-1 for none). A candidate in mode FIELD is left out of the gradient when the responder's harmonic
        is -2 or one of its keys (not -1) is the responder's. */
     const int64_t* fld;
+    /* The master's formation reservations (std::bitset<16> a leader, ReserveFormationSlot's bits), int64 [stride], or
+       NULL on a client's view: on the master, Stop(Ball*) of a FORMATION follower (a command's, or StopAllFollowers
+       inside the tick) clears its slot's bit in its leader's word (FreeFormationSlot; an atomic AND, so followers of
+       one leader stopping in one launch commute). Read with form_len and the leader's formation. */
+    int64_t* form_slots;
 } xb_modes_state;
 
 /* The ABI version (XB_ABI_VERSION). Replaces xbpm_abi (src/exact_ballpark/vec_modes_cuda.cu:750). */

@@ -23,6 +23,7 @@ MUSHROOM_OP, FREE_OP = 20, 21
 HARMONIC_OP = 22
 REMOVE_OP = 23
 BRING_OP = 24
+RESERVE_OP, FREESLOT_OP = 40, 41
 MASTER_OPS = {30: 1, 31: 2, 32: 3, 33: 4, 34: 5, 35: 6, 36: 7, 37: 8}   # keep alive .. CopyBubbles (oracle_master.c)
 EVENT_NAMES = {1: "DoProximity", 2: "DoRange", 3: "DoTargetTracking", 4: "DoFinalCleanup"}
 NT = 20                      # the turning state: rot, old_rot, ang_vel, old_ang_vel, torque, rot_roll, max_ang_vel, ...
@@ -89,6 +90,10 @@ def lib() -> ctypes.CDLL:
         L.or_load_formations.argtypes = [vp, i, ctypes.POINTER(ctypes.c_int), pd]
         L.or_set_ball_formation.argtypes = [vp, i64, i]
         L.or_formation_follow.argtypes = [vp, i64, i64, i]
+        L.or_reserve_formation_slot.argtypes = [vp, i64]
+        L.or_free_formation_slot.argtypes = [vp, i64, i64]
+        L.or_get_formation_slots.restype = i64
+        L.or_get_formation_slots.argtypes = [vp, i]
         L.or_cloak_ball.argtypes = [vp, i64, i]
         L.or_uncloak_ball.argtypes = [vp, i64]
         L.or_add_mushroom.argtypes = [vp, i64, d, d]
@@ -135,6 +140,7 @@ class World:
         self.L.or_park_set_dynamical(self.p, int(self.dynamical))
         self.L.or_park_set_collisions(self.p, int(self.collisions))
         self.L.or_park_set_ypr(self.p, int(bool(ypr)))
+        self.reserved = []                             # ReserveFormationSlot's answers since the last take_reserved
 
     def __del__(self):
         if getattr(self, "p", None):
@@ -192,6 +198,14 @@ class World:
         if rc:
             raise ValueError(f"static collidable {s['id']} refused")
 
+    def take_reserved(self) -> list:
+        out, self.reserved = self.reserved, []
+        return out
+
+    def formation_slots(self) -> list:
+        """Each ball's reserved formation slots (bits), in ascending id."""
+        return [int(self.L.or_get_formation_slots(self.p, k)) for k in range(int(self.L.or_count(self.p)))]
+
     def load_formations(self, formations: list) -> None:
         """LoadFormations: each formation's slot offsets ([x, y, z] each)."""
         counts = (ctypes.c_int * max(1, len(formations)))(*[len(f) for f in formations])
@@ -243,6 +257,10 @@ class World:
             L.or_set_ball_formation(p, a, w)
         elif op == FFOLLOW_OP:
             L.or_formation_follow(p, a, b, w)
+        elif op == RESERVE_OP:                  # ReserveFormationSlot: its answer kept for the record
+            self.reserved.append([int(a), int(L.or_reserve_formation_slot(p, a))])
+        elif op == FREESLOT_OP:
+            L.or_free_formation_slot(p, a, w)
         elif op == CLOAK_OP:
             L.or_cloak_ball(p, a, w)
         elif op == UNCLOAK_OP:

@@ -68,6 +68,7 @@ typedef struct {
     /* the display orientation without dynamical orientation (full.py: yaw, pitch, roll_angle and the rest) */
     double yaw, pitch, roll, old_yaw, old_pitch, old_roll, roll_speed, old_roll_speed, yaw_delta;
     int formation_id;                          /* a leader's formation (-1: none; a char) */
+    uint64_t formation_slots;                  /* its reserved slots (std::bitset<16>, full.py:740) */
     /* the master's (oracle_master.c): the ball dictionary's place, the old bubble, the keep-alive mark, the proximity
        sensor, the notification range and target tracking */
     int64_t seq, old_bubble;
@@ -752,7 +753,13 @@ int or_get_turning(const OPark* p, int k, double* d) {
 
 static int in_follow_modes(int64_t m) { return m == FOLLOW || m == ORBIT || m == MISSILE || m == FORMATION; }
 
+static void free_formation_slot(OPark* p, OBall* b, uint64_t slot);
+
 static void stop_ball(OPark* p, OBall* b) {                       /* Stop(Ball*), modes.py:206-220 (master) */
+    if (p->m && b->mode == FORMATION) {                  /* the master frees the slot first (full.py:1850-1857) */
+        OBall* t = find(p, b->follow_ptr);
+        if (t) free_formation_slot(p, t, (uint64_t)b->effect_stamp);
+    }
     if (in_follow_modes(b->mode)) {
         OBall* t = find(p, b->follow_ptr);
         if (t) fol_discard(t, b->id);
@@ -1067,6 +1074,7 @@ void or_set_ball_formation(OPark* p, int64_t id, int formation_id) {
     if (b->formation_id >= 0) {
         if (fid == -1) {
             stop_all_followers(p, b);
+            b->formation_slots = 0;                                     /* every reservation cleared */
         } else if (fid != b->formation_id) {
             if (fid < 0 || b->formation_id > p->nform - 1) {
                 p->overflow = 1;
@@ -1085,6 +1093,51 @@ void or_set_ball_formation(OPark* p, int64_t id, int formation_id) {
     }
     b->formation_id = fid;
 }
+
+/* FreeFormationSlot (full.py:1938-1954) with a size_t slot: nothing without a loaded formation or for a slot past its
+   end (an unsigned comparison); a negative formation or a slot of 16 or more is Destiny's undefined behaviour (the
+   overflow flag). */
+static void free_formation_slot(OPark* p, OBall* b, uint64_t slot) {
+    const int fid = b->formation_id;
+    if (fid == -1 || fid > p->nform - 1) return;
+    if (fid < 0) {
+        p->overflow = 1;
+        return;
+    }
+    if (slot > (uint64_t)(int64_t)(p->form_len[fid] - 1)) return;
+    if (slot >= 16) {
+        p->overflow = 1;
+        return;
+    }
+    b->formation_slots &= ~((uint64_t)1 << slot);
+}
+
+void or_free_formation_slot(OPark* p, int64_t id, int64_t slot) {
+    OBall* b = find(p, id);
+    if (b) free_formation_slot(p, b, (uint64_t)slot);
+}
+
+/* ReserveFormationSlot (full.py:1920-1936): the first free slot of the ball's formation (at most 16 slots), or -1 */
+int or_reserve_formation_slot(OPark* p, int64_t id) {
+    OBall* b = find(p, id);
+    if (!b) return -1;
+    const int fid = b->formation_id;
+    if (fid == -1 || fid > p->nform - 1) return -1;
+    if (fid < 0) {
+        p->overflow = 1;
+        return -1;
+    }
+    const int n = p->form_len[fid];
+    if (n > 16) return -1;
+    for (int i = 0; i < n; ++i) {
+        if (b->formation_slots >> i & 1) continue;
+        b->formation_slots |= (uint64_t)1 << i;
+        return i;
+    }
+    return -1;
+}
+
+int64_t or_get_formation_slots(const OPark* p, int k) { return (int64_t)p->balls[k].formation_slots; }
 
 /* FormationFollow (full.py:1899-1918): into slot `slot` (a char) of the leader's formation */
 void or_formation_follow(OPark* p, int64_t src, int64_t dst, int slot_in) {

@@ -3,7 +3,9 @@
 `FullBallpark(is_master=True, seed=...)`: space.py's partition with its top boxes and bubbles, the bubble
 dictionaries, the interactive and keep-alive sets, the dead bubbles' skip, HandleProximities with its sensors,
 notification ranges and target tracking in libstdc++'s unordered_map order, glibc rand() for shuffled sensors) on the
-seeded worlds of tests/scenarios.py (`space_cases`), after the load (step 0: the balls as added, then the step-0
+seeded worlds of tests/scenarios.py (`space_cases`; with `--formations`, formation_master_golden.npz:
+`formation_master_cases`, LoadFormations, ReserveFormationSlot and FreeFormationSlot by hand and the slots that
+Stop(Ball*) frees on the master, each ball's reservation bits and each reservation's answer in `master`), after the load (step 0: the balls as added, then the step-0
 commands) and after every tick:
 
 - every ball's state as bits (the flight fields, the filing, the display yaw, pitch and roll), in ascending id;
@@ -36,9 +38,13 @@ full = C.full
 EVENTS = ("DoProximity", "DoRange", "DoTargetTracking", "DoFinalCleanup")
 
 
-def _apply(park, c) -> None:
+def _apply(park, c, reserved: list) -> None:
     _, op, a, b, x, y, z, rr, w = c
-    if op == S.KEEPALIVE_OP:
+    if op == S.RESERVE_OP:
+        reserved.append([int(a), int(park.reserve_formation_slot(a))])
+    elif op == S.FREESLOT_OP:
+        park.free_formation_slot(a, w)
+    elif op == S.KEEPALIVE_OP:
         park.add_keep_alive_ball(a)
         bl = park.balls.get(a)
         if bl is not None:                      # the Python layer adds to the set only; the next AddToBubble counts it
@@ -61,7 +67,7 @@ def _apply(park, c) -> None:
         M._apply(park, c)
 
 
-def master_state(park, w: dict) -> dict:
+def master_state(park, w: dict, reserved: list) -> dict:
     ids = sorted(park.balls)
     tops = sorted([list(k) + [int(box.bubble)] for k, box in park.levels[0].items()])
     dicts = [[int(bub), [[int(i), int(a)] for i, a in d.items()]] for bub, d in park.bubbles.items()]
@@ -96,33 +102,38 @@ def master_state(park, w: dict) -> dict:
             "prox": [int(k) for k in pm.keys()], "prox_bkt": [int(pm.bucket_count), int(pm.next_resize)],
             "sensors": sensors, "bubble_id": int(park.bubble_id), "events": events,
             "vis": vis, "range": rng, "range_dist": rngd, "point": [float(c).hex() for c in pt], "point_ids": point,
-            "at": [None if a is None else int(a) for a in at], "remote": [int(x) for x in park.remote_followers()]}
+            "at": [None if a is None else int(a) for a in at], "remote": [int(x) for x in park.remote_followers()],
+            "fslots": [int(park.balls[i].formation_slots) for i in ids], "reserved": reserved}
 
 
 def run_world(w: dict) -> tuple:
     park = full.FullBallpark(friction=w["friction"], tick_interval_ms=w["tick"], use_new_orbit=w["new_orbit"],
                              is_master=True, seed=w["seed"])
     park.current_time = w["start"]
+    if w.get("formations"):
+        park.load_formations(w["formations"])
     for b in w["balls"]:
         park.add_ball(b["id"], b["mass"], b["radius"], b["max_vel"], is_free=b["free"], is_massive=b["massive"],
                       is_interactive=b["interactive"], x=b["pos"][0], y=b["pos"][1], z=b["pos"][2],
                       vx=b["vel"][0], vy=b["vel"][1], vz=b["vel"][2], agility=b["agility"], speed_fraction=b["sf"])
     out, master = [], []
     for step in range(0, w["steps"] + 1):
+        reserved = []
         for c in w["cmds"]:
             if c[0] == step:
-                _apply(park, c)
+                _apply(park, c, reserved)
         if step:
             park.evolve()
         for k, bid in enumerate(sorted(park.balls)):
             d, q = Y._state(park.balls[bid])
             out.append((step, k, d, q, list(park.balls[bid].collisions)))
-        master.append(master_state(park, w))
+        master.append(master_state(park, w, reserved))
     return out, master
 
 
-def main(seed: int = 1) -> None:
-    ws = S.space_cases(seed)
+def main(seed: int = 1, formations: bool = False) -> None:
+    ws = S.formation_master_cases(seed) if formations else S.space_cases(seed)
+    name = "formation_master_golden.npz" if formations else "space_golden.npz"
     for w in ws:
         w["ypr"] = True
     kept, skipped, masters = [], [], []
@@ -144,7 +155,7 @@ def main(seed: int = 1) -> None:
             hit_ids.extend(hits)
             hit_off.append(len(hit_ids))
     d = np.array(rec_d, dtype=np.float64).view(np.uint64)
-    np.savez_compressed(HERE / "space_golden.npz", seed=np.int64(seed), worlds=np.array(json.dumps(ws)),
+    np.savez_compressed(HERE / name, seed=np.int64(seed), worlds=np.array(json.dumps(ws)),
                         kept=np.array(kept, dtype=np.int64), skipped=np.array(json.dumps(skipped)),
                         world=np.array(rec_world, dtype=np.int32), step=np.array(rec_step, dtype=np.int32),
                         ball=np.array(rec_ball, dtype=np.int32), d=d, q=np.array(rec_q, dtype=np.int64),
@@ -152,10 +163,10 @@ def main(seed: int = 1) -> None:
                         doubles=np.array(C.DOUBLES + Y.YPR), ints=np.array(C.INTS),
                         master=np.array(json.dumps(masters)))
     n_ev = sum(len(r["events"]) for m in masters for r in m)
-    print(f"space_golden.npz: {len(kept)} worlds kept, {len(skipped)} skipped {skipped[:5]}, {len(rec_d)} records, "
+    print(f"{name}: {len(kept)} worlds kept, {len(skipped)} skipped {skipped[:5]}, {len(rec_d)} records, "
           f"{n_ev} sensor events", file=sys.stderr)
 
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
-    main(int(args[0]) if args else 1)
+    main(int(args[0]) if args else 1, formations="--formations" in sys.argv)

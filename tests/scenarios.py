@@ -1660,3 +1660,93 @@ def _remote_world(r: random.Random) -> tuple:
     cmds = [_cmd(1, GOTO_OP, 201, x=face - 4e5, y=base[1], z=base[2]),
             _cmd(1, FOLLOW_OP, 202, 201, rr=r.uniform(100.0, 600.0))]
     return balls, cmds, r.randrange(25, 45)
+
+
+RESERVE_OP = 40                 # ReserveFormationSlot(leader a): the returned slot recorded
+FREESLOT_OP = 41                # FreeFormationSlot(leader a, slot w)
+
+
+def formation_master_cases(seed: int, scale: int = 1) -> list:
+    """The master's formation worlds (about 100 x scale) for seed, in the space worlds' shape (one cluster, every ship
+    interactive so its bubble lives, not massive, the display yaw, pitch and roll): 1-3 formations of 1-12 slots and at
+    times more than 16, a leader with one, slots reserved (ReserveFormationSlot) and freed (FreeFormationSlot, in and past
+    the formation) by hand, followers put in slots (FormationFollow); then what Stop(Ball*) does to a FORMATION follower
+    on the master (its slot freed in its leader): a follower stopped or sent on GOTO, FOLLOW or into another slot, the
+    leader warping out (StopAllFollowers inside the tick), leaving its formation (every reservation cleared) or moving to
+    a smaller one (the followers in removed slots stop), a follower removed, the leader set fixed."""
+    r = random.Random(seed ^ 0xF5107)
+    out = []
+    for wi in range(100 * scale):
+        c = [r.randrange(-3, 4) * TOP_WIDTH + r.uniform(0.1, 0.9) * TOP_WIDTH for _ in range(3)]
+        forms = []
+        for _ in range(r.randint(1, 3)):
+            n = r.choice([r.randint(1, 12), r.randint(1, 12), r.randint(15, 20)])
+            forms.append([[r.uniform(-3000.0, 3000.0) for _ in range(3)] for _ in range(n)])
+        n_ships = r.randint(2, 8)
+        ids = r.sample(range(1, 5000), n_ships)
+        leader = ids[0]
+        balls = []
+        for bid in ids:
+            pos = [x + r.uniform(-4000.0, 4000.0) for x in c]
+            b = _ball(r, bid, pos=pos, massive=False, interactive=True, bubble=-1, cloaked=0, free=True,
+                      vel=[0.0, 0.0, 0.0], max_vel=r.uniform(50.0, 600.0))
+            balls.append(b)
+        steps = r.randrange(12, 31)
+        fid = r.randrange(len(forms))
+        nf = len(forms[fid])
+        cmds = [_cmd(0, SETFORM_OP, leader, w=fid)]
+        for _ in range(r.randint(0, 4)):
+            cmds.append(_cmd(r.choice([0, 0, r.randrange(1, steps)]), RESERVE_OP, leader))
+        for _ in range(r.randint(0, 2)):
+            cmds.append(_cmd(r.randrange(0, steps), FREESLOT_OP, leader, w=r.randrange(0, nf + 2)))
+        if r.random() < 0.5:
+            p = [c[i] + r.uniform(-20000.0, 20000.0) for i in range(3)]
+            cmds.append(_cmd(1, GOTO_OP, leader, x=p[0], y=p[1], z=p[2]))
+        used = set()
+        for k, f in enumerate(ids[1:]):
+            slot = k % min(nf, 16) if r.random() < 0.8 else r.randrange(min(nf, 16))
+            used.add(slot)
+            cmds.append(_cmd(r.choice([1, 1, 2, 3]), FFOLLOW_OP, f, leader, w=slot))
+        kind = "formation_master"
+        y = r.random()
+        fol = ids[1:]
+        at = r.randrange(5, steps)
+        if y < 0.2:
+            cmds.append(_cmd(at, STOP_OP, r.choice(fol)))
+            kind = "formation_master_stop"
+        elif y < 0.3:
+            p = [c[i] + r.uniform(-9000.0, 9000.0) for i in range(3)]
+            cmds.append(_cmd(at, GOTO_OP, r.choice(fol), x=p[0], y=p[1], z=p[2]))
+            kind = "formation_master_goto"
+        elif y < 0.4 and len(fol) > 1:
+            cmds.append(_cmd(at, FOLLOW_OP, fol[0], fol[1], rr=r.uniform(500.0, 5000.0)))
+            kind = "formation_master_follow"
+        elif y < 0.5:
+            cmds.append(_cmd(at, FFOLLOW_OP, r.choice(fol), leader, w=r.randrange(min(nf, 16))))
+            kind = "formation_master_reslot"
+        elif y < 0.65:
+            far = [x + r.choice([-1.0, 1.0]) * r.uniform(2e8, 5e8) for x in c]
+            cmds.append(_cmd(at, WARP_OP, leader, x=far[0], y=far[1], z=far[2], rr=0.0, w=20))
+            balls[0]["mass"] = 10 ** r.uniform(6.0, 7.0)
+            balls[0]["agility"] = r.uniform(0.05, 0.5)
+            balls[0]["max_vel"] = r.uniform(1000.0, 3000.0)
+            kind = "formation_master_warp"
+        elif y < 0.75:
+            cmds.append(_cmd(at, SETFORM_OP, leader, w=-1))
+            kind = "formation_master_leave"
+        elif y < 0.85:
+            cmds.append(_cmd(at, SETFORM_OP, leader, w=r.randrange(len(forms))))
+            kind = "formation_master_change"
+        elif y < 0.93:
+            cmds.append(_cmd(at, REMOVE_OP, r.choice(fol)))
+            kind = "formation_master_remove"
+        else:
+            cmds.append(_cmd(at, FREE_OP, r.choice(fol), w=0))
+            kind = "formation_master_fixed"
+        if r.random() < 0.3:
+            cmds.append(_cmd(r.randrange(at, steps + 1), RESERVE_OP, leader))
+        out.append({"kind": kind, "friction": 1.0e6, "tick": r.choice([1000, 1000, 500]), "steps": steps,
+                    "new_orbit": r.random() < 0.3, "start": 0, "master": True, "seed": r.randrange(1, 2 ** 31),
+                    "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]), "formations": forms,
+                    "queries": {"range": 5e3, "point_offset": [0.0, 0.0, 0.0]}})
+    return out

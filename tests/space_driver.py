@@ -42,21 +42,33 @@ def build(worlds: list, device: str, block: int = 256):
     rows = torch.full((len(worlds), width), -1, dtype=torch.int64)
     for wi, r in enumerate(rows_of):
         rows[wi, :len(r)] = torch.tensor(r)
+    if worlds[0].get("formations") is not None:      # one formation table a park (the worlds share it)
+        park.load_formations(worlds[0]["formations"])
     space = VecSpace(park, rows, seed=[w["seed"] for w in worlds])
     space.load()
     return park, space, slot_of, rows_of
 
 
-def _apply_round(park, space, items: list, slot_of: list, rows_of: list) -> None:
-    """One round of commands (at most one a world): the park's through park_driver, the layer's as one tape."""
+def _apply_round(park, space, items: list, slot_of: list, rows_of: list, reserved=None) -> None:
+    """One round of commands (at most one a world): the park's through park_driver, the layer's as one tape; the
+    formation reservations on the park, one ball at a time (ReserveFormationSlot's answers into `reserved`)."""
     index = {(wi, bid): rows_of[wi][j] for wi in range(len(rows_of)) for bid, j in slot_of[wi].items()}
     by_op = defaultdict(list)
     tape = [[] for _ in range(space.W)]
     for wi, c in items:
         _, op, a, b, x, y, z, rr, w = c
         j = slot_of[wi].get(a, -1)
-        if op == S.REMOVE_OP:
+        if op in (S.RESERVE_OP, S.FREESLOT_OP):
+            row = rows_of[wi][j]
+            if op == S.RESERVE_OP:
+                reserved[wi].append([int(a), park.reserve_formation_slot(row)])
+            else:
+                park.free_formation_slot(row, w)
+        elif op == S.REMOVE_OP:
             by_op["remove"].append((wi, c))
+        elif op == S.FREE_OP and not w and j >= 0 and bool(park.free[rows_of[wi][j]]):
+            by_op[op].append((wi, c))                  # SetBallFree(false) of a free ball: InsertInBoxes again
+            tape[wi].append(("REFILE", j))
         elif op == S.KEEPALIVE_OP:
             tape[wi].append(("KEEP_ALIVE", j))
         elif op == S.SENSOR_OP:
@@ -206,10 +218,13 @@ def run_space(worlds: list, device: str, block: int = 256, graph: bool = False) 
     out = {}
     g = None
     last = max(w["steps"] for w in worlds)
+    forms = worlds[0].get("formations") is not None
     for s in range(0, last + 1):
         todo = per_step.get(s, {})
+        reserved = [[] for _ in worlds]
         for j in range(max((len(v) for v in todo.values()), default=0)):
-            _apply_round(park, space, [(wi, cs[j]) for wi, cs in todo.items() if j < len(cs)], slot_of, rows_of)
+            _apply_round(park, space, [(wi, cs[j]) for wi, cs in todo.items() if j < len(cs)], slot_of, rows_of,
+                         reserved)
         if s:
             if graph:
                 if g is None:
@@ -219,6 +234,14 @@ def run_space(worlds: list, device: str, block: int = 256, graph: bool = False) 
             else:
                 space.tick()
         masters = master_states(park, space, worlds, slot_of)
+        if forms:                                      # each ball's reservation bits, in ascending id
+            bits = park.formation_slots.cpu()
+            for wi in range(len(worlds)):
+                rows = rows_of[wi]
+                live = sorted((k for k in range(len(rows)) if int(space.st["sflags"][k, wi]) & 1),
+                              key=lambda k: int(park.ids[rows[k]]))
+                masters[wi]["fslots"] = [int(bits[rows[k]]) for k in live]
+                masters[wi]["reserved"] = reserved[wi]
         for wi, w in enumerate(worlds):
             if s <= w["steps"]:
                 d, q = _flight(park, space, wi, rows_of[wi])
@@ -250,7 +273,7 @@ def groups(worlds: list, kept: list) -> dict:
     out = defaultdict(list)
     for wi in kept:
         w = worlds[wi]
-        out[(w["friction"], w["tick"], bool(w["new_orbit"]))].append(wi)
+        out[(w["friction"], w["tick"], bool(w["new_orbit"]), repr(w.get("formations")))].append(wi)
     return out
 
 
