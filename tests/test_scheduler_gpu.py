@@ -7,7 +7,7 @@
 - training on the park's own recorded ticks lowers the loss and beats the majority bucket;
 - the learned regrouping ticks every world to the same bits as the unsorted order (the unsorted order is the reference);
 - the trained network's regrouping lowers the modeled launch cost and the measured time per tick against the unsorted
-  order on the same worlds (measured figures printed; the modeled one asserted).
+  order on the same worlds (the modeled cost and the measured median time asserted).
 """
 
 import numpy as np
@@ -115,22 +115,27 @@ def _bits(park):
                                                                                        park._b.clone())]
 
 
-def _ms_per_tick(park, ticks, reps=10):
+def _ms_per_tick(park, ticks, reps=10, windows=5):
+    """The median over `windows` timed windows of `reps` graph replays, ms a tick."""
     g = ModesGraph(park, ticks=ticks)
     g.tick()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    s.record()
-    for _ in range(reps):
-        g.tick()
-    e.record()
-    torch.cuda.synchronize()
-    return s.elapsed_time(e) / (reps * ticks)
+    got = []
+    for _ in range(windows):
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(reps):
+            g.tick()
+        e.record()
+        torch.cuda.synchronize()
+        got.append(s.elapsed_time(e) / (reps * ticks))
+    return sorted(got)[len(got) // 2]
 
 
 def test_learned_regroup_is_exact_and_gains():
     """Train on the park's first ticks; then, from one saved state, the unsorted order and the learned order run the
-    same ticks: every field bit-identical, the learned order cheaper in the cost model and (printed) in time."""
+    same ticks: every field bit-identical, the learned order cheaper in the cost model and in measured time (the median
+    of 5 timed windows of 10 replays of 16 ticks each)."""
     park = _park(13, 32768)
     park.schedule = 1
     L = S.LearnedScheduler(park, seed=0)
@@ -160,3 +165,4 @@ def test_learned_regroup_is_exact_and_gains():
     print("modeled cost unsorted", out["unsorted"][1], "learned", out["learned"][1],
           "| ms per tick unsorted", round(out["unsorted"][2], 5), "learned", round(out["learned"][2], 5))
     assert out["learned"][1] < 0.8 * out["unsorted"][1]
+    assert out["learned"][2] < out["unsorted"][2], "the learned order is not faster (median of 5 windows)"
