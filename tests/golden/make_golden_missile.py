@@ -6,7 +6,9 @@ with `--minis`, mini_golden.npz: `mini_cases`, a fixed parent's miniballs, minic
 by SetBallHarmonic; every ball of the park recorded, the miniballs with their local ids; with `--mini-own`,
 mini_own_golden.npz: `mini_own_cases`, defender missiles through the miniballs of a parent whose id is their owner's
 (Gradient's skip) or another's; with `--remove`,
-remove_golden.npz: `remove_cases`, RemoveBall between ticks, a removed ball out of the records from then on): the reference's own scalar ballpark with missiles
+remove_golden.npz: `remove_cases`, RemoveBall between ticks, a removed ball out of the records from then on; with
+`--mass-remove`, mass_remove_golden.npz: `mass_remove_cases`, more delayed removals due at once than one
+BringOutTheDeads takes): the reference's own scalar ballpark with missiles
 (src/exact_ballpark/full.py, `FullBallpark(dynamical_orientation=False)`, a client's view: the partition and the
 gradient method) on the seeded worlds of tests/scenarios.py (`missile_cases`: LaunchMissile and MissileFollow among
 ships and fixed spheres), every ball's state as added and after every tick, as bits: the flight fields, mLastC and
@@ -124,6 +126,8 @@ def main(seed: int = 1, do: bool = False, what: str = "missile") -> None:
         ws = S.do_mushroom_cases(seed) if do else S.mushroom_cases(seed)
     elif what == "remove":
         ws = S.remove_cases(seed) + S.delayed_remove_cases(seed)
+    elif what == "mass_remove":
+        ws = S.mass_remove_cases(seed)
     elif what == "mini":
         ws = S.do_mini_cases(seed) if do else S.mini_cases(seed)
     elif what == "mini_own":
@@ -177,4 +181,4 @@ if __name__ == "__main__":
          what="formation" if "--formations" in sys.argv else "cloak" if "--cloak" in sys.argv
          else "mushroom" if "--mushrooms" in sys.argv else "mini_own" if "--mini-own" in sys.argv
          else "mini" if "--minis" in sys.argv
-         else "remove" if "--remove" in sys.argv else "missile")
+         else "remove" if "--remove" in sys.argv else "mass_remove" if "--mass-remove" in sys.argv else "missile")

@@ -1750,3 +1750,33 @@ def formation_master_cases(seed: int, scale: int = 1) -> list:
                     "balls": balls, "cmds": sorted(cmds, key=lambda q: q[0]), "formations": forms,
                     "queries": {"range": 5e3, "point_offset": [0.0, 0.0, 0.0]}})
     return out
+
+
+def mass_remove_cases(seed: int, scale: int = 1) -> list:
+    """The crowded delayed removals (about 40 x scale): worlds of 9-16 balls with collisions and the display yaw, pitch
+    and roll, 8 or more of them removed with delays of 1-3 ticks in one step (so more than the 7 that one
+    BringOutTheDeads call takes within two ticks of their stamp: the rest wait for a later call), BringOutTheDeads
+    before every tick, some balls flying GOTO or following a ball that is removed."""
+    r = random.Random(seed ^ 0x7DEAD)
+    out = []
+    for _ in range(40 * scale):
+        c = [r.uniform(-1e6, 1e6) for _ in range(3)]
+        n = r.randint(9, 16)
+        ids = _ids(r, n, big=r.random() < 0.2)
+        balls = [_coll_ball(r, bid, c, 6000.0, massive=r.random() < 0.6, max_vel=r.uniform(50.0, 600.0)) for bid in ids]
+        steps = r.randint(8, 16)
+        cmds = [(s, BRING_OP, 0, 0, 0.0, 0.0, 0.0, 0.0, 0) for s in range(1, steps + 1)]
+        for bid in r.sample(ids, r.randint(1, 4)):
+            p = [c[i] + r.uniform(-20000.0, 20000.0) for i in range(3)]
+            cmds.append(_cmd(1, GOTO_OP, bid, x=p[0], y=p[1], z=p[2]))
+        gone = r.sample(ids, r.randint(8, n - 1))                 # one ball stays: the world is recorded to the end
+        if r.random() < 0.5:
+            f = r.choice([i for i in ids if i != gone[0]])
+            cmds.append(_cmd(1, FOLLOW_OP, f, gone[0], rr=r.uniform(500.0, 5000.0)))
+        at = r.randint(1, 4)
+        for bid in gone:
+            cmds.append(_cmd(at, REMOVE_OP, bid, w=r.choice([1, 1, 2, 3])))
+        out.append({"kind": "mass_remove", "friction": 1.0e6, "tick": r.choice([1000, 1000, 500]), "steps": steps,
+                    "new_orbit": r.random() < 0.5, "start": r.randrange(0, 2_000_000), "balls": balls,
+                    "cmds": sorted(cmds, key=lambda q: (q[0], q[1] != BRING_OP)), "collisions": True})
+    return out
